@@ -1,0 +1,148 @@
+/*
+ * rr_hip.h -- C-ABI of the MI355X (gfx950) HIP backend for the rasterrocket
+ * GPU raster hot path: anti-aliased path fill + span compositing on device.
+ *
+ * Every entry point returns int32_t: 0 on success, a negative RR_E_* code on
+ * failure (the classes of BackendError, crates/gpu/src/backend/mod.rs:35-72);
+ * rr_hip_last_error(ctx) gives the message.  Callers turn any error into the
+ * reference's CPU fallback, as crates/pdf_interp/src/renderer/page/vk_ops.rs:
+ * 62-70 does for the Vulkan backend.  Buffers are owned by the caller; a
+ * context serialises its work on one HIP stream.
+ */
+#ifndef RR_HIP_H
+#define RR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error classes (BackendError variants, backend/mod.rs:35-72). */
+#define RR_OK 0
+#define RR_E_ZERO_SIZE_ALLOC (-1) /* BackendError::ZeroSizeAlloc */
+#define RR_E_INVARIANT (-2)       /* BackendError::InvariantViolation */
+#define RR_E_DEVICE (-3)          /* BackendError::Other: a HIP runtime failure */
+#define RR_E_NO_DEVICE (-4)       /* backend unavailable (init failure) */
+#define RR_E_CAPACITY (-5)        /* device scratch pool exhausted for this batch */
+
+/* AA row mode. DOCUMENTED: one renderAALine per device row, the Splash
+ * semantics the reference documents (crates/raster/src/scanner/mod.rs:380-384).
+ * LITERAL: crates/raster/src/fill/mod.rs:176-203 as written (output row y
+ * receives device row 4y+3). */
+#define RR_MODE_DOCUMENTED 0
+#define RR_MODE_LITERAL 1
+/* Canvas flavour. GRAY8: Bitmap<Gray8>, exact div255 (pipe/aa.rs:192-208).
+ * RGB8_FAST: Bitmap<Rgb8> on a grey page, (v+255)>>8 (simd/composite.rs:56-114),
+ * stored one byte per pixel because r=g=b stays invariant. */
+#define RR_FLAVOUR_GRAY8 0
+#define RR_FLAVOUR_RGB8_FAST 1
+/* Execution shape of record_fill_paths. */
+#define RR_LAUNCH_PLAIN 0   /* one path per kernel launch, in order */
+#define RR_LAUNCH_BATCHED 1 /* one persistent launch for all paths of the batch */
+
+/* Per-path fill attributes (device array).  Mirrors the arguments of
+ * raster::fill / eo_fill (crates/raster/src/fill/mod.rs:55,83) for a solid
+ * source: fill rule, grey value, PipeState::a_input. */
+typedef struct {
+    uint8_t eo;      /* 1 = even-odd, 0 = non-zero winding */
+    uint8_t color;   /* solid grey source value */
+    uint8_t a_input; /* fill alpha, 255 = opaque */
+    uint8_t route;   /* fill route chosen by the dispatcher (RR_ROUTE_*) */
+} rr_path_attr;
+
+#define RR_ROUTE_SPAN 0
+#define RR_ROUTE_TILE16 1
+#define RR_ROUTE_TILE32 2
+#define RR_ROUTE_TILE64 3
+#define RR_ROUTE_CPU 4
+
+/* One page of paths to fill and composite, in order.  Replaces the per-path
+ * AaFillParams / TileFillParams (crates/gpu/src/backend/params.rs:90-103,
+ * 170-185) + CPU gpu_coverage_to_bitmap (renderer/page/gpu_ops.rs:118-220):
+ * the coverage never leaves the device.
+ *
+ * edges: device, 4 doubles per edge = the (x0,y0,x1,y1) arguments of every
+ *   XPath::add_segment call (crates/raster/src/xpath.rs:324) in order, device
+ *   space, before aa_scale.  rr_xpath_edges() produces them.
+ * edge_off: device, n_paths+1 int64 offsets into edges.
+ * host_rows: host, 2 int32 per path = [first, last] device row touched by the
+ *   path's edges (floor of min/max y); only used to size plain launches.
+ * cov_hash: optional device array of n_paths uint64; each receives the sum of
+ *   rr_cov_hash_term() over the pixels the path composited with coverage > 0.
+ */
+typedef struct {
+    const double *edges;
+    const int64_t *edge_off;
+    const rr_path_attr *attrs;
+    const int32_t *host_rows;
+    int64_t n_paths;
+    int64_t n_edges;
+    uint8_t *canvas; /* device, 1 byte per pixel, row pitch `stride` */
+    int32_t width, height, stride;
+    int32_t mode, flavour, launch;
+    double clip[4]; /* x_min, y_min, x_max, y_max; page clip is {0,0,W-0.001,H-0.001} */
+    uint64_t *cov_hash;
+} rr_fill_batch;
+
+/* ---- context (GpuCtx::init, crates/gpu/src/lib.rs:239; CudaBackend, backend/cuda/mod.rs:27) ---- */
+typedef struct rr_hip_ctx rr_hip_ctx;
+int32_t rr_hip_create(int32_t device, rr_hip_ctx **out);
+int32_t rr_hip_destroy(rr_hip_ctx *ctx);
+const char *rr_hip_last_error(rr_hip_ctx *ctx);
+
+/* ---- buffers (GpuBackend::alloc_device / alloc_device_zeroed / free_device /
+ *      device_buffer_len, backend/mod.rs:241-282) ---- */
+int32_t rr_hip_alloc(rr_hip_ctx *ctx, size_t size, int32_t zeroed, void **buf);
+int32_t rr_hip_free(rr_hip_ctx *ctx, void *buf);
+
+/* ---- transfer (upload_async / download_async + wait, backend/mod.rs:422-457) ---- */
+int32_t rr_hip_upload(rr_hip_ctx *ctx, void *dst, const void *src, size_t n);
+int32_t rr_hip_download(rr_hip_ctx *ctx, void *dst, const void *src, size_t n);
+int32_t rr_hip_memset(rr_hip_ctx *ctx, void *dst, int32_t value, size_t n);
+
+/* ---- page recording (begin_page / record_* / submit_page / wait_page,
+ *      backend/mod.rs:306-405) ---- */
+int32_t rr_hip_begin_page(rr_hip_ctx *ctx);
+int32_t rr_hip_record_fill_paths(rr_hip_ctx *ctx, const rr_fill_batch *batch);
+int32_t rr_hip_submit_page(rr_hip_ctx *ctx, uint64_t *fence);
+int32_t rr_hip_wait_page(rr_hip_ctx *ctx, uint64_t fence);
+/* Milliseconds of device time spent in the fill kernels of the last
+ * submitted page (HIP events on the context stream), and the number of
+ * fill-kernel launches it took. */
+int32_t rr_hip_last_page_stats(rr_hip_ctx *ctx, double *kernel_ms, int64_t *launches);
+
+/* ---- host path ingestion (XPath::new, crates/raster/src/xpath.rs:162-260,
+ *      with flatten_curve, path/flatten.rs:109-215) ----
+ * Turns a PathBuilder point list (pts xy pairs + PathFlags bytes,
+ * path/mod.rs:70-79) into the add_segment edge list the device consumes.
+ * Writes at most cap edges; *n_out receives the full count. */
+int32_t rr_xpath_edges(const double *xy, const uint8_t *flags, int64_t n_pts,
+                       const double matrix[6], double flatness, double *out_edges, int64_t cap,
+                       int64_t *n_out);
+
+/* Coverage-hash term of one composited pixel (index y*width+x, shape 1..255). */
+uint64_t rr_cov_hash_term(int64_t pixel_index, uint8_t shape);
+
+/* ---- synthetic workloads (seeded; SplitMix64) ----
+ * config 1: 500 random polygons on one Letter@300 page, half NZ / half EO.
+ * config 2: 6000 synthetic glyph outlines on one Letter@300 page.
+ * config 3: the mixed 64-page batch (Letter/A3/A4@600/thumbnail).
+ * A workload holds, per page, the PathBuilder point lists and the edge list. */
+typedef struct rr_workload rr_workload;
+int32_t rr_workload_create(int32_t config, uint64_t seed, int32_t max_pages, rr_workload **out);
+void rr_workload_destroy(rr_workload *w);
+int32_t rr_workload_n_pages(const rr_workload *w);
+/* width, height, n_paths, n_edges, n_points of page p. */
+int32_t rr_workload_page_info(const rr_workload *w, int32_t p, int64_t info[5]);
+/* Borrowed host pointers valid until rr_workload_destroy. */
+int32_t rr_workload_page_data(const rr_workload *w, int32_t p, const double **edges,
+                              const int64_t **edge_off, const rr_path_attr **attrs,
+                              const int32_t **host_rows, const double **pts,
+                              const uint8_t **pt_flags, const int64_t **pt_off);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

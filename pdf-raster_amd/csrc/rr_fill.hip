@@ -1,0 +1,894 @@
+// rr_fill.hip -- gfx950 kernels for the anti-aliased scanline fill +
+// composite hot path, and the C-ABI backend that records them.
+//
+// Semantics are those of the reference CPU rasteriser, bit for bit:
+//   edge setup      XPath::add_segment + aa_scale (crates/raster/src/xpath.rs:283-402)
+//                   and the first-row setup of fill_intersections
+//                   (crates/raster/src/scanner/mod.rs:511-587)
+//   entries         per AA row, edges in path order, merged into the last
+//                   entry (write_intersect, scanner/mod.rs:600-642)
+//   row sort        stable by x0 (scanner/mod.rs:230)
+//   winding walk    render_aa_line (scanner/mod.rs:393-467)
+//   coverage        4 AA rows -> nibble popcount -> AA_GAMMA
+//                   (fill/mod.rs:46-48,344-401; simd/aa_coverage.rs:51-69)
+//   composite       Gray8 exact div255 (pipe/aa.rs:192-208) or the Rgb8
+//                   opaque fast path (simd/composite.rs:56-114)
+//
+// Work decomposition: one wave64 owns a band of 16 output rows = 64 AA rows,
+// one lane per AA row.  The wave walks the path's edge records in order 64 at
+// a time (coalesced 32 B records staged in LDS), ballots which overlap the
+// band, and every lane folds its own row's intersection into its entry list
+// (LDS, [slot][lane] so that lanes never conflict).  Entries whose winding
+// contribution cannot change the inside/outside state (count 0, or even under
+// even-odd) are dropped when they are retired: they never start or end a span.
+// The walk turns each row into disjoint sorted intervals; the wave then sweeps
+// the band 256 pixels at a time through an LDS bitmap (one row per lane, no
+// atomics) and composites 4 pixels per lane per dword, touching HBM only for
+// dwords that hold covered pixels.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rr_hip.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// device data layout
+// ---------------------------------------------------------------------------
+
+// One edge after add_segment + aa_scale, reduced to what the per-row integer
+// stepping needs.  32 bytes, so one wave loads 64 of them in 2 KB.
+struct EdgeRec {
+    int32_t r0, r1;  // AA rows [r0, r1] after the clip; empty when r0 > r1
+    int32_t meta;    // bits 0-1 kind (0 sloped, 1 vertical, 2 horizontal),
+                     // bit 2 count +1 (FLIPPED), bit 3 first row has count 0
+    int32_t a;       // sloped: px0 of row r0; vertical: x; horizontal: x0
+    int32_t b;       // sloped: 16.16 right edge at row r0; horizontal: x1
+    int32_t c;       // sloped: dxdy_fp
+    int32_t d, e;    // sloped: 16.16 clamp min / max
+};
+static_assert(sizeof(EdgeRec) == 32, "EdgeRec layout");
+
+enum { KIND_SLOPED = 0, KIND_VERT = 1, KIND_HORIZ = 2 };
+enum { HDR_EMPTY = 1, HDR_OUTSIDE = 2 };
+
+// Scanner bounding box of one path in AA space (scanner/mod.rs:138-169).
+struct PathHdr {
+    int32_t y_min, y_max, x_min, x_max;
+    int32_t flags, pad0, pad1, pad2;
+};
+
+struct ClipRect {
+    double x_min, y_min, x_max, y_max;
+    int32_t x_min_i, y_min_i, x_max_i, y_max_i;
+};
+
+struct FillArgs {
+    const EdgeRec *recs;
+    const PathHdr *hdrs;
+    const int64_t *edge_off;
+    const rr_path_attr *attrs;
+    uint8_t *canvas;
+    uint64_t *cov_hash;
+    int32_t *spill;               // overflow entry pool
+    unsigned long long *spill_top;  // bump allocator (in int32 units)
+    int32_t *err;                 // set to 1 when the pool runs out
+    long long spill_cap;
+    int32_t width, height, stride;
+    int32_t mode, flavour;
+    int32_t px_lo, px_hi;  // clip ∩ canvas, pixels
+};
+
+// AA_GAMMA (crates/raster/src/fill/mod.rs:46-48)
+__constant__ uint8_t kGamma[17] = {0, 4, 11, 21, 32, 45, 59, 74, 90, 108, 126, 145, 166, 187, 209, 231, 255};
+
+// ---------------------------------------------------------------------------
+// exact f64 helpers (crates/color/src/convert.rs:221-264; scanner/mod.rs:51-66)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int32_t sat_i32_of_int_valued(double f) {
+    if (f >= 2147483647.0) return INT32_MAX;
+    if (f <= -2147483648.0) return INT32_MIN;
+    return static_cast<int32_t>(f);
+}
+
+__device__ __forceinline__ int32_t splash_floor(double x) {
+    if (isnan(x)) return INT32_MIN;
+    if (isinf(x)) return x > 0.0 ? INT32_MAX : INT32_MIN;
+    return sat_i32_of_int_valued(floor(x));
+}
+
+__device__ __forceinline__ int32_t fp_from_f64(double x) {
+    const double fp = x * 65536.0;
+    if (fp >= 2147483647.0) return INT32_MAX;
+    if (fp <= -2147483648.0) return INT32_MIN;
+    if (isnan(fp)) return 0;
+    return static_cast<int32_t>(fp);  // truncation toward zero
+}
+
+__device__ __forceinline__ uint64_t bits_of(double v) { return __double_as_longlong(v); }
+
+__device__ __forceinline__ int32_t sat64_to_i32(long long v) {
+    return v > INT32_MAX ? INT32_MAX : (v < INT32_MIN ? INT32_MIN : static_cast<int32_t>(v));
+}
+
+__device__ __forceinline__ uint32_t div255(uint32_t x) {
+    const uint32_t s = (x + (x >> 8) + 0x80u) >> 8;
+    return s < 255u ? s : 255u;
+}
+
+__device__ __forceinline__ uint64_t cov_term(long long pix, uint32_t shape) {
+    uint64_t z = ((static_cast<uint64_t>(pix) << 8) | shape) + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ int32_t floor_div4(int32_t v) { return v >= 0 ? v / 4 : -((-v + 3) / 4); }
+
+// ---------------------------------------------------------------------------
+// kernel 1: per-path edge setup + scanner bounding box
+// ---------------------------------------------------------------------------
+// One workgroup per path.  Each thread takes edges of the path, applies
+// add_segment (flags, dxdy computed before the swap, dxdy_fp = round-half-away
+// of dxdy*65536 saturated) and aa_scale (x4, exact), and emits the integer
+// record of fill_intersections' first row.  The block reduces the scanner
+// bbox over the non-NaN segments that overlap the AA clip rows.
+__global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ edges,
+                                                    const int64_t *__restrict__ edge_off,
+                                                    EdgeRec *__restrict__ recs,
+                                                    PathHdr *__restrict__ hdrs, ClipRect clip) {
+    const int64_t p = blockIdx.x;
+    const int64_t e0 = edge_off[p], e1 = edge_off[p + 1];
+    const int32_t cy0 = clip.y_min_i * 4, cy1 = (clip.y_max_i + 1) * 4 - 1;
+    int32_t xmn = INT32_MAX, xmx = INT32_MIN, ymn = INT32_MAX, ymx = INT32_MIN;
+    int any = 0;
+    for (int64_t i = e0 + threadIdx.x; i < e1; i += blockDim.x) {
+        const double4 q = reinterpret_cast<const double4 *>(edges)[i];
+        double x0 = q.x, y0 = q.y, x1 = q.z, y1 = q.w;
+        const bool horiz = bits_of(y0) == bits_of(y1);
+        const bool vert = bits_of(x0) == bits_of(x1);
+        double dxdy = 0.0;
+        int32_t dxdy_fp = 0;
+        bool flipped = false;
+        if (!horiz) {
+            if (!vert) dxdy = (x1 - x0) / (y1 - y0);
+            if (y0 > y1) {
+                double t = x0;
+                x0 = x1;
+                x1 = t;
+                t = y0;
+                y0 = y1;
+                y1 = t;
+                flipped = true;
+            }
+            const double f = dxdy * 65536.0;
+            if (f >= 2147483647.0)
+                dxdy_fp = INT32_MAX;
+            else if (f <= -2147483648.0)
+                dxdy_fp = INT32_MIN;
+            else if (isnan(f))
+                dxdy_fp = 0;
+            else
+                dxdy_fp = static_cast<int32_t>(round(f));
+        }
+        x0 *= 4.0;
+        y0 *= 4.0;
+        x1 *= 4.0;
+        y1 *= 4.0;
+        const double sy_min = fmin(y0, y1), sy_max = fmax(y0, y1);
+        if (!(isnan(x0) || isnan(y0) || isnan(x1) || isnan(y1)) &&
+            !(sy_min >= static_cast<double>(cy1) + 1.0 || sy_max < static_cast<double>(cy0))) {
+            any = 1;
+            xmn = min(xmn, splash_floor(fmin(x0, x1)));
+            xmx = max(xmx, splash_floor(fmax(x0, x1)));
+            ymn = min(ymn, splash_floor(sy_min));
+            ymx = max(ymx, splash_floor(sy_max));
+        }
+        EdgeRec r{1, 0, 0, 0, 0, 0, 0, 0};
+        if (!(isnan(x0) || isnan(y0))) {
+            const int32_t fy0 = splash_floor(sy_min), fy1 = splash_floor(sy_max);
+            if (horiz) {
+                if (fy0 >= cy0 && fy0 <= cy1) {
+                    r.r0 = r.r1 = fy0;
+                    r.meta = KIND_HORIZ;
+                    r.a = splash_floor(fmin(x0, x1));
+                    r.b = splash_floor(fmax(x0, x1));
+                }
+            } else {
+                r.r0 = max(fy0, cy0);
+                r.r1 = min(fy1, cy1);
+                const int32_t first_zero = (sy_min < static_cast<double>(r.r0)) ? 0 : 8;
+                const int32_t pos = flipped ? 4 : 0;
+                if (vert) {
+                    r.meta = KIND_VERT | pos | first_zero;
+                    r.a = splash_floor(x0);
+                } else {
+                    r.meta = KIND_SLOPED | pos | first_zero;
+                    const double x_base = fma(y0, -dxdy, x0);
+                    const double sx_min = fmin(x0, x1), sx_max = fmax(x0, x1);
+                    const double rr0 = static_cast<double>(r.r0);
+                    double xx = (rr0 > y0) ? fma(rr0, dxdy, x_base) : x0;
+                    if (xx < sx_min) xx = sx_min;
+                    if (xx > sx_max) xx = sx_max;
+                    r.a = splash_floor(xx);
+                    r.b = fp_from_f64(fma(rr0 + 1.0, dxdy, x_base));
+                    r.c = dxdy_fp;
+                    r.d = fp_from_f64(sx_min);
+                    r.e = fp_from_f64(sx_max);
+                }
+            }
+        }
+        recs[i] = r;
+    }
+    __shared__ int32_t red[4][256];
+    __shared__ int32_t red_any[256];
+    red[0][threadIdx.x] = xmn;
+    red[1][threadIdx.x] = xmx;
+    red[2][threadIdx.x] = ymn;
+    red[3][threadIdx.x] = ymx;
+    red_any[threadIdx.x] = any;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            red[0][threadIdx.x] = min(red[0][threadIdx.x], red[0][threadIdx.x + s]);
+            red[1][threadIdx.x] = max(red[1][threadIdx.x], red[1][threadIdx.x + s]);
+            red[2][threadIdx.x] = min(red[2][threadIdx.x], red[2][threadIdx.x + s]);
+            red[3][threadIdx.x] = max(red[3][threadIdx.x], red[3][threadIdx.x + s]);
+            red_any[threadIdx.x] |= red_any[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        PathHdr h{0, -1, 1, 0, HDR_EMPTY, 0, 0, 0};
+        if (red_any[0]) {
+            const int32_t y_min = max(red[2][0], cy0), y_max = min(red[3][0], cy1);
+            if (y_min <= y_max) {
+                h.y_min = y_min;
+                h.y_max = y_max;
+                h.x_min = red[0][0];
+                h.x_max = red[1][0];
+                h.flags = 0;
+                // Clip::test_rect on the pixel bbox (clip.rs:196-220; fill/mod.rs:156-170)
+                const int32_t l = h.x_min / 4, t = h.y_min / 4, rt = h.x_max / 4, b = h.y_max / 4;
+                if (static_cast<double>(rt) + 1.0 <= clip.x_min || static_cast<double>(l) >= clip.x_max ||
+                    static_cast<double>(b) + 1.0 <= clip.y_min || static_cast<double>(t) >= clip.y_max)
+                    h.flags = HDR_OUTSIDE;
+            }
+        }
+        hdrs[p] = h;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kernel 2: scanline fill + composite of one band of 16 output rows
+// ---------------------------------------------------------------------------
+constexpr int kBandRows = 16;   // output rows per wave
+constexpr int kChunkPx = 256;   // pixels per composite sweep step
+constexpr int kBmPitch = 33;    // dwords per bitmap row (32 + 1 pad against bank conflicts)
+
+template <int K>
+struct BandLds {
+    EdgeRec rec[64];
+    int32_t ex0[K * 64];
+    int32_t ex1[K * 64];
+    int32_t ec[K * 64];
+    uint32_t bm[64 * kBmPitch];
+    uint8_t gamma[32];
+};
+
+struct Entry {
+    int32_t x0, x1, c;
+};
+
+// Intersection of edge record q with AA row r (fill_intersections,
+// scanner/mod.rs:528-585, with the per-row 16.16 stepping in closed form:
+// the saturating adds of a constant step are absorbing, so step k equals the
+// clamped 64-bit sum).
+__device__ __forceinline__ Entry row_entry(const EdgeRec &q, int32_t r) {
+    Entry en;
+    const int kind = q.meta & 3;
+    const int32_t cnt = (q.meta & 4) ? 1 : -1;
+    if (kind == KIND_HORIZ) {
+        en.x0 = q.a;
+        en.x1 = q.b;
+        en.c = 0;
+        return en;
+    }
+    en.c = ((q.meta & 8) && r == q.r0) ? 0 : cnt;
+    if (kind == KIND_VERT) {
+        en.x0 = en.x1 = q.a;
+        return en;
+    }
+    const long long k = static_cast<long long>(r) - q.r0;
+    const long long t1 = static_cast<long long>(q.b) + k * q.c;
+    int32_t v1 = sat64_to_i32(t1);
+    v1 = v1 < q.d ? q.d : (v1 > q.e ? q.e : v1);
+    const int32_t px1 = v1 >> 16;
+    int32_t px0;
+    if (k == 0) {
+        px0 = q.a;
+    } else {
+        int32_t v0 = sat64_to_i32(t1 - q.c);
+        v0 = v0 < q.d ? q.d : (v0 > q.e ? q.e : v0);
+        px0 = v0 >> 16;
+    }
+    en.x0 = min(px0, px1);
+    en.x1 = max(px0, px1);
+    return en;
+}
+
+// Walks the path's edges for this wave's band and builds each lane's list of
+// retired, state-changing entries in X0/X1/C ([slot * 64 + lane]).  Returns
+// the number of entries this lane produced (may exceed cap: then only the
+// first cap are stored and the caller reruns with more room).
+__device__ __forceinline__ int build_entries(const FillArgs &A, EdgeRec *stage, int64_t e0, int64_t e1,
+                                             int32_t band_lo, int32_t band_hi, bool valid, int32_t r,
+                                             int32_t mrule, int32_t *X0, int32_t *X1, int32_t *C,
+                                             int cap) {
+    const int lane = threadIdx.x;
+    int n = 0;
+    bool has_last = false;
+    int32_t lx0 = 0, lx1 = 0, lc = 0;
+    auto retire = [&](int32_t x0, int32_t x1, int32_t c) {
+        if ((c & mrule) == 0) return;  // no effect on the walk
+        if (n < cap) {
+            X0[n * 64 + lane] = x0;
+            X1[n * 64 + lane] = x1;
+            C[n * 64 + lane] = c;
+        }
+        ++n;
+    };
+    for (int64_t base = e0; base < e1; base += 64) {
+        const int64_t i = base + lane;
+        EdgeRec q{1, 0, 0, 0, 0, 0, 0, 0};
+        bool ov = false;
+        if (i < e1) {
+            q = A.recs[i];
+            ov = q.r0 <= q.r1 && q.r0 <= band_hi && q.r1 >= band_lo;
+        }
+        stage[lane] = q;
+        uint64_t mask = __ballot(ov);
+        __syncthreads();
+        while (mask) {
+            const int j = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+            mask &= mask - 1;
+            const EdgeRec g = stage[j];
+            if (valid && r >= g.r0 && r <= g.r1) {
+                const Entry en = row_entry(g, r);
+                if (has_last && static_cast<long long>(lx1) + 1 >= en.x0 &&
+                    static_cast<long long>(lx0) <= static_cast<long long>(en.x1) + 1) {
+                    lc = static_cast<int32_t>(static_cast<uint32_t>(lc) + static_cast<uint32_t>(en.c));
+                    lx0 = min(lx0, en.x0);
+                    lx1 = max(lx1, en.x1);
+                } else {
+                    if (has_last) retire(lx0, lx1, lc);
+                    lx0 = en.x0;
+                    lx1 = en.x1;
+                    lc = en.c;
+                    has_last = true;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (has_last) retire(lx0, lx1, lc);
+    return n;
+}
+
+// Stable insertion sort by x0, then the winding walk of render_aa_line.  The
+// spans of the row are OR-ed into the AaBuf there; here they are merged into
+// disjoint sorted intervals [s, e) clamped to [0, aa_width), written back in
+// place over X0 (s) / X1 (e).  Returns the interval count; *unbalanced is set
+// when the winding never returned to zero (the reference would loop forever
+// there; the span runs to aa_width and the row stops).
+__device__ __forceinline__ int sort_and_walk(int n, int32_t *X0, int32_t *X1, int32_t *C, int32_t mrule,
+                                             int32_t aa_width, bool *unbalanced) {
+    const int lane = threadIdx.x;
+    for (int a = 1; a < n; ++a) {
+        const int32_t vx0 = X0[a * 64 + lane], vx1 = X1[a * 64 + lane], vc = C[a * 64 + lane];
+        int b = a;
+        while (b > 0) {
+            const int32_t px = X0[(b - 1) * 64 + lane];
+            if (px <= vx0) break;
+            X0[b * 64 + lane] = px;
+            X1[b * 64 + lane] = X1[(b - 1) * 64 + lane];
+            C[b * 64 + lane] = C[(b - 1) * 64 + lane];
+            --b;
+        }
+        if (b != a) {
+            X0[b * 64 + lane] = vx0;
+            X1[b * 64 + lane] = vx1;
+            C[b * 64 + lane] = vc;
+        }
+    }
+    int i = 0, nout = 0;
+    int32_t cs = 0, ce = -1;
+    int32_t count = 0;
+    while (i < n) {
+        // outside: every stored entry changes the state, so it opens a span
+        const int32_t xx = X0[i * 64 + lane];
+        count = static_cast<int32_t>(static_cast<uint32_t>(count) + static_cast<uint32_t>(C[i * 64 + lane]));
+        ++i;
+        int32_t end = aa_width;
+        bool closed = false;
+        while (i < n) {
+            count = static_cast<int32_t>(static_cast<uint32_t>(count) + static_cast<uint32_t>(C[i * 64 + lane]));
+            const int32_t x1 = X1[i * 64 + lane];
+            ++i;
+            if ((count & mrule) == 0) {
+                end = x1 == INT32_MAX ? INT32_MAX : x1 + 1;
+                closed = true;
+                break;
+            }
+        }
+        const int32_t bx0 = xx > 0 ? xx : 0;
+        int32_t bx1 = end < aa_width ? end : aa_width;
+        if (bx1 < 0) bx1 = 0;
+        if (bx0 < bx1) {
+            if (ce >= 0 && bx0 <= ce) {
+                ce = max(ce, bx1);
+            } else {
+                if (ce >= 0) {
+                    X0[nout * 64 + lane] = cs;
+                    X1[nout * 64 + lane] = ce;
+                    ++nout;
+                }
+                cs = bx0;
+                ce = bx1;
+            }
+        }
+        if (!closed) {
+            *unbalanced = true;
+            break;
+        }
+    }
+    if (ce >= 0) {
+        X0[nout * 64 + lane] = cs;
+        X1[nout * 64 + lane] = ce;
+        ++nout;
+    }
+    return nout;
+}
+
+// Rasterises path p over output rows [Y0, Y0+15] and composites into the
+// canvas.  Called by one full wave64 (blockDim.x == 64).
+template <int K>
+__device__ void fill_band(const FillArgs &A, BandLds<K> &L, int64_t p, int32_t Y0) {
+    const int lane = threadIdx.x;
+    const PathHdr h = A.hdrs[p];
+    if (h.flags & (HDR_EMPTY | HDR_OUTSIDE)) return;
+    const rr_path_attr at = A.attrs[p];
+    const bool literal = A.mode == RR_MODE_LITERAL;
+    // drawn output rows (fill/mod.rs:176-203 literal; Splash per device row documented)
+    int32_t ylo, yhi;
+    if (literal) {
+        ylo = floor_div4(h.y_min - 3 + 3);  // ceil((y_min-3)/4)
+        yhi = floor_div4(h.y_max - 3);
+    } else {
+        ylo = h.y_min / 4;
+        yhi = h.y_max / 4;
+    }
+    ylo = max(ylo, 0);
+    yhi = min(yhi, A.height - 1);
+    if (Y0 > yhi || Y0 + kBandRows - 1 < ylo) return;
+
+    const int32_t d = lane >> 2, j = lane & 3;
+    const int32_t y = Y0 + d;
+    const int32_t r = literal ? 16 * y + 12 + j : 4 * y + j;
+    const int32_t band_lo = literal ? 16 * Y0 + 12 : 4 * Y0;
+    const int32_t band_hi = literal ? 16 * (Y0 + kBandRows - 1) + 15 : 4 * Y0 + 4 * kBandRows - 1;
+    const bool valid = y >= ylo && y <= yhi && r >= h.y_min && r <= h.y_max;
+    const int32_t mrule = at.eo ? 1 : -1;
+    const int32_t aa_width = A.width * 4;
+    const int64_t e0 = A.edge_off[p], e1 = A.edge_off[p + 1];
+
+    if (lane < 17) L.gamma[lane] = kGamma[lane];
+
+    int32_t *X0 = L.ex0, *X1 = L.ex1, *C = L.ec;
+    int n = build_entries(A, L.rec, e0, e1, band_lo, band_hi, valid, r, mrule, X0, X1, C, K);
+    int need = n;
+    for (int o = 32; o > 0; o >>= 1) need = max(need, __shfl_xor(need, o));
+    int32_t *G = nullptr;
+    if (need > K) {
+        // Rare: a row with more state-changing entries than the LDS holds.
+        // Redo the band with the entry lists in a slice of the global pool.
+        unsigned long long off = 0;
+        if (lane == 0) off = atomicAdd(A.spill_top, static_cast<unsigned long long>(need) * 64ull * 3ull);
+        off = __shfl(off, 0);
+        if (static_cast<long long>(off + static_cast<unsigned long long>(need) * 192ull) > A.spill_cap) {
+            if (lane == 0) atomicExch(A.err, 1);
+            return;
+        }
+        G = A.spill + off;
+        X0 = G;
+        X1 = G + static_cast<long long>(need) * 64;
+        C = G + static_cast<long long>(need) * 128;
+        n = build_entries(A, L.rec, e0, e1, band_lo, band_hi, valid, r, mrule, X0, X1, C, need);
+    }
+    bool unb = false;
+    const int nint = sort_and_walk(n, X0, X1, C, mrule, aa_width, &unb);
+    const bool any_unb = __ballot(unb) != 0;
+
+    // pixel range that can hold coverage, intersected with the clip
+    int32_t px_lo = max(A.px_lo, floor_div4(h.x_min));
+    int32_t px_hi = min(A.px_hi, any_unb ? A.width - 1 : floor_div4(h.x_max));
+    if (px_lo > px_hi) return;
+    const int32_t xs = px_lo & ~3;
+    const uint32_t a_in = at.a_input, src = at.color;
+    uint64_t hsum = 0;
+    int ip = 0;  // first interval of this lane not yet fully left of the chunk
+    uint32_t *bm_row = L.bm + lane * kBmPitch;
+    for (int32_t cx = xs; cx <= px_hi; cx += kChunkPx) {
+        const int32_t A0 = 4 * cx, A1 = A0 + 4 * kChunkPx;
+        for (int w = 0; w < 32; ++w) bm_row[w] = 0u;
+        while (ip < nint && X1[ip * 64 + lane] <= A0) ++ip;
+        for (int k = ip; k < nint; ++k) {
+            const int32_t s = X0[k * 64 + lane];
+            if (s >= A1) break;
+            const int32_t lo = max(s, A0) - A0, hi = min(X1[k * 64 + lane], A1) - A0;
+            const int wl = lo >> 5, wh = (hi - 1) >> 5;
+            const uint32_t ml = ~0u << (lo & 31);
+            const uint32_t mh = (hi & 31) ? ((1u << (hi & 31)) - 1u) : ~0u;
+            if (wl == wh) {
+                bm_row[wl] |= ml & mh;
+            } else {
+                bm_row[wl] |= ml;
+                for (int w = wl + 1; w < wh; ++w) bm_row[w] = ~0u;
+                bm_row[wh] |= mh;
+            }
+        }
+        __syncthreads();
+        const int32_t xq = cx + 4 * lane;  // this lane's 4 pixels
+        for (int dd = 0; dd < kBandRows; ++dd) {
+            const int32_t yy = Y0 + dd;
+            if (yy < ylo || yy > yhi) continue;
+            if (xq > px_hi) continue;
+            const int sh = (lane & 1) * 16;
+            uint32_t cov = 0;
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                const uint32_t f = (L.bm[(4 * dd + rr) * kBmPitch + (lane >> 1)] >> sh) & 0xffffu;
+                uint32_t t = f - ((f >> 1) & 0x5555u);
+                t = (t & 0x3333u) + ((t >> 2) & 0x3333u);
+                cov += (t & 0xfu) | ((t & 0xf0u) << 4) | ((t & 0xf00u) << 8) | ((t & 0xf000u) << 12);
+            }
+            if (cov == 0) continue;
+            uint32_t *dst = reinterpret_cast<uint32_t *>(A.canvas + static_cast<long long>(yy) * A.stride + xq);
+            const uint32_t old = *dst;
+            uint32_t out = old;
+#pragma unroll
+            for (int px = 0; px < 4; ++px) {
+                const uint32_t cnt = (cov >> (8 * px)) & 0xffu;
+                const int32_t x = xq + px;
+                if (cnt == 0 || x < px_lo || x > px_hi) continue;
+                const uint32_t shape = L.gamma[cnt];
+                const uint32_t dv = (old >> (8 * px)) & 0xffu;
+                uint32_t res;
+                if (A.flavour == RR_FLAVOUR_RGB8_FAST) {
+                    const uint32_t a = (a_in * shape + 255u) >> 8;
+                    res = ((255u - a) * dv + a * src + 255u) >> 8;
+                } else {
+                    const uint32_t a = div255(a_in * shape);
+                    res = div255((255u - a) * dv + a * src);
+                }
+                out = (out & ~(0xffu << (8 * px))) | (res << (8 * px));
+                if (A.cov_hash) hsum += cov_term(static_cast<long long>(yy) * A.width + x, shape);
+            }
+            if (out != old) *dst = out;
+        }
+        __syncthreads();
+    }
+    if (A.cov_hash) {
+        for (int o = 32; o > 0; o >>= 1) hsum += __shfl_xor(hsum, o);
+        if (lane == 0 && hsum) atomicAdd(reinterpret_cast<unsigned long long *>(A.cov_hash + p), hsum);
+    }
+}
+
+constexpr int kLdsK = 32;
+
+// Plain route: one launch per path, one wave per band of 16 output rows.
+__global__ void __launch_bounds__(64) k_fill_plain(FillArgs A, int64_t p, int32_t row0) {
+    __shared__ BandLds<kLdsK> L;
+    fill_band<kLdsK>(A, L, p, row0 + kBandRows * static_cast<int32_t>(blockIdx.x));
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct Fence {
+    hipEvent_t ev = nullptr;
+    uint64_t id = 0;
+};
+
+}  // namespace
+
+struct rr_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // workspace
+    EdgeRec *recs = nullptr;
+    size_t recs_cap = 0;
+    PathHdr *hdrs = nullptr;
+    size_t hdrs_cap = 0;
+    int32_t *spill = nullptr;
+    long long spill_cap = 0;
+    unsigned long long *spill_top = nullptr;
+    int32_t *err_flag = nullptr;
+    // page state
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    bool page_open = false, page_timed = false;
+    int64_t launches = 0;
+    double last_ms = 0.0;
+    int64_t last_launches = 0;
+    Fence fences[16];
+    uint64_t next_fence = 1;
+};
+
+namespace {
+
+int32_t fail(rr_hip_ctx *c, int32_t code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int32_t hip_fail(rr_hip_ctx *c, hipError_t e, const char *what) {
+    return fail(c, RR_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define RR_HIP_TRY(ctx, expr)                                  \
+    do {                                                       \
+        hipError_t e_ = (expr);                                \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+    } while (0)
+
+template <typename T>
+int32_t ensure(rr_hip_ctx *c, T **buf, size_t *cap, size_t need) {
+    if (need <= *cap) return RR_OK;
+    size_t n = *cap ? *cap : 1024;
+    while (n < need) n *= 2;
+    if (*buf) {
+        RR_HIP_TRY(c, hipStreamSynchronize(c->stream));
+        RR_HIP_TRY(c, hipFree(*buf));
+    }
+    *buf = nullptr;
+    *cap = 0;
+    RR_HIP_TRY(c, hipMalloc(reinterpret_cast<void **>(buf), n * sizeof(T)));
+    *cap = n;
+    return RR_OK;
+}
+
+ClipRect make_clip(const double cl[4]) {
+    ClipRect r;
+    r.x_min = std::fmin(cl[0], cl[2]);
+    r.x_max = std::fmax(cl[0], cl[2]);
+    r.y_min = std::fmin(cl[1], cl[3]);
+    r.y_max = std::fmax(cl[1], cl[3]);
+    auto sat = [](double v) -> int32_t {
+        if (std::isnan(v)) return INT32_MIN;
+        if (v >= 2147483647.0) return INT32_MAX;
+        if (v <= -2147483648.0) return INT32_MIN;
+        return static_cast<int32_t>(v);
+    };
+    r.x_min_i = sat(std::floor(r.x_min));
+    r.y_min_i = sat(std::floor(r.y_min));
+    r.x_max_i = sat(std::ceil(r.x_max)) - 1;
+    r.y_max_i = sat(std::ceil(r.y_max)) - 1;
+    return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t rr_hip_create(int32_t device, rr_hip_ctx **out) {
+    if (!out) return RR_E_INVARIANT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return RR_E_NO_DEVICE;
+    rr_hip_ctx *c = new rr_hip_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return RR_E_NO_DEVICE;
+    }
+    int32_t rc;
+    if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess) rc = RR_E_DEVICE;
+    else if (hipMalloc(&c->spill_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
+    else if (hipMalloc(&c->err_flag, sizeof(int32_t)) != hipSuccess) rc = RR_E_DEVICE;
+    else rc = RR_OK;
+    for (auto &f : c->fences)
+        if (rc == RR_OK && hipEventCreateWithFlags(&f.ev, hipEventDisableTiming) != hipSuccess) rc = RR_E_DEVICE;
+    if (rc != RR_OK) {
+        rr_hip_destroy(c);
+        return rc;
+    }
+    const long long pool_ints = 64ll << 20;  // 256 MiB overflow-entry pool
+    if (hipMalloc(&c->spill, pool_ints * sizeof(int32_t)) != hipSuccess) {
+        rr_hip_destroy(c);
+        return RR_E_DEVICE;
+    }
+    c->spill_cap = pool_ints;
+    hipMemset(c->err_flag, 0, sizeof(int32_t));
+    *out = c;
+    return RR_OK;
+}
+
+int32_t rr_hip_destroy(rr_hip_ctx *c) {
+    if (!c) return RR_E_INVARIANT;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    hipFree(c->recs);
+    hipFree(c->hdrs);
+    hipFree(c->spill);
+    hipFree(c->spill_top);
+    hipFree(c->err_flag);
+    if (c->t0) hipEventDestroy(c->t0);
+    if (c->t1) hipEventDestroy(c->t1);
+    for (auto &f : c->fences)
+        if (f.ev) hipEventDestroy(f.ev);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return RR_OK;
+}
+
+const char *rr_hip_last_error(rr_hip_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int32_t rr_hip_alloc(rr_hip_ctx *c, size_t size, int32_t zeroed, void **buf) {
+    if (!c || !buf) return RR_E_INVARIANT;
+    if (size == 0) return fail(c, RR_E_ZERO_SIZE_ALLOC, "rr_hip_alloc called with size = 0");
+    RR_HIP_TRY(c, hipSetDevice(c->device));
+    RR_HIP_TRY(c, hipMalloc(buf, size));
+    if (zeroed) RR_HIP_TRY(c, hipMemsetAsync(*buf, 0, size, c->stream));
+    return RR_OK;
+}
+
+int32_t rr_hip_free(rr_hip_ctx *c, void *buf) {
+    if (!c) return RR_E_INVARIANT;
+    RR_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    RR_HIP_TRY(c, hipFree(buf));
+    return RR_OK;
+}
+
+int32_t rr_hip_upload(rr_hip_ctx *c, void *dst, const void *src, size_t n) {
+    if (!c || (n && (!dst || !src))) return RR_E_INVARIANT;
+    RR_HIP_TRY(c, hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream));
+    RR_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return RR_OK;
+}
+
+int32_t rr_hip_download(rr_hip_ctx *c, void *dst, const void *src, size_t n) {
+    if (!c || (n && (!dst || !src))) return RR_E_INVARIANT;
+    RR_HIP_TRY(c, hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, c->stream));
+    RR_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return RR_OK;
+}
+
+int32_t rr_hip_memset(rr_hip_ctx *c, void *dst, int32_t value, size_t n) {
+    if (!c || (n && !dst)) return RR_E_INVARIANT;
+    RR_HIP_TRY(c, hipMemsetAsync(dst, value, n, c->stream));
+    return RR_OK;
+}
+
+int32_t rr_hip_begin_page(rr_hip_ctx *c) {
+    if (!c) return RR_E_INVARIANT;
+    c->page_open = true;
+    c->page_timed = false;
+    c->launches = 0;
+    RR_HIP_TRY(c, hipMemsetAsync(c->err_flag, 0, sizeof(int32_t), c->stream));
+    return RR_OK;
+}
+
+int32_t rr_hip_record_fill_paths(rr_hip_ctx *c, const rr_fill_batch *b) {
+    if (!c || !b) return RR_E_INVARIANT;
+    if (!c->page_open) return fail(c, RR_E_INVARIANT, "record_fill_paths outside begin_page/submit_page");
+    if (b->n_paths < 0 || b->n_edges < 0 || b->width <= 0 || b->height <= 0 || b->stride < ((b->width + 3) & ~3) ||
+        (b->stride & 3) != 0)
+        return fail(c, RR_E_INVARIANT, "fill batch: bad canvas geometry (stride must be a multiple of 4 >= width)");
+    if (b->mode != RR_MODE_DOCUMENTED && b->mode != RR_MODE_LITERAL)
+        return fail(c, RR_E_INVARIANT, "fill batch: unknown AA row mode");
+    if (b->flavour != RR_FLAVOUR_GRAY8 && b->flavour != RR_FLAVOUR_RGB8_FAST)
+        return fail(c, RR_E_INVARIANT, "fill batch: unknown canvas flavour");
+    if (b->launch != RR_LAUNCH_PLAIN) return fail(c, RR_E_INVARIANT, "fill batch: launch shape not available");
+    if (b->n_paths == 0) return RR_OK;
+    if (!b->edges || !b->edge_off || !b->attrs || !b->canvas || !b->host_rows)
+        return fail(c, RR_E_INVARIANT, "fill batch: null buffer");
+    if (reinterpret_cast<uintptr_t>(b->canvas) & 3u)
+        return fail(c, RR_E_INVARIANT, "fill batch: canvas must be 4-byte aligned");
+    if (reinterpret_cast<uintptr_t>(b->edges) & 31u)
+        return fail(c, RR_E_INVARIANT, "fill batch: edges must be 32-byte aligned");
+    RR_HIP_TRY(c, hipSetDevice(c->device));
+    int32_t rc = ensure(c, &c->recs, &c->recs_cap, static_cast<size_t>(b->n_edges > 0 ? b->n_edges : 1));
+    if (rc) return rc;
+    rc = ensure(c, &c->hdrs, &c->hdrs_cap, static_cast<size_t>(b->n_paths));
+    if (rc) return rc;
+
+    const ClipRect clip = make_clip(b->clip);
+    FillArgs A{};
+    A.recs = c->recs;
+    A.hdrs = c->hdrs;
+    A.edge_off = b->edge_off;
+    A.attrs = b->attrs;
+    A.canvas = b->canvas;
+    A.cov_hash = b->cov_hash;
+    A.spill = c->spill;
+    A.spill_top = c->spill_top;
+    A.err = c->err_flag;
+    A.spill_cap = c->spill_cap;
+    A.width = b->width;
+    A.height = b->height;
+    A.stride = b->stride;
+    A.mode = b->mode;
+    A.flavour = b->flavour;
+    A.px_lo = std::max(0, clip.x_min_i);
+    A.px_hi = std::min(b->width - 1, clip.x_max_i);
+
+    if (!c->page_timed) {
+        RR_HIP_TRY(c, hipEventRecord(c->t0, c->stream));
+        c->page_timed = true;
+    }
+    RR_HIP_TRY(c, hipMemsetAsync(c->spill_top, 0, sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_edge_setup, dim3(static_cast<unsigned>(b->n_paths)), dim3(256), 0, c->stream, b->edges,
+                       b->edge_off, c->recs, c->hdrs, clip);
+    RR_HIP_TRY(c, hipGetLastError());
+    ++c->launches;
+    for (int64_t p = 0; p < b->n_paths; ++p) {
+        int32_t r0 = b->host_rows[2 * p], r1 = b->host_rows[2 * p + 1];
+        r0 = std::max(r0, 0);
+        r1 = std::min(r1, b->height - 1);
+        if (r0 > r1) continue;
+        const unsigned bands = static_cast<unsigned>((r1 - r0) / kBandRows + 1);
+        hipLaunchKernelGGL(k_fill_plain, dim3(bands), dim3(64), 0, c->stream, A, p, r0);
+        ++c->launches;
+    }
+    RR_HIP_TRY(c, hipGetLastError());
+    return RR_OK;
+}
+
+int32_t rr_hip_submit_page(rr_hip_ctx *c, uint64_t *fence) {
+    if (!c || !fence) return RR_E_INVARIANT;
+    if (!c->page_open) return fail(c, RR_E_INVARIANT, "submit_page without begin_page");
+    if (c->page_timed) RR_HIP_TRY(c, hipEventRecord(c->t1, c->stream));
+    Fence &f = c->fences[c->next_fence % 16];
+    RR_HIP_TRY(c, hipEventRecord(f.ev, c->stream));
+    f.id = c->next_fence;
+    *fence = c->next_fence++;
+    c->page_open = false;
+    c->last_launches = c->launches;
+    return RR_OK;
+}
+
+int32_t rr_hip_wait_page(rr_hip_ctx *c, uint64_t fence) {
+    if (!c) return RR_E_INVARIANT;
+    Fence &f = c->fences[fence % 16];
+    if (f.id != fence) {
+        RR_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    } else {
+        RR_HIP_TRY(c, hipEventSynchronize(f.ev));
+    }
+    int32_t flag = 0;
+    RR_HIP_TRY(c, hipMemcpy(&flag, c->err_flag, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (c->page_timed) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->t0, c->t1) == hipSuccess) c->last_ms = ms;
+    }
+    if (flag) return fail(c, RR_E_CAPACITY, "overflow-entry pool exhausted: page output is incomplete");
+    return RR_OK;
+}
+
+int32_t rr_hip_last_page_stats(rr_hip_ctx *c, double *kernel_ms, int64_t *launches) {
+    if (!c) return RR_E_INVARIANT;
+    if (kernel_ms) *kernel_ms = c->last_ms;
+    if (launches) *launches = c->last_launches;
+    return RR_OK;
+}
+
+}  // extern "C"

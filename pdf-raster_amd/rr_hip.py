@@ -1,0 +1,327 @@
+"""ctypes binding of the HIP backend's C-ABI (include/rr_hip.h).
+
+This is the host-side harness the tests and bench use; it holds no compute of
+its own.  Every fill runs in librr_hip.so's gfx950 kernels, and loading fails
+loudly when the library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librr_hip.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "rr_hip.h")
+
+RR_OK = 0
+MODE_DOCUMENTED, MODE_LITERAL = 0, 1
+FLAVOUR_GRAY8, FLAVOUR_RGB8_FAST = 0, 1
+LAUNCH_PLAIN, LAUNCH_BATCHED = 0, 1
+ROUTE_SPAN, ROUTE_TILE16, ROUTE_TILE32, ROUTE_TILE64, ROUTE_CPU = 0, 1, 2, 3, 4
+
+PATH_ATTR_DTYPE = np.dtype([("eo", np.uint8), ("color", np.uint8), ("a_input", np.uint8),
+                            ("route", np.uint8)])
+
+
+class RrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rr_hip error {code}: {msg}")
+        self.code = code
+
+
+class FillBatch(ctypes.Structure):
+    _fields_ = [("edges", ctypes.c_void_p), ("edge_off", ctypes.c_void_p),
+                ("attrs", ctypes.c_void_p), ("host_rows", ctypes.c_void_p),
+                ("n_paths", ctypes.c_int64), ("n_edges", ctypes.c_int64),
+                ("canvas", ctypes.c_void_p), ("width", ctypes.c_int32),
+                ("height", ctypes.c_int32), ("stride", ctypes.c_int32), ("mode", ctypes.c_int32),
+                ("flavour", ctypes.c_int32), ("launch", ctypes.c_int32),
+                ("clip", ctypes.c_double * 4), ("cov_hash", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    """Load librr_hip.so (built by __graft_entry__.build()); raise if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        P, i32, i64, u64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64
+        sig = {
+            "rr_hip_create": (i32, [i32, ctypes.POINTER(ctypes.c_void_p)]),
+            "rr_hip_destroy": (i32, [P]),
+            "rr_hip_last_error": (ctypes.c_char_p, [P]),
+            "rr_hip_alloc": (i32, [P, ctypes.c_size_t, i32, ctypes.POINTER(ctypes.c_void_p)]),
+            "rr_hip_free": (i32, [P, P]),
+            "rr_hip_upload": (i32, [P, P, P, ctypes.c_size_t]),
+            "rr_hip_download": (i32, [P, P, P, ctypes.c_size_t]),
+            "rr_hip_memset": (i32, [P, P, i32, ctypes.c_size_t]),
+            "rr_hip_begin_page": (i32, [P]),
+            "rr_hip_record_fill_paths": (i32, [P, ctypes.POINTER(FillBatch)]),
+            "rr_hip_submit_page": (i32, [P, ctypes.POINTER(u64)]),
+            "rr_hip_wait_page": (i32, [P, u64]),
+            "rr_hip_last_page_stats": (i32, [P, ctypes.POINTER(ctypes.c_double),
+                                             ctypes.POINTER(i64)]),
+            "rr_xpath_edges": (i32, [P, P, i64, P, ctypes.c_double, P, i64,
+                                     ctypes.POINTER(i64)]),
+            "rr_cov_hash_term": (u64, [i64, ctypes.c_uint8]),
+            "rr_workload_create": (i32, [i32, u64, i32, ctypes.POINTER(ctypes.c_void_p)]),
+            "rr_workload_destroy": (None, [P]),
+            "rr_workload_n_pages": (i32, [P]),
+            "rr_workload_page_info": (i32, [P, i32, P]),
+            "rr_workload_page_data": (i32, [P, i32] + [ctypes.POINTER(ctypes.c_void_p)] * 7),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names declared in include/rr_hip.h (functions only)."""
+    import re
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*[A-Za-z_][\w\s\*]*?\b(rr_\w+)\s*\(", txt, re.M)))
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def xpath_edges(xy, flags, matrix=(1.0, 0.0, 0.0, 1.0, 0.0, 0.0), flatness=0.1):
+    xy = np.ascontiguousarray(xy, np.float64)
+    flags = np.ascontiguousarray(flags, np.uint8)
+    m = np.ascontiguousarray(matrix, np.float64)
+    n = ctypes.c_int64(0)
+    rc = lib().rr_xpath_edges(_p(xy), _p(flags), len(flags), _p(m), flatness, None, 0,
+                              ctypes.byref(n))
+    if rc:
+        raise RrError(rc, "rr_xpath_edges")
+    out = np.zeros((max(n.value, 1), 4), np.float64)
+    rc = lib().rr_xpath_edges(_p(xy), _p(flags), len(flags), _p(m), flatness, _p(out), n.value,
+                              ctypes.byref(n))
+    if rc:
+        raise RrError(rc, "rr_xpath_edges")
+    return out[:n.value].copy()
+
+
+def cov_hash_term(pix, shape):
+    return lib().rr_cov_hash_term(pix, shape)
+
+
+class Page:
+    """One page of a synthetic workload (host arrays, numpy views copied out)."""
+
+    def __init__(self, width, height, edges, edge_off, attrs, host_rows, pts, pt_flags, pt_off):
+        self.width, self.height = width, height
+        self.edges, self.edge_off, self.attrs, self.host_rows = edges, edge_off, attrs, host_rows
+        self.pts, self.pt_flags, self.pt_off = pts, pt_flags, pt_off
+
+    @property
+    def n_paths(self):
+        return len(self.edge_off) - 1
+
+    @property
+    def n_edges(self):
+        return len(self.edges)
+
+    def path_points(self, i):
+        a, b = self.pt_off[i], self.pt_off[i + 1]
+        return self.pts[a:b], self.pt_flags[a:b]
+
+    def path_edges(self, i):
+        return self.edges[self.edge_off[i]:self.edge_off[i + 1]]
+
+    def subset(self, idx):
+        """A page holding only paths idx (in the given order)."""
+        idx = list(idx)
+        e = [self.path_edges(i) for i in idx]
+        off = np.zeros(len(idx) + 1, np.int64)
+        off[1:] = np.cumsum([len(x) for x in e]) if idx else []
+        pts = [self.path_points(i) for i in idx]
+        poff = np.zeros(len(idx) + 1, np.int64)
+        poff[1:] = np.cumsum([len(f) for _, f in pts]) if idx else []
+        return Page(self.width, self.height,
+                    np.concatenate(e) if idx else np.zeros((0, 4)), off,
+                    self.attrs[idx].copy(), self.host_rows[idx].copy(),
+                    np.concatenate([p for p, _ in pts]) if idx else np.zeros((0, 2)),
+                    np.concatenate([f for _, f in pts]) if idx else np.zeros(0, np.uint8), poff)
+
+
+def workload(config, seed, max_pages=0):
+    """Generate a seeded workload (configs 1, 2, 102, 3); returns a list of Page."""
+    L = lib()
+    h = ctypes.c_void_p()
+    rc = L.rr_workload_create(config, seed, max_pages, ctypes.byref(h))
+    if rc:
+        raise RrError(rc, f"rr_workload_create({config})")
+    try:
+        pages = []
+        for p in range(L.rr_workload_n_pages(h)):
+            info = np.zeros(5, np.int64)
+            L.rr_workload_page_info(h, p, _p(info))
+            w, hh, n_paths, n_edges, n_pts = (int(v) for v in info)
+            ptrs = [ctypes.c_void_p() for _ in range(7)]
+            L.rr_workload_page_data(h, p, *[ctypes.byref(q) for q in ptrs])
+
+            def arr(ptr, ctype, count, dtype):
+                if count == 0:
+                    return np.zeros(0, dtype)
+                buf = (ctype * count).from_address(ptr.value)
+                return np.frombuffer(buf, dtype=dtype).copy()
+
+            edges = arr(ptrs[0], ctypes.c_double, 4 * n_edges, np.float64).reshape(-1, 4)
+            edge_off = arr(ptrs[1], ctypes.c_int64, n_paths + 1, np.int64)
+            attrs = arr(ptrs[2], ctypes.c_uint8, 4 * n_paths, np.uint8).view(PATH_ATTR_DTYPE)
+            host_rows = arr(ptrs[3], ctypes.c_int32, 2 * n_paths, np.int32).reshape(-1, 2)
+            pts = arr(ptrs[4], ctypes.c_double, 2 * n_pts, np.float64).reshape(-1, 2)
+            pt_flags = arr(ptrs[5], ctypes.c_uint8, n_pts, np.uint8)
+            pt_off = arr(ptrs[6], ctypes.c_int64, n_paths + 1, np.int64)
+            pages.append(Page(w, hh, edges, edge_off, attrs, host_rows, pts, pt_flags, pt_off))
+        return pages
+    finally:
+        L.rr_workload_destroy(h)
+
+
+class Backend:
+    """Mirror of GpuBackend (crates/gpu/src/backend/mod.rs:216-503) over the C-ABI."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        self.h = ctypes.c_void_p()
+        rc = self.L.rr_hip_create(device, ctypes.byref(self.h))
+        if rc:
+            raise RrError(rc, "rr_hip_create: no usable HIP device")
+        self._bufs = []
+
+    def close(self):
+        if self.h:
+            for b in self._bufs:
+                self.L.rr_hip_free(self.h, b)
+            self._bufs = []
+            self.L.rr_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc:
+            raise RrError(rc, f"{what}: {self.L.rr_hip_last_error(self.h).decode()}")
+
+    def alloc(self, nbytes, zeroed=False):
+        b = ctypes.c_void_p()
+        self._check(self.L.rr_hip_alloc(self.h, nbytes, int(zeroed), ctypes.byref(b)), "alloc")
+        self._bufs.append(b.value)
+        return b.value
+
+    def free(self, buf):
+        self._check(self.L.rr_hip_free(self.h, buf), "free")
+        self._bufs.remove(buf)
+
+    def upload(self, dst, arr):
+        arr = np.ascontiguousarray(arr)
+        self._check(self.L.rr_hip_upload(self.h, dst, _p(arr), arr.nbytes), "upload")
+
+    def to_device(self, arr):
+        arr = np.ascontiguousarray(arr)
+        buf = self.alloc(max(arr.nbytes, 1))
+        if arr.nbytes:
+            self.upload(buf, arr)
+        return buf
+
+    def download(self, src, arr):
+        self._check(self.L.rr_hip_download(self.h, _p(arr), src, arr.nbytes), "download")
+        return arr
+
+    def begin_page(self):
+        self._check(self.L.rr_hip_begin_page(self.h), "begin_page")
+
+    def record_fill_paths(self, batch):
+        self._check(self.L.rr_hip_record_fill_paths(self.h, ctypes.byref(batch)),
+                    "record_fill_paths")
+
+    def submit_page(self):
+        f = ctypes.c_uint64(0)
+        self._check(self.L.rr_hip_submit_page(self.h, ctypes.byref(f)), "submit_page")
+        return f.value
+
+    def wait_page(self, fence):
+        self._check(self.L.rr_hip_wait_page(self.h, fence), "wait_page")
+
+    def last_page_stats(self):
+        ms = ctypes.c_double(0)
+        n = ctypes.c_int64(0)
+        self._check(self.L.rr_hip_last_page_stats(self.h, ctypes.byref(ms), ctypes.byref(n)),
+                    "last_page_stats")
+        return ms.value, n.value
+
+
+def stride_for(width):
+    return (width + 3) & ~3
+
+
+class DevicePage:
+    """A page's edges, offsets and attributes resident on the device."""
+
+    def __init__(self, be, page, attrs=None):
+        self.be, self.page = be, page
+        self.edges = be.to_device(page.edges.astype(np.float64))
+        self.edge_off = be.to_device(page.edge_off.astype(np.int64))
+        self.attrs_host = (page.attrs if attrs is None else attrs).copy()
+        self.attrs = be.to_device(self.attrs_host)
+        self.host_rows = np.ascontiguousarray(page.host_rows, np.int32)
+        self.stride = stride_for(page.width)
+        self.canvas = be.alloc(self.stride * page.height)
+        self.hashes = be.alloc(8 * max(page.n_paths, 1))
+
+    def batch(self, mode=MODE_DOCUMENTED, flavour=FLAVOUR_GRAY8, launch=LAUNCH_PLAIN,
+              hashes=False, clip=None):
+        pg = self.page
+        b = FillBatch()
+        b.edges, b.edge_off, b.attrs = self.edges, self.edge_off, self.attrs
+        b.host_rows = self.host_rows.ctypes.data
+        b.n_paths, b.n_edges = pg.n_paths, pg.n_edges
+        b.canvas = self.canvas
+        b.width, b.height, b.stride = pg.width, pg.height, self.stride
+        b.mode, b.flavour, b.launch = mode, flavour, launch
+        c = clip if clip is not None else (0.0, 0.0, pg.width - 0.001, pg.height - 0.001)
+        for i in range(4):
+            b.clip[i] = c[i]
+        b.cov_hash = self.hashes if hashes else None
+        return b
+
+    def reset_canvas(self, value=255):
+        self.be._check(self.be.L.rr_hip_memset(self.be.h, self.canvas, value,
+                                               self.stride * self.page.height), "memset")
+        self.be._check(self.be.L.rr_hip_memset(self.be.h, self.hashes, 0,
+                                               8 * max(self.page.n_paths, 1)), "memset")
+
+    def read_canvas(self):
+        pg = self.page
+        out = np.zeros((pg.height, self.stride), np.uint8)
+        self.be.download(self.canvas, out)
+        return out[:, :pg.width].copy()
+
+    def read_hashes(self):
+        out = np.zeros(max(self.page.n_paths, 1), np.uint64)
+        self.be.download(self.hashes, out)
+        return out[:self.page.n_paths].copy()
+
+    def render(self, mode=MODE_DOCUMENTED, flavour=FLAVOUR_GRAY8, launch=LAUNCH_PLAIN,
+               hashes=True, clip=None, background=255):
+        """Fill all paths in order on a fresh canvas; returns (canvas, hashes)."""
+        self.reset_canvas(background)
+        b = self.batch(mode, flavour, launch, hashes, clip)
+        self.be.begin_page()
+        self.be.record_fill_paths(b)
+        self.be.wait_page(self.be.submit_page())
+        return self.read_canvas(), (self.read_hashes() if hashes else None)
