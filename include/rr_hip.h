@@ -71,6 +71,11 @@ typedef struct {
  *   path's edges (floor of min/max y); only used to size plain launches.
  * cov_hash: optional device array of n_paths uint64; each receives the sum of
  *   rr_cov_hash_term() over the pixels the path composited with coverage > 0.
+ * cov_count: optional device array of 2*n_paths uint64; [2p] receives the
+ *   number of pixels path p composited with coverage > 0, [2p+1] the sum over
+ *   its drawn rows of the reference's draw_aa_line width x1-x0+1
+ *   (crates/raster/src/fill/mod.rs:344-401), i.e. the pixel-composites the
+ *   reference CPU path performs.
  */
 typedef struct {
     const double *edges;
@@ -84,6 +89,7 @@ typedef struct {
     int32_t mode, flavour, launch;
     double clip[4]; /* x_min, y_min, x_max, y_max; page clip is {0,0,W-0.001,H-0.001} */
     uint64_t *cov_hash;
+    uint64_t *cov_count;
 } rr_fill_batch;
 
 /* ---- context (GpuCtx::init, crates/gpu/src/lib.rs:239; CudaBackend, backend/cuda/mod.rs:27) ---- */
@@ -108,10 +114,13 @@ int32_t rr_hip_begin_page(rr_hip_ctx *ctx);
 int32_t rr_hip_record_fill_paths(rr_hip_ctx *ctx, const rr_fill_batch *batch);
 int32_t rr_hip_submit_page(rr_hip_ctx *ctx, uint64_t *fence);
 int32_t rr_hip_wait_page(rr_hip_ctx *ctx, uint64_t fence);
-/* Milliseconds of device time spent in the fill kernels of the last
- * submitted page (HIP events on the context stream), and the number of
- * fill-kernel launches it took. */
-int32_t rr_hip_last_page_stats(rr_hip_ctx *ctx, double *kernel_ms, int64_t *launches);
+/* Timing of the last waited page, from HIP events on the context stream:
+ * span_ms from the first to the last recorded kernel; with profiling enabled
+ * (rr_hip_set_profiling) fill_ms is the summed duration of the fill kernels
+ * alone, each bracketed by its own event pair; launches counts the fill-kernel
+ * launches (the edge-setup pass excluded). */
+int32_t rr_hip_set_profiling(rr_hip_ctx *ctx, int32_t enable);
+int32_t rr_hip_last_page_stats(rr_hip_ctx *ctx, double *span_ms, double *fill_ms, int64_t *launches);
 
 /* ---- host path ingestion (XPath::new, crates/raster/src/xpath.rs:162-260,
  *      with flatten_curve, path/flatten.rs:109-215) ----

@@ -78,6 +78,7 @@ struct FillArgs {
     const rr_path_attr *attrs;
     uint8_t *canvas;
     uint64_t *cov_hash;
+    uint64_t *cov_count;
     int32_t *spill;               // overflow entry pool
     unsigned long long *spill_top;  // bump allocator (in int32 units)
     int32_t *err;                 // set to 1 when the pool runs out
@@ -516,17 +517,33 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, int64_t p, int32_t Y
     bool unb = false;
     const int nint = sort_and_walk(n, X0, X1, C, mrule, aa_width, &unb);
     const bool any_unb = __ballot(unb) != 0;
+    unsigned long long w_draw = 0;
+    if (A.cov_count) {
+        // draw_aa_line range of each output row: x0 = xx_min/4, x1 = (xx_max-1)/4
+        // over its 4 AA rows (scanner/mod.rs:459-466), clamped by the clip.
+        int32_t smin = nint ? X0[lane] : INT32_MAX;
+        int32_t emax = nint ? X1[(nint - 1) * 64 + lane] : -1;
+        smin = min(smin, __shfl_xor(smin, 1));
+        smin = min(smin, __shfl_xor(smin, 2));
+        emax = max(emax, __shfl_xor(emax, 1));
+        emax = max(emax, __shfl_xor(emax, 2));
+        if (j == 0 && y >= ylo && y <= yhi && smin < emax) {
+            const int32_t x0 = max(smin / 4, A.px_lo), x1 = min((emax - 1) / 4, A.px_hi);
+            if (x0 <= x1) w_draw = static_cast<unsigned long long>(x1 - x0 + 1);
+        }
+    }
 
     // pixel range that can hold coverage, intersected with the clip
     int32_t px_lo = max(A.px_lo, floor_div4(h.x_min));
     int32_t px_hi = min(A.px_hi, any_unb ? A.width - 1 : floor_div4(h.x_max));
-    if (px_lo > px_hi) return;
+    const bool any_px = px_lo <= px_hi;
     const int32_t xs = px_lo & ~3;
     const uint32_t a_in = at.a_input, src = at.color;
     uint64_t hsum = 0;
+    unsigned long long w_cov = 0;
     int ip = 0;  // first interval of this lane not yet fully left of the chunk
     uint32_t *bm_row = L.bm + lane * kBmPitch;
-    for (int32_t cx = xs; cx <= px_hi; cx += kChunkPx) {
+    for (int32_t cx = xs; any_px && cx <= px_hi; cx += kChunkPx) {
         const int32_t A0 = 4 * cx, A1 = A0 + 4 * kChunkPx;
         for (int w = 0; w < 32; ++w) bm_row[w] = 0u;
         while (ip < nint && X1[ip * 64 + lane] <= A0) ++ip;
@@ -580,6 +597,7 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, int64_t p, int32_t Y
                     res = div255((255u - a) * dv + a * src);
                 }
                 out = (out & ~(0xffu << (8 * px))) | (res << (8 * px));
+                ++w_cov;
                 if (A.cov_hash) hsum += cov_term(static_cast<long long>(yy) * A.width + x, shape);
             }
             if (out != old) *dst = out;
@@ -589,6 +607,16 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, int64_t p, int32_t Y
     if (A.cov_hash) {
         for (int o = 32; o > 0; o >>= 1) hsum += __shfl_xor(hsum, o);
         if (lane == 0 && hsum) atomicAdd(reinterpret_cast<unsigned long long *>(A.cov_hash + p), hsum);
+    }
+    if (A.cov_count) {
+        for (int o = 32; o > 0; o >>= 1) {
+            w_cov += __shfl_xor(w_cov, o);
+            w_draw += __shfl_xor(w_draw, o);
+        }
+        if (lane == 0) {
+            if (w_cov) atomicAdd(reinterpret_cast<unsigned long long *>(A.cov_count + 2 * p), w_cov);
+            if (w_draw) atomicAdd(reinterpret_cast<unsigned long long *>(A.cov_count + 2 * p + 1), w_draw);
+        }
     }
 }
 
@@ -631,6 +659,12 @@ struct rr_hip_ctx {
     int64_t last_launches = 0;
     Fence fences[16];
     uint64_t next_fence = 1;
+    // per-launch profiling
+    bool profiling = false;
+    std::vector<hipEvent_t> prof_ev;  // pairs
+    size_t prof_used = 0;
+    double last_fill_ms = 0.0;
+    int64_t fill_launches = 0;
 };
 
 namespace {
@@ -664,6 +698,21 @@ int32_t ensure(rr_hip_ctx *c, T **buf, size_t *cap, size_t need) {
     RR_HIP_TRY(c, hipMalloc(reinterpret_cast<void **>(buf), n * sizeof(T)));
     *cap = n;
     return RR_OK;
+}
+
+// Next free start/stop event pair of the per-launch profiling pool.
+hipEvent_t *prof_pair(rr_hip_ctx *c) {
+    if (c->prof_used + 2 > c->prof_ev.size()) {
+        const size_t grow = c->prof_ev.size() ? c->prof_ev.size() : 256;
+        for (size_t k = 0; k < grow; ++k) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            c->prof_ev.push_back(e);
+        }
+    }
+    hipEvent_t *p = &c->prof_ev[c->prof_used];
+    c->prof_used += 2;
+    return p;
 }
 
 ClipRect make_clip(const double cl[4]) {
@@ -717,25 +766,29 @@ int32_t rr_hip_create(int32_t device, rr_hip_ctx **out) {
         return RR_E_DEVICE;
     }
     c->spill_cap = pool_ints;
-    hipMemset(c->err_flag, 0, sizeof(int32_t));
+    if (hipMemset(c->err_flag, 0, sizeof(int32_t)) != hipSuccess) {
+        rr_hip_destroy(c);
+        return RR_E_DEVICE;
+    }
     *out = c;
     return RR_OK;
 }
 
 int32_t rr_hip_destroy(rr_hip_ctx *c) {
     if (!c) return RR_E_INVARIANT;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
-    hipFree(c->recs);
-    hipFree(c->hdrs);
-    hipFree(c->spill);
-    hipFree(c->spill_top);
-    hipFree(c->err_flag);
-    if (c->t0) hipEventDestroy(c->t0);
-    if (c->t1) hipEventDestroy(c->t1);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->recs);
+    (void)hipFree(c->hdrs);
+    (void)hipFree(c->spill);
+    (void)hipFree(c->spill_top);
+    (void)hipFree(c->err_flag);
+    if (c->t0) (void)hipEventDestroy(c->t0);
+    if (c->t1) (void)hipEventDestroy(c->t1);
     for (auto &f : c->fences)
-        if (f.ev) hipEventDestroy(f.ev);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (f.ev) (void)hipEventDestroy(f.ev);
+    for (auto e : c->prof_ev) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return RR_OK;
 }
@@ -783,6 +836,8 @@ int32_t rr_hip_begin_page(rr_hip_ctx *c) {
     c->page_open = true;
     c->page_timed = false;
     c->launches = 0;
+    c->fill_launches = 0;
+    c->prof_used = 0;
     RR_HIP_TRY(c, hipMemsetAsync(c->err_flag, 0, sizeof(int32_t), c->stream));
     return RR_OK;
 }
@@ -819,6 +874,7 @@ int32_t rr_hip_record_fill_paths(rr_hip_ctx *c, const rr_fill_batch *b) {
     A.attrs = b->attrs;
     A.canvas = b->canvas;
     A.cov_hash = b->cov_hash;
+    A.cov_count = b->cov_count;
     A.spill = c->spill;
     A.spill_top = c->spill_top;
     A.err = c->err_flag;
@@ -846,8 +902,13 @@ int32_t rr_hip_record_fill_paths(rr_hip_ctx *c, const rr_fill_batch *b) {
         r1 = std::min(r1, b->height - 1);
         if (r0 > r1) continue;
         const unsigned bands = static_cast<unsigned>((r1 - r0) / kBandRows + 1);
+        hipEvent_t *ev = c->profiling ? prof_pair(c) : nullptr;
+        if (c->profiling && !ev) return fail(c, RR_E_DEVICE, "profiling event allocation failed");
+        if (ev) RR_HIP_TRY(c, hipEventRecord(ev[0], c->stream));
         hipLaunchKernelGGL(k_fill_plain, dim3(bands), dim3(64), 0, c->stream, A, p, r0);
+        if (ev) RR_HIP_TRY(c, hipEventRecord(ev[1], c->stream));
         ++c->launches;
+        ++c->fill_launches;
     }
     RR_HIP_TRY(c, hipGetLastError());
     return RR_OK;
@@ -862,7 +923,7 @@ int32_t rr_hip_submit_page(rr_hip_ctx *c, uint64_t *fence) {
     f.id = c->next_fence;
     *fence = c->next_fence++;
     c->page_open = false;
-    c->last_launches = c->launches;
+    c->last_launches = c->fill_launches;
     return RR_OK;
 }
 
@@ -880,13 +941,27 @@ int32_t rr_hip_wait_page(rr_hip_ctx *c, uint64_t fence) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, c->t0, c->t1) == hipSuccess) c->last_ms = ms;
     }
+    double fill = 0.0;
+    for (size_t k = 0; k + 1 < c->prof_used; k += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->prof_ev[k], c->prof_ev[k + 1]) == hipSuccess) fill += ms;
+    }
+    c->last_fill_ms = fill;
     if (flag) return fail(c, RR_E_CAPACITY, "overflow-entry pool exhausted: page output is incomplete");
     return RR_OK;
 }
 
-int32_t rr_hip_last_page_stats(rr_hip_ctx *c, double *kernel_ms, int64_t *launches) {
+int32_t rr_hip_set_profiling(rr_hip_ctx *c, int32_t enable) {
     if (!c) return RR_E_INVARIANT;
-    if (kernel_ms) *kernel_ms = c->last_ms;
+    if (c->page_open) return fail(c, RR_E_INVARIANT, "set_profiling inside an open page");
+    c->profiling = enable != 0;
+    return RR_OK;
+}
+
+int32_t rr_hip_last_page_stats(rr_hip_ctx *c, double *span_ms, double *fill_ms, int64_t *launches) {
+    if (!c) return RR_E_INVARIANT;
+    if (span_ms) *span_ms = c->last_ms;
+    if (fill_ms) *fill_ms = c->last_fill_ms;
     if (launches) *launches = c->last_launches;
     return RR_OK;
 }

@@ -36,7 +36,8 @@ class FillBatch(ctypes.Structure):
                 ("canvas", ctypes.c_void_p), ("width", ctypes.c_int32),
                 ("height", ctypes.c_int32), ("stride", ctypes.c_int32), ("mode", ctypes.c_int32),
                 ("flavour", ctypes.c_int32), ("launch", ctypes.c_int32),
-                ("clip", ctypes.c_double * 4), ("cov_hash", ctypes.c_void_p)]
+                ("clip", ctypes.c_double * 4), ("cov_hash", ctypes.c_void_p),
+                ("cov_count", ctypes.c_void_p)]
 
 
 _lib = None
@@ -63,7 +64,9 @@ def lib():
             "rr_hip_record_fill_paths": (i32, [P, ctypes.POINTER(FillBatch)]),
             "rr_hip_submit_page": (i32, [P, ctypes.POINTER(u64)]),
             "rr_hip_wait_page": (i32, [P, u64]),
+            "rr_hip_set_profiling": (i32, [P, i32]),
             "rr_hip_last_page_stats": (i32, [P, ctypes.POINTER(ctypes.c_double),
+                                             ctypes.POINTER(ctypes.c_double),
                                              ctypes.POINTER(i64)]),
             "rr_xpath_edges": (i32, [P, P, i64, P, ctypes.c_double, P, i64,
                                      ctypes.POINTER(i64)]),
@@ -257,12 +260,17 @@ class Backend:
     def wait_page(self, fence):
         self._check(self.L.rr_hip_wait_page(self.h, fence), "wait_page")
 
+    def set_profiling(self, on):
+        self._check(self.L.rr_hip_set_profiling(self.h, int(on)), "set_profiling")
+
     def last_page_stats(self):
-        ms = ctypes.c_double(0)
+        """(span_ms, fill_kernel_ms, fill_launches) of the last waited page."""
+        span = ctypes.c_double(0)
+        fill = ctypes.c_double(0)
         n = ctypes.c_int64(0)
-        self._check(self.L.rr_hip_last_page_stats(self.h, ctypes.byref(ms), ctypes.byref(n)),
-                    "last_page_stats")
-        return ms.value, n.value
+        self._check(self.L.rr_hip_last_page_stats(self.h, ctypes.byref(span), ctypes.byref(fill),
+                                                  ctypes.byref(n)), "last_page_stats")
+        return span.value, fill.value, n.value
 
 
 def stride_for(width):
@@ -282,9 +290,14 @@ class DevicePage:
         self.stride = stride_for(page.width)
         self.canvas = be.alloc(self.stride * page.height)
         self.hashes = be.alloc(8 * max(page.n_paths, 1))
+        self.counts = be.alloc(16 * max(page.n_paths, 1))
+
+    def free(self):
+        for b in (self.edges, self.edge_off, self.attrs, self.canvas, self.hashes, self.counts):
+            self.be.free(b)
 
     def batch(self, mode=MODE_DOCUMENTED, flavour=FLAVOUR_GRAY8, launch=LAUNCH_PLAIN,
-              hashes=False, clip=None):
+              hashes=False, clip=None, counts=False):
         pg = self.page
         b = FillBatch()
         b.edges, b.edge_off, b.attrs = self.edges, self.edge_off, self.attrs
@@ -297,6 +310,7 @@ class DevicePage:
         for i in range(4):
             b.clip[i] = c[i]
         b.cov_hash = self.hashes if hashes else None
+        b.cov_count = self.counts if counts else None
         return b
 
     def reset_canvas(self, value=255):
@@ -304,6 +318,8 @@ class DevicePage:
                                                self.stride * self.page.height), "memset")
         self.be._check(self.be.L.rr_hip_memset(self.be.h, self.hashes, 0,
                                                8 * max(self.page.n_paths, 1)), "memset")
+        self.be._check(self.be.L.rr_hip_memset(self.be.h, self.counts, 0,
+                                               16 * max(self.page.n_paths, 1)), "memset")
 
     def read_canvas(self):
         pg = self.page
@@ -316,11 +332,16 @@ class DevicePage:
         self.be.download(self.hashes, out)
         return out[:self.page.n_paths].copy()
 
+    def read_counts(self):
+        out = np.zeros((max(self.page.n_paths, 1), 2), np.uint64)
+        self.be.download(self.counts, out)
+        return out[:self.page.n_paths].copy()
+
     def render(self, mode=MODE_DOCUMENTED, flavour=FLAVOUR_GRAY8, launch=LAUNCH_PLAIN,
-               hashes=True, clip=None, background=255):
+               hashes=True, clip=None, background=255, counts=False):
         """Fill all paths in order on a fresh canvas; returns (canvas, hashes)."""
         self.reset_canvas(background)
-        b = self.batch(mode, flavour, launch, hashes, clip)
+        b = self.batch(mode, flavour, launch, hashes, clip, counts)
         self.be.begin_page()
         self.be.record_fill_paths(b)
         self.be.wait_page(self.be.submit_page())

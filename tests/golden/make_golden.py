@@ -1,0 +1,86 @@
+"""Writes the golden fixtures under tests/golden/ from the CPU oracle.
+
+page_hashes.json: for each named page workload and AA row mode, the sha256 of
+  the oracle's final canvas bytes (H*W, row-major) followed by the per-path
+  coverage hashes (uint64 little-endian) -- the same fingerprint bench.py
+  computes from the GPU output.
+tiny_masks.npz: full coverage masks (gamma-mapped shape per pixel) and final
+  canvases of a few hand-built paths, for exact regression of the oracle.
+
+Run: python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "pdf-raster_amd"))
+
+import numpy as np  # noqa: E402
+
+import rr_hip as R  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+PAGES = {"glyph-page": (2, 0xC0F10002, 0), "polygon-page": (1, 0xC0F10001, 0)}
+MODES = {"documented": O.MODE_DOCUMENTED, "literal": O.MODE_LITERAL}
+
+
+def fingerprint(pages, mode):
+    d = hashlib.sha256()
+    for pg in pages:
+        c = np.full((pg.height, pg.width), 255, np.uint8)
+        a = pg.attrs
+        _, hs = O.fill_page(c, pg.edges, pg.edge_off, a["eo"], a["color"], a["a_input"], mode)
+        d.update(c.tobytes())
+        d.update(hs.astype("<u8").tobytes())
+    return d.hexdigest()
+
+
+def tiny_cases():
+    out = {}
+    def tri(p):
+        p.move_to(1.3, 0.7)
+        p.line_to(14.6, 5.2)
+        p.line_to(4.1, 11.9)
+        p.close(False)
+    def star(p):
+        pts = [(8, 0.5), (10, 6), (15.5, 6), (11, 9.5), (13, 15), (8, 11.5), (3, 15), (5, 9.5), (0.5, 6), (6, 6)]
+        p.move_to(*pts[0])
+        for q in pts[1:]:
+            p.line_to(*q)
+        p.close(False)
+    def blob(p):
+        p.move_to(8, 1)
+        p.curve_to(15, 1, 15, 15, 8, 15)
+        p.curve_to(1, 15, 1, 1, 8, 1)
+        p.close(False)
+    for name, fn in (("tri", tri), ("star", star), ("blob", blob)):
+        for eo in (0, 1):
+            p = O.Path()
+            fn(p)
+            e = O.xpath_edges(*p.arrays(), flatness=0.1)
+            canvas = np.full((16, 16), 255, np.uint8)
+            _, _, shape = O.fill(canvas, e, eo, 40, want_shape=True)
+            out[f"{name}_eo{eo}_shape"] = shape
+            out[f"{name}_eo{eo}_canvas"] = canvas
+            out[f"{name}_eo{eo}_edges"] = e
+    return out
+
+
+def main():
+    res = {}
+    for name, (cfg, seed, mp) in PAGES.items():
+        pages = R.workload(cfg, seed, mp)
+        for mname, mode in MODES.items():
+            res[f"{name}:{mname}"] = fingerprint(pages, mode)
+            print(name, mname, res[f"{name}:{mname}"][:16], flush=True)
+    with open(os.path.join(HERE, "page_hashes.json"), "w") as f:
+        json.dump(res, f, indent=1, sort_keys=True)
+    np.savez_compressed(os.path.join(HERE, "tiny_masks.npz"), **tiny_cases())
+
+
+if __name__ == "__main__":
+    main()

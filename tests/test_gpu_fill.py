@@ -28,8 +28,7 @@ def gpu_page(be, page, mode=R.MODE_DOCUMENTED, flavour=R.FLAVOUR_GRAY8, launch=R
     try:
         return dp.render(mode, flavour, launch, hashes=True, background=background)
     finally:
-        for b in (dp.edges, dp.edge_off, dp.attrs, dp.canvas, dp.hashes):
-            be.free(b)
+        dp.free()
 
 
 def assert_same(gpu, ora, what):
@@ -64,8 +63,9 @@ def hand_page(paths, width=64, height=48):
         poff.append(poff[-1] + len(fl))
     return R.Page(width, height, np.concatenate(edges) if edges else np.zeros((0, 4)),
                   np.array(off, np.int64), np.array(attrs, dtype=R.PATH_ATTR_DTYPE),
-                  np.array(rows, np.int32).reshape(-1, 2), np.concatenate(pts),
-                  np.concatenate(fls), np.array(poff, np.int64))
+                  np.array(rows, np.int32).reshape(-1, 2),
+                  np.concatenate(pts) if pts else np.zeros((0, 2)),
+                  np.concatenate(fls) if fls else np.zeros(0, np.uint8), np.array(poff, np.int64))
 
 
 def rect(x0, y0, x1, y1):

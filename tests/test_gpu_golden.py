@@ -1,0 +1,29 @@
+"""Full-page GPU renders against the oracle fingerprints in tests/golden/."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import rr_hip as R
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "page_hashes.json")))
+
+
+@pytest.mark.parametrize("name,cfg,seed", [("glyph-page", 2, 0xC0F10002),
+                                           ("polygon-page", 1, 0xC0F10001)])
+@pytest.mark.parametrize("mode", ["documented", "literal"])
+def test_full_page_matches_oracle_fingerprint(backend, name, cfg, seed, mode):
+    pages = R.workload(cfg, seed)
+    d = hashlib.sha256()
+    for pg in pages:
+        dp = R.DevicePage(backend, pg)
+        try:
+            c, h = dp.render(R.MODE_LITERAL if mode == "literal" else R.MODE_DOCUMENTED)
+        finally:
+            dp.free()
+        d.update(c.tobytes())
+        d.update(h.astype("<u8").tobytes())
+    assert d.hexdigest() == GOLD[f"{name}:{mode}"]
