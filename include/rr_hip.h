@@ -92,6 +92,34 @@ typedef struct {
     uint64_t *cov_count;
 } rr_fill_batch;
 
+/* One page of a multi-page batch: its canvas and the range of the batch's
+ * paths that belong to it (pages tile [0, n_paths) in order). */
+typedef struct {
+    uint8_t *canvas; /* device, 1 byte per pixel, row pitch `stride` */
+    int32_t width, height, stride, reserved;
+    int64_t path_begin, path_end;
+    double clip[4]; /* page clip is {0,0,W-0.001,H-0.001} (crates/raster/src/state.rs:108-121) */
+} rr_page_desc;
+
+/* Many pages of paths in one recording.  edges/edge_off/attrs/cov_* are as in
+ * rr_fill_batch with path indices global to the batch; pages is a host array.
+ * With launch = RR_LAUNCH_BATCHED every page goes through one edge-setup pass
+ * and one persistent fill+composite launch; with RR_LAUNCH_PLAIN each path is
+ * its own launch (host_rows required). */
+typedef struct {
+    const double *edges;
+    const int64_t *edge_off;
+    const rr_path_attr *attrs;
+    const int32_t *host_rows;
+    int64_t n_paths;
+    int64_t n_edges;
+    const rr_page_desc *pages;
+    int32_t n_pages;
+    int32_t mode, flavour, launch;
+    uint64_t *cov_hash;
+    uint64_t *cov_count;
+} rr_fill_pages;
+
 /* ---- context (GpuCtx::init, crates/gpu/src/lib.rs:239; CudaBackend, backend/cuda/mod.rs:27) ---- */
 typedef struct rr_hip_ctx rr_hip_ctx;
 int32_t rr_hip_create(int32_t device, rr_hip_ctx **out);
@@ -112,6 +140,7 @@ int32_t rr_hip_memset(rr_hip_ctx *ctx, void *dst, int32_t value, size_t n);
  *      backend/mod.rs:306-405) ---- */
 int32_t rr_hip_begin_page(rr_hip_ctx *ctx);
 int32_t rr_hip_record_fill_paths(rr_hip_ctx *ctx, const rr_fill_batch *batch);
+int32_t rr_hip_record_fill_pages(rr_hip_ctx *ctx, const rr_fill_pages *batch);
 int32_t rr_hip_submit_page(rr_hip_ctx *ctx, uint64_t *fence);
 int32_t rr_hip_wait_page(rr_hip_ctx *ctx, uint64_t fence);
 /* Timing of the last waited page, from HIP events on the context stream:

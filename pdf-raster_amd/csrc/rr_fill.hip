@@ -23,16 +23,22 @@
 // even-odd) are dropped when they are retired: they never start or end a span.
 // The walk turns each row into disjoint sorted intervals; the wave then sweeps
 // the band 256 pixels at a time through an LDS bitmap (one row per lane, no
-// atomics) and composites 4 pixels per lane per dword, touching HBM only for
-// dwords that hold covered pixels.
+// atomics) and composites 4 pixels per lane per dword, touching memory only
+// for dwords that hold covered pixels.
+//
+// Launch shapes: PLAIN launches one grid per path (one wave per band), in
+// path order on the context stream.  BATCHED launches one persistent grid for
+// every page of the batch; each wave pulls (page, band) work items from an
+// atomic queue and composites, in path order, every path of that page that
+// overlaps its band.  Bands are disjoint, so the in-order compositing of
+// overlapping paths holds by construction without any inter-wave ordering.
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
-#include <cstdio>
 #include <cstring>
-#include <mutex>
 #include <string>
 #include <vector>
 
@@ -63,7 +69,7 @@ enum { HDR_EMPTY = 1, HDR_OUTSIDE = 2 };
 // Scanner bounding box of one path in AA space (scanner/mod.rs:138-169).
 struct PathHdr {
     int32_t y_min, y_max, x_min, x_max;
-    int32_t flags, pad0, pad1, pad2;
+    int32_t flags, page, pad1, pad2;
 };
 
 struct ClipRect {
@@ -71,21 +77,32 @@ struct ClipRect {
     int32_t x_min_i, y_min_i, x_max_i, y_max_i;
 };
 
+// One page of a batch, as the kernels see it.
+struct DevPage {
+    uint8_t *canvas;
+    int64_t path_begin, path_end;
+    ClipRect clip;
+    int32_t width, height, stride;
+    int32_t px_lo, px_hi;  // clip ∩ canvas, pixels
+    int32_t pad;
+};
+
 struct FillArgs {
     const EdgeRec *recs;
     const PathHdr *hdrs;
+    const int2 *rows;  // per path: drawn output rows [lo, hi] (lo > hi: nothing)
     const int64_t *edge_off;
     const rr_path_attr *attrs;
-    uint8_t *canvas;
+    const DevPage *pages;
     uint64_t *cov_hash;
     uint64_t *cov_count;
-    int32_t *spill;               // overflow entry pool
+    int32_t *slots;                 // per-wave overflow entry slots
+    long long slot_ints;            // ints per slot
+    int32_t *spill;                 // shared overflow pool for rows beyond a slot
     unsigned long long *spill_top;  // bump allocator (in int32 units)
-    int32_t *err;                 // set to 1 when the pool runs out
+    int32_t *err;                   // set to 1 when the pool runs out
     long long spill_cap;
-    int32_t width, height, stride;
     int32_t mode, flavour;
-    int32_t px_lo, px_hi;  // clip ∩ canvas, pixels
 };
 
 // AA_GAMMA (crates/raster/src/fill/mod.rs:46-48)
@@ -134,6 +151,21 @@ __device__ __forceinline__ uint64_t cov_term(long long pix, uint32_t shape) {
 
 __device__ __forceinline__ int32_t floor_div4(int32_t v) { return v >= 0 ? v / 4 : -((-v + 3) / 4); }
 
+// Output rows a path draws (fill/mod.rs:176-203): documented mode draws device
+// rows y_min/4..y_max/4; the literal loop draws y = aa_y/4 for aa_y == 4y+3 in
+// [y_min, y_max].  Both are limited to 0 <= y < H.
+__device__ __forceinline__ int2 drawn_rows(const PathHdr &h, int mode, int32_t height) {
+    int32_t lo, hi;
+    if (mode == RR_MODE_LITERAL) {
+        lo = floor_div4(h.y_min);  // ceil((y_min - 3) / 4)
+        hi = floor_div4(h.y_max - 3);
+    } else {
+        lo = h.y_min / 4;
+        hi = h.y_max / 4;
+    }
+    return make_int2(max(lo, 0), min(hi, height - 1));
+}
+
 // ---------------------------------------------------------------------------
 // kernel 1: per-path edge setup + scanner bounding box
 // ---------------------------------------------------------------------------
@@ -144,9 +176,20 @@ __device__ __forceinline__ int32_t floor_div4(int32_t v) { return v >= 0 ? v / 4
 // bbox over the non-NaN segments that overlap the AA clip rows.
 __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ edges,
                                                     const int64_t *__restrict__ edge_off,
-                                                    EdgeRec *__restrict__ recs,
-                                                    PathHdr *__restrict__ hdrs, ClipRect clip) {
+                                                    const DevPage *__restrict__ pages, int32_t n_pages,
+                                                    EdgeRec *__restrict__ recs, PathHdr *__restrict__ hdrs,
+                                                    int2 *__restrict__ rows, int32_t mode) {
     const int64_t p = blockIdx.x;
+    int32_t lo = 0, hi = n_pages - 1;  // page of path p (pages sorted by path_begin)
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) / 2;
+        if (pages[mid].path_begin <= p)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const DevPage &pg = pages[lo];
+    const ClipRect clip = pg.clip;
     const int64_t e0 = edge_off[p], e1 = edge_off[p + 1];
     const int32_t cy0 = clip.y_min_i * 4, cy1 = (clip.y_max_i + 1) * 4 - 1;
     int32_t xmn = INT32_MAX, xmx = INT32_MIN, ymn = INT32_MAX, ymx = INT32_MIN;
@@ -229,13 +272,12 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
         }
         recs[i] = r;
     }
-    __shared__ int32_t red[4][256];
-    __shared__ int32_t red_any[256];
+    __shared__ int32_t red[5][256];
     red[0][threadIdx.x] = xmn;
     red[1][threadIdx.x] = xmx;
     red[2][threadIdx.x] = ymn;
     red[3][threadIdx.x] = ymx;
-    red_any[threadIdx.x] = any;
+    red[4][threadIdx.x] = any;
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
@@ -243,13 +285,14 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
             red[1][threadIdx.x] = max(red[1][threadIdx.x], red[1][threadIdx.x + s]);
             red[2][threadIdx.x] = min(red[2][threadIdx.x], red[2][threadIdx.x + s]);
             red[3][threadIdx.x] = max(red[3][threadIdx.x], red[3][threadIdx.x + s]);
-            red_any[threadIdx.x] |= red_any[threadIdx.x + s];
+            red[4][threadIdx.x] |= red[4][threadIdx.x + s];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        PathHdr h{0, -1, 1, 0, HDR_EMPTY, 0, 0, 0};
-        if (red_any[0]) {
+        PathHdr h{0, -1, 1, 0, HDR_EMPTY, lo, 0, 0};
+        int2 drawn = make_int2(1, 0);
+        if (red[4][0]) {
             const int32_t y_min = max(red[2][0], cy0), y_max = min(red[3][0], cy1);
             if (y_min <= y_max) {
                 h.y_min = y_min;
@@ -262,18 +305,23 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
                 if (static_cast<double>(rt) + 1.0 <= clip.x_min || static_cast<double>(l) >= clip.x_max ||
                     static_cast<double>(b) + 1.0 <= clip.y_min || static_cast<double>(t) >= clip.y_max)
                     h.flags = HDR_OUTSIDE;
+                else
+                    drawn = drawn_rows(h, mode, pg.height);
             }
         }
         hdrs[p] = h;
+        rows[p] = drawn;
     }
 }
 
 // ---------------------------------------------------------------------------
 // kernel 2: scanline fill + composite of one band of 16 output rows
 // ---------------------------------------------------------------------------
-constexpr int kBandRows = 16;   // output rows per wave
-constexpr int kChunkPx = 256;   // pixels per composite sweep step
-constexpr int kBmPitch = 33;    // dwords per bitmap row (32 + 1 pad against bank conflicts)
+constexpr int kBandRows = 16;  // output rows per wave
+constexpr int kChunkPx = 256;  // pixels per composite sweep step
+constexpr int kBmPitch = 33;   // dwords per bitmap row (32 + 1 pad against bank conflicts)
+constexpr int kLdsK = 32;      // LDS entry slots per AA row
+constexpr int kSlotK = 512;    // global overflow entry slots per AA row, per wave
 
 template <int K>
 struct BandLds {
@@ -332,8 +380,7 @@ __device__ __forceinline__ Entry row_entry(const EdgeRec &q, int32_t r) {
 // first cap are stored and the caller reruns with more room).
 __device__ __forceinline__ int build_entries(const FillArgs &A, EdgeRec *stage, int64_t e0, int64_t e1,
                                              int32_t band_lo, int32_t band_hi, bool valid, int32_t r,
-                                             int32_t mrule, int32_t *X0, int32_t *X1, int32_t *C,
-                                             int cap) {
+                                             int32_t mrule, int32_t *X0, int32_t *X1, int32_t *C, int cap) {
     const int lane = threadIdx.x;
     int n = 0;
     bool has_last = false;
@@ -459,27 +506,20 @@ __device__ __forceinline__ int sort_and_walk(int n, int32_t *X0, int32_t *X1, in
     return nout;
 }
 
-// Rasterises path p over output rows [Y0, Y0+15] and composites into the
-// canvas.  Called by one full wave64 (blockDim.x == 64).
+// Rasterises path p over output rows [Y0, Y0+15] of page `pg` and composites
+// into its canvas.  Called by one full wave64 (blockDim.x == 64); `slot` is
+// this wave's private overflow slot.
 template <int K>
-__device__ void fill_band(const FillArgs &A, BandLds<K> &L, int64_t p, int32_t Y0) {
+__device__ void fill_band(const FillArgs &A, BandLds<K> &L, const DevPage &pg, int64_t p, int32_t Y0,
+                          int64_t slot) {
     const int lane = threadIdx.x;
     const PathHdr h = A.hdrs[p];
     if (h.flags & (HDR_EMPTY | HDR_OUTSIDE)) return;
+    const int2 drawn = A.rows[p];
+    const int32_t ylo = drawn.x, yhi = drawn.y;
+    if (Y0 > yhi || Y0 + kBandRows - 1 < ylo) return;
     const rr_path_attr at = A.attrs[p];
     const bool literal = A.mode == RR_MODE_LITERAL;
-    // drawn output rows (fill/mod.rs:176-203 literal; Splash per device row documented)
-    int32_t ylo, yhi;
-    if (literal) {
-        ylo = floor_div4(h.y_min - 3 + 3);  // ceil((y_min-3)/4)
-        yhi = floor_div4(h.y_max - 3);
-    } else {
-        ylo = h.y_min / 4;
-        yhi = h.y_max / 4;
-    }
-    ylo = max(ylo, 0);
-    yhi = min(yhi, A.height - 1);
-    if (Y0 > yhi || Y0 + kBandRows - 1 < ylo) return;
 
     const int32_t d = lane >> 2, j = lane & 3;
     const int32_t y = Y0 + d;
@@ -488,27 +528,30 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, int64_t p, int32_t Y
     const int32_t band_hi = literal ? 16 * (Y0 + kBandRows - 1) + 15 : 4 * Y0 + 4 * kBandRows - 1;
     const bool valid = y >= ylo && y <= yhi && r >= h.y_min && r <= h.y_max;
     const int32_t mrule = at.eo ? 1 : -1;
-    const int32_t aa_width = A.width * 4;
+    const int32_t aa_width = pg.width * 4;
     const int64_t e0 = A.edge_off[p], e1 = A.edge_off[p + 1];
-
-    if (lane < 17) L.gamma[lane] = kGamma[lane];
 
     int32_t *X0 = L.ex0, *X1 = L.ex1, *C = L.ec;
     int n = build_entries(A, L.rec, e0, e1, band_lo, band_hi, valid, r, mrule, X0, X1, C, K);
     int need = n;
     for (int o = 32; o > 0; o >>= 1) need = max(need, __shfl_xor(need, o));
-    int32_t *G = nullptr;
     if (need > K) {
-        // Rare: a row with more state-changing entries than the LDS holds.
-        // Redo the band with the entry lists in a slice of the global pool.
-        unsigned long long off = 0;
-        if (lane == 0) off = atomicAdd(A.spill_top, static_cast<unsigned long long>(need) * 64ull * 3ull);
-        off = __shfl(off, 0);
-        if (static_cast<long long>(off + static_cast<unsigned long long>(need) * 192ull) > A.spill_cap) {
-            if (lane == 0) atomicExch(A.err, 1);
-            return;
+        // A row with more state-changing entries than the LDS holds: redo the
+        // band with the entry lists in this wave's global slot, or for very
+        // long rows in a slice of the shared pool.
+        int32_t *G;
+        if (static_cast<long long>(need) * 192 <= A.slot_ints) {
+            G = A.slots + slot * A.slot_ints;
+        } else {
+            unsigned long long off = 0;
+            if (lane == 0) off = atomicAdd(A.spill_top, static_cast<unsigned long long>(need) * 192ull);
+            off = __shfl(off, 0);
+            if (static_cast<long long>(off) + static_cast<long long>(need) * 192 > A.spill_cap) {
+                if (lane == 0) atomicExch(A.err, 1);
+                return;
+            }
+            G = A.spill + off;
         }
-        G = A.spill + off;
         X0 = G;
         X1 = G + static_cast<long long>(need) * 64;
         C = G + static_cast<long long>(need) * 128;
@@ -528,14 +571,14 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, int64_t p, int32_t Y
         emax = max(emax, __shfl_xor(emax, 1));
         emax = max(emax, __shfl_xor(emax, 2));
         if (j == 0 && y >= ylo && y <= yhi && smin < emax) {
-            const int32_t x0 = max(smin / 4, A.px_lo), x1 = min((emax - 1) / 4, A.px_hi);
+            const int32_t x0 = max(smin / 4, pg.px_lo), x1 = min((emax - 1) / 4, pg.px_hi);
             if (x0 <= x1) w_draw = static_cast<unsigned long long>(x1 - x0 + 1);
         }
     }
 
     // pixel range that can hold coverage, intersected with the clip
-    int32_t px_lo = max(A.px_lo, floor_div4(h.x_min));
-    int32_t px_hi = min(A.px_hi, any_unb ? A.width - 1 : floor_div4(h.x_max));
+    const int32_t px_lo = max(pg.px_lo, floor_div4(h.x_min));
+    const int32_t px_hi = min(pg.px_hi, any_unb ? pg.width - 1 : floor_div4(h.x_max));
     const bool any_px = px_lo <= px_hi;
     const int32_t xs = px_lo & ~3;
     const uint32_t a_in = at.a_input, src = at.color;
@@ -578,7 +621,7 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, int64_t p, int32_t Y
                 cov += (t & 0xfu) | ((t & 0xf0u) << 4) | ((t & 0xf00u) << 8) | ((t & 0xf000u) << 12);
             }
             if (cov == 0) continue;
-            uint32_t *dst = reinterpret_cast<uint32_t *>(A.canvas + static_cast<long long>(yy) * A.stride + xq);
+            uint32_t *dst = reinterpret_cast<uint32_t *>(pg.canvas + static_cast<long long>(yy) * pg.stride + xq);
             const uint32_t old = *dst;
             uint32_t out = old;
 #pragma unroll
@@ -598,7 +641,7 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, int64_t p, int32_t Y
                 }
                 out = (out & ~(0xffu << (8 * px))) | (res << (8 * px));
                 ++w_cov;
-                if (A.cov_hash) hsum += cov_term(static_cast<long long>(yy) * A.width + x, shape);
+                if (A.cov_hash) hsum += cov_term(static_cast<long long>(yy) * pg.width + x, shape);
             }
             if (out != old) *dst = out;
         }
@@ -620,12 +663,50 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, int64_t p, int32_t Y
     }
 }
 
-constexpr int kLdsK = 32;
+template <int K>
+__device__ __forceinline__ void init_lds(BandLds<K> &L) {
+    if (threadIdx.x < 17) L.gamma[threadIdx.x] = kGamma[threadIdx.x];
+    __syncthreads();
+}
 
-// Plain route: one launch per path, one wave per band of 16 output rows.
-__global__ void __launch_bounds__(64) k_fill_plain(FillArgs A, int64_t p, int32_t row0) {
+// PLAIN: one launch per path, one wave per band of 16 output rows.
+__global__ void __launch_bounds__(64) k_fill_plain(FillArgs A, int32_t page, int64_t p, int32_t row0) {
     __shared__ BandLds<kLdsK> L;
-    fill_band<kLdsK>(A, L, p, row0 + kBandRows * static_cast<int32_t>(blockIdx.x));
+    init_lds(L);
+    fill_band<kLdsK>(A, L, A.pages[page], p, row0 + kBandRows * static_cast<int32_t>(blockIdx.x), blockIdx.x);
+}
+
+// BATCHED: persistent waves pull (page, first row) items and composite every
+// overlapping path of the page, in path order, into that band.
+__global__ void __launch_bounds__(64) k_fill_batched(FillArgs A, const int2 *__restrict__ items, int32_t n_items,
+                                                     unsigned int *__restrict__ next_item) {
+    __shared__ BandLds<kLdsK> L;
+    init_lds(L);
+    const int lane = threadIdx.x;
+    for (;;) {
+        unsigned int it = 0;
+        if (lane == 0) it = atomicAdd(next_item, 1u);
+        it = __shfl(it, 0);
+        if (it >= static_cast<unsigned int>(n_items)) break;
+        const int2 item = items[it];
+        const DevPage &pg = A.pages[item.x];
+        const int32_t Y0 = item.y;
+        const int32_t Y1 = Y0 + kBandRows - 1;
+        for (int64_t base = pg.path_begin; base < pg.path_end; base += 64) {
+            const int64_t q = base + lane;
+            bool ov = false;
+            if (q < pg.path_end) {
+                const int2 rw = A.rows[q];
+                ov = rw.x <= rw.y && rw.x <= Y1 && rw.y >= Y0;
+            }
+            uint64_t mask = __ballot(ov);
+            while (mask) {
+                const int jj = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+                mask &= mask - 1;
+                fill_band<kLdsK>(A, L, pg, base + jj, Y0, blockIdx.x);
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -640,6 +721,7 @@ struct Fence {
 
 struct rr_hip_ctx {
     int device = 0;
+    int n_cu = 0;
     hipStream_t stream = nullptr;
     std::string err;
     // workspace
@@ -647,14 +729,23 @@ struct rr_hip_ctx {
     size_t recs_cap = 0;
     PathHdr *hdrs = nullptr;
     size_t hdrs_cap = 0;
+    int2 *rows = nullptr;
+    size_t rows_cap = 0;
+    DevPage *pages = nullptr;
+    size_t pages_cap = 0;
+    int2 *items = nullptr;
+    size_t items_cap = 0;
+    int32_t *slots = nullptr;
+    size_t slots_cap = 0;  // in int32
     int32_t *spill = nullptr;
     long long spill_cap = 0;
     unsigned long long *spill_top = nullptr;
+    unsigned int *next_item = nullptr;
     int32_t *err_flag = nullptr;
+    int batched_waves = 0;
     // page state
     hipEvent_t t0 = nullptr, t1 = nullptr;
     bool page_open = false, page_timed = false;
-    int64_t launches = 0;
     double last_ms = 0.0;
     int64_t last_launches = 0;
     Fence fences[16];
@@ -665,9 +756,13 @@ struct rr_hip_ctx {
     size_t prof_used = 0;
     double last_fill_ms = 0.0;
     int64_t fill_launches = 0;
+    std::vector<DevPage> host_pages;
+    std::vector<int2> host_items;
 };
 
 namespace {
+
+constexpr long long kSlotInts = static_cast<long long>(kSlotK) * 192;
 
 int32_t fail(rr_hip_ctx *c, int32_t code, const std::string &msg) {
     if (c) c->err = msg;
@@ -715,23 +810,166 @@ hipEvent_t *prof_pair(rr_hip_ctx *c) {
     return p;
 }
 
+int32_t sat_i32(double v) {
+    if (std::isnan(v)) return INT32_MIN;
+    if (v >= 2147483647.0) return INT32_MAX;
+    if (v <= -2147483648.0) return INT32_MIN;
+    return static_cast<int32_t>(v);
+}
+
+// Clip::new + recompute_int_bounds (crates/raster/src/clip.rs:73-88,266-271)
 ClipRect make_clip(const double cl[4]) {
     ClipRect r;
     r.x_min = std::fmin(cl[0], cl[2]);
     r.x_max = std::fmax(cl[0], cl[2]);
     r.y_min = std::fmin(cl[1], cl[3]);
     r.y_max = std::fmax(cl[1], cl[3]);
-    auto sat = [](double v) -> int32_t {
-        if (std::isnan(v)) return INT32_MIN;
-        if (v >= 2147483647.0) return INT32_MAX;
-        if (v <= -2147483648.0) return INT32_MIN;
-        return static_cast<int32_t>(v);
-    };
-    r.x_min_i = sat(std::floor(r.x_min));
-    r.y_min_i = sat(std::floor(r.y_min));
-    r.x_max_i = sat(std::ceil(r.x_max)) - 1;
-    r.y_max_i = sat(std::ceil(r.y_max)) - 1;
+    r.x_min_i = sat_i32(std::floor(r.x_min));
+    r.y_min_i = sat_i32(std::floor(r.y_min));
+    r.x_max_i = sat_i32(std::ceil(r.x_max)) - 1;
+    r.y_max_i = sat_i32(std::ceil(r.y_max)) - 1;
     return r;
+}
+
+int32_t validate_page(rr_hip_ctx *c, const rr_page_desc &p) {
+    if (p.width <= 0 || p.height <= 0 || p.stride < ((p.width + 3) & ~3) || (p.stride & 3) != 0)
+        return fail(c, RR_E_INVARIANT, "page: bad canvas geometry (stride must be a multiple of 4 >= width)");
+    if (!p.canvas || (reinterpret_cast<uintptr_t>(p.canvas) & 3u))
+        return fail(c, RR_E_INVARIANT, "page: canvas must be a 4-byte aligned device pointer");
+    if (p.path_begin < 0 || p.path_end < p.path_begin) return fail(c, RR_E_INVARIANT, "page: bad path range");
+    return RR_OK;
+}
+
+// Records the edge-setup pass and the fill kernels for every page of b.
+int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
+    if (!c->page_open) return fail(c, RR_E_INVARIANT, "record outside begin_page/submit_page");
+    if (b.n_paths < 0 || b.n_edges < 0 || b.n_pages < 0) return fail(c, RR_E_INVARIANT, "negative counts");
+    if (b.mode != RR_MODE_DOCUMENTED && b.mode != RR_MODE_LITERAL) return fail(c, RR_E_INVARIANT, "unknown AA row mode");
+    if (b.flavour != RR_FLAVOUR_GRAY8 && b.flavour != RR_FLAVOUR_RGB8_FAST)
+        return fail(c, RR_E_INVARIANT, "unknown canvas flavour");
+    if (b.launch != RR_LAUNCH_PLAIN && b.launch != RR_LAUNCH_BATCHED)
+        return fail(c, RR_E_INVARIANT, "unknown launch shape");
+    if (b.n_paths == 0 || b.n_pages == 0) return RR_OK;
+    if (!b.edges || !b.edge_off || !b.attrs || !b.pages || (b.launch == RR_LAUNCH_PLAIN && !b.host_rows))
+        return fail(c, RR_E_INVARIANT, "null buffer");
+    if (reinterpret_cast<uintptr_t>(b.edges) & 31u) return fail(c, RR_E_INVARIANT, "edges must be 32-byte aligned");
+    int64_t prev_end = 0;
+    c->host_pages.resize(static_cast<size_t>(b.n_pages));
+    for (int32_t i = 0; i < b.n_pages; ++i) {
+        const rr_page_desc &p = b.pages[i];
+        int32_t rc = validate_page(c, p);
+        if (rc) return rc;
+        if (p.path_begin != prev_end) return fail(c, RR_E_INVARIANT, "pages must tile [0, n_paths) in order");
+        prev_end = p.path_end;
+        DevPage &d = c->host_pages[static_cast<size_t>(i)];
+        d.canvas = p.canvas;
+        d.path_begin = p.path_begin;
+        d.path_end = p.path_end;
+        d.clip = make_clip(p.clip);
+        d.width = p.width;
+        d.height = p.height;
+        d.stride = p.stride;
+        d.px_lo = std::max(0, d.clip.x_min_i);
+        d.px_hi = std::min(p.width - 1, d.clip.x_max_i);
+        d.pad = 0;
+    }
+    if (prev_end != b.n_paths) return fail(c, RR_E_INVARIANT, "pages must tile [0, n_paths) in order");
+    RR_HIP_TRY(c, hipSetDevice(c->device));
+    int32_t rc = ensure(c, &c->recs, &c->recs_cap, static_cast<size_t>(b.n_edges > 0 ? b.n_edges : 1));
+    if (!rc) rc = ensure(c, &c->hdrs, &c->hdrs_cap, static_cast<size_t>(b.n_paths));
+    if (!rc) rc = ensure(c, &c->rows, &c->rows_cap, static_cast<size_t>(b.n_paths));
+    if (!rc) rc = ensure(c, &c->pages, &c->pages_cap, static_cast<size_t>(b.n_pages));
+    if (rc) return rc;
+    if (b.launch == RR_LAUNCH_BATCHED) {
+        // Work items = (page, first row of a 16-row band) for every band of
+        // every page; pages with more paths first so their bands start early.
+        std::vector<int32_t> order(static_cast<size_t>(b.n_pages));
+        for (int32_t i = 0; i < b.n_pages; ++i) order[static_cast<size_t>(i)] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+            return (b.pages[x].path_end - b.pages[x].path_begin) > (b.pages[y].path_end - b.pages[y].path_begin);
+        });
+        c->host_items.clear();
+        for (int32_t i : order)
+            for (int32_t y0 = 0; y0 < b.pages[i].height; y0 += kBandRows) c->host_items.push_back(make_int2(i, y0));
+        rc = ensure(c, &c->items, &c->items_cap, c->host_items.size());
+        if (rc) return rc;
+        RR_HIP_TRY(c, hipMemcpyAsync(c->items, c->host_items.data(), sizeof(int2) * c->host_items.size(),
+                                     hipMemcpyHostToDevice, c->stream));
+    }
+    RR_HIP_TRY(c, hipMemcpyAsync(c->pages, c->host_pages.data(), sizeof(DevPage) * b.n_pages, hipMemcpyHostToDevice,
+                                 c->stream));
+    // The host tables are reused by the next recording: let the copies land.
+    RR_HIP_TRY(c, hipStreamSynchronize(c->stream));
+
+    FillArgs A{};
+    A.recs = c->recs;
+    A.hdrs = c->hdrs;
+    A.rows = c->rows;
+    A.edge_off = b.edge_off;
+    A.attrs = b.attrs;
+    A.pages = c->pages;
+    A.cov_hash = b.cov_hash;
+    A.cov_count = b.cov_count;
+    A.spill = c->spill;
+    A.spill_top = c->spill_top;
+    A.err = c->err_flag;
+    A.spill_cap = c->spill_cap;
+    A.mode = b.mode;
+    A.flavour = b.flavour;
+    A.slot_ints = kSlotInts;
+
+    if (!c->page_timed) {
+        RR_HIP_TRY(c, hipEventRecord(c->t0, c->stream));
+        c->page_timed = true;
+    }
+    RR_HIP_TRY(c, hipMemsetAsync(c->spill_top, 0, sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_edge_setup, dim3(static_cast<unsigned>(b.n_paths)), dim3(256), 0, c->stream, b.edges,
+                       b.edge_off, c->pages, b.n_pages, c->recs, c->hdrs, c->rows, b.mode);
+    RR_HIP_TRY(c, hipGetLastError());
+
+    if (b.launch == RR_LAUNCH_PLAIN) {
+        int32_t max_bands = 1;
+        for (int32_t i = 0; i < b.n_pages; ++i)
+            max_bands = std::max(max_bands, (b.pages[i].height + kBandRows) / kBandRows + 1);
+        rc = ensure(c, &c->slots, &c->slots_cap, static_cast<size_t>(max_bands) * kSlotInts);
+        if (rc) return rc;
+        A.slots = c->slots;
+        for (int32_t pi = 0; pi < b.n_pages; ++pi) {
+            const rr_page_desc &pg = b.pages[pi];
+            for (int64_t p = pg.path_begin; p < pg.path_end; ++p) {
+                int32_t r0 = b.host_rows[2 * p], r1 = b.host_rows[2 * p + 1];
+                r0 = std::max(r0, 0);
+                r1 = std::min(r1, pg.height - 1);
+                if (r0 > r1) continue;
+                const unsigned bands = static_cast<unsigned>((r1 - r0) / kBandRows + 1);
+                hipEvent_t *ev = c->profiling ? prof_pair(c) : nullptr;
+                if (c->profiling && !ev) return fail(c, RR_E_DEVICE, "profiling event allocation failed");
+                if (ev) RR_HIP_TRY(c, hipEventRecord(ev[0], c->stream));
+                hipLaunchKernelGGL(k_fill_plain, dim3(bands), dim3(64), 0, c->stream, A, pi, p, r0);
+                if (ev) RR_HIP_TRY(c, hipEventRecord(ev[1], c->stream));
+                ++c->fill_launches;
+            }
+        }
+        RR_HIP_TRY(c, hipGetLastError());
+        return RR_OK;
+    }
+
+    // BATCHED: one persistent launch over the (page, band) items.
+    const size_t n_items = c->host_items.size();
+    const unsigned waves = static_cast<unsigned>(std::min<size_t>(static_cast<size_t>(c->batched_waves), n_items));
+    rc = ensure(c, &c->slots, &c->slots_cap, static_cast<size_t>(waves) * kSlotInts);
+    if (rc) return rc;
+    A.slots = c->slots;
+    RR_HIP_TRY(c, hipMemsetAsync(c->next_item, 0, sizeof(unsigned int), c->stream));
+    hipEvent_t *ev = c->profiling ? prof_pair(c) : nullptr;
+    if (c->profiling && !ev) return fail(c, RR_E_DEVICE, "profiling event allocation failed");
+    if (ev) RR_HIP_TRY(c, hipEventRecord(ev[0], c->stream));
+    hipLaunchKernelGGL(k_fill_batched, dim3(waves), dim3(64), 0, c->stream, A, c->items, static_cast<int32_t>(n_items),
+                       c->next_item);
+    if (ev) RR_HIP_TRY(c, hipEventRecord(ev[1], c->stream));
+    ++c->fill_launches;
+    RR_HIP_TRY(c, hipGetLastError());
+    return RR_OK;
 }
 
 }  // namespace
@@ -749,26 +987,25 @@ int32_t rr_hip_create(int32_t device, rr_hip_ctx **out) {
         delete c;
         return RR_E_NO_DEVICE;
     }
-    int32_t rc;
-    if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess) rc = RR_E_DEVICE;
-    else if (hipMalloc(&c->spill_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
-    else if (hipMalloc(&c->err_flag, sizeof(int32_t)) != hipSuccess) rc = RR_E_DEVICE;
-    else rc = RR_OK;
+    int32_t rc = RR_OK;
+    if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) rc = RR_E_DEVICE;
+    int per_cu = 0;
+    if (rc == RR_OK && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_batched, 64, 0) != hipSuccess)
+        rc = RR_E_DEVICE;
+    c->batched_waves = std::max(1, c->n_cu * std::max(1, per_cu));
+    if (rc == RR_OK && (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess)) rc = RR_E_DEVICE;
+    if (rc == RR_OK && hipMalloc(&c->spill_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
+    if (rc == RR_OK && hipMalloc(&c->next_item, sizeof(unsigned int)) != hipSuccess) rc = RR_E_DEVICE;
+    if (rc == RR_OK && hipMalloc(&c->err_flag, sizeof(int32_t)) != hipSuccess) rc = RR_E_DEVICE;
     for (auto &f : c->fences)
         if (rc == RR_OK && hipEventCreateWithFlags(&f.ev, hipEventDisableTiming) != hipSuccess) rc = RR_E_DEVICE;
+    const long long pool_ints = 64ll << 20;  // 256 MiB shared pool for rows beyond a wave's slot
+    if (rc == RR_OK && hipMalloc(&c->spill, pool_ints * sizeof(int32_t)) != hipSuccess) rc = RR_E_DEVICE;
+    c->spill_cap = pool_ints;
+    if (rc == RR_OK && hipMemset(c->err_flag, 0, sizeof(int32_t)) != hipSuccess) rc = RR_E_DEVICE;
     if (rc != RR_OK) {
         rr_hip_destroy(c);
         return rc;
-    }
-    const long long pool_ints = 64ll << 20;  // 256 MiB overflow-entry pool
-    if (hipMalloc(&c->spill, pool_ints * sizeof(int32_t)) != hipSuccess) {
-        rr_hip_destroy(c);
-        return RR_E_DEVICE;
-    }
-    c->spill_cap = pool_ints;
-    if (hipMemset(c->err_flag, 0, sizeof(int32_t)) != hipSuccess) {
-        rr_hip_destroy(c);
-        return RR_E_DEVICE;
     }
     *out = c;
     return RR_OK;
@@ -778,11 +1015,10 @@ int32_t rr_hip_destroy(rr_hip_ctx *c) {
     if (!c) return RR_E_INVARIANT;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    (void)hipFree(c->recs);
-    (void)hipFree(c->hdrs);
-    (void)hipFree(c->spill);
-    (void)hipFree(c->spill_top);
-    (void)hipFree(c->err_flag);
+    void *bufs[] = {c->recs, c->hdrs, c->rows, c->pages, c->items, c->slots, c->spill, c->spill_top, c->next_item,
+                    c->err_flag};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
     if (c->t0) (void)hipEventDestroy(c->t0);
     if (c->t1) (void)hipEventDestroy(c->t1);
     for (auto &f : c->fences)
@@ -835,83 +1071,42 @@ int32_t rr_hip_begin_page(rr_hip_ctx *c) {
     if (!c) return RR_E_INVARIANT;
     c->page_open = true;
     c->page_timed = false;
-    c->launches = 0;
     c->fill_launches = 0;
     c->prof_used = 0;
     RR_HIP_TRY(c, hipMemsetAsync(c->err_flag, 0, sizeof(int32_t), c->stream));
     return RR_OK;
 }
 
+int32_t rr_hip_record_fill_pages(rr_hip_ctx *c, const rr_fill_pages *b) {
+    if (!c || !b) return RR_E_INVARIANT;
+    return record_pages(c, *b);
+}
+
 int32_t rr_hip_record_fill_paths(rr_hip_ctx *c, const rr_fill_batch *b) {
     if (!c || !b) return RR_E_INVARIANT;
-    if (!c->page_open) return fail(c, RR_E_INVARIANT, "record_fill_paths outside begin_page/submit_page");
-    if (b->n_paths < 0 || b->n_edges < 0 || b->width <= 0 || b->height <= 0 || b->stride < ((b->width + 3) & ~3) ||
-        (b->stride & 3) != 0)
-        return fail(c, RR_E_INVARIANT, "fill batch: bad canvas geometry (stride must be a multiple of 4 >= width)");
-    if (b->mode != RR_MODE_DOCUMENTED && b->mode != RR_MODE_LITERAL)
-        return fail(c, RR_E_INVARIANT, "fill batch: unknown AA row mode");
-    if (b->flavour != RR_FLAVOUR_GRAY8 && b->flavour != RR_FLAVOUR_RGB8_FAST)
-        return fail(c, RR_E_INVARIANT, "fill batch: unknown canvas flavour");
-    if (b->launch != RR_LAUNCH_PLAIN) return fail(c, RR_E_INVARIANT, "fill batch: launch shape not available");
-    if (b->n_paths == 0) return RR_OK;
-    if (!b->edges || !b->edge_off || !b->attrs || !b->canvas || !b->host_rows)
-        return fail(c, RR_E_INVARIANT, "fill batch: null buffer");
-    if (reinterpret_cast<uintptr_t>(b->canvas) & 3u)
-        return fail(c, RR_E_INVARIANT, "fill batch: canvas must be 4-byte aligned");
-    if (reinterpret_cast<uintptr_t>(b->edges) & 31u)
-        return fail(c, RR_E_INVARIANT, "fill batch: edges must be 32-byte aligned");
-    RR_HIP_TRY(c, hipSetDevice(c->device));
-    int32_t rc = ensure(c, &c->recs, &c->recs_cap, static_cast<size_t>(b->n_edges > 0 ? b->n_edges : 1));
-    if (rc) return rc;
-    rc = ensure(c, &c->hdrs, &c->hdrs_cap, static_cast<size_t>(b->n_paths));
-    if (rc) return rc;
-
-    const ClipRect clip = make_clip(b->clip);
-    FillArgs A{};
-    A.recs = c->recs;
-    A.hdrs = c->hdrs;
-    A.edge_off = b->edge_off;
-    A.attrs = b->attrs;
-    A.canvas = b->canvas;
-    A.cov_hash = b->cov_hash;
-    A.cov_count = b->cov_count;
-    A.spill = c->spill;
-    A.spill_top = c->spill_top;
-    A.err = c->err_flag;
-    A.spill_cap = c->spill_cap;
-    A.width = b->width;
-    A.height = b->height;
-    A.stride = b->stride;
-    A.mode = b->mode;
-    A.flavour = b->flavour;
-    A.px_lo = std::max(0, clip.x_min_i);
-    A.px_hi = std::min(b->width - 1, clip.x_max_i);
-
-    if (!c->page_timed) {
-        RR_HIP_TRY(c, hipEventRecord(c->t0, c->stream));
-        c->page_timed = true;
-    }
-    RR_HIP_TRY(c, hipMemsetAsync(c->spill_top, 0, sizeof(unsigned long long), c->stream));
-    hipLaunchKernelGGL(k_edge_setup, dim3(static_cast<unsigned>(b->n_paths)), dim3(256), 0, c->stream, b->edges,
-                       b->edge_off, c->recs, c->hdrs, clip);
-    RR_HIP_TRY(c, hipGetLastError());
-    ++c->launches;
-    for (int64_t p = 0; p < b->n_paths; ++p) {
-        int32_t r0 = b->host_rows[2 * p], r1 = b->host_rows[2 * p + 1];
-        r0 = std::max(r0, 0);
-        r1 = std::min(r1, b->height - 1);
-        if (r0 > r1) continue;
-        const unsigned bands = static_cast<unsigned>((r1 - r0) / kBandRows + 1);
-        hipEvent_t *ev = c->profiling ? prof_pair(c) : nullptr;
-        if (c->profiling && !ev) return fail(c, RR_E_DEVICE, "profiling event allocation failed");
-        if (ev) RR_HIP_TRY(c, hipEventRecord(ev[0], c->stream));
-        hipLaunchKernelGGL(k_fill_plain, dim3(bands), dim3(64), 0, c->stream, A, p, r0);
-        if (ev) RR_HIP_TRY(c, hipEventRecord(ev[1], c->stream));
-        ++c->launches;
-        ++c->fill_launches;
-    }
-    RR_HIP_TRY(c, hipGetLastError());
-    return RR_OK;
+    rr_page_desc pg{};
+    pg.canvas = b->canvas;
+    pg.width = b->width;
+    pg.height = b->height;
+    pg.stride = b->stride;
+    pg.path_begin = 0;
+    pg.path_end = b->n_paths;
+    for (int k = 0; k < 4; ++k) pg.clip[k] = b->clip[k];
+    rr_fill_pages fp{};
+    fp.edges = b->edges;
+    fp.edge_off = b->edge_off;
+    fp.attrs = b->attrs;
+    fp.host_rows = b->host_rows;
+    fp.n_paths = b->n_paths;
+    fp.n_edges = b->n_edges;
+    fp.pages = &pg;
+    fp.n_pages = 1;
+    fp.mode = b->mode;
+    fp.flavour = b->flavour;
+    fp.launch = b->launch;
+    fp.cov_hash = b->cov_hash;
+    fp.cov_count = b->cov_count;
+    return record_pages(c, fp);
 }
 
 int32_t rr_hip_submit_page(rr_hip_ctx *c, uint64_t *fence) {

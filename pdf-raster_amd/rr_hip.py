@@ -40,6 +40,22 @@ class FillBatch(ctypes.Structure):
                 ("cov_count", ctypes.c_void_p)]
 
 
+class PageDesc(ctypes.Structure):
+    _fields_ = [("canvas", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("stride", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("path_begin", ctypes.c_int64), ("path_end", ctypes.c_int64),
+                ("clip", ctypes.c_double * 4)]
+
+
+class FillPages(ctypes.Structure):
+    _fields_ = [("edges", ctypes.c_void_p), ("edge_off", ctypes.c_void_p),
+                ("attrs", ctypes.c_void_p), ("host_rows", ctypes.c_void_p),
+                ("n_paths", ctypes.c_int64), ("n_edges", ctypes.c_int64),
+                ("pages", ctypes.c_void_p), ("n_pages", ctypes.c_int32),
+                ("mode", ctypes.c_int32), ("flavour", ctypes.c_int32), ("launch", ctypes.c_int32),
+                ("cov_hash", ctypes.c_void_p), ("cov_count", ctypes.c_void_p)]
+
+
 _lib = None
 
 
@@ -62,6 +78,7 @@ def lib():
             "rr_hip_memset": (i32, [P, P, i32, ctypes.c_size_t]),
             "rr_hip_begin_page": (i32, [P]),
             "rr_hip_record_fill_paths": (i32, [P, ctypes.POINTER(FillBatch)]),
+            "rr_hip_record_fill_pages": (i32, [P, ctypes.POINTER(FillPages)]),
             "rr_hip_submit_page": (i32, [P, ctypes.POINTER(u64)]),
             "rr_hip_wait_page": (i32, [P, u64]),
             "rr_hip_set_profiling": (i32, [P, i32]),
@@ -252,6 +269,10 @@ class Backend:
         self._check(self.L.rr_hip_record_fill_paths(self.h, ctypes.byref(batch)),
                     "record_fill_paths")
 
+    def record_fill_pages(self, batch):
+        self._check(self.L.rr_hip_record_fill_pages(self.h, ctypes.byref(batch)),
+                    "record_fill_pages")
+
     def submit_page(self):
         f = ctypes.c_uint64(0)
         self._check(self.L.rr_hip_submit_page(self.h, ctypes.byref(f)), "submit_page")
@@ -346,3 +367,86 @@ class DevicePage:
         self.be.record_fill_paths(b)
         self.be.wait_page(self.be.submit_page())
         return self.read_canvas(), (self.read_hashes() if hashes else None)
+
+
+class DeviceBatch:
+    """Several pages resident on the device: edges, offsets and attributes
+    concatenated with global path indices, one canvas per page."""
+
+    def __init__(self, be, pages, attrs=None):
+        self.be, self.pages = be, pages
+        offs = [0]
+        e_base = 0
+        edge_off = []
+        for pg in pages:
+            edge_off.append(pg.edge_off[:-1] + e_base)
+            e_base += pg.n_edges
+        edge_off.append(np.array([e_base], np.int64))
+        self.n_paths = sum(pg.n_paths for pg in pages)
+        self.n_edges = e_base
+        self.path_begin = np.cumsum([0] + [pg.n_paths for pg in pages])
+        self.edges = be.to_device(np.concatenate([pg.edges for pg in pages]).astype(np.float64)
+                                  if self.n_edges else np.zeros((1, 4)))
+        self.edge_off = be.to_device(np.concatenate(edge_off).astype(np.int64))
+        a = np.concatenate([pg.attrs for pg in pages]) if attrs is None else attrs
+        self.attrs_host = a.copy()
+        self.attrs = be.to_device(self.attrs_host)
+        self.host_rows = np.ascontiguousarray(np.concatenate([pg.host_rows for pg in pages]), np.int32)
+        self.strides = [stride_for(pg.width) for pg in pages]
+        self.canvases = [be.alloc(st * pg.height) for st, pg in zip(self.strides, pages)]
+        self.hashes = be.alloc(8 * max(self.n_paths, 1))
+        self.counts = be.alloc(16 * max(self.n_paths, 1))
+        self.descs = (PageDesc * len(pages))()
+        for i, pg in enumerate(pages):
+            d = self.descs[i]
+            d.canvas, d.width, d.height, d.stride = self.canvases[i], pg.width, pg.height, self.strides[i]
+            d.path_begin, d.path_end = int(self.path_begin[i]), int(self.path_begin[i + 1])
+            for k, v in enumerate((0.0, 0.0, pg.width - 0.001, pg.height - 0.001)):
+                d.clip[k] = v
+
+    def free(self):
+        for b in [self.edges, self.edge_off, self.attrs, self.hashes, self.counts] + self.canvases:
+            self.be.free(b)
+
+    def fill_pages(self, mode=MODE_DOCUMENTED, flavour=FLAVOUR_GRAY8, launch=LAUNCH_BATCHED,
+                   hashes=False, counts=False):
+        b = FillPages()
+        b.edges, b.edge_off, b.attrs = self.edges, self.edge_off, self.attrs
+        b.host_rows = self.host_rows.ctypes.data
+        b.n_paths, b.n_edges = self.n_paths, self.n_edges
+        b.pages = ctypes.cast(self.descs, ctypes.c_void_p)
+        b.n_pages = len(self.pages)
+        b.mode, b.flavour, b.launch = mode, flavour, launch
+        b.cov_hash = self.hashes if hashes else None
+        b.cov_count = self.counts if counts else None
+        return b
+
+    def reset(self, background=255):
+        L, h = self.be.L, self.be.h
+        for c, st, pg in zip(self.canvases, self.strides, self.pages):
+            self.be._check(L.rr_hip_memset(h, c, background, st * pg.height), "memset")
+        self.be._check(L.rr_hip_memset(h, self.hashes, 0, 8 * max(self.n_paths, 1)), "memset")
+        self.be._check(L.rr_hip_memset(h, self.counts, 0, 16 * max(self.n_paths, 1)), "memset")
+
+    def run(self, mode=MODE_DOCUMENTED, flavour=FLAVOUR_GRAY8, launch=LAUNCH_BATCHED,
+            hashes=False, counts=False, background=255):
+        self.reset(background)
+        self.be.begin_page()
+        self.be.record_fill_pages(self.fill_pages(mode, flavour, launch, hashes, counts))
+        self.be.wait_page(self.be.submit_page())
+
+    def read_canvas(self, i):
+        pg = self.pages[i]
+        out = np.zeros((pg.height, self.strides[i]), np.uint8)
+        self.be.download(self.canvases[i], out)
+        return out[:, :pg.width].copy()
+
+    def read_hashes(self):
+        out = np.zeros(max(self.n_paths, 1), np.uint64)
+        self.be.download(self.hashes, out)
+        return out[:self.n_paths].copy()
+
+    def read_counts(self):
+        out = np.zeros((max(self.n_paths, 1), 2), np.uint64)
+        self.be.download(self.counts, out)
+        return out[:self.n_paths].copy()

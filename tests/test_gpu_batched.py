@@ -1,0 +1,63 @@
+"""The batched persistent launch gives the same pixels and per-path coverage
+hashes as the oracle (and hence as the one-path-per-launch route)."""
+import numpy as np
+import pytest
+
+import rr_hip as R
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_pages(pages, mode=O.MODE_DOCUMENTED, flavour=O.FLAVOUR_GRAY8):
+    canv, hs = [], []
+    for pg in pages:
+        c = np.full((pg.height, pg.width), 255, np.uint8)
+        a = pg.attrs
+        _, h = O.fill_page(c, pg.edges, pg.edge_off, a["eo"], a["color"], a["a_input"], mode, flavour)
+        canv.append(c)
+        hs.append(h)
+    return canv, np.concatenate(hs)
+
+
+def check(be, pages, mode=R.MODE_DOCUMENTED, flavour=R.FLAVOUR_GRAY8, launch=R.LAUNCH_BATCHED):
+    db = R.DeviceBatch(be, pages)
+    try:
+        db.run(mode, flavour, launch, hashes=True)
+        got = [db.read_canvas(i) for i in range(len(pages))]
+        gh = db.read_hashes()
+    finally:
+        db.free()
+    oc, oh = oracle_pages(pages, mode, flavour)
+    for i, (g, o) in enumerate(zip(got, oc)):
+        if not np.array_equal(g, o):
+            ys, xs = np.nonzero(g != o)
+            raise AssertionError(f"page {i}: {len(ys)} px differ, first y={ys[0]} x={xs[0]}")
+    bad = np.nonzero(gh != oh)[0]
+    assert len(bad) == 0, f"coverage hashes differ for paths {bad[:10]}"
+
+
+def test_batched_polygon_subset(backend):
+    check(backend, [R.workload(1, 0xC0F10001)[0].subset(range(0, 500, 7))])
+
+
+def test_batched_glyph_subset(backend):
+    check(backend, [R.workload(2, 0xC0F10002)[0].subset(range(0, 6000, 5))])
+
+
+@pytest.mark.parametrize("mode", [R.MODE_DOCUMENTED, R.MODE_LITERAL])
+def test_batched_mixed_pages(backend, mode):
+    pages = R.workload(3, 0xC0F10003, 3)
+    pages = [pg.subset(range(0, pg.n_paths, 4)) for pg in pages]
+    check(backend, pages, mode)
+
+
+def test_batched_rgb8_flavour(backend):
+    pages = [R.workload(1, 0xC0F10001)[0].subset(range(1, 500, 11))]
+    check(backend, pages, flavour=R.FLAVOUR_RGB8_FAST)
+
+
+def test_plain_multi_page_matches(backend):
+    pages = R.workload(3, 0xC0F10003, 2)
+    pages = [pg.subset(range(0, pg.n_paths, 9)) for pg in pages]
+    check(backend, pages, launch=R.LAUNCH_PLAIN)
