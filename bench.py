@@ -5,7 +5,9 @@ Prints ONE JSON line.  A step renders one batch of the workload (every page:
 canvas reset to 255, all paths filled and composited in order) with the edge
 lists already resident in HBM.  `value` is the reference's pixel-composite
 count (the summed draw_aa_line widths, crates/raster/src/fill/mod.rs:344-401)
-divided by the wall time of the timed steps, in Mpx/s.
+divided by the wall time of the timed steps, in Mpx/s.  The same pages are
+also run through the plain one-path-per-launch route for the batched/plain
+ratio, and through the CPU oracle on a time-bounded sample.
 """
 import argparse
 import hashlib
@@ -26,7 +28,11 @@ METRIC = "AA-filled Mpx/s + pages/s @300dpi, batched+learned vs plain HIP; % HBM
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
-    # name: (config id, seed, max pages, launch shape, description)
+    # name: (config id, seed, max pages, primary launch shape, description)
+    "mixed-64": (3, 0xC0F10003, 0, R.LAUNCH_BATCHED,
+                 "64 pages (40 Letter@300, 12 A3@300, 8 A4@600, 4 thumbnails@150), 200-20000 "
+                 "paths per page log-uniform, bbox sides 4-4096 px, 3-2000 edges, 80/20 NZ/EO, "
+                 "grey U{0..255}; one persistent batched launch"),
     "glyph-page": (2, 0xC0F10002, 0, R.LAUNCH_PLAIN,
                    "Letter@300 2550x3300 u8, 6000 synthetic glyphs (1-3 contours of 12-40 "
                    "quadratic/cubic segments, flatness 0.1), NZ black, one path per launch"),
@@ -44,41 +50,42 @@ def load_golden(name):
         return json.load(f).get(name)
 
 
-def cpu_baseline(pages, budget_s, threads):
-    """Time the oracle (the reference CPU raster restated) on the same pages."""
+def cpu_baseline(pages, w_per_path, budget_s, threads, mode):
+    """Time the oracle (the reference CPU raster restated) on the same paths:
+    thread t renders page t mod len(pages) path by path, in order, until the
+    budget is spent; Mpx/s counts the reference composites of the paths done."""
     sys.path.insert(0, ROOT)
     from oracle import oracle as O  # the checker, timed here as the CPU reference
 
-    def render(pg):
-        c = np.full((pg.height, pg.width), 255, np.uint8)
+    base = np.cumsum([0] + [pg.n_paths for pg in pages])
+    done_w = [0] * threads
+    done_paths = [0] * threads
+
+    def worker(t):
+        pi = t % len(pages)
+        pg = pages[pi]
+        canvas = np.full((pg.height, pg.width), 255, np.uint8)
         a = pg.attrs
-        O.fill_page(c, pg.edges, pg.edge_off, a["eo"], a["color"], a["a_input"], hashes=False)
-
-    # one pass single-threaded to size the sample
-    t0 = time.perf_counter()
-    render(pages[0])
-    one = time.perf_counter() - t0
-    reps = max(1, int(budget_s / max(one, 1e-3)))
-    done = [0]
-    lock = threading.Lock()
-
-    def worker():
-        while True:
-            with lock:
-                if done[0] >= reps:
-                    return
-                done[0] += 1
-                i = done[0]
-            render(pages[i % len(pages)])
+        t_end = time.perf_counter() + budget_s
+        p = 0
+        while time.perf_counter() < t_end:
+            if p == pg.n_paths:
+                canvas[:] = 255
+                p = 0
+            O.fill(canvas, pg.path_edges(p), int(a["eo"][p]), int(a["color"][p]),
+                   int(a["a_input"][p]), mode)
+            done_w[t] += int(w_per_path[base[pi] + p])
+            done_paths[t] += 1
+            p += 1
 
     t0 = time.perf_counter()
-    ths = [threading.Thread(target=worker) for _ in range(threads)]
-    for t in ths:
-        t.start()
-    for t in ths:
-        t.join()
+    ths = [threading.Thread(target=worker, args=(t,)) for t in range(threads)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
     dt = time.perf_counter() - t0
-    return one, reps, dt
+    return sum(done_w), sum(done_paths), dt
 
 
 def main():
@@ -86,10 +93,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="glyph-page", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="mixed-64", choices=sorted(WORKLOADS))
+    ap.add_argument("--pages", type=int, default=0, help="limit the page count (0 = all)")
     ap.add_argument("--mode", default="documented", choices=["documented", "literal"])
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU oracle work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the one-path-per-launch comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1 "
@@ -97,91 +106,103 @@ def main():
         sys.exit(2)
 
     cfg, seed, max_pages, launch, desc = WORKLOADS[args.workload]
+    if args.pages:
+        max_pages = args.pages
     mode = R.MODE_LITERAL if args.mode == "literal" else R.MODE_DOCUMENTED
+    t_gen = time.perf_counter()
     pages = R.workload(cfg, seed, max_pages)
+    t_gen = time.perf_counter() - t_gen
+    print(f"bench: {args.workload}: {len(pages)} pages generated in {t_gen:.1f} s", file=sys.stderr,
+          flush=True)
     be = R.Backend(0)
-    dps = [R.DevicePage(be, pg) for pg in pages]
-    n_paths = sum(pg.n_paths for pg in pages)
-    n_edges = sum(pg.n_edges for pg in pages)
+    db = R.DeviceBatch(be, pages)
+    n_paths, n_edges = db.n_paths, db.n_edges
 
-    def step(counts=False, hashes=False):
-        for dp in dps:
-            dp.reset_canvas(255)
-            be.begin_page()
-            be.record_fill_paths(dp.batch(mode, R.FLAVOUR_GRAY8, launch, hashes=hashes,
-                                          counts=counts))
-            be.wait_page(be.submit_page())
+    def step(shape, hashes=False, counts=False):
+        db.run(mode, R.FLAVOUR_GRAY8, shape, hashes=hashes, counts=counts)
 
     # work counts + parity fingerprint (outside the timed region)
-    step(counts=True, hashes=True)
-    w_ref = w_cov = 0
+    step(launch, hashes=True, counts=True)
+    cnt = db.read_counts()
+    w_cov = int(cnt[:, 0].sum())
+    w_ref = int(cnt[:, 1].sum())
     digest = hashlib.sha256()
-    for dp in dps:
-        cnt = dp.read_counts()
-        w_cov += int(cnt[:, 0].sum())
-        w_ref += int(cnt[:, 1].sum())
-        digest.update(dp.read_canvas().tobytes())
-        digest.update(dp.read_hashes().tobytes())
+    for i in range(len(pages)):
+        digest.update(db.read_canvas(i).tobytes())
+    digest.update(db.read_hashes().astype("<u8").tobytes())
     fingerprint = digest.hexdigest()
     golden = load_golden(f"{args.workload}:{args.mode}")
     parity = "no-fixture" if golden is None else ("match" if golden == fingerprint else "MISMATCH")
 
-    for _ in range(args.warmup):
-        step()
-    # timed region
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    dt = time.perf_counter() - t0
-    ms_per_step = 1e3 * dt / args.steps
-    value = w_ref * args.steps / dt / 1e6
-    pages_per_s = len(pages) * args.steps / dt
+    def timed(shape, steps, warmup):
+        for _ in range(warmup):
+            step(shape)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(shape)
+        return (time.perf_counter() - t0) / steps
+
+    sec = timed(launch, args.steps, args.warmup)
+    print(f"bench: primary {sec * 1e3:.2f} ms/step", file=sys.stderr, flush=True)
+    value = w_ref / sec / 1e6
+    pages_per_s = len(pages) / sec
 
     # roofline: every fill launch bracketed by its own HIP event pair
     be.set_profiling(True)
-    fill_ms = 0.0
-    launches = 0
-    for dp in dps:
-        dp.reset_canvas(255)
-        be.begin_page()
-        be.record_fill_paths(dp.batch(mode, R.FLAVOUR_GRAY8, launch))
-        be.wait_page(be.submit_page())
-        _, f_ms, n_l = be.last_page_stats()
-        fill_ms += f_ms
-        launches += n_l
+    db.reset(255)
+    be.begin_page()
+    be.record_fill_pages(db.fill_pages(mode, R.FLAVOUR_GRAY8, launch))
+    be.wait_page(be.submit_page())
+    _, fill_ms, launches = be.last_page_stats()
     be.set_profiling(False)
-    alg_bytes = 2 * w_cov + 32 * n_edges + 8 * n_paths
+    if launch == R.LAUNCH_BATCHED:
+        canvas_bytes = sum(2 * pg.width * pg.height for pg in pages)
+        alg_bytes = canvas_bytes + 32 * n_edges + 8 * n_paths
+        kernel = "k_fill_batched"
+    else:
+        alg_bytes = 2 * w_cov + 32 * n_edges + 8 * n_paths
+        kernel = "k_fill_plain"
     per_launch_bytes = alg_bytes / max(launches, 1)
     avg_launch_s = fill_ms / 1e3 / max(launches, 1)
     achieved = per_launch_bytes / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mpx/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
         "data": "synthetic",
         "config": {"workload": args.workload, "description": desc, "pages": len(pages),
                    "paths": n_paths, "edges": n_edges, "aa_row_mode": args.mode,
-                   "canvas": "gray8-exact", "launch": "plain" if launch == R.LAUNCH_PLAIN else "batched"},
+                   "canvas": "gray8-exact",
+                   "launch": "batched" if launch == R.LAUNCH_BATCHED else "plain"},
         "pages_per_s": round(pages_per_s, 3),
         "covered_px_per_step": w_cov, "ref_composites_per_step": w_ref,
         "fill_launches_per_step": launches, "parity": parity, "fingerprint": fingerprint[:16],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                     "kernel": "k_fill_plain", "avg_launch_us": round(avg_launch_s * 1e6, 3),
+                     "kernel": kernel, "avg_launch_us": round(avg_launch_s * 1e6, 3),
                      "alg_bytes_per_launch": round(per_launch_bytes, 1)},
     }
+    if not args.no_plain:
+        other = R.LAUNCH_PLAIN if launch == R.LAUNCH_BATCHED else R.LAUNCH_BATCHED
+        sec_o = timed(other, 1, 0)
+        key = "plain" if other == R.LAUNCH_PLAIN else "batched"
+        out[f"{key}_ms_per_step"] = round(sec_o * 1e3, 3)
+        out[f"{key}_pages_per_s"] = round(len(pages) / sec_o, 3)
+        out[f"{key}_mpx_per_s"] = round(w_ref / sec_o / 1e6, 3)
+        out["batched_over_plain"] = round((sec_o / sec) if key == "plain" else (sec / sec_o), 3)
+    db.free()
+    be.close()
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
-        one, reps, cdt = cpu_baseline(pages, args.cpu_budget, threads)
-        w_page = w_ref / len(pages)
+        w_per_path = cnt[:, 1]
+        w_done, p_done, cdt = cpu_baseline(pages, w_per_path, args.cpu_budget, threads, mode)
         out["cpu_baseline"] = {
-            "value": round(w_page * reps / cdt / 1e6, 3), "unit": "Mpx/s", "cores": threads,
+            "value": round(w_done / cdt / 1e6, 3), "unit": "Mpx/s", "cores": threads,
             "kind": "port",
-            "sample": f"{reps} renders of the {args.workload} page(s) by the C oracle, one page per "
-                      f"thread over {threads} threads, {cdt:.1f} s",
-            "single_thread_mpx_s": round(w_page / one / 1e6, 3)}
-    be.close()
+            "sample": f"{p_done} paths of the first {min(threads, len(pages))} page(s) rendered "
+                      f"in order by the C oracle, one page per thread over {threads} threads, "
+                      f"{cdt:.1f} s"}
     print(json.dumps(out), flush=True)
 
 

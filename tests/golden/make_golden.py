@@ -1,9 +1,9 @@
 """Writes the golden fixtures under tests/golden/ from the CPU oracle.
 
 page_hashes.json: for each named page workload and AA row mode, the sha256 of
-  the oracle's final canvas bytes (H*W, row-major) followed by the per-path
-  coverage hashes (uint64 little-endian) -- the same fingerprint bench.py
-  computes from the GPU output.
+  the oracle's final canvas bytes (H*W, row-major, page after page) followed
+  by all per-path coverage hashes (uint64 little-endian) -- the same
+  fingerprint bench.py computes from the GPU output.
 tiny_masks.npz: full coverage masks (gamma-mapped shape per pixel) and final
   canvases of a few hand-built paths, for exact regression of the oracle.
 
@@ -29,13 +29,16 @@ MODES = {"documented": O.MODE_DOCUMENTED, "literal": O.MODE_LITERAL}
 
 
 def fingerprint(pages, mode):
+    """sha256 over every page's canvas bytes in order, then all coverage hashes."""
     d = hashlib.sha256()
+    hashes = []
     for pg in pages:
         c = np.full((pg.height, pg.width), 255, np.uint8)
         a = pg.attrs
         _, hs = O.fill_page(c, pg.edges, pg.edge_off, a["eo"], a["color"], a["a_input"], mode)
         d.update(c.tobytes())
-        d.update(hs.astype("<u8").tobytes())
+        hashes.append(hs)
+    d.update(np.concatenate(hashes).astype("<u8").tobytes())
     return d.hexdigest()
 
 

@@ -17,13 +17,15 @@ GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "page_ha
 @pytest.mark.parametrize("mode", ["documented", "literal"])
 def test_full_page_matches_oracle_fingerprint(backend, name, cfg, seed, mode):
     pages = R.workload(cfg, seed)
-    d = hashlib.sha256()
-    for pg in pages:
-        dp = R.DevicePage(backend, pg)
+    m = R.MODE_LITERAL if mode == "literal" else R.MODE_DOCUMENTED
+    for launch in (R.LAUNCH_PLAIN, R.LAUNCH_BATCHED):
+        db = R.DeviceBatch(backend, pages)
         try:
-            c, h = dp.render(R.MODE_LITERAL if mode == "literal" else R.MODE_DOCUMENTED)
+            db.run(m, R.FLAVOUR_GRAY8, launch, hashes=True)
+            d = hashlib.sha256()
+            for i in range(len(pages)):
+                d.update(db.read_canvas(i).tobytes())
+            d.update(db.read_hashes().astype("<u8").tobytes())
         finally:
-            dp.free()
-        d.update(c.tobytes())
-        d.update(h.astype("<u8").tobytes())
-    assert d.hexdigest() == GOLD[f"{name}:{mode}"]
+            db.free()
+        assert d.hexdigest() == GOLD[f"{name}:{mode}"], f"launch {launch}"
