@@ -91,6 +91,7 @@ struct FillArgs {
     const EdgeRec *recs;
     const PathHdr *hdrs;
     const int2 *rows;  // per path: drawn output rows [lo, hi] (lo > hi: nothing)
+    const int2 *csum;  // per 64-edge chunk (global edge index / 64): min r0, max r1
     const int64_t *edge_off;
     const rr_path_attr *attrs;
     const DevPage *pages;
@@ -314,6 +315,29 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
     }
 }
 
+// Row summary of every 64-edge chunk (global edge index / 64): the smallest r0
+// and largest r1 of its non-empty records, so that a band skips whole chunks.
+__global__ void __launch_bounds__(256) k_chunk_summary(const EdgeRec *__restrict__ recs, int64_t n_edges,
+                                                       int2 *__restrict__ csum, int64_t n_chunks) {
+    const int64_t ch = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (ch >= n_chunks) return;
+    const int64_t i = ch * 64 + lane;
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    if (i < n_edges) {
+        const EdgeRec q = recs[i];
+        if (q.r0 <= q.r1) {
+            lo = q.r0;
+            hi = q.r1;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o));
+        hi = max(hi, __shfl_xor(hi, o));
+    }
+    if (lane == 0) csum[ch] = make_int2(lo, hi);
+}
+
 // ---------------------------------------------------------------------------
 // kernel 2: scanline fill + composite of one band of 16 output rows
 // ---------------------------------------------------------------------------
@@ -326,6 +350,8 @@ constexpr int kSlotK = 512;    // global overflow entry slots per AA row, per wa
 template <int K>
 struct BandLds {
     EdgeRec rec[64];
+    unsigned long long smask[64];  // per lane row: edges of the chunk whose rows start there
+    unsigned long long emask[64];  // ... and end there
     int32_t ex0[K * 64];
     int32_t ex1[K * 64];
     int32_t ec[K * 64];
@@ -374,13 +400,50 @@ __device__ __forceinline__ Entry row_entry(const EdgeRec &q, int32_t r) {
     return en;
 }
 
+// Lane range [la, lb] of the band whose AA rows lie in [a, b].  Lane l holds
+// AA row 4*Y0 + l (documented) or 16*(Y0 + l/4) + 12 + l%4 (literal).
+__device__ __forceinline__ void lane_range(int32_t a, int32_t b, bool literal, int32_t Y0, int32_t band_lo,
+                                           int32_t band_hi, int *la, int *lb) {
+    if (!literal) {
+        *la = max(a, band_lo) - band_lo;
+        *lb = min(b, band_hi) - band_lo;
+        return;
+    }
+    const int32_t base = 16 * Y0 + 12;
+    int32_t rel = a - base;
+    int lo_l;
+    if (rel <= 0) {
+        lo_l = 0;
+    } else {
+        const int32_t q = rel >> 4, m = rel & 15;
+        lo_l = m <= 3 ? 4 * q + m : 4 * (q + 1);
+    }
+    rel = b - base;
+    int hi_l;
+    if (rel < 0) {
+        hi_l = -1;
+    } else {
+        const int32_t q = rel >> 4, m = rel & 15;
+        hi_l = m <= 3 ? 4 * q + m : 4 * q + 3;
+    }
+    *la = min(lo_l, 64);
+    *lb = min(hi_l, 63);
+}
+
 // Walks the path's edges for this wave's band and builds each lane's list of
-// retired, state-changing entries in X0/X1/C ([slot * 64 + lane]).  Returns
-// the number of entries this lane produced (may exceed cap: then only the
-// first cap are stored and the caller reruns with more room).
-__device__ __forceinline__ int build_entries(const FillArgs &A, EdgeRec *stage, int64_t e0, int64_t e1,
-                                             int32_t band_lo, int32_t band_hi, bool valid, int32_t r,
-                                             int32_t mrule, int32_t *X0, int32_t *X1, int32_t *C, int cap) {
+// retired, state-changing entries in X0/X1/C ([slot * 64 + lane]).  Chunks of
+// 64 edges whose row summary misses the band are skipped with one ballot.
+// Inside a chunk the edge->row relation is transposed: every edge marks the
+// first and last lane it covers, a 64-bit prefix OR across the wave gives each
+// lane the set of chunk edges crossing its row, and each lane folds only
+// those, in edge (path) order.  Returns the number of entries this lane
+// produced (may exceed cap: then only the first cap are stored and the caller
+// reruns with more room).
+template <int K>
+__device__ __forceinline__ int build_entries(const FillArgs &A, BandLds<K> &L, int64_t e0, int64_t e1,
+                                             int32_t band_lo, int32_t band_hi, bool literal, int32_t Y0, bool valid,
+                                             int32_t r, int32_t mrule, int32_t *X0, int32_t *X1, int32_t *C,
+                                             int cap) {
     const int lane = threadIdx.x;
     int n = 0;
     bool has_last = false;
@@ -394,22 +457,50 @@ __device__ __forceinline__ int build_entries(const FillArgs &A, EdgeRec *stage, 
         }
         ++n;
     };
-    for (int64_t base = e0; base < e1; base += 64) {
-        const int64_t i = base + lane;
-        EdgeRec q{1, 0, 0, 0, 0, 0, 0, 0};
-        bool ov = false;
-        if (i < e1) {
-            q = A.recs[i];
-            ov = q.r0 <= q.r1 && q.r0 <= band_hi && q.r1 >= band_lo;
+    if (e1 <= e0) return 0;
+    const int64_t c_first = e0 >> 6, c_last = (e1 - 1) >> 6;
+    for (int64_t cb = c_first; cb <= c_last; cb += 64) {
+        bool hit = false;
+        if (cb + lane <= c_last) {
+            const int2 sm = A.csum[cb + lane];
+            hit = sm.x <= band_hi && sm.y >= band_lo;
         }
-        stage[lane] = q;
-        uint64_t mask = __ballot(ov);
-        __syncthreads();
-        while (mask) {
-            const int j = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-            mask &= mask - 1;
-            const EdgeRec g = stage[j];
-            if (valid && r >= g.r0 && r <= g.r1) {
+        uint64_t cmask = __ballot(hit);
+        while (cmask) {
+            const int jc = __ffsll(static_cast<unsigned long long>(cmask)) - 1;
+            cmask &= cmask - 1;
+            const int64_t i = ((cb + jc) << 6) + lane;
+            EdgeRec q{1, 0, 0, 0, 0, 0, 0, 0};
+            int la = 64, lb = -1;
+            if (i >= e0 && i < e1) {
+                q = A.recs[i];
+                if (q.r0 <= q.r1) lane_range(q.r0, q.r1, literal, Y0, band_lo, band_hi, &la, &lb);
+            }
+            L.rec[lane] = q;
+            L.smask[lane] = 0ull;
+            L.emask[lane] = 0ull;
+            __syncthreads();
+            if (la <= lb) {
+                atomicOr(&L.smask[la], 1ull << lane);
+                atomicOr(&L.emask[lb], 1ull << lane);
+            }
+            __syncthreads();
+            unsigned long long ps = L.smask[lane], pe = L.emask[lane];
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned long long ts = __shfl_up(ps, o), te = __shfl_up(pe, o);
+                if (lane >= o) {
+                    ps |= ts;
+                    pe |= te;
+                }
+            }
+            unsigned long long pe_before = __shfl_up(pe, 1);
+            if (lane == 0) pe_before = 0ull;
+            unsigned long long m = valid ? (ps & ~pe_before) : 0ull;
+            while (m) {
+                const int j = __ffsll(m) - 1;
+                m &= m - 1;
+                const EdgeRec g = L.rec[j];
                 const Entry en = row_entry(g, r);
                 if (has_last && static_cast<long long>(lx1) + 1 >= en.x0 &&
                     static_cast<long long>(lx0) <= static_cast<long long>(en.x1) + 1) {
@@ -424,8 +515,8 @@ __device__ __forceinline__ int build_entries(const FillArgs &A, EdgeRec *stage, 
                     has_last = true;
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
     if (has_last) retire(lx0, lx1, lc);
     return n;
@@ -532,7 +623,7 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, const DevPage &pg, i
     const int64_t e0 = A.edge_off[p], e1 = A.edge_off[p + 1];
 
     int32_t *X0 = L.ex0, *X1 = L.ex1, *C = L.ec;
-    int n = build_entries(A, L.rec, e0, e1, band_lo, band_hi, valid, r, mrule, X0, X1, C, K);
+    int n = build_entries<K>(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, K);
     int need = n;
     for (int o = 32; o > 0; o >>= 1) need = max(need, __shfl_xor(need, o));
     if (need > K) {
@@ -555,7 +646,7 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, const DevPage &pg, i
         X0 = G;
         X1 = G + static_cast<long long>(need) * 64;
         C = G + static_cast<long long>(need) * 128;
-        n = build_entries(A, L.rec, e0, e1, band_lo, band_hi, valid, r, mrule, X0, X1, C, need);
+        n = build_entries<K>(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, need);
     }
     bool unb = false;
     const int nint = sort_and_walk(n, X0, X1, C, mrule, aa_width, &unb);
@@ -586,10 +677,12 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, const DevPage &pg, i
     unsigned long long w_cov = 0;
     int ip = 0;  // first interval of this lane not yet fully left of the chunk
     uint32_t *bm_row = L.bm + lane * kBmPitch;
+    const uint32_t src4 = src * 0x01010101u;
     for (int32_t cx = xs; any_px && cx <= px_hi; cx += kChunkPx) {
         const int32_t A0 = 4 * cx, A1 = A0 + 4 * kChunkPx;
-        for (int w = 0; w < 32; ++w) bm_row[w] = 0u;
         while (ip < nint && X1[ip * 64 + lane] <= A0) ++ip;
+        if (__ballot(ip < nint && X0[ip * 64 + lane] < A1) == 0) continue;  // nothing covered here
+        for (int w = 0; w < 32; ++w) bm_row[w] = 0u;
         for (int k = ip; k < nint; ++k) {
             const int32_t s = X0[k * 64 + lane];
             if (s >= A1) break;
@@ -622,6 +715,17 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, const DevPage &pg, i
             }
             if (cov == 0) continue;
             uint32_t *dst = reinterpret_cast<uint32_t *>(pg.canvas + static_cast<long long>(yy) * pg.stride + xq);
+            if (cov == 0x10101010u && a_in == 255u && xq >= px_lo && xq + 3 <= px_hi) {
+                // full coverage of an opaque source: both composites give src
+                *dst = src4;
+                w_cov += 4;
+                if (A.cov_hash) {
+                    const long long pix = static_cast<long long>(yy) * pg.width + xq;
+                    hsum += cov_term(pix, 255u) + cov_term(pix + 1, 255u) + cov_term(pix + 2, 255u) +
+                            cov_term(pix + 3, 255u);
+                }
+                continue;
+            }
             const uint32_t old = *dst;
             uint32_t out = old;
 #pragma unroll
@@ -731,6 +835,8 @@ struct rr_hip_ctx {
     size_t hdrs_cap = 0;
     int2 *rows = nullptr;
     size_t rows_cap = 0;
+    int2 *csum = nullptr;
+    size_t csum_cap = 0;
     DevPage *pages = nullptr;
     size_t pages_cap = 0;
     int2 *items = nullptr;
@@ -879,6 +985,8 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     if (!rc) rc = ensure(c, &c->hdrs, &c->hdrs_cap, static_cast<size_t>(b.n_paths));
     if (!rc) rc = ensure(c, &c->rows, &c->rows_cap, static_cast<size_t>(b.n_paths));
     if (!rc) rc = ensure(c, &c->pages, &c->pages_cap, static_cast<size_t>(b.n_pages));
+    const int64_t n_chunks = (b.n_edges + 63) / 64;
+    if (!rc) rc = ensure(c, &c->csum, &c->csum_cap, static_cast<size_t>(n_chunks > 0 ? n_chunks : 1));
     if (rc) return rc;
     if (b.launch == RR_LAUNCH_BATCHED) {
         // Work items = (page, first row of a 16-row band) for every band of
@@ -905,6 +1013,7 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     A.recs = c->recs;
     A.hdrs = c->hdrs;
     A.rows = c->rows;
+    A.csum = c->csum;
     A.edge_off = b.edge_off;
     A.attrs = b.attrs;
     A.pages = c->pages;
@@ -926,6 +1035,11 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     hipLaunchKernelGGL(k_edge_setup, dim3(static_cast<unsigned>(b.n_paths)), dim3(256), 0, c->stream, b.edges,
                        b.edge_off, c->pages, b.n_pages, c->recs, c->hdrs, c->rows, b.mode);
     RR_HIP_TRY(c, hipGetLastError());
+    if (n_chunks > 0) {
+        hipLaunchKernelGGL(k_chunk_summary, dim3(static_cast<unsigned>((n_chunks + 3) / 4)), dim3(256), 0, c->stream,
+                           c->recs, b.n_edges, c->csum, n_chunks);
+        RR_HIP_TRY(c, hipGetLastError());
+    }
 
     if (b.launch == RR_LAUNCH_PLAIN) {
         int32_t max_bands = 1;
@@ -1015,7 +1129,7 @@ int32_t rr_hip_destroy(rr_hip_ctx *c) {
     if (!c) return RR_E_INVARIANT;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void *bufs[] = {c->recs, c->hdrs, c->rows, c->pages, c->items, c->slots, c->spill, c->spill_top, c->next_item,
+    void *bufs[] = {c->recs, c->hdrs, c->rows, c->csum, c->pages, c->items, c->slots, c->spill, c->spill_top, c->next_item,
                     c->err_flag};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
