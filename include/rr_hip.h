@@ -149,6 +149,15 @@ int32_t rr_hip_wait_page(rr_hip_ctx *ctx, uint64_t fence);
  * alone, each bracketed by its own event pair; launches counts the fill-kernel
  * launches (the edge-setup pass excluded). */
 int32_t rr_hip_set_profiling(rr_hip_ctx *ctx, int32_t enable);
+/* Per-fill timing trace (no counterpart in the reference; it feeds the
+ * autotuner and the learned dispatcher).  While set, every fill of one path
+ * over one 16-row band appends a record of four int64 {path index,
+ * (page << 32) | first output row, start, end} (device wall-clock ticks,
+ * 100 MHz) to the caller's device buffer, up to `capacity` records; capacity
+ * 0 turns tracing off.  rr_hip_trace_count reports the records written since
+ * the last set (it may exceed capacity: the rest were dropped). */
+int32_t rr_hip_set_trace(rr_hip_ctx *ctx, void *records, int64_t capacity);
+int32_t rr_hip_trace_count(rr_hip_ctx *ctx, int64_t *count);
 int32_t rr_hip_last_page_stats(rr_hip_ctx *ctx, double *span_ms, double *fill_ms, int64_t *launches);
 
 /* ---- host path ingestion (XPath::new, crates/raster/src/xpath.rs:162-260,

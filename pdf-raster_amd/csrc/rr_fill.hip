@@ -104,6 +104,10 @@ struct FillArgs {
     int32_t *err;                   // set to 1 when the pool runs out
     long long spill_cap;
     int32_t mode, flavour;
+    // optional timing trace: one record per (path, band) fill
+    longlong4 *trace;  // {path, (page << 32) | first row, t_start, t_end} wall-clock ticks
+    unsigned long long *trace_top;
+    long long trace_cap;
 };
 
 // AA_GAMMA (crates/raster/src/fill/mod.rs:46-48)
@@ -767,6 +771,24 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, const DevPage &pg, i
     }
 }
 
+// fill_band plus, when tracing, one wall-clock record of its duration.
+template <int K>
+__device__ __forceinline__ void fill_band_traced(const FillArgs &A, BandLds<K> &L, const DevPage &pg, int32_t page,
+                                                 int64_t p, int32_t Y0, int64_t slot) {
+    if (!A.trace) {
+        fill_band<K>(A, L, pg, p, Y0, slot);
+        return;
+    }
+    const long long t0 = static_cast<long long>(wall_clock64());
+    fill_band<K>(A, L, pg, p, Y0, slot);
+    const long long t1 = static_cast<long long>(wall_clock64());
+    if (threadIdx.x == 0) {
+        const unsigned long long k = atomicAdd(A.trace_top, 1ull);
+        if (static_cast<long long>(k) < A.trace_cap)
+            A.trace[k] = make_longlong4(p, (static_cast<long long>(page) << 32) | static_cast<uint32_t>(Y0), t0, t1);
+    }
+}
+
 template <int K>
 __device__ __forceinline__ void init_lds(BandLds<K> &L) {
     if (threadIdx.x < 17) L.gamma[threadIdx.x] = kGamma[threadIdx.x];
@@ -777,7 +799,8 @@ __device__ __forceinline__ void init_lds(BandLds<K> &L) {
 __global__ void __launch_bounds__(64) k_fill_plain(FillArgs A, int32_t page, int64_t p, int32_t row0) {
     __shared__ BandLds<kLdsK> L;
     init_lds(L);
-    fill_band<kLdsK>(A, L, A.pages[page], p, row0 + kBandRows * static_cast<int32_t>(blockIdx.x), blockIdx.x);
+    fill_band_traced<kLdsK>(A, L, A.pages[page], page, p, row0 + kBandRows * static_cast<int32_t>(blockIdx.x),
+                            blockIdx.x);
 }
 
 // BATCHED: persistent waves pull (page, first row) items and composite every
@@ -807,7 +830,7 @@ __global__ void __launch_bounds__(64) k_fill_batched(FillArgs A, const int2 *__r
             while (mask) {
                 const int jj = __ffsll(static_cast<unsigned long long>(mask)) - 1;
                 mask &= mask - 1;
-                fill_band<kLdsK>(A, L, pg, base + jj, Y0, blockIdx.x);
+                fill_band_traced<kLdsK>(A, L, pg, item.x, base + jj, Y0, blockIdx.x);
             }
         }
     }
@@ -864,6 +887,10 @@ struct rr_hip_ctx {
     int64_t fill_launches = 0;
     std::vector<DevPage> host_pages;
     std::vector<int2> host_items;
+    // optional fill timing trace (caller-owned device buffer)
+    longlong4 *trace = nullptr;
+    long long trace_cap = 0;
+    unsigned long long *trace_top = nullptr;
 };
 
 namespace {
@@ -1026,6 +1053,9 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     A.mode = b.mode;
     A.flavour = b.flavour;
     A.slot_ints = kSlotInts;
+    A.trace = c->trace;
+    A.trace_cap = c->trace_cap;
+    A.trace_top = c->trace_top;
 
     if (!c->page_timed) {
         RR_HIP_TRY(c, hipEventRecord(c->t0, c->stream));
@@ -1111,6 +1141,7 @@ int32_t rr_hip_create(int32_t device, rr_hip_ctx **out) {
     if (rc == RR_OK && hipMalloc(&c->spill_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->next_item, sizeof(unsigned int)) != hipSuccess) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->err_flag, sizeof(int32_t)) != hipSuccess) rc = RR_E_DEVICE;
+    if (rc == RR_OK && hipMalloc(&c->trace_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
     for (auto &f : c->fences)
         if (rc == RR_OK && hipEventCreateWithFlags(&f.ev, hipEventDisableTiming) != hipSuccess) rc = RR_E_DEVICE;
     const long long pool_ints = 64ll << 20;  // 256 MiB shared pool for rows beyond a wave's slot
@@ -1129,8 +1160,8 @@ int32_t rr_hip_destroy(rr_hip_ctx *c) {
     if (!c) return RR_E_INVARIANT;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void *bufs[] = {c->recs, c->hdrs, c->rows, c->csum, c->pages, c->items, c->slots, c->spill, c->spill_top, c->next_item,
-                    c->err_flag};
+    void *bufs[] = {c->recs,      c->hdrs,      c->rows,     c->csum,     c->pages,    c->items,
+                    c->slots,     c->spill,     c->spill_top, c->next_item, c->err_flag, c->trace_top};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->t0) (void)hipEventDestroy(c->t0);
@@ -1257,6 +1288,25 @@ int32_t rr_hip_wait_page(rr_hip_ctx *c, uint64_t fence) {
     }
     c->last_fill_ms = fill;
     if (flag) return fail(c, RR_E_CAPACITY, "overflow-entry pool exhausted: page output is incomplete");
+    return RR_OK;
+}
+
+int32_t rr_hip_set_trace(rr_hip_ctx *c, void *records, int64_t capacity) {
+    if (!c || capacity < 0 || (capacity > 0 && !records)) return RR_E_INVARIANT;
+    if (c->page_open) return fail(c, RR_E_INVARIANT, "set_trace inside an open page");
+    if (reinterpret_cast<uintptr_t>(records) & 31u) return fail(c, RR_E_INVARIANT, "trace buffer must be 32-byte aligned");
+    c->trace = capacity ? static_cast<longlong4 *>(records) : nullptr;
+    c->trace_cap = capacity;
+    RR_HIP_TRY(c, hipMemsetAsync(c->trace_top, 0, sizeof(unsigned long long), c->stream));
+    return RR_OK;
+}
+
+int32_t rr_hip_trace_count(rr_hip_ctx *c, int64_t *count) {
+    if (!c || !count) return RR_E_INVARIANT;
+    unsigned long long n = 0;
+    RR_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    RR_HIP_TRY(c, hipMemcpy(&n, c->trace_top, sizeof(n), hipMemcpyDeviceToHost));
+    *count = static_cast<int64_t>(n);
     return RR_OK;
 }
 

@@ -82,6 +82,8 @@ def lib():
             "rr_hip_submit_page": (i32, [P, ctypes.POINTER(u64)]),
             "rr_hip_wait_page": (i32, [P, u64]),
             "rr_hip_set_profiling": (i32, [P, i32]),
+            "rr_hip_set_trace": (i32, [P, P, i64]),
+            "rr_hip_trace_count": (i32, [P, ctypes.POINTER(i64)]),
             "rr_hip_last_page_stats": (i32, [P, ctypes.POINTER(ctypes.c_double),
                                              ctypes.POINTER(ctypes.c_double),
                                              ctypes.POINTER(i64)]),
@@ -280,6 +282,14 @@ class Backend:
 
     def wait_page(self, fence):
         self._check(self.L.rr_hip_wait_page(self.h, fence), "wait_page")
+
+    def set_trace(self, buf, capacity):
+        self._check(self.L.rr_hip_set_trace(self.h, buf, capacity), "set_trace")
+
+    def trace_count(self):
+        n = ctypes.c_int64(0)
+        self._check(self.L.rr_hip_trace_count(self.h, ctypes.byref(n)), "trace_count")
+        return n.value
 
     def set_profiling(self, on):
         self._check(self.L.rr_hip_set_profiling(self.h, int(on)), "set_profiling")
