@@ -470,16 +470,21 @@ __device__ __forceinline__ int build_entries(const FillArgs &A, BandLds<K> &L, i
             hit = sm.x <= band_hi && sm.y >= band_lo;
         }
         uint64_t cmask = __ballot(hit);
-        while (cmask) {
-            const int jc = __ffsll(static_cast<unsigned long long>(cmask)) - 1;
-            cmask &= cmask - 1;
+        // records of the next overlapping chunk are loaded one chunk ahead
+        auto load_rec = [&](int jc) {
             const int64_t i = ((cb + jc) << 6) + lane;
             EdgeRec q{1, 0, 0, 0, 0, 0, 0, 0};
+            if (i >= e0 && i < e1) q = A.recs[i];
+            return q;
+        };
+        EdgeRec qn{1, 0, 0, 0, 0, 0, 0, 0};
+        if (cmask) qn = load_rec(__ffsll(static_cast<unsigned long long>(cmask)) - 1);
+        while (cmask) {
+            cmask &= cmask - 1;
+            const EdgeRec q = qn;
+            if (cmask) qn = load_rec(__ffsll(static_cast<unsigned long long>(cmask)) - 1);
             int la = 64, lb = -1;
-            if (i >= e0 && i < e1) {
-                q = A.recs[i];
-                if (q.r0 <= q.r1) lane_range(q.r0, q.r1, literal, Y0, band_lo, band_hi, &la, &lb);
-            }
+            if (q.r0 <= q.r1) lane_range(q.r0, q.r1, literal, Y0, band_lo, band_hi, &la, &lb);
             L.rec[lane] = q;
             L.smask[lane] = 0ull;
             L.emask[lane] = 0ull;
@@ -608,12 +613,14 @@ template <int K>
 __device__ void fill_band(const FillArgs &A, BandLds<K> &L, const DevPage &pg, int64_t p, int32_t Y0,
                           int64_t slot) {
     const int lane = threadIdx.x;
+    // independent loads first, so their latencies overlap
     const PathHdr h = A.hdrs[p];
-    if (h.flags & (HDR_EMPTY | HDR_OUTSIDE)) return;
     const int2 drawn = A.rows[p];
+    const rr_path_attr at = A.attrs[p];
+    const int64_t e0 = A.edge_off[p], e1 = A.edge_off[p + 1];
+    if (h.flags & (HDR_EMPTY | HDR_OUTSIDE)) return;
     const int32_t ylo = drawn.x, yhi = drawn.y;
     if (Y0 > yhi || Y0 + kBandRows - 1 < ylo) return;
-    const rr_path_attr at = A.attrs[p];
     const bool literal = A.mode == RR_MODE_LITERAL;
 
     const int32_t d = lane >> 2, j = lane & 3;
@@ -624,7 +631,6 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, const DevPage &pg, i
     const bool valid = y >= ylo && y <= yhi && r >= h.y_min && r <= h.y_max;
     const int32_t mrule = at.eo ? 1 : -1;
     const int32_t aa_width = pg.width * 4;
-    const int64_t e0 = A.edge_off[p], e1 = A.edge_off[p + 1];
 
     int32_t *X0 = L.ex0, *X1 = L.ex1, *C = L.ec;
     int n = build_entries<K>(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, K);
@@ -704,22 +710,40 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, const DevPage &pg, i
         }
         __syncthreads();
         const int32_t xq = cx + 4 * lane;  // this lane's 4 pixels
+        const int sh = (lane & 1) * 16;
+        const bool lane_in = xq <= px_hi;
+        const bool full_ok = a_in == 255u && xq >= px_lo && xq + 3 <= px_hi;
+        uint8_t *row0 = pg.canvas + static_cast<long long>(Y0) * pg.stride + xq;
+        // Pass 1: coverage of the 16 rows from the bitmap, and the canvas
+        // loads of every row that needs one, issued back to back so their
+        // latencies overlap.
+        uint32_t cov[kBandRows], old[kBandRows];
+#pragma unroll
         for (int dd = 0; dd < kBandRows; ++dd) {
             const int32_t yy = Y0 + dd;
-            if (yy < ylo || yy > yhi) continue;
-            if (xq > px_hi) continue;
-            const int sh = (lane & 1) * 16;
-            uint32_t cov = 0;
+            uint32_t cv = 0;
+            if (lane_in && yy >= ylo && yy <= yhi) {
 #pragma unroll
-            for (int rr = 0; rr < 4; ++rr) {
-                const uint32_t f = (L.bm[(4 * dd + rr) * kBmPitch + (lane >> 1)] >> sh) & 0xffffu;
-                uint32_t t = f - ((f >> 1) & 0x5555u);
-                t = (t & 0x3333u) + ((t >> 2) & 0x3333u);
-                cov += (t & 0xfu) | ((t & 0xf0u) << 4) | ((t & 0xf00u) << 8) | ((t & 0xf000u) << 12);
+                for (int rr = 0; rr < 4; ++rr) {
+                    const uint32_t f = (L.bm[(4 * dd + rr) * kBmPitch + (lane >> 1)] >> sh) & 0xffffu;
+                    uint32_t t = f - ((f >> 1) & 0x5555u);
+                    t = (t & 0x3333u) + ((t >> 2) & 0x3333u);
+                    cv += (t & 0xfu) | ((t & 0xf0u) << 4) | ((t & 0xf00u) << 8) | ((t & 0xf000u) << 12);
+                }
             }
-            if (cov == 0) continue;
-            uint32_t *dst = reinterpret_cast<uint32_t *>(pg.canvas + static_cast<long long>(yy) * pg.stride + xq);
-            if (cov == 0x10101010u && a_in == 255u && xq >= px_lo && xq + 3 <= px_hi) {
+            cov[dd] = cv;
+            old[dd] = 0;
+            if (cv != 0 && !(cv == 0x10101010u && full_ok))
+                old[dd] = *reinterpret_cast<const uint32_t *>(row0 + static_cast<long long>(dd) * pg.stride);
+        }
+        // Pass 2: composite and store.
+#pragma unroll
+        for (int dd = 0; dd < kBandRows; ++dd) {
+            const uint32_t cv = cov[dd];
+            if (cv == 0) continue;
+            const int32_t yy = Y0 + dd;
+            uint32_t *dst = reinterpret_cast<uint32_t *>(row0 + static_cast<long long>(dd) * pg.stride);
+            if (cv == 0x10101010u && full_ok) {
                 // full coverage of an opaque source: both composites give src
                 *dst = src4;
                 w_cov += 4;
@@ -730,15 +754,15 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, const DevPage &pg, i
                 }
                 continue;
             }
-            const uint32_t old = *dst;
-            uint32_t out = old;
+            const uint32_t ov = old[dd];
+            uint32_t out = ov;
 #pragma unroll
             for (int px = 0; px < 4; ++px) {
-                const uint32_t cnt = (cov >> (8 * px)) & 0xffu;
+                const uint32_t cnt = (cv >> (8 * px)) & 0xffu;
                 const int32_t x = xq + px;
                 if (cnt == 0 || x < px_lo || x > px_hi) continue;
                 const uint32_t shape = L.gamma[cnt];
-                const uint32_t dv = (old >> (8 * px)) & 0xffu;
+                const uint32_t dv = (ov >> (8 * px)) & 0xffu;
                 uint32_t res;
                 if (A.flavour == RR_FLAVOUR_RGB8_FAST) {
                     const uint32_t a = (a_in * shape + 255u) >> 8;
@@ -751,7 +775,7 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, const DevPage &pg, i
                 ++w_cov;
                 if (A.cov_hash) hsum += cov_term(static_cast<long long>(yy) * pg.width + x, shape);
             }
-            if (out != old) *dst = out;
+            if (out != ov) *dst = out;
         }
         __syncthreads();
     }
