@@ -90,6 +90,7 @@ typedef struct {
     double clip[4]; /* x_min, y_min, x_max, y_max; page clip is {0,0,W-0.001,H-0.001} */
     uint64_t *cov_hash;
     uint64_t *cov_count;
+    int32_t variant; /* band-fill shape, 0 <= variant < rr_hip_variant_count() */
 } rr_fill_batch;
 
 /* One page of a multi-page batch: its canvas and the range of the batch's
@@ -118,7 +119,15 @@ typedef struct {
     int32_t mode, flavour, launch;
     uint64_t *cov_hash;
     uint64_t *cov_count;
+    int32_t variant;
 } rr_fill_pages;
+
+/* Band-fill shapes the library is built with (the autotuner's search space;
+ * no counterpart in the reference).  Every variant gives identical pixels;
+ * they differ in the LDS entry slots per AA row and the composite chunk
+ * width, hence in occupancy.  variant_count() is the number of variants. */
+int32_t rr_hip_variant_count(void);
+int32_t rr_hip_variant_info(int32_t variant, int32_t *lds_entries, int32_t *chunk_px);
 
 /* ---- context (GpuCtx::init, crates/gpu/src/lib.rs:239; CudaBackend, backend/cuda/mod.rs:27) ---- */
 typedef struct rr_hip_ctx rr_hip_ctx;

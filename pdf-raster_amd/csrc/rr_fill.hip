@@ -346,20 +346,24 @@ __global__ void __launch_bounds__(256) k_chunk_summary(const EdgeRec *__restrict
 // kernel 2: scanline fill + composite of one band of 16 output rows
 // ---------------------------------------------------------------------------
 constexpr int kBandRows = 16;  // output rows per wave
-constexpr int kChunkPx = 256;  // pixels per composite sweep step
-constexpr int kBmPitch = 33;   // dwords per bitmap row (32 + 1 pad against bank conflicts)
-constexpr int kLdsK = 32;      // LDS entry slots per AA row
+// Tunable shape of a band fill (the autotuner picks among the instantiated
+// variants): K = LDS entry slots per AA row (rows with more state-changing
+// entries go to the wave's global slot), CH = pixels per composite sweep step
+// (the LDS coverage bitmap holds 64 AA rows x 4*CH bits, padded by one dword
+// per row against bank conflicts).
 constexpr int kSlotK = 512;    // global overflow entry slots per AA row, per wave
 
-template <int K>
+template <int K, int CH>
 struct BandLds {
+    static constexpr int kWords = CH / 8;  // bitmap dwords per AA row
+    static constexpr int kPitch = kWords + 1;
     EdgeRec rec[64];
     unsigned long long smask[64];  // per lane row: edges of the chunk whose rows start there
     unsigned long long emask[64];  // ... and end there
     int32_t ex0[K * 64];
     int32_t ex1[K * 64];
     int32_t ec[K * 64];
-    uint32_t bm[64 * kBmPitch];
+    uint32_t bm[64 * kPitch];
     uint8_t gamma[32];
 };
 
@@ -443,8 +447,8 @@ __device__ __forceinline__ void lane_range(int32_t a, int32_t b, bool literal, i
 // those, in edge (path) order.  Returns the number of entries this lane
 // produced (may exceed cap: then only the first cap are stored and the caller
 // reruns with more room).
-template <int K>
-__device__ __forceinline__ int build_entries(const FillArgs &A, BandLds<K> &L, int64_t e0, int64_t e1,
+template <int K, int CH>
+__device__ __forceinline__ int build_entries(const FillArgs &A, BandLds<K, CH> &L, int64_t e0, int64_t e1,
                                              int32_t band_lo, int32_t band_hi, bool literal, int32_t Y0, bool valid,
                                              int32_t r, int32_t mrule, int32_t *X0, int32_t *X1, int32_t *C,
                                              int cap) {
@@ -609,8 +613,8 @@ __device__ __forceinline__ int sort_and_walk(int n, int32_t *X0, int32_t *X1, in
 // Rasterises path p over output rows [Y0, Y0+15] of page `pg` and composites
 // into its canvas.  Called by one full wave64 (blockDim.x == 64); `slot` is
 // this wave's private overflow slot.
-template <int K>
-__device__ void fill_band(const FillArgs &A, BandLds<K> &L, const DevPage &pg, int64_t p, int32_t Y0,
+template <int K, int CH>
+__device__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &pg, int64_t p, int32_t Y0,
                           int64_t slot) {
     const int lane = threadIdx.x;
     // independent loads first, so their latencies overlap
@@ -633,7 +637,7 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, const DevPage &pg, i
     const int32_t aa_width = pg.width * 4;
 
     int32_t *X0 = L.ex0, *X1 = L.ex1, *C = L.ec;
-    int n = build_entries<K>(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, K);
+    int n = build_entries<K, CH>(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, K);
     int need = n;
     for (int o = 32; o > 0; o >>= 1) need = max(need, __shfl_xor(need, o));
     if (need > K) {
@@ -656,7 +660,7 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, const DevPage &pg, i
         X0 = G;
         X1 = G + static_cast<long long>(need) * 64;
         C = G + static_cast<long long>(need) * 128;
-        n = build_entries<K>(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, need);
+        n = build_entries<K, CH>(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, need);
     }
     bool unb = false;
     const int nint = sort_and_walk(n, X0, X1, C, mrule, aa_width, &unb);
@@ -686,13 +690,19 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, const DevPage &pg, i
     uint64_t hsum = 0;
     unsigned long long w_cov = 0;
     int ip = 0;  // first interval of this lane not yet fully left of the chunk
-    uint32_t *bm_row = L.bm + lane * kBmPitch;
+    constexpr int kWords = BandLds<K, CH>::kWords, kPitch = BandLds<K, CH>::kPitch;
+    constexpr int kLanesPerRow = CH / 4;            // one dword (4 pixels) per lane
+    constexpr int kRowsPerIter = 64 / kLanesPerRow;  // output rows composited per pass
+    constexpr int kIters = kBandRows / kRowsPerIter;
+    constexpr int kGroup = kIters < 4 ? kIters : 4;
+    uint32_t *bm_row = L.bm + lane * kPitch;
     const uint32_t src4 = src * 0x01010101u;
-    for (int32_t cx = xs; any_px && cx <= px_hi; cx += kChunkPx) {
-        const int32_t A0 = 4 * cx, A1 = A0 + 4 * kChunkPx;
+    const int q = lane % kLanesPerRow, sub = lane / kLanesPerRow;
+    for (int32_t cx = xs; any_px && cx <= px_hi; cx += CH) {
+        const int32_t A0 = 4 * cx, A1 = A0 + 4 * CH;
         while (ip < nint && X1[ip * 64 + lane] <= A0) ++ip;
         if (__ballot(ip < nint && X0[ip * 64 + lane] < A1) == 0) continue;  // nothing covered here
-        for (int w = 0; w < 32; ++w) bm_row[w] = 0u;
+        for (int w = 0; w < kWords; ++w) bm_row[w] = 0u;
         for (int k = ip; k < nint; ++k) {
             const int32_t s = X0[k * 64 + lane];
             if (s >= A1) break;
@@ -709,73 +719,72 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, const DevPage &pg, i
             }
         }
         __syncthreads();
-        const int32_t xq = cx + 4 * lane;  // this lane's 4 pixels
-        const int sh = (lane & 1) * 16;
+        const int32_t xq = cx + 4 * q;  // this lane's 4 pixels
+        const int sh = (q & 1) * 16;
         const bool lane_in = xq <= px_hi;
         const bool full_ok = a_in == 255u && xq >= px_lo && xq + 3 <= px_hi;
-        uint8_t *row0 = pg.canvas + static_cast<long long>(Y0) * pg.stride + xq;
-        // Pass 1: coverage of the 16 rows from the bitmap, and the canvas
-        // loads of every row that needs one, issued back to back so their
-        // latencies overlap.
-        uint32_t cov[kBandRows], old[kBandRows];
+        // Rows in groups: coverage from the bitmap, then the canvas loads of
+        // the group issued back to back, then composite + store.
+        for (int i0 = 0; i0 < kIters; i0 += kGroup) {
+            uint32_t cov[kGroup], old[kGroup];
 #pragma unroll
-        for (int dd = 0; dd < kBandRows; ++dd) {
-            const int32_t yy = Y0 + dd;
-            uint32_t cv = 0;
-            if (lane_in && yy >= ylo && yy <= yhi) {
+            for (int k = 0; k < kGroup; ++k) {
+                const int dd = (i0 + k) * kRowsPerIter + sub;
+                const int32_t yy = Y0 + dd;
+                uint32_t cv = 0;
+                if (lane_in && yy >= ylo && yy <= yhi) {
 #pragma unroll
-                for (int rr = 0; rr < 4; ++rr) {
-                    const uint32_t f = (L.bm[(4 * dd + rr) * kBmPitch + (lane >> 1)] >> sh) & 0xffffu;
-                    uint32_t t = f - ((f >> 1) & 0x5555u);
-                    t = (t & 0x3333u) + ((t >> 2) & 0x3333u);
-                    cv += (t & 0xfu) | ((t & 0xf0u) << 4) | ((t & 0xf00u) << 8) | ((t & 0xf000u) << 12);
+                    for (int rr = 0; rr < 4; ++rr) {
+                        const uint32_t f = (L.bm[(4 * dd + rr) * kPitch + (q >> 1)] >> sh) & 0xffffu;
+                        uint32_t t = f - ((f >> 1) & 0x5555u);
+                        t = (t & 0x3333u) + ((t >> 2) & 0x3333u);
+                        cv += (t & 0xfu) | ((t & 0xf0u) << 4) | ((t & 0xf00u) << 8) | ((t & 0xf000u) << 12);
+                    }
                 }
+                cov[k] = cv;
+                old[k] = 0;
+                if (cv != 0 && !(cv == 0x10101010u && full_ok))
+                    old[k] = *reinterpret_cast<const uint32_t *>(pg.canvas + static_cast<long long>(yy) * pg.stride + xq);
             }
-            cov[dd] = cv;
-            old[dd] = 0;
-            if (cv != 0 && !(cv == 0x10101010u && full_ok))
-                old[dd] = *reinterpret_cast<const uint32_t *>(row0 + static_cast<long long>(dd) * pg.stride);
-        }
-        // Pass 2: composite and store.
 #pragma unroll
-        for (int dd = 0; dd < kBandRows; ++dd) {
-            const uint32_t cv = cov[dd];
-            if (cv == 0) continue;
-            const int32_t yy = Y0 + dd;
-            uint32_t *dst = reinterpret_cast<uint32_t *>(row0 + static_cast<long long>(dd) * pg.stride);
-            if (cv == 0x10101010u && full_ok) {
-                // full coverage of an opaque source: both composites give src
-                *dst = src4;
-                w_cov += 4;
-                if (A.cov_hash) {
-                    const long long pix = static_cast<long long>(yy) * pg.width + xq;
-                    hsum += cov_term(pix, 255u) + cov_term(pix + 1, 255u) + cov_term(pix + 2, 255u) +
-                            cov_term(pix + 3, 255u);
-                }
-                continue;
-            }
-            const uint32_t ov = old[dd];
-            uint32_t out = ov;
-#pragma unroll
-            for (int px = 0; px < 4; ++px) {
-                const uint32_t cnt = (cv >> (8 * px)) & 0xffu;
-                const int32_t x = xq + px;
-                if (cnt == 0 || x < px_lo || x > px_hi) continue;
-                const uint32_t shape = L.gamma[cnt];
-                const uint32_t dv = (ov >> (8 * px)) & 0xffu;
-                uint32_t res;
-                if (A.flavour == RR_FLAVOUR_RGB8_FAST) {
-                    const uint32_t a = (a_in * shape + 255u) >> 8;
-                    res = ((255u - a) * dv + a * src + 255u) >> 8;
+            for (int k = 0; k < kGroup; ++k) {
+                const uint32_t cv = cov[k];
+                if (cv == 0) continue;
+                const int32_t yy = Y0 + (i0 + k) * kRowsPerIter + sub;
+                uint32_t *dst = reinterpret_cast<uint32_t *>(pg.canvas + static_cast<long long>(yy) * pg.stride + xq);
+                uint32_t out;
+                if (cv == 0x10101010u && full_ok) {
+                    out = src4;  // full coverage of an opaque source: both composites give src
                 } else {
-                    const uint32_t a = div255(a_in * shape);
-                    res = div255((255u - a) * dv + a * src);
+                    out = old[k];
+                    for (int px = 0; px < 4; ++px) {
+                        const uint32_t cnt = (cv >> (8 * px)) & 0xffu;
+                        const int32_t x = xq + px;
+                        if (cnt == 0 || x < px_lo || x > px_hi) continue;
+                        const uint32_t shape = L.gamma[cnt];
+                        const uint32_t dv = (out >> (8 * px)) & 0xffu;
+                        uint32_t res;
+                        if (A.flavour == RR_FLAVOUR_RGB8_FAST) {
+                            const uint32_t a = (a_in * shape + 255u) >> 8;
+                            res = ((255u - a) * dv + a * src + 255u) >> 8;
+                        } else {
+                            const uint32_t a = div255(a_in * shape);
+                            res = div255((255u - a) * dv + a * src);
+                        }
+                        out = (out & ~(0xffu << (8 * px))) | (res << (8 * px));
+                    }
                 }
-                out = (out & ~(0xffu << (8 * px))) | (res << (8 * px));
-                ++w_cov;
-                if (A.cov_hash) hsum += cov_term(static_cast<long long>(yy) * pg.width + x, shape);
+                if (out != old[k] || cv == 0x10101010u) *dst = out;
+                if (A.cov_count || A.cov_hash) {
+                    for (int px = 0; px < 4; ++px) {
+                        const uint32_t cnt = (cv >> (8 * px)) & 0xffu;
+                        const int32_t x = xq + px;
+                        if (cnt == 0 || x < px_lo || x > px_hi) continue;
+                        ++w_cov;
+                        if (A.cov_hash) hsum += cov_term(static_cast<long long>(yy) * pg.width + x, L.gamma[cnt]);
+                    }
+                }
             }
-            if (out != ov) *dst = out;
         }
         __syncthreads();
     }
@@ -796,15 +805,15 @@ __device__ void fill_band(const FillArgs &A, BandLds<K> &L, const DevPage &pg, i
 }
 
 // fill_band plus, when tracing, one wall-clock record of its duration.
-template <int K>
-__device__ __forceinline__ void fill_band_traced(const FillArgs &A, BandLds<K> &L, const DevPage &pg, int32_t page,
+template <int K, int CH>
+__device__ __forceinline__ void fill_band_traced(const FillArgs &A, BandLds<K, CH> &L, const DevPage &pg, int32_t page,
                                                  int64_t p, int32_t Y0, int64_t slot) {
     if (!A.trace) {
-        fill_band<K>(A, L, pg, p, Y0, slot);
+        fill_band<K, CH>(A, L, pg, p, Y0, slot);
         return;
     }
     const long long t0 = static_cast<long long>(wall_clock64());
-    fill_band<K>(A, L, pg, p, Y0, slot);
+    fill_band<K, CH>(A, L, pg, p, Y0, slot);
     const long long t1 = static_cast<long long>(wall_clock64());
     if (threadIdx.x == 0) {
         const unsigned long long k = atomicAdd(A.trace_top, 1ull);
@@ -813,25 +822,27 @@ __device__ __forceinline__ void fill_band_traced(const FillArgs &A, BandLds<K> &
     }
 }
 
-template <int K>
-__device__ __forceinline__ void init_lds(BandLds<K> &L) {
+template <int K, int CH>
+__device__ __forceinline__ void init_lds(BandLds<K, CH> &L) {
     if (threadIdx.x < 17) L.gamma[threadIdx.x] = kGamma[threadIdx.x];
     __syncthreads();
 }
 
 // PLAIN: one launch per path, one wave per band of 16 output rows.
+template <int K, int CH>
 __global__ void __launch_bounds__(64) k_fill_plain(FillArgs A, int32_t page, int64_t p, int32_t row0) {
-    __shared__ BandLds<kLdsK> L;
+    __shared__ BandLds<K, CH> L;
     init_lds(L);
-    fill_band_traced<kLdsK>(A, L, A.pages[page], page, p, row0 + kBandRows * static_cast<int32_t>(blockIdx.x),
+    fill_band_traced<K, CH>(A, L, A.pages[page], page, p, row0 + kBandRows * static_cast<int32_t>(blockIdx.x),
                             blockIdx.x);
 }
 
 // BATCHED: persistent waves pull (page, first row) items and composite every
 // overlapping path of the page, in path order, into that band.
+template <int K, int CH>
 __global__ void __launch_bounds__(64) k_fill_batched(FillArgs A, const int2 *__restrict__ items, int32_t n_items,
                                                      unsigned int *__restrict__ next_item) {
-    __shared__ BandLds<kLdsK> L;
+    __shared__ BandLds<K, CH> L;
     init_lds(L);
     const int lane = threadIdx.x;
     for (;;) {
@@ -854,7 +865,7 @@ __global__ void __launch_bounds__(64) k_fill_batched(FillArgs A, const int2 *__r
             while (mask) {
                 const int jj = __ffsll(static_cast<unsigned long long>(mask)) - 1;
                 mask &= mask - 1;
-                fill_band_traced<kLdsK>(A, L, pg, item.x, base + jj, Y0, blockIdx.x);
+                fill_band_traced<K, CH>(A, L, pg, item.x, base + jj, Y0, blockIdx.x);
             }
         }
     }
@@ -863,6 +874,23 @@ __global__ void __launch_bounds__(64) k_fill_batched(FillArgs A, const int2 *__r
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+
+// Instantiated band-fill shapes (the autotuner's search space).
+struct Variant {
+    int k, chunk;
+    const void *plain;
+    const void *batched;
+};
+const Variant kVariants[] = {
+    {32, 256, reinterpret_cast<const void *>(&k_fill_plain<32, 256>), reinterpret_cast<const void *>(&k_fill_batched<32, 256>)},
+    {16, 128, reinterpret_cast<const void *>(&k_fill_plain<16, 128>), reinterpret_cast<const void *>(&k_fill_batched<16, 128>)},
+    {8, 128, reinterpret_cast<const void *>(&k_fill_plain<8, 128>), reinterpret_cast<const void *>(&k_fill_batched<8, 128>)},
+    {16, 256, reinterpret_cast<const void *>(&k_fill_plain<16, 256>), reinterpret_cast<const void *>(&k_fill_batched<16, 256>)},
+    {32, 128, reinterpret_cast<const void *>(&k_fill_plain<32, 128>), reinterpret_cast<const void *>(&k_fill_batched<32, 128>)},
+    {8, 64, reinterpret_cast<const void *>(&k_fill_plain<8, 64>), reinterpret_cast<const void *>(&k_fill_batched<8, 64>)},
+};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
 struct Fence {
     hipEvent_t ev = nullptr;
     uint64_t id = 0;
@@ -895,7 +923,7 @@ struct rr_hip_ctx {
     unsigned long long *spill_top = nullptr;
     unsigned int *next_item = nullptr;
     int32_t *err_flag = nullptr;
-    int batched_waves = 0;
+    int batched_waves[kNumVariants] = {};
     // page state
     hipEvent_t t0 = nullptr, t1 = nullptr;
     bool page_open = false, page_timed = false;
@@ -1006,6 +1034,8 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
         return fail(c, RR_E_INVARIANT, "unknown canvas flavour");
     if (b.launch != RR_LAUNCH_PLAIN && b.launch != RR_LAUNCH_BATCHED)
         return fail(c, RR_E_INVARIANT, "unknown launch shape");
+    if (b.variant < 0 || b.variant >= kNumVariants) return fail(c, RR_E_INVARIANT, "unknown fill variant");
+    const Variant &var = kVariants[b.variant];
     if (b.n_paths == 0 || b.n_pages == 0) return RR_OK;
     if (!b.edges || !b.edge_off || !b.attrs || !b.pages || (b.launch == RR_LAUNCH_PLAIN && !b.host_rows))
         return fail(c, RR_E_INVARIANT, "null buffer");
@@ -1113,7 +1143,8 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
                 hipEvent_t *ev = c->profiling ? prof_pair(c) : nullptr;
                 if (c->profiling && !ev) return fail(c, RR_E_DEVICE, "profiling event allocation failed");
                 if (ev) RR_HIP_TRY(c, hipEventRecord(ev[0], c->stream));
-                hipLaunchKernelGGL(k_fill_plain, dim3(bands), dim3(64), 0, c->stream, A, pi, p, r0);
+                void *args[] = {&A, const_cast<int32_t *>(&pi), &p, &r0};
+                RR_HIP_TRY(c, hipLaunchKernel(var.plain, dim3(bands), dim3(64), args, 0, c->stream));
                 if (ev) RR_HIP_TRY(c, hipEventRecord(ev[1], c->stream));
                 ++c->fill_launches;
             }
@@ -1124,7 +1155,8 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
 
     // BATCHED: one persistent launch over the (page, band) items.
     const size_t n_items = c->host_items.size();
-    const unsigned waves = static_cast<unsigned>(std::min<size_t>(static_cast<size_t>(c->batched_waves), n_items));
+    const unsigned waves =
+        static_cast<unsigned>(std::min<size_t>(static_cast<size_t>(c->batched_waves[b.variant]), n_items));
     rc = ensure(c, &c->slots, &c->slots_cap, static_cast<size_t>(waves) * kSlotInts);
     if (rc) return rc;
     A.slots = c->slots;
@@ -1132,8 +1164,11 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     hipEvent_t *ev = c->profiling ? prof_pair(c) : nullptr;
     if (c->profiling && !ev) return fail(c, RR_E_DEVICE, "profiling event allocation failed");
     if (ev) RR_HIP_TRY(c, hipEventRecord(ev[0], c->stream));
-    hipLaunchKernelGGL(k_fill_batched, dim3(waves), dim3(64), 0, c->stream, A, c->items, static_cast<int32_t>(n_items),
-                       c->next_item);
+    int2 *items = c->items;
+    int32_t n_it = static_cast<int32_t>(n_items);
+    unsigned int *next_item = c->next_item;
+    void *args[] = {&A, &items, &n_it, &next_item};
+    RR_HIP_TRY(c, hipLaunchKernel(var.batched, dim3(waves), dim3(64), args, 0, c->stream));
     if (ev) RR_HIP_TRY(c, hipEventRecord(ev[1], c->stream));
     ++c->fill_launches;
     RR_HIP_TRY(c, hipGetLastError());
@@ -1157,10 +1192,12 @@ int32_t rr_hip_create(int32_t device, rr_hip_ctx **out) {
     }
     int32_t rc = RR_OK;
     if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) rc = RR_E_DEVICE;
-    int per_cu = 0;
-    if (rc == RR_OK && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_batched, 64, 0) != hipSuccess)
-        rc = RR_E_DEVICE;
-    c->batched_waves = std::max(1, c->n_cu * std::max(1, per_cu));
+    for (int v = 0; v < kNumVariants && rc == RR_OK; ++v) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kVariants[v].batched, 64, 0) != hipSuccess)
+            rc = RR_E_DEVICE;
+        c->batched_waves[v] = std::max(1, c->n_cu * std::max(1, per_cu));
+    }
     if (rc == RR_OK && (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess)) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->spill_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->next_item, sizeof(unsigned int)) != hipSuccess) rc = RR_E_DEVICE;
@@ -1275,6 +1312,7 @@ int32_t rr_hip_record_fill_paths(rr_hip_ctx *c, const rr_fill_batch *b) {
     fp.launch = b->launch;
     fp.cov_hash = b->cov_hash;
     fp.cov_count = b->cov_count;
+    fp.variant = b->variant;
     return record_pages(c, fp);
 }
 
@@ -1314,6 +1352,15 @@ int32_t rr_hip_wait_page(rr_hip_ctx *c, uint64_t fence) {
     if (flag) return fail(c, RR_E_CAPACITY, "overflow-entry pool exhausted: page output is incomplete");
     return RR_OK;
 }
+
+int32_t rr_hip_variant_info(int32_t variant, int32_t *lds_entries, int32_t *chunk_px) {
+    if (variant < 0 || variant >= kNumVariants) return RR_E_INVARIANT;
+    if (lds_entries) *lds_entries = kVariants[variant].k;
+    if (chunk_px) *chunk_px = kVariants[variant].chunk;
+    return RR_OK;
+}
+
+int32_t rr_hip_variant_count(void) { return kNumVariants; }
 
 int32_t rr_hip_set_trace(rr_hip_ctx *c, void *records, int64_t capacity) {
     if (!c || capacity < 0 || (capacity > 0 && !records)) return RR_E_INVARIANT;

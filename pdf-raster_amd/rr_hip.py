@@ -37,7 +37,7 @@ class FillBatch(ctypes.Structure):
                 ("height", ctypes.c_int32), ("stride", ctypes.c_int32), ("mode", ctypes.c_int32),
                 ("flavour", ctypes.c_int32), ("launch", ctypes.c_int32),
                 ("clip", ctypes.c_double * 4), ("cov_hash", ctypes.c_void_p),
-                ("cov_count", ctypes.c_void_p)]
+                ("cov_count", ctypes.c_void_p), ("variant", ctypes.c_int32)]
 
 
 class PageDesc(ctypes.Structure):
@@ -53,7 +53,8 @@ class FillPages(ctypes.Structure):
                 ("n_paths", ctypes.c_int64), ("n_edges", ctypes.c_int64),
                 ("pages", ctypes.c_void_p), ("n_pages", ctypes.c_int32),
                 ("mode", ctypes.c_int32), ("flavour", ctypes.c_int32), ("launch", ctypes.c_int32),
-                ("cov_hash", ctypes.c_void_p), ("cov_count", ctypes.c_void_p)]
+                ("cov_hash", ctypes.c_void_p), ("cov_count", ctypes.c_void_p),
+                ("variant", ctypes.c_int32)]
 
 
 _lib = None
@@ -83,6 +84,8 @@ def lib():
             "rr_hip_wait_page": (i32, [P, u64]),
             "rr_hip_set_profiling": (i32, [P, i32]),
             "rr_hip_set_trace": (i32, [P, P, i64]),
+            "rr_hip_variant_count": (i32, []),
+            "rr_hip_variant_info": (i32, [i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
             "rr_hip_trace_count": (i32, [P, ctypes.POINTER(i64)]),
             "rr_hip_last_page_stats": (i32, [P, ctypes.POINTER(ctypes.c_double),
                                              ctypes.POINTER(ctypes.c_double),
@@ -109,6 +112,17 @@ def exported_symbols():
     import re
     txt = open(HEADER).read()
     return sorted(set(re.findall(r"^\s*[A-Za-z_][\w\s\*]*?\b(rr_\w+)\s*\(", txt, re.M)))
+
+
+def variants():
+    """[(lds_entries, chunk_px)] of the band-fill variants built into the library."""
+    L = lib()
+    out = []
+    for v in range(L.rr_hip_variant_count()):
+        k, ch = ctypes.c_int32(), ctypes.c_int32()
+        L.rr_hip_variant_info(v, ctypes.byref(k), ctypes.byref(ch))
+        out.append((k.value, ch.value))
+    return out
 
 
 def _p(a):
@@ -419,7 +433,7 @@ class DeviceBatch:
             self.be.free(b)
 
     def fill_pages(self, mode=MODE_DOCUMENTED, flavour=FLAVOUR_GRAY8, launch=LAUNCH_BATCHED,
-                   hashes=False, counts=False):
+                   hashes=False, counts=False, variant=0):
         b = FillPages()
         b.edges, b.edge_off, b.attrs = self.edges, self.edge_off, self.attrs
         b.host_rows = self.host_rows.ctypes.data
@@ -429,6 +443,7 @@ class DeviceBatch:
         b.mode, b.flavour, b.launch = mode, flavour, launch
         b.cov_hash = self.hashes if hashes else None
         b.cov_count = self.counts if counts else None
+        b.variant = variant
         return b
 
     def reset(self, background=255):
@@ -439,10 +454,10 @@ class DeviceBatch:
         self.be._check(L.rr_hip_memset(h, self.counts, 0, 16 * max(self.n_paths, 1)), "memset")
 
     def run(self, mode=MODE_DOCUMENTED, flavour=FLAVOUR_GRAY8, launch=LAUNCH_BATCHED,
-            hashes=False, counts=False, background=255):
+            hashes=False, counts=False, background=255, variant=0):
         self.reset(background)
         self.be.begin_page()
-        self.be.record_fill_pages(self.fill_pages(mode, flavour, launch, hashes, counts))
+        self.be.record_fill_pages(self.fill_pages(mode, flavour, launch, hashes, counts, variant))
         self.be.wait_page(self.be.submit_page())
 
     def read_canvas(self, i):

@@ -61,3 +61,23 @@ def test_plain_multi_page_matches(backend):
     pages = R.workload(3, 0xC0F10003, 2)
     pages = [pg.subset(range(0, pg.n_paths, 9)) for pg in pages]
     check(backend, pages, launch=R.LAUNCH_PLAIN)
+
+
+@pytest.mark.parametrize("variant", range(len(R.variants())))
+def test_every_variant_is_bit_exact(backend, variant):
+    pages = R.workload(3, 0xC0F10003, 2)
+    pages = [pg.subset(range(0, pg.n_paths, 6)) for pg in pages]
+    db = R.DeviceBatch(backend, pages)
+    try:
+        db.run(launch=R.LAUNCH_BATCHED, hashes=True, variant=variant)
+        got = [db.read_canvas(i) for i in range(len(pages))]
+        gh = db.read_hashes()
+        db.run(launch=R.LAUNCH_PLAIN, hashes=True, variant=variant)
+        got_p = [db.read_canvas(i) for i in range(len(pages))]
+        gh_p = db.read_hashes()
+    finally:
+        db.free()
+    oc, oh = oracle_pages(pages)
+    for g, gp, o in zip(got, got_p, oc):
+        assert np.array_equal(g, o) and np.array_equal(gp, o)
+    assert np.array_equal(gh, oh) and np.array_equal(gh_p, oh)
