@@ -118,8 +118,13 @@ def main():
     db = R.DeviceBatch(be, pages)
     n_paths, n_edges = db.n_paths, db.n_edges
 
+    # autotuner: time every band-fill variant on this batch, keep the fastest
+    variant, tune_ms = db.autotune(mode, R.FLAVOUR_GRAY8, launch, reps=2)
+    print(f"bench: autotuned variant {variant} {R.variants()[variant]} ms={tune_ms}", file=sys.stderr,
+          flush=True)
+
     def step(shape, hashes=False, counts=False):
-        db.run(mode, R.FLAVOUR_GRAY8, shape, hashes=hashes, counts=counts)
+        db.run(mode, R.FLAVOUR_GRAY8, shape, hashes=hashes, counts=counts, variant=variant)
 
     # work counts + parity fingerprint (outside the timed region)
     step(launch, hashes=True, counts=True)
@@ -151,7 +156,7 @@ def main():
     be.set_profiling(True)
     db.reset(255)
     be.begin_page()
-    be.record_fill_pages(db.fill_pages(mode, R.FLAVOUR_GRAY8, launch))
+    be.record_fill_pages(db.fill_pages(mode, R.FLAVOUR_GRAY8, launch, variant=variant))
     be.wait_page(be.submit_page())
     _, fill_ms, launches = be.last_page_stats()
     be.set_profiling(False)
@@ -174,7 +179,10 @@ def main():
         "config": {"workload": args.workload, "description": desc, "pages": len(pages),
                    "paths": n_paths, "edges": n_edges, "aa_row_mode": args.mode,
                    "canvas": "gray8-exact",
-                   "launch": "batched" if launch == R.LAUNCH_BATCHED else "plain"},
+                   "launch": "batched" if launch == R.LAUNCH_BATCHED else "plain",
+                   "variant": {"id": variant, "lds_entries": R.variants()[variant][0],
+                               "chunk_px": R.variants()[variant][1]}},
+        "autotune_ms": [round(m, 3) for m in tune_ms],
         "pages_per_s": round(pages_per_s, 3),
         "covered_px_per_step": w_cov, "ref_composites_per_step": w_ref,
         "fill_launches_per_step": launches, "parity": parity, "fingerprint": fingerprint[:16],

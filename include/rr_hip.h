@@ -122,12 +122,7 @@ typedef struct {
     int32_t variant;
 } rr_fill_pages;
 
-/* Band-fill shapes the library is built with (the autotuner's search space;
- * no counterpart in the reference).  Every variant gives identical pixels;
- * they differ in the LDS entry slots per AA row and the composite chunk
- * width, hence in occupancy.  variant_count() is the number of variants. */
-int32_t rr_hip_variant_count(void);
-int32_t rr_hip_variant_info(int32_t variant, int32_t *lds_entries, int32_t *chunk_px);
+
 
 /* ---- context (GpuCtx::init, crates/gpu/src/lib.rs:239; CudaBackend, backend/cuda/mod.rs:27) ---- */
 typedef struct rr_hip_ctx rr_hip_ctx;
@@ -167,6 +162,20 @@ int32_t rr_hip_set_profiling(rr_hip_ctx *ctx, int32_t enable);
  * the last set (it may exceed capacity: the rest were dropped). */
 int32_t rr_hip_set_trace(rr_hip_ctx *ctx, void *records, int64_t capacity);
 int32_t rr_hip_trace_count(rr_hip_ctx *ctx, int64_t *count);
+
+/* Band-fill shapes the library is built with (the autotuner's search space;
+ * no counterpart in the reference).  Every variant gives identical pixels;
+ * they differ in the LDS entry slots per AA row and the composite chunk
+ * width, hence in occupancy.  variant_count() is the number of variants. */
+int32_t rr_hip_variant_count(void);
+int32_t rr_hip_variant_info(int32_t variant, int32_t *lds_entries, int32_t *chunk_px);
+/* Autotuner (no counterpart in the reference): records `batch` once per
+ * variant plus `reps` timed repetitions (HIP events around the recorded
+ * work), writes the mean milliseconds of each variant to ms[] (variant_count
+ * entries, may be NULL) and the fastest variant to *best.  The batch's
+ * canvases are overwritten: reset them before the real run. */
+int32_t rr_hip_autotune_fill(rr_hip_ctx *ctx, const rr_fill_pages *batch, int32_t reps, int32_t *best,
+                             double *ms);
 int32_t rr_hip_last_page_stats(rr_hip_ctx *ctx, double *span_ms, double *fill_ms, int64_t *launches);
 
 /* ---- host path ingestion (XPath::new, crates/raster/src/xpath.rs:162-260,

@@ -1362,6 +1362,46 @@ int32_t rr_hip_variant_info(int32_t variant, int32_t *lds_entries, int32_t *chun
 
 int32_t rr_hip_variant_count(void) { return kNumVariants; }
 
+int32_t rr_hip_autotune_fill(rr_hip_ctx *c, const rr_fill_pages *b, int32_t reps, int32_t *best, double *ms) {
+    if (!c || !b || !best || reps <= 0) return RR_E_INVARIANT;
+    if (c->page_open) return fail(c, RR_E_INVARIANT, "autotune inside an open page");
+    rr_fill_pages t = *b;
+    t.cov_hash = nullptr;
+    t.cov_count = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    RR_HIP_TRY(c, hipEventCreate(&e0));
+    RR_HIP_TRY(c, hipEventCreate(&e1));
+    double best_ms = 0.0;
+    int32_t rc = RR_OK;
+    *best = 0;
+    for (int v = 0; v < kNumVariants && rc == RR_OK; ++v) {
+        t.variant = v;
+        double sum = 0.0;
+        for (int r = 0; r <= reps && rc == RR_OK; ++r) {  // r == 0 is a warm-up
+            rr_hip_begin_page(c);
+            if (hipEventRecord(e0, c->stream) != hipSuccess) rc = RR_E_DEVICE;
+            if (rc == RR_OK) rc = record_pages(c, t);
+            if (rc == RR_OK && hipEventRecord(e1, c->stream) != hipSuccess) rc = RR_E_DEVICE;
+            uint64_t f = 0;
+            const int32_t rs = rr_hip_submit_page(c, &f);
+            if (rc == RR_OK) rc = rs;
+            if (rc == RR_OK) rc = rr_hip_wait_page(c, f);
+            float m = 0.f;
+            if (rc == RR_OK && hipEventElapsedTime(&m, e0, e1) != hipSuccess) rc = RR_E_DEVICE;
+            if (r > 0) sum += m;
+        }
+        const double avg = sum / reps;
+        if (ms) ms[v] = avg;
+        if (rc == RR_OK && (v == 0 || avg < best_ms)) {
+            best_ms = avg;
+            *best = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
 int32_t rr_hip_set_trace(rr_hip_ctx *c, void *records, int64_t capacity) {
     if (!c || capacity < 0 || (capacity > 0 && !records)) return RR_E_INVARIANT;
     if (c->page_open) return fail(c, RR_E_INVARIANT, "set_trace inside an open page");

@@ -85,6 +85,8 @@ def lib():
             "rr_hip_set_profiling": (i32, [P, i32]),
             "rr_hip_set_trace": (i32, [P, P, i64]),
             "rr_hip_variant_count": (i32, []),
+            "rr_hip_autotune_fill": (i32, [P, ctypes.POINTER(FillPages), i32,
+                                           ctypes.POINTER(i32), P]),
             "rr_hip_variant_info": (i32, [i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
             "rr_hip_trace_count": (i32, [P, ctypes.POINTER(i64)]),
             "rr_hip_last_page_stats": (i32, [P, ctypes.POINTER(ctypes.c_double),
@@ -459,6 +461,16 @@ class DeviceBatch:
         self.be.begin_page()
         self.be.record_fill_pages(self.fill_pages(mode, flavour, launch, hashes, counts, variant))
         self.be.wait_page(self.be.submit_page())
+
+    def autotune(self, mode=MODE_DOCUMENTED, flavour=FLAVOUR_GRAY8, launch=LAUNCH_BATCHED, reps=2):
+        """Time every band-fill variant on this batch; returns (best, [ms per variant])."""
+        n = self.be.L.rr_hip_variant_count()
+        ms = np.zeros(n, np.float64)
+        best = ctypes.c_int32(0)
+        b = self.fill_pages(mode, flavour, launch)
+        self.be._check(self.be.L.rr_hip_autotune_fill(self.be.h, ctypes.byref(b), reps,
+                                                       ctypes.byref(best), _p(ms)), "autotune")
+        return best.value, ms.tolist()
 
     def read_canvas(self, i):
         pg = self.pages[i]
