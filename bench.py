@@ -99,6 +99,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU oracle work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-path-per-launch comparison")
+    ap.add_argument("--variant", type=int, default=-1, help="band-fill variant (-1 = autotune)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="end with a 1 GiB device-to-device copy (FETCH/WRITE counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1 "
@@ -119,8 +122,11 @@ def main():
     n_paths, n_edges = db.n_paths, db.n_edges
 
     # autotuner: time every band-fill variant on this batch, keep the fastest
-    variant, tune_ms = db.autotune(mode, R.FLAVOUR_GRAY8, launch, reps=2)
-    print(f"bench: autotuned variant {variant} {R.variants()[variant]} ms={tune_ms}", file=sys.stderr,
+    if args.variant >= 0:
+        variant, tune_ms = args.variant, []
+    else:
+        variant, tune_ms = db.autotune(mode, R.FLAVOUR_GRAY8, launch, reps=2)
+    print(f"bench: variant {variant} {R.variants()[variant]} autotune ms={tune_ms}", file=sys.stderr,
           flush=True)
 
     def step(shape, hashes=False, counts=False):
@@ -200,6 +206,12 @@ def main():
         out[f"{key}_mpx_per_s"] = round(w_ref / sec_o / 1e6, 3)
         out["batched_over_plain"] = round((sec_o / sec) if key == "plain" else (sec / sec_o), 3)
     db.free()
+    if args.calibrate_copy:
+        n = 1 << 30
+        a, b = be.alloc(n), be.alloc(n)
+        be.copy(b, a, n)
+        be.free(a)
+        be.free(b)
     be.close()
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)

@@ -139,6 +139,7 @@ int32_t rr_hip_free(rr_hip_ctx *ctx, void *buf);
 int32_t rr_hip_upload(rr_hip_ctx *ctx, void *dst, const void *src, size_t n);
 int32_t rr_hip_download(rr_hip_ctx *ctx, void *dst, const void *src, size_t n);
 int32_t rr_hip_memset(rr_hip_ctx *ctx, void *dst, int32_t value, size_t n);
+int32_t rr_hip_copy(rr_hip_ctx *ctx, void *dst, const void *src, size_t n); /* device to device */
 
 /* ---- page recording (begin_page / record_* / submit_page / wait_page,
  *      backend/mod.rs:306-405) ---- */

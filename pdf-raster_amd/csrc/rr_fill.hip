@@ -1267,6 +1267,13 @@ int32_t rr_hip_download(rr_hip_ctx *c, void *dst, const void *src, size_t n) {
     return RR_OK;
 }
 
+int32_t rr_hip_copy(rr_hip_ctx *c, void *dst, const void *src, size_t n) {
+    if (!c || (n && (!dst || !src))) return RR_E_INVARIANT;
+    RR_HIP_TRY(c, hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, c->stream));
+    RR_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return RR_OK;
+}
+
 int32_t rr_hip_memset(rr_hip_ctx *c, void *dst, int32_t value, size_t n) {
     if (!c || (n && !dst)) return RR_E_INVARIANT;
     RR_HIP_TRY(c, hipMemsetAsync(dst, value, n, c->stream));

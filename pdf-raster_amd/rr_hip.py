@@ -77,6 +77,7 @@ def lib():
             "rr_hip_upload": (i32, [P, P, P, ctypes.c_size_t]),
             "rr_hip_download": (i32, [P, P, P, ctypes.c_size_t]),
             "rr_hip_memset": (i32, [P, P, i32, ctypes.c_size_t]),
+            "rr_hip_copy": (i32, [P, P, P, ctypes.c_size_t]),
             "rr_hip_begin_page": (i32, [P]),
             "rr_hip_record_fill_paths": (i32, [P, ctypes.POINTER(FillBatch)]),
             "rr_hip_record_fill_pages": (i32, [P, ctypes.POINTER(FillPages)]),
@@ -275,6 +276,9 @@ class Backend:
         if arr.nbytes:
             self.upload(buf, arr)
         return buf
+
+    def copy(self, dst, src, nbytes):
+        self._check(self.L.rr_hip_copy(self.h, dst, src, nbytes), "copy")
 
     def download(self, src, arr):
         self._check(self.L.rr_hip_download(self.h, _p(arr), src, arr.nbytes), "download")
