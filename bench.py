@@ -39,7 +39,27 @@ WORKLOADS = {
     "polygon-page": (1, 0xC0F10001, 0, R.LAUNCH_PLAIN,
                      "Letter@300 2550x3300 u8, 500 random 3-64 vertex polygons, half NZ half EO, "
                      "one path per launch"),
+    "images-32": (4, 0xC0F10004, 0, None,
+                  "32 images 1024-4096 px a side (CMYK via a 17^4 u16 grid / RGB via a 33^3 u16 "
+                  "grid to grey), rotation 0-30 deg, scale 0.5-2, 8-bit smooth soft mask, Normal "
+                  "and Multiply alternating, 4 per 2550x3300 grey page; one launch per image"),
 }
+IMAGES_PER_PAGE = 4
+
+
+def image_bbox_px(im, page_w, page_h):
+    """Canvas pixels the general affine path visits for one image: its clamped
+    device-space bounding box (crates/pdf_interp/src/renderer/page/mod.rs:1607-1647).
+    The corner products are exact, so plain f64 arithmetic matches the fused form."""
+    import math
+    a, b, c, d, e, f = im.ctm
+    xs = [e, a + e, c + e, a + c + e]
+    ys = [page_h - f, page_h - (b + f), page_h - (d + f), page_h - (b + d + f)]
+    x0 = min(max(math.floor(min(xs)), 0), page_w)
+    x1 = min(max(math.ceil(max(xs)), 0), page_w)
+    y0 = min(max(math.floor(min(ys)), 0), page_h)
+    y1 = min(max(math.ceil(max(ys)), 0), page_h)
+    return max(x1 - x0, 0) * max(y1 - y0, 0)
 
 
 def load_golden(name):
@@ -88,6 +108,102 @@ def cpu_baseline(pages, w_per_path, budget_s, threads, mode):
     return sum(done_w), sum(done_paths), dt
 
 
+def cpu_baseline_images(iset, budget_s, threads):
+    """Time the oracle's image placement: thread t places images t, t+threads, ...
+    onto its own page canvas, cycling, until the budget is spent."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O  # the checker, timed here as the CPU reference
+
+    done_px = [0] * threads
+    done_img = [0] * threads
+
+    def worker(t):
+        canvas = np.full((iset.page_h, iset.page_w), 255, np.uint8)
+        mine = list(range(t % len(iset.images), len(iset.images), threads)) or [0]
+        t_end = time.perf_counter() + budget_s
+        k = 0
+        while time.perf_counter() < t_end:
+            im = iset.images[mine[k % len(mine)]]
+            grid, g = (iset.grid4, iset.grid4_n) if im.channels == 4 else (iset.grid3, iset.grid3_n)
+            O.place_image(canvas, im.pixels, im.smask, im.ctm, im.blend, grid, g)
+            done_px[t] += image_bbox_px(im, iset.page_w, iset.page_h)
+            done_img[t] += 1
+            k += 1
+
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=worker, args=(t,)) for t in range(threads)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    return sum(done_px), sum(done_img), time.perf_counter() - t0
+
+
+def bench_images(args, desc):
+    """Image placement + soft mask + Normal/Multiply through the grey CLUT.  A
+    step resets every page canvas to 255 and places all images in order; value
+    is canvas pixels visited (the images' clamped bounding boxes) per second."""
+    _, seed, _, _, _ = WORKLOADS[args.workload]
+    t_gen = time.perf_counter()
+    iset = R.images(seed, 32, IMAGES_PER_PAGE)
+    print(f"bench: {args.workload}: {len(iset.images)} images on {iset.n_pages} pages generated in "
+          f"{time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+    be = R.Backend(0)
+    di = R.DeviceImages(be, iset)
+    bbox_px = sum(image_bbox_px(im, iset.page_w, iset.page_h) for im in iset.images)
+    img_bytes = sum(min(image_bbox_px(im, iset.page_w, iset.page_h), im.width * im.height) *
+                    (im.channels + 1) for im in iset.images)
+
+    di.run()
+    digest = hashlib.sha256()
+    for i in range(iset.n_pages):
+        digest.update(di.read_canvas(i).tobytes())
+    fingerprint = digest.hexdigest()
+    golden = load_golden(f"{args.workload}")
+    parity = "no-fixture" if golden is None else ("match" if golden == fingerprint else "MISMATCH")
+
+    for _ in range(args.warmup):
+        di.run()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        di.run()
+    sec = (time.perf_counter() - t0) / args.steps
+    print(f"bench: images {sec * 1e3:.2f} ms/step", file=sys.stderr, flush=True)
+
+    be.set_profiling(True)
+    di.run()
+    _, k_ms, launches = be.last_page_stats()
+    be.set_profiling(False)
+    alg_bytes = 2 * bbox_px + img_bytes
+    per_launch = alg_bytes / max(launches, 1)
+    avg_s = k_ms / 1e3 / max(launches, 1)
+    achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    out = {
+        "metric": METRIC, "value": round(bbox_px / sec / 1e6, 3), "unit": "Mpx/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "description": desc, "images": len(iset.images),
+                   "pages": iset.n_pages, "canvas_px_per_step": bbox_px},
+        "pages_per_s": round(iset.n_pages / sec, 3), "launches_per_step": launches,
+        "parity": parity, "fingerprint": fingerprint[:16],
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "kernel": "k_place_image", "avg_launch_us": round(avg_s * 1e6, 3),
+                     "alg_bytes_per_launch": round(per_launch, 1)},
+    }
+    di.free()
+    be.close()
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        px, n_img, cdt = cpu_baseline_images(iset, args.cpu_budget, threads)
+        out["cpu_baseline"] = {
+            "value": round(px / cdt / 1e6, 3), "unit": "Mpx/s", "cores": threads, "kind": "port",
+            "sample": f"{n_img} image placements by the C oracle over {threads} threads "
+                      f"(thread t cycles images t, t+{threads}, ...), {cdt:.1f} s"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +225,8 @@ def main():
         sys.exit(2)
 
     cfg, seed, max_pages, launch, desc = WORKLOADS[args.workload]
+    if cfg == 4:
+        return bench_images(args, desc)
     if args.pages:
         max_pages = args.pages
     mode = R.MODE_LITERAL if args.mode == "literal" else R.MODE_DOCUMENTED

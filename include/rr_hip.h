@@ -208,6 +208,63 @@ int32_t rr_workload_page_data(const rr_workload *w, int32_t p, const double **ed
                               const int32_t **host_rows, const double **pts,
                               const uint8_t **pt_flags, const int64_t **pt_off);
 
+/* ---- images + colour (image placement, soft mask, Normal/Multiply, CLUT) ---- */
+
+/* One image XObject to place.  pixels: `channels` interleaved bytes per pixel
+ * (3 = RGB, 4 = CMYK), row-major; smask: one byte per image pixel (or NULL =
+ * opaque), sampled at the same image pixel as the colour
+ * (crates/pdf_interp/src/renderer/page/mod.rs:1917-1920).  ctm: the PDF CTM
+ * mapping the unit square to PDF user space on a page of the canvas height
+ * (y up), as blit_image receives it (page/mod.rs:1600-1676).  blend: 0 Normal,
+ * 1 Multiply (blend_multiply, crates/raster/src/pipe/blend.rs:24-26, then
+ * blend_u8 under the mask).  page: index into the batch's page table. */
+typedef struct {
+    const uint8_t *pixels;
+    const uint8_t *smask;
+    int32_t width, height, channels, blend;
+    double ctm[6];
+    int32_t page, reserved;
+} rr_image_desc;
+
+/* Images applied in order onto the pages' canvases (device pointers in the
+ * descriptors, host descriptor arrays).  grid4: CMYK->grey u16 grid with
+ * grid4_n nodes per axis, index ((k*n + c)*n + m)*n + y; grid3: RGB->grey u16
+ * grid, index (r*n + g)*n + b.  Sampling follows the reference's general affine
+ * path; CTMs that would take its axis-aligned path are rejected (RR_E_INVARIANT). */
+typedef struct {
+    const rr_image_desc *images;
+    int32_t n_images;
+    const rr_page_desc *pages;
+    int32_t n_pages;
+    const uint16_t *grid4;
+    int32_t grid4_n;
+    const uint16_t *grid3;
+    int32_t grid3_n;
+} rr_image_batch;
+int32_t rr_hip_record_place_images(rr_hip_ctx *ctx, const rr_image_batch *batch);
+
+/* Reference GPU ops over device buffers (GpuBackend::record_icc_clut /
+ * record_composite / record_apply_soft_mask, crates/gpu/src/backend/mod.rs:
+ * 321-336, params.rs:109-205), bit-exact to their CPU twins:
+ * icc_cmyk_to_rgb_cpu (crates/gpu/src/cmyk.rs:523-634: clut == NULL selects the
+ * reflectance matrix, color/src/convert.rs:131-161), composite_rgba8_cpu and
+ * apply_soft_mask_cpu (crates/gpu/src/composite.rs:4-52). */
+int32_t rr_hip_record_icc_clut(rr_hip_ctx *ctx, const uint8_t *cmyk, uint8_t *rgb, const uint8_t *clut,
+                               int32_t grid_n, int64_t n_pixels);
+int32_t rr_hip_record_composite(rr_hip_ctx *ctx, const uint8_t *src, uint8_t *dst, int64_t n_pixels);
+int32_t rr_hip_record_apply_soft_mask(rr_hip_ctx *ctx, uint8_t *pixels, const uint8_t *mask, int64_t n_pixels);
+
+/* Seeded image workload: n_images images, images_per_page per 2550x3300 page,
+ * image i is CMYK + Normal when i is even, RGB + Multiply when odd.  info =
+ * {n_images, n_pages, page_w, page_h, grid4_n, grid3_n}; rr_images_get fills a
+ * descriptor with host pointers. */
+typedef struct rr_image_set rr_image_set;
+int32_t rr_images_create(uint64_t seed, int32_t n_images, int32_t images_per_page, rr_image_set **out);
+void rr_images_destroy(rr_image_set *set);
+int32_t rr_images_info(const rr_image_set *set, int32_t info[6]);
+int32_t rr_images_get(const rr_image_set *set, int32_t i, rr_image_desc *desc);
+int32_t rr_images_grids(const rr_image_set *set, const uint16_t **grid4, const uint16_t **grid3);
+
 #ifdef __cplusplus
 }
 #endif

@@ -84,6 +84,14 @@ def lib():
             "rro_fill_page": (i64, [P, i32, i32, i32, P, P, i64, P, P, P, ctypes.c_int,
                                     ctypes.c_int, P]),
             "rro_cov_hash_term": (ctypes.c_uint64, [i64, u8]),
+            "rro_blend_u8": (u8, [u8, u8, u32]),
+            "rro_blend_multiply": (u8, [u8, u8]),
+            "rro_composite_rgba8": (None, [P, P, i64]),
+            "rro_apply_soft_mask": (None, [P, P, i64]),
+            "rro_cmyk_matrix": (None, [P, P, i64]),
+            "rro_icc_clut_ref": (None, [P, P, i64, P, i32]),
+            "rro_clut_gray": (u8, [P, i32, P, i32]),
+            "rro_place_image": (i64, [P, i32, i32, i32, P, i32, i32, i32, P, P, i32, P, i32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -253,3 +261,66 @@ def fill_page(canvas, edges, edge_off, eo, color, a_input, mode=MODE_DOCUMENTED,
 
 def cov_hash_term(pix, shape):
     return lib().rro_cov_hash_term(pix, shape)
+
+
+# ---- images and colour (rr_oracle_img.c) ----
+
+def blend_u8(src, dst, alpha):
+    return lib().rro_blend_u8(src, dst, alpha)
+
+
+def blend_multiply(src, dst):
+    return lib().rro_blend_multiply(src, dst)
+
+
+def composite_rgba8(src, dst):
+    """In-place source-over of RGBA8 src onto dst (both (n, 4) uint8)."""
+    src = np.ascontiguousarray(src, np.uint8)
+    assert dst.flags.c_contiguous and dst.dtype == np.uint8 and dst.size == src.size
+    lib().rro_composite_rgba8(_p(src), _p(dst), src.size // 4)
+    return dst
+
+
+def apply_soft_mask(px, mask):
+    mask = np.ascontiguousarray(mask, np.uint8)
+    assert px.flags.c_contiguous and px.dtype == np.uint8 and px.size == 4 * mask.size
+    lib().rro_apply_soft_mask(_p(px), _p(mask), mask.size)
+    return px
+
+
+def cmyk_matrix(cmyk):
+    cmyk = np.ascontiguousarray(cmyk, np.uint8).reshape(-1, 4)
+    out = np.zeros((len(cmyk), 3), np.uint8)
+    lib().rro_cmyk_matrix(_p(cmyk), _p(out), len(cmyk))
+    return out
+
+
+def icc_clut(cmyk, table, g):
+    """icc_cmyk_to_rgb_cpu's CLUT arm: table is g^4*3 u8, K-major."""
+    cmyk = np.ascontiguousarray(cmyk, np.uint8).reshape(-1, 4)
+    table = np.ascontiguousarray(table, np.uint8)
+    assert table.size == g ** 4 * 3
+    out = np.zeros((len(cmyk), 3), np.uint8)
+    lib().rro_icc_clut_ref(_p(cmyk), _p(out), len(cmyk), _p(table), g)
+    return out
+
+
+def clut_gray(px, grid, g):
+    px = np.ascontiguousarray(px, np.uint8)
+    grid = np.ascontiguousarray(grid, np.uint16)
+    assert grid.size == g ** len(px)
+    return lib().rro_clut_gray(_p(px), len(px), _p(grid), g)
+
+
+def place_image(canvas, pixels, smask, ctm, blend, grid, g):
+    """Place one image on a grey canvas ((h, stride) view, width = shape[1]).
+    Returns the number of pixels written, or -1 for an axis-aligned CTM."""
+    h, w = canvas.shape
+    stride = canvas.strides[0]
+    pixels = np.ascontiguousarray(pixels, np.uint8)
+    ih, iw, ch = pixels.shape
+    sm = None if smask is None else np.ascontiguousarray(smask, np.uint8)
+    m = np.ascontiguousarray(ctm, np.float64)
+    grid = np.ascontiguousarray(grid, np.uint16)
+    return lib().rro_place_image(_p(canvas), w, h, stride, _p(pixels), iw, ih, ch, _p(sm), _p(m),
+                                 blend, _p(grid), g)

@@ -132,6 +132,20 @@ int64_t rro_fill_page(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, con
 /* Order-independent coverage hash term for one pixel. */
 uint64_t rro_cov_hash_term(int64_t pixel_index, uint8_t shape);
 
+/* ---- images and colour (rr_oracle_img.c) ---- */
+uint8_t rro_blend_u8(uint8_t src, uint8_t dst, uint32_t alpha);      /* page/mod.rs:2146-2149; pinned :2366-2395 */
+uint8_t rro_blend_multiply(uint8_t src, uint8_t dst);                /* pipe/blend.rs:24-26 */
+void rro_composite_rgba8(const uint8_t *src, uint8_t *dst, int64_t n); /* gpu/src/composite.rs:4-37; pinned :58-83 */
+void rro_apply_soft_mask(uint8_t *px, const uint8_t *mask, int64_t n); /* gpu/src/composite.rs:39-52; pinned :85-109 */
+void rro_cmyk_matrix(const uint8_t *cmyk, uint8_t *rgb, int64_t n);  /* color/src/convert.rs:131-161 */
+/* gpu/src/cmyk.rs:531-630; pinned by cmyk.rs:809-846 corners + closed forms */
+void rro_icc_clut_ref(const uint8_t *cmyk, uint8_t *rgb, int64_t n, const uint8_t *table, int32_t g);
+/* Grey CLUT of the image workload: own definition, pinned by invariants. */
+uint8_t rro_clut_gray(const uint8_t *px, int32_t dims, const uint16_t *grid, int32_t g);
+int64_t rro_place_image(uint8_t *canvas, int32_t width, int32_t height, int32_t stride, const uint8_t *pixels,
+                        int32_t img_w, int32_t img_h, int32_t channels, const uint8_t *smask, const double ctm[6],
+                        int32_t blend, const uint16_t *grid, int32_t g);
+
 #ifdef __cplusplus
 }
 #endif

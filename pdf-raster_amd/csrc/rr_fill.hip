@@ -43,6 +43,7 @@
 #include <vector>
 
 #include "rr_hip.h"
+#include "rr_internal.h"
 
 namespace {
 
@@ -1176,6 +1177,21 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
 }
 
 }  // namespace
+
+namespace rrint {
+hipStream_t stream(rr_hip_ctx *c) { return c->stream; }
+int device(rr_hip_ctx *c) { return c->device; }
+bool page_open(rr_hip_ctx *c) { return c->page_open; }
+int32_t fail(rr_hip_ctx *c, int32_t code, const std::string &msg) { return ::fail(c, code, msg); }
+hipEvent_t *prof_pair(rr_hip_ctx *c, bool *ok) {
+    *ok = true;
+    if (!c->profiling) return nullptr;
+    hipEvent_t *e = ::prof_pair(c);
+    *ok = e != nullptr;
+    return e;
+}
+void count_launch(rr_hip_ctx *c) { ++c->fill_launches; }
+}  // namespace rrint
 
 extern "C" {
 

@@ -359,7 +359,130 @@ void gen_mixed_path(Page &pg, SplitMix64 &rng) {
     pg.add_path(b, eo, color, 255, 0.1);
 }
 
+// ---------------------------------------------------------------------------
+// Image workload: 32 images, RGB or CMYK, 1024-4096 px per side, placed by a
+// rotation 0-30 deg and scale 0.5-2 onto Letter@300 pages under a smooth
+// image-space soft mask; grey through a 17^4 (CMYK) or 33^3 (RGB) u16 grid.
+// ---------------------------------------------------------------------------
+struct Image {
+    int32_t w = 0, h = 0, channels = 0, blend = 0, page = 0;
+    double ctm[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<uint8_t> pixels, smask;
+};
+
+inline uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+void gen_image(Image &im, SplitMix64 &rng, int idx, int32_t page_w, int32_t page_h) {
+    im.w = static_cast<int32_t>(rng.range(1024, 4096));
+    im.h = static_cast<int32_t>(rng.range(1024, 4096));
+    im.channels = (idx % 2 == 0) ? 4 : 3;
+    im.blend = (idx % 2 == 0) ? 0 : 1;
+    // content: per-channel gradients plus a little hashed noise
+    const size_t n = static_cast<size_t>(im.w) * im.h;
+    im.pixels.resize(n * im.channels);
+    int32_t gx[4], gy[4], base[4];
+    for (int c = 0; c < im.channels; ++c) {
+        gx[c] = static_cast<int32_t>(rng.range(-300, 300));
+        gy[c] = static_cast<int32_t>(rng.range(-300, 300));
+        base[c] = static_cast<int32_t>(rng.range(0, 255));
+    }
+    const uint32_t salt = static_cast<uint32_t>(rng.next());
+    for (int32_t y = 0; y < im.h; ++y)
+        for (int32_t x = 0; x < im.w; ++x) {
+            const size_t o = (static_cast<size_t>(y) * im.w + x) * im.channels;
+            const uint32_t hsh = mix32(salt ^ static_cast<uint32_t>(y * 65599 + x));
+            for (int c = 0; c < im.channels; ++c) {
+                const int32_t v = base[c] + ((x * gx[c] + y * gy[c]) >> 10) + static_cast<int32_t>((hsh >> (8 * c)) & 15);
+                im.pixels[o + c] = static_cast<uint8_t>(v & 255);
+            }
+        }
+    // soft mask: integer bilinear upsampling of a 64x64 lattice of U{0..255}
+    uint8_t lat[64][64];
+    for (auto &row : lat)
+        for (auto &v : row) v = static_cast<uint8_t>(rng.range(0, 255));
+    im.smask.resize(n);
+    for (int32_t y = 0; y < im.h; ++y) {
+        const int64_t fy = static_cast<int64_t>(y) * 63 * 256 / (im.h - 1);
+        const int32_t j = static_cast<int32_t>(fy >> 8), u = static_cast<int32_t>(fy & 255), j1 = j < 63 ? j + 1 : 63;
+        for (int32_t x = 0; x < im.w; ++x) {
+            const int64_t fx = static_cast<int64_t>(x) * 63 * 256 / (im.w - 1);
+            const int32_t i = static_cast<int32_t>(fx >> 8), t = static_cast<int32_t>(fx & 255), i1 = i < 63 ? i + 1 : 63;
+            const int32_t top = lat[j][i] * (256 - t) + lat[j][i1] * t;
+            const int32_t bot = lat[j1][i] * (256 - t) + lat[j1][i1] * t;
+            im.smask[static_cast<size_t>(y) * im.w + x] = static_cast<uint8_t>((top * (256 - u) + bot * u + 32768) >> 16);
+        }
+    }
+    // placement: rotation, scale, centre uniform over the page (device y down);
+    // the CTM maps the unit square in PDF space (y up) as the reference expects.
+    double th, a, b, c, d;
+    const double s = rng.uniform(0.5, 2.0);
+    do {
+        th = rng.uniform(0.0, 30.0) * 3.141592653589793 / 180.0;
+        a = im.w * s * std::cos(th);
+        b = im.w * s * std::sin(th);
+        c = -im.h * s * std::sin(th);
+        d = im.h * s * std::cos(th);
+    } while (std::fabs(b) * im.w < 0.5 && std::fabs(c) * im.h < 0.5);  // keep the general affine path
+    const double cx = rng.uniform(0.0, page_w), cy = page_h - rng.uniform(0.0, page_h);
+    im.ctm[0] = a;
+    im.ctm[1] = b;
+    im.ctm[2] = c;
+    im.ctm[3] = d;
+    im.ctm[4] = cx - 0.5 * a - 0.5 * c;
+    im.ctm[5] = cy - 0.5 * b - 0.5 * d;
+}
+
+// "Monotone-ish" grids: per axis a seeded non-decreasing curve (normalised
+// cumulative sum of U[0,1) increments) combined per node, plus +-1% noise.
+std::vector<uint16_t> gen_grid(SplitMix64 &rng, int dims, int g) {
+    std::vector<std::vector<double>> curve(static_cast<size_t>(dims), std::vector<double>(static_cast<size_t>(g)));
+    for (auto &cv : curve) {
+        double acc = 0.0;
+        cv[0] = 0.0;
+        for (int i = 1; i < g; ++i) cv[static_cast<size_t>(i)] = (acc += rng.unit());
+        for (auto &v : cv) v /= acc;
+    }
+    size_t n = 1;
+    for (int a = 0; a < dims; ++a) n *= static_cast<size_t>(g);
+    std::vector<uint16_t> grid(n);
+    for (size_t idx = 0; idx < n; ++idx) {
+        size_t r = idx;
+        int ia[4] = {0, 0, 0, 0};
+        // index layout: 4-D (k, c, m, y) with y fastest; 3-D (r, g, b) with b fastest
+        for (int a = dims - 1; a >= 0; --a) {
+            ia[a] = static_cast<int>(r % static_cast<size_t>(g));
+            r /= static_cast<size_t>(g);
+        }
+        double v;
+        if (dims == 4) {  // ia = {k, c, m, y}: reflectance falls with every ink
+            const double k = curve[3][static_cast<size_t>(ia[0])], cc = curve[0][static_cast<size_t>(ia[1])];
+            const double m = curve[1][static_cast<size_t>(ia[2])], y = curve[2][static_cast<size_t>(ia[3])];
+            v = (1.0 - 0.3 * cc) * (1.0 - 0.59 * m) * (1.0 - 0.11 * y) * (1.0 - k);
+        } else {  // ia = {r, g, b}: BT.709 weights
+            v = 0.2126 * curve[0][static_cast<size_t>(ia[0])] + 0.7152 * curve[1][static_cast<size_t>(ia[1])] +
+                0.0722 * curve[2][static_cast<size_t>(ia[2])];
+        }
+        v += rng.uniform(-0.01, 0.01);
+        v = std::fmin(1.0, std::fmax(0.0, v));
+        grid[idx] = static_cast<uint16_t>(std::lround(v * 65535.0));
+    }
+    return grid;
+}
+
 }  // namespace
+
+struct rr_image_set {
+    int32_t page_w = 2550, page_h = 3300, n_pages = 0, grid4_n = 17, grid3_n = 33;
+    std::vector<Image> images;
+    std::vector<uint16_t> grid4, grid3;
+};
 
 struct rr_workload {
     std::vector<Page> pages;
@@ -455,6 +578,57 @@ int32_t rr_workload_page_data(const rr_workload *w, int32_t p, const double **ed
     if (pts) *pts = pg.pts.data();
     if (pt_flags) *pt_flags = pg.pt_flags.data();
     if (pt_off) *pt_off = pg.pt_off.data();
+    return RR_OK;
+}
+
+int32_t rr_images_create(uint64_t seed, int32_t n_images, int32_t images_per_page, rr_image_set **out) {
+    if (!out || n_images <= 0 || images_per_page <= 0) return RR_E_INVARIANT;
+    auto set = std::make_unique<rr_image_set>();
+    SplitMix64 rng(seed);
+    set->grid4 = gen_grid(rng, 4, set->grid4_n);
+    set->grid3 = gen_grid(rng, 3, set->grid3_n);
+    set->images.resize(static_cast<size_t>(n_images));
+    for (int32_t i = 0; i < n_images; ++i) {
+        gen_image(set->images[static_cast<size_t>(i)], rng, i, set->page_w, set->page_h);
+        set->images[static_cast<size_t>(i)].page = i / images_per_page;
+    }
+    set->n_pages = (n_images + images_per_page - 1) / images_per_page;
+    *out = set.release();
+    return RR_OK;
+}
+
+void rr_images_destroy(rr_image_set *s) { delete s; }
+
+int32_t rr_images_info(const rr_image_set *s, int32_t info[6]) {
+    if (!s || !info) return RR_E_INVARIANT;
+    info[0] = static_cast<int32_t>(s->images.size());
+    info[1] = s->n_pages;
+    info[2] = s->page_w;
+    info[3] = s->page_h;
+    info[4] = s->grid4_n;
+    info[5] = s->grid3_n;
+    return RR_OK;
+}
+
+int32_t rr_images_get(const rr_image_set *s, int32_t i, rr_image_desc *desc) {
+    if (!s || !desc || i < 0 || i >= static_cast<int32_t>(s->images.size())) return RR_E_INVARIANT;
+    const Image &im = s->images[static_cast<size_t>(i)];
+    desc->pixels = im.pixels.data();
+    desc->smask = im.smask.data();
+    desc->width = im.w;
+    desc->height = im.h;
+    desc->channels = im.channels;
+    desc->blend = im.blend;
+    for (int k = 0; k < 6; ++k) desc->ctm[k] = im.ctm[k];
+    desc->page = im.page;
+    desc->reserved = 0;
+    return RR_OK;
+}
+
+int32_t rr_images_grids(const rr_image_set *s, const uint16_t **grid4, const uint16_t **grid3) {
+    if (!s) return RR_E_INVARIANT;
+    if (grid4) *grid4 = s->grid4.data();
+    if (grid3) *grid3 = s->grid3.data();
     return RR_OK;
 }
 

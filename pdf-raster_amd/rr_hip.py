@@ -57,6 +57,20 @@ class FillPages(ctypes.Structure):
                 ("variant", ctypes.c_int32)]
 
 
+class ImageDesc(ctypes.Structure):
+    _fields_ = [("pixels", ctypes.c_void_p), ("smask", ctypes.c_void_p),
+                ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("channels", ctypes.c_int32), ("blend", ctypes.c_int32),
+                ("ctm", ctypes.c_double * 6), ("page", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class ImageBatch(ctypes.Structure):
+    _fields_ = [("images", ctypes.c_void_p), ("n_images", ctypes.c_int32),
+                ("pages", ctypes.c_void_p), ("n_pages", ctypes.c_int32),
+                ("grid4", ctypes.c_void_p), ("grid4_n", ctypes.c_int32),
+                ("grid3", ctypes.c_void_p), ("grid3_n", ctypes.c_int32)]
+
+
 _lib = None
 
 
@@ -101,6 +115,16 @@ def lib():
             "rr_workload_n_pages": (i32, [P]),
             "rr_workload_page_info": (i32, [P, i32, P]),
             "rr_workload_page_data": (i32, [P, i32] + [ctypes.POINTER(ctypes.c_void_p)] * 7),
+            "rr_hip_record_place_images": (i32, [P, ctypes.POINTER(ImageBatch)]),
+            "rr_hip_record_icc_clut": (i32, [P, P, P, P, i32, i64]),
+            "rr_hip_record_composite": (i32, [P, P, P, i64]),
+            "rr_hip_record_apply_soft_mask": (i32, [P, P, P, i64]),
+            "rr_images_create": (i32, [u64, i32, i32, ctypes.POINTER(ctypes.c_void_p)]),
+            "rr_images_destroy": (None, [P]),
+            "rr_images_info": (i32, [P, P]),
+            "rr_images_get": (i32, [P, i32, ctypes.POINTER(ImageDesc)]),
+            "rr_images_grids": (i32, [P, ctypes.POINTER(ctypes.c_void_p),
+                                      ctypes.POINTER(ctypes.c_void_p)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -227,6 +251,55 @@ def workload(config, seed, max_pages=0):
         L.rr_workload_destroy(h)
 
 
+class Image:
+    """One host image of the image workload (see rr_images_create)."""
+
+    def __init__(self, pixels, smask, channels, blend, ctm, page):
+        self.pixels, self.smask = pixels, smask
+        self.height, self.width = pixels.shape[:2]
+        self.channels, self.blend, self.ctm, self.page = channels, blend, ctm, page
+
+
+class ImageSet:
+    """Seeded image workload: images, page geometry and the two grey grids."""
+
+    def __init__(self, images, n_pages, page_w, page_h, grid4, grid4_n, grid3, grid3_n):
+        self.images, self.n_pages, self.page_w, self.page_h = images, n_pages, page_w, page_h
+        self.grid4, self.grid4_n, self.grid3, self.grid3_n = grid4, grid4_n, grid3, grid3_n
+
+
+def images(seed, n_images, images_per_page):
+    """Generate the seeded image workload with the library's own generator."""
+    L = lib()
+    h = ctypes.c_void_p()
+    rc = L.rr_images_create(seed, n_images, images_per_page, ctypes.byref(h))
+    if rc:
+        raise RrError(rc, "rr_images_create")
+    try:
+        info = np.zeros(6, np.int32)
+        L.rr_images_info(h, _p(info))
+        n, n_pages, pw, ph, g4, g3 = (int(v) for v in info)
+        out = []
+        for i in range(n):
+            d = ImageDesc()
+            L.rr_images_get(h, i, ctypes.byref(d))
+            npx = d.width * d.height
+            px = np.frombuffer((ctypes.c_uint8 * (npx * d.channels)).from_address(d.pixels),
+                               np.uint8).copy().reshape(d.height, d.width, d.channels)
+            sm = None
+            if d.smask:
+                sm = np.frombuffer((ctypes.c_uint8 * npx).from_address(d.smask),
+                                   np.uint8).copy().reshape(d.height, d.width)
+            out.append(Image(px, sm, d.channels, d.blend, tuple(d.ctm), d.page))
+        p4, p3 = ctypes.c_void_p(), ctypes.c_void_p()
+        L.rr_images_grids(h, ctypes.byref(p4), ctypes.byref(p3))
+        grid4 = np.frombuffer((ctypes.c_uint16 * g4 ** 4).from_address(p4.value), np.uint16).copy()
+        grid3 = np.frombuffer((ctypes.c_uint16 * g3 ** 3).from_address(p3.value), np.uint16).copy()
+        return ImageSet(out, n_pages, pw, ph, grid4, g4, grid3, g3)
+    finally:
+        L.rr_images_destroy(h)
+
+
 class Backend:
     """Mirror of GpuBackend (crates/gpu/src/backend/mod.rs:216-503) over the C-ABI."""
 
@@ -294,6 +367,21 @@ class Backend:
     def record_fill_pages(self, batch):
         self._check(self.L.rr_hip_record_fill_pages(self.h, ctypes.byref(batch)),
                     "record_fill_pages")
+
+    def record_place_images(self, batch):
+        self._check(self.L.rr_hip_record_place_images(self.h, ctypes.byref(batch)),
+                    "record_place_images")
+
+    def record_icc_clut(self, cmyk, rgb, clut, grid_n, n_pixels):
+        self._check(self.L.rr_hip_record_icc_clut(self.h, cmyk, rgb, clut, grid_n, n_pixels),
+                    "record_icc_clut")
+
+    def record_composite(self, src, dst, n_pixels):
+        self._check(self.L.rr_hip_record_composite(self.h, src, dst, n_pixels), "record_composite")
+
+    def record_apply_soft_mask(self, pixels, mask, n_pixels):
+        self._check(self.L.rr_hip_record_apply_soft_mask(self.h, pixels, mask, n_pixels),
+                    "record_apply_soft_mask")
 
     def submit_page(self):
         f = ctypes.c_uint64(0)
@@ -491,3 +579,66 @@ class DeviceBatch:
         out = np.zeros((max(self.n_paths, 1), 2), np.uint64)
         self.be.download(self.counts, out)
         return out[:self.n_paths].copy()
+
+
+class DeviceImages:
+    """An image set resident on the device: pixels, masks, grids and one grey
+    canvas per page."""
+
+    def __init__(self, be, iset, n_pages=None):
+        self.be, self.iset = be, iset
+        self.n_pages = iset.n_pages if n_pages is None else n_pages
+        self.stride = stride_for(iset.page_w)
+        self.canvases = [be.alloc(self.stride * iset.page_h) for _ in range(self.n_pages)]
+        self.grid4 = be.to_device(iset.grid4)
+        self.grid3 = be.to_device(iset.grid3)
+        self.bufs = []
+        self.descs = (ImageDesc * max(len(iset.images), 1))()
+        for i, im in enumerate(iset.images):
+            d = self.descs[i]
+            d.pixels = be.to_device(im.pixels)
+            self.bufs.append(d.pixels)
+            if im.smask is not None:
+                d.smask = be.to_device(im.smask)
+                self.bufs.append(d.smask)
+            d.width, d.height, d.channels, d.blend = im.width, im.height, im.channels, im.blend
+            for k, v in enumerate(im.ctm):
+                d.ctm[k] = v
+            d.page = im.page
+        self.pages = (PageDesc * max(self.n_pages, 1))()
+        for i in range(self.n_pages):
+            pd = self.pages[i]
+            pd.canvas, pd.width, pd.height, pd.stride = (self.canvases[i], iset.page_w,
+                                                         iset.page_h, self.stride)
+
+    def free(self):
+        for b in self.bufs + self.canvases + [self.grid4, self.grid3]:
+            self.be.free(b)
+
+    def batch(self, first=0, count=None):
+        b = ImageBatch()
+        n = len(self.iset.images) - first if count is None else count
+        b.images = ctypes.addressof(self.descs) + first * ctypes.sizeof(ImageDesc)
+        b.n_images = n
+        b.pages = ctypes.cast(self.pages, ctypes.c_void_p)
+        b.n_pages = self.n_pages
+        b.grid4, b.grid4_n = self.grid4, self.iset.grid4_n
+        b.grid3, b.grid3_n = self.grid3, self.iset.grid3_n
+        return b
+
+    def reset(self, background=255):
+        L, h = self.be.L, self.be.h
+        for c in self.canvases:
+            self.be._check(L.rr_hip_memset(h, c, background, self.stride * self.iset.page_h),
+                           "memset")
+
+    def run(self, background=255):
+        self.reset(background)
+        self.be.begin_page()
+        self.be.record_place_images(self.batch())
+        self.be.wait_page(self.be.submit_page())
+
+    def read_canvas(self, i):
+        out = np.zeros((self.iset.page_h, self.stride), np.uint8)
+        self.be.download(self.canvases[i], out)
+        return out[:, :self.iset.page_w].copy()

@@ -1,0 +1,180 @@
+"""Image placement + soft mask + Normal/Multiply through the grey CLUT, and the
+reference's elementwise GPU ops (ICC CLUT / matrix, RGBA composite, soft mask),
+on the device through the C-ABI, bit-exact against the oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import rr_hip as R
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_pages(iset, n_pages=None, background=255):
+    n_pages = iset.n_pages if n_pages is None else n_pages
+    canv = [np.full((iset.page_h, iset.page_w), background, np.uint8) for _ in range(n_pages)]
+    for im in iset.images:
+        grid, g = (iset.grid4, iset.grid4_n) if im.channels == 4 else (iset.grid3, iset.grid3_n)
+        assert O.place_image(canv[im.page], im.pixels, im.smask, im.ctm, im.blend, grid, g) >= 0
+    return canv
+
+
+def _compare(got, want, what):
+    if not np.array_equal(got, want):
+        ys, xs = np.nonzero(got != want)
+        raise AssertionError(f"{what}: {len(ys)} px differ, first y={ys[0]} x={xs[0]} "
+                             f"got {got[ys[0], xs[0]]} want {want[ys[0], xs[0]]}")
+
+
+def test_image_workload_pages(backend):
+    iset = R.images(7, 8, 4)   # CMYK/Normal and RGB/Multiply alternate, 2 pages
+    di = R.DeviceImages(backend, iset)
+    try:
+        di.run()
+        got = [di.read_canvas(i) for i in range(iset.n_pages)]
+    finally:
+        di.free()
+    for i, (g, w) in enumerate(zip(got, _oracle_pages(iset))):
+        _compare(g, w, f"page {i}")
+
+
+def _single(be, px, smask, ctm, blend, grid, g, pw, ph, background):
+    """Place one hand-made image on a pw x ph canvas on the device."""
+    iset = R.ImageSet([R.Image(px, smask, px.shape[2], blend, ctm, 0)], 1, pw, ph,
+                      grid if px.shape[2] == 4 else np.zeros(16, np.uint16), g if px.shape[2] == 4 else 2,
+                      grid if px.shape[2] == 3 else np.zeros(8, np.uint16), g if px.shape[2] == 3 else 2)
+    di = R.DeviceImages(be, iset)
+    try:
+        di.reset(0)
+        be.upload(di.canvases[0], np.pad(background, ((0, 0), (0, di.stride - pw))))
+        be.begin_page()
+        be.record_place_images(di.batch())
+        be.wait_page(be.submit_page())
+        return di.read_canvas(0)
+    finally:
+        di.free()
+
+
+@pytest.mark.parametrize("channels,blend,masked", [(3, 0, False), (3, 1, True), (4, 0, True), (4, 1, False)])
+def test_quarter_turn_and_ragged_edges(backend, channels, blend, masked):
+    # Canvas width 157 (not a multiple of 4) and a bbox starting at x = 61, so
+    # the dword read-modify-write covers partial words on both sides.
+    rng = np.random.default_rng(50 + channels + 2 * blend)
+    W, H, pw, ph = 64, 32, 157, 120
+    px = rng.integers(0, 256, (H, W, channels)).astype(np.uint8)
+    sm = rng.integers(0, 256, (H, W)).astype(np.uint8) if masked else None
+    g = 17 if channels == 4 else 33
+    grid = rng.integers(0, 65536, g ** channels).astype(np.uint16)
+    ctm = (0.0, float(W), -float(H), 0.0, 93.0, 21.0)
+    bg = rng.integers(0, 256, (ph, pw)).astype(np.uint8)
+    got = _single(backend, px, sm, ctm, blend, grid, g, pw, ph, bg)
+    want = bg.copy()
+    O.place_image(want, px, sm, ctm, blend, grid, g)
+    _compare(got, want, "quarter turn")
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_random_small_affine(backend, seed):
+    # Small images under random rotations/scales, partly off the canvas.
+    rng = np.random.default_rng(100 + seed)
+    pw, ph = 301, 233
+    for k in range(6):
+        ch = 4 if k % 2 == 0 else 3
+        W, H = (int(v) for v in rng.integers(5, 90, 2))
+        px = rng.integers(0, 256, (H, W, ch)).astype(np.uint8)
+        sm = rng.integers(0, 256, (H, W)).astype(np.uint8) if k % 3 else None
+        th = np.deg2rad(rng.uniform(3, 80))
+        s = rng.uniform(0.5, 3.0)
+        a, b, c, d = s * W * np.cos(th), s * W * np.sin(th), -s * H * np.sin(th), s * H * np.cos(th)
+        e, f = rng.uniform(-50, pw), rng.uniform(-50, ph)
+        ctm = (a, b, c, d, e, f)
+        g = 17 if ch == 4 else 33
+        grid = rng.integers(0, 65536, g ** ch).astype(np.uint16)
+        bg = rng.integers(0, 256, (ph, pw)).astype(np.uint8)
+        got = _single(backend, px, sm, ctm, k % 2, grid, g, pw, ph, bg)
+        want = bg.copy()
+        assert O.place_image(want, px, sm, ctm, k % 2, grid, g) >= 0
+        _compare(got, want, f"seed {seed} image {k}")
+
+
+def test_axis_aligned_ctm_is_rejected(backend):
+    px = np.zeros((8, 8, 3), np.uint8)
+    with pytest.raises(R.RrError) as ei:
+        _single(backend, px, None, (32.0, 0.0, 0.0, 32.0, 4.0, 4.0), 0, np.zeros(33 ** 3, np.uint16),
+                33, 64, 64, np.zeros((64, 64), np.uint8))
+    assert "axis-aligned" in str(ei.value)
+    backend.wait_page(backend.submit_page())  # close the page left open by the failed record
+
+
+# ---- elementwise reference ops ----
+
+def _run(be, fn):
+    be.begin_page()
+    fn()
+    be.wait_page(be.submit_page())
+
+
+@pytest.mark.parametrize("g", [2, 17, 33])
+def test_icc_clut(backend, g):
+    rng = np.random.default_rng(60 + g)
+    n = 100_003
+    cmyk = rng.integers(0, 256, (n, 4)).astype(np.uint8)
+    table = rng.integers(0, 256, g ** 4 * 3).astype(np.uint8)
+    d_in, d_t = backend.to_device(cmyk), backend.to_device(table)
+    d_out = backend.alloc(n * 3)
+    try:
+        _run(backend, lambda: backend.record_icc_clut(d_in, d_out, d_t, g, n))
+        got = backend.download(d_out, np.zeros((n, 3), np.uint8))
+    finally:
+        for b in (d_in, d_t, d_out):
+            backend.free(b)
+    want = O.icc_clut(cmyk, table, g)
+    assert np.array_equal(got, want), np.argwhere(got != want)[:5]
+
+
+def test_cmyk_matrix(backend):
+    a = np.arange(256, dtype=np.uint8)
+    rng = np.random.default_rng(61)
+    cmyk = np.concatenate([np.stack([a, a[::-1], a, a[::-1]], 1),
+                           rng.integers(0, 256, (77_777, 4)).astype(np.uint8)])
+    n = len(cmyk)
+    d_in, d_out = backend.to_device(cmyk), backend.alloc(n * 3)
+    try:
+        _run(backend, lambda: backend.record_icc_clut(d_in, d_out, None, 0, n))
+        got = backend.download(d_out, np.zeros((n, 3), np.uint8))
+    finally:
+        backend.free(d_in)
+        backend.free(d_out)
+    assert np.array_equal(got, O.cmyk_matrix(cmyk))
+
+
+def test_composite_and_soft_mask(backend):
+    rng = np.random.default_rng(62)
+    n = 123_457
+    src = rng.integers(0, 256, (n, 4)).astype(np.uint8)
+    src[::7, 3] = 0
+    src[1::7, 3] = 255
+    dst = rng.integers(0, 256, (n, 4)).astype(np.uint8)
+    mask = rng.integers(0, 256, n).astype(np.uint8)
+    d_src, d_dst, d_m = backend.to_device(src), backend.to_device(dst), backend.to_device(mask)
+    try:
+        _run(backend, lambda: backend.record_composite(d_src, d_dst, n))
+        got = backend.download(d_dst, np.zeros((n, 4), np.uint8))
+        _run(backend, lambda: backend.record_apply_soft_mask(d_dst, d_m, n))
+        got2 = backend.download(d_dst, np.zeros((n, 4), np.uint8))
+    finally:
+        for b in (d_src, d_dst, d_m):
+            backend.free(b)
+    want = dst.copy()
+    O.composite_rgba8(src, want)
+    assert np.array_equal(got, want)
+    O.apply_soft_mask(want, mask)
+    assert np.array_equal(got2, want)
+
+
+def test_elementwise_empty_is_noop(backend):
+    _run(backend, lambda: (backend.record_composite(None, None, 0),
+                           backend.record_apply_soft_mask(None, None, 0),
+                           backend.record_icc_clut(None, None, None, 0, 0)))
