@@ -254,6 +254,14 @@ int32_t rr_hip_record_icc_clut(rr_hip_ctx *ctx, const uint8_t *cmyk, uint8_t *rg
 int32_t rr_hip_record_composite(rr_hip_ctx *ctx, const uint8_t *src, uint8_t *dst, int64_t n_pixels);
 int32_t rr_hip_record_apply_soft_mask(rr_hip_ctx *ctx, uint8_t *pixels, const uint8_t *mask, int64_t n_pixels);
 
+/* In-place exclusive prefix sum of len_elems uint32 (wrapping), replacing the
+ * three record_scan phases of the reference (GpuBackend::record_scan,
+ * crates/gpu/src/backend/mod.rs:338-353; ScanParams, params.rs:207-367;
+ * crates/gpu/kernels/blelloch_scan.cu): one call records all three.
+ * block_sums: device scratch of ceil(len_elems / 1024) uint32.  len_elems == 0
+ * is an invariant violation, as in ScanParams::validate. */
+int32_t rr_hip_record_scan(rr_hip_ctx *ctx, uint32_t *data, uint32_t *block_sums, uint32_t len_elems);
+
 /* Seeded image workload: n_images images, images_per_page per 2550x3300 page,
  * image i is CMYK + Normal when i is even, RGB + Multiply when odd.  info =
  * {n_images, n_pages, page_w, page_h, grid4_n, grid3_n}; rr_images_get fills a

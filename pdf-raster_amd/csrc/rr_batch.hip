@@ -1,0 +1,957 @@
+// rr_batch.hip -- the batched (many pages, one recording) fill + composite,
+// in two phases, and the device exclusive scan it is built on.
+//
+// Phase A (spans): one work item per (path, 16-row band) the path draws.
+//   Persistent waves pull items in any order; each builds the band's 64 AA
+//   row entry lists and winding walk exactly as the plain route does
+//   (scanner/mod.rs:511-642, render_aa_line :393-467) and stores the result,
+//   per AA row, as disjoint sorted intervals [s, e) in a span pool.  Items are
+//   independent, so the load balances over the whole batch.
+// Phase B (composite): one work item per (page, band, x tile of CH pixels).
+//   The wave keeps the tile's 16 x CH canvas bytes in LDS, walks the band's
+//   path list in path order, turns each path's stored intervals into the
+//   4x4 coverage bitmap of the tile and composites source-over in LDS
+//   (fill/mod.rs:344-401, pipe/aa.rs:192-208, simd/composite.rs:56-114), then
+//   writes the tile back once.  Tiles are disjoint, so per-pixel path order
+//   holds without inter-wave ordering.
+// The per-band path lists come from a device stable bucketing of the span
+// items: count per band, exclusive scan, and an in-order ballot sweep per band.
+//
+// The scan (three kernels: tile sums, scan of the sums, tile rescan) replaces
+// the reference's Blelloch scan (crates/gpu/kernels/blelloch_scan.cu) with
+// wave64 shuffle scans: 256 threads x 4 elements = 1024-element tiles.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "rr_hip.h"
+#include "rr_internal.h"
+
+using namespace rrdev;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// exclusive scan
+// ---------------------------------------------------------------------------
+constexpr int kScanThreads = 256, kScanPer = 4, kScanTile = kScanThreads * kScanPer;
+
+template <typename T>
+__device__ __forceinline__ T wave_incl(T v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const T t = __shfl_up(v, o);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan of one value per thread (256 threads); *total gets
+// the block sum.  ws: 4 LDS slots.
+template <typename T>
+__device__ __forceinline__ T block_excl(T v, T *ws, T *total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const T inc = wave_incl(v);
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    T base = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kScanThreads / 64; ++k) {
+        if (k < w) base += ws[k];
+        tot += ws[k];
+    }
+    __syncthreads();
+    *total = tot;
+    return base + inc - v;
+}
+
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const Tin *__restrict__ in, long long n,
+                                                              Tout *__restrict__ sums) {
+    __shared__ Tout ws[kScanThreads / 64];
+    const long long i0 = static_cast<long long>(blockIdx.x) * kScanTile + kScanPer * threadIdx.x;
+    Tout v = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k)
+        if (i0 + k < n) v += static_cast<Tout>(in[i0 + k]);
+    Tout tot;
+    (void)block_excl(v, ws, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// One block: exclusive scan of the tile sums in place, 1024 at a time with a
+// carry; *total (optional) receives the grand total.
+template <typename Tout>
+__global__ void __launch_bounds__(kScanThreads) k_scan_sums(Tout *__restrict__ sums, long long nb, Tout *total) {
+    __shared__ Tout ws[kScanThreads / 64];
+    Tout carry = 0;
+    for (long long base = 0; base < nb; base += kScanTile) {
+        const long long i0 = base + kScanPer * threadIdx.x;
+        Tout v[kScanPer];
+        Tout s = 0;
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            v[k] = i0 + k < nb ? sums[i0 + k] : static_cast<Tout>(0);
+            s += v[k];
+        }
+        Tout tot;
+        Tout run = carry + block_excl(s, ws, &tot);
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            if (i0 + k < nb) sums[i0 + k] = run;
+            run += v[k];
+        }
+        carry += tot;
+    }
+    if (total && threadIdx.x == 0) *total = carry;
+}
+
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(kScanThreads) k_scan_down(const Tin *in, Tout *out, long long n,
+                                                            const Tout *__restrict__ sums) {
+    __shared__ Tout ws[kScanThreads / 64];
+    const long long i0 = static_cast<long long>(blockIdx.x) * kScanTile + kScanPer * threadIdx.x;
+    Tout v[kScanPer];
+    Tout s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        v[k] = i0 + k < n ? static_cast<Tout>(in[i0 + k]) : static_cast<Tout>(0);
+        s += v[k];
+    }
+    Tout tot;
+    Tout run = sums[blockIdx.x] + block_excl(s, ws, &tot);
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        if (i0 + k < n) out[i0 + k] = run;
+        run += v[k];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill(T *__restrict__ p, long long n, T value) {
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += static_cast<long long>(gridDim.x) * 256)
+        p[i] = value;
+}
+
+// ---------------------------------------------------------------------------
+// span items and per-band path lists
+// ---------------------------------------------------------------------------
+struct BatchArgs {
+    FillArgs F;
+    int4 *sel;
+    const long long *item_base;
+    long long n_paths;
+    const int2 *sitems;
+    long long sitems_cap;
+    SpanHdr *shdr;
+    int2 *spans;
+    long long spans_cap;
+    const long long *band_base;
+    const int2 *band_list;
+    const CPage *cpages;
+    int32_t n_cpages;
+    unsigned long long *counters;
+};
+
+// Span items of path p: (p, first row of each band it draws), in band order,
+// and the per-band path counts.  One wave per path.
+__global__ void __launch_bounds__(256) k_expand(const int4 *__restrict__ sel, const int32_t *__restrict__ nb,
+                                                const PathHdr *__restrict__ hdrs, const DevPage *__restrict__ pages,
+                                                const long long *__restrict__ item_base, long long n_paths,
+                                                int2 *__restrict__ sitems, long long sitems_cap,
+                                                int32_t *__restrict__ band_cnt, int32_t *__restrict__ err) {
+    const long long p = static_cast<long long>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= n_paths) return;
+    const int32_t n = nb[p];
+    if (n == 0) return;
+    const long long base = item_base[p];
+    const int32_t b0 = sel[p].x / kBandRows;
+    const long long gb0 = static_cast<long long>(pages[hdrs[p].page].band_base) + b0;
+    for (int32_t k = lane; k < n; k += 64) {
+        const long long it = base + k;
+        if (it >= sitems_cap) {
+            atomicOr(err, 2);
+            break;
+        }
+        sitems[it] = make_int2(static_cast<int32_t>(p), kBandRows * (b0 + k));
+        atomicAdd(band_cnt + gb0 + k, 1);
+    }
+}
+
+// Band gb's list of (path, span item) in path order: an in-order sweep of its
+// page's paths, compacted with ballots.  One wave per band.
+__global__ void __launch_bounds__(64) k_band_lists(const int4 *__restrict__ sel, const DevPage *__restrict__ pages,
+                                                   int32_t n_pages, const long long *__restrict__ item_base,
+                                                   const long long *__restrict__ band_base,
+                                                   int2 *__restrict__ band_list, long long list_cap,
+                                                   int32_t *__restrict__ err) {
+    const long long gb = blockIdx.x;
+    const int lane = threadIdx.x;
+    int32_t lo = 0, hi = n_pages - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) / 2;
+        if (pages[mid].band_base <= gb)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const DevPage &pg = pages[lo];
+    const int32_t band = static_cast<int32_t>(gb - pg.band_base);
+    const int32_t Y0 = kBandRows * band, Y1 = Y0 + kBandRows - 1;
+    long long w = band_base[gb];
+    const long long wend = band_base[gb + 1];
+    const long long p_end = pg.path_end;
+    for (long long base = pg.path_begin; base < p_end; base += 64) {
+        const long long q = base + lane;
+        bool ov = false;
+        int4 s = make_int4(1, 0, 1, 0);
+        if (q < p_end) {
+            s = sel[q];
+            ov = s.x <= s.y && s.x <= Y1 && s.y >= Y0;
+        }
+        const unsigned long long m = __ballot(ov);
+        if (ov) {
+            const long long pos = w + __popcll(m & ((1ull << lane) - 1ull));
+            if (pos < wend && pos < list_cap)
+                band_list[pos] = make_int2(static_cast<int32_t>(q),
+                                           static_cast<int32_t>(item_base[q] + (band - s.x / kBandRows)));
+            else
+                atomicOr(err, 4);
+        }
+        w += __popcll(m);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// phase A: spans of one (path, band)
+// ---------------------------------------------------------------------------
+template <int K>
+struct SpanLds {
+    EdgeRec rec[64];
+    unsigned long long smask[64];
+    unsigned long long emask[64];
+    int32_t ex0[K * 64];
+    int32_t ex1[K * 64];
+    int32_t ec[K * 64];
+};
+
+// Per-wave sub-allocator over the span pool: one global atomic per chunk of
+// kSpanChunk records instead of one per item.
+constexpr long long kSpanChunk = 8192;
+struct SpanCursor {
+    unsigned long long next = 0, end = 0;
+};
+
+template <int K>
+__device__ void span_band(const BatchArgs &B, SpanLds<K> &L, int64_t p, int32_t Y0, long long it, int64_t slot,
+                          bool count_draw, SpanCursor &cur) {
+    const FillArgs &A = B.F;
+    const int lane = threadIdx.x;
+    const PathHdr h = A.hdrs[p];
+    const int4 s = B.sel[p];
+    const rr_path_attr at = A.attrs[p];
+    const int64_t e0 = A.edge_off[p], e1 = A.edge_off[p + 1];
+    const DevPage &pg = A.pages[h.page];
+    const bool literal = A.mode == RR_MODE_LITERAL;
+
+    const int32_t d = lane >> 2, j = lane & 3;
+    const int32_t y = Y0 + d;
+    const int32_t r = literal ? 16 * y + 12 + j : 4 * y + j;
+    const int32_t band_lo = literal ? 16 * Y0 + 12 : 4 * Y0;
+    const int32_t band_hi = literal ? 16 * (Y0 + kBandRows - 1) + 15 : 4 * Y0 + 4 * kBandRows - 1;
+    const bool valid = y >= s.x && y <= s.y && r >= h.y_min && r <= h.y_max;
+    const int32_t mrule = at.eo ? 1 : -1;
+    const int32_t aa_width = pg.width * 4;
+
+    SpanHdr *H = B.shdr + it;
+    int32_t *X0 = L.ex0, *X1 = L.ex1, *C = L.ec;
+    int n = build_entries(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, K);
+    int need = n;
+    for (int o = 32; o > 0; o >>= 1) need = max(need, __shfl_xor(need, o));
+    if (need > K) {
+        // more state-changing entries on a row than the LDS holds: redo the
+        // band with the entry lists in this wave's global slot or the pool
+        int32_t *G;
+        if (static_cast<long long>(need) * 192 <= A.slot_ints) {
+            G = A.slots + slot * A.slot_ints;
+        } else {
+            unsigned long long off = 0;
+            if (lane == 0) off = atomicAdd(A.spill_top, static_cast<unsigned long long>(need) * 192ull);
+            off = __shfl(off, 0);
+            if (static_cast<long long>(off) + static_cast<long long>(need) * 192 > A.spill_cap) {
+                if (lane == 0) {
+                    atomicOr(A.err, 1);
+                    H->off = -1;
+                }
+                H->cnt[lane] = 0;
+                return;
+            }
+            G = A.spill + off;
+        }
+        X0 = G;
+        X1 = G + static_cast<long long>(need) * 64;
+        C = G + static_cast<long long>(need) * 128;
+        n = build_entries(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, need);
+    }
+    bool unb = false;
+    const int nint = sort_and_walk(n, X0, X1, C, mrule, aa_width, &unb);
+    // an unbalanced row spans to the canvas edge: widen the path's columns
+    if (__ballot(unb) != 0 && lane == 0) atomicMax(reinterpret_cast<int32_t *>(B.sel + p) + 3, pg.px_hi);
+    if (A.cov_count && count_draw) {
+        // draw_aa_line range of each output row (scanner/mod.rs:459-466)
+        int32_t smin = nint ? X0[lane] : INT32_MAX;
+        int32_t emax = nint ? X1[(nint - 1) * 64 + lane] : -1;
+        smin = min(smin, __shfl_xor(smin, 1));
+        smin = min(smin, __shfl_xor(smin, 2));
+        emax = max(emax, __shfl_xor(emax, 1));
+        emax = max(emax, __shfl_xor(emax, 2));
+        unsigned long long w_draw = 0;
+        if (j == 0 && y >= s.x && y <= s.y && smin < emax) {
+            const int32_t x0 = max(smin / 4, pg.px_lo), x1 = min((emax - 1) / 4, pg.px_hi);
+            if (x0 <= x1) w_draw = static_cast<unsigned long long>(x1 - x0 + 1);
+        }
+        for (int o = 32; o > 0; o >>= 1) w_draw += __shfl_xor(w_draw, o);
+        if (lane == 0 && w_draw) atomicAdd(reinterpret_cast<unsigned long long *>(A.cov_count + 2 * p + 1), w_draw);
+    }
+    // store the intervals: lane-major, one int2 per interval
+    const int32_t incl = wave_incl(nint);
+    const int32_t excl = incl - nint;
+    const int32_t total = __shfl(incl, 63);
+    if (cur.next + static_cast<unsigned long long>(total) > cur.end) {
+        const unsigned long long want =
+            static_cast<unsigned long long>(total > kSpanChunk ? total : kSpanChunk);
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(B.counters + 2, want);
+        base = __shfl(base, 0);
+        cur.next = base;
+        cur.end = base + want;
+    }
+    const unsigned long long off = cur.next;
+    cur.next += static_cast<unsigned long long>(total);
+    const bool fits = static_cast<long long>(off) + total <= B.spans_cap;
+    if (fits) {
+        int2 *dst = B.spans + off + excl;
+        for (int k = 0; k < nint; ++k) dst[k] = make_int2(X0[k * 64 + lane], X1[k * 64 + lane]);
+    }
+    H->cnt[lane] = fits ? static_cast<uint16_t>(nint) : static_cast<uint16_t>(0);
+    if (lane == 0) H->off = fits ? static_cast<long long>(off) : -1ll;
+}
+
+constexpr int kSpanGrab = 8;  // span items taken per queue atomic
+
+template <int K>
+__global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t count_draw) {
+    __shared__ SpanLds<K> L;
+    const int lane = threadIdx.x;
+    const long long n_items = min(B.item_base[B.n_paths], B.sitems_cap);
+    SpanCursor cur;
+    for (;;) {
+        // items are pulled kSpanGrab at a time: one queue atomic per group
+        unsigned long long it0 = 0;
+        if (lane == 0) it0 = atomicAdd(B.counters, static_cast<unsigned long long>(kSpanGrab));
+        it0 = __shfl(it0, 0);
+        if (static_cast<long long>(it0) >= n_items) break;
+        const long long it_end = min(static_cast<long long>(it0) + kSpanGrab, n_items);
+        for (long long it = static_cast<long long>(it0); it < it_end; ++it) {
+            const int2 item = B.sitems[it];
+            long long t0 = 0;
+            if (B.F.trace) t0 = static_cast<long long>(wall_clock64());
+            span_band<K>(B, L, item.x, item.y, it, blockIdx.x, count_draw != 0, cur);
+            if (B.F.trace && lane == 0) {
+                const long long t1 = static_cast<long long>(wall_clock64());
+                const unsigned long long k = atomicAdd(B.F.trace_top, 1ull);
+                if (static_cast<long long>(k) < B.F.trace_cap)
+                    B.F.trace[k] = make_longlong4(item.x, (static_cast<long long>(B.F.hdrs[item.x].page) << 32) |
+                                                              static_cast<uint32_t>(item.y), t0, t1);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// phase B: composite of one (page, band, tile)
+// ---------------------------------------------------------------------------
+template <int CH>
+struct ComposeLds {
+    static constexpr int kWords = CH / 8;  // bitmap dwords per AA row (4 bits per pixel)
+    static constexpr int kPitch = kWords + 1;
+    static constexpr int kRowWords = CH / 4;  // canvas dwords per output row
+    uint32_t cv[kBandRows * kRowWords];
+    uint32_t bm[64 * kPitch];
+    uint8_t gamma[32];
+};
+
+// One path of a band list as the composite pass consumes it.
+struct PathRef {
+    int64_t p;
+    int32_t hi;  // span item
+    int4 s;      // {first row, last row, first px, last px}
+};
+
+// Walks a band list in order, 64 entries per chunk, yielding the paths whose
+// columns meet [tx0, txe].
+struct ListCursor {
+    long long lb, l1;
+    int2 ent;
+    int4 s;
+    unsigned long long mask;
+};
+
+__device__ __forceinline__ bool list_next(const BatchArgs &B, const DevPage &pg, ListCursor &c, int32_t tx0,
+                                          int32_t txe, PathRef &out) {
+    const int lane = threadIdx.x;
+    while (c.mask == 0) {
+        c.lb += 64;
+        if (c.lb >= c.l1) return false;
+        c.ent = make_int2(0, 0);
+        c.s = make_int4(1, 0, 1, 0);
+        bool ov = false;
+        if (c.lb + lane < c.l1) {
+            c.ent = B.band_list[c.lb + lane];
+            if (c.ent.x >= pg.path_begin && c.ent.x < pg.path_end && c.ent.y >= 0 && c.ent.y < B.sitems_cap) {
+                c.s = B.sel[c.ent.x];
+                ov = c.s.z <= c.s.w && c.s.z <= txe && c.s.w >= tx0;
+            } else {
+                atomicOr(B.F.err, 16);  // a band list entry that was never written
+            }
+        }
+        c.mask = __ballot(ov);
+    }
+    const int jj = __ffsll(c.mask) - 1;
+    c.mask &= c.mask - 1;
+    out.p = __shfl(c.ent.x, jj);
+    out.hi = __shfl(c.ent.y, jj);
+    out.s = make_int4(__shfl(c.s.x, jj), __shfl(c.s.y, jj), __shfl(c.s.z, jj), __shfl(c.s.w, jj));
+    return true;
+}
+
+// Stored spans of one path: the pool offset of this lane's intervals, their
+// count, and the first interval (loaded ahead of use).
+struct SpanView {
+    long long base;
+    int32_t n;
+    int2 v0;
+};
+
+__device__ __forceinline__ void hdr_fetch(const BatchArgs &B, int32_t hi, long long *off, int32_t *n) {
+    const SpanHdr *H = B.shdr + __builtin_amdgcn_readfirstlane(hi);
+    *off = H->off;
+    *n = H->cnt[threadIdx.x];
+}
+
+__device__ __forceinline__ SpanView span_fetch(const BatchArgs &B, long long off, int32_t n) {
+    SpanView v;
+    const int32_t excl = wave_incl(n) - n;
+    v.base = off + excl;
+    v.n = off < 0 ? 0 : n;
+    v.v0 = make_int2(INT32_MAX, INT32_MAX);
+    if (v.n > 0) v.v0 = B.spans[v.base];
+    return v;
+}
+
+// Source-over of one pixel (pipe/aa.rs:192-208 Gray8; simd/composite.rs:56-114
+// Rgb8 fast path, identical per channel on a grey canvas).
+__device__ __forceinline__ uint32_t comp1(uint32_t dv, uint32_t shape, uint32_t a_in, uint32_t src, int flavour) {
+    if (flavour == RR_FLAVOUR_RGB8_FAST) {
+        const uint32_t a = (a_in * shape + 255u) >> 8;
+        return ((255u - a) * dv + a * src + 255u) >> 8;
+    }
+    const uint32_t a = div255(a_in * shape);
+    return div255((255u - a) * dv + a * src);
+}
+
+// Path p over the tile held in LDS.  Lane l holds AA row l of the band; the
+// four lanes of output row d = l/4 work that row together.  A row whose four
+// AA rows each have at most one interval in the tile is done from the
+// interval ends: pixels covered by all four (coverage 16) are written whole
+// dwords at a time, the rest get their coverage count in closed form
+// (sum over AA rows of the interval's overlap with [4x, 4x+4)).  Rows with
+// more intervals go through the 4x4-bit coverage bitmap.  Both give the
+// nibble-popcount coverage of fill/mod.rs:344-401 exactly.
+template <int CH>
+__device__ __forceinline__ void compose_rows(const BatchArgs &B, ComposeLds<CH> &L, const DevPage &pg, int64_t p,
+                                             const SpanView &V, int32_t Y0, int32_t tx0, int32_t xa, int32_t xb,
+                                             int32_t ya, int32_t yb) {
+    const FillArgs &A = B.F;
+    const int lane = threadIdx.x, d = lane >> 2, j = lane & 3, g0 = lane & ~3;
+    constexpr int kPitch = ComposeLds<CH>::kPitch, kRowWords = ComposeLds<CH>::kRowWords;
+    const rr_path_attr at = A.attrs[p];
+    const uint32_t a_in = at.a_input, src = at.color, src4 = src * 0x01010101u;
+    const int flavour = A.flavour;
+    const bool want_cov = A.cov_hash || A.cov_count;
+    uint64_t hsum = 0;
+    unsigned long long w_cov = 0;
+    const int32_t yy = Y0 + d;
+    const bool rowv = yy >= ya && yy <= yb;
+    const int32_t Alo = 4 * xa, Ahi = 4 * (xb + 1);
+    const int32_t n = rowv ? V.n : 0;
+    int32_t s0 = 0, e0 = 0;
+    if (n >= 1) {
+        s0 = max(V.v0.x, Alo);
+        e0 = min(V.v0.y, Ahi);
+        if (e0 <= s0) s0 = e0 = 0;
+    }
+    const int ne = e0 > s0 ? 1 : 0;
+    int cx = n >= 2 ? 1 : 0;
+    cx |= __shfl_xor(cx, 1);
+    cx |= __shfl_xor(cx, 2);
+    int all_ne = ne & __shfl_xor(ne, 1);
+    all_ne &= __shfl_xor(all_ne, 2);
+    int32_t fa = ne ? (s0 + 3) >> 2 : INT32_MIN, fb = ne ? (e0 >> 2) - 1 : INT32_MAX;
+    int32_t ua = ne ? s0 >> 2 : INT32_MAX, ub = ne ? (e0 - 1) >> 2 : INT32_MIN;
+#pragma unroll
+    for (int o = 1; o <= 2; o <<= 1) {
+        fa = max(fa, __shfl_xor(fa, o));
+        fb = min(fb, __shfl_xor(fb, o));
+        ua = min(ua, __shfl_xor(ua, o));
+        ub = max(ub, __shfl_xor(ub, o));
+    }
+    int32_t sk[4], ek[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        sk[k] = __shfl(s0, g0 + k);
+        ek[k] = __shfl(e0, g0 + k);
+    }
+    uint32_t *rowc = L.cv + d * kRowWords;
+    uint8_t *rowb = reinterpret_cast<uint8_t *>(rowc);
+    const int32_t dwbase = tx0 >> 2;
+    const long long pixrow = static_cast<long long>(yy) * pg.width;
+    if (rowv && !cx && ua <= ub) {
+        int32_t dA = 1, dB = 0;  // whole dwords inside the fully covered run
+        if (all_ne && fa <= fb) {
+            dA = (fa + 3) >> 2;
+            dB = ((fb + 1) >> 2) - 1;
+        }
+        for (int32_t w = dA + j; w <= dB; w += 4) {
+            uint32_t *cell = rowc + (w - dwbase);
+            uint32_t out = src4;
+            if (a_in != 255u) {
+                const uint32_t old = *cell;
+                out = 0;
+#pragma unroll
+                for (int px = 0; px < 4; ++px)
+                    out |= comp1((old >> (8 * px)) & 0xffu, 255u, a_in, src, flavour) << (8 * px);
+            }
+            *cell = out;
+            if (want_cov) {
+                w_cov += 4;
+                if (A.cov_hash)
+                    for (int px = 0; px < 4; ++px) hsum += cov_term(pixrow + 4 * w + px, 255u);
+            }
+        }
+        int32_t r1a = ua, r1b = ub, r2a = 1, r2b = 0;
+        if (dA <= dB) {
+            r1b = min(ub, 4 * dA - 1);
+            r2a = max(ua, 4 * (dB + 1));
+            r2b = ub;
+        }
+        const int32_t len1 = max(r1b - r1a + 1, 0), len2 = max(r2b - r2a + 1, 0);
+        for (int32_t k = j; k < len1 + len2; k += 4) {
+            const int32_t x = k < len1 ? r1a + k : r2a + (k - len1);
+            const int32_t X0 = 4 * x, X1 = X0 + 4;
+            int32_t cnt = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cnt += max(0, min(ek[q], X1) - max(sk[q], X0));
+            if (cnt == 0) continue;
+            uint8_t *cb = rowb + (x - tx0);
+            const uint32_t shape = L.gamma[cnt];
+            const uint32_t res = (cnt == 16 && a_in == 255u) ? src : comp1(*cb, shape, a_in, src, flavour);
+            *cb = static_cast<uint8_t>(res);
+            if (want_cov) {
+                ++w_cov;
+                if (A.cov_hash) hsum += cov_term(pixrow + x, shape);
+            }
+        }
+    } else if (rowv && cx) {
+        // bitmap route: each lane sets its AA row's bits over [xa, xb]
+        uint32_t *brow = L.bm + lane * kPitch;
+        const int wa = (xa - tx0) >> 3, wb = (xb - tx0) >> 3;
+        for (int w = wa; w <= wb; ++w) brow[w] = 0u;
+        const int32_t A0 = 4 * tx0;
+        for (int k = 0; k < n; ++k) {
+            const int2 v = k == 0 ? V.v0 : B.spans[V.base + k];
+            if (v.x >= Ahi) break;
+            if (v.y <= Alo) continue;
+            const int32_t lo = max(v.x, Alo) - A0, hi2 = min(v.y, Ahi) - A0;
+            const int wl = lo >> 5, wh = (hi2 - 1) >> 5;
+            const uint32_t ml = ~0u << (lo & 31);
+            const uint32_t mh = (hi2 & 31) ? ((1u << (hi2 & 31)) - 1u) : ~0u;
+            if (wl == wh) {
+                brow[wl] |= ml & mh;
+            } else {
+                brow[wl] |= ml;
+                for (int w = wl + 1; w < wh; ++w) brow[w] = ~0u;
+                brow[wh] |= mh;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        const int32_t dw0 = (xa - tx0) >> 2, dw1 = (xb - tx0) >> 2;
+        for (int32_t w = dw0 + j; w <= dw1; w += 4) {
+            const int sh = (w & 1) * 16;
+            uint32_t cvv = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t f = (L.bm[(g0 + q) * kPitch + (w >> 1)] >> sh) & 0xffffu;
+                uint32_t t = f - ((f >> 1) & 0x5555u);
+                t = (t & 0x3333u) + ((t >> 2) & 0x3333u);
+                cvv += (t & 0xfu) | ((t & 0xf0u) << 4) | ((t & 0xf00u) << 8) | ((t & 0xf000u) << 12);
+            }
+            if (cvv == 0) continue;
+            uint32_t *cell = rowc + w;
+            const uint32_t old = *cell;
+            uint32_t out;
+            if (cvv == 0x10101010u && a_in == 255u) {
+                out = src4;
+            } else {
+                out = old;
+#pragma unroll
+                for (int px = 0; px < 4; ++px) {
+                    const uint32_t cnt = (cvv >> (8 * px)) & 0xffu;
+                    if (cnt == 0) continue;
+                    const uint32_t res = comp1((out >> (8 * px)) & 0xffu, L.gamma[cnt], a_in, src, flavour);
+                    out = (out & ~(0xffu << (8 * px))) | (res << (8 * px));
+                }
+            }
+            if (out != old) *cell = out;
+            if (want_cov) {
+                const long long pix0 = pixrow + tx0 + 4 * w;
+#pragma unroll
+                for (int px = 0; px < 4; ++px) {
+                    const uint32_t cnt = (cvv >> (8 * px)) & 0xffu;
+                    if (cnt == 0) continue;
+                    ++w_cov;
+                    if (A.cov_hash) hsum += cov_term(pix0 + px, L.gamma[cnt]);
+                }
+            }
+        }
+    }
+    if (want_cov) {
+        for (int o = 32; o > 0; o >>= 1) {
+            hsum += __shfl_xor(hsum, o);
+            w_cov += __shfl_xor(w_cov, o);
+        }
+        if (lane == 0) {
+            if (A.cov_hash && hsum) atomicAdd(reinterpret_cast<unsigned long long *>(A.cov_hash + p), hsum);
+            if (A.cov_count && w_cov) atomicAdd(reinterpret_cast<unsigned long long *>(A.cov_count + 2 * p), w_cov);
+        }
+    }
+}
+
+// All paths of band `band` that meet tile `tile` of page `page`, in path
+// order, composited into the tile held in LDS.  Two-deep software pipeline:
+// while path j composites, the first span of path j+1 and the header of path
+// j+2 are in flight.
+template <int CH>
+__device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, int32_t page, int32_t band, int32_t tile) {
+    const FillArgs &A = B.F;
+    const int lane = threadIdx.x;
+    constexpr int kRowWords = ComposeLds<CH>::kRowWords;
+    const DevPage &pg = A.pages[page];
+    const int32_t Y0 = kBandRows * band, tx0 = CH * tile;
+    const int32_t width = pg.width;
+    const int32_t txe = min(tx0 + CH, width) - 1;
+    if (txe < pg.px_lo || tx0 > pg.px_hi) return;
+    const long long gb = static_cast<long long>(pg.band_base) + band;
+    ListCursor cur;
+    cur.lb = B.band_base[gb] - 64;
+    cur.l1 = B.band_base[gb + 1];
+    cur.mask = 0;
+    PathRef P0, P1;
+    if (!list_next(B, pg, cur, tx0, txe, P0)) return;
+    const int32_t nrows = min(kBandRows, pg.height - Y0);
+    const int32_t ndw = ((txe - tx0) >> 2) + 1;
+    uint8_t *const base = pg.canvas + static_cast<long long>(Y0) * pg.stride + tx0;
+    const int32_t stride = pg.stride;
+    long long off0, off1 = -1;
+    int32_t n0, n1 = 0;
+    hdr_fetch(B, P0.hi, &off0, &n0);
+    bool has1 = list_next(B, pg, cur, tx0, txe, P1);
+    if (has1) hdr_fetch(B, P1.hi, &off1, &n1);
+    for (int idx = lane; idx < kBandRows * kRowWords; idx += 64) {
+        const int r = idx / kRowWords, w = idx % kRowWords;
+        if (r < nrows && w < ndw)
+            L.cv[idx] = *reinterpret_cast<const uint32_t *>(base + static_cast<long long>(r) * stride + 4 * w);
+    }
+    SpanView V0 = span_fetch(B, off0, n0);
+    __syncthreads();
+    if (off0 < 0 && lane == 0) atomicOr(A.err, 8);
+    for (;;) {
+        const int32_t xa = max(P0.s.z, tx0), xb = min(P0.s.w, txe);
+        const int32_t ya = max(P0.s.x, Y0), yb = min(P0.s.y, Y0 + nrows - 1);
+        const bool live = xa <= xb && ya <= yb;
+        // next path: its first span; the one after: its header
+        SpanView V1;
+        V1.n = 0;
+        PathRef P2;
+        bool has2 = false;
+        long long off2 = -1;
+        int32_t n2 = 0;
+        if (has1) {
+            V1 = span_fetch(B, off1, n1);
+            if (off1 < 0 && lane == 0) atomicOr(A.err, 8);
+            has2 = list_next(B, pg, cur, tx0, txe, P2);
+            if (has2) hdr_fetch(B, P2.hi, &off2, &n2);
+        }
+        if (live) compose_rows<CH>(B, L, pg, P0.p, V0, Y0, tx0, xa, xb, ya, yb);
+        __syncthreads();
+        if (!has1) break;
+        P0 = P1;
+        V0 = V1;
+        P1 = P2;
+        has1 = has2;
+        off1 = off2;
+        n1 = n2;
+    }
+    for (int idx = lane; idx < kBandRows * kRowWords; idx += 64) {
+        const int r = idx / kRowWords, w = idx % kRowWords;
+        if (r < nrows && w < ndw)
+            *reinterpret_cast<uint32_t *>(base + static_cast<long long>(r) * stride + 4 * w) = L.cv[idx];
+    }
+    __syncthreads();
+}
+
+template <int CH>
+__global__ void __launch_bounds__(64) k_compose(BatchArgs B, long long n_items) {
+    __shared__ ComposeLds<CH> L;
+    if (threadIdx.x < 17) L.gamma[threadIdx.x] = kGamma[threadIdx.x];
+    __syncthreads();
+    const int lane = threadIdx.x;
+    for (;;) {
+        unsigned long long it = 0;
+        if (lane == 0) it = atomicAdd(B.counters + 1, 1ull);
+        it = __shfl(it, 0);
+        if (static_cast<long long>(it) >= n_items) break;
+        int32_t lo = 0, hi = B.n_cpages - 1;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi + 1) / 2;
+            if (static_cast<long long>(B.cpages[mid].base) <= static_cast<long long>(it))
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const CPage cp = B.cpages[lo];
+        const int32_t local = static_cast<int32_t>(static_cast<long long>(it) - cp.base);
+        const int32_t band = local / cp.tiles, tile = local - band * cp.tiles;
+        compose_tile<CH>(B, L, cp.page, band, tile);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host
+// ---------------------------------------------------------------------------
+#define RR_TRY(ctx, expr)                                                                                          \
+    do {                                                                                                           \
+        hipError_t e_ = (expr);                                                                                    \
+        if (e_ != hipSuccess) return rrint::fail(ctx, RR_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr long long kSlotInts = static_cast<long long>(kSlotK) * 192;
+
+const void *span_kernel(int k) {
+    switch (k) {
+        case 8: return reinterpret_cast<const void *>(&k_spans<8>);
+        case 16: return reinterpret_cast<const void *>(&k_spans<16>);
+        default: return reinterpret_cast<const void *>(&k_spans<32>);
+    }
+}
+
+const void *compose_kernel(int ch) {
+    switch (ch) {
+        case 64: return reinterpret_cast<const void *>(&k_compose<64>);
+        case 128: return reinterpret_cast<const void *>(&k_compose<128>);
+        default: return reinterpret_cast<const void *>(&k_compose<256>);
+    }
+}
+
+template <typename T>
+int32_t fill(rr_hip_ctx *c, T *p, long long n, T value) {
+    if (n <= 0) return RR_OK;
+    const long long blocks = std::min<long long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill<T>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, c->stream, p, n, value);
+    RR_TRY(c, hipGetLastError());
+    return RR_OK;
+}
+
+// Exclusive scan of n inputs into out (may alias in when the types match);
+// total (optional, device) receives the sum.  tmp holds ceil(n / 1024) values.
+template <typename Tin, typename Tout>
+int32_t scan_exclusive(rr_hip_ctx *c, const Tin *in, Tout *out, long long n, Tout *total, Tout *tmp) {
+    if (n <= 0) {
+        if (total) return fill<Tout>(c, total, 1, static_cast<Tout>(0));
+        return RR_OK;
+    }
+    const long long nb = (n + kScanTile - 1) / kScanTile;
+    if (nb > 0x7fffffffll) return rrint::fail(c, RR_E_INVARIANT, "scan: too many elements");
+    hipLaunchKernelGGL((k_scan_reduce<Tin, Tout>), dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, c->stream,
+                       in, n, tmp);
+    RR_TRY(c, hipGetLastError());
+    hipLaunchKernelGGL(k_scan_sums<Tout>, dim3(1), dim3(kScanThreads), 0, c->stream, tmp, nb, total);
+    RR_TRY(c, hipGetLastError());
+    hipLaunchKernelGGL((k_scan_down<Tin, Tout>), dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, c->stream,
+                       in, out, n, static_cast<const Tout *>(tmp));
+    RR_TRY(c, hipGetLastError());
+    return RR_OK;
+}
+
+int32_t launch_profiled(rr_hip_ctx *c, const void *kernel, unsigned grid, void **args) {
+    bool ok = true;
+    hipEvent_t *ev = rrint::prof_pair(c, &ok);
+    if (!ok) return rrint::fail(c, RR_E_DEVICE, "profiling event allocation failed");
+    if (ev) RR_TRY(c, hipEventRecord(ev[0], c->stream));
+    RR_TRY(c, hipLaunchKernel(kernel, dim3(grid), dim3(64), args, 0, c->stream));
+    if (ev) RR_TRY(c, hipEventRecord(ev[1], c->stream));
+    rrint::count_launch(c);
+    return RR_OK;
+}
+
+template <typename T>
+int32_t read_back(rr_hip_ctx *c, const T *dev, T *host) {
+    RR_TRY(c, hipMemcpyAsync(host, dev, sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    RR_TRY(c, hipStreamSynchronize(c->stream));
+    return RR_OK;
+}
+
+}  // namespace
+
+namespace rrint {
+
+int32_t init_batched(rr_hip_ctx *c) {
+    for (int v = 0; v < kNumVariants; ++v) {
+        int a = 0, b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, span_kernel(kVariantK[v]), 64, 0) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, compose_kernel(kVariantCH[v]), 64, 0) != hipSuccess)
+            return fail(c, RR_E_DEVICE, "occupancy query failed");
+        c->span_waves[v] = std::max(1, c->n_cu * std::max(1, a));
+        c->compose_waves[v] = std::max(1, c->n_cu * std::max(1, b));
+    }
+    return RR_OK;
+}
+
+int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A, int k_entries, int chunk_px,
+                       int variant) {
+    const long long n_paths = b.n_paths;
+    if (n_paths >= 0x7fffffffll) return fail(c, RR_E_INVARIANT, "batched launch: too many paths");
+    long long n_bands = 0;
+    for (int32_t i = 0; i < b.n_pages; ++i) n_bands += (b.pages[i].height + kBandRows - 1) / kBandRows;
+    int32_t rc = ensure(c, &c->item_base, &c->item_base_cap, static_cast<size_t>(n_paths + 1));
+    const long long scan_n = std::max<long long>(n_paths, n_bands);
+    if (!rc) rc = ensure(c, &c->scan_tmp, &c->scan_tmp_cap, static_cast<size_t>((scan_n + kScanTile - 1) / kScanTile + 1));
+    if (!rc) rc = ensure(c, &c->band_cnt, &c->band_cnt_cap, static_cast<size_t>(std::max<long long>(n_bands, 1)));
+    if (!rc) rc = ensure(c, &c->band_base, &c->band_base_cap, static_cast<size_t>(n_bands + 1));
+    if (!rc) rc = ensure(c, &c->cpages, &c->cpages_cap, static_cast<size_t>(b.n_pages));
+    if (rc) return rc;
+
+    // composite items: pages with more paths first, bands x tiles each
+    std::vector<int32_t> order(static_cast<size_t>(b.n_pages));
+    for (int32_t i = 0; i < b.n_pages; ++i) order[static_cast<size_t>(i)] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        return (b.pages[x].path_end - b.pages[x].path_begin) > (b.pages[y].path_end - b.pages[y].path_begin);
+    });
+    c->host_cpages.resize(static_cast<size_t>(b.n_pages));
+    long long n_citems = 0;
+    for (size_t k = 0; k < order.size(); ++k) {
+        const rr_page_desc &pg = b.pages[order[k]];
+        CPage &cp = c->host_cpages[k];
+        cp.page = order[k];
+        cp.tiles = (pg.width + chunk_px - 1) / chunk_px;
+        cp.base = static_cast<int32_t>(n_citems);
+        cp.pad = 0;
+        n_citems += static_cast<long long>(cp.tiles) * ((pg.height + kBandRows - 1) / kBandRows);
+        if (n_citems >= 0x7fffffffll) return fail(c, RR_E_INVARIANT, "batched launch: too many composite tiles");
+    }
+    RR_TRY(c, hipMemcpyAsync(c->cpages, c->host_cpages.data(), sizeof(CPage) * b.n_pages, hipMemcpyHostToDevice,
+                             c->stream));
+
+    // span items: per path band counts -> item_base (total at item_base[n_paths])
+    rc = scan_exclusive<int32_t, long long>(c, c->nb, c->item_base, n_paths, c->item_base + n_paths, c->scan_tmp);
+    if (!rc) rc = fill<int32_t>(c, c->band_cnt, n_bands, 0);
+    if (!rc) rc = fill<unsigned long long>(c, c->counters, 4, 0ull);
+    if (!rc) rc = fill<unsigned long long>(c, c->spill_top, 1, 0ull);
+    if (rc) return rc;
+    long long n_items = 0;
+    rc = read_back(c, c->item_base + n_paths, &n_items);  // also lets the cpage copy land
+    if (rc) return rc;
+    if (n_items >= 0x7fffffffll) return fail(c, RR_E_INVARIANT, "batched launch: too many (path, band) items");
+    const size_t cap_items = static_cast<size_t>(std::max<long long>(n_items, 1));
+    rc = ensure(c, &c->sitems, &c->sitems_cap, cap_items);
+    if (!rc) rc = ensure(c, &c->shdr, &c->shdr_cap, cap_items);
+    if (!rc) rc = ensure(c, &c->band_list, &c->band_list_cap, cap_items);
+    if (!rc && c->spans_cap == 0) rc = ensure(c, &c->spans, &c->spans_cap, std::max<size_t>(cap_items * 24, 1u << 20));
+    if (rc) return rc;
+    if (n_items == 0) return RR_OK;
+
+    hipLaunchKernelGGL(k_expand, dim3(static_cast<unsigned>((n_paths + 3) / 4)), dim3(256), 0, c->stream, c->sel,
+                       c->nb, c->hdrs, c->pages, c->item_base, n_paths, c->sitems,
+                       static_cast<long long>(c->sitems_cap), c->band_cnt, c->err_flag);
+    RR_TRY(c, hipGetLastError());
+    rc = scan_exclusive<int32_t, long long>(c, c->band_cnt, c->band_base, n_bands, c->band_base + n_bands,
+                                            c->scan_tmp);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_band_lists, dim3(static_cast<unsigned>(n_bands)), dim3(64), 0, c->stream, c->sel, c->pages,
+                       b.n_pages, c->item_base, c->band_base, c->band_list, static_cast<long long>(c->band_list_cap),
+                       c->err_flag);
+    RR_TRY(c, hipGetLastError());
+
+    // phase A
+    const unsigned a_waves = static_cast<unsigned>(std::min<long long>(c->span_waves[variant], n_items));
+    rc = ensure(c, &c->slots, &c->slots_cap, static_cast<size_t>(a_waves) * kSlotInts);
+    if (rc) return rc;
+    BatchArgs B{};
+    B.F = A;
+    B.F.slots = c->slots;
+    B.sel = c->sel;
+    B.item_base = c->item_base;
+    B.n_paths = n_paths;
+    B.sitems = c->sitems;
+    B.sitems_cap = static_cast<long long>(c->sitems_cap);
+    B.shdr = c->shdr;
+    B.spans = c->spans;
+    B.spans_cap = static_cast<long long>(c->spans_cap);
+    B.band_base = c->band_base;
+    B.band_list = c->band_list;
+    B.cpages = c->cpages;
+    B.n_cpages = b.n_pages;
+    B.counters = c->counters;
+    int32_t count_draw = 1;
+    void *a_args[] = {&B, &count_draw};
+    rc = launch_profiled(c, span_kernel(k_entries), a_waves, a_args);
+    if (rc) return rc;
+    unsigned long long used = 0;
+    rc = read_back(c, c->counters + 2, &used);
+    if (rc) return rc;
+    if (static_cast<long long>(used) > B.spans_cap) {
+        // the span pool was too small: grow it and redo phase A (the canvas
+        // is untouched so far; the drawn-width counts are already complete)
+        rc = ensure(c, &c->spans, &c->spans_cap, static_cast<size_t>(used + used / 4));
+        if (!rc) rc = fill<unsigned long long>(c, c->counters, 4, 0ull);
+        if (!rc) rc = fill<unsigned long long>(c, c->spill_top, 1, 0ull);
+        if (rc) return rc;
+        B.spans = c->spans;
+        B.spans_cap = static_cast<long long>(c->spans_cap);
+        count_draw = 0;
+        rc = launch_profiled(c, span_kernel(k_entries), a_waves, a_args);
+        if (rc) return rc;
+    }
+
+    // phase B
+    const unsigned b_waves = static_cast<unsigned>(std::min<long long>(c->compose_waves[variant], n_citems));
+    void *b_args[] = {&B, &n_citems};
+    return launch_profiled(c, compose_kernel(chunk_px), b_waves, b_args);
+}
+
+}  // namespace rrint
+
+extern "C" int32_t rr_hip_record_scan(rr_hip_ctx *c, uint32_t *data, uint32_t *block_sums, uint32_t len_elems) {
+    if (!c) return RR_E_INVARIANT;
+    if (!rrint::page_open(c)) return rrint::fail(c, RR_E_INVARIANT, "record outside begin_page/submit_page");
+    if (len_elems == 0) return rrint::fail(c, RR_E_INVARIANT, "scan: len_elems must be > 0");
+    if (!data || !block_sums) return rrint::fail(c, RR_E_INVARIANT, "scan: null buffer");
+    RR_TRY(c, hipSetDevice(c->device));
+    return scan_exclusive<uint32_t, uint32_t>(c, data, data, len_elems, nullptr, block_sums);
+}

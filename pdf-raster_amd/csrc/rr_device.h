@@ -1,0 +1,408 @@
+// rr_device.h -- device-side data layout and scanline helpers shared by the
+// fill kernels (plain and batched routes).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "rr_hip.h"
+
+namespace rrdev {
+
+// ---------------------------------------------------------------------------
+// device data layout
+// ---------------------------------------------------------------------------
+
+// One edge after add_segment + aa_scale, reduced to what the per-row integer
+// stepping needs.  32 bytes, so one wave loads 64 of them in 2 KB.
+struct EdgeRec {
+    int32_t r0, r1;  // AA rows [r0, r1] after the clip; empty when r0 > r1
+    int32_t meta;    // bits 0-1 kind (0 sloped, 1 vertical, 2 horizontal),
+                     // bit 2 count +1 (FLIPPED), bit 3 first row has count 0
+    int32_t a;       // sloped: px0 of row r0; vertical: x; horizontal: x0
+    int32_t b;       // sloped: 16.16 right edge at row r0; horizontal: x1
+    int32_t c;       // sloped: dxdy_fp
+    int32_t d, e;    // sloped: 16.16 clamp min / max
+};
+static_assert(sizeof(EdgeRec) == 32, "EdgeRec layout");
+
+enum { KIND_SLOPED = 0, KIND_VERT = 1, KIND_HORIZ = 2 };
+enum { HDR_EMPTY = 1, HDR_OUTSIDE = 2 };
+
+// Scanner bounding box of one path in AA space (scanner/mod.rs:138-169).
+struct PathHdr {
+    int32_t y_min, y_max, x_min, x_max;
+    int32_t flags, page, pad1, pad2;
+};
+
+struct ClipRect {
+    double x_min, y_min, x_max, y_max;
+    int32_t x_min_i, y_min_i, x_max_i, y_max_i;
+};
+
+// One page of a batch, as the kernels see it.
+struct DevPage {
+    uint8_t *canvas;
+    int64_t path_begin, path_end;
+    ClipRect clip;
+    int32_t width, height, stride;
+    int32_t px_lo, px_hi;  // clip ∩ canvas, pixels
+    int32_t band_base;     // batch-global index of the page's first 16-row band
+};
+
+struct FillArgs {
+    const EdgeRec *recs;
+    const PathHdr *hdrs;
+    const int2 *rows;  // per path: drawn output rows [lo, hi] (lo > hi: nothing)
+    const int2 *csum;  // per 64-edge chunk (global edge index / 64): min r0, max r1
+    const int64_t *edge_off;
+    const rr_path_attr *attrs;
+    const DevPage *pages;
+    uint64_t *cov_hash;
+    uint64_t *cov_count;
+    int32_t *slots;                 // per-wave overflow entry slots
+    long long slot_ints;            // ints per slot
+    int32_t *spill;                 // shared overflow pool for rows beyond a slot
+    unsigned long long *spill_top;  // bump allocator (in int32 units)
+    int32_t *err;                   // set to 1 when the pool runs out
+    long long spill_cap;
+    int32_t mode, flavour;
+    // optional timing trace: one record per (path, band) fill
+    longlong4 *trace;  // {path, (page << 32) | first row, t_start, t_end} wall-clock ticks
+    unsigned long long *trace_top;
+    long long trace_cap;
+};
+
+// AA_GAMMA (crates/raster/src/fill/mod.rs:46-48)
+static __constant__ uint8_t kGamma[17] = {0, 4, 11, 21, 32, 45, 59, 74, 90, 108, 126, 145, 166, 187, 209, 231, 255};
+
+// ---------------------------------------------------------------------------
+// exact f64 helpers (crates/color/src/convert.rs:221-264; scanner/mod.rs:51-66)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int32_t sat_i32_of_int_valued(double f) {
+    if (f >= 2147483647.0) return INT32_MAX;
+    if (f <= -2147483648.0) return INT32_MIN;
+    return static_cast<int32_t>(f);
+}
+
+__device__ __forceinline__ int32_t splash_floor(double x) {
+    if (isnan(x)) return INT32_MIN;
+    if (isinf(x)) return x > 0.0 ? INT32_MAX : INT32_MIN;
+    return sat_i32_of_int_valued(floor(x));
+}
+
+__device__ __forceinline__ int32_t fp_from_f64(double x) {
+    const double fp = x * 65536.0;
+    if (fp >= 2147483647.0) return INT32_MAX;
+    if (fp <= -2147483648.0) return INT32_MIN;
+    if (isnan(fp)) return 0;
+    return static_cast<int32_t>(fp);  // truncation toward zero
+}
+
+__device__ __forceinline__ uint64_t bits_of(double v) { return __double_as_longlong(v); }
+
+__device__ __forceinline__ int32_t sat64_to_i32(long long v) {
+    return v > INT32_MAX ? INT32_MAX : (v < INT32_MIN ? INT32_MIN : static_cast<int32_t>(v));
+}
+
+__device__ __forceinline__ uint32_t div255(uint32_t x) {
+    const uint32_t s = (x + (x >> 8) + 0x80u) >> 8;
+    return s < 255u ? s : 255u;
+}
+
+__device__ __forceinline__ uint64_t cov_term(long long pix, uint32_t shape) {
+    uint64_t z = ((static_cast<uint64_t>(pix) << 8) | shape) + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ int32_t floor_div4(int32_t v) { return v >= 0 ? v / 4 : -((-v + 3) / 4); }
+
+// Output rows a path draws (fill/mod.rs:176-203): documented mode draws device
+// rows y_min/4..y_max/4; the literal loop draws y = aa_y/4 for aa_y == 4y+3 in
+// [y_min, y_max].  Both are limited to 0 <= y < H.
+__device__ __forceinline__ int2 drawn_rows(const PathHdr &h, int mode, int32_t height) {
+    int32_t lo, hi;
+    if (mode == RR_MODE_LITERAL) {
+        lo = floor_div4(h.y_min);  // ceil((y_min - 3) / 4)
+        hi = floor_div4(h.y_max - 3);
+    } else {
+        lo = h.y_min / 4;
+        hi = h.y_max / 4;
+    }
+    return make_int2(max(lo, 0), min(hi, height - 1));
+}
+
+
+// ---------------------------------------------------------------------------
+// kernel 2: scanline fill + composite of one band of 16 output rows
+// ---------------------------------------------------------------------------
+constexpr int kBandRows = 16;  // output rows per wave
+// Tunable shape of a band fill (the autotuner picks among the instantiated
+// variants): K = LDS entry slots per AA row (rows with more state-changing
+// entries go to the wave's global slot), CH = pixels per composite sweep step
+// (the LDS coverage bitmap holds 64 AA rows x 4*CH bits, padded by one dword
+// per row against bank conflicts).
+constexpr int kSlotK = 512;    // global overflow entry slots per AA row, per wave
+
+template <int K, int CH>
+struct BandLds {
+    static constexpr int kWords = CH / 8;  // bitmap dwords per AA row
+    static constexpr int kPitch = kWords + 1;
+    EdgeRec rec[64];
+    unsigned long long smask[64];  // per lane row: edges of the chunk whose rows start there
+    unsigned long long emask[64];  // ... and end there
+    int32_t ex0[K * 64];
+    int32_t ex1[K * 64];
+    int32_t ec[K * 64];
+    uint32_t bm[64 * kPitch];
+    uint8_t gamma[32];
+};
+
+struct Entry {
+    int32_t x0, x1, c;
+};
+
+// Intersection of edge record q with AA row r (fill_intersections,
+// scanner/mod.rs:528-585, with the per-row 16.16 stepping in closed form:
+// the saturating adds of a constant step are absorbing, so step k equals the
+// clamped 64-bit sum).
+__device__ __forceinline__ Entry row_entry(const EdgeRec &q, int32_t r) {
+    Entry en;
+    const int kind = q.meta & 3;
+    const int32_t cnt = (q.meta & 4) ? 1 : -1;
+    if (kind == KIND_HORIZ) {
+        en.x0 = q.a;
+        en.x1 = q.b;
+        en.c = 0;
+        return en;
+    }
+    en.c = ((q.meta & 8) && r == q.r0) ? 0 : cnt;
+    if (kind == KIND_VERT) {
+        en.x0 = en.x1 = q.a;
+        return en;
+    }
+    const long long k = static_cast<long long>(r) - q.r0;
+    const long long t1 = static_cast<long long>(q.b) + k * q.c;
+    int32_t v1 = sat64_to_i32(t1);
+    v1 = v1 < q.d ? q.d : (v1 > q.e ? q.e : v1);
+    const int32_t px1 = v1 >> 16;
+    int32_t px0;
+    if (k == 0) {
+        px0 = q.a;
+    } else {
+        int32_t v0 = sat64_to_i32(t1 - q.c);
+        v0 = v0 < q.d ? q.d : (v0 > q.e ? q.e : v0);
+        px0 = v0 >> 16;
+    }
+    en.x0 = min(px0, px1);
+    en.x1 = max(px0, px1);
+    return en;
+}
+
+// Lane range [la, lb] of the band whose AA rows lie in [a, b].  Lane l holds
+// AA row 4*Y0 + l (documented) or 16*(Y0 + l/4) + 12 + l%4 (literal).
+__device__ __forceinline__ void lane_range(int32_t a, int32_t b, bool literal, int32_t Y0, int32_t band_lo,
+                                           int32_t band_hi, int *la, int *lb) {
+    if (!literal) {
+        *la = max(a, band_lo) - band_lo;
+        *lb = min(b, band_hi) - band_lo;
+        return;
+    }
+    const int32_t base = 16 * Y0 + 12;
+    int32_t rel = a - base;
+    int lo_l;
+    if (rel <= 0) {
+        lo_l = 0;
+    } else {
+        const int32_t q = rel >> 4, m = rel & 15;
+        lo_l = m <= 3 ? 4 * q + m : 4 * (q + 1);
+    }
+    rel = b - base;
+    int hi_l;
+    if (rel < 0) {
+        hi_l = -1;
+    } else {
+        const int32_t q = rel >> 4, m = rel & 15;
+        hi_l = m <= 3 ? 4 * q + m : 4 * q + 3;
+    }
+    *la = min(lo_l, 64);
+    *lb = min(hi_l, 63);
+}
+
+// Walks the path's edges for this wave's band and builds each lane's list of
+// retired, state-changing entries in X0/X1/C ([slot * 64 + lane]).  Chunks of
+// 64 edges whose row summary misses the band are skipped with one ballot.
+// Inside a chunk the edge->row relation is transposed: every edge marks the
+// first and last lane it covers, a 64-bit prefix OR across the wave gives each
+// lane the set of chunk edges crossing its row, and each lane folds only
+// those, in edge (path) order.  Returns the number of entries this lane
+// produced (may exceed cap: then only the first cap are stored and the caller
+// reruns with more room).
+template <typename Lds>
+__device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t e0, int64_t e1,
+                                             int32_t band_lo, int32_t band_hi, bool literal, int32_t Y0, bool valid,
+                                             int32_t r, int32_t mrule, int32_t *X0, int32_t *X1, int32_t *C,
+                                             int cap) {
+    const int lane = threadIdx.x;
+    int n = 0;
+    bool has_last = false;
+    int32_t lx0 = 0, lx1 = 0, lc = 0;
+    auto retire = [&](int32_t x0, int32_t x1, int32_t c) {
+        if ((c & mrule) == 0) return;  // no effect on the walk
+        if (n < cap) {
+            X0[n * 64 + lane] = x0;
+            X1[n * 64 + lane] = x1;
+            C[n * 64 + lane] = c;
+        }
+        ++n;
+    };
+    if (e1 <= e0) return 0;
+    const int64_t c_first = e0 >> 6, c_last = (e1 - 1) >> 6;
+    for (int64_t cb = c_first; cb <= c_last; cb += 64) {
+        bool hit = false;
+        if (cb + lane <= c_last) {
+            const int2 sm = A.csum[cb + lane];
+            hit = sm.x <= band_hi && sm.y >= band_lo;
+        }
+        uint64_t cmask = __ballot(hit);
+        // records of the next overlapping chunk are loaded one chunk ahead
+        auto load_rec = [&](int jc) {
+            const int64_t i = ((cb + jc) << 6) + lane;
+            EdgeRec q{1, 0, 0, 0, 0, 0, 0, 0};
+            if (i >= e0 && i < e1) q = A.recs[i];
+            return q;
+        };
+        EdgeRec qn{1, 0, 0, 0, 0, 0, 0, 0};
+        if (cmask) qn = load_rec(__ffsll(static_cast<unsigned long long>(cmask)) - 1);
+        while (cmask) {
+            cmask &= cmask - 1;
+            const EdgeRec q = qn;
+            if (cmask) qn = load_rec(__ffsll(static_cast<unsigned long long>(cmask)) - 1);
+            int la = 64, lb = -1;
+            if (q.r0 <= q.r1) lane_range(q.r0, q.r1, literal, Y0, band_lo, band_hi, &la, &lb);
+            L.rec[lane] = q;
+            L.smask[lane] = 0ull;
+            L.emask[lane] = 0ull;
+            __syncthreads();
+            if (la <= lb) {
+                atomicOr(&L.smask[la], 1ull << lane);
+                atomicOr(&L.emask[lb], 1ull << lane);
+            }
+            __syncthreads();
+            unsigned long long ps = L.smask[lane], pe = L.emask[lane];
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned long long ts = __shfl_up(ps, o), te = __shfl_up(pe, o);
+                if (lane >= o) {
+                    ps |= ts;
+                    pe |= te;
+                }
+            }
+            unsigned long long pe_before = __shfl_up(pe, 1);
+            if (lane == 0) pe_before = 0ull;
+            unsigned long long m = valid ? (ps & ~pe_before) : 0ull;
+            while (m) {
+                const int j = __ffsll(m) - 1;
+                m &= m - 1;
+                const EdgeRec g = L.rec[j];
+                const Entry en = row_entry(g, r);
+                if (has_last && static_cast<long long>(lx1) + 1 >= en.x0 &&
+                    static_cast<long long>(lx0) <= static_cast<long long>(en.x1) + 1) {
+                    lc = static_cast<int32_t>(static_cast<uint32_t>(lc) + static_cast<uint32_t>(en.c));
+                    lx0 = min(lx0, en.x0);
+                    lx1 = max(lx1, en.x1);
+                } else {
+                    if (has_last) retire(lx0, lx1, lc);
+                    lx0 = en.x0;
+                    lx1 = en.x1;
+                    lc = en.c;
+                    has_last = true;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (has_last) retire(lx0, lx1, lc);
+    return n;
+}
+
+// Stable insertion sort by x0, then the winding walk of render_aa_line.  The
+// spans of the row are OR-ed into the AaBuf there; here they are merged into
+// disjoint sorted intervals [s, e) clamped to [0, aa_width), written back in
+// place over X0 (s) / X1 (e).  Returns the interval count; *unbalanced is set
+// when the winding never returned to zero (the reference would loop forever
+// there; the span runs to aa_width and the row stops).
+__device__ __forceinline__ int sort_and_walk(int n, int32_t *X0, int32_t *X1, int32_t *C, int32_t mrule,
+                                             int32_t aa_width, bool *unbalanced) {
+    const int lane = threadIdx.x;
+    for (int a = 1; a < n; ++a) {
+        const int32_t vx0 = X0[a * 64 + lane], vx1 = X1[a * 64 + lane], vc = C[a * 64 + lane];
+        int b = a;
+        while (b > 0) {
+            const int32_t px = X0[(b - 1) * 64 + lane];
+            if (px <= vx0) break;
+            X0[b * 64 + lane] = px;
+            X1[b * 64 + lane] = X1[(b - 1) * 64 + lane];
+            C[b * 64 + lane] = C[(b - 1) * 64 + lane];
+            --b;
+        }
+        if (b != a) {
+            X0[b * 64 + lane] = vx0;
+            X1[b * 64 + lane] = vx1;
+            C[b * 64 + lane] = vc;
+        }
+    }
+    int i = 0, nout = 0;
+    int32_t cs = 0, ce = -1;
+    int32_t count = 0;
+    while (i < n) {
+        // outside: every stored entry changes the state, so it opens a span
+        const int32_t xx = X0[i * 64 + lane];
+        count = static_cast<int32_t>(static_cast<uint32_t>(count) + static_cast<uint32_t>(C[i * 64 + lane]));
+        ++i;
+        int32_t end = aa_width;
+        bool closed = false;
+        while (i < n) {
+            count = static_cast<int32_t>(static_cast<uint32_t>(count) + static_cast<uint32_t>(C[i * 64 + lane]));
+            const int32_t x1 = X1[i * 64 + lane];
+            ++i;
+            if ((count & mrule) == 0) {
+                end = x1 == INT32_MAX ? INT32_MAX : x1 + 1;
+                closed = true;
+                break;
+            }
+        }
+        const int32_t bx0 = xx > 0 ? xx : 0;
+        int32_t bx1 = end < aa_width ? end : aa_width;
+        if (bx1 < 0) bx1 = 0;
+        if (bx0 < bx1) {
+            if (ce >= 0 && bx0 <= ce) {
+                ce = max(ce, bx1);
+            } else {
+                if (ce >= 0) {
+                    X0[nout * 64 + lane] = cs;
+                    X1[nout * 64 + lane] = ce;
+                    ++nout;
+                }
+                cs = bx0;
+                ce = bx1;
+            }
+        }
+        if (!closed) {
+            *unbalanced = true;
+            break;
+        }
+    }
+    if (ce >= 0) {
+        X0[nout * 64 + lane] = cs;
+        X1[nout * 64 + lane] = ce;
+        ++nout;
+    }
+    return nout;
+}
+
+
+}  // namespace rrdev

@@ -45,133 +45,11 @@
 #include "rr_hip.h"
 #include "rr_internal.h"
 
+#include "rr_device.h"
+
+using namespace rrdev;
+
 namespace {
-
-// ---------------------------------------------------------------------------
-// device data layout
-// ---------------------------------------------------------------------------
-
-// One edge after add_segment + aa_scale, reduced to what the per-row integer
-// stepping needs.  32 bytes, so one wave loads 64 of them in 2 KB.
-struct EdgeRec {
-    int32_t r0, r1;  // AA rows [r0, r1] after the clip; empty when r0 > r1
-    int32_t meta;    // bits 0-1 kind (0 sloped, 1 vertical, 2 horizontal),
-                     // bit 2 count +1 (FLIPPED), bit 3 first row has count 0
-    int32_t a;       // sloped: px0 of row r0; vertical: x; horizontal: x0
-    int32_t b;       // sloped: 16.16 right edge at row r0; horizontal: x1
-    int32_t c;       // sloped: dxdy_fp
-    int32_t d, e;    // sloped: 16.16 clamp min / max
-};
-static_assert(sizeof(EdgeRec) == 32, "EdgeRec layout");
-
-enum { KIND_SLOPED = 0, KIND_VERT = 1, KIND_HORIZ = 2 };
-enum { HDR_EMPTY = 1, HDR_OUTSIDE = 2 };
-
-// Scanner bounding box of one path in AA space (scanner/mod.rs:138-169).
-struct PathHdr {
-    int32_t y_min, y_max, x_min, x_max;
-    int32_t flags, page, pad1, pad2;
-};
-
-struct ClipRect {
-    double x_min, y_min, x_max, y_max;
-    int32_t x_min_i, y_min_i, x_max_i, y_max_i;
-};
-
-// One page of a batch, as the kernels see it.
-struct DevPage {
-    uint8_t *canvas;
-    int64_t path_begin, path_end;
-    ClipRect clip;
-    int32_t width, height, stride;
-    int32_t px_lo, px_hi;  // clip ∩ canvas, pixels
-    int32_t pad;
-};
-
-struct FillArgs {
-    const EdgeRec *recs;
-    const PathHdr *hdrs;
-    const int2 *rows;  // per path: drawn output rows [lo, hi] (lo > hi: nothing)
-    const int2 *csum;  // per 64-edge chunk (global edge index / 64): min r0, max r1
-    const int64_t *edge_off;
-    const rr_path_attr *attrs;
-    const DevPage *pages;
-    uint64_t *cov_hash;
-    uint64_t *cov_count;
-    int32_t *slots;                 // per-wave overflow entry slots
-    long long slot_ints;            // ints per slot
-    int32_t *spill;                 // shared overflow pool for rows beyond a slot
-    unsigned long long *spill_top;  // bump allocator (in int32 units)
-    int32_t *err;                   // set to 1 when the pool runs out
-    long long spill_cap;
-    int32_t mode, flavour;
-    // optional timing trace: one record per (path, band) fill
-    longlong4 *trace;  // {path, (page << 32) | first row, t_start, t_end} wall-clock ticks
-    unsigned long long *trace_top;
-    long long trace_cap;
-};
-
-// AA_GAMMA (crates/raster/src/fill/mod.rs:46-48)
-__constant__ uint8_t kGamma[17] = {0, 4, 11, 21, 32, 45, 59, 74, 90, 108, 126, 145, 166, 187, 209, 231, 255};
-
-// ---------------------------------------------------------------------------
-// exact f64 helpers (crates/color/src/convert.rs:221-264; scanner/mod.rs:51-66)
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ int32_t sat_i32_of_int_valued(double f) {
-    if (f >= 2147483647.0) return INT32_MAX;
-    if (f <= -2147483648.0) return INT32_MIN;
-    return static_cast<int32_t>(f);
-}
-
-__device__ __forceinline__ int32_t splash_floor(double x) {
-    if (isnan(x)) return INT32_MIN;
-    if (isinf(x)) return x > 0.0 ? INT32_MAX : INT32_MIN;
-    return sat_i32_of_int_valued(floor(x));
-}
-
-__device__ __forceinline__ int32_t fp_from_f64(double x) {
-    const double fp = x * 65536.0;
-    if (fp >= 2147483647.0) return INT32_MAX;
-    if (fp <= -2147483648.0) return INT32_MIN;
-    if (isnan(fp)) return 0;
-    return static_cast<int32_t>(fp);  // truncation toward zero
-}
-
-__device__ __forceinline__ uint64_t bits_of(double v) { return __double_as_longlong(v); }
-
-__device__ __forceinline__ int32_t sat64_to_i32(long long v) {
-    return v > INT32_MAX ? INT32_MAX : (v < INT32_MIN ? INT32_MIN : static_cast<int32_t>(v));
-}
-
-__device__ __forceinline__ uint32_t div255(uint32_t x) {
-    const uint32_t s = (x + (x >> 8) + 0x80u) >> 8;
-    return s < 255u ? s : 255u;
-}
-
-__device__ __forceinline__ uint64_t cov_term(long long pix, uint32_t shape) {
-    uint64_t z = ((static_cast<uint64_t>(pix) << 8) | shape) + 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
-__device__ __forceinline__ int32_t floor_div4(int32_t v) { return v >= 0 ? v / 4 : -((-v + 3) / 4); }
-
-// Output rows a path draws (fill/mod.rs:176-203): documented mode draws device
-// rows y_min/4..y_max/4; the literal loop draws y = aa_y/4 for aa_y == 4y+3 in
-// [y_min, y_max].  Both are limited to 0 <= y < H.
-__device__ __forceinline__ int2 drawn_rows(const PathHdr &h, int mode, int32_t height) {
-    int32_t lo, hi;
-    if (mode == RR_MODE_LITERAL) {
-        lo = floor_div4(h.y_min);  // ceil((y_min - 3) / 4)
-        hi = floor_div4(h.y_max - 3);
-    } else {
-        lo = h.y_min / 4;
-        hi = h.y_max / 4;
-    }
-    return make_int2(max(lo, 0), min(hi, height - 1));
-}
-
 // ---------------------------------------------------------------------------
 // kernel 1: per-path edge setup + scanner bounding box
 // ---------------------------------------------------------------------------
@@ -184,7 +62,8 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
                                                     const int64_t *__restrict__ edge_off,
                                                     const DevPage *__restrict__ pages, int32_t n_pages,
                                                     EdgeRec *__restrict__ recs, PathHdr *__restrict__ hdrs,
-                                                    int2 *__restrict__ rows, int32_t mode) {
+                                                    int2 *__restrict__ rows, int4 *__restrict__ sel,
+                                                    int32_t *__restrict__ nb, int32_t mode) {
     const int64_t p = blockIdx.x;
     int32_t lo = 0, hi = n_pages - 1;  // page of path p (pages sorted by path_begin)
     while (lo < hi) {
@@ -317,6 +196,16 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
         }
         hdrs[p] = h;
         rows[p] = drawn;
+        // pixel columns that can hold coverage (widened to the clip's right
+        // edge later for a path with an unbalanced row), and its band count
+        int4 s4 = make_int4(1, 0, 1, 0);
+        int32_t bands = 0;
+        if (drawn.x <= drawn.y) {
+            s4 = make_int4(drawn.x, drawn.y, max(pg.px_lo, floor_div4(h.x_min)), min(pg.px_hi, floor_div4(h.x_max)));
+            bands = drawn.y / kBandRows - drawn.x / kBandRows + 1;
+        }
+        sel[p] = s4;
+        nb[p] = bands;
     }
 }
 
@@ -341,274 +230,6 @@ __global__ void __launch_bounds__(256) k_chunk_summary(const EdgeRec *__restrict
         hi = max(hi, __shfl_xor(hi, o));
     }
     if (lane == 0) csum[ch] = make_int2(lo, hi);
-}
-
-// ---------------------------------------------------------------------------
-// kernel 2: scanline fill + composite of one band of 16 output rows
-// ---------------------------------------------------------------------------
-constexpr int kBandRows = 16;  // output rows per wave
-// Tunable shape of a band fill (the autotuner picks among the instantiated
-// variants): K = LDS entry slots per AA row (rows with more state-changing
-// entries go to the wave's global slot), CH = pixels per composite sweep step
-// (the LDS coverage bitmap holds 64 AA rows x 4*CH bits, padded by one dword
-// per row against bank conflicts).
-constexpr int kSlotK = 512;    // global overflow entry slots per AA row, per wave
-
-template <int K, int CH>
-struct BandLds {
-    static constexpr int kWords = CH / 8;  // bitmap dwords per AA row
-    static constexpr int kPitch = kWords + 1;
-    EdgeRec rec[64];
-    unsigned long long smask[64];  // per lane row: edges of the chunk whose rows start there
-    unsigned long long emask[64];  // ... and end there
-    int32_t ex0[K * 64];
-    int32_t ex1[K * 64];
-    int32_t ec[K * 64];
-    uint32_t bm[64 * kPitch];
-    uint8_t gamma[32];
-};
-
-struct Entry {
-    int32_t x0, x1, c;
-};
-
-// Intersection of edge record q with AA row r (fill_intersections,
-// scanner/mod.rs:528-585, with the per-row 16.16 stepping in closed form:
-// the saturating adds of a constant step are absorbing, so step k equals the
-// clamped 64-bit sum).
-__device__ __forceinline__ Entry row_entry(const EdgeRec &q, int32_t r) {
-    Entry en;
-    const int kind = q.meta & 3;
-    const int32_t cnt = (q.meta & 4) ? 1 : -1;
-    if (kind == KIND_HORIZ) {
-        en.x0 = q.a;
-        en.x1 = q.b;
-        en.c = 0;
-        return en;
-    }
-    en.c = ((q.meta & 8) && r == q.r0) ? 0 : cnt;
-    if (kind == KIND_VERT) {
-        en.x0 = en.x1 = q.a;
-        return en;
-    }
-    const long long k = static_cast<long long>(r) - q.r0;
-    const long long t1 = static_cast<long long>(q.b) + k * q.c;
-    int32_t v1 = sat64_to_i32(t1);
-    v1 = v1 < q.d ? q.d : (v1 > q.e ? q.e : v1);
-    const int32_t px1 = v1 >> 16;
-    int32_t px0;
-    if (k == 0) {
-        px0 = q.a;
-    } else {
-        int32_t v0 = sat64_to_i32(t1 - q.c);
-        v0 = v0 < q.d ? q.d : (v0 > q.e ? q.e : v0);
-        px0 = v0 >> 16;
-    }
-    en.x0 = min(px0, px1);
-    en.x1 = max(px0, px1);
-    return en;
-}
-
-// Lane range [la, lb] of the band whose AA rows lie in [a, b].  Lane l holds
-// AA row 4*Y0 + l (documented) or 16*(Y0 + l/4) + 12 + l%4 (literal).
-__device__ __forceinline__ void lane_range(int32_t a, int32_t b, bool literal, int32_t Y0, int32_t band_lo,
-                                           int32_t band_hi, int *la, int *lb) {
-    if (!literal) {
-        *la = max(a, band_lo) - band_lo;
-        *lb = min(b, band_hi) - band_lo;
-        return;
-    }
-    const int32_t base = 16 * Y0 + 12;
-    int32_t rel = a - base;
-    int lo_l;
-    if (rel <= 0) {
-        lo_l = 0;
-    } else {
-        const int32_t q = rel >> 4, m = rel & 15;
-        lo_l = m <= 3 ? 4 * q + m : 4 * (q + 1);
-    }
-    rel = b - base;
-    int hi_l;
-    if (rel < 0) {
-        hi_l = -1;
-    } else {
-        const int32_t q = rel >> 4, m = rel & 15;
-        hi_l = m <= 3 ? 4 * q + m : 4 * q + 3;
-    }
-    *la = min(lo_l, 64);
-    *lb = min(hi_l, 63);
-}
-
-// Walks the path's edges for this wave's band and builds each lane's list of
-// retired, state-changing entries in X0/X1/C ([slot * 64 + lane]).  Chunks of
-// 64 edges whose row summary misses the band are skipped with one ballot.
-// Inside a chunk the edge->row relation is transposed: every edge marks the
-// first and last lane it covers, a 64-bit prefix OR across the wave gives each
-// lane the set of chunk edges crossing its row, and each lane folds only
-// those, in edge (path) order.  Returns the number of entries this lane
-// produced (may exceed cap: then only the first cap are stored and the caller
-// reruns with more room).
-template <int K, int CH>
-__device__ __forceinline__ int build_entries(const FillArgs &A, BandLds<K, CH> &L, int64_t e0, int64_t e1,
-                                             int32_t band_lo, int32_t band_hi, bool literal, int32_t Y0, bool valid,
-                                             int32_t r, int32_t mrule, int32_t *X0, int32_t *X1, int32_t *C,
-                                             int cap) {
-    const int lane = threadIdx.x;
-    int n = 0;
-    bool has_last = false;
-    int32_t lx0 = 0, lx1 = 0, lc = 0;
-    auto retire = [&](int32_t x0, int32_t x1, int32_t c) {
-        if ((c & mrule) == 0) return;  // no effect on the walk
-        if (n < cap) {
-            X0[n * 64 + lane] = x0;
-            X1[n * 64 + lane] = x1;
-            C[n * 64 + lane] = c;
-        }
-        ++n;
-    };
-    if (e1 <= e0) return 0;
-    const int64_t c_first = e0 >> 6, c_last = (e1 - 1) >> 6;
-    for (int64_t cb = c_first; cb <= c_last; cb += 64) {
-        bool hit = false;
-        if (cb + lane <= c_last) {
-            const int2 sm = A.csum[cb + lane];
-            hit = sm.x <= band_hi && sm.y >= band_lo;
-        }
-        uint64_t cmask = __ballot(hit);
-        // records of the next overlapping chunk are loaded one chunk ahead
-        auto load_rec = [&](int jc) {
-            const int64_t i = ((cb + jc) << 6) + lane;
-            EdgeRec q{1, 0, 0, 0, 0, 0, 0, 0};
-            if (i >= e0 && i < e1) q = A.recs[i];
-            return q;
-        };
-        EdgeRec qn{1, 0, 0, 0, 0, 0, 0, 0};
-        if (cmask) qn = load_rec(__ffsll(static_cast<unsigned long long>(cmask)) - 1);
-        while (cmask) {
-            cmask &= cmask - 1;
-            const EdgeRec q = qn;
-            if (cmask) qn = load_rec(__ffsll(static_cast<unsigned long long>(cmask)) - 1);
-            int la = 64, lb = -1;
-            if (q.r0 <= q.r1) lane_range(q.r0, q.r1, literal, Y0, band_lo, band_hi, &la, &lb);
-            L.rec[lane] = q;
-            L.smask[lane] = 0ull;
-            L.emask[lane] = 0ull;
-            __syncthreads();
-            if (la <= lb) {
-                atomicOr(&L.smask[la], 1ull << lane);
-                atomicOr(&L.emask[lb], 1ull << lane);
-            }
-            __syncthreads();
-            unsigned long long ps = L.smask[lane], pe = L.emask[lane];
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const unsigned long long ts = __shfl_up(ps, o), te = __shfl_up(pe, o);
-                if (lane >= o) {
-                    ps |= ts;
-                    pe |= te;
-                }
-            }
-            unsigned long long pe_before = __shfl_up(pe, 1);
-            if (lane == 0) pe_before = 0ull;
-            unsigned long long m = valid ? (ps & ~pe_before) : 0ull;
-            while (m) {
-                const int j = __ffsll(m) - 1;
-                m &= m - 1;
-                const EdgeRec g = L.rec[j];
-                const Entry en = row_entry(g, r);
-                if (has_last && static_cast<long long>(lx1) + 1 >= en.x0 &&
-                    static_cast<long long>(lx0) <= static_cast<long long>(en.x1) + 1) {
-                    lc = static_cast<int32_t>(static_cast<uint32_t>(lc) + static_cast<uint32_t>(en.c));
-                    lx0 = min(lx0, en.x0);
-                    lx1 = max(lx1, en.x1);
-                } else {
-                    if (has_last) retire(lx0, lx1, lc);
-                    lx0 = en.x0;
-                    lx1 = en.x1;
-                    lc = en.c;
-                    has_last = true;
-                }
-            }
-            __syncthreads();
-        }
-    }
-    if (has_last) retire(lx0, lx1, lc);
-    return n;
-}
-
-// Stable insertion sort by x0, then the winding walk of render_aa_line.  The
-// spans of the row are OR-ed into the AaBuf there; here they are merged into
-// disjoint sorted intervals [s, e) clamped to [0, aa_width), written back in
-// place over X0 (s) / X1 (e).  Returns the interval count; *unbalanced is set
-// when the winding never returned to zero (the reference would loop forever
-// there; the span runs to aa_width and the row stops).
-__device__ __forceinline__ int sort_and_walk(int n, int32_t *X0, int32_t *X1, int32_t *C, int32_t mrule,
-                                             int32_t aa_width, bool *unbalanced) {
-    const int lane = threadIdx.x;
-    for (int a = 1; a < n; ++a) {
-        const int32_t vx0 = X0[a * 64 + lane], vx1 = X1[a * 64 + lane], vc = C[a * 64 + lane];
-        int b = a;
-        while (b > 0) {
-            const int32_t px = X0[(b - 1) * 64 + lane];
-            if (px <= vx0) break;
-            X0[b * 64 + lane] = px;
-            X1[b * 64 + lane] = X1[(b - 1) * 64 + lane];
-            C[b * 64 + lane] = C[(b - 1) * 64 + lane];
-            --b;
-        }
-        if (b != a) {
-            X0[b * 64 + lane] = vx0;
-            X1[b * 64 + lane] = vx1;
-            C[b * 64 + lane] = vc;
-        }
-    }
-    int i = 0, nout = 0;
-    int32_t cs = 0, ce = -1;
-    int32_t count = 0;
-    while (i < n) {
-        // outside: every stored entry changes the state, so it opens a span
-        const int32_t xx = X0[i * 64 + lane];
-        count = static_cast<int32_t>(static_cast<uint32_t>(count) + static_cast<uint32_t>(C[i * 64 + lane]));
-        ++i;
-        int32_t end = aa_width;
-        bool closed = false;
-        while (i < n) {
-            count = static_cast<int32_t>(static_cast<uint32_t>(count) + static_cast<uint32_t>(C[i * 64 + lane]));
-            const int32_t x1 = X1[i * 64 + lane];
-            ++i;
-            if ((count & mrule) == 0) {
-                end = x1 == INT32_MAX ? INT32_MAX : x1 + 1;
-                closed = true;
-                break;
-            }
-        }
-        const int32_t bx0 = xx > 0 ? xx : 0;
-        int32_t bx1 = end < aa_width ? end : aa_width;
-        if (bx1 < 0) bx1 = 0;
-        if (bx0 < bx1) {
-            if (ce >= 0 && bx0 <= ce) {
-                ce = max(ce, bx1);
-            } else {
-                if (ce >= 0) {
-                    X0[nout * 64 + lane] = cs;
-                    X1[nout * 64 + lane] = ce;
-                    ++nout;
-                }
-                cs = bx0;
-                ce = bx1;
-            }
-        }
-        if (!closed) {
-            *unbalanced = true;
-            break;
-        }
-    }
-    if (ce >= 0) {
-        X0[nout * 64 + lane] = cs;
-        X1[nout * 64 + lane] = ce;
-        ++nout;
-    }
-    return nout;
 }
 
 // Rasterises path p over output rows [Y0, Y0+15] of page `pg` and composites
@@ -638,7 +259,7 @@ __device__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &p
     const int32_t aa_width = pg.width * 4;
 
     int32_t *X0 = L.ex0, *X1 = L.ex1, *C = L.ec;
-    int n = build_entries<K, CH>(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, K);
+    int n = build_entries(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, K);
     int need = n;
     for (int o = 32; o > 0; o >>= 1) need = max(need, __shfl_xor(need, o));
     if (need > K) {
@@ -661,7 +282,7 @@ __device__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &p
         X0 = G;
         X1 = G + static_cast<long long>(need) * 64;
         C = G + static_cast<long long>(need) * 128;
-        n = build_entries<K, CH>(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, need);
+        n = build_entries(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, need);
     }
     bool unb = false;
     const int nint = sort_and_walk(n, X0, X1, C, mrule, aa_width, &unb);
@@ -838,113 +459,17 @@ __global__ void __launch_bounds__(64) k_fill_plain(FillArgs A, int32_t page, int
                             blockIdx.x);
 }
 
-// BATCHED: persistent waves pull (page, first row) items and composite every
-// overlapping path of the page, in path order, into that band.
-template <int K, int CH>
-__global__ void __launch_bounds__(64) k_fill_batched(FillArgs A, const int2 *__restrict__ items, int32_t n_items,
-                                                     unsigned int *__restrict__ next_item) {
-    __shared__ BandLds<K, CH> L;
-    init_lds(L);
-    const int lane = threadIdx.x;
-    for (;;) {
-        unsigned int it = 0;
-        if (lane == 0) it = atomicAdd(next_item, 1u);
-        it = __shfl(it, 0);
-        if (it >= static_cast<unsigned int>(n_items)) break;
-        const int2 item = items[it];
-        const DevPage &pg = A.pages[item.x];
-        const int32_t Y0 = item.y;
-        const int32_t Y1 = Y0 + kBandRows - 1;
-        for (int64_t base = pg.path_begin; base < pg.path_end; base += 64) {
-            const int64_t q = base + lane;
-            bool ov = false;
-            if (q < pg.path_end) {
-                const int2 rw = A.rows[q];
-                ov = rw.x <= rw.y && rw.x <= Y1 && rw.y >= Y0;
-            }
-            uint64_t mask = __ballot(ov);
-            while (mask) {
-                const int jj = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-                mask &= mask - 1;
-                fill_band_traced<K, CH>(A, L, pg, item.x, base + jj, Y0, blockIdx.x);
-            }
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 
-// Instantiated band-fill shapes (the autotuner's search space).
-struct Variant {
-    int k, chunk;
-    const void *plain;
-    const void *batched;
-};
-const Variant kVariants[] = {
-    {32, 256, reinterpret_cast<const void *>(&k_fill_plain<32, 256>), reinterpret_cast<const void *>(&k_fill_batched<32, 256>)},
-    {16, 128, reinterpret_cast<const void *>(&k_fill_plain<16, 128>), reinterpret_cast<const void *>(&k_fill_batched<16, 128>)},
-    {8, 128, reinterpret_cast<const void *>(&k_fill_plain<8, 128>), reinterpret_cast<const void *>(&k_fill_batched<8, 128>)},
-    {16, 256, reinterpret_cast<const void *>(&k_fill_plain<16, 256>), reinterpret_cast<const void *>(&k_fill_batched<16, 256>)},
-    {32, 128, reinterpret_cast<const void *>(&k_fill_plain<32, 128>), reinterpret_cast<const void *>(&k_fill_batched<32, 128>)},
-    {8, 64, reinterpret_cast<const void *>(&k_fill_plain<8, 64>), reinterpret_cast<const void *>(&k_fill_batched<8, 64>)},
-};
-constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
-
-struct Fence {
-    hipEvent_t ev = nullptr;
-    uint64_t id = 0;
-};
+// Plain-launch kernel of every instantiated fill shape (kVariantK/kVariantCH).
+#define RR_PLAIN(v) reinterpret_cast<const void *>(&k_fill_plain<kVariantK[v], kVariantCH[v]>)
+const void *const kPlainKernels[kNumVariants] = {RR_PLAIN(0), RR_PLAIN(1), RR_PLAIN(2),
+                                                 RR_PLAIN(3), RR_PLAIN(4), RR_PLAIN(5)};
+#undef RR_PLAIN
 
 }  // namespace
-
-struct rr_hip_ctx {
-    int device = 0;
-    int n_cu = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    // workspace
-    EdgeRec *recs = nullptr;
-    size_t recs_cap = 0;
-    PathHdr *hdrs = nullptr;
-    size_t hdrs_cap = 0;
-    int2 *rows = nullptr;
-    size_t rows_cap = 0;
-    int2 *csum = nullptr;
-    size_t csum_cap = 0;
-    DevPage *pages = nullptr;
-    size_t pages_cap = 0;
-    int2 *items = nullptr;
-    size_t items_cap = 0;
-    int32_t *slots = nullptr;
-    size_t slots_cap = 0;  // in int32
-    int32_t *spill = nullptr;
-    long long spill_cap = 0;
-    unsigned long long *spill_top = nullptr;
-    unsigned int *next_item = nullptr;
-    int32_t *err_flag = nullptr;
-    int batched_waves[kNumVariants] = {};
-    // page state
-    hipEvent_t t0 = nullptr, t1 = nullptr;
-    bool page_open = false, page_timed = false;
-    double last_ms = 0.0;
-    int64_t last_launches = 0;
-    Fence fences[16];
-    uint64_t next_fence = 1;
-    // per-launch profiling
-    bool profiling = false;
-    std::vector<hipEvent_t> prof_ev;  // pairs
-    size_t prof_used = 0;
-    double last_fill_ms = 0.0;
-    int64_t fill_launches = 0;
-    std::vector<DevPage> host_pages;
-    std::vector<int2> host_items;
-    // optional fill timing trace (caller-owned device buffer)
-    longlong4 *trace = nullptr;
-    long long trace_cap = 0;
-    unsigned long long *trace_top = nullptr;
-};
 
 namespace {
 
@@ -965,21 +490,6 @@ int32_t hip_fail(rr_hip_ctx *c, hipError_t e, const char *what) {
         if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
     } while (0)
 
-template <typename T>
-int32_t ensure(rr_hip_ctx *c, T **buf, size_t *cap, size_t need) {
-    if (need <= *cap) return RR_OK;
-    size_t n = *cap ? *cap : 1024;
-    while (n < need) n *= 2;
-    if (*buf) {
-        RR_HIP_TRY(c, hipStreamSynchronize(c->stream));
-        RR_HIP_TRY(c, hipFree(*buf));
-    }
-    *buf = nullptr;
-    *cap = 0;
-    RR_HIP_TRY(c, hipMalloc(reinterpret_cast<void **>(buf), n * sizeof(T)));
-    *cap = n;
-    return RR_OK;
-}
 
 // Next free start/stop event pair of the per-launch profiling pool.
 hipEvent_t *prof_pair(rr_hip_ctx *c) {
@@ -1036,7 +546,6 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     if (b.launch != RR_LAUNCH_PLAIN && b.launch != RR_LAUNCH_BATCHED)
         return fail(c, RR_E_INVARIANT, "unknown launch shape");
     if (b.variant < 0 || b.variant >= kNumVariants) return fail(c, RR_E_INVARIANT, "unknown fill variant");
-    const Variant &var = kVariants[b.variant];
     if (b.n_paths == 0 || b.n_pages == 0) return RR_OK;
     if (!b.edges || !b.edge_off || !b.attrs || !b.pages || (b.launch == RR_LAUNCH_PLAIN && !b.host_rows))
         return fail(c, RR_E_INVARIANT, "null buffer");
@@ -1059,32 +568,29 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
         d.stride = p.stride;
         d.px_lo = std::max(0, d.clip.x_min_i);
         d.px_hi = std::min(p.width - 1, d.clip.x_max_i);
-        d.pad = 0;
+        d.band_base = 0;
     }
     if (prev_end != b.n_paths) return fail(c, RR_E_INVARIANT, "pages must tile [0, n_paths) in order");
     RR_HIP_TRY(c, hipSetDevice(c->device));
-    int32_t rc = ensure(c, &c->recs, &c->recs_cap, static_cast<size_t>(b.n_edges > 0 ? b.n_edges : 1));
-    if (!rc) rc = ensure(c, &c->hdrs, &c->hdrs_cap, static_cast<size_t>(b.n_paths));
-    if (!rc) rc = ensure(c, &c->rows, &c->rows_cap, static_cast<size_t>(b.n_paths));
-    if (!rc) rc = ensure(c, &c->pages, &c->pages_cap, static_cast<size_t>(b.n_pages));
+    int32_t rc = rrint::ensure(c, &c->recs, &c->recs_cap, static_cast<size_t>(b.n_edges > 0 ? b.n_edges : 1));
+    if (!rc) rc = rrint::ensure(c, &c->hdrs, &c->hdrs_cap, static_cast<size_t>(b.n_paths));
+    if (!rc) rc = rrint::ensure(c, &c->rows, &c->rows_cap, static_cast<size_t>(b.n_paths));
+    if (!rc) rc = rrint::ensure(c, &c->sel, &c->sel_cap, static_cast<size_t>(b.n_paths));
+    if (!rc) rc = rrint::ensure(c, &c->nb, &c->nb_cap, static_cast<size_t>(b.n_paths));
+    if (!rc) rc = rrint::ensure(c, &c->pages, &c->pages_cap, static_cast<size_t>(b.n_pages));
     const int64_t n_chunks = (b.n_edges + 63) / 64;
-    if (!rc) rc = ensure(c, &c->csum, &c->csum_cap, static_cast<size_t>(n_chunks > 0 ? n_chunks : 1));
+    if (!rc) rc = rrint::ensure(c, &c->csum, &c->csum_cap, static_cast<size_t>(n_chunks > 0 ? n_chunks : 1));
     if (rc) return rc;
     if (b.launch == RR_LAUNCH_BATCHED) {
-        // Work items = (page, first row of a 16-row band) for every band of
-        // every page; pages with more paths first so their bands start early.
-        std::vector<int32_t> order(static_cast<size_t>(b.n_pages));
-        for (int32_t i = 0; i < b.n_pages; ++i) order[static_cast<size_t>(i)] = i;
-        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-            return (b.pages[x].path_end - b.pages[x].path_begin) > (b.pages[y].path_end - b.pages[y].path_begin);
-        });
-        c->host_items.clear();
-        for (int32_t i : order)
-            for (int32_t y0 = 0; y0 < b.pages[i].height; y0 += kBandRows) c->host_items.push_back(make_int2(i, y0));
-        rc = ensure(c, &c->items, &c->items_cap, c->host_items.size());
-        if (rc) return rc;
-        RR_HIP_TRY(c, hipMemcpyAsync(c->items, c->host_items.data(), sizeof(int2) * c->host_items.size(),
-                                     hipMemcpyHostToDevice, c->stream));
+        // batch-global band numbering for the per-band path lists
+        int64_t bands = 0;
+        for (int32_t i = 0; i < b.n_pages; ++i) {
+            if (b.pages[i].width > 32767)
+                return fail(c, RR_E_INVARIANT, "batched launch: page width must be <= 32767");
+            c->host_pages[static_cast<size_t>(i)].band_base = static_cast<int32_t>(bands);
+            bands += (b.pages[i].height + kBandRows - 1) / kBandRows;
+            if (bands > (1ll << 30)) return fail(c, RR_E_INVARIANT, "batched launch: too many bands");
+        }
     }
     RR_HIP_TRY(c, hipMemcpyAsync(c->pages, c->host_pages.data(), sizeof(DevPage) * b.n_pages, hipMemcpyHostToDevice,
                                  c->stream));
@@ -1118,7 +624,7 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     }
     RR_HIP_TRY(c, hipMemsetAsync(c->spill_top, 0, sizeof(unsigned long long), c->stream));
     hipLaunchKernelGGL(k_edge_setup, dim3(static_cast<unsigned>(b.n_paths)), dim3(256), 0, c->stream, b.edges,
-                       b.edge_off, c->pages, b.n_pages, c->recs, c->hdrs, c->rows, b.mode);
+                       b.edge_off, c->pages, b.n_pages, c->recs, c->hdrs, c->rows, c->sel, c->nb, b.mode);
     RR_HIP_TRY(c, hipGetLastError());
     if (n_chunks > 0) {
         hipLaunchKernelGGL(k_chunk_summary, dim3(static_cast<unsigned>((n_chunks + 3) / 4)), dim3(256), 0, c->stream,
@@ -1130,7 +636,7 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
         int32_t max_bands = 1;
         for (int32_t i = 0; i < b.n_pages; ++i)
             max_bands = std::max(max_bands, (b.pages[i].height + kBandRows) / kBandRows + 1);
-        rc = ensure(c, &c->slots, &c->slots_cap, static_cast<size_t>(max_bands) * kSlotInts);
+        rc = rrint::ensure(c, &c->slots, &c->slots_cap, static_cast<size_t>(max_bands) * kSlotInts);
         if (rc) return rc;
         A.slots = c->slots;
         for (int32_t pi = 0; pi < b.n_pages; ++pi) {
@@ -1145,7 +651,7 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
                 if (c->profiling && !ev) return fail(c, RR_E_DEVICE, "profiling event allocation failed");
                 if (ev) RR_HIP_TRY(c, hipEventRecord(ev[0], c->stream));
                 void *args[] = {&A, const_cast<int32_t *>(&pi), &p, &r0};
-                RR_HIP_TRY(c, hipLaunchKernel(var.plain, dim3(bands), dim3(64), args, 0, c->stream));
+                RR_HIP_TRY(c, hipLaunchKernel(kPlainKernels[b.variant], dim3(bands), dim3(64), args, 0, c->stream));
                 if (ev) RR_HIP_TRY(c, hipEventRecord(ev[1], c->stream));
                 ++c->fill_launches;
             }
@@ -1154,26 +660,7 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
         return RR_OK;
     }
 
-    // BATCHED: one persistent launch over the (page, band) items.
-    const size_t n_items = c->host_items.size();
-    const unsigned waves =
-        static_cast<unsigned>(std::min<size_t>(static_cast<size_t>(c->batched_waves[b.variant]), n_items));
-    rc = ensure(c, &c->slots, &c->slots_cap, static_cast<size_t>(waves) * kSlotInts);
-    if (rc) return rc;
-    A.slots = c->slots;
-    RR_HIP_TRY(c, hipMemsetAsync(c->next_item, 0, sizeof(unsigned int), c->stream));
-    hipEvent_t *ev = c->profiling ? prof_pair(c) : nullptr;
-    if (c->profiling && !ev) return fail(c, RR_E_DEVICE, "profiling event allocation failed");
-    if (ev) RR_HIP_TRY(c, hipEventRecord(ev[0], c->stream));
-    int2 *items = c->items;
-    int32_t n_it = static_cast<int32_t>(n_items);
-    unsigned int *next_item = c->next_item;
-    void *args[] = {&A, &items, &n_it, &next_item};
-    RR_HIP_TRY(c, hipLaunchKernel(var.batched, dim3(waves), dim3(64), args, 0, c->stream));
-    if (ev) RR_HIP_TRY(c, hipEventRecord(ev[1], c->stream));
-    ++c->fill_launches;
-    RR_HIP_TRY(c, hipGetLastError());
-    return RR_OK;
+    return rrint::record_batched(c, b, A, kVariantK[b.variant], kVariantCH[b.variant], b.variant);
 }
 
 }  // namespace
@@ -1208,15 +695,10 @@ int32_t rr_hip_create(int32_t device, rr_hip_ctx **out) {
     }
     int32_t rc = RR_OK;
     if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) rc = RR_E_DEVICE;
-    for (int v = 0; v < kNumVariants && rc == RR_OK; ++v) {
-        int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kVariants[v].batched, 64, 0) != hipSuccess)
-            rc = RR_E_DEVICE;
-        c->batched_waves[v] = std::max(1, c->n_cu * std::max(1, per_cu));
-    }
+    if (rc == RR_OK) rc = rrint::init_batched(c);
     if (rc == RR_OK && (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess)) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->spill_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
-    if (rc == RR_OK && hipMalloc(&c->next_item, sizeof(unsigned int)) != hipSuccess) rc = RR_E_DEVICE;
+    if (rc == RR_OK && hipMalloc(&c->counters, 4 * sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->err_flag, sizeof(int32_t)) != hipSuccess) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->trace_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
     for (auto &f : c->fences)
@@ -1237,8 +719,10 @@ int32_t rr_hip_destroy(rr_hip_ctx *c) {
     if (!c) return RR_E_INVARIANT;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void *bufs[] = {c->recs,      c->hdrs,      c->rows,     c->csum,     c->pages,    c->items,
-                    c->slots,     c->spill,     c->spill_top, c->next_item, c->err_flag, c->trace_top};
+    void *bufs[] = {c->recs,      c->hdrs,     c->rows,       c->sel,     c->nb,        c->csum,
+                    c->pages,     c->slots,    c->spill,      c->spill_top, c->err_flag, c->trace_top,
+                    c->item_base, c->scan_tmp, c->sitems,     c->shdr,    c->spans,     c->band_cnt,
+                    c->band_base, c->band_list, c->cpages,    c->counters};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->t0) (void)hipEventDestroy(c->t0);
@@ -1372,14 +856,15 @@ int32_t rr_hip_wait_page(rr_hip_ctx *c, uint64_t fence) {
         if (hipEventElapsedTime(&ms, c->prof_ev[k], c->prof_ev[k + 1]) == hipSuccess) fill += ms;
     }
     c->last_fill_ms = fill;
-    if (flag) return fail(c, RR_E_CAPACITY, "overflow-entry pool exhausted: page output is incomplete");
+    if (flag & 1) return fail(c, RR_E_CAPACITY, "overflow-entry pool exhausted: page output is incomplete");
+    if (flag) return fail(c, RR_E_CAPACITY, "batched workspace overflow (code " + std::to_string(flag) + "): page output is incomplete");
     return RR_OK;
 }
 
 int32_t rr_hip_variant_info(int32_t variant, int32_t *lds_entries, int32_t *chunk_px) {
     if (variant < 0 || variant >= kNumVariants) return RR_E_INVARIANT;
-    if (lds_entries) *lds_entries = kVariants[variant].k;
-    if (chunk_px) *chunk_px = kVariants[variant].chunk;
+    if (lds_entries) *lds_entries = kVariantK[variant];
+    if (chunk_px) *chunk_px = kVariantCH[variant];
     return RR_OK;
 }
 

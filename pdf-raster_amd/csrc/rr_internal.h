@@ -1,12 +1,113 @@
-// rr_internal.h -- context accessors shared by the translation units of the
-// HIP backend (not part of the C ABI).
+// rr_internal.h -- the backend context and helpers shared by the translation
+// units of the HIP backend (not part of the C ABI).
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
+#include "rr_device.h"
 #include "rr_hip.h"
+
+namespace rrdev {
+
+constexpr int kNumVariants = 6;  // instantiated fill shapes (the autotuner's search space)
+// K = LDS entry slots per AA row; CH = pixels per composite sweep (plain) /
+// pixels per composite tile (batched).
+constexpr int kVariantK[kNumVariants] = {32, 16, 8, 16, 32, 8};
+constexpr int kVariantCH[kNumVariants] = {256, 128, 128, 256, 128, 64};
+
+struct Fence {
+    hipEvent_t ev = nullptr;
+    uint64_t id = 0;
+};
+
+// Spans of one (path, 16-row band) item written by the batched span pass:
+// lane l's disjoint sorted intervals [s, e) (AA x units) are the cnt[l]
+// int2 records starting at off + sum(cnt[0..l)).  off = -1: not stored.
+struct SpanHdr {
+    long long off;
+    uint16_t cnt[64];
+};
+static_assert(sizeof(SpanHdr) == 136, "SpanHdr layout");
+
+// Composite work of one page: tiles per band and the index of its first
+// (band, tile) item; pages are listed in the order their items are served.
+struct CPage {
+    int32_t page, tiles, base, pad;
+};
+
+}  // namespace rrdev
+
+struct rr_hip_ctx {
+    int device = 0;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // workspace shared by both launch shapes
+    rrdev::EdgeRec *recs = nullptr;
+    size_t recs_cap = 0;
+    rrdev::PathHdr *hdrs = nullptr;
+    size_t hdrs_cap = 0;
+    int2 *rows = nullptr;
+    size_t rows_cap = 0;
+    int4 *sel = nullptr;  // per path {first drawn row, last drawn row, first px, last px}
+    size_t sel_cap = 0;
+    int32_t *nb = nullptr;  // per path: 16-row bands it draws
+    size_t nb_cap = 0;
+    int2 *csum = nullptr;
+    size_t csum_cap = 0;
+    rrdev::DevPage *pages = nullptr;
+    size_t pages_cap = 0;
+    int32_t *slots = nullptr;
+    size_t slots_cap = 0;  // in int32
+    int32_t *spill = nullptr;
+    long long spill_cap = 0;
+    unsigned long long *spill_top = nullptr;
+    int32_t *err_flag = nullptr;
+    // batched two-phase workspace
+    long long *item_base = nullptr;  // per path: first span item (n_paths + 1)
+    size_t item_base_cap = 0;
+    long long *scan_tmp = nullptr;
+    size_t scan_tmp_cap = 0;
+    int2 *sitems = nullptr;  // span items (path, first row)
+    size_t sitems_cap = 0;
+    rrdev::SpanHdr *shdr = nullptr;
+    size_t shdr_cap = 0;
+    int2 *spans = nullptr;
+    size_t spans_cap = 0;
+    int32_t *band_cnt = nullptr;  // per batch-global band: paths drawing it
+    size_t band_cnt_cap = 0;
+    long long *band_base = nullptr;  // exclusive prefix of band_cnt (n_bands + 1)
+    size_t band_base_cap = 0;
+    int2 *band_list = nullptr;  // per band, in path order: (path, span item)
+    size_t band_list_cap = 0;
+    rrdev::CPage *cpages = nullptr;
+    size_t cpages_cap = 0;
+    unsigned long long *counters = nullptr;  // [0] span items served, [1] composite items, [2] span pool top
+    int span_waves[rrdev::kNumVariants] = {};
+    int compose_waves[rrdev::kNumVariants] = {};
+    std::vector<rrdev::CPage> host_cpages;
+    // page state
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    bool page_open = false, page_timed = false;
+    double last_ms = 0.0;
+    int64_t last_launches = 0;
+    rrdev::Fence fences[16];
+    uint64_t next_fence = 1;
+    // per-launch profiling
+    bool profiling = false;
+    std::vector<hipEvent_t> prof_ev;  // pairs
+    size_t prof_used = 0;
+    double last_fill_ms = 0.0;
+    int64_t fill_launches = 0;
+    std::vector<rrdev::DevPage> host_pages;
+    // optional fill timing trace (caller-owned device buffer)
+    longlong4 *trace = nullptr;
+    long long trace_cap = 0;
+    unsigned long long *trace_top = nullptr;
+};
 
 namespace rrint {
 hipStream_t stream(rr_hip_ctx *c);
@@ -17,4 +118,33 @@ int32_t fail(rr_hip_ctx *c, int32_t code, const std::string &msg);
 // nullptr otherwise; *ok is false when the pool could not grow.
 hipEvent_t *prof_pair(rr_hip_ctx *c, bool *ok);
 void count_launch(rr_hip_ctx *c);
+
+// Grows a device workspace array to hold at least `need` elements (doubling,
+// contents not kept; the stream is drained before the old array is freed).
+template <typename T>
+int32_t ensure(rr_hip_ctx *c, T **buf, size_t *cap, size_t need) {
+    if (need <= *cap) return RR_OK;
+    size_t n = *cap ? *cap : 1024;
+    while (n < need) n *= 2;
+    hipError_t e = hipSuccess;
+    if (*buf) {
+        e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipFree(*buf);
+        if (e != hipSuccess) return fail(c, RR_E_DEVICE, std::string("workspace free: ") + hipGetErrorString(e));
+    }
+    *buf = nullptr;
+    *cap = 0;
+    e = hipMalloc(reinterpret_cast<void **>(buf), n * sizeof(T));
+    if (e != hipSuccess) return fail(c, RR_E_DEVICE, std::string("workspace alloc: ") + hipGetErrorString(e));
+    *cap = n;
+    return RR_OK;
+}
+
+// Records the batched two-phase fill + composite of b (rr_batch.hip).  A holds
+// the per-batch arguments already set up by the caller; the edge-setup and
+// chunk-summary kernels have been recorded.
+int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const rrdev::FillArgs &A, int k_entries,
+                       int chunk_px, int variant);
+// Occupancy-derived persistent wave counts of the batched kernels.
+int32_t init_batched(rr_hip_ctx *c);
 }  // namespace rrint

@@ -119,6 +119,7 @@ def lib():
             "rr_hip_record_icc_clut": (i32, [P, P, P, P, i32, i64]),
             "rr_hip_record_composite": (i32, [P, P, P, i64]),
             "rr_hip_record_apply_soft_mask": (i32, [P, P, P, i64]),
+            "rr_hip_record_scan": (i32, [P, P, P, ctypes.c_uint32]),
             "rr_images_create": (i32, [u64, i32, i32, ctypes.POINTER(ctypes.c_void_p)]),
             "rr_images_destroy": (None, [P]),
             "rr_images_info": (i32, [P, P]),
@@ -382,6 +383,9 @@ class Backend:
     def record_apply_soft_mask(self, pixels, mask, n_pixels):
         self._check(self.L.rr_hip_record_apply_soft_mask(self.h, pixels, mask, n_pixels),
                     "record_apply_soft_mask")
+
+    def record_scan(self, data, block_sums, n):
+        self._check(self.L.rr_hip_record_scan(self.h, data, block_sums, n), "record_scan")
 
     def submit_page(self):
         f = ctypes.c_uint64(0)
