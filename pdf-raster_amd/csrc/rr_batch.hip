@@ -307,10 +307,10 @@ __device__ void span_band(const BatchArgs &B, SpanLds<K> &L, int64_t p, int32_t 
         // draw_aa_line range of each output row (scanner/mod.rs:459-466)
         int32_t smin = nint ? X0[lane] : INT32_MAX;
         int32_t emax = nint ? X1[(nint - 1) * 64 + lane] : -1;
-        smin = min(smin, __shfl_xor(smin, 1));
-        smin = min(smin, __shfl_xor(smin, 2));
-        emax = max(emax, __shfl_xor(emax, 1));
-        emax = max(emax, __shfl_xor(emax, 2));
+        smin = min(smin, dpp_i32<kDppQuadXor1>(smin));
+        smin = min(smin, dpp_i32<kDppQuadXor2>(smin));
+        emax = max(emax, dpp_i32<kDppQuadXor1>(emax));
+        emax = max(emax, dpp_i32<kDppQuadXor2>(emax));
         unsigned long long w_draw = 0;
         if (j == 0 && y >= s.x && y <= s.y && smin < emax) {
             const int32_t x0 = max(smin / 4, pg.px_lo), x1 = min((emax - 1) / 4, pg.px_hi);
@@ -320,15 +320,15 @@ __device__ void span_band(const BatchArgs &B, SpanLds<K> &L, int64_t p, int32_t 
         if (lane == 0 && w_draw) atomicAdd(reinterpret_cast<unsigned long long *>(A.cov_count + 2 * p + 1), w_draw);
     }
     // store the intervals: lane-major, one int2 per interval
-    const int32_t incl = wave_incl(nint);
+    const int32_t incl = wave_incl_add(nint);
     const int32_t excl = incl - nint;
-    const int32_t total = __shfl(incl, 63);
+    const int32_t total = __builtin_amdgcn_readlane(incl, 63);
     if (cur.next + static_cast<unsigned long long>(total) > cur.end) {
         const unsigned long long want =
             static_cast<unsigned long long>(total > kSpanChunk ? total : kSpanChunk);
         unsigned long long base = 0;
         if (lane == 0) base = atomicAdd(B.counters + 2, want);
-        base = __shfl(base, 0);
+        base = first_lane_u64(base);
         cur.next = base;
         cur.end = base + want;
     }
@@ -382,7 +382,8 @@ struct ComposeLds {
     static constexpr int kWords = CH / 8;  // bitmap dwords per AA row (4 bits per pixel)
     static constexpr int kPitch = kWords + 1;
     static constexpr int kRowWords = CH / 4;  // canvas dwords per output row
-    uint32_t cv[kBandRows * kRowWords];
+    static constexpr int kRowPitch = kRowWords + 1;
+    uint32_t cv[kBandRows * kRowPitch];
     uint32_t bm[64 * kPitch];
     uint8_t gamma[32];
 };
@@ -425,9 +426,10 @@ __device__ __forceinline__ bool list_next(const BatchArgs &B, const DevPage &pg,
     }
     const int jj = __ffsll(c.mask) - 1;
     c.mask &= c.mask - 1;
-    out.p = __shfl(c.ent.x, jj);
-    out.hi = __shfl(c.ent.y, jj);
-    out.s = make_int4(__shfl(c.s.x, jj), __shfl(c.s.y, jj), __shfl(c.s.z, jj), __shfl(c.s.w, jj));
+    out.p = __builtin_amdgcn_readlane(c.ent.x, jj);
+    out.hi = __builtin_amdgcn_readlane(c.ent.y, jj);
+    out.s = make_int4(__builtin_amdgcn_readlane(c.s.x, jj), __builtin_amdgcn_readlane(c.s.y, jj),
+                      __builtin_amdgcn_readlane(c.s.z, jj), __builtin_amdgcn_readlane(c.s.w, jj));
     return true;
 }
 
@@ -447,7 +449,7 @@ __device__ __forceinline__ void hdr_fetch(const BatchArgs &B, int32_t hi, long l
 
 __device__ __forceinline__ SpanView span_fetch(const BatchArgs &B, long long off, int32_t n) {
     SpanView v;
-    const int32_t excl = wave_incl(n) - n;
+    const int32_t excl = wave_incl_add(n) - n;
     v.base = off + excl;
     v.n = off < 0 ? 0 : n;
     v.v0 = make_int2(INT32_MAX, INT32_MAX);
@@ -480,7 +482,7 @@ __device__ __forceinline__ void compose_rows(const BatchArgs &B, ComposeLds<CH> 
                                              int32_t ya, int32_t yb) {
     const FillArgs &A = B.F;
     const int lane = threadIdx.x, d = lane >> 2, j = lane & 3, g0 = lane & ~3;
-    constexpr int kPitch = ComposeLds<CH>::kPitch, kRowWords = ComposeLds<CH>::kRowWords;
+    constexpr int kPitch = ComposeLds<CH>::kPitch, kRowPitch = ComposeLds<CH>::kRowPitch;
     const rr_path_attr at = A.attrs[p];
     const uint32_t a_in = at.a_input, src = at.color, src4 = src * 0x01010101u;
     const int flavour = A.flavour;
@@ -499,26 +501,23 @@ __device__ __forceinline__ void compose_rows(const BatchArgs &B, ComposeLds<CH> 
     }
     const int ne = e0 > s0 ? 1 : 0;
     int cx = n >= 2 ? 1 : 0;
-    cx |= __shfl_xor(cx, 1);
-    cx |= __shfl_xor(cx, 2);
-    int all_ne = ne & __shfl_xor(ne, 1);
-    all_ne &= __shfl_xor(all_ne, 2);
+    cx |= dpp_i32<kDppQuadXor1>(cx);
+    cx |= dpp_i32<kDppQuadXor2>(cx);
+    int all_ne = ne & dpp_i32<kDppQuadXor1>(ne);
+    all_ne &= dpp_i32<kDppQuadXor2>(all_ne);
     int32_t fa = ne ? (s0 + 3) >> 2 : INT32_MIN, fb = ne ? (e0 >> 2) - 1 : INT32_MAX;
     int32_t ua = ne ? s0 >> 2 : INT32_MAX, ub = ne ? (e0 - 1) >> 2 : INT32_MIN;
-#pragma unroll
-    for (int o = 1; o <= 2; o <<= 1) {
-        fa = max(fa, __shfl_xor(fa, o));
-        fb = min(fb, __shfl_xor(fb, o));
-        ua = min(ua, __shfl_xor(ua, o));
-        ub = max(ub, __shfl_xor(ub, o));
-    }
-    int32_t sk[4], ek[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        sk[k] = __shfl(s0, g0 + k);
-        ek[k] = __shfl(e0, g0 + k);
-    }
-    uint32_t *rowc = L.cv + d * kRowWords;
+    fa = max(fa, dpp_i32<kDppQuadXor1>(fa));
+    fb = min(fb, dpp_i32<kDppQuadXor1>(fb));
+    ua = min(ua, dpp_i32<kDppQuadXor1>(ua));
+    ub = max(ub, dpp_i32<kDppQuadXor1>(ub));
+    fa = max(fa, dpp_i32<kDppQuadXor2>(fa));
+    fb = min(fb, dpp_i32<kDppQuadXor2>(fb));
+    ua = min(ua, dpp_i32<kDppQuadXor2>(ua));
+    ub = max(ub, dpp_i32<kDppQuadXor2>(ub));
+    const int32_t sk[4] = {quad_bcast<0>(s0), quad_bcast<1>(s0), quad_bcast<2>(s0), quad_bcast<3>(s0)};
+    const int32_t ek[4] = {quad_bcast<0>(e0), quad_bcast<1>(e0), quad_bcast<2>(e0), quad_bcast<3>(e0)};
+    uint32_t *rowc = L.cv + d * kRowPitch;
     uint8_t *rowb = reinterpret_cast<uint8_t *>(rowc);
     const int32_t dwbase = tx0 >> 2;
     const long long pixrow = static_cast<long long>(yy) * pg.width;
@@ -651,7 +650,7 @@ template <int CH>
 __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, int32_t page, int32_t band, int32_t tile) {
     const FillArgs &A = B.F;
     const int lane = threadIdx.x;
-    constexpr int kRowWords = ComposeLds<CH>::kRowWords;
+    constexpr int kRowWords = ComposeLds<CH>::kRowWords, kRowPitch = ComposeLds<CH>::kRowPitch;
     const DevPage &pg = A.pages[page];
     const int32_t Y0 = kBandRows * band, tx0 = CH * tile;
     const int32_t width = pg.width;
@@ -676,7 +675,8 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, int32_t page
     for (int idx = lane; idx < kBandRows * kRowWords; idx += 64) {
         const int r = idx / kRowWords, w = idx % kRowWords;
         if (r < nrows && w < ndw)
-            L.cv[idx] = *reinterpret_cast<const uint32_t *>(base + static_cast<long long>(r) * stride + 4 * w);
+            L.cv[r * kRowPitch + w] =
+                *reinterpret_cast<const uint32_t *>(base + static_cast<long long>(r) * stride + 4 * w);
     }
     SpanView V0 = span_fetch(B, off0, n0);
     __syncthreads();
@@ -711,7 +711,7 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, int32_t page
     for (int idx = lane; idx < kBandRows * kRowWords; idx += 64) {
         const int r = idx / kRowWords, w = idx % kRowWords;
         if (r < nrows && w < ndw)
-            *reinterpret_cast<uint32_t *>(base + static_cast<long long>(r) * stride + 4 * w) = L.cv[idx];
+            *reinterpret_cast<uint32_t *>(base + static_cast<long long>(r) * stride + 4 * w) = L.cv[r * kRowPitch + w];
     }
     __syncthreads();
 }
@@ -765,7 +765,8 @@ const void *compose_kernel(int ch) {
     switch (ch) {
         case 64: return reinterpret_cast<const void *>(&k_compose<64>);
         case 128: return reinterpret_cast<const void *>(&k_compose<128>);
-        default: return reinterpret_cast<const void *>(&k_compose<256>);
+        case 256: return reinterpret_cast<const void *>(&k_compose<256>);
+        default: return reinterpret_cast<const void *>(&k_compose<512>);
     }
 }
 

@@ -75,6 +75,72 @@ struct FillArgs {
 };
 
 // AA_GAMMA (crates/raster/src/fill/mod.rs:46-48)
+// ---------------------------------------------------------------------------
+// wave64 cross-lane helpers on DPP (no LDS round trip).  All lanes of the
+// wave must be active where they are used.
+// ---------------------------------------------------------------------------
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143, kDppWaveShr1 = 0x138;
+constexpr int kDppQuadXor1 = 0xB1, kDppQuadXor2 = 0x4E;  // quad_perm [1,0,3,2], [2,3,0,1]
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ int32_t dpp_i32(int32_t v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xf, true);
+}
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int32_t>(v), CTRL, ROW_MASK, 0xf, true));
+}
+
+// Value of lane (lane & ~3) + K, the K-th lane of this lane's quad.
+template <int K>
+__device__ __forceinline__ int32_t quad_bcast(int32_t v) {
+    return dpp_i32<0x55 * K>(v);
+}
+
+// Lane 0's value of a 64-bit quantity, in scalar registers.
+__device__ __forceinline__ unsigned long long first_lane_u64(unsigned long long v) {
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int32_t>(v)));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int32_t>(v >> 32)));
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+// Inclusive prefix sum over the wave.
+__device__ __forceinline__ int32_t wave_incl_add(int32_t v) {
+    v += dpp_i32<kDppRowShr1>(v);
+    v += dpp_i32<kDppRowShr2>(v);
+    v += dpp_i32<kDppRowShr4>(v);
+    v += dpp_i32<kDppRowShr8>(v);
+    v += dpp_i32<kDppRowBcast15, 0xa>(v);
+    v += dpp_i32<kDppRowBcast31, 0xc>(v);
+    return v;
+}
+
+// Inclusive prefix OR over the wave.
+__device__ __forceinline__ uint32_t wave_incl_or(uint32_t v) {
+    v |= dpp_u32<kDppRowShr1>(v);
+    v |= dpp_u32<kDppRowShr2>(v);
+    v |= dpp_u32<kDppRowShr4>(v);
+    v |= dpp_u32<kDppRowShr8>(v);
+    v |= dpp_u32<kDppRowBcast15, 0xa>(v);
+    v |= dpp_u32<kDppRowBcast31, 0xc>(v);
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_incl_or64(unsigned long long v) {
+    const uint32_t lo = wave_incl_or(static_cast<uint32_t>(v)), hi = wave_incl_or(static_cast<uint32_t>(v >> 32));
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+// Value of lane - 1 (0 for lane 0).
+__device__ __forceinline__ unsigned long long wave_shr1_u64(unsigned long long v) {
+    const uint32_t lo = dpp_u32<kDppWaveShr1>(static_cast<uint32_t>(v));
+    const uint32_t hi = dpp_u32<kDppWaveShr1>(static_cast<uint32_t>(v >> 32));
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
 static __constant__ uint8_t kGamma[17] = {0, 4, 11, 21, 32, 45, 59, 74, 90, 108, 126, 145, 166, 187, 209, 231, 255};
 
 // ---------------------------------------------------------------------------
@@ -292,17 +358,9 @@ __device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t 
                 atomicOr(&L.emask[lb], 1ull << lane);
             }
             __syncthreads();
-            unsigned long long ps = L.smask[lane], pe = L.emask[lane];
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const unsigned long long ts = __shfl_up(ps, o), te = __shfl_up(pe, o);
-                if (lane >= o) {
-                    ps |= ts;
-                    pe |= te;
-                }
-            }
-            unsigned long long pe_before = __shfl_up(pe, 1);
-            if (lane == 0) pe_before = 0ull;
+            const unsigned long long ps = wave_incl_or64(L.smask[lane]);
+            const unsigned long long pe = wave_incl_or64(L.emask[lane]);
+            const unsigned long long pe_before = wave_shr1_u64(pe);
             unsigned long long m = valid ? (ps & ~pe_before) : 0ull;
             while (m) {
                 const int j = __ffsll(m) - 1;

@@ -463,10 +463,13 @@ __global__ void __launch_bounds__(64) k_fill_plain(FillArgs A, int32_t page, int
 // host side
 // ---------------------------------------------------------------------------
 
-// Plain-launch kernel of every instantiated fill shape (kVariantK/kVariantCH).
-#define RR_PLAIN(v) reinterpret_cast<const void *>(&k_fill_plain<kVariantK[v], kVariantCH[v]>)
-const void *const kPlainKernels[kNumVariants] = {RR_PLAIN(0), RR_PLAIN(1), RR_PLAIN(2),
-                                                 RR_PLAIN(3), RR_PLAIN(4), RR_PLAIN(5)};
+// Plain-launch kernel of every instantiated fill shape (kVariantK/kVariantCH);
+// the plain composite sweep is at most 256 pixels (one dword per lane per row).
+constexpr int plain_ch(int v) { return kVariantCH[v] > 256 ? 256 : kVariantCH[v]; }
+#define RR_PLAIN(v) reinterpret_cast<const void *>(&k_fill_plain<kVariantK[v], plain_ch(v)>)
+const void *const kPlainKernels[kNumVariants] = {RR_PLAIN(0), RR_PLAIN(1), RR_PLAIN(2),  RR_PLAIN(3),
+                                                 RR_PLAIN(4), RR_PLAIN(5), RR_PLAIN(6),  RR_PLAIN(7),
+                                                 RR_PLAIN(8), RR_PLAIN(9), RR_PLAIN(10), RR_PLAIN(11)};
 #undef RR_PLAIN
 
 }  // namespace

@@ -12,11 +12,11 @@
 
 namespace rrdev {
 
-constexpr int kNumVariants = 6;  // instantiated fill shapes (the autotuner's search space)
+constexpr int kNumVariants = 12;  // instantiated fill shapes (the autotuner's search space)
 // K = LDS entry slots per AA row; CH = pixels per composite sweep (plain) /
 // pixels per composite tile (batched).
-constexpr int kVariantK[kNumVariants] = {32, 16, 8, 16, 32, 8};
-constexpr int kVariantCH[kNumVariants] = {256, 128, 128, 256, 128, 64};
+constexpr int kVariantK[kNumVariants] = {8, 8, 8, 8, 16, 16, 16, 16, 32, 32, 32, 32};
+constexpr int kVariantCH[kNumVariants] = {64, 128, 256, 512, 64, 128, 256, 512, 64, 128, 256, 512};
 
 struct Fence {
     hipEvent_t ev = nullptr;
