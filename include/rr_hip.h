@@ -254,6 +254,28 @@ int32_t rr_hip_record_icc_clut(rr_hip_ctx *ctx, const uint8_t *cmyk, uint8_t *rg
 int32_t rr_hip_record_composite(rr_hip_ctx *ctx, const uint8_t *src, uint8_t *dst, int64_t n_pixels);
 int32_t rr_hip_record_apply_soft_mask(rr_hip_ctx *ctx, uint8_t *pixels, const uint8_t *mask, int64_t n_pixels);
 
+/* HIP-graph capture of a batched recording (no reference counterpart: the
+ * batched pattern as one replayable graph).  graph_create runs an eager
+ * sizing pass (edge setup, span pass; the canvases are not touched), sizes the
+ * workspace, then captures the whole device sequence of rr_hip_record_fill_pages
+ * for *batch (launch must be RR_LAUNCH_BATCHED; the device buffers and page
+ * table it names are baked in).  graph_launch replays it inside
+ * begin_page/submit_page; it fails with RR_E_INVARIANT once any recording on
+ * the context has reallocated the workspace (capture again).  The caller
+ * clears canvases / coverage buffers between replays as for eager records. */
+typedef struct rr_hip_graph rr_hip_graph;
+int32_t rr_hip_graph_create(rr_hip_ctx *ctx, const rr_fill_pages *batch, rr_hip_graph **out);
+int32_t rr_hip_graph_launch(rr_hip_ctx *ctx, rr_hip_graph *graph);
+int32_t rr_hip_graph_destroy(rr_hip_ctx *ctx, rr_hip_graph *graph);
+
+/* Statistics of the last batched recording (no reference counterpart; read
+ * after wait_page).  ms: {span pass, composite pass} kernel milliseconds when
+ * per-launch profiling was on for that page, else 0.  counts: {(path, band)
+ * span items, intervals stored, (page, band, tile) composite items,
+ * (path, tile) composites, tiles holding at least one path, span-pass reruns
+ * after growing the interval pool}. */
+int32_t rr_hip_last_batch_stats(rr_hip_ctx *ctx, double ms[2], int64_t counts[6]);
+
 /* In-place exclusive prefix sum of len_elems uint32 (wrapping), replacing the
  * three record_scan phases of the reference (GpuBackend::record_scan,
  * crates/gpu/src/backend/mod.rs:338-353; ScanParams, params.rs:207-367;

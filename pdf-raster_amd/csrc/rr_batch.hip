@@ -245,7 +245,7 @@ struct SpanLds {
 // kSpanChunk records instead of one per item.
 constexpr long long kSpanChunk = 8192;
 struct SpanCursor {
-    unsigned long long next = 0, end = 0;
+    unsigned long long next = 0, end = 0, stored = 0;
 };
 
 template <int K>
@@ -335,6 +335,7 @@ __device__ void span_band(const BatchArgs &B, SpanLds<K> &L, int64_t p, int32_t 
     const unsigned long long off = cur.next;
     cur.next += static_cast<unsigned long long>(total);
     const bool fits = static_cast<long long>(off) + total <= B.spans_cap;
+    cur.stored += fits ? total : 0;
     if (fits) {
         int2 *dst = B.spans + off + excl;
         for (int k = 0; k < nint; ++k) dst[k] = make_int2(X0[k * 64 + lane], X1[k * 64 + lane]);
@@ -372,6 +373,7 @@ __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t count_draw) {
             }
         }
     }
+    if (lane == 0 && cur.stored) atomicAdd(B.counters + 3, cur.stored);
 }
 
 // ---------------------------------------------------------------------------
@@ -647,7 +649,8 @@ __device__ __forceinline__ void compose_rows(const BatchArgs &B, ComposeLds<CH> 
 // while path j composites, the first span of path j+1 and the header of path
 // j+2 are in flight.
 template <int CH>
-__device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, int32_t page, int32_t band, int32_t tile) {
+__device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, int32_t page, int32_t band, int32_t tile,
+                             unsigned long long *n_paths, unsigned long long *n_tiles) {
     const FillArgs &A = B.F;
     const int lane = threadIdx.x;
     constexpr int kRowWords = ComposeLds<CH>::kRowWords, kRowPitch = ComposeLds<CH>::kRowPitch;
@@ -663,6 +666,7 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, int32_t page
     cur.mask = 0;
     PathRef P0, P1;
     if (!list_next(B, pg, cur, tx0, txe, P0)) return;
+    ++*n_tiles;
     const int32_t nrows = min(kBandRows, pg.height - Y0);
     const int32_t ndw = ((txe - tx0) >> 2) + 1;
     uint8_t *const base = pg.canvas + static_cast<long long>(Y0) * pg.stride + tx0;
@@ -699,6 +703,7 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, int32_t page
             if (has2) hdr_fetch(B, P2.hi, &off2, &n2);
         }
         if (live) compose_rows<CH>(B, L, pg, P0.p, V0, Y0, tx0, xa, xb, ya, yb);
+        ++*n_paths;
         __syncthreads();
         if (!has1) break;
         P0 = P1;
@@ -722,6 +727,7 @@ __global__ void __launch_bounds__(64) k_compose(BatchArgs B, long long n_items) 
     if (threadIdx.x < 17) L.gamma[threadIdx.x] = kGamma[threadIdx.x];
     __syncthreads();
     const int lane = threadIdx.x;
+    unsigned long long n_paths = 0, n_tiles = 0;
     for (;;) {
         unsigned long long it = 0;
         if (lane == 0) it = atomicAdd(B.counters + 1, 1ull);
@@ -738,7 +744,11 @@ __global__ void __launch_bounds__(64) k_compose(BatchArgs B, long long n_items) 
         const CPage cp = B.cpages[lo];
         const int32_t local = static_cast<int32_t>(static_cast<long long>(it) - cp.base);
         const int32_t band = local / cp.tiles, tile = local - band * cp.tiles;
-        compose_tile<CH>(B, L, cp.page, band, tile);
+        compose_tile<CH>(B, L, cp.page, band, tile, &n_paths, &n_tiles);
+    }
+    if (lane == 0) {
+        if (n_paths) atomicAdd(B.counters + 4, n_paths);
+        if (n_tiles) atomicAdd(B.counters + 5, n_tiles);
     }
 }
 
@@ -866,18 +876,21 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
         n_citems += static_cast<long long>(cp.tiles) * ((pg.height + kBandRows - 1) / kBandRows);
         if (n_citems >= 0x7fffffffll) return fail(c, RR_E_INVARIANT, "batched launch: too many composite tiles");
     }
-    RR_TRY(c, hipMemcpyAsync(c->cpages, c->host_cpages.data(), sizeof(CPage) * b.n_pages, hipMemcpyHostToDevice,
-                             c->stream));
+    if (!c->capturing)
+        RR_TRY(c, hipMemcpyAsync(c->cpages, c->host_cpages.data(), sizeof(CPage) * b.n_pages,
+                                 hipMemcpyHostToDevice, c->stream));
 
     // span items: per path band counts -> item_base (total at item_base[n_paths])
     rc = scan_exclusive<int32_t, long long>(c, c->nb, c->item_base, n_paths, c->item_base + n_paths, c->scan_tmp);
     if (!rc) rc = fill<int32_t>(c, c->band_cnt, n_bands, 0);
-    if (!rc) rc = fill<unsigned long long>(c, c->counters, 4, 0ull);
+    if (!rc) rc = fill<unsigned long long>(c, c->counters, 8, 0ull);
     if (!rc) rc = fill<unsigned long long>(c, c->spill_top, 1, 0ull);
     if (rc) return rc;
-    long long n_items = 0;
-    rc = read_back(c, c->item_base + n_paths, &n_items);  // also lets the cpage copy land
-    if (rc) return rc;
+    long long n_items = c->batch_items;  // while capturing: the sizing pass's count
+    if (!c->capturing) {
+        rc = read_back(c, c->item_base + n_paths, &n_items);  // also lets the cpage copy land
+        if (rc) return rc;
+    }
     if (n_items >= 0x7fffffffll) return fail(c, RR_E_INVARIANT, "batched launch: too many (path, band) items");
     const size_t cap_items = static_cast<size_t>(std::max<long long>(n_items, 1));
     rc = ensure(c, &c->sitems, &c->sitems_cap, cap_items);
@@ -921,24 +934,44 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     B.counters = c->counters;
     int32_t count_draw = 1;
     void *a_args[] = {&B, &count_draw};
+    c->batch_valid = true;
+    c->batch_items = n_items;
+    c->batch_citems = n_citems;
+    c->batch_reruns = 0;
+    c->prof_a0 = c->prof_used;
     rc = launch_profiled(c, span_kernel(k_entries), a_waves, a_args);
     if (rc) return rc;
+    if (c->capturing) {
+        // sized by the eager pass: the pool holds every interval plus each
+        // wave's unused chunk tail
+        c->prof_a1 = c->prof_b = c->prof_used;
+        const unsigned b_waves = static_cast<unsigned>(std::min<long long>(c->compose_waves[variant], n_citems));
+        void *b_args[] = {&B, &n_citems};
+        return launch_profiled(c, compose_kernel(chunk_px), b_waves, b_args);
+    }
     unsigned long long used = 0;
     rc = read_back(c, c->counters + 2, &used);
     if (rc) return rc;
+    if (c->size_only) {
+        c->prof_a1 = c->prof_b = c->prof_used;
+        return RR_OK;
+    }
     if (static_cast<long long>(used) > B.spans_cap) {
         // the span pool was too small: grow it and redo phase A (the canvas
         // is untouched so far; the drawn-width counts are already complete)
         rc = ensure(c, &c->spans, &c->spans_cap, static_cast<size_t>(used + used / 4));
-        if (!rc) rc = fill<unsigned long long>(c, c->counters, 4, 0ull);
+        if (!rc) rc = fill<unsigned long long>(c, c->counters, 8, 0ull);
         if (!rc) rc = fill<unsigned long long>(c, c->spill_top, 1, 0ull);
         if (rc) return rc;
         B.spans = c->spans;
         B.spans_cap = static_cast<long long>(c->spans_cap);
         count_draw = 0;
+        c->batch_reruns = 1;
         rc = launch_profiled(c, span_kernel(k_entries), a_waves, a_args);
         if (rc) return rc;
     }
+    c->prof_a1 = c->prof_used;
+    c->prof_b = c->prof_used;
 
     // phase B
     const unsigned b_waves = static_cast<unsigned>(std::min<long long>(c->compose_waves[variant], n_citems));
@@ -946,7 +979,159 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     return launch_profiled(c, compose_kernel(chunk_px), b_waves, b_args);
 }
 
+int32_t zero_u64(rr_hip_ctx *c, unsigned long long *p, long long n) { return fill<unsigned long long>(c, p, n, 0ull); }
+
 }  // namespace rrint
+
+struct rr_hip_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t ws_gen = 0;
+    int64_t items = 0, citems = 0;
+    // the host-provided tables the sequence reads, owned by the graph so that
+    // later recordings on the context cannot change what a replay sees
+    DevPage *pages = nullptr;
+    CPage *cpages = nullptr;
+};
+
+namespace {
+void graph_free(rr_hip_graph *g) {
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->pages) (void)hipFree(g->pages);
+    if (g->cpages) (void)hipFree(g->cpages);
+    delete g;
+}
+}  // namespace
+
+extern "C" int32_t rr_hip_graph_create(rr_hip_ctx *c, const rr_fill_pages *b, rr_hip_graph **out) {
+    if (!c || !b || !out) return RR_E_INVARIANT;
+    *out = nullptr;
+    if (c->page_open) return rrint::fail(c, RR_E_INVARIANT, "graph_create inside an open page");
+    if (b->launch != RR_LAUNCH_BATCHED) return rrint::fail(c, RR_E_INVARIANT, "graph capture covers RR_LAUNCH_BATCHED");
+    if (c->trace) return rrint::fail(c, RR_E_INVARIANT, "graph capture with a trace buffer set");
+    RR_TRY(c, hipSetDevice(c->device));
+    rr_fill_pages t = *b;
+    const bool prof = c->profiling;
+    c->profiling = false;
+    // 1. eager sizing pass: tables uploaded, items counted, intervals counted
+    //    (no composite, so the canvases are untouched)
+    c->page_open = true;
+    c->size_only = true;
+    int32_t rc = rrint::record_pages_internal(c, t);
+    c->size_only = false;
+    c->page_open = false;
+    unsigned long long k[8] = {};
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = rrint::fail(c, RR_E_DEVICE, "sizing pass failed");
+    if (!rc && hipMemcpy(k, c->counters, sizeof(k), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = rrint::fail(c, RR_E_DEVICE, "sizing pass read-back failed");
+    if (!rc) {
+        const size_t need = static_cast<size_t>(k[3]) +
+                            static_cast<size_t>(c->span_waves[t.variant]) * static_cast<size_t>(kSpanChunk) + 1024;
+        rc = rrint::ensure(c, &c->spans, &c->spans_cap, need);
+    }
+    if (rc) {
+        c->profiling = prof;
+        return rc;
+    }
+    // 2. graph-owned copies of the page and composite-tile tables
+    rr_hip_graph *g = new rr_hip_graph();
+    g->items = c->batch_items;
+    g->citems = c->batch_citems;
+    const size_t np = static_cast<size_t>(t.n_pages);
+    if (hipMalloc(&g->pages, np * sizeof(DevPage)) != hipSuccess ||
+        hipMalloc(&g->cpages, np * sizeof(CPage)) != hipSuccess ||
+        hipMemcpy(g->pages, c->pages, np * sizeof(DevPage), hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(g->cpages, c->cpages, np * sizeof(CPage), hipMemcpyDeviceToDevice) != hipSuccess) {
+        c->profiling = prof;
+        graph_free(g);
+        return rrint::fail(c, RR_E_DEVICE, "graph table copies failed");
+    }
+    // 3. capture the whole device sequence, reading the graph's tables
+    DevPage *const ctx_pages = c->pages;
+    CPage *const ctx_cpages = c->cpages;
+    c->pages = g->pages;
+    c->cpages = g->cpages;
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        c->capturing = true;
+        c->page_open = true;
+        rc = rrint::record_pages_internal(c, t);
+        c->page_open = false;
+        c->capturing = false;
+        const hipError_t e2 = hipStreamEndCapture(c->stream, &g->graph);
+        if (!rc && e2 != hipSuccess) rc = rrint::fail(c, RR_E_DEVICE, std::string("end capture: ") + hipGetErrorString(e2));
+    } else {
+        rc = rrint::fail(c, RR_E_DEVICE, std::string("begin capture: ") + hipGetErrorString(e));
+    }
+    c->pages = ctx_pages;
+    c->cpages = ctx_cpages;
+    if (!rc && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+        rc = rrint::fail(c, RR_E_DEVICE, "graph instantiate failed");
+    c->profiling = prof;
+    if (rc) {
+        graph_free(g);
+        return rc;
+    }
+    g->ws_gen = c->ws_gen;
+    *out = g;
+    return RR_OK;
+}
+
+extern "C" int32_t rr_hip_graph_launch(rr_hip_ctx *c, rr_hip_graph *g) {
+    if (!c || !g) return RR_E_INVARIANT;
+    if (!c->page_open) return rrint::fail(c, RR_E_INVARIANT, "graph_launch outside begin_page/submit_page");
+    if (g->ws_gen != c->ws_gen)
+        return rrint::fail(c, RR_E_INVARIANT, "the context's workspace was reallocated since capture: capture again");
+    RR_TRY(c, hipSetDevice(c->device));
+    bool ok = true;
+    hipEvent_t *ev = rrint::prof_pair(c, &ok);
+    if (!ok) return rrint::fail(c, RR_E_DEVICE, "profiling event allocation failed");
+    if (ev) RR_TRY(c, hipEventRecord(ev[0], c->stream));
+    RR_TRY(c, hipGraphLaunch(g->exec, c->stream));
+    if (ev) RR_TRY(c, hipEventRecord(ev[1], c->stream));
+    rrint::count_launch(c);
+    c->batch_items = g->items;
+    c->batch_citems = g->citems;
+    c->batch_reruns = 0;
+    c->batch_valid = false;  // per-phase events are not inside the graph
+    return RR_OK;
+}
+
+extern "C" int32_t rr_hip_graph_destroy(rr_hip_ctx *c, rr_hip_graph *g) {
+    if (!g) return RR_E_INVARIANT;
+    if (c) {
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+    }
+    graph_free(g);
+    return RR_OK;
+}
+
+extern "C" int32_t rr_hip_last_batch_stats(rr_hip_ctx *c, double ms[2], int64_t counts[6]) {
+    if (!c || !ms || !counts) return RR_E_INVARIANT;
+    if (!c->batch_valid) return rrint::fail(c, RR_E_INVARIANT, "no batched recording since the context was created");
+    RR_TRY(c, hipSetDevice(c->device));
+    RR_TRY(c, hipStreamSynchronize(c->stream));
+    unsigned long long k[8] = {};
+    RR_TRY(c, hipMemcpy(k, c->counters, sizeof(k), hipMemcpyDeviceToHost));
+    ms[0] = ms[1] = 0.0;
+    if (c->profiling && c->prof_b + 2 <= c->prof_used) {
+        for (size_t i = c->prof_a0; i + 1 < c->prof_a1; i += 2) {
+            float m = 0.f;
+            if (hipEventElapsedTime(&m, c->prof_ev[i], c->prof_ev[i + 1]) == hipSuccess) ms[0] += m;
+        }
+        float m = 0.f;
+        if (hipEventElapsedTime(&m, c->prof_ev[c->prof_b], c->prof_ev[c->prof_b + 1]) == hipSuccess) ms[1] = m;
+    }
+    counts[0] = c->batch_items;
+    counts[1] = static_cast<int64_t>(k[3]);
+    counts[2] = c->batch_citems;
+    counts[3] = static_cast<int64_t>(k[4]);
+    counts[4] = static_cast<int64_t>(k[5]);
+    counts[5] = c->batch_reruns;
+    return RR_OK;
+}
 
 extern "C" int32_t rr_hip_record_scan(rr_hip_ctx *c, uint32_t *data, uint32_t *block_sums, uint32_t len_elems) {
     if (!c) return RR_E_INVARIANT;

@@ -595,10 +595,14 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
             if (bands > (1ll << 30)) return fail(c, RR_E_INVARIANT, "batched launch: too many bands");
         }
     }
-    RR_HIP_TRY(c, hipMemcpyAsync(c->pages, c->host_pages.data(), sizeof(DevPage) * b.n_pages, hipMemcpyHostToDevice,
-                                 c->stream));
-    // The host tables are reused by the next recording: let the copies land.
-    RR_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->capturing && b.launch != RR_LAUNCH_BATCHED)
+        return fail(c, RR_E_INVARIANT, "graph capture covers the batched launch shape only");
+    if (!c->capturing) {
+        RR_HIP_TRY(c, hipMemcpyAsync(c->pages, c->host_pages.data(), sizeof(DevPage) * b.n_pages,
+                                     hipMemcpyHostToDevice, c->stream));
+        // The host tables are reused by the next recording: let the copies land.
+        RR_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
 
     FillArgs A{};
     A.recs = c->recs;
@@ -621,11 +625,12 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     A.trace_cap = c->trace_cap;
     A.trace_top = c->trace_top;
 
-    if (!c->page_timed) {
+    if (!c->page_timed && !c->capturing) {
         RR_HIP_TRY(c, hipEventRecord(c->t0, c->stream));
         c->page_timed = true;
     }
-    RR_HIP_TRY(c, hipMemsetAsync(c->spill_top, 0, sizeof(unsigned long long), c->stream));
+    rc = rrint::zero_u64(c, c->spill_top, 1);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_edge_setup, dim3(static_cast<unsigned>(b.n_paths)), dim3(256), 0, c->stream, b.edges,
                        b.edge_off, c->pages, b.n_pages, c->recs, c->hdrs, c->rows, c->sel, c->nb, b.mode);
     RR_HIP_TRY(c, hipGetLastError());
@@ -669,6 +674,7 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
 }  // namespace
 
 namespace rrint {
+int32_t record_pages_internal(rr_hip_ctx *c, const rr_fill_pages &b) { return record_pages(c, b); }
 hipStream_t stream(rr_hip_ctx *c) { return c->stream; }
 int device(rr_hip_ctx *c) { return c->device; }
 bool page_open(rr_hip_ctx *c) { return c->page_open; }
@@ -701,7 +707,7 @@ int32_t rr_hip_create(int32_t device, rr_hip_ctx **out) {
     if (rc == RR_OK) rc = rrint::init_batched(c);
     if (rc == RR_OK && (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess)) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->spill_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
-    if (rc == RR_OK && hipMalloc(&c->counters, 4 * sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
+    if (rc == RR_OK && hipMalloc(&c->counters, 8 * sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->err_flag, sizeof(int32_t)) != hipSuccess) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->trace_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
     for (auto &f : c->fences)

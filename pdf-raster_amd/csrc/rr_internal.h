@@ -85,7 +85,17 @@ struct rr_hip_ctx {
     size_t band_list_cap = 0;
     rrdev::CPage *cpages = nullptr;
     size_t cpages_cap = 0;
-    unsigned long long *counters = nullptr;  // [0] span items served, [1] composite items, [2] span pool top
+    // [0] span items served, [1] composite items served, [2] span pool top,
+    // [3] intervals stored, [4] (path, tile) composites, [5] tiles touched
+    unsigned long long *counters = nullptr;
+    // profiling event pairs of the last batched recording: phase A [a0, a1), phase B b
+    size_t prof_a0 = 0, prof_a1 = 0, prof_b = 0;
+    int64_t batch_items = 0, batch_citems = 0, batch_reruns = 0;
+    bool batch_valid = false;
+    // graph capture of a batched recording: while capturing, nothing syncs or
+    // uploads (the tables and sizes come from the eager sizing pass before it)
+    bool capturing = false, size_only = false;
+    uint64_t ws_gen = 0;  // bumped whenever a workspace array is reallocated
     int span_waves[rrdev::kNumVariants] = {};
     int compose_waves[rrdev::kNumVariants] = {};
     std::vector<rrdev::CPage> host_cpages;
@@ -124,6 +134,8 @@ void count_launch(rr_hip_ctx *c);
 template <typename T>
 int32_t ensure(rr_hip_ctx *c, T **buf, size_t *cap, size_t need) {
     if (need <= *cap) return RR_OK;
+    if (c->capturing) return fail(c, RR_E_INVARIANT, "workspace would grow during graph capture");
+    ++c->ws_gen;
     size_t n = *cap ? *cap : 1024;
     while (n < need) n *= 2;
     hipError_t e = hipSuccess;
@@ -147,4 +159,8 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const rrdev::FillA
                        int chunk_px, int variant);
 // Occupancy-derived persistent wave counts of the batched kernels.
 int32_t init_batched(rr_hip_ctx *c);
+// Zero n 64-bit words with a kernel (safe inside graph capture).
+int32_t zero_u64(rr_hip_ctx *c, unsigned long long *p, long long n);
+// Records the edge setup + fill of b (rr_fill.hip); used by graph capture.
+int32_t record_pages_internal(rr_hip_ctx *c, const rr_fill_pages &b);
 }  // namespace rrint

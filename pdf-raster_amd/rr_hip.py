@@ -120,6 +120,10 @@ def lib():
             "rr_hip_record_composite": (i32, [P, P, P, i64]),
             "rr_hip_record_apply_soft_mask": (i32, [P, P, P, i64]),
             "rr_hip_record_scan": (i32, [P, P, P, ctypes.c_uint32]),
+            "rr_hip_last_batch_stats": (i32, [P, P, P]),
+            "rr_hip_graph_create": (i32, [P, ctypes.POINTER(FillPages), ctypes.POINTER(ctypes.c_void_p)]),
+            "rr_hip_graph_launch": (i32, [P, P]),
+            "rr_hip_graph_destroy": (i32, [P, P]),
             "rr_images_create": (i32, [u64, i32, i32, ctypes.POINTER(ctypes.c_void_p)]),
             "rr_images_destroy": (None, [P]),
             "rr_images_info": (i32, [P, P]),
@@ -384,6 +388,15 @@ class Backend:
         self._check(self.L.rr_hip_record_apply_soft_mask(self.h, pixels, mask, n_pixels),
                     "record_apply_soft_mask")
 
+    def last_batch_stats(self):
+        """({span_ms, compose_ms}, {items, intervals, composite_items, path_tiles,
+        tiles, reruns}) of the last batched recording."""
+        ms = np.zeros(2, np.float64)
+        cnt = np.zeros(6, np.int64)
+        self._check(self.L.rr_hip_last_batch_stats(self.h, _p(ms), _p(cnt)), "last_batch_stats")
+        keys = ("items", "intervals", "composite_items", "path_tiles", "tiles", "reruns")
+        return {"span_ms": float(ms[0]), "compose_ms": float(ms[1])}, dict(zip(keys, (int(v) for v in cnt)))
+
     def record_scan(self, data, block_sums, n):
         self._check(self.L.rr_hip_record_scan(self.h, data, block_sums, n), "record_scan")
 
@@ -557,6 +570,23 @@ class DeviceBatch:
         self.be.begin_page()
         self.be.record_fill_pages(self.fill_pages(mode, flavour, launch, hashes, counts, variant))
         self.be.wait_page(self.be.submit_page())
+
+    def capture(self, mode=MODE_DOCUMENTED, flavour=FLAVOUR_GRAY8, hashes=False, counts=False, variant=0):
+        """Capture the batched fill of this batch as a HIP graph; returns a handle."""
+        b = self.fill_pages(mode, flavour, LAUNCH_BATCHED, hashes, counts, variant)
+        g = ctypes.c_void_p()
+        self.be._check(self.be.L.rr_hip_graph_create(self.be.h, ctypes.byref(b), ctypes.byref(g)),
+                       "graph_create")
+        return g.value
+
+    def run_graph(self, graph, background=255):
+        self.reset(background)
+        self.be.begin_page()
+        self.be._check(self.be.L.rr_hip_graph_launch(self.be.h, graph), "graph_launch")
+        self.be.wait_page(self.be.submit_page())
+
+    def destroy_graph(self, graph):
+        self.be._check(self.be.L.rr_hip_graph_destroy(self.be.h, graph), "graph_destroy")
 
     def autotune(self, mode=MODE_DOCUMENTED, flavour=FLAVOUR_GRAY8, launch=LAUNCH_BATCHED, reps=2):
         """Time every band-fill variant on this batch; returns (best, [ms per variant])."""
