@@ -156,7 +156,18 @@ struct BatchArgs {
     const CPage *cpages;
     int32_t n_cpages;
     unsigned long long *counters;
+    int32_t *occ;  // per composite tile: highest opaque path index covering it fully
+    int32_t chunk_px;
+    int32_t *pstate;  // per path: bit 0 = widened by an unbalanced row, bits 1.. = skipped items
 };
+
+// Span pass modes: every item, counting drawn widths and skipping occluded
+// items (skipping only when no per-path coverage output is asked for); the
+// same without counting (a rerun after growing the interval pool); every item
+// without skipping (the sizing pass before a capture); or only the skipped
+// items of paths an unbalanced row widened after they were skipped.
+enum { SPAN_ALL = 0, SPAN_ALL_NODRAW = 1, SPAN_ALL_NOSKIP = 2, SPAN_FIXUP = 3 };
+constexpr long long kSkipped = -2;  // SpanHdr::off of an item skipped as occluded
 
 // Span items of path p: (p, first row of each band it draws), in band order,
 // and the per-band path counts.  One wave per path.
@@ -250,7 +261,7 @@ struct SpanCursor {
 
 template <int K>
 __device__ void span_band(const BatchArgs &B, SpanLds<K> &L, int64_t p, int32_t Y0, long long it, int64_t slot,
-                          bool count_draw, SpanCursor &cur) {
+                          bool count_draw, bool occlusion, SpanCursor &cur) {
     const FillArgs &A = B.F;
     const int lane = threadIdx.x;
     const PathHdr h = A.hdrs[p];
@@ -259,6 +270,35 @@ __device__ void span_band(const BatchArgs &B, SpanLds<K> &L, int64_t p, int32_t 
     const int64_t e0 = A.edge_off[p], e1 = A.edge_off[p + 1];
     const DevPage &pg = A.pages[h.page];
     const bool literal = A.mode == RR_MODE_LITERAL;
+    SpanHdr *H = B.shdr + it;
+    const int32_t band = Y0 / kBandRows;
+    const int32_t *occ_row = B.occ + pg.cbase + static_cast<long long>(band) * pg.ctiles;
+    if (occlusion && s.z <= s.w) {
+        // Every composite tile this item can reach already has a later opaque
+        // path covering it fully: the composite pass starts after that path,
+        // so these intervals would never be read.  The per-path state word
+        // keeps a skip from racing with a widening of the path's columns.
+        bool hidden = true;
+        for (int32_t t = s.z / B.chunk_px + lane; hidden && t <= s.w / B.chunk_px; t += 64) hidden = occ_row[t] > p;
+        hidden = __ballot(!hidden) == 0;
+        if (hidden) {
+            int32_t old = 0;
+            if (lane == 0) {
+                old = atomicAdd(B.pstate + p, 2);
+                if (old & 1) atomicSub(B.pstate + p, 2);
+            }
+            hidden = (__builtin_amdgcn_readfirstlane(old) & 1) == 0;
+        }
+        if (hidden) {
+            H->cnt[lane] = 0;
+            if (lane == 0) {
+                H->off = kSkipped;
+                H->fx0 = 1;
+                H->fx1 = 0;
+            }
+            return;
+        }
+    }
 
     const int32_t d = lane >> 2, j = lane & 3;
     const int32_t y = Y0 + d;
@@ -269,7 +309,6 @@ __device__ void span_band(const BatchArgs &B, SpanLds<K> &L, int64_t p, int32_t 
     const int32_t mrule = at.eo ? 1 : -1;
     const int32_t aa_width = pg.width * 4;
 
-    SpanHdr *H = B.shdr + it;
     int32_t *X0 = L.ex0, *X1 = L.ex1, *C = L.ec;
     int n = build_entries(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, K);
     int need = n;
@@ -288,6 +327,8 @@ __device__ void span_band(const BatchArgs &B, SpanLds<K> &L, int64_t p, int32_t 
                 if (lane == 0) {
                     atomicOr(A.err, 1);
                     H->off = -1;
+                    H->fx0 = 1;
+                    H->fx1 = 0;
                 }
                 H->cnt[lane] = 0;
                 return;
@@ -301,8 +342,13 @@ __device__ void span_band(const BatchArgs &B, SpanLds<K> &L, int64_t p, int32_t 
     }
     bool unb = false;
     const int nint = sort_and_walk(n, X0, X1, C, mrule, aa_width, &unb);
-    // an unbalanced row spans to the canvas edge: widen the path's columns
-    if (__ballot(unb) != 0 && lane == 0) atomicMax(reinterpret_cast<int32_t *>(B.sel + p) + 3, pg.px_hi);
+    // an unbalanced row spans to the canvas edge: widen the path's columns;
+    // items of this path skipped before the widening are redone by the fixup
+    if (__ballot(unb) != 0 && lane == 0) {
+        atomicMax(reinterpret_cast<int32_t *>(B.sel + p) + 3, pg.px_hi);
+        const int32_t old = atomicOr(B.pstate + p, 1);
+        if ((old & 1) == 0 && (old >> 1) > 0) atomicAdd(B.counters + 6, 1ull);
+    }
     if (A.cov_count && count_draw) {
         // draw_aa_line range of each output row (scanner/mod.rs:459-466)
         int32_t smin = nint ? X0[lane] : INT32_MAX;
@@ -318,6 +364,29 @@ __device__ void span_band(const BatchArgs &B, SpanLds<K> &L, int64_t p, int32_t 
         }
         for (int o = 32; o > 0; o >>= 1) w_draw += __shfl_xor(w_draw, o);
         if (lane == 0 && w_draw) atomicAdd(reinterpret_cast<unsigned long long *>(A.cov_count + 2 * p + 1), w_draw);
+    }
+    // columns fully covered on all 64 AA rows: per row the longest interval's
+    // whole pixels, intersected over the band (a subset of the exact set)
+    int32_t fx0 = INT32_MAX, fx1 = INT32_MIN;
+    if (nint > 0) {
+        int32_t best = -1;
+        for (int k = 0; k < nint; ++k) {
+            const int32_t a = X0[k * 64 + lane], b = X1[k * 64 + lane];
+            if (b - a > best) {
+                best = b - a;
+                fx0 = (a + 3) >> 2;
+                fx1 = (b >> 2) - 1;
+            }
+        }
+    }
+    {
+        int32_t lo = nint > 0 ? fx0 : INT32_MAX, hi = nint > 0 ? fx1 : INT32_MIN;
+        for (int o = 32; o > 0; o >>= 1) {
+            lo = max(lo, __shfl_xor(lo, o));
+            hi = min(hi, __shfl_xor(hi, o));
+        }
+        fx0 = lo;
+        fx1 = hi;
     }
     // store the intervals: lane-major, one int2 per interval
     const int32_t incl = wave_incl_add(nint);
@@ -341,16 +410,37 @@ __device__ void span_band(const BatchArgs &B, SpanLds<K> &L, int64_t p, int32_t 
         for (int k = 0; k < nint; ++k) dst[k] = make_int2(X0[k * 64 + lane], X1[k * 64 + lane]);
     }
     H->cnt[lane] = fits ? static_cast<uint16_t>(nint) : static_cast<uint16_t>(0);
-    if (lane == 0) H->off = fits ? static_cast<long long>(off) : -1ll;
+    if (lane == 0) {
+        H->off = fits ? static_cast<long long>(off) : -1ll;
+        H->fx0 = fx0;
+        H->fx1 = fx1;
+    }
+    // publish: composite tiles this opaque item covers fully (the same test
+    // as the composite pass's occlusion scan)
+    if (occlusion && at.a_input == 255 && Y0 + kBandRows <= pg.height) {
+        const int32_t lo = max(fx0, s.z), hi = min(fx1, s.w);
+        if (lo <= hi) {
+            const int32_t ch = B.chunk_px;
+            const int32_t ta = (lo + ch - 1) / ch;
+            for (int32_t t = ta + lane; t < pg.ctiles; t += 64) {
+                const int32_t txe = min((t + 1) * ch, pg.width) - 1;
+                if (txe > hi) break;
+                atomicMax(const_cast<int32_t *>(occ_row) + t, static_cast<int32_t>(p));
+            }
+        }
+    }
 }
 
 constexpr int kSpanGrab = 8;  // span items taken per queue atomic
 
 template <int K>
-__global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t count_draw) {
+__global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
     __shared__ SpanLds<K> L;
     const int lane = threadIdx.x;
     const long long n_items = min(B.item_base[B.n_paths], B.sitems_cap);
+    if (mode == SPAN_FIXUP && B.counters[6] == 0) return;
+    const bool count_draw = mode == SPAN_ALL;
+    const bool occlusion = (mode == SPAN_ALL || mode == SPAN_ALL_NODRAW) && !(B.F.cov_hash || B.F.cov_count);
     SpanCursor cur;
     for (;;) {
         // items are pulled kSpanGrab at a time: one queue atomic per group
@@ -359,11 +449,14 @@ __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t count_draw) {
         it0 = __shfl(it0, 0);
         if (static_cast<long long>(it0) >= n_items) break;
         const long long it_end = min(static_cast<long long>(it0) + kSpanGrab, n_items);
-        for (long long it = static_cast<long long>(it0); it < it_end; ++it) {
+        for (long long k = static_cast<long long>(it0); k < it_end; ++k) {
+            // descending path order, so that occluders are met first
+            const long long it = n_items - 1 - k;
             const int2 item = B.sitems[it];
+            if (mode == SPAN_FIXUP && (B.shdr[it].off != kSkipped || (B.pstate[item.x] & 1) == 0)) continue;
             long long t0 = 0;
             if (B.F.trace) t0 = static_cast<long long>(wall_clock64());
-            span_band<K>(B, L, item.x, item.y, it, blockIdx.x, count_draw != 0, cur);
+            span_band<K>(B, L, item.x, item.y, it, blockIdx.x, count_draw, occlusion, cur);
             if (B.F.trace && lane == 0) {
                 const long long t1 = static_cast<long long>(wall_clock64());
                 const unsigned long long k = atomicAdd(B.F.trace_top, 1ull);
@@ -660,32 +753,70 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, int32_t page
     const int32_t txe = min(tx0 + CH, width) - 1;
     if (txe < pg.px_lo || tx0 > pg.px_hi) return;
     const long long gb = static_cast<long long>(pg.band_base) + band;
-    ListCursor cur;
-    cur.lb = B.band_base[gb] - 64;
-    cur.l1 = B.band_base[gb + 1];
-    cur.mask = 0;
-    PathRef P0, P1;
-    if (!list_next(B, pg, cur, tx0, txe, P0)) return;
-    ++*n_tiles;
+    const long long l0 = B.band_base[gb], l1 = B.band_base[gb + 1];
     const int32_t nrows = min(kBandRows, pg.height - Y0);
     const int32_t ndw = ((txe - tx0) >> 2) + 1;
+    // Occlusion: the last opaque path that covers the whole tile on all 64 AA
+    // rows leaves exactly its colour there (both composites give src for
+    // coverage 16 at a_input 255), so compositing starts after it on a tile
+    // filled with that colour.  Skipped when per-path coverage is counted.
+    long long start = l0;
+    int32_t fill_color = -1;
+    if (!(A.cov_hash || A.cov_count) && nrows == kBandRows) {
+        long long last = -1;
+        for (long long lb = l0; lb < l1; lb += 64) {
+            const long long q = lb + lane;
+            bool full = false;
+            if (q < l1) {
+                const int2 ent = B.band_list[q];
+                if (ent.x >= pg.path_begin && ent.x < pg.path_end && ent.y >= 0 && ent.y < B.sitems_cap) {
+                    const int4 s = B.sel[ent.x];
+                    if (s.z <= tx0 && s.w >= txe) {
+                        const SpanHdr *H = B.shdr + ent.y;
+                        full = H->fx0 <= tx0 && H->fx1 >= txe && A.attrs[ent.x].a_input == 255;
+                    }
+                }
+            }
+            const unsigned long long m = __ballot(full);
+            if (m) last = lb + 63 - __clzll(m);
+        }
+        if (last >= 0) {
+            start = last + 1;
+            fill_color = A.attrs[B.band_list[last].x].color;
+        }
+    }
+    ListCursor cur;
+    cur.lb = start - 64;
+    cur.l1 = l1;
+    cur.mask = 0;
+    PathRef P0, P1;
+    const bool has0 = list_next(B, pg, cur, tx0, txe, P0);
+    if (!has0 && fill_color < 0) return;
+    ++*n_tiles;
     uint8_t *const base = pg.canvas + static_cast<long long>(Y0) * pg.stride + tx0;
     const int32_t stride = pg.stride;
-    long long off0, off1 = -1;
-    int32_t n0, n1 = 0;
-    hdr_fetch(B, P0.hi, &off0, &n0);
-    bool has1 = list_next(B, pg, cur, tx0, txe, P1);
-    if (has1) hdr_fetch(B, P1.hi, &off1, &n1);
+    long long off0 = -1, off1 = -1;
+    int32_t n0 = 0, n1 = 0;
+    bool has1 = false;
+    if (has0) {
+        hdr_fetch(B, P0.hi, &off0, &n0);
+        has1 = list_next(B, pg, cur, tx0, txe, P1);
+        if (has1) hdr_fetch(B, P1.hi, &off1, &n1);
+    }
+    const uint32_t fill4 = static_cast<uint32_t>(fill_color) * 0x01010101u;
     for (int idx = lane; idx < kBandRows * kRowWords; idx += 64) {
         const int r = idx / kRowWords, w = idx % kRowWords;
         if (r < nrows && w < ndw)
             L.cv[r * kRowPitch + w] =
-                *reinterpret_cast<const uint32_t *>(base + static_cast<long long>(r) * stride + 4 * w);
+                fill_color >= 0 ? fill4
+                                : *reinterpret_cast<const uint32_t *>(base + static_cast<long long>(r) * stride + 4 * w);
     }
-    SpanView V0 = span_fetch(B, off0, n0);
+    SpanView V0;
+    V0.n = 0;
+    if (has0) V0 = span_fetch(B, off0, n0);
     __syncthreads();
-    if (off0 < 0 && lane == 0) atomicOr(A.err, 8);
-    for (;;) {
+    if (has0 && off0 < 0 && lane == 0) atomicOr(A.err, 8);
+    for (; has0;) {
         const int32_t xa = max(P0.s.z, tx0), xb = min(P0.s.w, txe);
         const int32_t ya = max(P0.s.x, Y0), yb = min(P0.s.y, Y0 + nrows - 1);
         const bool live = xa <= xb && ya <= yb;
@@ -858,24 +989,9 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     if (!rc) rc = ensure(c, &c->cpages, &c->cpages_cap, static_cast<size_t>(b.n_pages));
     if (rc) return rc;
 
-    // composite items: pages with more paths first, bands x tiles each
-    std::vector<int32_t> order(static_cast<size_t>(b.n_pages));
-    for (int32_t i = 0; i < b.n_pages; ++i) order[static_cast<size_t>(i)] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-        return (b.pages[x].path_end - b.pages[x].path_begin) > (b.pages[y].path_end - b.pages[y].path_begin);
-    });
-    c->host_cpages.resize(static_cast<size_t>(b.n_pages));
-    long long n_citems = 0;
-    for (size_t k = 0; k < order.size(); ++k) {
-        const rr_page_desc &pg = b.pages[order[k]];
-        CPage &cp = c->host_cpages[k];
-        cp.page = order[k];
-        cp.tiles = (pg.width + chunk_px - 1) / chunk_px;
-        cp.base = static_cast<int32_t>(n_citems);
-        cp.pad = 0;
-        n_citems += static_cast<long long>(cp.tiles) * ((pg.height + kBandRows - 1) / kBandRows);
-        if (n_citems >= 0x7fffffffll) return fail(c, RR_E_INVARIANT, "batched launch: too many composite tiles");
-    }
+    long long n_citems = c->batch_citems;  // set with the page table by the caller
+    rc = ensure(c, &c->occ, &c->occ_cap, static_cast<size_t>(std::max<long long>(n_citems, 1)));
+    if (rc) return rc;
     if (!c->capturing)
         RR_TRY(c, hipMemcpyAsync(c->cpages, c->host_cpages.data(), sizeof(CPage) * b.n_pages,
                                  hipMemcpyHostToDevice, c->stream));
@@ -885,6 +1001,9 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     if (!rc) rc = fill<int32_t>(c, c->band_cnt, n_bands, 0);
     if (!rc) rc = fill<unsigned long long>(c, c->counters, 8, 0ull);
     if (!rc) rc = fill<unsigned long long>(c, c->spill_top, 1, 0ull);
+    if (!rc) rc = fill<int32_t>(c, c->occ, n_citems, -1);
+    if (!rc) rc = ensure(c, &c->pstate, &c->pstate_cap, static_cast<size_t>(n_paths));
+    if (!rc) rc = fill<int32_t>(c, c->pstate, n_paths, 0);
     if (rc) return rc;
     long long n_items = c->batch_items;  // while capturing: the sizing pass's count
     if (!c->capturing) {
@@ -932,43 +1051,56 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     B.cpages = c->cpages;
     B.n_cpages = b.n_pages;
     B.counters = c->counters;
-    int32_t count_draw = 1;
-    void *a_args[] = {&B, &count_draw};
+    B.occ = c->occ;
+    B.chunk_px = chunk_px;
+    B.pstate = c->pstate;
+    // the sizing pass before a capture counts every interval (no skipping)
+    int32_t mode = c->size_only ? SPAN_ALL_NOSKIP : SPAN_ALL;
+    void *a_args[] = {&B, &mode};
     c->batch_valid = true;
     c->batch_items = n_items;
-    c->batch_citems = n_citems;
     c->batch_reruns = 0;
     c->prof_a0 = c->prof_used;
-    rc = launch_profiled(c, span_kernel(k_entries), a_waves, a_args);
-    if (rc) return rc;
-    if (c->capturing) {
-        // sized by the eager pass: the pool holds every interval plus each
-        // wave's unused chunk tail
-        c->prof_a1 = c->prof_b = c->prof_used;
-        const unsigned b_waves = static_cast<unsigned>(std::min<long long>(c->compose_waves[variant], n_citems));
-        void *b_args[] = {&B, &n_citems};
-        return launch_profiled(c, compose_kernel(chunk_px), b_waves, b_args);
-    }
-    unsigned long long used = 0;
-    rc = read_back(c, c->counters + 2, &used);
-    if (rc) return rc;
-    if (c->size_only) {
-        c->prof_a1 = c->prof_b = c->prof_used;
-        return RR_OK;
-    }
-    if (static_cast<long long>(used) > B.spans_cap) {
-        // the span pool was too small: grow it and redo phase A (the canvas
-        // is untouched so far; the drawn-width counts are already complete)
-        rc = ensure(c, &c->spans, &c->spans_cap, static_cast<size_t>(used + used / 4));
+    for (int attempt = 0;; ++attempt) {
+        const int32_t pass_mode = mode;
+        rc = launch_profiled(c, span_kernel(k_entries), a_waves, a_args);
+        if (rc) return rc;
+        if (pass_mode != SPAN_ALL_NOSKIP) {
+            // skipped items of paths widened afterwards (exits at once when none)
+            mode = SPAN_FIXUP;
+            rc = launch_profiled(c, span_kernel(k_entries), a_waves, a_args);
+            if (rc) return rc;
+        }
+        if (c->capturing) {
+            // sized by the eager pass: the pool holds every interval plus each
+            // wave's unused chunk tail
+            c->prof_a1 = c->prof_b = c->prof_used;
+            const unsigned b_waves = static_cast<unsigned>(std::min<long long>(c->compose_waves[variant], n_citems));
+            void *b_args[] = {&B, &n_citems};
+            return launch_profiled(c, compose_kernel(chunk_px), b_waves, b_args);
+        }
+        unsigned long long used = 0;
+        rc = read_back(c, c->counters + 2, &used);
+        if (rc) return rc;
+        if (c->size_only) {
+            c->prof_a1 = c->prof_b = c->prof_used;
+            return RR_OK;
+        }
+        if (static_cast<long long>(used) <= B.spans_cap) break;
+        if (attempt >= 4) return fail(c, RR_E_CAPACITY, "interval pool still too small after growing it");
+        // the interval pool was too small: grow it and redo the span pass
+        // (the canvas is untouched so far; the drawn-width counts are complete)
+        rc = ensure(c, &c->spans, &c->spans_cap,
+                    std::max(static_cast<size_t>(used + used / 4), 2 * static_cast<size_t>(B.spans_cap)));
         if (!rc) rc = fill<unsigned long long>(c, c->counters, 8, 0ull);
         if (!rc) rc = fill<unsigned long long>(c, c->spill_top, 1, 0ull);
+        if (!rc) rc = fill<int32_t>(c, c->occ, n_citems, -1);
+        if (!rc) rc = fill<int32_t>(c, c->pstate, n_paths, 0);
         if (rc) return rc;
         B.spans = c->spans;
         B.spans_cap = static_cast<long long>(c->spans_cap);
-        count_draw = 0;
-        c->batch_reruns = 1;
-        rc = launch_profiled(c, span_kernel(k_entries), a_waves, a_args);
-        if (rc) return rc;
+        mode = SPAN_ALL_NODRAW;
+        c->batch_reruns = attempt + 1;
     }
     c->prof_a1 = c->prof_used;
     c->prof_b = c->prof_used;

@@ -49,6 +49,8 @@ struct DevPage {
     int32_t width, height, stride;
     int32_t px_lo, px_hi;  // clip ∩ canvas, pixels
     int32_t band_base;     // batch-global index of the page's first 16-row band
+    int32_t cbase, ctiles;  // batched: first composite tile item of the page, tiles per band
+    int32_t pad2;
 };
 
 struct FillArgs {

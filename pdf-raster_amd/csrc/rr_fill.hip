@@ -572,6 +572,9 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
         d.px_lo = std::max(0, d.clip.x_min_i);
         d.px_hi = std::min(p.width - 1, d.clip.x_max_i);
         d.band_base = 0;
+        d.cbase = 0;
+        d.ctiles = 0;
+        d.pad2 = 0;
     }
     if (prev_end != b.n_paths) return fail(c, RR_E_INVARIANT, "pages must tile [0, n_paths) in order");
     RR_HIP_TRY(c, hipSetDevice(c->device));
@@ -594,6 +597,29 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
             bands += (b.pages[i].height + kBandRows - 1) / kBandRows;
             if (bands > (1ll << 30)) return fail(c, RR_E_INVARIANT, "batched launch: too many bands");
         }
+        // composite tile items: pages with more paths first, bands x tiles each
+        const int32_t chunk_px = kVariantCH[b.variant];
+        std::vector<int32_t> order(static_cast<size_t>(b.n_pages));
+        for (int32_t i = 0; i < b.n_pages; ++i) order[static_cast<size_t>(i)] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+            return (b.pages[x].path_end - b.pages[x].path_begin) > (b.pages[y].path_end - b.pages[y].path_begin);
+        });
+        c->host_cpages.resize(static_cast<size_t>(b.n_pages));
+        long long n_citems = 0;
+        for (size_t k = 0; k < order.size(); ++k) {
+            const rr_page_desc &pg = b.pages[order[k]];
+            CPage &cp = c->host_cpages[k];
+            cp.page = order[k];
+            cp.tiles = (pg.width + chunk_px - 1) / chunk_px;
+            cp.base = static_cast<int32_t>(n_citems);
+            cp.pad = 0;
+            DevPage &d = c->host_pages[static_cast<size_t>(order[k])];
+            d.cbase = cp.base;
+            d.ctiles = cp.tiles;
+            n_citems += static_cast<long long>(cp.tiles) * ((pg.height + kBandRows - 1) / kBandRows);
+            if (n_citems >= 0x7fffffffll) return fail(c, RR_E_INVARIANT, "batched launch: too many composite tiles");
+        }
+        c->batch_citems = n_citems;
     }
     if (c->capturing && b.launch != RR_LAUNCH_BATCHED)
         return fail(c, RR_E_INVARIANT, "graph capture covers the batched launch shape only");
@@ -731,7 +757,7 @@ int32_t rr_hip_destroy(rr_hip_ctx *c) {
     void *bufs[] = {c->recs,      c->hdrs,     c->rows,       c->sel,     c->nb,        c->csum,
                     c->pages,     c->slots,    c->spill,      c->spill_top, c->err_flag, c->trace_top,
                     c->item_base, c->scan_tmp, c->sitems,     c->shdr,    c->spans,     c->band_cnt,
-                    c->band_base, c->band_list, c->cpages,    c->counters};
+                    c->band_base, c->band_list, c->cpages,    c->counters, c->occ,      c->pstate};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->t0) (void)hipEventDestroy(c->t0);

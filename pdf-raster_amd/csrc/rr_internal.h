@@ -26,11 +26,14 @@ struct Fence {
 // Spans of one (path, 16-row band) item written by the batched span pass:
 // lane l's disjoint sorted intervals [s, e) (AA x units) are the cnt[l]
 // int2 records starting at off + sum(cnt[0..l)).  off = -1: not stored.
+// fx0..fx1: pixel columns that every one of the band's 64 AA rows covers
+// fully (empty when fx0 > fx1): an opaque path paints them with its colour.
 struct SpanHdr {
     long long off;
+    int32_t fx0, fx1;
     uint16_t cnt[64];
 };
-static_assert(sizeof(SpanHdr) == 136, "SpanHdr layout");
+static_assert(sizeof(SpanHdr) == 144, "SpanHdr layout");
 
 // Composite work of one page: tiles per band and the index of its first
 // (band, tile) item; pages are listed in the order their items are served.
@@ -85,6 +88,10 @@ struct rr_hip_ctx {
     size_t band_list_cap = 0;
     rrdev::CPage *cpages = nullptr;
     size_t cpages_cap = 0;
+    int32_t *occ = nullptr;  // per composite tile item: highest opaque path covering it fully, or -1
+    size_t occ_cap = 0;
+    int32_t *pstate = nullptr;  // per path: widened / skipped-item bookkeeping of the span pass
+    size_t pstate_cap = 0;
     // [0] span items served, [1] composite items served, [2] span pool top,
     // [3] intervals stored, [4] (path, tile) composites, [5] tiles touched
     unsigned long long *counters = nullptr;

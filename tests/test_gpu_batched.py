@@ -20,10 +20,11 @@ def oracle_pages(pages, mode=O.MODE_DOCUMENTED, flavour=O.FLAVOUR_GRAY8):
     return canv, np.concatenate(hs)
 
 
-def check(be, pages, mode=R.MODE_DOCUMENTED, flavour=R.FLAVOUR_GRAY8, launch=R.LAUNCH_BATCHED):
+def check(be, pages, mode=R.MODE_DOCUMENTED, flavour=R.FLAVOUR_GRAY8, launch=R.LAUNCH_BATCHED, hashes=True,
+          variant=0):
     db = R.DeviceBatch(be, pages)
     try:
-        db.run(mode, flavour, launch, hashes=True)
+        db.run(mode, flavour, launch, hashes=hashes, variant=variant)
         got = [db.read_canvas(i) for i in range(len(pages))]
         gh = db.read_hashes()
     finally:
@@ -33,8 +34,9 @@ def check(be, pages, mode=R.MODE_DOCUMENTED, flavour=R.FLAVOUR_GRAY8, launch=R.L
         if not np.array_equal(g, o):
             ys, xs = np.nonzero(g != o)
             raise AssertionError(f"page {i}: {len(ys)} px differ, first y={ys[0]} x={xs[0]}")
-    bad = np.nonzero(gh != oh)[0]
-    assert len(bad) == 0, f"coverage hashes differ for paths {bad[:10]}"
+    if hashes:
+        bad = np.nonzero(gh != oh)[0]
+        assert len(bad) == 0, f"coverage hashes differ for paths {bad[:10]}"
 
 
 def test_batched_polygon_subset(backend):
@@ -81,3 +83,29 @@ def test_every_variant_is_bit_exact(backend, variant):
     for g, gp, o in zip(got, got_p, oc):
         assert np.array_equal(g, o) and np.array_equal(gp, o)
     assert np.array_equal(gh, oh) and np.array_equal(gh_p, oh)
+
+
+@pytest.mark.parametrize("mode", [R.MODE_DOCUMENTED, R.MODE_LITERAL])
+def test_batched_without_coverage_outputs(backend, mode):
+    # Without per-path coverage outputs the composite pass starts each tile at
+    # the last opaque path covering it fully; the pixels must not change.
+    # Whole pages, so that large opaque paths overlap many others.
+    # A fresh context, so the first recording also overflows the initial
+    # interval pool and goes through the grow-and-redo path.
+    pages = R.workload(3, 0xC0F10003, 3)
+    oc, _ = oracle_pages(pages, mode)
+    be = R.Backend(0)
+    db = R.DeviceBatch(be, pages)
+    try:
+        for variant in (0, 2, 7, 11):
+            db.run(mode, R.FLAVOUR_GRAY8, R.LAUNCH_BATCHED, variant=variant)
+            for i, o in enumerate(oc):
+                g = db.read_canvas(i)
+                if not np.array_equal(g, o):
+                    ys, xs = np.nonzero(g != o)
+                    raise AssertionError(f"variant {variant} page {i}: {len(ys)} px differ, first y={ys[0]} x={xs[0]}")
+            if variant == 0:
+                assert be.last_batch_stats()[1]["reruns"] >= 1  # the grow-and-redo path ran
+    finally:
+        db.free()
+        be.close()
