@@ -10,6 +10,7 @@ also run through the plain one-path-per-launch route for the batched/plain
 ratio, and through the CPU oracle on a time-bounded sample.
 """
 import argparse
+import ctypes
 import hashlib
 import json
 import os
@@ -45,6 +46,12 @@ WORKLOADS = {
                   "and Multiply alternating, 4 per 2550x3300 grey page; one launch per image"),
 }
 IMAGES_PER_PAGE = 4
+WORKLOADS["dispatch"] = (5, 0xC0F10005, 0, None,
+                         "learned dispatch over a glyph-page stream (6000 glyphs per Letter@300 page); "
+                         "MLP trained on span-pass traces of 8 pages, evaluated on 32 held-out pages, "
+                         "a quarter of them also carrying 4 CMYK/RGB images each")
+DISPATCH_TRAIN_PAGES, DISPATCH_EVAL_PAGES = 8, 32
+GPU_ROUTES = (R.ROUTE_SPAN, R.ROUTE_TILE16, R.ROUTE_TILE32, R.ROUTE_TILE64)
 
 
 def image_bbox_px(im, page_w, page_h):
@@ -204,6 +211,220 @@ def bench_images(args, desc):
     print(json.dumps(out), flush=True)
 
 
+def route_valid(bands):
+    """(n, 5) validity of each class for paths with `bands` 16-row bands."""
+    b = np.asarray(bands)
+    return np.stack([b > 0, (b > 0) & (b <= 1), (b > 0) & (b <= 2), (b > 0) & (b <= 4), b > 0], 1)
+
+
+def traced_costs(be, db, route, variant, reps=2):
+    """Span-pass cost of every path with all paths on `route`: the wave time of
+    its work items scaled by (kernel time / total wave time), i.e. its share of
+    the span kernel's wall time, in microseconds (min over reps)."""
+    n = db.n_paths
+    cap = 8 * n + 4096
+    buf = be.alloc(cap * 32)
+    best = np.full(n, np.inf)
+    db.set_routes(route)
+    try:
+        for _ in range(reps):
+            be.set_trace(buf, cap)
+            be.set_profiling(True)
+            db.run(launch=R.LAUNCH_BATCHED, variant=variant)
+            ms, _ = be.last_batch_stats()
+            be.set_profiling(False)
+            k = min(be.trace_count(), cap)
+            tr = np.zeros((k, 4), np.int64)
+            be.download(buf, tr)
+            wave = np.bincount(tr[:, 0], weights=(tr[:, 3] - tr[:, 2]) / 100.0, minlength=n)
+            scale = ms["span_ms"] * 1e3 / max(wave.sum(), 1e-9)
+            best = np.minimum(best, wave * scale)
+    finally:
+        be.set_trace(None, 0)
+        be.free(buf)
+    return best
+
+
+def cpu_path_costs(pages, threads):
+    """Reference CPU raster (the C oracle) time of every path, one page per
+    thread, divided by the threads used: each path's share of the CPU pool."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O  # the checker, timed here as the CPU reference
+
+    out = [None] * len(pages)
+
+    def worker(t):
+        for pi in range(t, len(pages), threads):
+            pg = pages[pi]
+            canvas = np.full((pg.height, pg.width), 255, np.uint8)
+            a = pg.attrs
+            ts = np.zeros(pg.n_paths)
+            for p in range(pg.n_paths):
+                t0 = time.perf_counter()
+                O.fill(canvas, pg.path_edges(p), int(a["eo"][p]), int(a["color"][p]), int(a["a_input"][p]))
+                ts[p] = time.perf_counter() - t0
+            out[pi] = ts
+
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=worker, args=(t,)) for t in range(threads)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    wall = time.perf_counter() - t0
+    return np.concatenate(out) * 1e6 / threads, wall
+
+
+def bench_dispatch(args, desc):
+    """Learned dispatcher: per-path classes from an FP32 MLP trained on the
+    bench's own span-pass timing traces, against the static area rule."""
+    _, seed, _, _, _ = WORKLOADS[args.workload]
+    t_gen = time.perf_counter()
+    train_pages = R.workload(5, seed, DISPATCH_TRAIN_PAGES)
+    eval_pages = R.workload(5, seed + 1, DISPATCH_EVAL_PAGES)
+    img_pages = list(range(0, DISPATCH_EVAL_PAGES, 4))  # a quarter of the pages
+    iset = R.images(0xC0F10004, IMAGES_PER_PAGE * len(img_pages), IMAGES_PER_PAGE)
+    print(f"bench: dispatch: {len(train_pages)} train + {len(eval_pages)} eval pages, {len(iset.images)} images, "
+          f"generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+    be = R.Backend(0)
+    dtr, dev = R.DeviceBatch(be, train_pages), R.DeviceBatch(be, eval_pages)
+    di = R.DeviceImages(be, iset, n_pages=0)
+    # images of eval page k*4 go onto that page's canvas
+    img_descs = (R.ImageDesc * len(iset.images))()
+    for i, im in enumerate(iset.images):
+        img_descs[i] = di.descs[i]
+        img_descs[i].page = img_pages[im.page]
+    ib = R.ImageBatch()
+    ib.images = ctypes.addressof(img_descs)
+    ib.n_images = len(iset.images)
+    ib.pages = ctypes.cast(dev.descs, ctypes.c_void_p)
+    ib.n_pages = len(eval_pages)
+    ib.grid4, ib.grid4_n, ib.grid3, ib.grid3_n = di.grid4, iset.grid4_n, di.grid3, iset.grid3_n
+
+    dev.set_routes(R.ROUTE_SPAN)
+    variant = args.variant if args.variant >= 0 else dev.autotune(reps=1)[0]
+
+    def features(db):
+        db.set_routes(R.ROUTE_SPAN)
+        db.run(launch=R.LAUNCH_BATCHED, variant=variant)
+        return be.last_dispatch(db.n_paths)[0]
+
+    def cost_table(db):
+        cols = [traced_costs(be, db, r, variant) for r in GPU_ROUTES]
+        return np.stack(cols + [np.full(db.n_paths, np.nan)], 1)
+
+    f_tr, f_ev = features(dtr), features(dev)
+    c_tr, c_ev = cost_table(dtr), cost_table(dev)
+    cpu_note = "cpu class not measured (--no-cpu-baseline)"
+    cpu_wall = None
+    threads = min(16, os.cpu_count() or 1)
+    if not args.no_cpu_baseline:
+        c_tr[:, 4], _ = cpu_path_costs(train_pages, threads)
+        c_ev[:, 4], cpu_wall = cpu_path_costs(eval_pages, threads)
+        cpu_note = f"cpu class = reference CPU raster time per path / {threads} threads"
+    v_tr, v_ev = route_valid(f_tr[:, 5]), route_valid(f_ev[:, 5])
+    c_tr[~v_tr] = np.nan
+    c_ev[~v_ev] = np.nan
+    keep = v_tr[:, 0]
+    targets = np.log2(np.maximum(c_tr[keep], 1e-3)).astype(np.float32)
+    t0 = time.perf_counter()
+    params, loss = be.train_dispatcher(f_tr[keep], targets, steps=6000, batch=64, lr=2e-3, seed=seed)
+    t_train = time.perf_counter() - t0
+    print(f"bench: dispatcher trained on {int(keep.sum())} paths in {t_train:.2f} s, loss {loss:.4f}",
+          file=sys.stderr, flush=True)
+
+    def step(routes_auto, mlp):
+        be.set_dispatcher(params if mlp else None)
+        dev.set_routes(R.ROUTE_AUTO if routes_auto else R.ROUTE_SPAN)
+        dev.reset(255)
+        be.begin_page()
+        be.record_fill_pages(dev.fill_pages(launch=R.LAUNCH_BATCHED, variant=variant))
+        be.record_place_images(ib)
+        be.wait_page(be.submit_page())
+
+    results = {}
+    fingerprints = {}
+    for name, auto, mlp in (("span", False, False), ("static", True, False), ("learned", True, True)):
+        step(auto, mlp)
+        _, cls = be.last_dispatch(dev.n_paths)
+        h = hashlib.sha256()
+        for i in range(len(eval_pages)):
+            h.update(dev.read_canvas(i).tobytes())
+        fingerprints[name] = h.hexdigest()
+        for _ in range(args.warmup):
+            step(auto, mlp)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(auto, mlp)
+        sec = (time.perf_counter() - t0) / args.steps
+        cost = np.where(v_ev[:, 0], c_ev[np.arange(len(cls)), cls], 0.0)
+        results[name] = {"ms_per_step": round(sec * 1e3, 3), "pages_per_s": round(len(eval_pages) / sec, 3),
+                         "classes": np.bincount(cls, minlength=5).tolist(), "cost_us": float(np.nansum(cost))}
+    best = np.where(v_ev[:, 0], np.nanmin(np.where(np.isnan(c_ev), np.inf, c_ev), 1), 0.0)
+    best_total = float(best.sum())
+    for name in results:
+        results[name]["regret_us"] = round(results[name]["cost_us"] - best_total, 3)
+        results[name]["cost_us"] = round(results[name]["cost_us"], 3)
+    parity = "match" if len(set(fingerprints.values())) == 1 else "MISMATCH"
+    be.set_dispatcher(params)
+    dev.set_routes(R.ROUTE_AUTO)
+    dev.reset(255)
+    be.begin_page()
+    be.record_fill_pages(dev.fill_pages(launch=R.LAUNCH_BATCHED, counts=True, variant=variant))
+    be.wait_page(be.submit_page())
+    w_ref = int(dev.read_counts()[:, 1].sum())
+    learned_sec = results["learned"]["ms_per_step"] / 1e3
+    be.set_profiling(True)
+    step(True, True)
+    ms, st = be.last_batch_stats()
+    be.set_profiling(False)
+    ch = R.variants()[variant][1]
+    n_edges = dev.n_edges
+    a_bytes = 32 * n_edges + 8 * st["intervals"] + (136 + 8) * st["items"]
+    b_bytes = 2 * 16 * ch * st["tiles"] + 8 * st["intervals"] + (136 + 8 + 16) * st["items"]
+    kernels = {}
+    for name, t_ms, nbytes in (("k_spans", ms["span_ms"], a_bytes), ("k_compose", ms["compose_ms"], b_bytes)):
+        gbs = nbytes / (t_ms / 1e3) / 1e9 if t_ms > 0 else 0.0
+        kernels[name] = {"ms": round(t_ms, 3), "alg_bytes": int(nbytes), "achieved_gbs": round(gbs, 3),
+                         "frac": round(gbs / HBM_PEAK_GBS, 6)}
+    kern = max(("k_spans", "k_compose"), key=lambda k: kernels[k]["ms"])
+    out = {
+        "metric": METRIC, "value": round(w_ref / learned_sec / 1e6, 3), "unit": "Mpx/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": results["learned"]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8+f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "description": desc, "train_pages": len(train_pages),
+                   "eval_pages": len(eval_pages), "image_pages": len(img_pages), "images": len(iset.images),
+                   "eval_paths": dev.n_paths, "variant": {"id": variant, "lds_entries": R.variants()[variant][0],
+                                                          "chunk_px": ch}},
+        "pages_per_s": results["learned"]["pages_per_s"],
+        "dispatch": {"learned": results["learned"], "static": results["static"], "span_only": results["span"],
+                     "best_cost_us": round(best_total, 3),
+                     "cost_definition": "span-pass share of kernel wall time per path (traced wave time x "
+                                        "kernel time / total wave time); " + cpu_note,
+                     "classes_order": ["span", "tile16", "tile32", "tile64", "cpu"],
+                     "train_loss": round(loss, 5), "train_s": round(t_train, 3)},
+        "parity": parity, "fingerprint": fingerprints["learned"][:16],
+        "roofline": {"bound": "hbm", "achieved": kernels[kern]["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": kernels[kern]["frac"], "traffic": None, "kernel": kern,
+                     "avg_launch_us": round(kernels[kern]["ms"] * 1e3, 3),
+                     "alg_bytes_per_launch": kernels[kern]["alg_bytes"]},
+        "kernels": kernels,
+    }
+    if cpu_wall is not None:
+        n_ev = sum(pg.n_paths for pg in eval_pages)
+        out["cpu_baseline"] = {"value": round(len(eval_pages) / cpu_wall, 3), "unit": "pages/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"all {n_ev} paths of the {len(eval_pages)} eval pages filled by the C "
+                                         f"oracle, one page per thread, {cpu_wall:.1f} s (no images)"}
+    be.set_dispatcher(None)
+    dtr.free()
+    dev.free()
+    di.free()
+    be.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,6 +448,8 @@ def main():
     cfg, seed, max_pages, launch, desc = WORKLOADS[args.workload]
     if cfg == 4:
         return bench_images(args, desc)
+    if cfg == 5:
+        return bench_dispatch(args, desc)
     if args.pages:
         max_pages = args.pages
     mode = R.MODE_LITERAL if args.mode == "literal" else R.MODE_DOCUMENTED

@@ -52,11 +52,23 @@ typedef struct {
     uint8_t route;   /* fill route chosen by the dispatcher (RR_ROUTE_*) */
 } rr_path_attr;
 
+/* Fill routes (the dispatcher's classes).  Every route yields the same pixels.
+ * SPAN: the batched span pass takes the path one 16-row band per work item.
+ * TILE16/32/64: one wave takes all of the path's bands (at most 1/2/4) in one
+ *   work item; a path with more bands falls back to SPAN.
+ * CPU: the path is best filled by the caller's CPU raster (the static rule's
+ *   class for bbox area < 256 px, crates/gpu/src/lib.rs:112,120); handed to
+ *   the library anyway, it is filled like SPAN.
+ * AUTO: chosen in-kernel ahead of the fill: by the dispatcher MLP when one is
+ *   loaded (rr_hip_set_dispatcher), else by the static rule (area < 256 px ->
+ *   CPU, else the smallest TILE that holds the path's bands, else SPAN).
+ * Routes are honoured by RR_LAUNCH_BATCHED; RR_LAUNCH_PLAIN is always SPAN. */
 #define RR_ROUTE_SPAN 0
 #define RR_ROUTE_TILE16 1
 #define RR_ROUTE_TILE32 2
 #define RR_ROUTE_TILE64 3
 #define RR_ROUTE_CPU 4
+#define RR_ROUTE_AUTO 255
 
 /* One page of paths to fill and composite, in order.  Replaces the per-path
  * AaFillParams / TileFillParams (crates/gpu/src/backend/params.rs:90-103,
@@ -195,6 +207,8 @@ uint64_t rr_cov_hash_term(int64_t pixel_index, uint8_t shape);
  * config 1: 500 random polygons on one Letter@300 page, half NZ / half EO.
  * config 2: 6000 synthetic glyph outlines on one Letter@300 page.
  * config 3: the mixed 64-page batch (Letter/A3/A4@600/thumbnail).
+ * config 5: a stream of max_pages (32 if 0) glyph pages of the config-2
+ *   distribution, one generator across the stream.
  * A workload holds, per page, the PathBuilder point lists and the edge list. */
 typedef struct rr_workload rr_workload;
 int32_t rr_workload_create(int32_t config, uint64_t seed, int32_t max_pages, rr_workload **out);
@@ -267,6 +281,40 @@ typedef struct rr_hip_graph rr_hip_graph;
 int32_t rr_hip_graph_create(rr_hip_ctx *ctx, const rr_fill_pages *batch, rr_hip_graph **out);
 int32_t rr_hip_graph_launch(rr_hip_ctx *ctx, rr_hip_graph *graph);
 int32_t rr_hip_graph_destroy(rr_hip_ctx *ctx, rr_hip_graph *graph);
+
+/* ---- learned dispatcher (no reference counterpart) ----
+ * MLP: 12 features -> 64 -> 64 -> 5 (ReLU), FP32; the outputs predict log2 of
+ * the span-pass microseconds of each class (SPAN, TILE16, TILE32, TILE64,
+ * CPU); AUTO takes the smallest prediction among the classes valid for the
+ * path.  Parameters: RR_MLP_PARAMS floats = feature mean[12], feature
+ * scale[12], W1[64][12], b1[64], W2[64][64], b2[64], W3[5][64], b3[5].
+ * Features of path p (device pixels; W = column range, H = drawn rows,
+ * E = edges): log2(1+E), log2(1+W), log2(1+H), log2(1+W*H),
+ * log2(1+E/(4H)), bands, even-odd, log2((W+1)/(H+1)), |shoelace area|/(W*H)
+ * clamped to 1, log2(1+AA rows per edge), a_input/255, log2(1+W*H/page area). */
+#define RR_MLP_FEATURES 12
+#define RR_MLP_CLASSES 5
+#define RR_MLP_PARAMS 5341
+/* Load (params != NULL, n == RR_MLP_PARAMS) or clear (params == NULL) the MLP. */
+int32_t rr_hip_set_dispatcher(rr_hip_ctx *ctx, const float *params, int32_t n);
+/* Features and chosen classes of the last batched recording: out_features
+ * (host, n_paths*12 floats, may be NULL), out_routes (host, n_paths bytes,
+ * may be NULL). */
+int32_t rr_hip_last_dispatch(rr_hip_ctx *ctx, float *out_features, uint8_t *out_routes, int64_t n_paths);
+/* MLP forward pass on the device: features (device, n*12) -> out (device,
+ * n*5) with the loaded parameters. */
+int32_t rr_hip_mlp_forward(rr_hip_ctx *ctx, const float *features, float *out, int64_t n);
+/* Trains the dispatcher MLP on the device (one workgroup, Adam, masked
+ * squared error): features (device, n*12), targets (device, n*5 log2
+ * microseconds, NaN where a class is not valid), `steps` minibatches of
+ * `batch` (<= 64) samples drawn with `seed`, learning rate lr.  params_out
+ * (host, RR_MLP_PARAMS floats) receives the trained parameters (feature
+ * standardisation from the data, He-initialised weights; a class with no
+ * valid target keeps output bias 64, i.e. is never chosen); loss_out (host,
+ * may be NULL) the mean masked loss of the last 100 steps. */
+int32_t rr_hip_train_dispatcher(rr_hip_ctx *ctx, const float *features, const float *targets, int64_t n,
+                                int32_t steps, int32_t batch, float lr, uint64_t seed, float *params_out,
+                                float *loss_out);
 
 /* Statistics of the last batched recording (no reference counterpart; read
  * after wait_page).  ms: {span pass, composite pass} kernel milliseconds when

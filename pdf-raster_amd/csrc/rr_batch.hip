@@ -30,6 +30,7 @@
 
 #include "rr_hip.h"
 #include "rr_internal.h"
+#include "rr_mlp.h"
 
 using namespace rrdev;
 
@@ -146,8 +147,10 @@ struct BatchArgs {
     int4 *sel;
     const long long *item_base;
     long long n_paths;
-    const int2 *sitems;
+    const int4 *sitems;  // work items: (path, first row, bands, first span item)
     long long sitems_cap;
+    const long long *work_base;  // n_paths + 1: work items per path, exclusive prefix
+    long long shdr_cap;
     SpanHdr *shdr;
     int2 *spans;
     long long spans_cap;
@@ -169,28 +172,86 @@ struct BatchArgs {
 enum { SPAN_ALL = 0, SPAN_ALL_NODRAW = 1, SPAN_ALL_NOSKIP = 2, SPAN_FIXUP = 3 };
 constexpr long long kSkipped = -2;  // SpanHdr::off of an item skipped as occluded
 
-// Span items of path p: (p, first row of each band it draws), in band order,
-// and the per-band path counts.  One wave per path.
+// The dispatcher, in-kernel ahead of the fill: the class of every path (the
+// caller's route, or for RR_ROUTE_AUTO the MLP's smallest predicted span-pass
+// time among the valid classes, or the static rule without an MLP) and the
+// span-pass work items it takes.  One thread per path.
+__global__ void __launch_bounds__(256) k_dispatch(const rr_path_attr *__restrict__ attrs, const int32_t *__restrict__ nb,
+                                                  const int4 *__restrict__ sel, const float *__restrict__ feat,
+                                                  const float *__restrict__ mlp, long long n_paths,
+                                                  uint8_t *__restrict__ route_cls, int32_t *__restrict__ n_work) {
+    __shared__ float P[kMlpParams];
+    if (mlp) {
+        for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) P[i] = mlp[i];
+        __syncthreads();
+    }
+    const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n_paths) return;
+    const int32_t bands = nb[p];
+    int cls = attrs[p].route;
+    auto valid = [&](int c) {
+        return c == RR_ROUTE_SPAN || c == RR_ROUTE_CPU || (c == RR_ROUTE_TILE16 && bands <= 1) ||
+               (c == RR_ROUTE_TILE32 && bands <= 2) || (c == RR_ROUTE_TILE64 && bands <= 4);
+    };
+    if (cls == RR_ROUTE_AUTO) {
+        if (mlp) {
+            float out[kClasses];
+            mlp_forward(P, feat + p * kFeatures, out);
+            cls = RR_ROUTE_SPAN;
+            for (int c = 1; c < kClasses; ++c)
+                if (valid(c) && out[c] < out[cls]) cls = c;
+        } else {
+            // static rule: bbox area < 256 px -> CPU (crates/gpu/src/lib.rs:112,120),
+            // else the smallest tile that holds the path's bands, else span
+            const int4 s = sel[p];
+            const long long w = s.z <= s.w ? s.w - s.z + 1 : 0, h = s.x <= s.y ? s.y - s.x + 1 : 0;
+            if (w * h < 256)
+                cls = RR_ROUTE_CPU;
+            else if (bands <= 1)
+                cls = RR_ROUTE_TILE16;
+            else if (bands <= 2)
+                cls = RR_ROUTE_TILE32;
+            else if (bands <= 4)
+                cls = RR_ROUTE_TILE64;
+            else
+                cls = RR_ROUTE_SPAN;
+        }
+    }
+    if (cls > RR_ROUTE_CPU || !valid(cls)) cls = RR_ROUTE_SPAN;
+    route_cls[p] = static_cast<uint8_t>(cls);
+    const bool tile = cls == RR_ROUTE_TILE16 || cls == RR_ROUTE_TILE32 || cls == RR_ROUTE_TILE64;
+    n_work[p] = bands == 0 ? 0 : (tile ? 1 : bands);
+}
+
+// Work items of path p (one per band for SPAN, one for all bands for a TILE
+// class) and the per-band path counts.  One wave per path.
 __global__ void __launch_bounds__(256) k_expand(const int4 *__restrict__ sel, const int32_t *__restrict__ nb,
-                                                const PathHdr *__restrict__ hdrs, const DevPage *__restrict__ pages,
-                                                const long long *__restrict__ item_base, long long n_paths,
-                                                int2 *__restrict__ sitems, long long sitems_cap,
+                                                const int32_t *__restrict__ n_work, const PathHdr *__restrict__ hdrs,
+                                                const DevPage *__restrict__ pages,
+                                                const long long *__restrict__ item_base,
+                                                const long long *__restrict__ work_base, long long n_paths,
+                                                int4 *__restrict__ sitems, long long sitems_cap,
                                                 int32_t *__restrict__ band_cnt, int32_t *__restrict__ err) {
     const long long p = static_cast<long long>(blockIdx.x) * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (p >= n_paths) return;
     const int32_t n = nb[p];
     if (n == 0) return;
-    const long long base = item_base[p];
+    const long long base = item_base[p], wbase = work_base[p];
     const int32_t b0 = sel[p].x / kBandRows;
     const long long gb0 = static_cast<long long>(pages[hdrs[p].page].band_base) + b0;
+    const int32_t nw = n_work[p];
     for (int32_t k = lane; k < n; k += 64) {
-        const long long it = base + k;
-        if (it >= sitems_cap) {
-            atomicOr(err, 2);
-            break;
+        if (k < nw) {
+            const long long it = wbase + k;
+            if (it >= sitems_cap) {
+                atomicOr(err, 2);
+            } else if (nw == n) {
+                sitems[it] = make_int4(static_cast<int32_t>(p), kBandRows * (b0 + k), 1, static_cast<int32_t>(base + k));
+            } else {
+                sitems[it] = make_int4(static_cast<int32_t>(p), kBandRows * b0, n, static_cast<int32_t>(base));
+            }
         }
-        sitems[it] = make_int2(static_cast<int32_t>(p), kBandRows * (b0 + k));
         atomicAdd(band_cnt + gb0 + k, 1);
     }
 }
@@ -437,7 +498,7 @@ template <int K>
 __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
     __shared__ SpanLds<K> L;
     const int lane = threadIdx.x;
-    const long long n_items = min(B.item_base[B.n_paths], B.sitems_cap);
+    const long long n_items = min(B.work_base[B.n_paths], B.sitems_cap);
     if (mode == SPAN_FIXUP && B.counters[6] == 0) return;
     const bool count_draw = mode == SPAN_ALL;
     const bool occlusion = (mode == SPAN_ALL || mode == SPAN_ALL_NODRAW) && !(B.F.cov_hash || B.F.cov_count);
@@ -452,11 +513,18 @@ __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
         for (long long k = static_cast<long long>(it0); k < it_end; ++k) {
             // descending path order, so that occluders are met first
             const long long it = n_items - 1 - k;
-            const int2 item = B.sitems[it];
-            if (mode == SPAN_FIXUP && (B.shdr[it].off != kSkipped || (B.pstate[item.x] & 1) == 0)) continue;
+            const int4 item = B.sitems[it];
+            if (item.w < 0 || static_cast<long long>(item.w) + item.z > B.shdr_cap) {
+                if (lane == 0) atomicOr(B.F.err, 2);
+                continue;
+            }
             long long t0 = 0;
             if (B.F.trace) t0 = static_cast<long long>(wall_clock64());
-            span_band<K>(B, L, item.x, item.y, it, blockIdx.x, count_draw, occlusion, cur);
+            for (int32_t bnd = 0; bnd < item.z; ++bnd) {
+                const long long hi = static_cast<long long>(item.w) + bnd;
+                if (mode == SPAN_FIXUP && (B.shdr[hi].off != kSkipped || (B.pstate[item.x] & 1) == 0)) continue;
+                span_band<K>(B, L, item.x, item.y + kBandRows * bnd, hi, blockIdx.x, count_draw, occlusion, cur);
+            }
             if (B.F.trace && lane == 0) {
                 const long long t1 = static_cast<long long>(wall_clock64());
                 const unsigned long long k = atomicAdd(B.F.trace_top, 1ull);
@@ -510,7 +578,7 @@ __device__ __forceinline__ bool list_next(const BatchArgs &B, const DevPage &pg,
         bool ov = false;
         if (c.lb + lane < c.l1) {
             c.ent = B.band_list[c.lb + lane];
-            if (c.ent.x >= pg.path_begin && c.ent.x < pg.path_end && c.ent.y >= 0 && c.ent.y < B.sitems_cap) {
+            if (c.ent.x >= pg.path_begin && c.ent.x < pg.path_end && c.ent.y >= 0 && c.ent.y < B.shdr_cap) {
                 c.s = B.sel[c.ent.x];
                 ov = c.s.z <= c.s.w && c.s.z <= txe && c.s.w >= tx0;
             } else {
@@ -769,7 +837,7 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, int32_t page
             bool full = false;
             if (q < l1) {
                 const int2 ent = B.band_list[q];
-                if (ent.x >= pg.path_begin && ent.x < pg.path_end && ent.y >= 0 && ent.y < B.sitems_cap) {
+                if (ent.x >= pg.path_begin && ent.x < pg.path_end && ent.y >= 0 && ent.y < B.shdr_cap) {
                     const int4 s = B.sel[ent.x];
                     if (s.z <= tx0 && s.w >= txe) {
                         const SpanHdr *H = B.shdr + ent.y;
@@ -996,8 +1064,20 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
         RR_TRY(c, hipMemcpyAsync(c->cpages, c->host_cpages.data(), sizeof(CPage) * b.n_pages,
                                  hipMemcpyHostToDevice, c->stream));
 
-    // span items: per path band counts -> item_base (total at item_base[n_paths])
+    // dispatch (classes + work items per path), then span items per path
+    // (one per band) -> item_base, work items -> work_base
+    rc = ensure(c, &c->route_cls, &c->route_cls_cap, static_cast<size_t>(n_paths));
+    if (!rc) rc = ensure(c, &c->n_work, &c->n_work_cap, static_cast<size_t>(n_paths));
+    if (!rc) rc = ensure(c, &c->work_base, &c->work_base_cap, static_cast<size_t>(n_paths + 1));
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_dispatch, dim3(static_cast<unsigned>((n_paths + 255) / 256)), dim3(256), 0, c->stream,
+                       b.attrs, c->nb, c->sel, c->feat, c->mlp_loaded ? c->mlp : nullptr, n_paths, c->route_cls,
+                       c->n_work);
+    RR_TRY(c, hipGetLastError());
     rc = scan_exclusive<int32_t, long long>(c, c->nb, c->item_base, n_paths, c->item_base + n_paths, c->scan_tmp);
+    if (!rc)
+        rc = scan_exclusive<int32_t, long long>(c, c->n_work, c->work_base, n_paths, c->work_base + n_paths,
+                                                c->scan_tmp);
     if (!rc) rc = fill<int32_t>(c, c->band_cnt, n_bands, 0);
     if (!rc) rc = fill<unsigned long long>(c, c->counters, 8, 0ull);
     if (!rc) rc = fill<unsigned long long>(c, c->spill_top, 1, 0ull);
@@ -1005,14 +1085,15 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     if (!rc) rc = ensure(c, &c->pstate, &c->pstate_cap, static_cast<size_t>(n_paths));
     if (!rc) rc = fill<int32_t>(c, c->pstate, n_paths, 0);
     if (rc) return rc;
-    long long n_items = c->batch_items;  // while capturing: the sizing pass's count
+    long long n_items = c->batch_items, n_work = c->batch_work;  // while capturing: the sizing pass's counts
     if (!c->capturing) {
         rc = read_back(c, c->item_base + n_paths, &n_items);  // also lets the cpage copy land
+        if (!rc) rc = read_back(c, c->work_base + n_paths, &n_work);
         if (rc) return rc;
     }
     if (n_items >= 0x7fffffffll) return fail(c, RR_E_INVARIANT, "batched launch: too many (path, band) items");
     const size_t cap_items = static_cast<size_t>(std::max<long long>(n_items, 1));
-    rc = ensure(c, &c->sitems, &c->sitems_cap, cap_items);
+    rc = ensure(c, &c->sitems, &c->sitems_cap, static_cast<size_t>(std::max<long long>(n_work, 1)));
     if (!rc) rc = ensure(c, &c->shdr, &c->shdr_cap, cap_items);
     if (!rc) rc = ensure(c, &c->band_list, &c->band_list_cap, cap_items);
     if (!rc && c->spans_cap == 0) rc = ensure(c, &c->spans, &c->spans_cap, std::max<size_t>(cap_items * 24, 1u << 20));
@@ -1020,7 +1101,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     if (n_items == 0) return RR_OK;
 
     hipLaunchKernelGGL(k_expand, dim3(static_cast<unsigned>((n_paths + 3) / 4)), dim3(256), 0, c->stream, c->sel,
-                       c->nb, c->hdrs, c->pages, c->item_base, n_paths, c->sitems,
+                       c->nb, c->n_work, c->hdrs, c->pages, c->item_base, c->work_base, n_paths, c->sitems,
                        static_cast<long long>(c->sitems_cap), c->band_cnt, c->err_flag);
     RR_TRY(c, hipGetLastError());
     rc = scan_exclusive<int32_t, long long>(c, c->band_cnt, c->band_base, n_bands, c->band_base + n_bands,
@@ -1032,7 +1113,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     RR_TRY(c, hipGetLastError());
 
     // phase A
-    const unsigned a_waves = static_cast<unsigned>(std::min<long long>(c->span_waves[variant], n_items));
+    const unsigned a_waves = static_cast<unsigned>(std::min<long long>(c->span_waves[variant], std::max(n_work, 1ll)));
     rc = ensure(c, &c->slots, &c->slots_cap, static_cast<size_t>(a_waves) * kSlotInts);
     if (rc) return rc;
     BatchArgs B{};
@@ -1043,6 +1124,8 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     B.n_paths = n_paths;
     B.sitems = c->sitems;
     B.sitems_cap = static_cast<long long>(c->sitems_cap);
+    B.work_base = c->work_base;
+    B.shdr_cap = static_cast<long long>(c->shdr_cap);
     B.shdr = c->shdr;
     B.spans = c->spans;
     B.spans_cap = static_cast<long long>(c->spans_cap);
@@ -1059,6 +1142,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     void *a_args[] = {&B, &mode};
     c->batch_valid = true;
     c->batch_items = n_items;
+    c->batch_work = n_work;
     c->batch_reruns = 0;
     c->prof_a0 = c->prof_used;
     for (int attempt = 0;; ++attempt) {
@@ -1119,7 +1203,7 @@ struct rr_hip_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint64_t ws_gen = 0;
-    int64_t items = 0, citems = 0;
+    int64_t items = 0, citems = 0, work = 0;
     // the host-provided tables the sequence reads, owned by the graph so that
     // later recordings on the context cannot change what a replay sees
     DevPage *pages = nullptr;
@@ -1170,6 +1254,7 @@ extern "C" int32_t rr_hip_graph_create(rr_hip_ctx *c, const rr_fill_pages *b, rr
     rr_hip_graph *g = new rr_hip_graph();
     g->items = c->batch_items;
     g->citems = c->batch_citems;
+    g->work = c->batch_work;
     const size_t np = static_cast<size_t>(t.n_pages);
     if (hipMalloc(&g->pages, np * sizeof(DevPage)) != hipSuccess ||
         hipMalloc(&g->cpages, np * sizeof(CPage)) != hipSuccess ||
@@ -1225,6 +1310,7 @@ extern "C" int32_t rr_hip_graph_launch(rr_hip_ctx *c, rr_hip_graph *g) {
     rrint::count_launch(c);
     c->batch_items = g->items;
     c->batch_citems = g->citems;
+    c->batch_work = g->work;
     c->batch_reruns = 0;
     c->batch_valid = false;  // per-phase events are not inside the graph
     return RR_OK;

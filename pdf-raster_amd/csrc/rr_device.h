@@ -208,6 +208,14 @@ __device__ __forceinline__ int2 drawn_rows(const PathHdr &h, int mode, int32_t h
 // kernel 2: scanline fill + composite of one band of 16 output rows
 // ---------------------------------------------------------------------------
 constexpr int kBandRows = 16;  // output rows per wave
+constexpr int kFeatures = 12;  // dispatcher features per path
+// Dispatcher MLP (12 -> 64 -> 64 -> 5, ReLU), parameters as one float array:
+// feature mean and scale, W1 [64][12], b1, W2 [64][64], b2, W3 [5][64], b3.
+constexpr int kHidden = 64, kClasses = 5;
+constexpr int kMlpMu = 0, kMlpSd = kMlpMu + kFeatures, kMlpW1 = kMlpSd + kFeatures;
+constexpr int kMlpB1 = kMlpW1 + kHidden * kFeatures, kMlpW2 = kMlpB1 + kHidden;
+constexpr int kMlpB2 = kMlpW2 + kHidden * kHidden, kMlpW3 = kMlpB2 + kHidden;
+constexpr int kMlpB3 = kMlpW3 + kClasses * kHidden, kMlpParams = kMlpB3 + kClasses;
 // Tunable shape of a band fill (the autotuner picks among the instantiated
 // variants): K = LDS entry slots per AA row (rows with more state-changing
 // entries go to the wave's global slot), CH = pixels per composite sweep step

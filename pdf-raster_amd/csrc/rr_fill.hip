@@ -63,7 +63,8 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
                                                     const DevPage *__restrict__ pages, int32_t n_pages,
                                                     EdgeRec *__restrict__ recs, PathHdr *__restrict__ hdrs,
                                                     int2 *__restrict__ rows, int4 *__restrict__ sel,
-                                                    int32_t *__restrict__ nb, int32_t mode) {
+                                                    int32_t *__restrict__ nb, const rr_path_attr *__restrict__ attrs,
+                                                    float *__restrict__ feat, int32_t mode) {
     const int64_t p = blockIdx.x;
     int32_t lo = 0, hi = n_pages - 1;  // page of path p (pages sorted by path_begin)
     while (lo < hi) {
@@ -79,9 +80,14 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
     const int32_t cy0 = clip.y_min_i * 4, cy1 = (clip.y_max_i + 1) * 4 - 1;
     int32_t xmn = INT32_MAX, xmx = INT32_MIN, ymn = INT32_MAX, ymx = INT32_MIN;
     int any = 0;
+    double shoelace = 0.0, dy_sum = 0.0;  // dispatcher features
     for (int64_t i = e0 + threadIdx.x; i < e1; i += blockDim.x) {
         const double4 q = reinterpret_cast<const double4 *>(edges)[i];
         double x0 = q.x, y0 = q.y, x1 = q.z, y1 = q.w;
+        if (feat && isfinite(x0) && isfinite(y0) && isfinite(x1) && isfinite(y1)) {
+            shoelace += x0 * y1 - x1 * y0;
+            dy_sum += fabs(y1 - y0);
+        }
         const bool horiz = bits_of(y0) == bits_of(y1);
         const bool vert = bits_of(x0) == bits_of(x1);
         double dxdy = 0.0;
@@ -157,6 +163,11 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
         }
         recs[i] = r;
     }
+    __shared__ double fred[2][256];
+    if (feat) {
+        fred[0][threadIdx.x] = shoelace;
+        fred[1][threadIdx.x] = dy_sum;
+    }
     __shared__ int32_t red[5][256];
     red[0][threadIdx.x] = xmn;
     red[1][threadIdx.x] = xmx;
@@ -171,6 +182,10 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
             red[2][threadIdx.x] = min(red[2][threadIdx.x], red[2][threadIdx.x + s]);
             red[3][threadIdx.x] = max(red[3][threadIdx.x], red[3][threadIdx.x + s]);
             red[4][threadIdx.x] |= red[4][threadIdx.x + s];
+            if (feat) {
+                fred[0][threadIdx.x] += fred[0][threadIdx.x + s];
+                fred[1][threadIdx.x] += fred[1][threadIdx.x + s];
+            }
         }
         __syncthreads();
     }
@@ -206,6 +221,27 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
         }
         sel[p] = s4;
         nb[p] = bands;
+        if (feat) {
+            // the dispatcher's 12 per-path features (sizes in device pixels)
+            const rr_path_attr at = attrs[p];
+            const float E = static_cast<float>(e1 - e0);
+            const float W = s4.z <= s4.w ? static_cast<float>(s4.w - s4.z + 1) : 0.f;
+            const float Hh = drawn.x <= drawn.y ? static_cast<float>(drawn.y - drawn.x + 1) : 0.f;
+            const float area = W * Hh;
+            float *f = feat + p * kFeatures;
+            f[0] = log2f(1.f + E);
+            f[1] = log2f(1.f + W);
+            f[2] = log2f(1.f + Hh);
+            f[3] = log2f(1.f + area);
+            f[4] = log2f(1.f + E / fmaxf(1.f, 4.f * Hh));
+            f[5] = static_cast<float>(bands);
+            f[6] = at.eo ? 1.f : 0.f;
+            f[7] = log2f((W + 1.f) / (Hh + 1.f));
+            f[8] = fminf(1.f, static_cast<float>(fabs(0.5 * fred[0][0])) / fmaxf(1.f, area));
+            f[9] = log2f(1.f + static_cast<float>(4.0 * fred[1][0]) / fmaxf(1.f, E));
+            f[10] = static_cast<float>(at.a_input) / 255.f;
+            f[11] = log2f(1.f + area / fmaxf(1.f, static_cast<float>(pg.width) * static_cast<float>(pg.height)));
+        }
     }
 }
 
@@ -583,6 +619,8 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     if (!rc) rc = rrint::ensure(c, &c->rows, &c->rows_cap, static_cast<size_t>(b.n_paths));
     if (!rc) rc = rrint::ensure(c, &c->sel, &c->sel_cap, static_cast<size_t>(b.n_paths));
     if (!rc) rc = rrint::ensure(c, &c->nb, &c->nb_cap, static_cast<size_t>(b.n_paths));
+    if (!rc && b.launch == RR_LAUNCH_BATCHED)
+        rc = rrint::ensure(c, &c->feat, &c->feat_cap, static_cast<size_t>(b.n_paths) * kFeatures);
     if (!rc) rc = rrint::ensure(c, &c->pages, &c->pages_cap, static_cast<size_t>(b.n_pages));
     const int64_t n_chunks = (b.n_edges + 63) / 64;
     if (!rc) rc = rrint::ensure(c, &c->csum, &c->csum_cap, static_cast<size_t>(n_chunks > 0 ? n_chunks : 1));
@@ -658,7 +696,8 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     rc = rrint::zero_u64(c, c->spill_top, 1);
     if (rc) return rc;
     hipLaunchKernelGGL(k_edge_setup, dim3(static_cast<unsigned>(b.n_paths)), dim3(256), 0, c->stream, b.edges,
-                       b.edge_off, c->pages, b.n_pages, c->recs, c->hdrs, c->rows, c->sel, c->nb, b.mode);
+                       b.edge_off, c->pages, b.n_pages, c->recs, c->hdrs, c->rows, c->sel, c->nb, b.attrs,
+                       b.launch == RR_LAUNCH_BATCHED ? c->feat : nullptr, b.mode);
     RR_HIP_TRY(c, hipGetLastError());
     if (n_chunks > 0) {
         hipLaunchKernelGGL(k_chunk_summary, dim3(static_cast<unsigned>((n_chunks + 3) / 4)), dim3(256), 0, c->stream,
@@ -757,7 +796,8 @@ int32_t rr_hip_destroy(rr_hip_ctx *c) {
     void *bufs[] = {c->recs,      c->hdrs,     c->rows,       c->sel,     c->nb,        c->csum,
                     c->pages,     c->slots,    c->spill,      c->spill_top, c->err_flag, c->trace_top,
                     c->item_base, c->scan_tmp, c->sitems,     c->shdr,    c->spans,     c->band_cnt,
-                    c->band_base, c->band_list, c->cpages,    c->counters, c->occ,      c->pstate};
+                    c->band_base, c->band_list, c->cpages,    c->counters, c->occ,      c->pstate,
+                    c->feat,      c->route_cls, c->n_work,    c->work_base, c->mlp};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->t0) (void)hipEventDestroy(c->t0);

@@ -526,6 +526,19 @@ int32_t rr_workload_create(int32_t config, uint64_t seed, int32_t max_pages, rr_
             w->pages.push_back(std::move(pg));
             break;
         }
+        case 5: {
+            // A stream of glyph pages drawn from the glyph-page distribution
+            // (max_pages of them, 32 when 0), one generator across the stream.
+            const int32_t n = max_pages > 0 ? max_pages : 32;
+            for (int32_t k = 0; k < n; ++k) {
+                Page pg;
+                pg.w = 2550;
+                pg.h = 3300;
+                gen_glyphs(pg, rng, 60, 100, false);
+                w->pages.push_back(std::move(pg));
+            }
+            break;
+        }
         case 3: {
             // 40 Letter@300, 12 A3@300, 8 A4@600, 4 thumbnails@150, in this order.
             const int32_t dims[4][3] = {{40, 2550, 3300}, {12, 3508, 4961}, {8, 4960, 7016}, {4, 1275, 1650}};

@@ -74,7 +74,7 @@ struct rr_hip_ctx {
     size_t item_base_cap = 0;
     long long *scan_tmp = nullptr;
     size_t scan_tmp_cap = 0;
-    int2 *sitems = nullptr;  // span items (path, first row)
+    int4 *sitems = nullptr;  // span-pass work items (path, first row, bands, first span item)
     size_t sitems_cap = 0;
     rrdev::SpanHdr *shdr = nullptr;
     size_t shdr_cap = 0;
@@ -92,12 +92,23 @@ struct rr_hip_ctx {
     size_t occ_cap = 0;
     int32_t *pstate = nullptr;  // per path: widened / skipped-item bookkeeping of the span pass
     size_t pstate_cap = 0;
+    // dispatcher
+    float *feat = nullptr;  // per path: kFeatures floats
+    size_t feat_cap = 0;
+    uint8_t *route_cls = nullptr;  // per path: class chosen (RR_ROUTE_*)
+    size_t route_cls_cap = 0;
+    int32_t *n_work = nullptr;  // per path: span-pass work items
+    size_t n_work_cap = 0;
+    long long *work_base = nullptr;  // exclusive prefix of n_work (n_paths + 1)
+    size_t work_base_cap = 0;
+    float *mlp = nullptr;  // dispatcher MLP parameters (kMlpParams floats) or unused
+    bool mlp_loaded = false;
     // [0] span items served, [1] composite items served, [2] span pool top,
     // [3] intervals stored, [4] (path, tile) composites, [5] tiles touched
     unsigned long long *counters = nullptr;
     // profiling event pairs of the last batched recording: phase A [a0, a1), phase B b
     size_t prof_a0 = 0, prof_a1 = 0, prof_b = 0;
-    int64_t batch_items = 0, batch_citems = 0, batch_reruns = 0;
+    int64_t batch_items = 0, batch_citems = 0, batch_reruns = 0, batch_work = 0;
     bool batch_valid = false;
     // graph capture of a batched recording: while capturing, nothing syncs or
     // uploads (the tables and sizes come from the eager sizing pass before it)

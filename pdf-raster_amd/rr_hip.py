@@ -17,7 +17,8 @@ RR_OK = 0
 MODE_DOCUMENTED, MODE_LITERAL = 0, 1
 FLAVOUR_GRAY8, FLAVOUR_RGB8_FAST = 0, 1
 LAUNCH_PLAIN, LAUNCH_BATCHED = 0, 1
-ROUTE_SPAN, ROUTE_TILE16, ROUTE_TILE32, ROUTE_TILE64, ROUTE_CPU = 0, 1, 2, 3, 4
+ROUTE_SPAN, ROUTE_TILE16, ROUTE_TILE32, ROUTE_TILE64, ROUTE_CPU, ROUTE_AUTO = 0, 1, 2, 3, 4, 255
+MLP_FEATURES, MLP_CLASSES, MLP_PARAMS = 12, 5, 5341
 
 PATH_ATTR_DTYPE = np.dtype([("eo", np.uint8), ("color", np.uint8), ("a_input", np.uint8),
                             ("route", np.uint8)])
@@ -121,6 +122,10 @@ def lib():
             "rr_hip_record_apply_soft_mask": (i32, [P, P, P, i64]),
             "rr_hip_record_scan": (i32, [P, P, P, ctypes.c_uint32]),
             "rr_hip_last_batch_stats": (i32, [P, P, P]),
+            "rr_hip_set_dispatcher": (i32, [P, P, i32]),
+            "rr_hip_last_dispatch": (i32, [P, P, P, i64]),
+            "rr_hip_mlp_forward": (i32, [P, P, P, i64]),
+            "rr_hip_train_dispatcher": (i32, [P, P, P, i64, i32, i32, ctypes.c_float, u64, P, P]),
             "rr_hip_graph_create": (i32, [P, ctypes.POINTER(FillPages), ctypes.POINTER(ctypes.c_void_p)]),
             "rr_hip_graph_launch": (i32, [P, P]),
             "rr_hip_graph_destroy": (i32, [P, P]),
@@ -397,6 +402,49 @@ class Backend:
         keys = ("items", "intervals", "composite_items", "path_tiles", "tiles", "reruns")
         return {"span_ms": float(ms[0]), "compose_ms": float(ms[1])}, dict(zip(keys, (int(v) for v in cnt)))
 
+    def set_dispatcher(self, params):
+        """Load the dispatcher MLP (MLP_PARAMS float32) or clear it (None)."""
+        if params is None:
+            self._check(self.L.rr_hip_set_dispatcher(self.h, None, 0), "set_dispatcher")
+            return
+        params = np.ascontiguousarray(params, np.float32)
+        self._check(self.L.rr_hip_set_dispatcher(self.h, _p(params), params.size), "set_dispatcher")
+
+    def last_dispatch(self, n_paths):
+        """(features (n, 12) float32, classes (n,) uint8) of the last batched recording."""
+        f = np.zeros((n_paths, MLP_FEATURES), np.float32)
+        r = np.zeros(n_paths, np.uint8)
+        self._check(self.L.rr_hip_last_dispatch(self.h, _p(f), _p(r), n_paths), "last_dispatch")
+        return f, r
+
+    def mlp_forward(self, features):
+        features = np.ascontiguousarray(features, np.float32).reshape(-1, MLP_FEATURES)
+        n = len(features)
+        d_f = self.to_device(features)
+        d_o = self.alloc(max(4 * MLP_CLASSES * n, 4))
+        try:
+            self._check(self.L.rr_hip_mlp_forward(self.h, d_f, d_o, n), "mlp_forward")
+            return self.download(d_o, np.zeros((n, MLP_CLASSES), np.float32))
+        finally:
+            self.free(d_f)
+            self.free(d_o)
+
+    def train_dispatcher(self, features, targets, steps=4000, batch=64, lr=2e-3, seed=1):
+        """Train the MLP on the device; targets (n, 5) log2 microseconds, NaN = invalid.
+        Returns (params, mean masked loss of the last 100 steps)."""
+        features = np.ascontiguousarray(features, np.float32).reshape(-1, MLP_FEATURES)
+        targets = np.ascontiguousarray(targets, np.float32).reshape(-1, MLP_CLASSES)
+        d_f, d_t = self.to_device(features), self.to_device(targets)
+        params = np.zeros(MLP_PARAMS, np.float32)
+        loss = ctypes.c_float(0)
+        try:
+            self._check(self.L.rr_hip_train_dispatcher(self.h, d_f, d_t, len(features), steps, batch, lr, seed,
+                                                        _p(params), ctypes.byref(loss)), "train_dispatcher")
+        finally:
+            self.free(d_f)
+            self.free(d_t)
+        return params, loss.value
+
     def record_scan(self, data, block_sums, n):
         self._check(self.L.rr_hip_record_scan(self.h, data, block_sums, n), "record_scan")
 
@@ -570,6 +618,11 @@ class DeviceBatch:
         self.be.begin_page()
         self.be.record_fill_pages(self.fill_pages(mode, flavour, launch, hashes, counts, variant))
         self.be.wait_page(self.be.submit_page())
+
+    def set_routes(self, routes):
+        """Set every path's route (a scalar RR_ROUTE_* or one per path) and upload."""
+        self.attrs_host["route"] = np.asarray(routes, np.uint8)
+        self.be.upload(self.attrs, self.attrs_host)
 
     def capture(self, mode=MODE_DOCUMENTED, flavour=FLAVOUR_GRAY8, hashes=False, counts=False, variant=0):
         """Capture the batched fill of this batch as a HIP graph; returns a handle."""

@@ -90,8 +90,6 @@ def test_batched_without_coverage_outputs(backend, mode):
     # Without per-path coverage outputs the composite pass starts each tile at
     # the last opaque path covering it fully; the pixels must not change.
     # Whole pages, so that large opaque paths overlap many others.
-    # A fresh context, so the first recording also overflows the initial
-    # interval pool and goes through the grow-and-redo path.
     pages = R.workload(3, 0xC0F10003, 3)
     oc, _ = oracle_pages(pages, mode)
     be = R.Backend(0)
@@ -104,8 +102,25 @@ def test_batched_without_coverage_outputs(backend, mode):
                 if not np.array_equal(g, o):
                     ys, xs = np.nonzero(g != o)
                     raise AssertionError(f"variant {variant} page {i}: {len(ys)} px differ, first y={ys[0]} x={xs[0]}")
-            if variant == 0:
-                assert be.last_batch_stats()[1]["reruns"] >= 1  # the grow-and-redo path ran
+    finally:
+        db.free()
+        be.close()
+
+
+def test_interval_pool_grow_and_redo():
+    # A fresh context sizes its interval pool at 24 per (path, band); whole
+    # mixed pages with coverage hashes (no occlusion skipping) need more, so
+    # the first recording grows the pool and redoes the span pass.
+    pages = R.workload(3, 0xC0F10003, 3)
+    oc, oh = oracle_pages(pages)
+    be = R.Backend(0)
+    db = R.DeviceBatch(be, pages)
+    try:
+        db.run(launch=R.LAUNCH_BATCHED, hashes=True, counts=True, variant=2)
+        assert be.last_batch_stats()[1]["reruns"] >= 1
+        for i, o in enumerate(oc):
+            assert np.array_equal(db.read_canvas(i), o), f"page {i}"
+        assert np.array_equal(db.read_hashes(), oh)
     finally:
         db.free()
         be.close()
