@@ -144,6 +144,7 @@ __global__ void __launch_bounds__(256) k_fill(T *__restrict__ p, long long n, T 
 // ---------------------------------------------------------------------------
 struct BatchArgs {
     FillArgs F;
+    const SpanPath *spath;
     int4 *sel;
     const long long *item_base;
     long long n_paths;
@@ -161,14 +162,16 @@ struct BatchArgs {
     unsigned long long *counters;
     int32_t *occ;  // per composite tile: highest opaque path index covering it fully
     int32_t chunk_px;
-    int32_t *pstate;  // per path: bit 0 = widened by an unbalanced row, bits 1.. = skipped items
+    int32_t *pstate;  // per path: bit 0 = widened by an unbalanced row
+    int4 *ovq;        // deferred bands: {path, first row, span item, 0}
+    long long ovq_cap;
 };
 
 // Span pass modes: every item, counting drawn widths and skipping occluded
 // items (skipping only when no per-path coverage output is asked for); the
 // same without counting (a rerun after growing the interval pool); every item
-// without skipping (the sizing pass before a capture); or only the skipped
-// items of paths an unbalanced row widened after they were skipped.
+// without skipping (the sizing pass before a capture); or (wide pass only)
+// the skipped items of paths an unbalanced row widened after they were skipped.
 enum { SPAN_ALL = 0, SPAN_ALL_NODRAW = 1, SPAN_ALL_NOSKIP = 2, SPAN_FIXUP = 3 };
 constexpr long long kSkipped = -2;  // SpanHdr::off of an item skipped as occluded
 
@@ -179,7 +182,9 @@ constexpr long long kSkipped = -2;  // SpanHdr::off of an item skipped as occlud
 __global__ void __launch_bounds__(256) k_dispatch(const rr_path_attr *__restrict__ attrs, const int32_t *__restrict__ nb,
                                                   const int4 *__restrict__ sel, const float *__restrict__ feat,
                                                   const float *__restrict__ mlp, long long n_paths,
-                                                  uint8_t *__restrict__ route_cls, int32_t *__restrict__ n_work) {
+                                                  uint8_t *__restrict__ route_cls, int32_t *__restrict__ n_work,
+                                                  const PathHdr *__restrict__ hdrs, const DevPage *__restrict__ pages,
+                                                  const int64_t *__restrict__ edge_off, SpanPath *__restrict__ spath) {
     __shared__ float P[kMlpParams];
     if (mlp) {
         for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) P[i] = mlp[i];
@@ -188,7 +193,30 @@ __global__ void __launch_bounds__(256) k_dispatch(const rr_path_attr *__restrict
     const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= n_paths) return;
     const int32_t bands = nb[p];
-    int cls = attrs[p].route;
+    const rr_path_attr at = attrs[p];
+    const int4 s = sel[p];
+    {
+        const PathHdr h = hdrs[p];
+        const DevPage &pg = pages[h.page];
+        SpanPath sp;
+        sp.e0 = edge_off[p];
+        sp.e1 = edge_off[p + 1];
+        sp.y_min = h.y_min;
+        sp.y_max = h.y_max;
+        sp.row0 = s.x;
+        sp.row1 = s.y;
+        sp.px0 = s.z;
+        sp.px1 = s.w;
+        sp.width = pg.width;
+        sp.height = pg.height;
+        sp.px_hi = pg.px_hi;
+        sp.occ_base = pg.cbase;
+        sp.ctiles = pg.ctiles;
+        sp.flags = (at.eo ? 1u : 0u) | (static_cast<uint32_t>(at.a_input) << 8) |
+                   (static_cast<uint32_t>(min(max(pg.px_lo, 0), 0xffff)) << 16);
+        spath[p] = sp;
+    }
+    int cls = at.route;
     auto valid = [&](int c) {
         return c == RR_ROUTE_SPAN || c == RR_ROUTE_CPU || (c == RR_ROUTE_TILE16 && bands <= 1) ||
                (c == RR_ROUTE_TILE32 && bands <= 2) || (c == RR_ROUTE_TILE64 && bands <= 4);
@@ -203,7 +231,6 @@ __global__ void __launch_bounds__(256) k_dispatch(const rr_path_attr *__restrict
         } else {
             // static rule: bbox area < 256 px -> CPU (crates/gpu/src/lib.rs:112,120),
             // else the smallest tile that holds the path's bands, else span
-            const int4 s = sel[p];
             const long long w = s.z <= s.w ? s.w - s.z + 1 : 0, h = s.x <= s.y ? s.y - s.x + 1 : 0;
             if (w * h < 256)
                 cls = RR_ROUTE_CPU;
@@ -320,95 +347,31 @@ struct SpanCursor {
     unsigned long long next = 0, end = 0, stored = 0;
 };
 
-template <int K>
-__device__ void span_band(const BatchArgs &B, SpanLds<K> &L, int64_t p, int32_t Y0, long long it, int64_t slot,
-                          bool count_draw, bool occlusion, SpanCursor &cur) {
+// A span item's path as wave-uniform values (unpacked from its SpanPath).
+struct ItemPath {
+    int64_t p, e0, e1;
+    int32_t y_min, y_max, row0, row1, px0, px1, width, height, px_lo, px_hi, occ_base, ctiles;
+    int32_t eo, a_input;
+};
+
+// The intervals of one (path, band) from its entry lists X0/X1/C (LDS or the
+// wave's global slot): sort and walk, drawn widths, fully covered columns,
+// store, header, occlusion publish.
+template <typename Ptr>
+__device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &P, int32_t Y0, SpanHdr *H,
+                                            const int32_t *occ_row, Ptr X0, Ptr X1, Ptr C, int n, bool count_draw,
+                                            bool occlusion, SpanCursor &cur) {
     const FillArgs &A = B.F;
     const int lane = threadIdx.x;
-    const PathHdr h = A.hdrs[p];
-    const int4 s = B.sel[p];
-    const rr_path_attr at = A.attrs[p];
-    const int64_t e0 = A.edge_off[p], e1 = A.edge_off[p + 1];
-    const DevPage &pg = A.pages[h.page];
-    const bool literal = A.mode == RR_MODE_LITERAL;
-    SpanHdr *H = B.shdr + it;
-    const int32_t band = Y0 / kBandRows;
-    const int32_t *occ_row = B.occ + pg.cbase + static_cast<long long>(band) * pg.ctiles;
-    if (occlusion && s.z <= s.w) {
-        // Every composite tile this item can reach already has a later opaque
-        // path covering it fully: the composite pass starts after that path,
-        // so these intervals would never be read.  The per-path state word
-        // keeps a skip from racing with a widening of the path's columns.
-        bool hidden = true;
-        for (int32_t t = s.z / B.chunk_px + lane; hidden && t <= s.w / B.chunk_px; t += 64) hidden = occ_row[t] > p;
-        hidden = __ballot(!hidden) == 0;
-        if (hidden) {
-            int32_t old = 0;
-            if (lane == 0) {
-                old = atomicAdd(B.pstate + p, 2);
-                if (old & 1) atomicSub(B.pstate + p, 2);
-            }
-            hidden = (__builtin_amdgcn_readfirstlane(old) & 1) == 0;
-        }
-        if (hidden) {
-            H->cnt[lane] = 0;
-            if (lane == 0) {
-                H->off = kSkipped;
-                H->fx0 = 1;
-                H->fx1 = 0;
-            }
-            return;
-        }
-    }
-
-    const int32_t d = lane >> 2, j = lane & 3;
-    const int32_t y = Y0 + d;
-    const int32_t r = literal ? 16 * y + 12 + j : 4 * y + j;
-    const int32_t band_lo = literal ? 16 * Y0 + 12 : 4 * Y0;
-    const int32_t band_hi = literal ? 16 * (Y0 + kBandRows - 1) + 15 : 4 * Y0 + 4 * kBandRows - 1;
-    const bool valid = y >= s.x && y <= s.y && r >= h.y_min && r <= h.y_max;
-    const int32_t mrule = at.eo ? 1 : -1;
-    const int32_t aa_width = pg.width * 4;
-
-    int32_t *X0 = L.ex0, *X1 = L.ex1, *C = L.ec;
-    int n = build_entries(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, K);
-    int need = n;
-    for (int o = 32; o > 0; o >>= 1) need = max(need, __shfl_xor(need, o));
-    if (need > K) {
-        // more state-changing entries on a row than the LDS holds: redo the
-        // band with the entry lists in this wave's global slot or the pool
-        int32_t *G;
-        if (static_cast<long long>(need) * 192 <= A.slot_ints) {
-            G = A.slots + slot * A.slot_ints;
-        } else {
-            unsigned long long off = 0;
-            if (lane == 0) off = atomicAdd(A.spill_top, static_cast<unsigned long long>(need) * 192ull);
-            off = __shfl(off, 0);
-            if (static_cast<long long>(off) + static_cast<long long>(need) * 192 > A.spill_cap) {
-                if (lane == 0) {
-                    atomicOr(A.err, 1);
-                    H->off = -1;
-                    H->fx0 = 1;
-                    H->fx1 = 0;
-                }
-                H->cnt[lane] = 0;
-                return;
-            }
-            G = A.spill + off;
-        }
-        X0 = G;
-        X1 = G + static_cast<long long>(need) * 64;
-        C = G + static_cast<long long>(need) * 128;
-        n = build_entries(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, need);
-    }
+    const int32_t y = Y0 + (lane >> 2), j = lane & 3;
+    const int32_t mrule = P.eo ? 1 : -1;
     bool unb = false;
-    const int nint = sort_and_walk(n, X0, X1, C, mrule, aa_width, &unb);
+    const int nint = sort_and_walk(n, X0, X1, C, mrule, P.width * 4, &unb);
     // an unbalanced row spans to the canvas edge: widen the path's columns;
-    // items of this path skipped before the widening are redone by the fixup
+    // its items skipped as occluded are redone by the fixup pass
     if (__ballot(unb) != 0 && lane == 0) {
-        atomicMax(reinterpret_cast<int32_t *>(B.sel + p) + 3, pg.px_hi);
-        const int32_t old = atomicOr(B.pstate + p, 1);
-        if ((old & 1) == 0 && (old >> 1) > 0) atomicAdd(B.counters + 6, 1ull);
+        atomicMax(reinterpret_cast<int32_t *>(B.sel + P.p) + 3, P.px_hi);
+        if ((atomicOr(B.pstate + P.p, 1) & 1) == 0) atomicAdd(B.counters + kCntWidened, 1ull);
     }
     if (A.cov_count && count_draw) {
         // draw_aa_line range of each output row (scanner/mod.rs:459-466)
@@ -418,13 +381,15 @@ __device__ void span_band(const BatchArgs &B, SpanLds<K> &L, int64_t p, int32_t 
         smin = min(smin, dpp_i32<kDppQuadXor2>(smin));
         emax = max(emax, dpp_i32<kDppQuadXor1>(emax));
         emax = max(emax, dpp_i32<kDppQuadXor2>(emax));
-        unsigned long long w_draw = 0;
-        if (j == 0 && y >= s.x && y <= s.y && smin < emax) {
-            const int32_t x0 = max(smin / 4, pg.px_lo), x1 = min((emax - 1) / 4, pg.px_hi);
-            if (x0 <= x1) w_draw = static_cast<unsigned long long>(x1 - x0 + 1);
+        int32_t w_draw = 0;
+        if (j == 0 && y >= P.row0 && y <= P.row1 && smin < emax) {
+            const int32_t x0 = max(smin / 4, P.px_lo), x1 = min((emax - 1) / 4, P.px_hi);
+            if (x0 <= x1) w_draw = x1 - x0 + 1;
         }
-        for (int o = 32; o > 0; o >>= 1) w_draw += __shfl_xor(w_draw, o);
-        if (lane == 0 && w_draw) atomicAdd(reinterpret_cast<unsigned long long *>(A.cov_count + 2 * p + 1), w_draw);
+        const int32_t tot = __builtin_amdgcn_readlane(wave_incl_add(w_draw), 63);
+        if (lane == 0 && tot)
+            atomicAdd(reinterpret_cast<unsigned long long *>(A.cov_count + 2 * P.p + 1),
+                      static_cast<unsigned long long>(tot));
     }
     // columns fully covered on all 64 AA rows: per row the longest interval's
     // whole pixels, intersected over the band (a subset of the exact set)
@@ -440,15 +405,8 @@ __device__ void span_band(const BatchArgs &B, SpanLds<K> &L, int64_t p, int32_t 
             }
         }
     }
-    {
-        int32_t lo = nint > 0 ? fx0 : INT32_MAX, hi = nint > 0 ? fx1 : INT32_MIN;
-        for (int o = 32; o > 0; o >>= 1) {
-            lo = max(lo, __shfl_xor(lo, o));
-            hi = min(hi, __shfl_xor(hi, o));
-        }
-        fx0 = lo;
-        fx1 = hi;
-    }
+    fx0 = wave_max(fx0);
+    fx1 = wave_min(fx1);
     // store the intervals: lane-major, one int2 per interval
     const int32_t incl = wave_incl_add(nint);
     const int32_t excl = incl - nint;
@@ -457,7 +415,7 @@ __device__ void span_band(const BatchArgs &B, SpanLds<K> &L, int64_t p, int32_t 
         const unsigned long long want =
             static_cast<unsigned long long>(total > kSpanChunk ? total : kSpanChunk);
         unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(B.counters + 2, want);
+        if (lane == 0) base = atomicAdd(B.counters + kCntPoolTop, want);
         base = first_lane_u64(base);
         cur.next = base;
         cur.end = base + want;
@@ -478,63 +436,205 @@ __device__ void span_band(const BatchArgs &B, SpanLds<K> &L, int64_t p, int32_t 
     }
     // publish: composite tiles this opaque item covers fully (the same test
     // as the composite pass's occlusion scan)
-    if (occlusion && at.a_input == 255 && Y0 + kBandRows <= pg.height) {
-        const int32_t lo = max(fx0, s.z), hi = min(fx1, s.w);
+    if (occlusion && P.a_input == 255 && Y0 + kBandRows <= P.height) {
+        const int32_t lo = max(fx0, P.px0), hi = min(fx1, P.px1);
         if (lo <= hi) {
             const int32_t ch = B.chunk_px;
             const int32_t ta = (lo + ch - 1) / ch;
-            for (int32_t t = ta + lane; t < pg.ctiles; t += 64) {
-                const int32_t txe = min((t + 1) * ch, pg.width) - 1;
+            for (int32_t t = ta + lane; t < P.ctiles; t += 64) {
+                const int32_t txe = min((t + 1) * ch, P.width) - 1;
                 if (txe > hi) break;
-                atomicMax(const_cast<int32_t *>(occ_row) + t, static_cast<int32_t>(p));
+                atomicMax(const_cast<int32_t *>(occ_row) + t, static_cast<int32_t>(P.p));
             }
         }
     }
 }
 
-constexpr int kSpanGrab = 8;  // span items taken per queue atomic
+// DEFER: a band with more state-changing entries on a row than the K LDS
+// slots hold goes to the deferred queue (the main pass); otherwise it is redone
+// at once with the entry lists in global memory (the wide pass).
+template <int K, bool DEFER>
+__device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, const ItemPath &P, int32_t Y0,
+                                          long long it, int64_t slot, bool count_draw, bool occlusion,
+                                          SpanCursor &cur) {
+    const FillArgs &A = B.F;
+    const int lane = threadIdx.x;
+    SpanHdr *H = B.shdr + it;
+    const int32_t band = Y0 / kBandRows;
+    const int32_t *occ_row = B.occ + P.occ_base + static_cast<long long>(band) * P.ctiles;
+    // the path's chunk summaries are in flight with the occlusion test
+    const int2 sm0 = chunk_summary_first(A, P.e0, P.e1);
+    if (occlusion && P.px0 <= P.px1) {
+        // Every composite tile this item can reach already has a later opaque
+        // path covering it fully: the composite pass starts after that path,
+        // so these intervals would never be read.  Should the path be widened
+        // later by an unbalanced row, the fixup pass redoes the item.
+        bool seen = false;
+        for (int32_t t = P.px0 / B.chunk_px + lane; t <= P.px1 / B.chunk_px; t += 64) seen |= occ_row[t] <= P.p;
+        if (__ballot(seen) == 0) {
+            if (lane == 0) {
+                H->off = kSkipped;
+                H->fx0 = 1;
+                H->fx1 = 0;
+            }
+            return;
+        }
+    }
+    const bool literal = A.mode == RR_MODE_LITERAL;
+    const int32_t y = Y0 + (lane >> 2), j = lane & 3;
+    const int32_t r = literal ? 16 * y + 12 + j : 4 * y + j;
+    const int32_t band_lo = literal ? 16 * Y0 + 12 : 4 * Y0;
+    const int32_t band_hi = literal ? 16 * (Y0 + kBandRows - 1) + 15 : 4 * Y0 + 4 * kBandRows - 1;
+    const bool valid = y >= P.row0 && y <= P.row1 && r >= P.y_min && r <= P.y_max;
+    const int32_t mrule = P.eo ? 1 : -1;
+    int n = build_entries(A, L, P.e0, P.e1, band_lo, band_hi, literal, Y0, valid, r, mrule, L.ex0, L.ex1, L.ec, K,
+                          sm0);
+    if (__ballot(n > K) == 0) {
+        span_finish(B, P, Y0, H, occ_row, L.ex0, L.ex1, L.ec, n, count_draw, occlusion, cur);
+        return;
+    }
+    if constexpr (DEFER) {
+        if (lane == 0) {
+            const unsigned long long q = atomicAdd(B.counters + kCntDeferred, 1ull);
+            if (static_cast<long long>(q) < B.ovq_cap)
+                B.ovq[q] = make_int4(static_cast<int32_t>(P.p), Y0, static_cast<int32_t>(it), 0);
+            else
+                atomicOr(A.err, 32);
+        }
+        return;
+    }
+    // more state-changing entries on a row than the LDS holds: redo the band
+    // with the entry lists in this wave's global slot or the shared pool
+    const int32_t need = wave_max(n);
+    int32_t *G;
+    if (static_cast<long long>(need) * 192 <= A.slot_ints) {
+        G = A.slots + slot * A.slot_ints;
+    } else {
+        unsigned long long off = 0;
+        if (lane == 0) off = atomicAdd(A.spill_top, static_cast<unsigned long long>(need) * 192ull);
+        off = first_lane_u64(off);
+        if (static_cast<long long>(off) + static_cast<long long>(need) * 192 > A.spill_cap) {
+            if (lane == 0) {
+                atomicOr(A.err, 1);
+                H->off = -1;
+                H->fx0 = 1;
+                H->fx1 = 0;
+            }
+            H->cnt[lane] = 0;
+            return;
+        }
+        G = A.spill + off;
+    }
+    int32_t *G1 = G + static_cast<long long>(need) * 64, *G2 = G + static_cast<long long>(need) * 128;
+    n = build_entries(A, L, P.e0, P.e1, band_lo, band_hi, literal, Y0, valid, r, mrule, G, G1, G2, need, sm0);
+    span_finish(B, P, Y0, H, occ_row, G, G1, G2, n, count_draw, occlusion, cur);
+}
 
+constexpr int kSpanGrab = 16;  // span items taken per queue atomic
+constexpr int kWideK = 32;     // LDS entry slots of the wide pass
+
+// The path record of item path p as wave-uniform values (scalar loads).
+__device__ __forceinline__ ItemPath item_path(const BatchArgs &B, int32_t p) {
+    const SpanPath &S = B.spath[p];
+    ItemPath P;
+    P.p = p;
+    P.e0 = S.e0;
+    P.e1 = S.e1;
+    P.y_min = S.y_min;
+    P.y_max = S.y_max;
+    P.row0 = S.row0;
+    P.row1 = S.row1;
+    P.px0 = S.px0;
+    P.px1 = S.px1;
+    P.width = S.width;
+    P.height = S.height;
+    P.px_hi = S.px_hi;
+    P.occ_base = S.occ_base;
+    P.ctiles = S.ctiles;
+    P.eo = S.flags & 1;
+    P.a_input = (S.flags >> 8) & 0xff;
+    P.px_lo = (S.flags >> 16) & 0xffff;
+    return P;
+}
+
+// Main span pass: every work item, descending path order (occluders are met
+// first), entry lists in LDS only; bands whose rows need more than K entries
+// are deferred to the wide pass.
 template <int K>
 __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
     __shared__ SpanLds<K> L;
     const int lane = threadIdx.x;
     const long long n_items = min(B.work_base[B.n_paths], B.sitems_cap);
-    if (mode == SPAN_FIXUP && B.counters[6] == 0) return;
     const bool count_draw = mode == SPAN_ALL;
     const bool occlusion = (mode == SPAN_ALL || mode == SPAN_ALL_NODRAW) && !(B.F.cov_hash || B.F.cov_count);
     SpanCursor cur;
     for (;;) {
-        // items are pulled kSpanGrab at a time: one queue atomic per group
-        unsigned long long it0 = 0;
-        if (lane == 0) it0 = atomicAdd(B.counters, static_cast<unsigned long long>(kSpanGrab));
-        it0 = __shfl(it0, 0);
-        if (static_cast<long long>(it0) >= n_items) break;
-        const long long it_end = min(static_cast<long long>(it0) + kSpanGrab, n_items);
-        for (long long k = static_cast<long long>(it0); k < it_end; ++k) {
-            // descending path order, so that occluders are met first
-            const long long it = n_items - 1 - k;
-            const int4 item = B.sitems[it];
+        unsigned long long g = 0;
+        if (lane == 0) g = atomicAdd(B.counters + kCntSpanItems, static_cast<unsigned long long>(kSpanGrab));
+        g = first_lane_u64(g);  // wave-uniform: the item metadata goes through scalar loads
+        if (static_cast<long long>(g) >= n_items) break;
+        const long long g_end = min(static_cast<long long>(g) + kSpanGrab, n_items);
+        for (long long k = static_cast<long long>(g); k < g_end; ++k) {
+            const int4 item = B.sitems[n_items - 1 - k];
             if (item.w < 0 || static_cast<long long>(item.w) + item.z > B.shdr_cap) {
                 if (lane == 0) atomicOr(B.F.err, 2);
                 continue;
             }
+            const ItemPath P = item_path(B, item.x);
             long long t0 = 0;
             if (B.F.trace) t0 = static_cast<long long>(wall_clock64());
-            for (int32_t bnd = 0; bnd < item.z; ++bnd) {
-                const long long hi = static_cast<long long>(item.w) + bnd;
-                if (mode == SPAN_FIXUP && (B.shdr[hi].off != kSkipped || (B.pstate[item.x] & 1) == 0)) continue;
-                span_band<K>(B, L, item.x, item.y + kBandRows * bnd, hi, blockIdx.x, count_draw, occlusion, cur);
-            }
+            for (int32_t bnd = 0; bnd < item.z; ++bnd)
+                span_band<K, true>(B, L, P, item.y + kBandRows * bnd, static_cast<long long>(item.w) + bnd, 0,
+                                   count_draw, occlusion, cur);
             if (B.F.trace && lane == 0) {
                 const long long t1 = static_cast<long long>(wall_clock64());
-                const unsigned long long k = atomicAdd(B.F.trace_top, 1ull);
-                if (static_cast<long long>(k) < B.F.trace_cap)
-                    B.F.trace[k] = make_longlong4(item.x, (static_cast<long long>(B.F.hdrs[item.x].page) << 32) |
-                                                              static_cast<uint32_t>(item.y), t0, t1);
+                const unsigned long long kk = atomicAdd(B.F.trace_top, 1ull);
+                if (static_cast<long long>(kk) < B.F.trace_cap)
+                    B.F.trace[kk] = make_longlong4(item.x, (static_cast<long long>(B.F.hdrs[item.x].page) << 32) |
+                                                               static_cast<uint32_t>(item.y), t0, t1);
             }
         }
     }
-    if (lane == 0 && cur.stored) atomicAdd(B.counters + 3, cur.stored);
+    if (lane == 0 && cur.stored) atomicAdd(B.counters + kCntStored, cur.stored);
+}
+
+// Wide span pass: the bands the main pass deferred (mode = the main pass's
+// mode), or the items skipped as occluded of paths an unbalanced row widened
+// afterwards (SPAN_FIXUP).  kWideK LDS slots, then the wave's global slot.
+template <int K>
+__global__ void __launch_bounds__(64) k_spans_wide(BatchArgs B, int32_t mode) {
+    __shared__ SpanLds<K> L;
+    const int lane = threadIdx.x;
+    const bool fixup = mode == SPAN_FIXUP;
+    const long long n = fixup ? min(B.work_base[B.n_paths], B.sitems_cap)
+                              : min(static_cast<long long>(B.counters[kCntDeferred]), B.ovq_cap);
+    if (fixup && B.counters[kCntWidened] == 0) return;
+    const bool count_draw = mode == SPAN_ALL;
+    const bool occlusion = (mode == SPAN_ALL || mode == SPAN_ALL_NODRAW) && !(B.F.cov_hash || B.F.cov_count);
+    unsigned long long *const served = B.counters + (fixup ? kCntFixupServed : kCntWideServed);
+    SpanCursor cur;
+    for (;;) {
+        unsigned long long k = 0;
+        if (lane == 0) k = atomicAdd(served, 1ull);
+        k = first_lane_u64(k);
+        if (static_cast<long long>(k) >= n) break;
+        if (fixup) {
+            const int4 item = B.sitems[k];
+            if (item.w < 0 || static_cast<long long>(item.w) + item.z > B.shdr_cap || (B.pstate[item.x] & 1) == 0)
+                continue;
+            const ItemPath P = item_path(B, item.x);
+            for (int32_t bnd = 0; bnd < item.z; ++bnd) {
+                const long long hi = static_cast<long long>(item.w) + bnd;
+                if (B.shdr[hi].off != kSkipped) continue;
+                span_band<K, false>(B, L, P, item.y + kBandRows * bnd, hi, blockIdx.x, false, false, cur);
+            }
+        } else {
+            const int4 q = B.ovq[k];
+            const ItemPath P = item_path(B, q.x);
+            span_band<K, false>(B, L, P, q.y, q.z, blockIdx.x, count_draw, occlusion, cur);
+        }
+    }
+    if (lane == 0 && cur.stored) atomicAdd(B.counters + kCntStored, cur.stored);
 }
 
 // ---------------------------------------------------------------------------
@@ -929,8 +1029,8 @@ __global__ void __launch_bounds__(64) k_compose(BatchArgs B, long long n_items) 
     unsigned long long n_paths = 0, n_tiles = 0;
     for (;;) {
         unsigned long long it = 0;
-        if (lane == 0) it = atomicAdd(B.counters + 1, 1ull);
-        it = __shfl(it, 0);
+        if (lane == 0) it = atomicAdd(B.counters + kCntComposeItems, 1ull);
+        it = first_lane_u64(it);  // wave-uniform: the page search runs on scalar loads
         if (static_cast<long long>(it) >= n_items) break;
         int32_t lo = 0, hi = B.n_cpages - 1;
         while (lo < hi) {
@@ -946,8 +1046,8 @@ __global__ void __launch_bounds__(64) k_compose(BatchArgs B, long long n_items) 
         compose_tile<CH>(B, L, cp.page, band, tile, &n_paths, &n_tiles);
     }
     if (lane == 0) {
-        if (n_paths) atomicAdd(B.counters + 4, n_paths);
-        if (n_tiles) atomicAdd(B.counters + 5, n_tiles);
+        if (n_paths) atomicAdd(B.counters + kCntPathTiles, n_paths);
+        if (n_tiles) atomicAdd(B.counters + kCntTiles, n_tiles);
     }
 }
 
@@ -964,6 +1064,7 @@ constexpr long long kSlotInts = static_cast<long long>(kSlotK) * 192;
 
 const void *span_kernel(int k) {
     switch (k) {
+        case 4: return reinterpret_cast<const void *>(&k_spans<4>);
         case 8: return reinterpret_cast<const void *>(&k_spans<8>);
         case 16: return reinterpret_cast<const void *>(&k_spans<16>);
         default: return reinterpret_cast<const void *>(&k_spans<32>);
@@ -1040,6 +1141,11 @@ int32_t init_batched(rr_hip_ctx *c) {
         c->span_waves[v] = std::max(1, c->n_cu * std::max(1, a));
         c->compose_waves[v] = std::max(1, c->n_cu * std::max(1, b));
     }
+    int w = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&w, reinterpret_cast<const void *>(&k_spans_wide<kWideK>), 64, 0) !=
+        hipSuccess)
+        return fail(c, RR_E_DEVICE, "occupancy query failed");
+    c->wide_waves = std::max(1, c->n_cu * std::max(1, w));
     return RR_OK;
 }
 
@@ -1067,19 +1173,20 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     // dispatch (classes + work items per path), then span items per path
     // (one per band) -> item_base, work items -> work_base
     rc = ensure(c, &c->route_cls, &c->route_cls_cap, static_cast<size_t>(n_paths));
+    if (!rc) rc = ensure(c, &c->spath, &c->spath_cap, static_cast<size_t>(n_paths));
     if (!rc) rc = ensure(c, &c->n_work, &c->n_work_cap, static_cast<size_t>(n_paths));
     if (!rc) rc = ensure(c, &c->work_base, &c->work_base_cap, static_cast<size_t>(n_paths + 1));
     if (rc) return rc;
     hipLaunchKernelGGL(k_dispatch, dim3(static_cast<unsigned>((n_paths + 255) / 256)), dim3(256), 0, c->stream,
                        b.attrs, c->nb, c->sel, c->feat, c->mlp_loaded ? c->mlp : nullptr, n_paths, c->route_cls,
-                       c->n_work);
+                       c->n_work, c->hdrs, c->pages, b.edge_off, c->spath);
     RR_TRY(c, hipGetLastError());
     rc = scan_exclusive<int32_t, long long>(c, c->nb, c->item_base, n_paths, c->item_base + n_paths, c->scan_tmp);
     if (!rc)
         rc = scan_exclusive<int32_t, long long>(c, c->n_work, c->work_base, n_paths, c->work_base + n_paths,
                                                 c->scan_tmp);
     if (!rc) rc = fill<int32_t>(c, c->band_cnt, n_bands, 0);
-    if (!rc) rc = fill<unsigned long long>(c, c->counters, 8, 0ull);
+    if (!rc) rc = fill<unsigned long long>(c, c->counters, kCounters, 0ull);
     if (!rc) rc = fill<unsigned long long>(c, c->spill_top, 1, 0ull);
     if (!rc) rc = fill<int32_t>(c, c->occ, n_citems, -1);
     if (!rc) rc = ensure(c, &c->pstate, &c->pstate_cap, static_cast<size_t>(n_paths));
@@ -1114,11 +1221,14 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
 
     // phase A
     const unsigned a_waves = static_cast<unsigned>(std::min<long long>(c->span_waves[variant], std::max(n_work, 1ll)));
-    rc = ensure(c, &c->slots, &c->slots_cap, static_cast<size_t>(a_waves) * kSlotInts);
+    const unsigned w_waves = static_cast<unsigned>(c->wide_waves);
+    rc = ensure(c, &c->slots, &c->slots_cap, static_cast<size_t>(w_waves) * kSlotInts);
+    if (!rc) rc = ensure(c, &c->ovq, &c->ovq_cap, static_cast<size_t>(n_items / 16 + 65536));
     if (rc) return rc;
     BatchArgs B{};
     B.F = A;
     B.F.slots = c->slots;
+    B.spath = c->spath;
     B.sel = c->sel;
     B.item_base = c->item_base;
     B.n_paths = n_paths;
@@ -1137,6 +1247,8 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     B.occ = c->occ;
     B.chunk_px = chunk_px;
     B.pstate = c->pstate;
+    B.ovq = c->ovq;
+    B.ovq_cap = static_cast<long long>(c->ovq_cap);
     // the sizing pass before a capture counts every interval (no skipping)
     int32_t mode = c->size_only ? SPAN_ALL_NOSKIP : SPAN_ALL;
     void *a_args[] = {&B, &mode};
@@ -1148,11 +1260,13 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     for (int attempt = 0;; ++attempt) {
         const int32_t pass_mode = mode;
         rc = launch_profiled(c, span_kernel(k_entries), a_waves, a_args);
+        // the bands the main pass deferred (exits at once when none)
+        if (!rc) rc = launch_profiled(c, reinterpret_cast<const void *>(&k_spans_wide<kWideK>), w_waves, a_args);
         if (rc) return rc;
         if (pass_mode != SPAN_ALL_NOSKIP) {
             // skipped items of paths widened afterwards (exits at once when none)
             mode = SPAN_FIXUP;
-            rc = launch_profiled(c, span_kernel(k_entries), a_waves, a_args);
+            rc = launch_profiled(c, reinterpret_cast<const void *>(&k_spans_wide<kWideK>), w_waves, a_args);
             if (rc) return rc;
         }
         if (c->capturing) {
@@ -1163,26 +1277,37 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
             void *b_args[] = {&B, &n_citems};
             return launch_profiled(c, compose_kernel(chunk_px), b_waves, b_args);
         }
-        unsigned long long used = 0;
-        rc = read_back(c, c->counters + 2, &used);
-        if (rc) return rc;
+        unsigned long long k[kCounters] = {};
+        RR_TRY(c, hipMemcpyAsync(k, c->counters, sizeof(k), hipMemcpyDeviceToHost, c->stream));
+        RR_TRY(c, hipStreamSynchronize(c->stream));
+        const unsigned long long used = k[kCntPoolTop], deferred = k[kCntDeferred];
         if (c->size_only) {
             c->prof_a1 = c->prof_b = c->prof_used;
             return RR_OK;
         }
-        if (static_cast<long long>(used) <= B.spans_cap) break;
-        if (attempt >= 4) return fail(c, RR_E_CAPACITY, "interval pool still too small after growing it");
-        // the interval pool was too small: grow it and redo the span pass
-        // (the canvas is untouched so far; the drawn-width counts are complete)
-        rc = ensure(c, &c->spans, &c->spans_cap,
-                    std::max(static_cast<size_t>(used + used / 4), 2 * static_cast<size_t>(B.spans_cap)));
-        if (!rc) rc = fill<unsigned long long>(c, c->counters, 8, 0ull);
+        const bool pool_ok = static_cast<long long>(used) <= B.spans_cap;
+        const bool queue_ok = static_cast<long long>(deferred) <= B.ovq_cap;
+        if (pool_ok && queue_ok) break;
+        if (attempt >= 4) return fail(c, RR_E_CAPACITY, "span workspace still too small after growing it");
+        // the interval pool or the deferred queue was too small: grow it and
+        // redo the span pass (the canvas is untouched so far; the drawn-width
+        // counts are complete)
+        if (!pool_ok)
+            rc = ensure(c, &c->spans, &c->spans_cap,
+                        std::max(static_cast<size_t>(used + used / 4), 2 * static_cast<size_t>(B.spans_cap)));
+        if (!rc && !queue_ok)
+            rc = ensure(c, &c->ovq, &c->ovq_cap,
+                        std::max(static_cast<size_t>(deferred + deferred / 4), 2 * static_cast<size_t>(B.ovq_cap)));
+        if (!rc) rc = fill<int32_t>(c, c->err_flag, 1, 0);  // the lost deferrals set bit 32
+        if (!rc) rc = fill<unsigned long long>(c, c->counters, kCounters, 0ull);
         if (!rc) rc = fill<unsigned long long>(c, c->spill_top, 1, 0ull);
         if (!rc) rc = fill<int32_t>(c, c->occ, n_citems, -1);
         if (!rc) rc = fill<int32_t>(c, c->pstate, n_paths, 0);
         if (rc) return rc;
         B.spans = c->spans;
         B.spans_cap = static_cast<long long>(c->spans_cap);
+        B.ovq = c->ovq;
+        B.ovq_cap = static_cast<long long>(c->ovq_cap);
         mode = SPAN_ALL_NODRAW;
         c->batch_reruns = attempt + 1;
     }
@@ -1237,14 +1362,18 @@ extern "C" int32_t rr_hip_graph_create(rr_hip_ctx *c, const rr_fill_pages *b, rr
     int32_t rc = rrint::record_pages_internal(c, t);
     c->size_only = false;
     c->page_open = false;
-    unsigned long long k[8] = {};
+    unsigned long long k[kCounters] = {};
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = rrint::fail(c, RR_E_DEVICE, "sizing pass failed");
     if (!rc && hipMemcpy(k, c->counters, sizeof(k), hipMemcpyDeviceToHost) != hipSuccess)
         rc = rrint::fail(c, RR_E_DEVICE, "sizing pass read-back failed");
     if (!rc) {
-        const size_t need = static_cast<size_t>(k[3]) +
-                            static_cast<size_t>(c->span_waves[t.variant]) * static_cast<size_t>(kSpanChunk) + 1024;
+        // every interval plus each wave's unused chunk tail (main and wide
+        // passes); every band the main pass can defer (nothing is skipped in
+        // the sizing pass, so replays defer no more)
+        const size_t waves = static_cast<size_t>(c->span_waves[t.variant]) + static_cast<size_t>(c->wide_waves);
+        const size_t need = static_cast<size_t>(k[kCntStored]) + waves * static_cast<size_t>(kSpanChunk) + 1024;
         rc = rrint::ensure(c, &c->spans, &c->spans_cap, need);
+        if (!rc) rc = rrint::ensure(c, &c->ovq, &c->ovq_cap, static_cast<size_t>(k[kCntDeferred]) + 1024);
     }
     if (rc) {
         c->profiling = prof;
@@ -1331,7 +1460,7 @@ extern "C" int32_t rr_hip_last_batch_stats(rr_hip_ctx *c, double ms[2], int64_t 
     if (!c->batch_valid) return rrint::fail(c, RR_E_INVARIANT, "no batched recording since the context was created");
     RR_TRY(c, hipSetDevice(c->device));
     RR_TRY(c, hipStreamSynchronize(c->stream));
-    unsigned long long k[8] = {};
+    unsigned long long k[kCounters] = {};
     RR_TRY(c, hipMemcpy(k, c->counters, sizeof(k), hipMemcpyDeviceToHost));
     ms[0] = ms[1] = 0.0;
     if (c->profiling && c->prof_b + 2 <= c->prof_used) {
@@ -1343,10 +1472,10 @@ extern "C" int32_t rr_hip_last_batch_stats(rr_hip_ctx *c, double ms[2], int64_t 
         if (hipEventElapsedTime(&m, c->prof_ev[c->prof_b], c->prof_ev[c->prof_b + 1]) == hipSuccess) ms[1] = m;
     }
     counts[0] = c->batch_items;
-    counts[1] = static_cast<int64_t>(k[3]);
+    counts[1] = static_cast<int64_t>(k[kCntStored]);
     counts[2] = c->batch_citems;
-    counts[3] = static_cast<int64_t>(k[4]);
-    counts[4] = static_cast<int64_t>(k[5]);
+    counts[3] = static_cast<int64_t>(k[kCntPathTiles]);
+    counts[4] = static_cast<int64_t>(k[kCntTiles]);
     counts[5] = c->batch_reruns;
     return RR_OK;
 }

@@ -96,6 +96,12 @@ __device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
         __builtin_amdgcn_update_dpp(0, static_cast<int32_t>(v), CTRL, ROW_MASK, 0xf, true));
 }
 
+// DPP move whose lanes without a source (or outside ROW_MASK) keep `keep`.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ int32_t dpp_keep(int32_t v, int32_t keep) {
+    return __builtin_amdgcn_update_dpp(keep, v, CTRL, ROW_MASK, 0xf, false);
+}
+
 // Value of lane (lane & ~3) + K, the K-th lane of this lane's quad.
 template <int K>
 __device__ __forceinline__ int32_t quad_bcast(int32_t v) {
@@ -118,6 +124,27 @@ __device__ __forceinline__ int32_t wave_incl_add(int32_t v) {
     v += dpp_i32<kDppRowBcast15, 0xa>(v);
     v += dpp_i32<kDppRowBcast31, 0xc>(v);
     return v;
+}
+
+// Maximum / minimum over the wave (a DPP scan to lane 63), wave-uniform.
+__device__ __forceinline__ int32_t wave_max(int32_t v) {
+    v = max(v, dpp_keep<kDppRowShr1>(v, INT32_MIN));
+    v = max(v, dpp_keep<kDppRowShr2>(v, INT32_MIN));
+    v = max(v, dpp_keep<kDppRowShr4>(v, INT32_MIN));
+    v = max(v, dpp_keep<kDppRowShr8>(v, INT32_MIN));
+    v = max(v, dpp_keep<kDppRowBcast15, 0xa>(v, INT32_MIN));
+    v = max(v, dpp_keep<kDppRowBcast31, 0xc>(v, INT32_MIN));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+__device__ __forceinline__ int32_t wave_min(int32_t v) {
+    v = min(v, dpp_keep<kDppRowShr1>(v, INT32_MAX));
+    v = min(v, dpp_keep<kDppRowShr2>(v, INT32_MAX));
+    v = min(v, dpp_keep<kDppRowShr4>(v, INT32_MAX));
+    v = min(v, dpp_keep<kDppRowShr8>(v, INT32_MAX));
+    v = min(v, dpp_keep<kDppRowBcast15, 0xa>(v, INT32_MAX));
+    v = min(v, dpp_keep<kDppRowBcast31, 0xc>(v, INT32_MAX));
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
 // Inclusive prefix OR over the wave.
@@ -316,12 +343,19 @@ __device__ __forceinline__ void lane_range(int32_t a, int32_t b, bool literal, i
 // lane the set of chunk edges crossing its row, and each lane folds only
 // those, in edge (path) order.  Returns the number of entries this lane
 // produced (may exceed cap: then only the first cap are stored and the caller
-// reruns with more room).
-template <typename Lds>
+// reruns with more room).  sm0 is this lane's summary of chunk
+// (e0 >> 6) + lane (chunk_summary_first), loaded by the caller ahead of time;
+// X0/X1/C are LDS or global pointers (the two are separate instantiations).
+__device__ __forceinline__ int2 chunk_summary_first(const FillArgs &A, int64_t e0, int64_t e1) {
+    int2 sm = make_int2(INT32_MAX, INT32_MIN);
+    if (e1 > e0 && (e0 >> 6) + threadIdx.x <= ((e1 - 1) >> 6)) sm = A.csum[(e0 >> 6) + threadIdx.x];
+    return sm;
+}
+
+template <typename Lds, typename Ptr>
 __device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t e0, int64_t e1,
                                              int32_t band_lo, int32_t band_hi, bool literal, int32_t Y0, bool valid,
-                                             int32_t r, int32_t mrule, int32_t *X0, int32_t *X1, int32_t *C,
-                                             int cap) {
+                                             int32_t r, int32_t mrule, Ptr X0, Ptr X1, Ptr C, int cap, int2 sm0) {
     const int lane = threadIdx.x;
     int n = 0;
     bool has_last = false;
@@ -340,7 +374,7 @@ __device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t 
     for (int64_t cb = c_first; cb <= c_last; cb += 64) {
         bool hit = false;
         if (cb + lane <= c_last) {
-            const int2 sm = A.csum[cb + lane];
+            const int2 sm = cb == c_first ? sm0 : A.csum[cb + lane];
             hit = sm.x <= band_hi && sm.y >= band_lo;
         }
         uint64_t cmask = __ballot(hit);
@@ -403,8 +437,9 @@ __device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t 
 // place over X0 (s) / X1 (e).  Returns the interval count; *unbalanced is set
 // when the winding never returned to zero (the reference would loop forever
 // there; the span runs to aa_width and the row stops).
-__device__ __forceinline__ int sort_and_walk(int n, int32_t *X0, int32_t *X1, int32_t *C, int32_t mrule,
-                                             int32_t aa_width, bool *unbalanced) {
+template <typename Ptr>
+__device__ __forceinline__ int sort_and_walk(int n, Ptr X0, Ptr X1, Ptr C, int32_t mrule, int32_t aa_width,
+                                             bool *unbalanced) {
     const int lane = threadIdx.x;
     for (int a = 1; a < n; ++a) {
         const int32_t vx0 = X0[a * 64 + lane], vx1 = X1[a * 64 + lane], vc = C[a * 64 + lane];

@@ -295,7 +295,8 @@ __device__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &p
     const int32_t aa_width = pg.width * 4;
 
     int32_t *X0 = L.ex0, *X1 = L.ex1, *C = L.ec;
-    int n = build_entries(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, K);
+    const int2 sm0 = chunk_summary_first(A, e0, e1);
+    int n = build_entries(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, K, sm0);
     int need = n;
     for (int o = 32; o > 0; o >>= 1) need = max(need, __shfl_xor(need, o));
     if (need > K) {
@@ -318,7 +319,7 @@ __device__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &p
         X0 = G;
         X1 = G + static_cast<long long>(need) * 64;
         C = G + static_cast<long long>(need) * 128;
-        n = build_entries(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, need);
+        n = build_entries(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, need, sm0);
     }
     bool unb = false;
     const int nint = sort_and_walk(n, X0, X1, C, mrule, aa_width, &unb);
@@ -503,9 +504,10 @@ __global__ void __launch_bounds__(64) k_fill_plain(FillArgs A, int32_t page, int
 // the plain composite sweep is at most 256 pixels (one dword per lane per row).
 constexpr int plain_ch(int v) { return kVariantCH[v] > 256 ? 256 : kVariantCH[v]; }
 #define RR_PLAIN(v) reinterpret_cast<const void *>(&k_fill_plain<kVariantK[v], plain_ch(v)>)
-const void *const kPlainKernels[kNumVariants] = {RR_PLAIN(0), RR_PLAIN(1), RR_PLAIN(2),  RR_PLAIN(3),
-                                                 RR_PLAIN(4), RR_PLAIN(5), RR_PLAIN(6),  RR_PLAIN(7),
-                                                 RR_PLAIN(8), RR_PLAIN(9), RR_PLAIN(10), RR_PLAIN(11)};
+const void *const kPlainKernels[kNumVariants] = {RR_PLAIN(0),  RR_PLAIN(1),  RR_PLAIN(2),  RR_PLAIN(3),
+                                                 RR_PLAIN(4),  RR_PLAIN(5),  RR_PLAIN(6),  RR_PLAIN(7),
+                                                 RR_PLAIN(8),  RR_PLAIN(9),  RR_PLAIN(10), RR_PLAIN(11),
+                                                 RR_PLAIN(12), RR_PLAIN(13), RR_PLAIN(14), RR_PLAIN(15)};
 #undef RR_PLAIN
 
 }  // namespace
@@ -772,7 +774,7 @@ int32_t rr_hip_create(int32_t device, rr_hip_ctx **out) {
     if (rc == RR_OK) rc = rrint::init_batched(c);
     if (rc == RR_OK && (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess)) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->spill_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
-    if (rc == RR_OK && hipMalloc(&c->counters, 8 * sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
+    if (rc == RR_OK && hipMalloc(&c->counters, kCounters * sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->err_flag, sizeof(int32_t)) != hipSuccess) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->trace_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
     for (auto &f : c->fences)
@@ -797,7 +799,8 @@ int32_t rr_hip_destroy(rr_hip_ctx *c) {
                     c->pages,     c->slots,    c->spill,      c->spill_top, c->err_flag, c->trace_top,
                     c->item_base, c->scan_tmp, c->sitems,     c->shdr,    c->spans,     c->band_cnt,
                     c->band_base, c->band_list, c->cpages,    c->counters, c->occ,      c->pstate,
-                    c->feat,      c->route_cls, c->n_work,    c->work_base, c->mlp};
+                    c->feat,      c->route_cls, c->n_work,    c->work_base, c->mlp,       c->spath,
+                    c->ovq};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->t0) (void)hipEventDestroy(c->t0);

@@ -12,11 +12,27 @@
 
 namespace rrdev {
 
-constexpr int kNumVariants = 12;  // instantiated fill shapes (the autotuner's search space)
+// Device counters of a batched recording (unsigned long long each).
+constexpr int kCounters = 16;
+enum {
+    kCntSpanItems = 0,     // span-pass work items served
+    kCntComposeItems = 1,  // composite items served
+    kCntPoolTop = 2,       // interval pool bump pointer (records handed out)
+    kCntStored = 3,        // intervals stored
+    kCntPathTiles = 4,     // (path, tile) composites
+    kCntTiles = 5,         // tiles holding at least one path
+    kCntWidened = 6,       // paths widened by an unbalanced row
+    kCntDeferred = 7,      // bands deferred to the wide pass
+    kCntWideServed = 8,    // deferred bands served
+    kCntFixupServed = 9,   // fixup items served
+};
+
+constexpr int kNumVariants = 16;  // instantiated fill shapes (the autotuner's search space)
 // K = LDS entry slots per AA row; CH = pixels per composite sweep (plain) /
 // pixels per composite tile (batched).
-constexpr int kVariantK[kNumVariants] = {8, 8, 8, 8, 16, 16, 16, 16, 32, 32, 32, 32};
-constexpr int kVariantCH[kNumVariants] = {64, 128, 256, 512, 64, 128, 256, 512, 64, 128, 256, 512};
+constexpr int kVariantK[kNumVariants] = {4, 4, 4, 4, 8, 8, 8, 8, 16, 16, 16, 16, 32, 32, 32, 32};
+constexpr int kVariantCH[kNumVariants] = {64, 128, 256, 512, 64, 128, 256, 512,
+                                          64, 128, 256, 512, 64, 128, 256, 512};
 
 struct Fence {
     hipEvent_t ev = nullptr;
@@ -34,6 +50,21 @@ struct SpanHdr {
     uint16_t cnt[64];
 };
 static_assert(sizeof(SpanHdr) == 144, "SpanHdr layout");
+
+// Everything a span item needs about its path and page, one 64-byte record
+// per path (written by k_dispatch): a group of items is then described by two
+// loads, the items and their path records.
+struct SpanPath {
+    long long e0, e1;          // edge records [e0, e1)
+    int32_t y_min, y_max;      // scanner rows (AA space)
+    int32_t row0, row1;        // drawn output rows
+    int32_t px0, px1;          // drawn columns at dispatch (an unbalanced row widens the copy in sel)
+    int32_t width, height;     // canvas
+    int32_t px_hi;             // last column of the page clip
+    int32_t occ_base, ctiles;  // the page's first composite tile, tiles per band
+    uint32_t flags;            // bit 0 even-odd, bits 8-15 a_input, bits 16-31 first clip column (<= 65535)
+};
+static_assert(sizeof(SpanPath) == 64, "SpanPath layout");
 
 // Composite work of one page: tiles per band and the index of its first
 // (band, tile) item; pages are listed in the order their items are served.
@@ -90,8 +121,12 @@ struct rr_hip_ctx {
     size_t cpages_cap = 0;
     int32_t *occ = nullptr;  // per composite tile item: highest opaque path covering it fully, or -1
     size_t occ_cap = 0;
-    int32_t *pstate = nullptr;  // per path: widened / skipped-item bookkeeping of the span pass
+    int32_t *pstate = nullptr;  // per path: bit 0 set once an unbalanced row widened the path
     size_t pstate_cap = 0;
+    rrdev::SpanPath *spath = nullptr;  // per path: the span pass's record of the path
+    size_t spath_cap = 0;
+    int4 *ovq = nullptr;  // bands the main span pass deferred to the wide pass
+    size_t ovq_cap = 0;
     // dispatcher
     float *feat = nullptr;  // per path: kFeatures floats
     size_t feat_cap = 0;
@@ -116,6 +151,7 @@ struct rr_hip_ctx {
     uint64_t ws_gen = 0;  // bumped whenever a workspace array is reallocated
     int span_waves[rrdev::kNumVariants] = {};
     int compose_waves[rrdev::kNumVariants] = {};
+    int wide_waves = 1;  // persistent grid of the wide span pass
     std::vector<rrdev::CPage> host_cpages;
     // page state
     hipEvent_t t0 = nullptr, t1 = nullptr;
