@@ -345,6 +345,7 @@ struct SpanLds {
 constexpr long long kSpanChunk = 8192;
 struct SpanCursor {
     unsigned long long next = 0, end = 0, stored = 0;
+    unsigned long long skipped = 0;  // bands skipped as occluded
 };
 
 // A span item's path as wave-uniform values (unpacked from its SpanPath).
@@ -453,18 +454,26 @@ __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &
 // DEFER: a band with more state-changing entries on a row than the K LDS
 // slots hold goes to the deferred queue (the main pass); otherwise it is redone
 // at once with the entry lists in global memory (the wide pass).
+// Header of an item skipped as occluded (its interval counts are never read).
+__device__ __forceinline__ void write_skipped(SpanHdr *H) {
+    if (threadIdx.x == 0)
+        *reinterpret_cast<int4 *>(H) = make_int4(static_cast<int32_t>(kSkipped & 0xffffffffll),
+                                                 static_cast<int32_t>(kSkipped >> 32), 1, 0);
+}
+
+// One (path, band).  sm0: the path's first chunk summaries
+// (chunk_summary_first).  check: test the band for occlusion first (else the
+// caller has); occlusion: publish the tiles an opaque band covers.
 template <int K, bool DEFER>
 __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, const ItemPath &P, int32_t Y0,
-                                          long long it, int64_t slot, bool count_draw, bool occlusion,
-                                          SpanCursor &cur) {
+                                          long long it, int64_t slot, bool count_draw, bool occlusion, bool check,
+                                          int2 sm0, SpanCursor &cur) {
     const FillArgs &A = B.F;
     const int lane = threadIdx.x;
     SpanHdr *H = B.shdr + it;
     const int32_t band = Y0 / kBandRows;
     const int32_t *occ_row = B.occ + P.occ_base + static_cast<long long>(band) * P.ctiles;
-    // the path's chunk summaries are in flight with the occlusion test
-    const int2 sm0 = chunk_summary_first(A, P.e0, P.e1);
-    if (occlusion && P.px0 <= P.px1) {
+    if (check && P.px0 <= P.px1) {
         // Every composite tile this item can reach already has a later opaque
         // path covering it fully: the composite pass starts after that path,
         // so these intervals would never be read.  Should the path be widened
@@ -472,11 +481,8 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
         bool seen = false;
         for (int32_t t = P.px0 / B.chunk_px + lane; t <= P.px1 / B.chunk_px; t += 64) seen |= occ_row[t] <= P.p;
         if (__ballot(seen) == 0) {
-            if (lane == 0) {
-                H->off = kSkipped;
-                H->fx0 = 1;
-                H->fx1 = 0;
-            }
+            ++cur.skipped;
+            write_skipped(H);
             return;
         }
     }
@@ -557,45 +563,112 @@ __device__ __forceinline__ ItemPath item_path(const BatchArgs &B, int32_t p) {
     return P;
 }
 
+// The same from the record's quarters held by lanes base..base+3.
+__device__ __forceinline__ ItemPath item_path_lanes(const int4 &q, int32_t p, int base) {
+    auto rl = [&](int32_t v, int k) { return __builtin_amdgcn_readlane(v, base + k); };
+    ItemPath P;
+    P.p = p;
+    P.e0 = static_cast<int64_t>((static_cast<uint64_t>(static_cast<uint32_t>(rl(q.y, 0))) << 32) |
+                                static_cast<uint32_t>(rl(q.x, 0)));
+    P.e1 = static_cast<int64_t>((static_cast<uint64_t>(static_cast<uint32_t>(rl(q.w, 0))) << 32) |
+                                static_cast<uint32_t>(rl(q.z, 0)));
+    P.y_min = rl(q.x, 1);
+    P.y_max = rl(q.y, 1);
+    P.row0 = rl(q.z, 1);
+    P.row1 = rl(q.w, 1);
+    P.px0 = rl(q.x, 2);
+    P.px1 = rl(q.y, 2);
+    P.width = rl(q.z, 2);
+    P.height = rl(q.w, 2);
+    P.px_hi = rl(q.x, 3);
+    P.occ_base = rl(q.y, 3);
+    P.ctiles = rl(q.z, 3);
+    const uint32_t fl = static_cast<uint32_t>(rl(q.w, 3));
+    P.eo = fl & 1;
+    P.a_input = (fl >> 8) & 0xff;
+    P.px_lo = (fl >> 16) & 0xffff;
+    return P;
+}
+
 // Main span pass: every work item, descending path order (occluders are met
 // first), entry lists in LDS only; bands whose rows need more than K entries
-// are deferred to the wide pass.
+// are deferred to the wide pass.  Items are taken kSpanGrab at a time; the
+// group's single-band items reaching at most 4 composite tiles are tested for
+// occlusion together (4 lanes per item), so a run of hidden items costs one
+// round trip, and a path's chunk summaries are loaded once for its run of
+// consecutive items.
 template <int K>
 __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
     __shared__ SpanLds<K> L;
+    const FillArgs &A = B.F;
     const int lane = threadIdx.x;
     const long long n_items = min(B.work_base[B.n_paths], B.sitems_cap);
     const bool count_draw = mode == SPAN_ALL;
-    const bool occlusion = (mode == SPAN_ALL || mode == SPAN_ALL_NODRAW) && !(B.F.cov_hash || B.F.cov_count);
+    const bool occlusion = (mode == SPAN_ALL || mode == SPAN_ALL_NODRAW) && !(A.cov_hash || A.cov_count);
     SpanCursor cur;
+    int32_t sm_path = -1;
+    int2 sm0 = make_int2(INT32_MAX, INT32_MIN);
     for (;;) {
         unsigned long long g = 0;
         if (lane == 0) g = atomicAdd(B.counters + kCntSpanItems, static_cast<unsigned long long>(kSpanGrab));
-        g = first_lane_u64(g);  // wave-uniform: the item metadata goes through scalar loads
+        g = first_lane_u64(g);
         if (static_cast<long long>(g) >= n_items) break;
-        const long long g_end = min(static_cast<long long>(g) + kSpanGrab, n_items);
-        for (long long k = static_cast<long long>(g); k < g_end; ++k) {
-            const int4 item = B.sitems[n_items - 1 - k];
-            if (item.w < 0 || static_cast<long long>(item.w) + item.z > B.shdr_cap) {
-                if (lane == 0) atomicOr(B.F.err, 2);
+        const int cnt = static_cast<int>(min(static_cast<long long>(kSpanGrab), n_items - static_cast<long long>(g)));
+        // lanes 4k..4k+3 hold item k of the group and quarter (lane & 3) of
+        // its path record, read back per item with readlane
+        const int qi = lane >> 2, sub = lane & 3;
+        int4 gi = make_int4(0, 0, 0, -1), gq = make_int4(0, 0, 0, 0);
+        if (qi < cnt) {
+            gi = B.sitems[n_items - 1 - static_cast<long long>(g) - qi];
+            gq = reinterpret_cast<const int4 *>(B.spath + gi.x)[sub];
+        }
+        // {px0, px1, width, height} sit in quarter 2, {px_hi, occ_base, ctiles, flags} in quarter 3
+        const int32_t px0 = quad_bcast<2>(gq.x), px1 = quad_bcast<2>(gq.y);
+        const int32_t obase = quad_bcast<3>(gq.y), ctl = quad_bcast<3>(gq.z);
+        bool tested = false, seen = false;
+        if (occlusion && qi < cnt && gi.w >= 0 && gi.z == 1) {
+            const int32_t t0 = px0 / B.chunk_px, t1 = px1 / B.chunk_px;
+            if (px0 <= px1 && t1 - t0 < 4) {
+                tested = true;
+                const int32_t t = t0 + sub;
+                if (t <= t1) seen = B.occ[obase + static_cast<long long>(gi.y / kBandRows) * ctl + t] <= gi.x;
+            }
+        }
+        const uint64_t tmask = __ballot(tested), smask = __ballot(seen);
+        for (int k = 0; k < cnt; ++k) {
+            const int32_t ip = __builtin_amdgcn_readlane(gi.x, 4 * k), iy = __builtin_amdgcn_readlane(gi.y, 4 * k);
+            const int32_t iz = __builtin_amdgcn_readlane(gi.z, 4 * k), iw = __builtin_amdgcn_readlane(gi.w, 4 * k);
+            if (iw < 0 || static_cast<long long>(iw) + iz > B.shdr_cap) {
+                if (lane == 0) atomicOr(A.err, 2);
                 continue;
             }
-            const ItemPath P = item_path(B, item.x);
+            const bool was_tested = (tmask >> (4 * k)) & 1;
+            if (was_tested && ((smask >> (4 * k)) & 0xf) == 0) {
+                ++cur.skipped;
+                write_skipped(B.shdr + iw);
+                continue;
+            }
+            const ItemPath P = item_path_lanes(gq, ip, 4 * k);
+            if (ip != sm_path) {
+                sm0 = chunk_summary_first(A, P.e0, P.e1);
+                sm_path = ip;
+            }
             long long t0 = 0;
-            if (B.F.trace) t0 = static_cast<long long>(wall_clock64());
-            for (int32_t bnd = 0; bnd < item.z; ++bnd)
-                span_band<K, true>(B, L, P, item.y + kBandRows * bnd, static_cast<long long>(item.w) + bnd, 0,
-                                   count_draw, occlusion, cur);
-            if (B.F.trace && lane == 0) {
+            if (A.trace) t0 = static_cast<long long>(wall_clock64());
+            for (int32_t bnd = 0; bnd < iz; ++bnd)
+                span_band<K, true>(B, L, P, iy + kBandRows * bnd, static_cast<long long>(iw) + bnd, 0, count_draw,
+                                   occlusion, occlusion && !was_tested, sm0, cur);
+            if (A.trace && lane == 0) {
                 const long long t1 = static_cast<long long>(wall_clock64());
-                const unsigned long long kk = atomicAdd(B.F.trace_top, 1ull);
-                if (static_cast<long long>(kk) < B.F.trace_cap)
-                    B.F.trace[kk] = make_longlong4(item.x, (static_cast<long long>(B.F.hdrs[item.x].page) << 32) |
-                                                               static_cast<uint32_t>(item.y), t0, t1);
+                const unsigned long long kk = atomicAdd(A.trace_top, 1ull);
+                if (static_cast<long long>(kk) < A.trace_cap)
+                    A.trace[kk] = make_longlong4(ip, (static_cast<long long>(A.hdrs[ip].page) << 32) |
+                                                         static_cast<uint32_t>(iy), t0, t1);
             }
         }
     }
     if (lane == 0 && cur.stored) atomicAdd(B.counters + kCntStored, cur.stored);
+    if (lane == 0 && cur.skipped) atomicAdd(B.counters + kCntSkipped, cur.skipped);
 }
 
 // Wide span pass: the bands the main pass deferred (mode = the main pass's
@@ -626,15 +699,18 @@ __global__ void __launch_bounds__(64) k_spans_wide(BatchArgs B, int32_t mode) {
             for (int32_t bnd = 0; bnd < item.z; ++bnd) {
                 const long long hi = static_cast<long long>(item.w) + bnd;
                 if (B.shdr[hi].off != kSkipped) continue;
-                span_band<K, false>(B, L, P, item.y + kBandRows * bnd, hi, blockIdx.x, false, false, cur);
+                span_band<K, false>(B, L, P, item.y + kBandRows * bnd, hi, blockIdx.x, false, false, false,
+                                    chunk_summary_first(B.F, P.e0, P.e1), cur);
             }
         } else {
             const int4 q = B.ovq[k];
             const ItemPath P = item_path(B, q.x);
-            span_band<K, false>(B, L, P, q.y, q.z, blockIdx.x, count_draw, occlusion, cur);
+            span_band<K, false>(B, L, P, q.y, q.z, blockIdx.x, count_draw, occlusion, occlusion,
+                                chunk_summary_first(B.F, P.e0, P.e1), cur);
         }
     }
     if (lane == 0 && cur.stored) atomicAdd(B.counters + kCntStored, cur.stored);
+    if (lane == 0 && cur.skipped) atomicAdd(B.counters + kCntSkipped, cur.skipped);
 }
 
 // ---------------------------------------------------------------------------
@@ -1455,7 +1531,7 @@ extern "C" int32_t rr_hip_graph_destroy(rr_hip_ctx *c, rr_hip_graph *g) {
     return RR_OK;
 }
 
-extern "C" int32_t rr_hip_last_batch_stats(rr_hip_ctx *c, double ms[2], int64_t counts[6]) {
+extern "C" int32_t rr_hip_last_batch_stats(rr_hip_ctx *c, double ms[2], int64_t counts[8]) {
     if (!c || !ms || !counts) return RR_E_INVARIANT;
     if (!c->batch_valid) return rrint::fail(c, RR_E_INVARIANT, "no batched recording since the context was created");
     RR_TRY(c, hipSetDevice(c->device));
@@ -1477,6 +1553,8 @@ extern "C" int32_t rr_hip_last_batch_stats(rr_hip_ctx *c, double ms[2], int64_t 
     counts[3] = static_cast<int64_t>(k[kCntPathTiles]);
     counts[4] = static_cast<int64_t>(k[kCntTiles]);
     counts[5] = c->batch_reruns;
+    counts[6] = static_cast<int64_t>(k[kCntSkipped]);
+    counts[7] = static_cast<int64_t>(k[kCntDeferred]);
     return RR_OK;
 }
 

@@ -378,19 +378,25 @@ __device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t 
             hit = sm.x <= band_hi && sm.y >= band_lo;
         }
         uint64_t cmask = __ballot(hit);
-        // records of the next overlapping chunk are loaded one chunk ahead
+        // records of the overlapping chunks are loaded two chunks ahead
         auto load_rec = [&](int jc) {
             const int64_t i = ((cb + jc) << 6) + lane;
             EdgeRec q{1, 0, 0, 0, 0, 0, 0, 0};
             if (i >= e0 && i < e1) q = A.recs[i];
             return q;
         };
-        EdgeRec qn{1, 0, 0, 0, 0, 0, 0, 0};
+        EdgeRec qn{1, 0, 0, 0, 0, 0, 0, 0}, qnn{1, 0, 0, 0, 0, 0, 0, 0};
         if (cmask) qn = load_rec(__ffsll(static_cast<unsigned long long>(cmask)) - 1);
+        {
+            const uint64_t rest = cmask & (cmask - 1);
+            if (rest) qnn = load_rec(__ffsll(static_cast<unsigned long long>(rest)) - 1);
+        }
         while (cmask) {
             cmask &= cmask - 1;
             const EdgeRec q = qn;
-            if (cmask) qn = load_rec(__ffsll(static_cast<unsigned long long>(cmask)) - 1);
+            qn = qnn;
+            const uint64_t after = cmask & (cmask - 1);
+            if (after) qnn = load_rec(__ffsll(static_cast<unsigned long long>(after)) - 1);
             int la = 64, lb = -1;
             if (q.r0 <= q.r1) lane_range(q.r0, q.r1, literal, Y0, band_lo, band_hi, &la, &lb);
             L.rec[lane] = q;

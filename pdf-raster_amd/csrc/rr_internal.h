@@ -25,6 +25,7 @@ enum {
     kCntDeferred = 7,      // bands deferred to the wide pass
     kCntWideServed = 8,    // deferred bands served
     kCntFixupServed = 9,   // fixup items served
+    kCntSkipped = 10,      // bands skipped as occluded
 };
 
 constexpr int kNumVariants = 16;  // instantiated fill shapes (the autotuner's search space)
