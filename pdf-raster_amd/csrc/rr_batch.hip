@@ -165,6 +165,8 @@ struct BatchArgs {
     int32_t *pstate;  // per path: bit 0 = widened by an unbalanced row
     int4 *ovq;        // deferred bands: {path, first row, span item, 0}
     long long ovq_cap;
+    const long long *bin_off;  // per composite item: first entry of its tile bin (n + 1)
+    const int2 *bins;          // tile bin entries: {span item, path}
 };
 
 // Span pass modes: every item, counting drawn widths and skipping occluded
@@ -327,6 +329,80 @@ __global__ void __launch_bounds__(64) k_band_lists(const int4 *__restrict__ sel,
     }
 }
 
+// Composite tile bins: for every (band, tile) composite item, the span items
+// of the paths whose stored intervals touch the tile, in path order.  One
+// wave per band sweeps the band's path list (already in path order); each
+// visible item's header gives its column extent.  The counting pass writes
+// the bin sizes, indexed like the composite items (cbase + band * ctiles +
+// tile); the placing pass puts each item at bin_off[bin] + its rank among the
+// list's earlier items in that bin.  Items skipped as occluded and items
+// without intervals are left out.
+constexpr int kMaxTiles = 512;  // a 32767-pixel page at 64-pixel tiles
+template <bool PLACE>
+__global__ void __launch_bounds__(64) k_tile_bins(const DevPage *__restrict__ pages, int32_t n_pages,
+                                                  const long long *__restrict__ band_base,
+                                                  const int2 *__restrict__ band_list, const SpanHdr *__restrict__ shdr,
+                                                  long long shdr_cap, int32_t chunk_px, int32_t *__restrict__ bin_cnt,
+                                                  const long long *__restrict__ bin_off, int2 *__restrict__ bins,
+                                                  long long bins_cap, int32_t *__restrict__ err) {
+    __shared__ long long run[kMaxTiles];
+    const long long gb = blockIdx.x;
+    const int lane = threadIdx.x;
+    int32_t lo = 0, hi = n_pages - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) / 2;
+        if (pages[mid].band_base <= gb)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const DevPage &pg = pages[lo];
+    const int32_t band = static_cast<int32_t>(gb - pg.band_base);
+    const int32_t T = min(pg.ctiles, kMaxTiles);
+    const long long ci0 = static_cast<long long>(pg.cbase) + static_cast<long long>(band) * pg.ctiles;
+    for (int32_t t = lane; t < T; t += 64) run[t] = PLACE ? bin_off[ci0 + t] : 0ll;
+    __syncthreads();
+    const long long l0 = band_base[gb], l1 = band_base[gb + 1];
+    for (long long lb = l0; lb < l1; lb += 64) {
+        const long long q = lb + lane;
+        int2 ent = make_int2(0, 0);
+        int32_t ta = INT32_MAX, tb = -1;
+        if (q < l1) {
+            ent = band_list[q];
+            if (ent.y >= 0 && ent.y < shdr_cap) {
+                const int4 h = *reinterpret_cast<const int4 *>(shdr + ent.y);
+                const long long off = static_cast<long long>((static_cast<uint64_t>(static_cast<uint32_t>(h.y)) << 32) |
+                                                             static_cast<uint32_t>(h.x));
+                const int32_t cx0 = static_cast<int16_t>(h.w & 0xffff), cx1 = static_cast<int16_t>(h.w >> 16);
+                if (off != kSkipped && cx0 <= cx1) {
+                    ta = cx0 / chunk_px;
+                    tb = min(cx1 / chunk_px, T - 1);
+                }
+            } else {
+                atomicOr(err, 16);
+            }
+        }
+        const int32_t u0 = wave_min(ta), u1 = wave_max(tb);
+        for (int32_t t = u0; t <= u1; ++t) {
+            const bool in = ta <= t && t <= tb;
+            const uint64_t m = __ballot(in);
+            if (m == 0) continue;
+            if (PLACE && in) {
+                const long long pos = run[t] + __popcll(m & ((1ull << lane) - 1ull));
+                if (pos < bins_cap)
+                    bins[pos] = make_int2(ent.y, ent.x);
+                else
+                    atomicOr(err, 64);
+            }
+            __syncthreads();
+            if (lane == 0) run[t] += __popcll(m);
+            __syncthreads();
+        }
+    }
+    if (!PLACE)
+        for (int32_t t = lane; t < T; t += 64) bin_cnt[ci0 + t] = static_cast<int32_t>(run[t]);
+}
+
 // ---------------------------------------------------------------------------
 // phase A: spans of one (path, band)
 // ---------------------------------------------------------------------------
@@ -355,6 +431,17 @@ struct ItemPath {
     int32_t eo, a_input;
 };
 
+// First 16 bytes of a span header (offset, full-cover and touched columns),
+// one store by lane 0.
+__device__ __forceinline__ void hdr_head(SpanHdr *H, long long off, int32_t fx0, int32_t fx1, int32_t cx0,
+                                         int32_t cx1) {
+    if (threadIdx.x == 0)
+        *reinterpret_cast<int4 *>(H) =
+            make_int4(static_cast<int32_t>(off & 0xffffffffll), static_cast<int32_t>(off >> 32),
+                      static_cast<int32_t>((static_cast<uint32_t>(fx0) & 0xffffu) | (static_cast<uint32_t>(fx1) << 16)),
+                      static_cast<int32_t>((static_cast<uint32_t>(cx0) & 0xffffu) | (static_cast<uint32_t>(cx1) << 16)));
+}
+
 // The intervals of one (path, band) from its entry lists X0/X1/C (LDS or the
 // wave's global slot): sort and walk, drawn widths, fully covered columns,
 // store, header, occlusion publish.
@@ -370,10 +457,12 @@ __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &
     const int nint = sort_and_walk(n, X0, X1, C, mrule, P.width * 4, &unb);
     // an unbalanced row spans to the canvas edge: widen the path's columns;
     // its items skipped as occluded are redone by the fixup pass
-    if (__ballot(unb) != 0 && lane == 0) {
+    const bool any_unb = __ballot(unb) != 0;
+    if (any_unb && lane == 0) {
         atomicMax(reinterpret_cast<int32_t *>(B.sel + P.p) + 3, P.px_hi);
         if ((atomicOr(B.pstate + P.p, 1) & 1) == 0) atomicAdd(B.counters + kCntWidened, 1ull);
     }
+    const int32_t px1 = any_unb ? P.px_hi : P.px1;  // last drawn column of this item
     if (A.cov_count && count_draw) {
         // draw_aa_line range of each output row (scanner/mod.rs:459-466)
         int32_t smin = nint ? X0[lane] : INT32_MAX;
@@ -406,8 +495,19 @@ __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &
             }
         }
     }
-    fx0 = wave_max(fx0);
-    fx1 = wave_min(fx1);
+    fx0 = max(wave_max(fx0), P.px0);
+    fx1 = min(wave_min(fx1), px1);
+    if (P.a_input != 255 || fx0 > fx1) {
+        fx0 = 1;
+        fx1 = 0;
+    }
+    // columns the intervals touch, within the drawn columns
+    int32_t cx0 = max(wave_min(nint ? X0[lane] >> 2 : INT32_MAX), P.px0);
+    int32_t cx1 = min(wave_max(nint ? (X1[(nint - 1) * 64 + lane] - 1) >> 2 : INT32_MIN), px1);
+    if (cx0 > cx1) {
+        cx0 = 1;
+        cx1 = 0;
+    }
     // store the intervals: lane-major, one int2 per interval
     const int32_t incl = wave_incl_add(nint);
     const int32_t excl = incl - nint;
@@ -430,40 +530,29 @@ __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &
         for (int k = 0; k < nint; ++k) dst[k] = make_int2(X0[k * 64 + lane], X1[k * 64 + lane]);
     }
     H->cnt[lane] = fits ? static_cast<uint16_t>(nint) : static_cast<uint16_t>(0);
-    if (lane == 0) {
-        H->off = fits ? static_cast<long long>(off) : -1ll;
-        H->fx0 = fx0;
-        H->fx1 = fx1;
-    }
+    hdr_head(H, fits ? static_cast<long long>(off) : -1ll, fx0, fx1, cx0, cx1);
     // publish: composite tiles this opaque item covers fully (the same test
     // as the composite pass's occlusion scan)
-    if (occlusion && P.a_input == 255 && Y0 + kBandRows <= P.height) {
-        const int32_t lo = max(fx0, P.px0), hi = min(fx1, P.px1);
-        if (lo <= hi) {
-            const int32_t ch = B.chunk_px;
-            const int32_t ta = (lo + ch - 1) / ch;
-            for (int32_t t = ta + lane; t < P.ctiles; t += 64) {
-                const int32_t txe = min((t + 1) * ch, P.width) - 1;
-                if (txe > hi) break;
-                atomicMax(const_cast<int32_t *>(occ_row) + t, static_cast<int32_t>(P.p));
-            }
+    if (occlusion && Y0 + kBandRows <= P.height && fx0 <= fx1) {
+        const int32_t ch = B.chunk_px;
+        const int32_t ta = (fx0 + ch - 1) / ch;
+        for (int32_t t = ta + lane; t < P.ctiles; t += 64) {
+            const int32_t txe = min((t + 1) * ch, P.width) - 1;
+            if (txe > fx1) break;
+            atomicMax(const_cast<int32_t *>(occ_row) + t, static_cast<int32_t>(P.p));
         }
     }
 }
 
-// DEFER: a band with more state-changing entries on a row than the K LDS
-// slots hold goes to the deferred queue (the main pass); otherwise it is redone
-// at once with the entry lists in global memory (the wide pass).
 // Header of an item skipped as occluded (its interval counts are never read).
-__device__ __forceinline__ void write_skipped(SpanHdr *H) {
-    if (threadIdx.x == 0)
-        *reinterpret_cast<int4 *>(H) = make_int4(static_cast<int32_t>(kSkipped & 0xffffffffll),
-                                                 static_cast<int32_t>(kSkipped >> 32), 1, 0);
-}
+__device__ __forceinline__ void write_skipped(SpanHdr *H) { hdr_head(H, kSkipped, 1, 0, 1, 0); }
 
 // One (path, band).  sm0: the path's first chunk summaries
 // (chunk_summary_first).  check: test the band for occlusion first (else the
-// caller has); occlusion: publish the tiles an opaque band covers.
+// caller has); occlusion: publish the tiles an opaque band covers.  DEFER: a
+// band with more state-changing entries on a row than the K LDS slots hold
+// goes to the deferred queue (the main pass); otherwise it is redone at once
+// with the entry lists in global memory (the wide pass).
 template <int K, bool DEFER>
 __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, const ItemPath &P, int32_t Y0,
                                           long long it, int64_t slot, bool count_draw, bool occlusion, bool check,
@@ -520,12 +609,8 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
         if (lane == 0) off = atomicAdd(A.spill_top, static_cast<unsigned long long>(need) * 192ull);
         off = first_lane_u64(off);
         if (static_cast<long long>(off) + static_cast<long long>(need) * 192 > A.spill_cap) {
-            if (lane == 0) {
-                atomicOr(A.err, 1);
-                H->off = -1;
-                H->fx0 = 1;
-                H->fx1 = 0;
-            }
+            if (lane == 0) atomicOr(A.err, 1);
+            hdr_head(H, -1, 1, 0, 1, 0);
             H->cnt[lane] = 0;
             return;
         }
@@ -727,48 +812,51 @@ struct ComposeLds {
     uint8_t gamma[32];
 };
 
-// One path of a band list as the composite pass consumes it.
+// One path of a tile bin as the composite pass consumes it.
 struct PathRef {
     int64_t p;
-    int32_t hi;  // span item
-    int4 s;      // {first row, last row, first px, last px}
+    int32_t hi;   // span item
+    int4 s;       // {first row, last row, first px, last px}
+    uint32_t at;  // rr_path_attr bytes {eo, color, a_input, route}
 };
 
-// Walks a band list in order, 64 entries per chunk, yielding the paths whose
-// columns meet [tx0, txe].
-struct ListCursor {
+// Walks a tile bin in order, 64 entries per chunk, each chunk's paths' rows,
+// columns and attributes fetched with it.
+struct BinCursor {
     long long lb, l1;
+    int32_t k, n;
     int2 ent;
     int4 s;
-    unsigned long long mask;
+    uint32_t at;
 };
 
-__device__ __forceinline__ bool list_next(const BatchArgs &B, const DevPage &pg, ListCursor &c, int32_t tx0,
-                                          int32_t txe, PathRef &out) {
+__device__ __forceinline__ bool bin_next(const BatchArgs &B, const DevPage &pg, BinCursor &c, PathRef &out) {
     const int lane = threadIdx.x;
-    while (c.mask == 0) {
+    if (c.k >= c.n) {
         c.lb += 64;
         if (c.lb >= c.l1) return false;
+        c.n = static_cast<int32_t>(min(64ll, c.l1 - c.lb));
+        c.k = 0;
         c.ent = make_int2(0, 0);
         c.s = make_int4(1, 0, 1, 0);
-        bool ov = false;
-        if (c.lb + lane < c.l1) {
-            c.ent = B.band_list[c.lb + lane];
-            if (c.ent.x >= pg.path_begin && c.ent.x < pg.path_end && c.ent.y >= 0 && c.ent.y < B.shdr_cap) {
-                c.s = B.sel[c.ent.x];
-                ov = c.s.z <= c.s.w && c.s.z <= txe && c.s.w >= tx0;
+        c.at = 0;
+        if (lane < c.n) {
+            c.ent = B.bins[c.lb + lane];
+            if (c.ent.y >= pg.path_begin && c.ent.y < pg.path_end && c.ent.x >= 0 && c.ent.x < B.shdr_cap) {
+                c.s = B.sel[c.ent.y];
+                c.at = reinterpret_cast<const uint32_t *>(B.F.attrs)[c.ent.y];
             } else {
-                atomicOr(B.F.err, 16);  // a band list entry that was never written
+                c.ent = make_int2(0, 0);
+                atomicOr(B.F.err, 16);  // a bin entry that was never written
             }
         }
-        c.mask = __ballot(ov);
     }
-    const int jj = __ffsll(c.mask) - 1;
-    c.mask &= c.mask - 1;
-    out.p = __builtin_amdgcn_readlane(c.ent.x, jj);
-    out.hi = __builtin_amdgcn_readlane(c.ent.y, jj);
-    out.s = make_int4(__builtin_amdgcn_readlane(c.s.x, jj), __builtin_amdgcn_readlane(c.s.y, jj),
-                      __builtin_amdgcn_readlane(c.s.z, jj), __builtin_amdgcn_readlane(c.s.w, jj));
+    const int k = c.k++;
+    out.hi = __builtin_amdgcn_readlane(c.ent.x, k);
+    out.p = __builtin_amdgcn_readlane(c.ent.y, k);
+    out.s = make_int4(__builtin_amdgcn_readlane(c.s.x, k), __builtin_amdgcn_readlane(c.s.y, k),
+                      __builtin_amdgcn_readlane(c.s.z, k), __builtin_amdgcn_readlane(c.s.w, k));
+    out.at = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int32_t>(c.at), k));
     return true;
 }
 
@@ -817,13 +905,12 @@ __device__ __forceinline__ uint32_t comp1(uint32_t dv, uint32_t shape, uint32_t 
 // nibble-popcount coverage of fill/mod.rs:344-401 exactly.
 template <int CH>
 __device__ __forceinline__ void compose_rows(const BatchArgs &B, ComposeLds<CH> &L, const DevPage &pg, int64_t p,
-                                             const SpanView &V, int32_t Y0, int32_t tx0, int32_t xa, int32_t xb,
-                                             int32_t ya, int32_t yb) {
+                                             uint32_t at, const SpanView &V, int32_t Y0, int32_t tx0, int32_t xa,
+                                             int32_t xb, int32_t ya, int32_t yb) {
     const FillArgs &A = B.F;
     const int lane = threadIdx.x, d = lane >> 2, j = lane & 3, g0 = lane & ~3;
     constexpr int kPitch = ComposeLds<CH>::kPitch, kRowPitch = ComposeLds<CH>::kRowPitch;
-    const rr_path_attr at = A.attrs[p];
-    const uint32_t a_in = at.a_input, src = at.color, src4 = src * 0x01010101u;
+    const uint32_t a_in = (at >> 16) & 0xffu, src = (at >> 8) & 0xffu, src4 = src * 0x01010101u;
     const int flavour = A.flavour;
     const bool want_cov = A.cov_hash || A.cov_count;
     uint64_t hsum = 0;
@@ -981,60 +1068,61 @@ __device__ __forceinline__ void compose_rows(const BatchArgs &B, ComposeLds<CH> 
     }
 }
 
-// All paths of band `band` that meet tile `tile` of page `page`, in path
-// order, composited into the tile held in LDS.  Two-deep software pipeline:
-// while path j composites, the first span of path j+1 and the header of path
-// j+2 are in flight.
+// The paths of tile bin `ci` (band `band`, tile `tile` of page `page`), in
+// path order, composited into the tile held in LDS.  Two-deep software
+// pipeline: while path j composites, the first span of path j+1 and the
+// header of path j+2 are in flight.
 template <int CH>
-__device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, int32_t page, int32_t band, int32_t tile,
-                             unsigned long long *n_paths, unsigned long long *n_tiles) {
+__device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, long long ci, int32_t page, int32_t band,
+                             int32_t tile, unsigned long long *n_paths, unsigned long long *n_tiles) {
     const FillArgs &A = B.F;
     const int lane = threadIdx.x;
     constexpr int kRowWords = ComposeLds<CH>::kRowWords, kRowPitch = ComposeLds<CH>::kRowPitch;
+    const long long b0 = B.bin_off[ci], b1 = B.bin_off[ci + 1];
+    if (b1 <= b0) return;
     const DevPage &pg = A.pages[page];
     const int32_t Y0 = kBandRows * band, tx0 = CH * tile;
     const int32_t width = pg.width;
     const int32_t txe = min(tx0 + CH, width) - 1;
     if (txe < pg.px_lo || tx0 > pg.px_hi) return;
-    const long long gb = static_cast<long long>(pg.band_base) + band;
-    const long long l0 = B.band_base[gb], l1 = B.band_base[gb + 1];
     const int32_t nrows = min(kBandRows, pg.height - Y0);
     const int32_t ndw = ((txe - tx0) >> 2) + 1;
     // Occlusion: the last opaque path that covers the whole tile on all 64 AA
     // rows leaves exactly its colour there (both composites give src for
     // coverage 16 at a_input 255), so compositing starts after it on a tile
-    // filled with that colour.  Skipped when per-path coverage is counted.
-    long long start = l0;
+    // filled with that colour.  The bin is searched from its end.  Skipped
+    // when per-path coverage is counted.
+    long long start = b0;
     int32_t fill_color = -1;
     if (!(A.cov_hash || A.cov_count) && nrows == kBandRows) {
         long long last = -1;
-        for (long long lb = l0; lb < l1; lb += 64) {
+        for (long long lb = b1 - 64; lb + 64 > b0; lb -= 64) {
             const long long q = lb + lane;
             bool full = false;
-            if (q < l1) {
-                const int2 ent = B.band_list[q];
-                if (ent.x >= pg.path_begin && ent.x < pg.path_end && ent.y >= 0 && ent.y < B.shdr_cap) {
-                    const int4 s = B.sel[ent.x];
-                    if (s.z <= tx0 && s.w >= txe) {
-                        const SpanHdr *H = B.shdr + ent.y;
-                        full = H->fx0 <= tx0 && H->fx1 >= txe && A.attrs[ent.x].a_input == 255;
-                    }
+            if (q >= b0 && q < b1) {
+                const int2 e = B.bins[q];
+                if (e.x >= 0 && e.x < B.shdr_cap) {
+                    const int32_t fx = *reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(B.shdr + e.x) + 8);
+                    full = static_cast<int16_t>(fx & 0xffff) <= tx0 && static_cast<int16_t>(fx >> 16) >= txe;
                 }
             }
             const unsigned long long m = __ballot(full);
-            if (m) last = lb + 63 - __clzll(m);
+            if (m) {
+                last = lb + 63 - __clzll(m);
+                break;
+            }
         }
         if (last >= 0) {
             start = last + 1;
-            fill_color = A.attrs[B.band_list[last].x].color;
+            fill_color = A.attrs[B.bins[last].y].color;
         }
     }
-    ListCursor cur;
+    BinCursor cur;
     cur.lb = start - 64;
-    cur.l1 = l1;
-    cur.mask = 0;
+    cur.l1 = b1;
+    cur.k = cur.n = 0;
     PathRef P0, P1;
-    const bool has0 = list_next(B, pg, cur, tx0, txe, P0);
+    const bool has0 = bin_next(B, pg, cur, P0);
     if (!has0 && fill_color < 0) return;
     ++*n_tiles;
     uint8_t *const base = pg.canvas + static_cast<long long>(Y0) * pg.stride + tx0;
@@ -1044,7 +1132,7 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, int32_t page
     bool has1 = false;
     if (has0) {
         hdr_fetch(B, P0.hi, &off0, &n0);
-        has1 = list_next(B, pg, cur, tx0, txe, P1);
+        has1 = bin_next(B, pg, cur, P1);
         if (has1) hdr_fetch(B, P1.hi, &off1, &n1);
     }
     const uint32_t fill4 = static_cast<uint32_t>(fill_color) * 0x01010101u;
@@ -1074,10 +1162,10 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, int32_t page
         if (has1) {
             V1 = span_fetch(B, off1, n1);
             if (off1 < 0 && lane == 0) atomicOr(A.err, 8);
-            has2 = list_next(B, pg, cur, tx0, txe, P2);
+            has2 = bin_next(B, pg, cur, P2);
             if (has2) hdr_fetch(B, P2.hi, &off2, &n2);
         }
-        if (live) compose_rows<CH>(B, L, pg, P0.p, V0, Y0, tx0, xa, xb, ya, yb);
+        if (live) compose_rows<CH>(B, L, pg, P0.p, P0.at, V0, Y0, tx0, xa, xb, ya, yb);
         ++*n_paths;
         __syncthreads();
         if (!has1) break;
@@ -1119,7 +1207,7 @@ __global__ void __launch_bounds__(64) k_compose(BatchArgs B, long long n_items) 
         const CPage cp = B.cpages[lo];
         const int32_t local = static_cast<int32_t>(static_cast<long long>(it) - cp.base);
         const int32_t band = local / cp.tiles, tile = local - band * cp.tiles;
-        compose_tile<CH>(B, L, cp.page, band, tile, &n_paths, &n_tiles);
+        compose_tile<CH>(B, L, static_cast<long long>(it), cp.page, band, tile, &n_paths, &n_tiles);
     }
     if (lane == 0) {
         if (n_paths) atomicAdd(B.counters + kCntPathTiles, n_paths);
@@ -1232,7 +1320,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     long long n_bands = 0;
     for (int32_t i = 0; i < b.n_pages; ++i) n_bands += (b.pages[i].height + kBandRows - 1) / kBandRows;
     int32_t rc = ensure(c, &c->item_base, &c->item_base_cap, static_cast<size_t>(n_paths + 1));
-    const long long scan_n = std::max<long long>(n_paths, n_bands);
+    const long long scan_n = std::max<long long>(std::max<long long>(n_paths, n_bands), c->batch_citems);
     if (!rc) rc = ensure(c, &c->scan_tmp, &c->scan_tmp_cap, static_cast<size_t>((scan_n + kScanTile - 1) / kScanTile + 1));
     if (!rc) rc = ensure(c, &c->band_cnt, &c->band_cnt_cap, static_cast<size_t>(std::max<long long>(n_bands, 1)));
     if (!rc) rc = ensure(c, &c->band_base, &c->band_base_cap, static_cast<size_t>(n_bands + 1));
@@ -1241,6 +1329,8 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
 
     long long n_citems = c->batch_citems;  // set with the page table by the caller
     rc = ensure(c, &c->occ, &c->occ_cap, static_cast<size_t>(std::max<long long>(n_citems, 1)));
+    if (!rc) rc = ensure(c, &c->bin_cnt, &c->bin_cnt_cap, static_cast<size_t>(std::max<long long>(n_citems, 1)));
+    if (!rc) rc = ensure(c, &c->bin_off, &c->bin_off_cap, static_cast<size_t>(n_citems + 1));
     if (rc) return rc;
     if (!c->capturing)
         RR_TRY(c, hipMemcpyAsync(c->cpages, c->host_cpages.data(), sizeof(CPage) * b.n_pages,
@@ -1325,6 +1415,26 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     B.pstate = c->pstate;
     B.ovq = c->ovq;
     B.ovq_cap = static_cast<long long>(c->ovq_cap);
+    B.bin_off = c->bin_off;
+    B.bins = c->bins;
+    // tile bins of the composite pass: sizes, then offsets (the sizes follow
+    // the span pass: skipped items and items without intervals are left out)
+    auto count_bins = [&]() -> int32_t {
+        hipLaunchKernelGGL(k_tile_bins<false>, dim3(static_cast<unsigned>(n_bands)), dim3(64), 0, c->stream, c->pages,
+                           b.n_pages, c->band_base, c->band_list, c->shdr, static_cast<long long>(c->shdr_cap), chunk_px,
+                           c->bin_cnt, c->bin_off, c->bins, static_cast<long long>(c->bins_cap), c->err_flag);
+        RR_TRY(c, hipGetLastError());
+        return scan_exclusive<int32_t, long long>(c, c->bin_cnt, c->bin_off, n_citems, c->bin_off + n_citems,
+                                                  c->scan_tmp);
+    };
+    auto place_bins = [&]() -> int32_t {
+        B.bins = c->bins;
+        hipLaunchKernelGGL(k_tile_bins<true>, dim3(static_cast<unsigned>(n_bands)), dim3(64), 0, c->stream, c->pages,
+                           b.n_pages, c->band_base, c->band_list, c->shdr, static_cast<long long>(c->shdr_cap), chunk_px,
+                           c->bin_cnt, c->bin_off, c->bins, static_cast<long long>(c->bins_cap), c->err_flag);
+        RR_TRY(c, hipGetLastError());
+        return RR_OK;
+    };
     // the sizing pass before a capture counts every interval (no skipping)
     int32_t mode = c->size_only ? SPAN_ALL_NOSKIP : SPAN_ALL;
     void *a_args[] = {&B, &mode};
@@ -1345,20 +1455,28 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
             rc = launch_profiled(c, reinterpret_cast<const void *>(&k_spans_wide<kWideK>), w_waves, a_args);
             if (rc) return rc;
         }
+        c->prof_a1 = c->prof_used;
+        rc = count_bins();
+        if (rc) return rc;
         if (c->capturing) {
             // sized by the eager pass: the pool holds every interval plus each
-            // wave's unused chunk tail
-            c->prof_a1 = c->prof_b = c->prof_used;
+            // wave's unused chunk tail, the bins every visible item
+            rc = place_bins();
+            if (rc) return rc;
+            c->prof_b = c->prof_used;
             const unsigned b_waves = static_cast<unsigned>(std::min<long long>(c->compose_waves[variant], n_citems));
             void *b_args[] = {&B, &n_citems};
             return launch_profiled(c, compose_kernel(chunk_px), b_waves, b_args);
         }
         unsigned long long k[kCounters] = {};
+        long long n_bin = 0;
         RR_TRY(c, hipMemcpyAsync(k, c->counters, sizeof(k), hipMemcpyDeviceToHost, c->stream));
+        RR_TRY(c, hipMemcpyAsync(&n_bin, c->bin_off + n_citems, sizeof(n_bin), hipMemcpyDeviceToHost, c->stream));
         RR_TRY(c, hipStreamSynchronize(c->stream));
         const unsigned long long used = k[kCntPoolTop], deferred = k[kCntDeferred];
+        c->batch_bins = n_bin;
         if (c->size_only) {
-            c->prof_a1 = c->prof_b = c->prof_used;
+            c->prof_b = c->prof_used;
             return RR_OK;
         }
         const bool pool_ok = static_cast<long long>(used) <= B.spans_cap;
@@ -1387,7 +1505,9 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
         mode = SPAN_ALL_NODRAW;
         c->batch_reruns = attempt + 1;
     }
-    c->prof_a1 = c->prof_used;
+    rc = ensure(c, &c->bins, &c->bins_cap, static_cast<size_t>(std::max<long long>(c->batch_bins, 1)));
+    if (!rc) rc = place_bins();
+    if (rc) return rc;
     c->prof_b = c->prof_used;
 
     // phase B
@@ -1450,6 +1570,7 @@ extern "C" int32_t rr_hip_graph_create(rr_hip_ctx *c, const rr_fill_pages *b, rr
         const size_t need = static_cast<size_t>(k[kCntStored]) + waves * static_cast<size_t>(kSpanChunk) + 1024;
         rc = rrint::ensure(c, &c->spans, &c->spans_cap, need);
         if (!rc) rc = rrint::ensure(c, &c->ovq, &c->ovq_cap, static_cast<size_t>(k[kCntDeferred]) + 1024);
+        if (!rc) rc = rrint::ensure(c, &c->bins, &c->bins_cap, static_cast<size_t>(c->batch_bins) + 1024);
     }
     if (rc) {
         c->profiling = prof;

@@ -591,6 +591,7 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     if (!b.edges || !b.edge_off || !b.attrs || !b.pages || (b.launch == RR_LAUNCH_PLAIN && !b.host_rows))
         return fail(c, RR_E_INVARIANT, "null buffer");
     if (reinterpret_cast<uintptr_t>(b.edges) & 31u) return fail(c, RR_E_INVARIANT, "edges must be 32-byte aligned");
+    if (reinterpret_cast<uintptr_t>(b.attrs) & 3u) return fail(c, RR_E_INVARIANT, "attrs must be 4-byte aligned");
     int64_t prev_end = 0;
     c->host_pages.resize(static_cast<size_t>(b.n_pages));
     for (int32_t i = 0; i < b.n_pages; ++i) {
@@ -800,7 +801,7 @@ int32_t rr_hip_destroy(rr_hip_ctx *c) {
                     c->item_base, c->scan_tmp, c->sitems,     c->shdr,    c->spans,     c->band_cnt,
                     c->band_base, c->band_list, c->cpages,    c->counters, c->occ,      c->pstate,
                     c->feat,      c->route_cls, c->n_work,    c->work_base, c->mlp,       c->spath,
-                    c->ovq};
+                    c->ovq,       c->bin_cnt,  c->bin_off,    c->bins};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->t0) (void)hipEventDestroy(c->t0);

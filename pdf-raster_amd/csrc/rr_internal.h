@@ -42,12 +42,16 @@ struct Fence {
 
 // Spans of one (path, 16-row band) item written by the batched span pass:
 // lane l's disjoint sorted intervals [s, e) (AA x units) are the cnt[l]
-// int2 records starting at off + sum(cnt[0..l)).  off = -1: not stored.
-// fx0..fx1: pixel columns that every one of the band's 64 AA rows covers
-// fully (empty when fx0 > fx1): an opaque path paints them with its colour.
+// int2 records starting at off + sum(cnt[0..l)).  off = -1: not stored;
+// off = -2: skipped as occluded (the counts are not written).  fx0..fx1:
+// pixel columns, within the path's drawn columns, that every one of the
+// band's 64 AA rows covers fully; empty (fx0 > fx1) unless the path is opaque,
+// so that the path paints them with exactly its colour.  cx0..cx1: the pixel
+// columns the intervals touch within the drawn columns (empty when none), the
+// item's extent for the composite tile bins.
 struct SpanHdr {
     long long off;
-    int32_t fx0, fx1;
+    int16_t fx0, fx1, cx0, cx1;
     uint16_t cnt[64];
 };
 static_assert(sizeof(SpanHdr) == 144, "SpanHdr layout");
@@ -128,6 +132,13 @@ struct rr_hip_ctx {
     size_t spath_cap = 0;
     int4 *ovq = nullptr;  // bands the main span pass deferred to the wide pass
     size_t ovq_cap = 0;
+    int32_t *bin_cnt = nullptr;  // per composite tile: entries of its bin
+    size_t bin_cnt_cap = 0;
+    long long *bin_off = nullptr;  // per composite tile: first entry of its bin (n + 1)
+    size_t bin_off_cap = 0;
+    int2 *bins = nullptr;  // composite tile bins: {span item, path}
+    size_t bins_cap = 0;
+    long long batch_bins = 0;  // bin entries of the last batched recording (the sizing pass's while capturing)
     // dispatcher
     float *feat = nullptr;  // per path: kFeatures floats
     size_t feat_cap = 0;
