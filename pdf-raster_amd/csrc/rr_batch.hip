@@ -167,6 +167,7 @@ struct BatchArgs {
     long long ovq_cap;
     const long long *bin_off;  // per composite item: first entry of its tile bin (n + 1)
     const int2 *bins;          // tile bin entries: {span item, path}
+    unsigned long long *queues;  // sliced queue counters: span pass, then composite pass
 };
 
 // Span pass modes: every item, counting drawn widths and skipping occluded
@@ -335,17 +336,21 @@ __global__ void __launch_bounds__(64) k_band_lists(const int4 *__restrict__ sel,
 // visible item's header gives its column extent.  The counting pass writes
 // the bin sizes, indexed like the composite items (cbase + band * ctiles +
 // tile); the placing pass puts each item at bin_off[bin] + its rank among the
-// list's earlier items in that bin.  Items skipped as occluded and items
-// without intervals are left out.
+// list's earlier items in that bin.  Items skipped as occluded, items
+// without intervals and (item, tile) pairs that a later opaque path covers
+// fully (occ[tile] above the path: the composite starts after that path) are
+// left out.
 constexpr int kMaxTiles = 512;  // a 32767-pixel page at 64-pixel tiles
 template <bool PLACE>
 __global__ void __launch_bounds__(64) k_tile_bins(const DevPage *__restrict__ pages, int32_t n_pages,
                                                   const long long *__restrict__ band_base,
                                                   const int2 *__restrict__ band_list, const SpanHdr *__restrict__ shdr,
-                                                  long long shdr_cap, int32_t chunk_px, int32_t *__restrict__ bin_cnt,
-                                                  const long long *__restrict__ bin_off, int2 *__restrict__ bins,
-                                                  long long bins_cap, int32_t *__restrict__ err) {
+                                                  long long shdr_cap, int32_t chunk_px, const int32_t *__restrict__ occ,
+                                                  int32_t *__restrict__ bin_cnt, const long long *__restrict__ bin_off,
+                                                  int2 *__restrict__ bins, long long bins_cap,
+                                                  int32_t *__restrict__ err) {
     __shared__ long long run[kMaxTiles];
+    __shared__ int32_t cover[kMaxTiles];
     const long long gb = blockIdx.x;
     const int lane = threadIdx.x;
     int32_t lo = 0, hi = n_pages - 1;
@@ -360,7 +365,10 @@ __global__ void __launch_bounds__(64) k_tile_bins(const DevPage *__restrict__ pa
     const int32_t band = static_cast<int32_t>(gb - pg.band_base);
     const int32_t T = min(pg.ctiles, kMaxTiles);
     const long long ci0 = static_cast<long long>(pg.cbase) + static_cast<long long>(band) * pg.ctiles;
-    for (int32_t t = lane; t < T; t += 64) run[t] = PLACE ? bin_off[ci0 + t] : 0ll;
+    for (int32_t t = lane; t < T; t += 64) {
+        run[t] = PLACE ? bin_off[ci0 + t] : 0ll;
+        cover[t] = occ[ci0 + t];
+    }
     __syncthreads();
     const long long l0 = band_base[gb], l1 = band_base[gb + 1];
     for (long long lb = l0; lb < l1; lb += 64) {
@@ -384,7 +392,7 @@ __global__ void __launch_bounds__(64) k_tile_bins(const DevPage *__restrict__ pa
         }
         const int32_t u0 = wave_min(ta), u1 = wave_max(tb);
         for (int32_t t = u0; t <= u1; ++t) {
-            const bool in = ta <= t && t <= tb;
+            const bool in = ta <= t && t <= tb && cover[t] <= ent.x;
             const uint64_t m = __ballot(in);
             if (m == 0) continue;
             if (PLACE && in) {
@@ -448,7 +456,7 @@ __device__ __forceinline__ void hdr_head(SpanHdr *H, long long off, int32_t fx0,
 template <typename Ptr>
 __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &P, int32_t Y0, SpanHdr *H,
                                             const int32_t *occ_row, Ptr X0, Ptr X1, Ptr C, int n, bool count_draw,
-                                            bool occlusion, SpanCursor &cur) {
+                                            bool occlusion, uint64_t vmask, SpanCursor &cur) {
     const FillArgs &A = B.F;
     const int lane = threadIdx.x;
     const int32_t y = Y0 + (lane >> 2), j = lane & 3;
@@ -501,16 +509,33 @@ __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &
         fx0 = 1;
         fx1 = 0;
     }
-    // columns the intervals touch, within the drawn columns
-    int32_t cx0 = max(wave_min(nint ? X0[lane] >> 2 : INT32_MAX), P.px0);
-    int32_t cx1 = min(wave_max(nint ? (X1[(nint - 1) * 64 + lane] - 1) >> 2 : INT32_MIN), px1);
+    // Intervals kept: those that reach, within the drawn columns, a composite
+    // tile not yet covered by a later opaque path (bit i of vmask: tile
+    // px0 / chunk_px + i; tiles past the mask count as visible).  The others
+    // would never be composited.  An unbalanced item keeps everything.
+    const uint64_t vm = any_unb ? ~0ull : vmask;
+    const int32_t ch = B.chunk_px, t0 = P.px0 / ch;
+    int keep = 0;
+    int32_t cx0 = INT32_MAX, cx1 = INT32_MIN;
+    for (int k = 0; k < nint; ++k) {
+        const int32_t pa = max(X0[k * 64 + lane] >> 2, P.px0), pb = min((X1[k * 64 + lane] - 1) >> 2, px1);
+        if (pa > pb) continue;
+        const int32_t ta = pa / ch - t0, tb = pb / ch - t0;
+        const uint64_t bits = tb >= 63 ? ~0ull << ta : ((2ull << tb) - 1ull) & (~0ull << ta);
+        if (tb < 64 && (vm & bits) == 0) continue;
+        ++keep;
+        cx0 = min(cx0, pa);
+        cx1 = max(cx1, pb);
+    }
+    cx0 = wave_min(cx0);
+    cx1 = wave_max(cx1);
     if (cx0 > cx1) {
         cx0 = 1;
         cx1 = 0;
     }
-    // store the intervals: lane-major, one int2 per interval
-    const int32_t incl = wave_incl_add(nint);
-    const int32_t excl = incl - nint;
+    // store the kept intervals: lane-major, one int2 per interval
+    const int32_t incl = wave_incl_add(keep);
+    const int32_t excl = incl - keep;
     const int32_t total = __builtin_amdgcn_readlane(incl, 63);
     if (cur.next + static_cast<unsigned long long>(total) > cur.end) {
         const unsigned long long want =
@@ -527,14 +552,24 @@ __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &
     cur.stored += fits ? total : 0;
     if (fits) {
         int2 *dst = B.spans + off + excl;
-        for (int k = 0; k < nint; ++k) dst[k] = make_int2(X0[k * 64 + lane], X1[k * 64 + lane]);
+        int w = 0;
+        for (int k = 0; k < nint && w < keep; ++k) {
+            const int32_t a = X0[k * 64 + lane], b = X1[k * 64 + lane];
+            if (keep != nint) {
+                const int32_t pa = max(a >> 2, P.px0), pb = min((b - 1) >> 2, px1);
+                if (pa > pb) continue;
+                const int32_t ta = pa / ch - t0, tb = pb / ch - t0;
+                const uint64_t bits = tb >= 63 ? ~0ull << ta : ((2ull << tb) - 1ull) & (~0ull << ta);
+                if (tb < 64 && (vm & bits) == 0) continue;
+            }
+            dst[w++] = make_int2(a, b);
+        }
     }
-    H->cnt[lane] = fits ? static_cast<uint16_t>(nint) : static_cast<uint16_t>(0);
+    H->cnt[lane] = fits ? static_cast<uint16_t>(keep) : static_cast<uint16_t>(0);
     hdr_head(H, fits ? static_cast<long long>(off) : -1ll, fx0, fx1, cx0, cx1);
     // publish: composite tiles this opaque item covers fully (the same test
     // as the composite pass's occlusion scan)
     if (occlusion && Y0 + kBandRows <= P.height && fx0 <= fx1) {
-        const int32_t ch = B.chunk_px;
         const int32_t ta = (fx0 + ch - 1) / ch;
         for (int32_t t = ta + lane; t < P.ctiles; t += 64) {
             const int32_t txe = min((t + 1) * ch, P.width) - 1;
@@ -556,7 +591,7 @@ __device__ __forceinline__ void write_skipped(SpanHdr *H) { hdr_head(H, kSkipped
 template <int K, bool DEFER>
 __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, const ItemPath &P, int32_t Y0,
                                           long long it, int64_t slot, bool count_draw, bool occlusion, bool check,
-                                          int2 sm0, SpanCursor &cur) {
+                                          uint64_t vmask, int2 sm0, SpanCursor &cur) {
     const FillArgs &A = B.F;
     const int lane = threadIdx.x;
     SpanHdr *H = B.shdr + it;
@@ -567,9 +602,11 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
         // path covering it fully: the composite pass starts after that path,
         // so these intervals would never be read.  Should the path be widened
         // later by an unbalanced row, the fixup pass redoes the item.
-        bool seen = false;
-        for (int32_t t = P.px0 / B.chunk_px + lane; t <= P.px1 / B.chunk_px; t += 64) seen |= occ_row[t] <= P.p;
-        if (__ballot(seen) == 0) {
+        const int32_t t0 = P.px0 / B.chunk_px, t1 = P.px1 / B.chunk_px;
+        bool seen = t0 + lane <= t1 && occ_row[t0 + lane] <= P.p;
+        vmask = __ballot(seen);
+        for (int32_t t = t0 + 64 + lane; vmask == 0 && t <= t1; t += 64) vmask = __ballot(occ_row[t] <= P.p);
+        if (vmask == 0) {
             ++cur.skipped;
             write_skipped(H);
             return;
@@ -585,7 +622,7 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
     int n = build_entries(A, L, P.e0, P.e1, band_lo, band_hi, literal, Y0, valid, r, mrule, L.ex0, L.ex1, L.ec, K,
                           sm0);
     if (__ballot(n > K) == 0) {
-        span_finish(B, P, Y0, H, occ_row, L.ex0, L.ex1, L.ec, n, count_draw, occlusion, cur);
+        span_finish(B, P, Y0, H, occ_row, L.ex0, L.ex1, L.ec, n, count_draw, occlusion, vmask, cur);
         return;
     }
     if constexpr (DEFER) {
@@ -618,10 +655,41 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
     }
     int32_t *G1 = G + static_cast<long long>(need) * 64, *G2 = G + static_cast<long long>(need) * 128;
     n = build_entries(A, L, P.e0, P.e1, band_lo, band_hi, literal, Y0, valid, r, mrule, G, G1, G2, need, sm0);
-    span_finish(B, P, Y0, H, occ_row, G, G1, G2, n, count_draw, occlusion, cur);
+    span_finish(B, P, Y0, H, occ_row, G, G1, G2, n, count_draw, occlusion, vmask, cur);
 }
 
-constexpr int kSpanGrab = 16;  // span items taken per queue atomic
+constexpr int kSpanGroup = 16;  // items tested for occlusion together (4 lanes each)
+
+// Work queue split into kSlices contiguous slices with one counter each (in
+// its own 256-byte line).  A wave starts on slice blockIdx.x % kSlices --
+// workgroups are dispatched to the 8 XCDs round-robin, so each XCD works
+// through its own slice, out of its own L2 -- and moves on to the next slice
+// when its own is used up.  Each grab takes `grab` items; returns the first
+// item (n when every slice is used up) and sets *end.
+constexpr int kSlices = 8;
+constexpr int kSliceStride = 32;  // counters per slice line (unsigned long long)
+struct SliceCursor {
+    int32_t slice, tried;
+};
+__device__ __forceinline__ long long slice_grab(unsigned long long *q, long long n, long long grab, SliceCursor &c,
+                                                long long *end) {
+    while (c.tried < kSlices) {
+        const long long lo = n * c.slice / kSlices, hi = n * (c.slice + 1) / kSlices;
+        unsigned long long g = 0;
+        if (threadIdx.x == 0) g = atomicAdd(q + c.slice * kSliceStride, static_cast<unsigned long long>(grab));
+        const long long g0 = lo + static_cast<long long>(first_lane_u64(g));
+        if (g0 < hi) {
+            *end = g0 + grab < hi ? g0 + grab : hi;
+            return g0;
+        }
+        ++c.tried;
+        c.slice = (c.slice + 1) % kSlices;
+    }
+    *end = n;
+    return n;
+}
+
+constexpr int kSpanGrab = 64;  // span items per grab
 constexpr int kWideK = 32;     // LDS entry slots of the wide pass
 
 // The path record of item path p as wave-uniform values (scalar loads).
@@ -677,8 +745,9 @@ __device__ __forceinline__ ItemPath item_path_lanes(const int4 &q, int32_t p, in
 
 // Main span pass: every work item, descending path order (occluders are met
 // first), entry lists in LDS only; bands whose rows need more than K entries
-// are deferred to the wide pass.  Items are taken kSpanGrab at a time; the
-// group's single-band items reaching at most 4 composite tiles are tested for
+// are deferred to the wide pass.  Items are taken kSpanGrab at a time from
+// the sliced queue and handled kSpanGroup at a time; the group's single-band
+// items reaching at most 4 composite tiles are tested for
 // occlusion together (4 lanes per item), so a run of hidden items costs one
 // round trip, and a path's chunk summaries are loaded once for its run of
 // consecutive items.
@@ -693,18 +762,19 @@ __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
     SpanCursor cur;
     int32_t sm_path = -1;
     int2 sm0 = make_int2(INT32_MAX, INT32_MIN);
+    SliceCursor sc{static_cast<int32_t>(blockIdx.x % kSlices), 0};
     for (;;) {
-        unsigned long long g = 0;
-        if (lane == 0) g = atomicAdd(B.counters + kCntSpanItems, static_cast<unsigned long long>(kSpanGrab));
-        g = first_lane_u64(g);
-        if (static_cast<long long>(g) >= n_items) break;
-        const int cnt = static_cast<int>(min(static_cast<long long>(kSpanGrab), n_items - static_cast<long long>(g)));
+        long long g_end = 0;
+        const long long g0 = slice_grab(B.queues, n_items, kSpanGrab, sc, &g_end);
+        if (g0 >= n_items) break;
+      for (long long g = g0; g < g_end; g += kSpanGroup) {
+        const int cnt = static_cast<int>(min(static_cast<long long>(kSpanGroup), g_end - g));
         // lanes 4k..4k+3 hold item k of the group and quarter (lane & 3) of
         // its path record, read back per item with readlane
         const int qi = lane >> 2, sub = lane & 3;
         int4 gi = make_int4(0, 0, 0, -1), gq = make_int4(0, 0, 0, 0);
         if (qi < cnt) {
-            gi = B.sitems[n_items - 1 - static_cast<long long>(g) - qi];
+            gi = B.sitems[n_items - 1 - g - qi];
             gq = reinterpret_cast<const int4 *>(B.spath + gi.x)[sub];
         }
         // {px0, px1, width, height} sit in quarter 2, {px_hi, occ_base, ctiles, flags} in quarter 3
@@ -742,7 +812,8 @@ __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
             if (A.trace) t0 = static_cast<long long>(wall_clock64());
             for (int32_t bnd = 0; bnd < iz; ++bnd)
                 span_band<K, true>(B, L, P, iy + kBandRows * bnd, static_cast<long long>(iw) + bnd, 0, count_draw,
-                                   occlusion, occlusion && !was_tested, sm0, cur);
+                                   occlusion, occlusion && !was_tested,
+                                   was_tested ? (smask >> (4 * k)) & 0xfull : ~0ull, sm0, cur);
             if (A.trace && lane == 0) {
                 const long long t1 = static_cast<long long>(wall_clock64());
                 const unsigned long long kk = atomicAdd(A.trace_top, 1ull);
@@ -751,6 +822,7 @@ __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
                                                          static_cast<uint32_t>(iy), t0, t1);
             }
         }
+      }
     }
     if (lane == 0 && cur.stored) atomicAdd(B.counters + kCntStored, cur.stored);
     if (lane == 0 && cur.skipped) atomicAdd(B.counters + kCntSkipped, cur.skipped);
@@ -784,13 +856,13 @@ __global__ void __launch_bounds__(64) k_spans_wide(BatchArgs B, int32_t mode) {
             for (int32_t bnd = 0; bnd < item.z; ++bnd) {
                 const long long hi = static_cast<long long>(item.w) + bnd;
                 if (B.shdr[hi].off != kSkipped) continue;
-                span_band<K, false>(B, L, P, item.y + kBandRows * bnd, hi, blockIdx.x, false, false, false,
+                span_band<K, false>(B, L, P, item.y + kBandRows * bnd, hi, blockIdx.x, false, false, false, ~0ull,
                                     chunk_summary_first(B.F, P.e0, P.e1), cur);
             }
         } else {
             const int4 q = B.ovq[k];
             const ItemPath P = item_path(B, q.x);
-            span_band<K, false>(B, L, P, q.y, q.z, blockIdx.x, count_draw, occlusion, occlusion,
+            span_band<K, false>(B, L, P, q.y, q.z, blockIdx.x, count_draw, occlusion, occlusion, ~0ull,
                                 chunk_summary_first(B.F, P.e0, P.e1), cur);
         }
     }
@@ -1191,23 +1263,26 @@ __global__ void __launch_bounds__(64) k_compose(BatchArgs B, long long n_items) 
     __syncthreads();
     const int lane = threadIdx.x;
     unsigned long long n_paths = 0, n_tiles = 0;
+    SliceCursor sc{static_cast<int32_t>(blockIdx.x % kSlices), 0};
+    long long it = 0, it_end = 0;
     for (;;) {
-        unsigned long long it = 0;
-        if (lane == 0) it = atomicAdd(B.counters + kCntComposeItems, 1ull);
-        it = first_lane_u64(it);  // wave-uniform: the page search runs on scalar loads
-        if (static_cast<long long>(it) >= n_items) break;
+        if (it >= it_end) {
+            it = slice_grab(B.queues + kSlices * kSliceStride, n_items, 1, sc, &it_end);
+            if (it >= n_items) break;
+        }
         int32_t lo = 0, hi = B.n_cpages - 1;
         while (lo < hi) {
             const int32_t mid = (lo + hi + 1) / 2;
-            if (static_cast<long long>(B.cpages[mid].base) <= static_cast<long long>(it))
+            if (static_cast<long long>(B.cpages[mid].base) <= it)
                 lo = mid;
             else
                 hi = mid - 1;
         }
         const CPage cp = B.cpages[lo];
-        const int32_t local = static_cast<int32_t>(static_cast<long long>(it) - cp.base);
+        const int32_t local = static_cast<int32_t>(it - cp.base);
         const int32_t band = local / cp.tiles, tile = local - band * cp.tiles;
-        compose_tile<CH>(B, L, static_cast<long long>(it), cp.page, band, tile, &n_paths, &n_tiles);
+        compose_tile<CH>(B, L, it, cp.page, band, tile, &n_paths, &n_tiles);
+        ++it;
     }
     if (lane == 0) {
         if (n_paths) atomicAdd(B.counters + kCntPathTiles, n_paths);
@@ -1353,6 +1428,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
                                                 c->scan_tmp);
     if (!rc) rc = fill<int32_t>(c, c->band_cnt, n_bands, 0);
     if (!rc) rc = fill<unsigned long long>(c, c->counters, kCounters, 0ull);
+    if (!rc) rc = fill<unsigned long long>(c, c->queues, kQueueWords, 0ull);
     if (!rc) rc = fill<unsigned long long>(c, c->spill_top, 1, 0ull);
     if (!rc) rc = fill<int32_t>(c, c->occ, n_citems, -1);
     if (!rc) rc = ensure(c, &c->pstate, &c->pstate_cap, static_cast<size_t>(n_paths));
@@ -1417,12 +1493,13 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     B.ovq_cap = static_cast<long long>(c->ovq_cap);
     B.bin_off = c->bin_off;
     B.bins = c->bins;
+    B.queues = c->queues;
     // tile bins of the composite pass: sizes, then offsets (the sizes follow
     // the span pass: skipped items and items without intervals are left out)
     auto count_bins = [&]() -> int32_t {
         hipLaunchKernelGGL(k_tile_bins<false>, dim3(static_cast<unsigned>(n_bands)), dim3(64), 0, c->stream, c->pages,
                            b.n_pages, c->band_base, c->band_list, c->shdr, static_cast<long long>(c->shdr_cap), chunk_px,
-                           c->bin_cnt, c->bin_off, c->bins, static_cast<long long>(c->bins_cap), c->err_flag);
+                           c->occ, c->bin_cnt, c->bin_off, c->bins, static_cast<long long>(c->bins_cap), c->err_flag);
         RR_TRY(c, hipGetLastError());
         return scan_exclusive<int32_t, long long>(c, c->bin_cnt, c->bin_off, n_citems, c->bin_off + n_citems,
                                                   c->scan_tmp);
@@ -1431,7 +1508,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
         B.bins = c->bins;
         hipLaunchKernelGGL(k_tile_bins<true>, dim3(static_cast<unsigned>(n_bands)), dim3(64), 0, c->stream, c->pages,
                            b.n_pages, c->band_base, c->band_list, c->shdr, static_cast<long long>(c->shdr_cap), chunk_px,
-                           c->bin_cnt, c->bin_off, c->bins, static_cast<long long>(c->bins_cap), c->err_flag);
+                           c->occ, c->bin_cnt, c->bin_off, c->bins, static_cast<long long>(c->bins_cap), c->err_flag);
         RR_TRY(c, hipGetLastError());
         return RR_OK;
     };
@@ -1494,6 +1571,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
                         std::max(static_cast<size_t>(deferred + deferred / 4), 2 * static_cast<size_t>(B.ovq_cap)));
         if (!rc) rc = fill<int32_t>(c, c->err_flag, 1, 0);  // the lost deferrals set bit 32
         if (!rc) rc = fill<unsigned long long>(c, c->counters, kCounters, 0ull);
+        if (!rc) rc = fill<unsigned long long>(c, c->queues, kQueueWords, 0ull);
         if (!rc) rc = fill<unsigned long long>(c, c->spill_top, 1, 0ull);
         if (!rc) rc = fill<int32_t>(c, c->occ, n_citems, -1);
         if (!rc) rc = fill<int32_t>(c, c->pstate, n_paths, 0);

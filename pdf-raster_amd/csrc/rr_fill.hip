@@ -776,6 +776,7 @@ int32_t rr_hip_create(int32_t device, rr_hip_ctx **out) {
     if (rc == RR_OK && (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess)) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->spill_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->counters, kCounters * sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
+    if (rc == RR_OK && hipMalloc(&c->queues, kQueueWords * sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->err_flag, sizeof(int32_t)) != hipSuccess) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->trace_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
     for (auto &f : c->fences)
@@ -801,7 +802,7 @@ int32_t rr_hip_destroy(rr_hip_ctx *c) {
                     c->item_base, c->scan_tmp, c->sitems,     c->shdr,    c->spans,     c->band_cnt,
                     c->band_base, c->band_list, c->cpages,    c->counters, c->occ,      c->pstate,
                     c->feat,      c->route_cls, c->n_work,    c->work_base, c->mlp,       c->spath,
-                    c->ovq,       c->bin_cnt,  c->bin_off,    c->bins};
+                    c->ovq,       c->bin_cnt,  c->bin_off,    c->bins,    c->queues};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->t0) (void)hipEventDestroy(c->t0);

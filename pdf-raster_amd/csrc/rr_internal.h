@@ -15,8 +15,8 @@ namespace rrdev {
 // Device counters of a batched recording (unsigned long long each).
 constexpr int kCounters = 16;
 enum {
-    kCntSpanItems = 0,     // span-pass work items served
-    kCntComposeItems = 1,  // composite items served
+    kCntReserved0 = 0,     // (the work queues live in the sliced queue counters)
+    kCntReserved1 = 1,
     kCntPoolTop = 2,       // interval pool bump pointer (records handed out)
     kCntStored = 3,        // intervals stored
     kCntPathTiles = 4,     // (path, tile) composites
@@ -27,6 +27,10 @@ enum {
     kCntFixupServed = 9,   // fixup items served
     kCntSkipped = 10,      // bands skipped as occluded
 };
+
+// Sliced work-queue counters of the span and composite passes (8 slices
+// each, one 256-byte line per slice).
+constexpr int kQueueWords = 2 * 8 * 32;
 
 constexpr int kNumVariants = 16;  // instantiated fill shapes (the autotuner's search space)
 // K = LDS entry slots per AA row; CH = pixels per composite sweep (plain) /
@@ -153,6 +157,7 @@ struct rr_hip_ctx {
     // [0] span items served, [1] composite items served, [2] span pool top,
     // [3] intervals stored, [4] (path, tile) composites, [5] tiles touched
     unsigned long long *counters = nullptr;
+    unsigned long long *queues = nullptr;  // kQueueWords
     // profiling event pairs of the last batched recording: phase A [a0, a1), phase B b
     size_t prof_a0 = 0, prof_a1 = 0, prof_b = 0;
     int64_t batch_items = 0, batch_citems = 0, batch_reruns = 0, batch_work = 0;
