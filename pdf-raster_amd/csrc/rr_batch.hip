@@ -161,7 +161,7 @@ struct BatchArgs {
     int32_t n_cpages;
     unsigned long long *counters;
     int32_t *occ;  // per composite tile: highest opaque path index covering it fully
-    int32_t chunk_px;
+    int32_t chunk_px, chunk_shift;  // composite tile width, a power of two, and its log2
     int32_t *pstate;  // per path: bit 0 = widened by an unbalanced row
     int4 *ovq;        // deferred bands: {path, first row, span item, 0}
     long long ovq_cap;
@@ -418,7 +418,6 @@ template <int K>
 struct SpanLds {
     EdgeRec rec[64];
     unsigned long long smask[64];
-    unsigned long long emask[64];
     int32_t ex0[K * 64];
     int32_t ex1[K * 64];
     int32_t ec[K * 64];
@@ -453,16 +452,47 @@ __device__ __forceinline__ void hdr_head(SpanHdr *H, long long off, int32_t fx0,
 // The intervals of one (path, band) from its entry lists X0/X1/C (LDS or the
 // wave's global slot): sort and walk, drawn widths, fully covered columns,
 // store, header, occlusion publish.
+// Sort and walk of a row holding at most two entries, in registers (what
+// sort_and_walk does for n <= 2): its one interval goes back to slot 0.
+template <typename Ptr>
+__device__ __forceinline__ int walk_short(int n, Ptr X0, Ptr X1, Ptr C, int32_t mrule, int32_t aa_width,
+                                          bool *unbalanced) {
+    const int lane = threadIdx.x;
+    if (n == 0) return 0;
+    const int32_t a0 = X0[lane], b0 = X1[lane], c0 = C[lane];
+    int32_t start = a0, end = aa_width;
+    if (n == 2) {
+        const int32_t a1 = X0[64 + lane], b1 = X1[64 + lane], c1 = C[64 + lane];
+        // stable order: the second entry goes first only when strictly left
+        const bool swap = a1 < a0;
+        start = swap ? a1 : a0;
+        const int32_t x1 = swap ? b0 : b1;
+        const int32_t count = static_cast<int32_t>(static_cast<uint32_t>(c0) + static_cast<uint32_t>(c1));
+        if ((count & mrule) == 0)
+            end = x1 == INT32_MAX ? INT32_MAX : x1 + 1;
+        else
+            *unbalanced = true;
+    } else {
+        *unbalanced = true;  // one state-changing entry never closes
+    }
+    const int32_t bx0 = start > 0 ? start : 0;
+    int32_t bx1 = end < aa_width ? end : aa_width;
+    if (bx1 < 0) bx1 = 0;
+    if (bx0 >= bx1) return 0;
+    X0[lane] = bx0;
+    X1[lane] = bx1;
+    return 1;
+}
+
+// nint, unb: the walk's interval count of this lane's row (in X0/X1 slots)
+// and whether the row is unbalanced.
 template <typename Ptr>
 __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &P, int32_t Y0, SpanHdr *H,
-                                            const int32_t *occ_row, Ptr X0, Ptr X1, Ptr C, int n, bool count_draw,
-                                            bool occlusion, uint64_t vmask, SpanCursor &cur) {
+                                            const int32_t *occ_row, Ptr X0, Ptr X1, int nint, bool unb,
+                                            bool count_draw, bool occlusion, uint64_t vmask, SpanCursor &cur) {
     const FillArgs &A = B.F;
     const int lane = threadIdx.x;
     const int32_t y = Y0 + (lane >> 2), j = lane & 3;
-    const int32_t mrule = P.eo ? 1 : -1;
-    bool unb = false;
-    const int nint = sort_and_walk(n, X0, X1, C, mrule, P.width * 4, &unb);
     // an unbalanced row spans to the canvas edge: widen the path's columns;
     // its items skipped as occluded are redone by the fixup pass
     const bool any_unb = __ballot(unb) != 0;
@@ -514,15 +544,20 @@ __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &
     // px0 / chunk_px + i; tiles past the mask count as visible).  The others
     // would never be composited.  An unbalanced item keeps everything.
     const uint64_t vm = any_unb ? ~0ull : vmask;
-    const int32_t ch = B.chunk_px, t0 = P.px0 / ch;
+    const int32_t sh = B.chunk_shift, t0 = P.px0 >> sh;
     int keep = 0;
     int32_t cx0 = INT32_MAX, cx1 = INT32_MIN;
     for (int k = 0; k < nint; ++k) {
-        const int32_t pa = max(X0[k * 64 + lane] >> 2, P.px0), pb = min((X1[k * 64 + lane] - 1) >> 2, px1);
+        const int32_t a = X0[k * 64 + lane], b = X1[k * 64 + lane];
+        const int32_t pa = max(a >> 2, P.px0), pb = min((b - 1) >> 2, px1);
         if (pa > pb) continue;
-        const int32_t ta = pa / ch - t0, tb = pb / ch - t0;
+        const int32_t ta = (pa >> sh) - t0, tb = (pb >> sh) - t0;
         const uint64_t bits = tb >= 63 ? ~0ull << ta : ((2ull << tb) - 1ull) & (~0ull << ta);
         if (tb < 64 && (vm & bits) == 0) continue;
+        if (keep != k) {  // compact in place: the kept intervals move to the front
+            X0[keep * 64 + lane] = a;
+            X1[keep * 64 + lane] = b;
+        }
         ++keep;
         cx0 = min(cx0, pa);
         cx1 = max(cx1, pb);
@@ -552,27 +587,16 @@ __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &
     cur.stored += fits ? total : 0;
     if (fits) {
         int2 *dst = B.spans + off + excl;
-        int w = 0;
-        for (int k = 0; k < nint && w < keep; ++k) {
-            const int32_t a = X0[k * 64 + lane], b = X1[k * 64 + lane];
-            if (keep != nint) {
-                const int32_t pa = max(a >> 2, P.px0), pb = min((b - 1) >> 2, px1);
-                if (pa > pb) continue;
-                const int32_t ta = pa / ch - t0, tb = pb / ch - t0;
-                const uint64_t bits = tb >= 63 ? ~0ull << ta : ((2ull << tb) - 1ull) & (~0ull << ta);
-                if (tb < 64 && (vm & bits) == 0) continue;
-            }
-            dst[w++] = make_int2(a, b);
-        }
+        for (int k = 0; k < keep; ++k) dst[k] = make_int2(X0[k * 64 + lane], X1[k * 64 + lane]);
     }
     H->cnt[lane] = fits ? static_cast<uint16_t>(keep) : static_cast<uint16_t>(0);
     hdr_head(H, fits ? static_cast<long long>(off) : -1ll, fx0, fx1, cx0, cx1);
     // publish: composite tiles this opaque item covers fully (the same test
     // as the composite pass's occlusion scan)
     if (occlusion && Y0 + kBandRows <= P.height && fx0 <= fx1) {
-        const int32_t ta = (fx0 + ch - 1) / ch;
+        const int32_t ta = (fx0 + (1 << sh) - 1) >> sh;
         for (int32_t t = ta + lane; t < P.ctiles; t += 64) {
-            const int32_t txe = min((t + 1) * ch, P.width) - 1;
+            const int32_t txe = min((t + 1) << sh, P.width) - 1;
             if (txe > fx1) break;
             atomicMax(const_cast<int32_t *>(occ_row) + t, static_cast<int32_t>(P.p));
         }
@@ -602,7 +626,7 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
         // path covering it fully: the composite pass starts after that path,
         // so these intervals would never be read.  Should the path be widened
         // later by an unbalanced row, the fixup pass redoes the item.
-        const int32_t t0 = P.px0 / B.chunk_px, t1 = P.px1 / B.chunk_px;
+        const int32_t t0 = P.px0 >> B.chunk_shift, t1 = P.px1 >> B.chunk_shift;
         bool seen = t0 + lane <= t1 && occ_row[t0 + lane] <= P.p;
         vmask = __ballot(seen);
         for (int32_t t = t0 + 64 + lane; vmask == 0 && t <= t1; t += 64) vmask = __ballot(occ_row[t] <= P.p);
@@ -622,7 +646,10 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
     int n = build_entries(A, L, P.e0, P.e1, band_lo, band_hi, literal, Y0, valid, r, mrule, L.ex0, L.ex1, L.ec, K,
                           sm0);
     if (__ballot(n > K) == 0) {
-        span_finish(B, P, Y0, H, occ_row, L.ex0, L.ex1, L.ec, n, count_draw, occlusion, vmask, cur);
+        bool unb = false;
+        const int nint = __ballot(n > 2) == 0 ? walk_short(n, L.ex0, L.ex1, L.ec, mrule, P.width * 4, &unb)
+                                               : sort_and_walk(n, L.ex0, L.ex1, L.ec, mrule, P.width * 4, &unb);
+        span_finish(B, P, Y0, H, occ_row, L.ex0, L.ex1, nint, unb, count_draw, occlusion, vmask, cur);
         return;
     }
     if constexpr (DEFER) {
@@ -655,7 +682,9 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
     }
     int32_t *G1 = G + static_cast<long long>(need) * 64, *G2 = G + static_cast<long long>(need) * 128;
     n = build_entries(A, L, P.e0, P.e1, band_lo, band_hi, literal, Y0, valid, r, mrule, G, G1, G2, need, sm0);
-    span_finish(B, P, Y0, H, occ_row, G, G1, G2, n, count_draw, occlusion, vmask, cur);
+    bool unb = false;
+    const int nint = sort_and_walk(n, G, G1, G2, mrule, P.width * 4, &unb);
+    span_finish(B, P, Y0, H, occ_row, G, G1, nint, unb, count_draw, occlusion, vmask, cur);
 }
 
 constexpr int kSpanGroup = 16;  // items tested for occlusion together (4 lanes each)
@@ -782,7 +811,7 @@ __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
         const int32_t obase = quad_bcast<3>(gq.y), ctl = quad_bcast<3>(gq.z);
         bool tested = false, seen = false;
         if (occlusion && qi < cnt && gi.w >= 0 && gi.z == 1) {
-            const int32_t t0 = px0 / B.chunk_px, t1 = px1 / B.chunk_px;
+            const int32_t t0 = px0 >> B.chunk_shift, t1 = px1 >> B.chunk_shift;
             if (px0 <= px1 && t1 - t0 < 4) {
                 tested = true;
                 const int32_t t = t0 + sub;
@@ -1488,6 +1517,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     B.counters = c->counters;
     B.occ = c->occ;
     B.chunk_px = chunk_px;
+    B.chunk_shift = __builtin_ctz(static_cast<unsigned>(chunk_px));
     B.pstate = c->pstate;
     B.ovq = c->ovq;
     B.ovq_cap = static_cast<long long>(c->ovq_cap);

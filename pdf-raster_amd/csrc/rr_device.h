@@ -19,7 +19,8 @@ namespace rrdev {
 struct EdgeRec {
     int32_t r0, r1;  // AA rows [r0, r1] after the clip; empty when r0 > r1
     int32_t meta;    // bits 0-1 kind (0 sloped, 1 vertical, 2 horizontal),
-                     // bit 2 count +1 (FLIPPED), bit 3 first row has count 0
+                     // bit 2 count +1 (FLIPPED), bit 3 first row has count 0,
+                     // bit 4 EDGE_FAST (sloped, no saturation over its rows)
     int32_t a;       // sloped: px0 of row r0; vertical: x; horizontal: x0
     int32_t b;       // sloped: 16.16 right edge at row r0; horizontal: x1
     int32_t c;       // sloped: dxdy_fp
@@ -28,6 +29,7 @@ struct EdgeRec {
 static_assert(sizeof(EdgeRec) == 32, "EdgeRec layout");
 
 enum { KIND_SLOPED = 0, KIND_VERT = 1, KIND_HORIZ = 2 };
+constexpr int32_t EDGE_FAST = 16;
 enum { HDR_EMPTY = 1, HDR_OUTSIDE = 2 };
 
 // Scanner bounding box of one path in AA space (scanner/mod.rs:138-169).
@@ -158,6 +160,22 @@ __device__ __forceinline__ uint32_t wave_incl_or(uint32_t v) {
     return v;
 }
 
+// Inclusive prefix XOR over the wave, 64-bit.
+__device__ __forceinline__ uint32_t wave_incl_xor(uint32_t v) {
+    v ^= dpp_u32<kDppRowShr1>(v);
+    v ^= dpp_u32<kDppRowShr2>(v);
+    v ^= dpp_u32<kDppRowShr4>(v);
+    v ^= dpp_u32<kDppRowShr8>(v);
+    v ^= dpp_u32<kDppRowBcast15, 0xa>(v);
+    v ^= dpp_u32<kDppRowBcast31, 0xc>(v);
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_incl_xor64(unsigned long long v) {
+    const uint32_t lo = wave_incl_xor(static_cast<uint32_t>(v)), hi = wave_incl_xor(static_cast<uint32_t>(v >> 32));
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
 __device__ __forceinline__ unsigned long long wave_incl_or64(unsigned long long v) {
     const uint32_t lo = wave_incl_or(static_cast<uint32_t>(v)), hi = wave_incl_or(static_cast<uint32_t>(v >> 32));
     return (static_cast<unsigned long long>(hi) << 32) | lo;
@@ -255,8 +273,7 @@ struct BandLds {
     static constexpr int kWords = CH / 8;  // bitmap dwords per AA row
     static constexpr int kPitch = kWords + 1;
     EdgeRec rec[64];
-    unsigned long long smask[64];  // per lane row: edges of the chunk whose rows start there
-    unsigned long long emask[64];  // ... and end there
+    unsigned long long smask[64];  // per lane row: chunk edges whose row range starts or ends there
     int32_t ex0[K * 64];
     int32_t ex1[K * 64];
     int32_t ec[K * 64];
@@ -274,8 +291,20 @@ struct Entry {
 // clamped 64-bit sum).
 __device__ __forceinline__ Entry row_entry(const EdgeRec &q, int32_t r) {
     Entry en;
-    const int kind = q.meta & 3;
     const int32_t cnt = (q.meta & 4) ? 1 : -1;
+    if (q.meta & EDGE_FAST) {
+        // b + k*c stays inside int32 over the record's rows: 32-bit stepping,
+        // clamped to [d, e] (d <= e; one v_med3_i32)
+        const int32_t k = r - q.r0;
+        const int32_t t1 = q.b + k * q.c;
+        const int32_t px1 = min(max(t1, q.d), q.e) >> 16;
+        const int32_t px0 = k == 0 ? q.a : min(max(t1 - q.c, q.d), q.e) >> 16;
+        en.x0 = min(px0, px1);
+        en.x1 = max(px0, px1);
+        en.c = ((q.meta & 8) && k == 0) ? 0 : cnt;
+        return en;
+    }
+    const int kind = q.meta & 3;
     if (kind == KIND_HORIZ) {
         en.x0 = q.a;
         en.x1 = q.b;
@@ -401,17 +430,16 @@ __device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t 
             if (q.r0 <= q.r1) lane_range(q.r0, q.r1, literal, Y0, band_lo, band_hi, &la, &lb);
             L.rec[lane] = q;
             L.smask[lane] = 0ull;
-            L.emask[lane] = 0ull;
             __syncthreads();
+            // edge j toggles its bit on at lane la and off at lane lb + 1: the
+            // prefix XOR leaves it set exactly on lanes la..lb
             if (la <= lb) {
-                atomicOr(&L.smask[la], 1ull << lane);
-                atomicOr(&L.emask[lb], 1ull << lane);
+                atomicXor(&L.smask[la], 1ull << lane);
+                if (lb < 63) atomicXor(&L.smask[lb + 1], 1ull << lane);
             }
             __syncthreads();
-            const unsigned long long ps = wave_incl_or64(L.smask[lane]);
-            const unsigned long long pe = wave_incl_or64(L.emask[lane]);
-            const unsigned long long pe_before = wave_shr1_u64(pe);
-            unsigned long long m = valid ? (ps & ~pe_before) : 0ull;
+            unsigned long long m = wave_incl_xor64(L.smask[lane]);
+            if (!valid) m = 0ull;
             while (m) {
                 const int j = __ffsll(m) - 1;
                 m &= m - 1;

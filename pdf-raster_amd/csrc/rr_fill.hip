@@ -58,6 +58,31 @@ namespace {
 // of dxdy*65536 saturated) and aa_scale (x4, exact), and emits the integer
 // record of fill_intersections' first row.  The block reduces the scanner
 // bbox over the non-NaN segments that overlap the AA clip rows.
+// Vertical and horizontal edges whose columns fit 16 bits become sloped
+// records with no slope (b = d = e = column << 16: every row gives the same
+// column; a horizontal edge has one row, whose count is 0, so its entry is
+// {a, b} as before), and a sloped record whose 16.16 stepping stays inside
+// int32 over its rows is marked EDGE_FAST: row_entry then steps it in 32-bit
+// arithmetic, which equals the saturating 64-bit form there.
+__device__ __forceinline__ void to_fast_form(EdgeRec &r) {
+    if (r.r0 > r.r1) return;
+    const int kind = r.meta & 3;
+    const bool fits_a = r.a >= -32768 && r.a <= 32767;
+    if (kind == KIND_VERT && fits_a) {
+        r.meta = (r.meta & ~3) | KIND_SLOPED;
+        r.b = r.d = r.e = static_cast<int32_t>(static_cast<uint32_t>(r.a) << 16);
+        r.c = 0;
+    } else if (kind == KIND_HORIZ && fits_a && r.b >= -32768 && r.b <= 32767) {
+        r.meta = KIND_SLOPED | 8;
+        r.b = r.d = r.e = static_cast<int32_t>(static_cast<uint32_t>(r.b) << 16);
+        r.c = 0;
+    }
+    if ((r.meta & 3) == KIND_SLOPED) {
+        const long long last = static_cast<long long>(r.b) + static_cast<long long>(r.r1 - r.r0) * r.c;
+        if (last >= INT32_MIN && last <= INT32_MAX) r.meta |= EDGE_FAST;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ edges,
                                                     const int64_t *__restrict__ edge_off,
                                                     const DevPage *__restrict__ pages, int32_t n_pages,
@@ -160,6 +185,7 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
                     r.e = fp_from_f64(sx_max);
                 }
             }
+            to_fast_form(r);
         }
         recs[i] = r;
     }
