@@ -1625,6 +1625,9 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
 }
 
 int32_t zero_u64(rr_hip_ctx *c, unsigned long long *p, long long n) { return fill<unsigned long long>(c, p, n, 0ull); }
+int32_t fill_u64(rr_hip_ctx *c, unsigned long long *p, long long n, unsigned long long value) {
+    return fill<unsigned long long>(c, p, n, value);
+}
 
 }  // namespace rrint
 

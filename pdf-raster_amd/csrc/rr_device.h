@@ -160,6 +160,13 @@ __device__ __forceinline__ uint32_t wave_incl_or(uint32_t v) {
     return v;
 }
 
+// Sum over the wave of an f64 by an xor butterfly: every lane ends with the
+// same value (each step adds the same two operands, in either order).
+__device__ __forceinline__ double wave_sum_f64(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
 // Inclusive prefix XOR over the wave, 64-bit.
 __device__ __forceinline__ uint32_t wave_incl_xor(uint32_t v) {
     v ^= dpp_u32<kDppRowShr1>(v);

@@ -83,14 +83,21 @@ __device__ __forceinline__ void to_fast_form(EdgeRec &r) {
     }
 }
 
+// One wave per path (four per block): add_segment + aa_scale of every edge
+// into its EdgeRec, the scanner bbox, the chunk summaries (min first row, max
+// last row of every global 64-edge chunk: a plain store for a chunk inside the
+// path, atomics for one it shares with a neighbour; csum starts at
+// {INT32_MAX, INT32_MIN}), and the dispatcher's features.
 __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ edges,
-                                                    const int64_t *__restrict__ edge_off,
+                                                    const int64_t *__restrict__ edge_off, int64_t n_paths,
                                                     const DevPage *__restrict__ pages, int32_t n_pages,
                                                     EdgeRec *__restrict__ recs, PathHdr *__restrict__ hdrs,
                                                     int2 *__restrict__ rows, int4 *__restrict__ sel,
                                                     int32_t *__restrict__ nb, const rr_path_attr *__restrict__ attrs,
-                                                    float *__restrict__ feat, int32_t mode) {
-    const int64_t p = blockIdx.x;
+                                                    float *__restrict__ feat, int2 *__restrict__ csum, int32_t mode) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= n_paths) return;
     int32_t lo = 0, hi = n_pages - 1;  // page of path p (pages sorted by path_begin)
     while (lo < hi) {
         const int32_t mid = (lo + hi + 1) / 2;
@@ -106,125 +113,127 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
     int32_t xmn = INT32_MAX, xmx = INT32_MIN, ymn = INT32_MAX, ymx = INT32_MIN;
     int any = 0;
     double shoelace = 0.0, dy_sum = 0.0;  // dispatcher features
-    for (int64_t i = e0 + threadIdx.x; i < e1; i += blockDim.x) {
-        const double4 q = reinterpret_cast<const double4 *>(edges)[i];
-        double x0 = q.x, y0 = q.y, x1 = q.z, y1 = q.w;
-        if (feat && isfinite(x0) && isfinite(y0) && isfinite(x1) && isfinite(y1)) {
-            shoelace += x0 * y1 - x1 * y0;
-            dy_sum += fabs(y1 - y0);
-        }
-        const bool horiz = bits_of(y0) == bits_of(y1);
-        const bool vert = bits_of(x0) == bits_of(x1);
-        double dxdy = 0.0;
-        int32_t dxdy_fp = 0;
-        bool flipped = false;
-        if (!horiz) {
-            if (!vert) dxdy = (x1 - x0) / (y1 - y0);
-            if (y0 > y1) {
-                double t = x0;
-                x0 = x1;
-                x1 = t;
-                t = y0;
-                y0 = y1;
-                y1 = t;
-                flipped = true;
+    for (int64_t base = e0 & ~63ll; base < e1; base += 64) {
+        const int64_t i = base + lane;
+        int32_t cr0 = INT32_MAX, cr1 = INT32_MIN;
+        if (i >= e0 && i < e1) {
+            const double4 q = reinterpret_cast<const double4 *>(edges)[i];
+            double x0 = q.x, y0 = q.y, x1 = q.z, y1 = q.w;
+            if (feat && isfinite(x0) && isfinite(y0) && isfinite(x1) && isfinite(y1)) {
+                shoelace += x0 * y1 - x1 * y0;
+                dy_sum += fabs(y1 - y0);
             }
-            const double f = dxdy * 65536.0;
-            if (f >= 2147483647.0)
-                dxdy_fp = INT32_MAX;
-            else if (f <= -2147483648.0)
-                dxdy_fp = INT32_MIN;
-            else if (isnan(f))
-                dxdy_fp = 0;
-            else
-                dxdy_fp = static_cast<int32_t>(round(f));
-        }
-        x0 *= 4.0;
-        y0 *= 4.0;
-        x1 *= 4.0;
-        y1 *= 4.0;
-        const double sy_min = fmin(y0, y1), sy_max = fmax(y0, y1);
-        if (!(isnan(x0) || isnan(y0) || isnan(x1) || isnan(y1)) &&
-            !(sy_min >= static_cast<double>(cy1) + 1.0 || sy_max < static_cast<double>(cy0))) {
-            any = 1;
-            xmn = min(xmn, splash_floor(fmin(x0, x1)));
-            xmx = max(xmx, splash_floor(fmax(x0, x1)));
-            ymn = min(ymn, splash_floor(sy_min));
-            ymx = max(ymx, splash_floor(sy_max));
-        }
-        EdgeRec r{1, 0, 0, 0, 0, 0, 0, 0};
-        if (!(isnan(x0) || isnan(y0))) {
-            const int32_t fy0 = splash_floor(sy_min), fy1 = splash_floor(sy_max);
-            if (horiz) {
-                if (fy0 >= cy0 && fy0 <= cy1) {
-                    r.r0 = r.r1 = fy0;
-                    r.meta = KIND_HORIZ;
-                    r.a = splash_floor(fmin(x0, x1));
-                    r.b = splash_floor(fmax(x0, x1));
+            const bool horiz = bits_of(y0) == bits_of(y1);
+            const bool vert = bits_of(x0) == bits_of(x1);
+            double dxdy = 0.0;
+            int32_t dxdy_fp = 0;
+            bool flipped = false;
+            if (!horiz) {
+                if (!vert) dxdy = (x1 - x0) / (y1 - y0);
+                if (y0 > y1) {
+                    double t = x0;
+                    x0 = x1;
+                    x1 = t;
+                    t = y0;
+                    y0 = y1;
+                    y1 = t;
+                    flipped = true;
                 }
-            } else {
-                r.r0 = max(fy0, cy0);
-                r.r1 = min(fy1, cy1);
-                const int32_t first_zero = (sy_min < static_cast<double>(r.r0)) ? 0 : 8;
-                const int32_t pos = flipped ? 4 : 0;
-                if (vert) {
-                    r.meta = KIND_VERT | pos | first_zero;
-                    r.a = splash_floor(x0);
+                const double f = dxdy * 65536.0;
+                if (f >= 2147483647.0)
+                    dxdy_fp = INT32_MAX;
+                else if (f <= -2147483648.0)
+                    dxdy_fp = INT32_MIN;
+                else if (isnan(f))
+                    dxdy_fp = 0;
+                else
+                    dxdy_fp = static_cast<int32_t>(round(f));
+            }
+            x0 *= 4.0;
+            y0 *= 4.0;
+            x1 *= 4.0;
+            y1 *= 4.0;
+            const double sy_min = fmin(y0, y1), sy_max = fmax(y0, y1);
+            if (!(isnan(x0) || isnan(y0) || isnan(x1) || isnan(y1)) &&
+                !(sy_min >= static_cast<double>(cy1) + 1.0 || sy_max < static_cast<double>(cy0))) {
+                any = 1;
+                xmn = min(xmn, splash_floor(fmin(x0, x1)));
+                xmx = max(xmx, splash_floor(fmax(x0, x1)));
+                ymn = min(ymn, splash_floor(sy_min));
+                ymx = max(ymx, splash_floor(sy_max));
+            }
+            EdgeRec r{1, 0, 0, 0, 0, 0, 0, 0};
+            if (!(isnan(x0) || isnan(y0))) {
+                const int32_t fy0 = splash_floor(sy_min), fy1 = splash_floor(sy_max);
+                if (horiz) {
+                    if (fy0 >= cy0 && fy0 <= cy1) {
+                        r.r0 = r.r1 = fy0;
+                        r.meta = KIND_HORIZ;
+                        r.a = splash_floor(fmin(x0, x1));
+                        r.b = splash_floor(fmax(x0, x1));
+                    }
                 } else {
-                    r.meta = KIND_SLOPED | pos | first_zero;
-                    const double x_base = fma(y0, -dxdy, x0);
-                    const double sx_min = fmin(x0, x1), sx_max = fmax(x0, x1);
-                    const double rr0 = static_cast<double>(r.r0);
-                    double xx = (rr0 > y0) ? fma(rr0, dxdy, x_base) : x0;
-                    if (xx < sx_min) xx = sx_min;
-                    if (xx > sx_max) xx = sx_max;
-                    r.a = splash_floor(xx);
-                    r.b = fp_from_f64(fma(rr0 + 1.0, dxdy, x_base));
-                    r.c = dxdy_fp;
-                    r.d = fp_from_f64(sx_min);
-                    r.e = fp_from_f64(sx_max);
+                    r.r0 = max(fy0, cy0);
+                    r.r1 = min(fy1, cy1);
+                    const int32_t first_zero = (sy_min < static_cast<double>(r.r0)) ? 0 : 8;
+                    const int32_t pos = flipped ? 4 : 0;
+                    if (vert) {
+                        r.meta = KIND_VERT | pos | first_zero;
+                        r.a = splash_floor(x0);
+                    } else {
+                        r.meta = KIND_SLOPED | pos | first_zero;
+                        const double x_base = fma(y0, -dxdy, x0);
+                        const double sx_min = fmin(x0, x1), sx_max = fmax(x0, x1);
+                        const double rr0 = static_cast<double>(r.r0);
+                        double xx = (rr0 > y0) ? fma(rr0, dxdy, x_base) : x0;
+                        if (xx < sx_min) xx = sx_min;
+                        if (xx > sx_max) xx = sx_max;
+                        r.a = splash_floor(xx);
+                        r.b = fp_from_f64(fma(rr0 + 1.0, dxdy, x_base));
+                        r.c = dxdy_fp;
+                        r.d = fp_from_f64(sx_min);
+                        r.e = fp_from_f64(sx_max);
+                    }
                 }
+                to_fast_form(r);
             }
-            to_fast_form(r);
+            recs[i] = r;
+            if (r.r0 <= r.r1) {
+                cr0 = r.r0;
+                cr1 = r.r1;
+            }
         }
-        recs[i] = r;
+        cr0 = wave_min(cr0);
+        cr1 = wave_max(cr1);
+        if (lane == 0) {
+            const int64_t ch = base >> 6;
+            if (base >= e0 && base + 64 <= e1) {
+                csum[ch] = make_int2(cr0, cr1);
+            } else {
+                atomicMin(&csum[ch].x, cr0);
+                atomicMax(&csum[ch].y, cr1);
+            }
+        }
     }
-    __shared__ double fred[2][256];
+    xmn = wave_min(xmn);
+    xmx = wave_max(xmx);
+    ymn = wave_min(ymn);
+    ymx = wave_max(ymx);
+    any = __ballot(any != 0) != 0;
     if (feat) {
-        fred[0][threadIdx.x] = shoelace;
-        fred[1][threadIdx.x] = dy_sum;
+        shoelace = wave_sum_f64(shoelace);
+        dy_sum = wave_sum_f64(dy_sum);
     }
-    __shared__ int32_t red[5][256];
-    red[0][threadIdx.x] = xmn;
-    red[1][threadIdx.x] = xmx;
-    red[2][threadIdx.x] = ymn;
-    red[3][threadIdx.x] = ymx;
-    red[4][threadIdx.x] = any;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            red[0][threadIdx.x] = min(red[0][threadIdx.x], red[0][threadIdx.x + s]);
-            red[1][threadIdx.x] = max(red[1][threadIdx.x], red[1][threadIdx.x + s]);
-            red[2][threadIdx.x] = min(red[2][threadIdx.x], red[2][threadIdx.x + s]);
-            red[3][threadIdx.x] = max(red[3][threadIdx.x], red[3][threadIdx.x + s]);
-            red[4][threadIdx.x] |= red[4][threadIdx.x + s];
-            if (feat) {
-                fred[0][threadIdx.x] += fred[0][threadIdx.x + s];
-                fred[1][threadIdx.x] += fred[1][threadIdx.x + s];
-            }
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
+    if (lane == 0) {
         PathHdr h{0, -1, 1, 0, HDR_EMPTY, lo, 0, 0};
         int2 drawn = make_int2(1, 0);
-        if (red[4][0]) {
-            const int32_t y_min = max(red[2][0], cy0), y_max = min(red[3][0], cy1);
+        if (any) {
+            const int32_t y_min = max(ymn, cy0), y_max = min(ymx, cy1);
             if (y_min <= y_max) {
                 h.y_min = y_min;
                 h.y_max = y_max;
-                h.x_min = red[0][0];
-                h.x_max = red[1][0];
+                h.x_min = xmn;
+                h.x_max = xmx;
                 h.flags = 0;
                 // Clip::test_rect on the pixel bbox (clip.rs:196-220; fill/mod.rs:156-170)
                 const int32_t l = h.x_min / 4, t = h.y_min / 4, rt = h.x_max / 4, b = h.y_max / 4;
@@ -263,8 +272,8 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
             f[5] = static_cast<float>(bands);
             f[6] = at.eo ? 1.f : 0.f;
             f[7] = log2f((W + 1.f) / (Hh + 1.f));
-            f[8] = fminf(1.f, static_cast<float>(fabs(0.5 * fred[0][0])) / fmaxf(1.f, area));
-            f[9] = log2f(1.f + static_cast<float>(4.0 * fred[1][0]) / fmaxf(1.f, E));
+            f[8] = fminf(1.f, static_cast<float>(fabs(0.5 * shoelace)) / fmaxf(1.f, area));
+            f[9] = log2f(1.f + static_cast<float>(4.0 * dy_sum) / fmaxf(1.f, E));
             f[10] = static_cast<float>(at.a_input) / 255.f;
             f[11] = log2f(1.f + area / fmaxf(1.f, static_cast<float>(pg.width) * static_cast<float>(pg.height)));
         }
@@ -273,26 +282,6 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
 
 // Row summary of every 64-edge chunk (global edge index / 64): the smallest r0
 // and largest r1 of its non-empty records, so that a band skips whole chunks.
-__global__ void __launch_bounds__(256) k_chunk_summary(const EdgeRec *__restrict__ recs, int64_t n_edges,
-                                                       int2 *__restrict__ csum, int64_t n_chunks) {
-    const int64_t ch = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (ch >= n_chunks) return;
-    const int64_t i = ch * 64 + lane;
-    int32_t lo = INT32_MAX, hi = INT32_MIN;
-    if (i < n_edges) {
-        const EdgeRec q = recs[i];
-        if (q.r0 <= q.r1) {
-            lo = q.r0;
-            hi = q.r1;
-        }
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        lo = min(lo, __shfl_xor(lo, o));
-        hi = max(hi, __shfl_xor(hi, o));
-    }
-    if (lane == 0) csum[ch] = make_int2(lo, hi);
-}
 
 // Rasterises path p over output rows [Y0, Y0+15] of page `pg` and composites
 // into its canvas.  Called by one full wave64 (blockDim.x == 64); `slot` is
@@ -724,15 +713,14 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     }
     rc = rrint::zero_u64(c, c->spill_top, 1);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_edge_setup, dim3(static_cast<unsigned>(b.n_paths)), dim3(256), 0, c->stream, b.edges,
-                       b.edge_off, c->pages, b.n_pages, c->recs, c->hdrs, c->rows, c->sel, c->nb, b.attrs,
-                       b.launch == RR_LAUNCH_BATCHED ? c->feat : nullptr, b.mode);
+    // chunk summaries start empty: {INT32_MAX, INT32_MIN}
+    if (n_chunks > 0) rc = rrint::fill_u64(c, reinterpret_cast<unsigned long long *>(c->csum), n_chunks,
+                                            0x800000007fffffffull);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_edge_setup, dim3(static_cast<unsigned>((b.n_paths + 3) / 4)), dim3(256), 0, c->stream,
+                       b.edges, b.edge_off, b.n_paths, c->pages, b.n_pages, c->recs, c->hdrs, c->rows, c->sel, c->nb,
+                       b.attrs, b.launch == RR_LAUNCH_BATCHED ? c->feat : nullptr, c->csum, b.mode);
     RR_HIP_TRY(c, hipGetLastError());
-    if (n_chunks > 0) {
-        hipLaunchKernelGGL(k_chunk_summary, dim3(static_cast<unsigned>((n_chunks + 3) / 4)), dim3(256), 0, c->stream,
-                           c->recs, b.n_edges, c->csum, n_chunks);
-        RR_HIP_TRY(c, hipGetLastError());
-    }
 
     if (b.launch == RR_LAUNCH_PLAIN) {
         int32_t max_bands = 1;

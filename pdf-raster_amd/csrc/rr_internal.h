@@ -232,6 +232,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const rrdev::FillA
 int32_t init_batched(rr_hip_ctx *c);
 // Zero n 64-bit words with a kernel (safe inside graph capture).
 int32_t zero_u64(rr_hip_ctx *c, unsigned long long *p, long long n);
+int32_t fill_u64(rr_hip_ctx *c, unsigned long long *p, long long n, unsigned long long value);
 // Records the edge setup + fill of b (rr_fill.hip); used by graph capture.
 int32_t record_pages_internal(rr_hip_ctx *c, const rr_fill_pages &b);
 }  // namespace rrint
