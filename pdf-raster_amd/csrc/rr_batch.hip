@@ -166,7 +166,7 @@ struct BatchArgs {
     int4 *ovq;        // deferred bands: {path, first row, span item, 0}
     long long ovq_cap;
     const long long *bin_off;  // per composite item: first entry of its tile bin (n + 1)
-    const int2 *bins;          // tile bin entries: {span item, path}
+    const int4 *bins;          // tile bin entries: {span item, path, full-cover columns, attribute bytes}
     unsigned long long *queues;  // sliced queue counters: span pass, then composite pass
 };
 
@@ -346,8 +346,9 @@ __global__ void __launch_bounds__(64) k_tile_bins(const DevPage *__restrict__ pa
                                                   const long long *__restrict__ band_base,
                                                   const int2 *__restrict__ band_list, const SpanHdr *__restrict__ shdr,
                                                   long long shdr_cap, int32_t chunk_px, const int32_t *__restrict__ occ,
+                                                  const rr_path_attr *__restrict__ attrs,
                                                   int32_t *__restrict__ bin_cnt, const long long *__restrict__ bin_off,
-                                                  int2 *__restrict__ bins, long long bins_cap,
+                                                  int4 *__restrict__ bins, long long bins_cap,
                                                   int32_t *__restrict__ err) {
     __shared__ long long run[kMaxTiles];
     __shared__ int32_t cover[kMaxTiles];
@@ -374,11 +375,12 @@ __global__ void __launch_bounds__(64) k_tile_bins(const DevPage *__restrict__ pa
     for (long long lb = l0; lb < l1; lb += 64) {
         const long long q = lb + lane;
         int2 ent = make_int2(0, 0);
-        int32_t ta = INT32_MAX, tb = -1;
+        int32_t ta = INT32_MAX, tb = -1, fx = 0;
         if (q < l1) {
             ent = band_list[q];
             if (ent.y >= 0 && ent.y < shdr_cap) {
                 const int4 h = *reinterpret_cast<const int4 *>(shdr + ent.y);
+                fx = h.z;
                 const long long off = static_cast<long long>((static_cast<uint64_t>(static_cast<uint32_t>(h.y)) << 32) |
                                                              static_cast<uint32_t>(h.x));
                 const int32_t cx0 = static_cast<int16_t>(h.w & 0xffff), cx1 = static_cast<int16_t>(h.w >> 16);
@@ -398,7 +400,7 @@ __global__ void __launch_bounds__(64) k_tile_bins(const DevPage *__restrict__ pa
             if (PLACE && in) {
                 const long long pos = run[t] + __popcll(m & ((1ull << lane) - 1ull));
                 if (pos < bins_cap)
-                    bins[pos] = make_int2(ent.y, ent.x);
+                    bins[pos] = make_int4(ent.y, ent.x, fx, static_cast<int32_t>(reinterpret_cast<const uint32_t *>(attrs)[ent.x]));
                 else
                     atomicOr(err, 64);
             }
@@ -542,7 +544,8 @@ __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &
     // Intervals kept: those that reach, within the drawn columns, a composite
     // tile not yet covered by a later opaque path (bit i of vmask: tile
     // px0 / chunk_px + i; tiles past the mask count as visible).  The others
-    // would never be composited.  An unbalanced item keeps everything.
+    // would never be composited.  An unbalanced item keeps everything.  Kept
+    // intervals are clipped to the drawn columns.
     const uint64_t vm = any_unb ? ~0ull : vmask;
     const int32_t sh = B.chunk_shift, t0 = P.px0 >> sh;
     int keep = 0;
@@ -554,10 +557,10 @@ __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &
         const int32_t ta = (pa >> sh) - t0, tb = (pb >> sh) - t0;
         const uint64_t bits = tb >= 63 ? ~0ull << ta : ((2ull << tb) - 1ull) & (~0ull << ta);
         if (tb < 64 && (vm & bits) == 0) continue;
-        if (keep != k) {  // compact in place: the kept intervals move to the front
-            X0[keep * 64 + lane] = a;
-            X1[keep * 64 + lane] = b;
-        }
+        // compact in place, clipped to the drawn columns (the composite pass
+        // then clips to its tile only)
+        X0[keep * 64 + lane] = max(a, 4 * P.px0);
+        X1[keep * 64 + lane] = min(b, 4 * (px1 + 1));
         ++keep;
         cx0 = min(cx0, pa);
         cx1 = max(cx1, pb);
@@ -917,18 +920,15 @@ struct ComposeLds {
 struct PathRef {
     int64_t p;
     int32_t hi;   // span item
-    int4 s;       // {first row, last row, first px, last px}
     uint32_t at;  // rr_path_attr bytes {eo, color, a_input, route}
 };
 
-// Walks a tile bin in order, 64 entries per chunk, each chunk's paths' rows,
-// columns and attributes fetched with it.
+// Walks a tile bin in order, 64 entries per chunk.  A chunk already loaded
+// (the occlusion search's last one) is taken over when the walk starts in it.
 struct BinCursor {
     long long lb, l1;
     int32_t k, n;
-    int2 ent;
-    int4 s;
-    uint32_t at;
+    int4 ent;
 };
 
 __device__ __forceinline__ bool bin_next(const BatchArgs &B, const DevPage &pg, BinCursor &c, PathRef &out) {
@@ -938,26 +938,18 @@ __device__ __forceinline__ bool bin_next(const BatchArgs &B, const DevPage &pg, 
         if (c.lb >= c.l1) return false;
         c.n = static_cast<int32_t>(min(64ll, c.l1 - c.lb));
         c.k = 0;
-        c.ent = make_int2(0, 0);
-        c.s = make_int4(1, 0, 1, 0);
-        c.at = 0;
-        if (lane < c.n) {
-            c.ent = B.bins[c.lb + lane];
-            if (c.ent.y >= pg.path_begin && c.ent.y < pg.path_end && c.ent.x >= 0 && c.ent.x < B.shdr_cap) {
-                c.s = B.sel[c.ent.y];
-                c.at = reinterpret_cast<const uint32_t *>(B.F.attrs)[c.ent.y];
-            } else {
-                c.ent = make_int2(0, 0);
-                atomicOr(B.F.err, 16);  // a bin entry that was never written
-            }
-        }
+        c.ent = make_int4(0, 0, 0, 0);
+        if (lane < c.n) c.ent = B.bins[c.lb + lane];
     }
     const int k = c.k++;
     out.hi = __builtin_amdgcn_readlane(c.ent.x, k);
     out.p = __builtin_amdgcn_readlane(c.ent.y, k);
-    out.s = make_int4(__builtin_amdgcn_readlane(c.s.x, k), __builtin_amdgcn_readlane(c.s.y, k),
-                      __builtin_amdgcn_readlane(c.s.z, k), __builtin_amdgcn_readlane(c.s.w, k));
-    out.at = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int32_t>(c.at), k));
+    out.at = static_cast<uint32_t>(__builtin_amdgcn_readlane(c.ent.w, k));
+    if (out.p < pg.path_begin || out.p >= pg.path_end || out.hi < 0 || out.hi >= B.shdr_cap) {
+        if (lane == 0) atomicOr(B.F.err, 16);  // a bin entry that was never written
+        out.hi = 0;
+        out.at = 0;
+    }
     return true;
 }
 
@@ -1195,33 +1187,38 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, long long ci
     // when per-path coverage is counted.
     long long start = b0;
     int32_t fill_color = -1;
+    BinCursor cur;
+    cur.lb = b0 - 64;
+    cur.l1 = b1;
+    cur.k = cur.n = 0;
     if (!(A.cov_hash || A.cov_count) && nrows == kBandRows) {
-        long long last = -1;
-        for (long long lb = b1 - 64; lb + 64 > b0; lb -= 64) {
+        long long last = -1, lb = b1 - 64;
+        int4 e = make_int4(0, 0, 0, 0);
+        for (;; lb -= 64) {
             const long long q = lb + lane;
+            e = make_int4(0, 0, 0, 0);
             bool full = false;
             if (q >= b0 && q < b1) {
-                const int2 e = B.bins[q];
-                if (e.x >= 0 && e.x < B.shdr_cap) {
-                    const int32_t fx = *reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(B.shdr + e.x) + 8);
-                    full = static_cast<int16_t>(fx & 0xffff) <= tx0 && static_cast<int16_t>(fx >> 16) >= txe;
-                }
+                e = B.bins[q];
+                full = static_cast<int16_t>(e.z & 0xffff) <= tx0 && static_cast<int16_t>(e.z >> 16) >= txe;
             }
             const unsigned long long m = __ballot(full);
             if (m) {
                 last = lb + 63 - __clzll(m);
                 break;
             }
+            if (lb <= b0) break;
         }
         if (last >= 0) {
             start = last + 1;
-            fill_color = A.attrs[B.bins[last].y].color;
+            fill_color = (__builtin_amdgcn_readlane(e.w, static_cast<int>(last - lb)) >> 8) & 0xff;
         }
+        // the walk starts inside the chunk just searched
+        cur.lb = lb;
+        cur.n = static_cast<int32_t>(min(64ll, b1 - lb));
+        cur.k = static_cast<int32_t>(start - lb);
+        cur.ent = e;
     }
-    BinCursor cur;
-    cur.lb = start - 64;
-    cur.l1 = b1;
-    cur.k = cur.n = 0;
     PathRef P0, P1;
     const bool has0 = bin_next(B, pg, cur, P0);
     if (!has0 && fill_color < 0) return;
@@ -1250,9 +1247,7 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, long long ci
     __syncthreads();
     if (has0 && off0 < 0 && lane == 0) atomicOr(A.err, 8);
     for (; has0;) {
-        const int32_t xa = max(P0.s.z, tx0), xb = min(P0.s.w, txe);
-        const int32_t ya = max(P0.s.x, Y0), yb = min(P0.s.y, Y0 + nrows - 1);
-        const bool live = xa <= xb && ya <= yb;
+        // stored intervals lie within the path's drawn columns and rows
         // next path: its first span; the one after: its header
         SpanView V1;
         V1.n = 0;
@@ -1266,7 +1261,7 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, long long ci
             has2 = bin_next(B, pg, cur, P2);
             if (has2) hdr_fetch(B, P2.hi, &off2, &n2);
         }
-        if (live) compose_rows<CH>(B, L, pg, P0.p, P0.at, V0, Y0, tx0, xa, xb, ya, yb);
+        compose_rows<CH>(B, L, pg, P0.p, P0.at, V0, Y0, tx0, tx0, txe, Y0, Y0 + nrows - 1);
         ++*n_paths;
         __syncthreads();
         if (!has1) break;
@@ -1529,7 +1524,8 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     auto count_bins = [&]() -> int32_t {
         hipLaunchKernelGGL(k_tile_bins<false>, dim3(static_cast<unsigned>(n_bands)), dim3(64), 0, c->stream, c->pages,
                            b.n_pages, c->band_base, c->band_list, c->shdr, static_cast<long long>(c->shdr_cap), chunk_px,
-                           c->occ, c->bin_cnt, c->bin_off, c->bins, static_cast<long long>(c->bins_cap), c->err_flag);
+                           c->occ, b.attrs, c->bin_cnt, c->bin_off, c->bins, static_cast<long long>(c->bins_cap),
+                           c->err_flag);
         RR_TRY(c, hipGetLastError());
         return scan_exclusive<int32_t, long long>(c, c->bin_cnt, c->bin_off, n_citems, c->bin_off + n_citems,
                                                   c->scan_tmp);
@@ -1538,7 +1534,8 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
         B.bins = c->bins;
         hipLaunchKernelGGL(k_tile_bins<true>, dim3(static_cast<unsigned>(n_bands)), dim3(64), 0, c->stream, c->pages,
                            b.n_pages, c->band_base, c->band_list, c->shdr, static_cast<long long>(c->shdr_cap), chunk_px,
-                           c->occ, c->bin_cnt, c->bin_off, c->bins, static_cast<long long>(c->bins_cap), c->err_flag);
+                           c->occ, b.attrs, c->bin_cnt, c->bin_off, c->bins, static_cast<long long>(c->bins_cap),
+                           c->err_flag);
         RR_TRY(c, hipGetLastError());
         return RR_OK;
     };

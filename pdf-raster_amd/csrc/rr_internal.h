@@ -140,7 +140,7 @@ struct rr_hip_ctx {
     size_t bin_cnt_cap = 0;
     long long *bin_off = nullptr;  // per composite tile: first entry of its bin (n + 1)
     size_t bin_off_cap = 0;
-    int2 *bins = nullptr;  // composite tile bins: {span item, path}
+    int4 *bins = nullptr;  // composite tile bins: {span item, path, full-cover columns, attribute bytes}
     size_t bins_cap = 0;
     long long batch_bins = 0;  // bin entries of the last batched recording (the sizing pass's while capturing)
     // dispatcher
