@@ -254,14 +254,17 @@ __global__ void __launch_bounds__(256) k_dispatch(const rr_path_attr *__restrict
 }
 
 // Work items of path p (one per band for SPAN, one for all bands for a TILE
-// class) and the per-band path counts.  One wave per path.
+// class) and the per-band path counts as marks: +1 at the path's first
+// batch-global band, -1 one past its last (the next page's first band or the
+// end), so that the prefix sum of the marks counts the paths drawing each
+// band.  One wave per path.
 __global__ void __launch_bounds__(256) k_expand(const int4 *__restrict__ sel, const int32_t *__restrict__ nb,
                                                 const int32_t *__restrict__ n_work, const PathHdr *__restrict__ hdrs,
                                                 const DevPage *__restrict__ pages,
                                                 const long long *__restrict__ item_base,
                                                 const long long *__restrict__ work_base, long long n_paths,
                                                 int4 *__restrict__ sitems, long long sitems_cap,
-                                                int32_t *__restrict__ band_cnt, int32_t *__restrict__ err) {
+                                                int32_t *__restrict__ band_marks, int32_t *__restrict__ err) {
     const long long p = static_cast<long long>(blockIdx.x) * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (p >= n_paths) return;
@@ -282,7 +285,10 @@ __global__ void __launch_bounds__(256) k_expand(const int4 *__restrict__ sel, co
                 sitems[it] = make_int4(static_cast<int32_t>(p), kBandRows * b0, n, static_cast<int32_t>(base));
             }
         }
-        atomicAdd(band_cnt + gb0 + k, 1);
+    }
+    if (lane == 0) {
+        atomicAdd(band_marks + gb0, 1);
+        atomicAdd(band_marks + gb0 + n, -1);
     }
 }
 
@@ -1421,7 +1427,8 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     int32_t rc = ensure(c, &c->item_base, &c->item_base_cap, static_cast<size_t>(n_paths + 1));
     const long long scan_n = std::max<long long>(std::max<long long>(n_paths, n_bands), c->batch_citems);
     if (!rc) rc = ensure(c, &c->scan_tmp, &c->scan_tmp_cap, static_cast<size_t>((scan_n + kScanTile - 1) / kScanTile + 1));
-    if (!rc) rc = ensure(c, &c->band_cnt, &c->band_cnt_cap, static_cast<size_t>(std::max<long long>(n_bands, 1)));
+    if (!rc) rc = ensure(c, &c->band_cnt, &c->band_cnt_cap, static_cast<size_t>(n_bands + 2));
+    if (!rc) rc = ensure(c, &c->band_run, &c->band_run_cap, static_cast<size_t>(n_bands + 2));
     if (!rc) rc = ensure(c, &c->band_base, &c->band_base_cap, static_cast<size_t>(n_bands + 1));
     if (!rc) rc = ensure(c, &c->cpages, &c->cpages_cap, static_cast<size_t>(b.n_pages));
     if (rc) return rc;
@@ -1450,7 +1457,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     if (!rc)
         rc = scan_exclusive<int32_t, long long>(c, c->n_work, c->work_base, n_paths, c->work_base + n_paths,
                                                 c->scan_tmp);
-    if (!rc) rc = fill<int32_t>(c, c->band_cnt, n_bands, 0);
+    if (!rc) rc = fill<int32_t>(c, c->band_cnt, n_bands + 2, 0);
     if (!rc) rc = fill<unsigned long long>(c, c->counters, kCounters, 0ull);
     if (!rc) rc = fill<unsigned long long>(c, c->queues, kQueueWords, 0ull);
     if (!rc) rc = fill<unsigned long long>(c, c->spill_top, 1, 0ull);
@@ -1477,8 +1484,11 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
                        c->nb, c->n_work, c->hdrs, c->pages, c->item_base, c->work_base, n_paths, c->sitems,
                        static_cast<long long>(c->sitems_cap), c->band_cnt, c->err_flag);
     RR_TRY(c, hipGetLastError());
-    rc = scan_exclusive<int32_t, long long>(c, c->band_cnt, c->band_base, n_bands, c->band_base + n_bands,
-                                            c->scan_tmp);
+    // paths per band = prefix sum of the marks (band_run[i + 1]); then the
+    // bands' list offsets
+    rc = scan_exclusive<int32_t, long long>(c, c->band_cnt, c->band_run, n_bands + 1, nullptr, c->scan_tmp);
+    if (!rc) rc = scan_exclusive<long long, long long>(c, c->band_run + 1, c->band_base, n_bands,
+                                                       c->band_base + n_bands, c->scan_tmp);
     if (rc) return rc;
     hipLaunchKernelGGL(k_band_lists, dim3(static_cast<unsigned>(n_bands)), dim3(64), 0, c->stream, c->sel, c->pages,
                        b.n_pages, c->item_base, c->band_base, c->band_list, static_cast<long long>(c->band_list_cap),

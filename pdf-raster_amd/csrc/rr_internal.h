@@ -120,9 +120,11 @@ struct rr_hip_ctx {
     size_t shdr_cap = 0;
     int2 *spans = nullptr;
     size_t spans_cap = 0;
-    int32_t *band_cnt = nullptr;  // per batch-global band: paths drawing it
+    int32_t *band_cnt = nullptr;  // per batch-global band: +1/-1 marks of the paths' band ranges (n_bands + 2)
     size_t band_cnt_cap = 0;
-    long long *band_base = nullptr;  // exclusive prefix of band_cnt (n_bands + 1)
+    long long *band_run = nullptr;  // exclusive prefix of the marks: band_run[i + 1] = paths drawing band i
+    size_t band_run_cap = 0;
+    long long *band_base = nullptr;  // exclusive prefix of the per-band path counts (n_bands + 1)
     size_t band_base_cap = 0;
     int2 *band_list = nullptr;  // per band, in path order: (path, span item)
     size_t band_list_cap = 0;
