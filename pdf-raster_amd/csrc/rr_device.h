@@ -279,7 +279,7 @@ template <int K, int CH>
 struct BandLds {
     static constexpr int kWords = CH / 8;  // bitmap dwords per AA row
     static constexpr int kPitch = kWords + 1;
-    EdgeRec rec[64];
+    int4 recA[64], recB[64];       // the chunk's EdgeRecs as two 16-byte halves (fewer bank conflicts)
     unsigned long long smask[64];  // per lane row: chunk edges whose row range starts or ends there
     int32_t ex0[K * 64];
     int32_t ex1[K * 64];
@@ -435,22 +435,27 @@ __device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t 
             if (after) qnn = load_rec(__ffsll(static_cast<unsigned long long>(after)) - 1);
             int la = 64, lb = -1;
             if (q.r0 <= q.r1) lane_range(q.r0, q.r1, literal, Y0, band_lo, band_hi, &la, &lb);
-            L.rec[lane] = q;
+            L.recA[lane] = make_int4(q.r0, q.meta, q.a, q.b);
+            L.recB[lane] = make_int4(q.c, q.d, q.e, q.r1);
             L.smask[lane] = 0ull;
             __syncthreads();
             // edge j toggles its bit on at lane la and off at lane lb + 1: the
-            // prefix XOR leaves it set exactly on lanes la..lb
+            // prefix XOR leaves it set exactly on lanes la..lb.  Edges that
+            // start at lane 0 (most: they enter the band from above) are a
+            // ballot instead of a pile of atomics on one word.
+            const unsigned long long from0 = __ballot(la == 0 && lb >= 0);
             if (la <= lb) {
-                atomicXor(&L.smask[la], 1ull << lane);
+                if (la > 0) atomicXor(&L.smask[la], 1ull << lane);
                 if (lb < 63) atomicXor(&L.smask[lb + 1], 1ull << lane);
             }
             __syncthreads();
-            unsigned long long m = wave_incl_xor64(L.smask[lane]);
+            unsigned long long m = wave_incl_xor64(L.smask[lane]) ^ from0;
             if (!valid) m = 0ull;
             while (m) {
                 const int j = __ffsll(m) - 1;
                 m &= m - 1;
-                const EdgeRec g = L.rec[j];
+                const int4 ga = L.recA[j], gb = L.recB[j];
+                const EdgeRec g{ga.x, gb.w, ga.y, ga.z, ga.w, gb.x, gb.y, gb.z};
                 const Entry en = row_entry(g, r);
                 if (has_last && static_cast<long long>(lx1) + 1 >= en.x0 &&
                     static_cast<long long>(lx0) <= static_cast<long long>(en.x1) + 1) {
