@@ -727,7 +727,7 @@ __device__ __forceinline__ long long slice_grab(unsigned long long *q, long long
     return n;
 }
 
-constexpr int kSpanGrab = 64;  // span items per grab
+constexpr int kSpanGrab = 64;  // span items per grab at most (fewer when the batch is small)
 constexpr int kWideK = 32;     // LDS entry slots of the wide pass
 
 // The path record of item path p as wave-uniform values (scalar loads).
@@ -801,9 +801,12 @@ __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
     int32_t sm_path = -1;
     int2 sm0 = make_int2(INT32_MAX, INT32_MIN);
     SliceCursor sc{static_cast<int32_t>(blockIdx.x % kSlices), 0};
+    // about eight grabs per wave, so that a small batch still spreads over
+    // the whole grid
+    const long long grab = max(1ll, min(static_cast<long long>(kSpanGrab), n_items / (8ll * gridDim.x)));
     for (;;) {
         long long g_end = 0;
-        const long long g0 = slice_grab(B.queues, n_items, kSpanGrab, sc, &g_end);
+        const long long g0 = slice_grab(B.queues, n_items, grab, sc, &g_end);
         if (g0 >= n_items) break;
       for (long long g = g0; g < g_end; g += kSpanGroup) {
         const int cnt = static_cast<int>(min(static_cast<long long>(kSpanGroup), g_end - g));
