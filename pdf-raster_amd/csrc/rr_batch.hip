@@ -424,7 +424,7 @@ __global__ void __launch_bounds__(64) k_tile_bins(const DevPage *__restrict__ pa
 // ---------------------------------------------------------------------------
 template <int K>
 struct SpanLds {
-    int4 recA[64], recB[64];  // the chunk's EdgeRecs as two 16-byte halves
+    int4 recA[1][64], recB[1][64];  // the chunk's EdgeRecs as their two 16-byte halves
     unsigned long long smask[64];
     int32_t ex0[K * 64];
     int32_t ex1[K * 64];

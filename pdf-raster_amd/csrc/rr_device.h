@@ -279,7 +279,7 @@ template <int K, int CH>
 struct BandLds {
     static constexpr int kWords = CH / 8;  // bitmap dwords per AA row
     static constexpr int kPitch = kWords + 1;
-    int4 recA[64], recB[64];       // the chunk's EdgeRecs as two 16-byte halves (fewer bank conflicts)
+    int4 recA[1][64], recB[1][64];  // the chunk's EdgeRecs as their two 16-byte halves
     unsigned long long smask[64];  // per lane row: chunk edges whose row range starts or ends there
     int32_t ex0[K * 64];
     int32_t ex1[K * 64];
@@ -405,6 +405,47 @@ __device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t 
         }
         ++n;
     };
+    // The chunk in LDS slot s (lane j's record in recA[s][j] / recB[s][j]):
+    // fold the edges crossing this lane's row, in edge order.  r0/r1: this
+    // lane's own record's rows.
+    auto process = [&](int s, int32_t r0, int32_t r1) {
+        int la = 64, lb = -1;
+        if (r0 <= r1) lane_range(r0, r1, literal, Y0, band_lo, band_hi, &la, &lb);
+        L.smask[lane] = 0ull;
+        __syncthreads();
+        // edge j toggles its bit on at lane la and off at lane lb + 1: the
+        // prefix XOR leaves it set exactly on lanes la..lb.  Edges that start
+        // at lane 0 (most: they enter the band from above) are a ballot
+        // instead of a pile of atomics on one word.
+        const unsigned long long from0 = __ballot(la == 0 && lb >= 0);
+        if (la <= lb) {
+            if (la > 0) atomicXor(&L.smask[la], 1ull << lane);
+            if (lb < 63) atomicXor(&L.smask[lb + 1], 1ull << lane);
+        }
+        __syncthreads();
+        unsigned long long m = wave_incl_xor64(L.smask[lane]) ^ from0;
+        if (!valid) m = 0ull;
+        while (m) {
+            const int j = __ffsll(m) - 1;
+            m &= m - 1;
+            const int4 ga = L.recA[s][j], gb = L.recB[s][j];
+            const EdgeRec g{ga.x, ga.y, ga.z, ga.w, gb.x, gb.y, gb.z, gb.w};
+            const Entry en = row_entry(g, r);
+            if (has_last && static_cast<long long>(lx1) + 1 >= en.x0 &&
+                static_cast<long long>(lx0) <= static_cast<long long>(en.x1) + 1) {
+                lc = static_cast<int32_t>(static_cast<uint32_t>(lc) + static_cast<uint32_t>(en.c));
+                lx0 = min(lx0, en.x0);
+                lx1 = max(lx1, en.x1);
+            } else {
+                if (has_last) retire(lx0, lx1, lc);
+                lx0 = en.x0;
+                lx1 = en.x1;
+                lc = en.c;
+                has_last = true;
+            }
+        }
+        __syncthreads();
+    };
     if (e1 <= e0) return 0;
     const int64_t c_first = e0 >> 6, c_last = (e1 - 1) >> 6;
     for (int64_t cb = c_first; cb <= c_last; cb += 64) {
@@ -433,44 +474,9 @@ __device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t 
             qn = qnn;
             const uint64_t after = cmask & (cmask - 1);
             if (after) qnn = load_rec(__ffsll(static_cast<unsigned long long>(after)) - 1);
-            int la = 64, lb = -1;
-            if (q.r0 <= q.r1) lane_range(q.r0, q.r1, literal, Y0, band_lo, band_hi, &la, &lb);
-            L.recA[lane] = make_int4(q.r0, q.meta, q.a, q.b);
-            L.recB[lane] = make_int4(q.c, q.d, q.e, q.r1);
-            L.smask[lane] = 0ull;
-            __syncthreads();
-            // edge j toggles its bit on at lane la and off at lane lb + 1: the
-            // prefix XOR leaves it set exactly on lanes la..lb.  Edges that
-            // start at lane 0 (most: they enter the band from above) are a
-            // ballot instead of a pile of atomics on one word.
-            const unsigned long long from0 = __ballot(la == 0 && lb >= 0);
-            if (la <= lb) {
-                if (la > 0) atomicXor(&L.smask[la], 1ull << lane);
-                if (lb < 63) atomicXor(&L.smask[lb + 1], 1ull << lane);
-            }
-            __syncthreads();
-            unsigned long long m = wave_incl_xor64(L.smask[lane]) ^ from0;
-            if (!valid) m = 0ull;
-            while (m) {
-                const int j = __ffsll(m) - 1;
-                m &= m - 1;
-                const int4 ga = L.recA[j], gb = L.recB[j];
-                const EdgeRec g{ga.x, gb.w, ga.y, ga.z, ga.w, gb.x, gb.y, gb.z};
-                const Entry en = row_entry(g, r);
-                if (has_last && static_cast<long long>(lx1) + 1 >= en.x0 &&
-                    static_cast<long long>(lx0) <= static_cast<long long>(en.x1) + 1) {
-                    lc = static_cast<int32_t>(static_cast<uint32_t>(lc) + static_cast<uint32_t>(en.c));
-                    lx0 = min(lx0, en.x0);
-                    lx1 = max(lx1, en.x1);
-                } else {
-                    if (has_last) retire(lx0, lx1, lc);
-                    lx0 = en.x0;
-                    lx1 = en.x1;
-                    lc = en.c;
-                    has_last = true;
-                }
-            }
-            __syncthreads();
+            L.recA[0][lane] = make_int4(q.r0, q.r1, q.meta, q.a);
+            L.recB[0][lane] = make_int4(q.b, q.c, q.d, q.e);
+            process(0, q.r0, q.r1);
         }
     }
     if (has_last) retire(lx0, lx1, lc);
