@@ -525,11 +525,18 @@ def main():
     if launch == R.LAUNCH_BATCHED:
         ms, st = be.last_batch_stats()
         ch = R.variants()[variant][1]
-        # span pass: edge records in, intervals + headers + items out/in
-        a_bytes = 32 * n_edges + 8 * st["intervals"] + (136 + 8) * st["items"]
-        # composite pass: each touched tile read + written once, each interval,
-        # header, band-list entry and path column range read once
-        b_bytes = 2 * 16 * ch * st["tiles"] + 8 * st["intervals"] + (136 + 8 + 16) * st["items"]
+        # span pass, compulsory bytes: the edge records of every path with a
+        # built band read once, the work items and path records read, the
+        # intervals written, a 144-byte header per built band and a 16-byte
+        # one per band skipped as occluded
+        built = st["items"] - st["skipped"]
+        a_bytes = (32 * st["built_edges"] + 16 * st["items"] + 64 * n_paths + 8 * st["intervals"]
+                   + 144 * built + 16 * st["skipped"])
+        # composite pass: each touched tile written once and read unless it
+        # starts from an opaque cover's colour, every interval it composites
+        # read once, a bin entry and a header per composited path-tile
+        b_bytes = (16 * ch * (2 * st["tiles"] - st["filled_tiles"]) + 8 * st["intervals_read"]
+                   + (16 + 144) * st["path_tiles"])
         for name, t_ms, nbytes in (("k_spans", ms["span_ms"], a_bytes), ("k_compose", ms["compose_ms"], b_bytes)):
             gbs = nbytes / (t_ms / 1e3) / 1e9 if t_ms > 0 else 0.0
             kernels[name] = {"ms": round(t_ms, 3), "alg_bytes": int(nbytes), "achieved_gbs": round(gbs, 3),

@@ -322,8 +322,11 @@ int32_t rr_hip_train_dispatcher(rr_hip_ctx *ctx, const float *features, const fl
  * span items, intervals stored, (page, band, tile) composite items,
  * (path, tile) composites, tiles holding at least one path, span-pass reruns
  * after growing the interval pool or the deferred queue, (path, band) items
- * skipped as occluded, (path, band) items deferred to the wide span pass}. */
-int32_t rr_hip_last_batch_stats(rr_hip_ctx *ctx, double ms[2], int64_t counts[8]);
+ * skipped as occluded, (path, band) items deferred to the wide span pass,
+ * paths with at least one band built, the edge records of those paths,
+ * composite tiles started from an opaque cover's colour (the canvas tile is
+ * not read), intervals the composite pass read}. */
+int32_t rr_hip_last_batch_stats(rr_hip_ctx *ctx, double ms[2], int64_t counts[12]);
 
 /* In-place exclusive prefix sum of len_elems uint32 (wrapping), replacing the
  * three record_scan phases of the reference (GpuBackend::record_scan,

@@ -162,7 +162,7 @@ struct BatchArgs {
     unsigned long long *counters;
     int32_t *occ;  // per composite tile: highest opaque path index covering it fully
     int32_t chunk_px, chunk_shift;  // composite tile width, a power of two, and its log2
-    int32_t *pstate;  // per path: bit 0 = widened by an unbalanced row
+    int32_t *pstate;  // per path: bit 0 = widened by an unbalanced row, bit 1 = a band was built
     int4 *ovq;        // deferred bands: {path, first row, span item, 0}
     long long ovq_cap;
     const long long *bin_off;  // per composite item: first entry of its tile bin (n + 1)
@@ -437,6 +437,7 @@ constexpr long long kSpanChunk = 8192;
 struct SpanCursor {
     unsigned long long next = 0, end = 0, stored = 0;
     unsigned long long skipped = 0;  // bands skipped as occluded
+    int32_t marked = -1;             // last path marked as having a visible band
 };
 
 // A span item's path as wave-uniform values (unpacked from its SpanPath).
@@ -644,6 +645,11 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
             write_skipped(H);
             return;
         }
+    }
+    // the path has a band that is built (pstate bit 1; for the statistics)
+    if (P.p != cur.marked) {
+        if (lane == 0) atomicOr(B.pstate + P.p, 2);
+        cur.marked = static_cast<int32_t>(P.p);
     }
     const bool literal = A.mode == RR_MODE_LITERAL;
     const int32_t y = Y0 + (lane >> 2), j = lane & 3;
@@ -1005,7 +1011,9 @@ __device__ __forceinline__ uint32_t comp1(uint32_t dv, uint32_t shape, uint32_t 
 // (sum over AA rows of the interval's overlap with [4x, 4x+4)).  Rows with
 // more intervals go through the 4x4-bit coverage bitmap.  Both give the
 // nibble-popcount coverage of fill/mod.rs:344-401 exactly.
-template <int CH>
+// COV: per-path coverage outputs (hashes / counts) were asked for; the kernel
+// instance without them carries none of that code.
+template <int CH, bool COV>
 __device__ __forceinline__ void compose_rows(const BatchArgs &B, ComposeLds<CH> &L, const DevPage &pg, int64_t p,
                                              uint32_t at, const SpanView &V, int32_t Y0, int32_t tx0, int32_t xa,
                                              int32_t xb, int32_t ya, int32_t yb) {
@@ -1014,7 +1022,7 @@ __device__ __forceinline__ void compose_rows(const BatchArgs &B, ComposeLds<CH> 
     constexpr int kPitch = ComposeLds<CH>::kPitch, kRowPitch = ComposeLds<CH>::kRowPitch;
     const uint32_t a_in = (at >> 16) & 0xffu, src = (at >> 8) & 0xffu, src4 = src * 0x01010101u;
     const int flavour = A.flavour;
-    const bool want_cov = A.cov_hash || A.cov_count;
+    constexpr bool want_cov = COV;
     uint64_t hsum = 0;
     unsigned long long w_cov = 0;
     const int32_t yy = Y0 + d;
@@ -1174,9 +1182,10 @@ __device__ __forceinline__ void compose_rows(const BatchArgs &B, ComposeLds<CH> 
 // path order, composited into the tile held in LDS.  Two-deep software
 // pipeline: while path j composites, the first span of path j+1 and the
 // header of path j+2 are in flight.
-template <int CH>
+template <int CH, bool COV>
 __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, long long ci, int32_t page, int32_t band,
-                             int32_t tile, unsigned long long *n_paths, unsigned long long *n_tiles) {
+                             int32_t tile, unsigned long long *n_paths, unsigned long long *n_tiles,
+                             unsigned long long *n_filled, uint32_t *n_ivals) {
     const FillArgs &A = B.F;
     const int lane = threadIdx.x;
     constexpr int kRowWords = ComposeLds<CH>::kRowWords, kRowPitch = ComposeLds<CH>::kRowPitch;
@@ -1200,7 +1209,7 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, long long ci
     cur.lb = b0 - 64;
     cur.l1 = b1;
     cur.k = cur.n = 0;
-    if (!(A.cov_hash || A.cov_count) && nrows == kBandRows) {
+    if (!COV && nrows == kBandRows) {
         long long last = -1, lb = b1 - 64;
         int4 e = make_int4(0, 0, 0, 0);
         for (;; lb -= 64) {
@@ -1232,6 +1241,7 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, long long ci
     const bool has0 = bin_next(B, pg, cur, P0);
     if (!has0 && fill_color < 0) return;
     ++*n_tiles;
+    if (fill_color >= 0) ++*n_filled;
     uint8_t *const base = pg.canvas + static_cast<long long>(Y0) * pg.stride + tx0;
     const int32_t stride = pg.stride;
     long long off0 = -1, off1 = -1;
@@ -1270,8 +1280,9 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, long long ci
             has2 = bin_next(B, pg, cur, P2);
             if (has2) hdr_fetch(B, P2.hi, &off2, &n2);
         }
-        compose_rows<CH>(B, L, pg, P0.p, P0.at, V0, Y0, tx0, tx0, txe, Y0, Y0 + nrows - 1);
+        compose_rows<CH, COV>(B, L, pg, P0.p, P0.at, V0, Y0, tx0, tx0, txe, Y0, Y0 + nrows - 1);
         ++*n_paths;
+        *n_ivals += static_cast<uint32_t>(V0.n);
         __syncthreads();
         if (!has1) break;
         P0 = P1;
@@ -1289,13 +1300,14 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, long long ci
     __syncthreads();
 }
 
-template <int CH>
+template <int CH, bool COV>
 __global__ void __launch_bounds__(64) k_compose(BatchArgs B, long long n_items) {
     __shared__ ComposeLds<CH> L;
     if (threadIdx.x < 17) L.gamma[threadIdx.x] = kGamma[threadIdx.x];
     __syncthreads();
     const int lane = threadIdx.x;
-    unsigned long long n_paths = 0, n_tiles = 0;
+    unsigned long long n_paths = 0, n_tiles = 0, n_filled = 0;
+    uint32_t n_ivals = 0;  // intervals this lane read
     SliceCursor sc{static_cast<int32_t>(blockIdx.x % kSlices), 0};
     long long it = 0, it_end = 0;
     for (;;) {
@@ -1314,12 +1326,15 @@ __global__ void __launch_bounds__(64) k_compose(BatchArgs B, long long n_items) 
         const CPage cp = B.cpages[lo];
         const int32_t local = static_cast<int32_t>(it - cp.base);
         const int32_t band = local / cp.tiles, tile = local - band * cp.tiles;
-        compose_tile<CH>(B, L, it, cp.page, band, tile, &n_paths, &n_tiles);
+        compose_tile<CH, COV>(B, L, it, cp.page, band, tile, &n_paths, &n_tiles, &n_filled, &n_ivals);
         ++it;
     }
+    const int32_t ivals = __builtin_amdgcn_readlane(wave_incl_add(static_cast<int32_t>(n_ivals)), 63);
     if (lane == 0) {
         if (n_paths) atomicAdd(B.counters + kCntPathTiles, n_paths);
         if (n_tiles) atomicAdd(B.counters + kCntTiles, n_tiles);
+        if (n_filled) atomicAdd(B.counters + kCntFilledTiles, n_filled);
+        if (ivals) atomicAdd(B.counters + kCntIvalsRead, static_cast<unsigned long long>(static_cast<uint32_t>(ivals)));
     }
 }
 
@@ -1343,12 +1358,20 @@ const void *span_kernel(int k) {
     }
 }
 
-const void *compose_kernel(int ch) {
+const void *compose_kernel(int ch, bool cov) {
+    if (cov) {
+        switch (ch) {
+            case 64: return reinterpret_cast<const void *>(&k_compose<64, true>);
+            case 128: return reinterpret_cast<const void *>(&k_compose<128, true>);
+            case 256: return reinterpret_cast<const void *>(&k_compose<256, true>);
+            default: return reinterpret_cast<const void *>(&k_compose<512, true>);
+        }
+    }
     switch (ch) {
-        case 64: return reinterpret_cast<const void *>(&k_compose<64>);
-        case 128: return reinterpret_cast<const void *>(&k_compose<128>);
-        case 256: return reinterpret_cast<const void *>(&k_compose<256>);
-        default: return reinterpret_cast<const void *>(&k_compose<512>);
+        case 64: return reinterpret_cast<const void *>(&k_compose<64, false>);
+        case 128: return reinterpret_cast<const void *>(&k_compose<128, false>);
+        case 256: return reinterpret_cast<const void *>(&k_compose<256, false>);
+        default: return reinterpret_cast<const void *>(&k_compose<512, false>);
     }
 }
 
@@ -1408,7 +1431,7 @@ int32_t init_batched(rr_hip_ctx *c) {
     for (int v = 0; v < kNumVariants; ++v) {
         int a = 0, b = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, span_kernel(kVariantK[v]), 64, 0) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, compose_kernel(kVariantCH[v]), 64, 0) != hipSuccess)
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, compose_kernel(kVariantCH[v], false), 64, 0) != hipSuccess)
             return fail(c, RR_E_DEVICE, "occupancy query failed");
         c->span_waves[v] = std::max(1, c->n_cu * std::max(1, a));
         c->compose_waves[v] = std::max(1, c->n_cu * std::max(1, b));
@@ -1556,6 +1579,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     int32_t mode = c->size_only ? SPAN_ALL_NOSKIP : SPAN_ALL;
     void *a_args[] = {&B, &mode};
     c->batch_valid = true;
+    c->batch_paths = n_paths;
     c->batch_items = n_items;
     c->batch_work = n_work;
     c->batch_reruns = 0;
@@ -1583,7 +1607,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
             c->prof_b = c->prof_used;
             const unsigned b_waves = static_cast<unsigned>(std::min<long long>(c->compose_waves[variant], n_citems));
             void *b_args[] = {&B, &n_citems};
-            return launch_profiled(c, compose_kernel(chunk_px), b_waves, b_args);
+            return launch_profiled(c, compose_kernel(chunk_px, A.cov_hash || A.cov_count), b_waves, b_args);
         }
         unsigned long long k[kCounters] = {};
         long long n_bin = 0;
@@ -1631,7 +1655,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     // phase B
     const unsigned b_waves = static_cast<unsigned>(std::min<long long>(c->compose_waves[variant], n_citems));
     void *b_args[] = {&B, &n_citems};
-    return launch_profiled(c, compose_kernel(chunk_px), b_waves, b_args);
+    return launch_profiled(c, compose_kernel(chunk_px, A.cov_hash || A.cov_count), b_waves, b_args);
 }
 
 int32_t zero_u64(rr_hip_ctx *c, unsigned long long *p, long long n) { return fill<unsigned long long>(c, p, n, 0ull); }
@@ -1773,7 +1797,32 @@ extern "C" int32_t rr_hip_graph_destroy(rr_hip_ctx *c, rr_hip_graph *g) {
     return RR_OK;
 }
 
-extern "C" int32_t rr_hip_last_batch_stats(rr_hip_ctx *c, double ms[2], int64_t counts[8]) {
+namespace {
+// Paths of the last batched recording with at least one band built (not
+// skipped as occluded), and their edge records.
+__global__ void __launch_bounds__(256) k_built_paths(const SpanPath *__restrict__ spath,
+                                                     const int32_t *__restrict__ pstate, long long n_paths,
+                                                     unsigned long long *__restrict__ out) {
+    unsigned long long paths = 0, edges = 0;
+    for (long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; p < n_paths;
+         p += static_cast<long long>(gridDim.x) * blockDim.x) {
+        if (pstate[p] & 2) {
+            ++paths;
+            edges += static_cast<unsigned long long>(spath[p].e1 - spath[p].e0);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        paths += __shfl_xor(paths, o);
+        edges += __shfl_xor(edges, o);
+    }
+    if ((threadIdx.x & 63) == 0 && (paths || edges)) {
+        atomicAdd(out, paths);
+        atomicAdd(out + 1, edges);
+    }
+}
+}  // namespace
+
+extern "C" int32_t rr_hip_last_batch_stats(rr_hip_ctx *c, double ms[2], int64_t counts[12]) {
     if (!c || !ms || !counts) return RR_E_INVARIANT;
     if (!c->batch_valid) return rrint::fail(c, RR_E_INVARIANT, "no batched recording since the context was created");
     RR_TRY(c, hipSetDevice(c->device));
@@ -1797,6 +1846,22 @@ extern "C" int32_t rr_hip_last_batch_stats(rr_hip_ctx *c, double ms[2], int64_t 
     counts[5] = c->batch_reruns;
     counts[6] = static_cast<int64_t>(k[kCntSkipped]);
     counts[7] = static_cast<int64_t>(k[kCntDeferred]);
+    // paths with a built band and their edge records (a small reduction)
+    unsigned long long built[2] = {0, 0};
+    if (c->batch_paths > 0) {
+        int32_t rc = fill<unsigned long long>(c, c->counters + kCntBuiltPaths, 2, 0ull);
+        if (rc) return rc;
+        const long long blocks = std::min<long long>((c->batch_paths + 255) / 256, 1024);
+        hipLaunchKernelGGL(k_built_paths, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, c->stream, c->spath,
+                           c->pstate, c->batch_paths, c->counters + kCntBuiltPaths);
+        RR_TRY(c, hipGetLastError());
+        RR_TRY(c, hipMemcpyAsync(built, c->counters + kCntBuiltPaths, sizeof(built), hipMemcpyDeviceToHost, c->stream));
+        RR_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    counts[8] = static_cast<int64_t>(built[0]);
+    counts[9] = static_cast<int64_t>(built[1]);
+    counts[10] = static_cast<int64_t>(k[kCntFilledTiles]);
+    counts[11] = static_cast<int64_t>(k[kCntIvalsRead]);
     return RR_OK;
 }
 

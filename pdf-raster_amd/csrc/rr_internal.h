@@ -26,6 +26,9 @@ enum {
     kCntWideServed = 8,    // deferred bands served
     kCntFixupServed = 9,   // fixup items served
     kCntSkipped = 10,      // bands skipped as occluded
+    kCntBuiltPaths = 11,   // statistics: paths with a built band, then their edges (two words)
+    kCntFilledTiles = 13,  // composite tiles started from an opaque cover's colour (not read)
+    kCntIvalsRead = 14,    // intervals the composite pass read
 };
 
 // Sliced work-queue counters of the span and composite passes (8 slices
@@ -144,6 +147,7 @@ struct rr_hip_ctx {
     size_t bin_off_cap = 0;
     int4 *bins = nullptr;  // composite tile bins: {span item, path, full-cover columns, attribute bytes}
     size_t bins_cap = 0;
+    long long batch_paths = 0;  // paths of the last batched recording
     long long batch_bins = 0;  // bin entries of the last batched recording (the sizing pass's while capturing)
     // dispatcher
     float *feat = nullptr;  // per path: kFeatures floats

@@ -395,11 +395,13 @@ class Backend:
 
     def last_batch_stats(self):
         """({span_ms, compose_ms}, {items, intervals, composite_items, path_tiles,
-        tiles, reruns, skipped, deferred}) of the last batched recording."""
+        tiles, reruns, skipped, deferred, built_paths, built_edges, filled_tiles,
+        intervals_read}) of the last batched recording."""
         ms = np.zeros(2, np.float64)
-        cnt = np.zeros(8, np.int64)
+        cnt = np.zeros(12, np.int64)
         self._check(self.L.rr_hip_last_batch_stats(self.h, _p(ms), _p(cnt)), "last_batch_stats")
-        keys = ("items", "intervals", "composite_items", "path_tiles", "tiles", "reruns", "skipped", "deferred")
+        keys = ("items", "intervals", "composite_items", "path_tiles", "tiles", "reruns", "skipped", "deferred",
+                "built_paths", "built_edges", "filled_tiles", "intervals_read")
         return {"span_ms": float(ms[0]), "compose_ms": float(ms[1])}, dict(zip(keys, (int(v) for v in cnt)))
 
     def set_dispatcher(self, params):
