@@ -28,6 +28,29 @@ import rr_hip as R  # noqa: E402
 METRIC = "AA-filled Mpx/s + pages/s @300dpi, batched+learned vs plain HIP; % HBM peak"
 HBM_PEAK_GBS = 8000.0
 
+
+def batched_kernel_bytes(ms, st, ch, n_paths):
+    """Algorithmic bytes and achieved bandwidth of the batched span and
+    composite passes from one recording's statistics."""
+    # span pass, compulsory bytes: the edge records of every path with a built
+    # band read once, the work items and path records read, the intervals
+    # written, a 144-byte header per built band and a 16-byte one per band
+    # skipped as occluded
+    built = st["items"] - st["skipped"]
+    a_bytes = (32 * st["built_edges"] + 16 * st["items"] + 64 * n_paths + 8 * st["intervals"]
+               + 144 * built + 16 * st["skipped"])
+    # composite pass: each touched tile written once and read unless it starts
+    # from an opaque cover's colour, the intervals it reads (per tile), a bin
+    # entry and a header per composited path-tile
+    b_bytes = (16 * ch * (2 * st["tiles"] - st["filled_tiles"]) + 8 * st["intervals_read"]
+               + (16 + 144) * st["path_tiles"])
+    kernels = {}
+    for name, t_ms, nbytes in (("k_spans", ms["span_ms"], a_bytes), ("k_compose", ms["compose_ms"], b_bytes)):
+        gbs = nbytes / (t_ms / 1e3) / 1e9 if t_ms > 0 else 0.0
+        kernels[name] = {"ms": round(t_ms, 3), "alg_bytes": int(nbytes), "achieved_gbs": round(gbs, 3),
+                         "frac": round(gbs / HBM_PEAK_GBS, 6)}
+    return kernels
+
 WORKLOADS = {
     # name: (config id, seed, max pages, primary launch shape, description)
     "mixed-64": (3, 0xC0F10003, 0, R.LAUNCH_BATCHED,
@@ -378,15 +401,7 @@ def bench_dispatch(args, desc):
     step(True, True)
     ms, st = be.last_batch_stats()
     be.set_profiling(False)
-    ch = R.variants()[variant][1]
-    n_edges = dev.n_edges
-    a_bytes = 32 * n_edges + 8 * st["intervals"] + (136 + 8) * st["items"]
-    b_bytes = 2 * 16 * ch * st["tiles"] + 8 * st["intervals"] + (136 + 8 + 16) * st["items"]
-    kernels = {}
-    for name, t_ms, nbytes in (("k_spans", ms["span_ms"], a_bytes), ("k_compose", ms["compose_ms"], b_bytes)):
-        gbs = nbytes / (t_ms / 1e3) / 1e9 if t_ms > 0 else 0.0
-        kernels[name] = {"ms": round(t_ms, 3), "alg_bytes": int(nbytes), "achieved_gbs": round(gbs, 3),
-                         "frac": round(gbs / HBM_PEAK_GBS, 6)}
+    kernels = batched_kernel_bytes(ms, st, R.variants()[variant][1], sum(p.n_paths for p in eval_pages))
     kern = max(("k_spans", "k_compose"), key=lambda k: kernels[k]["ms"])
     out = {
         "metric": METRIC, "value": round(w_ref / learned_sec / 1e6, 3), "unit": "Mpx/s", "n_gpus": 1,
@@ -524,23 +539,7 @@ def main():
     kernels = {}
     if launch == R.LAUNCH_BATCHED:
         ms, st = be.last_batch_stats()
-        ch = R.variants()[variant][1]
-        # span pass, compulsory bytes: the edge records of every path with a
-        # built band read once, the work items and path records read, the
-        # intervals written, a 144-byte header per built band and a 16-byte
-        # one per band skipped as occluded
-        built = st["items"] - st["skipped"]
-        a_bytes = (32 * st["built_edges"] + 16 * st["items"] + 64 * n_paths + 8 * st["intervals"]
-                   + 144 * built + 16 * st["skipped"])
-        # composite pass: each touched tile written once and read unless it
-        # starts from an opaque cover's colour, every interval it composites
-        # read once, a bin entry and a header per composited path-tile
-        b_bytes = (16 * ch * (2 * st["tiles"] - st["filled_tiles"]) + 8 * st["intervals_read"]
-                   + (16 + 144) * st["path_tiles"])
-        for name, t_ms, nbytes in (("k_spans", ms["span_ms"], a_bytes), ("k_compose", ms["compose_ms"], b_bytes)):
-            gbs = nbytes / (t_ms / 1e3) / 1e9 if t_ms > 0 else 0.0
-            kernels[name] = {"ms": round(t_ms, 3), "alg_bytes": int(nbytes), "achieved_gbs": round(gbs, 3),
-                             "frac": round(gbs / HBM_PEAK_GBS, 6)}
+        kernels = batched_kernel_bytes(ms, st, R.variants()[variant][1], n_paths)
         kernels["counts"] = st
         kernel = max(("k_spans", "k_compose"), key=lambda k: kernels[k]["ms"])
         per_launch_bytes = kernels[kernel]["alg_bytes"]
