@@ -41,26 +41,30 @@ __device__ __forceinline__ uint32_t div255(uint32_t x) {
     return s < 255u ? s : 255u;
 }
 
-// Integer simplex interpolation of a grey u16 grid (see rr_hip.h): weights in
-// 1/255 units from t = v*(g-1), axes taken in decreasing fraction order.
+// Integer simplex interpolation of a grey u16 grid: for channel bytes c,
+// t = c*(g-1) splits into a cell index and a fraction in 1/255 units; the axes,
+// taken in decreasing fraction order (equal fractions keep channel order), step
+// from the cell's base node to its far corner.  Returns the D + 1 nodes and
+// their weights (summing to 255); the grey value is
+// (sum of weight * node + 32767) / 65535.
 template <int D>
-__device__ __forceinline__ uint32_t clut_gray(const uint8_t *px, const uint16_t *__restrict__ grid, int32_t g) {
+__device__ __forceinline__ void clut_cell(const uint32_t c[D], int32_t g, int32_t idx[D + 1], uint32_t w[D + 1]) {
     int32_t i0[D], fr[D], ord[D], st[D];
-    st[0] = g * g;  // c / r
-    st[1] = g;      // m / g
-    st[2] = 1;      // y / b
-    if constexpr (D == 4) st[3] = g * g * g;  // k, outermost
+    st[0] = g * g;
+    st[1] = g;
+    st[2] = 1;
+    if constexpr (D == 4) st[3] = g * g * g;
     int32_t base = 0;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-        const int32_t t = static_cast<int32_t>(px[a]) * (g - 1);
+        const int32_t t = static_cast<int32_t>(c[a]) * (g - 1);
         i0[a] = t / 255;
         fr[a] = t - 255 * i0[a];
         base += i0[a] * st[a];
         ord[a] = a;
     }
 #pragma unroll
-    for (int a = 1; a < D; ++a) {  // stable: equal fractions keep channel order
+    for (int a = 1; a < D; ++a) {
 #pragma unroll
         for (int b = a; b > 0; --b) {
             if (fr[ord[b - 1]] < fr[ord[b]]) {
@@ -70,18 +74,22 @@ __device__ __forceinline__ uint32_t clut_gray(const uint8_t *px, const uint16_t 
             }
         }
     }
-    uint32_t acc = static_cast<uint32_t>(255 - fr[ord[0]]) * grid[base];
+    idx[0] = base;
+    w[0] = static_cast<uint32_t>(255 - fr[ord[0]]);
     int32_t cur = base;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         const int a = ord[k];
         if (i0[a] + 1 < g) cur += st[a];
-        const int32_t w = fr[a] - (k + 1 < D ? fr[ord[k + 1]] : 0);
-        acc += static_cast<uint32_t>(w) * grid[cur];
+        idx[k + 1] = cur;
+        w[k + 1] = static_cast<uint32_t>(fr[a] - (k + 1 < D ? fr[ord[k + 1]] : 0));
     }
-    return (acc + 32767u) / 65535u;
 }
 
+// One thread per aligned canvas dword (4 pixels).  The work runs in three
+// phases over the 4 pixels, so that each phase's loads are in flight together:
+// colour bytes and soft-mask bytes of the sampled image pixels, then the
+// simplex grid taps, then the blends (blend_multiply, blend_u8).
 template <int D>
 __global__ void __launch_bounds__(256) k_place_image(ImgArgs A) {
     const int32_t xs = A.bx0 & ~3;
@@ -92,14 +100,12 @@ __global__ void __launch_bounds__(256) k_place_image(ImgArgs A) {
     const double dy_rel = dy_pdf - A.f;
     const double u_row = (-A.c * dy_rel) * A.inv_det;
     const double v_row = (A.a * dy_rel) * A.inv_det;
-    uint32_t src[4], alpha[4];
-    uint32_t live = 0;
+    uint32_t c[4][D], alpha[4];
+    bool in[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int32_t dx = x0 + k;
-        src[k] = 0;
-        alpha[k] = 0;
-        if (dx < A.bx0 || dx >= A.bx1) continue;
+        in[k] = dx >= A.bx0 && dx < A.bx1;
         const double dx_rel = static_cast<double>(dx) - A.e;
         double u = fma(A.d * dx_rel, A.inv_det, u_row);
         double v = fma((-A.b) * dx_rel, A.inv_det, v_row);
@@ -107,28 +113,45 @@ __global__ void __launch_bounds__(256) k_place_image(ImgArgs A) {
         v = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
         const int32_t ix = static_cast<int32_t>(fmin(u * A.iw, A.iw - 1.0));
         const int32_t iy = static_cast<int32_t>(fmin((1.0 - v) * A.ih, A.ih - 1.0));
-        const int64_t idx = static_cast<int64_t>(iy) * A.img_w + ix;
-        const uint32_t al = A.smask ? A.smask[idx] : 255u;
-        if (al == 0) continue;
-        alpha[k] = al;
-        src[k] = clut_gray<D>(A.pixels + idx * D, A.grid, A.g);
-        live |= 1u << k;
+        // a pixel outside the bbox reads image pixel 0 and is discarded
+        const int64_t idx = in[k] ? static_cast<int64_t>(iy) * A.img_w + ix : 0;
+        alpha[k] = A.smask ? A.smask[idx] : 255u;
+        if constexpr (D == 4) {
+            const uint32_t q = *reinterpret_cast<const uint32_t *>(A.pixels + idx * 4);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) c[k][a] = (q >> (8 * a)) & 0xffu;
+        } else {
+#pragma unroll
+            for (int a = 0; a < D; ++a) c[k][a] = A.pixels[idx * D + a];
+        }
     }
-    if (!live) return;
+    uint32_t tap[4][D + 1], wt[4][D + 1];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int32_t gi[D + 1];
+        clut_cell<D>(c[k], A.g, gi, wt[k]);
+#pragma unroll
+        for (int v = 0; v <= D; ++v) tap[k][v] = A.grid[gi[v]];
+    }
     uint32_t *dst = reinterpret_cast<uint32_t *>(A.canvas + static_cast<int64_t>(dy) * A.stride + x0);
     const uint32_t old = *dst;
     uint32_t out = old;
+    bool any = false;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if (!(live & (1u << k))) continue;
+        if (!in[k] || alpha[k] == 0) continue;
+        any = true;
+        uint32_t acc = 0;
+#pragma unroll
+        for (int v = 0; v <= D; ++v) acc += wt[k][v] * tap[k][v];
+        uint32_t s = (acc + 32767u) / 65535u;
         const uint32_t dv = (old >> (8 * k)) & 0xffu;
-        uint32_t s = src[k];
         if (A.blend == 1) s = div255(s * dv);  // blend_multiply
         const uint32_t al = alpha[k];
         const uint32_t r = al == 255u ? s : (s * al + dv * (255u - al)) / 255u;  // blend_u8
         out = (out & ~(0xffu << (8 * k))) | (r << (8 * k));
     }
-    *dst = out;
+    if (any) *dst = out;
 }
 
 // icc_cmyk_to_rgb_cpu, CLUT arm (crates/gpu/src/cmyk.rs:531-630): f32
