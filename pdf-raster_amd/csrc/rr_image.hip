@@ -12,12 +12,16 @@
 // RGB, 4-D for CMYK) with integer simplex interpolation, so the kernel and the
 // CPU oracle agree bit for bit.
 //
-// Work decomposition: one launch per image (images of a page are ordered),
-// 256-thread blocks, one thread per 4 consecutive canvas pixels so every
-// canvas access is one aligned dword read-modify-write.
+// Work decomposition: one launch per image (images of a page are ordered) of
+// persistent 1024-thread blocks.  Each block first copies the grid (or, when it
+// is larger than the block's LDS, its first nodes) into LDS, then walks
+// 256 px x 16-row canvas tiles: a wave per 32 px x 8-row block, one thread per
+// 4 consecutive canvas pixels so every canvas access is one aligned dword
+// read-modify-write.  Grid taps read LDS, or global memory past the LDS copy.
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -33,6 +37,8 @@ struct ImgArgs {
     const uint16_t *grid;
     int32_t g, channels, blend, img_w;
     int32_t stride, bx0, bx1, by0, by1;
+    int32_t lds_nodes;       // grid nodes [0, lds_nodes) are read from the LDS copy
+    int32_t tiles_x, n_tiles;  // 256 px x 16-row canvas tiles over the bbox
     double a, b, c, d, e, f, inv_det, page_h, iw, ih;
 };
 
@@ -47,55 +53,65 @@ __device__ __forceinline__ uint32_t div255(uint32_t x) {
 // from the cell's base node to its far corner.  Returns the D + 1 nodes and
 // their weights (summing to 255); the grey value is
 // (sum of weight * node + 32767) / 65535.
+//
+// The order comes from a min/max sorting network over keys fraction * 4 +
+// (3 - axis): distinct keys whose descending order is exactly decreasing
+// fraction with ties in channel order, and whose low two bits name the axis.
 template <int D>
 __device__ __forceinline__ void clut_cell(const uint32_t c[D], int32_t g, int32_t idx[D + 1], uint32_t w[D + 1]) {
-    int32_t i0[D], fr[D], ord[D], st[D];
-    st[0] = g * g;
-    st[1] = g;
-    st[2] = 1;
-    if constexpr (D == 4) st[3] = g * g * g;
+    int32_t st[4] = {g * g, g, 1, g * g * g};  // channel 0 outermost of the first three; K outermost of all
+    int32_t step[D];
+    uint32_t key[D];
     int32_t base = 0;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         const int32_t t = static_cast<int32_t>(c[a]) * (g - 1);
-        i0[a] = t / 255;
-        fr[a] = t - 255 * i0[a];
-        base += i0[a] * st[a];
-        ord[a] = a;
+        const int32_t i0 = t / 255;
+        const int32_t fr = t - 255 * i0;
+        base += i0 * st[a];
+        step[a] = i0 + 1 < g ? st[a] : 0;
+        key[a] = static_cast<uint32_t>(fr << 2 | (3 - a));
     }
-#pragma unroll
-    for (int a = 1; a < D; ++a) {
-#pragma unroll
-        for (int b = a; b > 0; --b) {
-            if (fr[ord[b - 1]] < fr[ord[b]]) {
-                const int tmp = ord[b - 1];
-                ord[b - 1] = ord[b];
-                ord[b] = tmp;
-            }
-        }
+    auto cs = [](uint32_t &x, uint32_t &y) {
+        const uint32_t hi = max(x, y);
+        y = min(x, y);
+        x = hi;
+    };
+    if constexpr (D == 4) {
+        cs(key[0], key[1]);
+        cs(key[2], key[3]);
+        cs(key[0], key[2]);
+        cs(key[1], key[3]);
+        cs(key[1], key[2]);
+    } else {
+        cs(key[0], key[1]);
+        cs(key[1], key[2]);
+        cs(key[0], key[1]);
     }
     idx[0] = base;
-    w[0] = static_cast<uint32_t>(255 - fr[ord[0]]);
+    w[0] = 255u - (key[0] >> 2);
     int32_t cur = base;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-        const int a = ord[k];
-        if (i0[a] + 1 < g) cur += st[a];
+        const uint32_t sel = key[k] & 3u;  // 3 - axis
+        int32_t s = sel == 3u ? step[0] : (sel == 2u ? step[1] : step[2]);
+        if constexpr (D == 4) s = sel == 0u ? step[3] : s;
+        cur += s;
         idx[k + 1] = cur;
-        w[k + 1] = static_cast<uint32_t>(fr[a] - (k + 1 < D ? fr[ord[k + 1]] : 0));
+        w[k + 1] = (key[k] >> 2) - (k + 1 < D ? key[k + 1] >> 2 : 0u);
     }
 }
+
+constexpr int kImgThreads = 1024;  // 16 waves, each on a 32 px x 8-row block
+constexpr int kImgTileW = 256;     // tile: 8 x 2 wave blocks
+constexpr int kImgTileH = 16;
 
 // One thread per aligned canvas dword (4 pixels).  The work runs in three
 // phases over the 4 pixels, so that each phase's loads are in flight together:
 // colour bytes and soft-mask bytes of the sampled image pixels, then the
 // simplex grid taps, then the blends (blend_multiply, blend_u8).
 template <int D>
-__global__ void __launch_bounds__(256) k_place_image(ImgArgs A) {
-    const int32_t xs = A.bx0 & ~3;
-    const int32_t x0 = xs + 4 * (static_cast<int32_t>(blockIdx.x) * 64 + static_cast<int32_t>(threadIdx.x & 63));
-    const int32_t dy = A.by0 + static_cast<int32_t>(blockIdx.y) * 4 + static_cast<int32_t>(threadIdx.x >> 6);
-    if (dy >= A.by1 || x0 >= A.bx1) return;
+__device__ __forceinline__ void place_dword(const ImgArgs &A, const uint16_t *lgrid, int32_t x0, int32_t dy) {
     const double dy_pdf = A.page_h - static_cast<double>(dy);
     const double dy_rel = dy_pdf - A.f;
     const double u_row = (-A.c * dy_rel) * A.inv_det;
@@ -131,7 +147,14 @@ __global__ void __launch_bounds__(256) k_place_image(ImgArgs A) {
         int32_t gi[D + 1];
         clut_cell<D>(c[k], A.g, gi, wt[k]);
 #pragma unroll
-        for (int v = 0; v <= D; ++v) tap[k][v] = A.grid[gi[v]];
+        for (int v = 0; v <= D; ++v) {
+            // an unconditional LDS read (node 0 stands in past the copy) and a
+            // separate global read, so the compiler cannot merge the two into
+            // one generic-pointer load
+            const bool far = gi[v] >= A.lds_nodes;
+            tap[k][v] = lgrid[far ? 0 : gi[v]];
+            if (far) tap[k][v] = A.grid[gi[v]];
+        }
     }
     uint32_t *dst = reinterpret_cast<uint32_t *>(A.canvas + static_cast<int64_t>(dy) * A.stride + x0);
     const uint32_t old = *dst;
@@ -152,6 +175,37 @@ __global__ void __launch_bounds__(256) k_place_image(ImgArgs A) {
         out = (out & ~(0xffu << (8 * k))) | (r << (8 * k));
     }
     if (any) *dst = out;
+}
+
+// Persistent blocks: copy grid nodes [0, lds_nodes) into LDS (16-byte loads
+// when the grid is 16-byte aligned), then place tiles blockIdx.x,
+// blockIdx.x + gridDim.x, ...
+template <int D>
+__global__ void __launch_bounds__(kImgThreads) k_place_image(ImgArgs A) {
+    extern __shared__ uint4 lds_raw[];
+    uint16_t *lgrid = reinterpret_cast<uint16_t *>(lds_raw);
+    const int32_t tid = static_cast<int32_t>(threadIdx.x);
+    int32_t done = 0;
+    if ((reinterpret_cast<uintptr_t>(A.grid) & 15u) == 0) {
+        const int32_t n16 = A.lds_nodes >> 3;
+        const uint4 *src = reinterpret_cast<const uint4 *>(A.grid);
+#pragma unroll 4
+        for (int32_t i = tid; i < n16; i += kImgThreads) lds_raw[i] = src[i];
+        done = n16 << 3;
+    }
+    for (int32_t i = done + tid; i < A.lds_nodes; i += kImgThreads) lgrid[i] = A.grid[i];
+    __syncthreads();
+    // a wave covers a 32 px x 8-row block (8 x 8 lanes), so that under rotation
+    // its image samples fall in a compact patch of few cache lines
+    const int32_t xs = A.bx0 & ~3;
+    const int32_t wave = tid >> 6, lane = tid & 63;
+    const int32_t ox = (wave & 7) * 32 + (lane & 7) * 4, oy = (wave >> 3) * 8 + (lane >> 3);
+    for (int32_t t = static_cast<int32_t>(blockIdx.x); t < A.n_tiles; t += static_cast<int32_t>(gridDim.x)) {
+        const int32_t ty = t / A.tiles_x, tx = t - ty * A.tiles_x;
+        const int32_t x0 = xs + tx * kImgTileW + ox;
+        const int32_t dy = A.by0 + ty * kImgTileH + oy;
+        if (dy < A.by1 && x0 < A.bx1) place_dword<D>(A, lgrid, x0, dy);
+    }
 }
 
 // icc_cmyk_to_rgb_cpu, CLUT arm (crates/gpu/src/cmyk.rs:531-630): f32
@@ -273,6 +327,8 @@ int32_t rr_hip_record_place_images(rr_hip_ctx *c, const rr_image_batch *b) {
     if (b->n_images < 0 || b->n_pages < 0 || (b->n_images > 0 && (!b->images || !b->pages)))
         return rrint::fail(c, RR_E_INVARIANT, "image batch: bad image/page arrays");
     RR_TRY(c, hipSetDevice(rrint::device(c)));
+    int lds_max = 0;
+    RR_TRY(c, hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, rrint::device(c)));
     for (int32_t i = 0; i < b->n_images; ++i) {
         const rr_image_desc &im = b->images[i];
         if (im.page < 0 || im.page >= b->n_pages) return rrint::fail(c, RR_E_INVARIANT, "image: bad page index");
@@ -340,12 +396,28 @@ int32_t rr_hip_record_place_images(rr_hip_ctx *c, const rr_image_batch *b) {
         A.iw = iw;
         A.ih = ih;
         const int32_t xs0 = A.bx0 & ~3;
-        const dim3 grid_dim(static_cast<unsigned>((A.bx1 - xs0 + 255) / 256), static_cast<unsigned>((A.by1 - A.by0 + 3) / 4));
+        A.tiles_x = (A.bx1 - xs0 + kImgTileW - 1) / kImgTileW;
+        A.n_tiles = A.tiles_x * ((A.by1 - A.by0 + kImgTileH - 1) / kImgTileH);
+        // as much of the grid as one block's LDS holds; the rest stays in global memory
+        int64_t nodes = g;
+        for (int k = 1; k < im.channels; ++k) nodes *= g;
+        const int64_t lds_nodes = std::min<int64_t>(nodes, (lds_max / 2) & ~int64_t{7});
+        if (lds_nodes < 8) return rrint::fail(c, RR_E_DEVICE, "image: device has no LDS for the grid");
+        A.lds_nodes = static_cast<int32_t>(lds_nodes);
+        const size_t lds_bytes = static_cast<size_t>(lds_nodes) * 2;
+        const void *kfn = im.channels == 4 ? reinterpret_cast<const void *>(&k_place_image<4>)
+                                           : reinterpret_cast<const void *>(&k_place_image<3>);
+        RR_TRY(c, hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_bytes)));
+        int per_cu = 0;
+        RR_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kImgThreads, lds_bytes));
+        const int64_t blocks = std::min<int64_t>(A.n_tiles, static_cast<int64_t>(c->n_cu) * std::max(1, per_cu));
         const int32_t rc = launch_profiled(c, [&] {
             if (im.channels == 4)
-                hipLaunchKernelGGL(k_place_image<4>, grid_dim, dim3(256), 0, rrint::stream(c), A);
+                hipLaunchKernelGGL(k_place_image<4>, dim3(static_cast<unsigned>(blocks)), dim3(kImgThreads), lds_bytes,
+                                   rrint::stream(c), A);
             else
-                hipLaunchKernelGGL(k_place_image<3>, grid_dim, dim3(256), 0, rrint::stream(c), A);
+                hipLaunchKernelGGL(k_place_image<3>, dim3(static_cast<unsigned>(blocks)), dim3(kImgThreads), lds_bytes,
+                                   rrint::stream(c), A);
         });
         if (rc) return rc;
     }

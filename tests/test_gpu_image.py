@@ -40,8 +40,11 @@ def test_image_workload_pages(backend):
         _compare(g, w, f"page {i}")
 
 
-def _single(be, px, smask, ctm, blend, grid, g, pw, ph, background):
-    """Place one hand-made image on a pw x ph canvas on the device."""
+def _single(be, px, smask, ctm, blend, grid, g, pw, ph, background, grid_shift=0):
+    """Place one hand-made image on a pw x ph canvas on the device; grid_shift
+    nodes of padding in front of the grid move its device address off 16-byte
+    alignment."""
+    grid = np.concatenate([np.zeros(grid_shift, np.uint16), grid])
     iset = R.ImageSet([R.Image(px, smask, px.shape[2], blend, ctm, 0)], 1, pw, ph,
                       grid if px.shape[2] == 4 else np.zeros(16, np.uint16), g if px.shape[2] == 4 else 2,
                       grid if px.shape[2] == 3 else np.zeros(8, np.uint16), g if px.shape[2] == 3 else 2)
@@ -49,8 +52,13 @@ def _single(be, px, smask, ctm, blend, grid, g, pw, ph, background):
     try:
         di.reset(0)
         be.upload(di.canvases[0], np.pad(background, ((0, 0), (0, di.stride - pw))))
+        b = di.batch()
+        if px.shape[2] == 4:
+            b.grid4 += 2 * grid_shift
+        else:
+            b.grid3 += 2 * grid_shift
         be.begin_page()
-        be.record_place_images(di.batch())
+        be.record_place_images(b)
         be.wait_page(be.submit_page())
         return di.read_canvas(0)
     finally:
@@ -97,6 +105,24 @@ def test_random_small_affine(backend, seed):
         want = bg.copy()
         assert O.place_image(want, px, sm, ctm, k % 2, grid, g) >= 0
         _compare(got, want, f"seed {seed} image {k}")
+
+
+@pytest.mark.parametrize("channels,g,shift", [(4, 40, 0), (3, 64, 0), (4, 17, 1), (3, 33, 3)])
+def test_grid_beyond_lds_and_unaligned_grid(backend, channels, g, shift):
+    # 40^4 and 64^3 u16 grids are larger than a block's LDS, so taps past the
+    # LDS copy read global memory; shifted grids take the 2-byte copy loop.
+    rng = np.random.default_rng(70 + g + shift)
+    W, H, pw, ph = 120, 90, 260, 230
+    px = rng.integers(0, 256, (H, W, channels)).astype(np.uint8)
+    sm = rng.integers(0, 256, (H, W)).astype(np.uint8)
+    grid = rng.integers(0, 65536, g ** channels).astype(np.uint16)
+    th, s = np.deg2rad(20.0), 1.7
+    ctm = (s * W * np.cos(th), s * W * np.sin(th), -s * H * np.sin(th), s * H * np.cos(th), 70.0, 15.0)
+    bg = rng.integers(0, 256, (ph, pw)).astype(np.uint8)
+    got = _single(backend, px, sm, ctm, int(channels == 3), grid, g, pw, ph, bg, grid_shift=shift)
+    want = bg.copy()
+    assert O.place_image(want, px, sm, ctm, int(channels == 3), grid, g) >= 0
+    _compare(got, want, f"g {g} shift {shift}")
 
 
 def test_axis_aligned_ctm_is_rejected(backend):
