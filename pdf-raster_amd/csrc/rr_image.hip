@@ -35,9 +35,11 @@ struct ImgArgs {
     const uint8_t *smask;
     uint8_t *canvas;
     const uint16_t *grid;
-    int32_t g, channels, blend, img_w;
+    uint32_t g;
+    int32_t channels, blend, img_w;
     int32_t stride, bx0, bx1, by0, by1;
-    int32_t lds_nodes;       // grid nodes [0, lds_nodes) are read from the LDS copy
+    uint32_t lds_nodes;      // grid nodes [0, lds_nodes) are read from the LDS copy
+    uint32_t last_node;      // g^D - 1
     int32_t tiles_x, n_tiles;  // 256 px x 16-row canvas tiles over the bbox
     double a, b, c, d, e, f, inv_det, page_h, iw, ih;
 };
@@ -47,6 +49,19 @@ __device__ __forceinline__ uint32_t div255(uint32_t x) {
     return s < 255u ? s : 255u;
 }
 
+// 24-bit multiplies (full-rate VALU).  Operands must be below 2^24; the
+// compiler otherwise picks the 32-bit multiply for these products.
+__device__ __forceinline__ uint32_t mul24(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ uint32_t mad24(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 // Integer simplex interpolation of a grey u16 grid: for channel bytes c,
 // t = c*(g-1) splits into a cell index and a fraction in 1/255 units; the axes,
 // taken in decreasing fraction order (equal fractions keep channel order), step
@@ -54,50 +69,62 @@ __device__ __forceinline__ uint32_t div255(uint32_t x) {
 // their weights (summing to 255); the grey value is
 // (sum of weight * node + 32767) / 65535.
 //
-// The order comes from a min/max sorting network over keys fraction * 4 +
-// (3 - axis): distinct keys whose descending order is exactly decreasing
-// fraction with ties in channel order, and whose low two bits name the axis.
+// The order comes from a sorting network over keys fraction * 4 + (3 - axis):
+// distinct keys whose descending order is exactly decreasing fraction with
+// ties in channel order; each key carries its axis' stride along.  An axis at
+// the grid's last node (byte 255) has fraction 0, so it sorts behind every
+// axis with a nonzero fraction and every vertex that steps along it has
+// weight 0: such a vertex's node is clamped into the grid (last_node) instead
+// of holding the step back, which leaves the grey value unchanged.
 template <int D>
-__device__ __forceinline__ void clut_cell(const uint32_t c[D], int32_t g, int32_t idx[D + 1], uint32_t w[D + 1]) {
-    int32_t st[4] = {g * g, g, 1, g * g * g};  // channel 0 outermost of the first three; K outermost of all
-    int32_t step[D];
-    uint32_t key[D];
-    int32_t base = 0;
+__device__ __forceinline__ void clut_cell(const uint32_t c[D], uint32_t g, uint32_t last_node, uint32_t idx[D + 1],
+                                          uint32_t w[D + 1]) {
+    uint32_t key[D], st[D], i0[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-        const int32_t t = static_cast<int32_t>(c[a]) * (g - 1);
-        const int32_t i0 = t / 255;
-        const int32_t fr = t - 255 * i0;
-        base += i0 * st[a];
-        step[a] = i0 + 1 < g ? st[a] : 0;
-        key[a] = static_cast<uint32_t>(fr << 2 | (3 - a));
+        // t < 2^16 and g <= 256, so 24-bit multiplies are exact, and
+        // (t * 32897) >> 23 == t / 255 for every t < 2^16
+        const uint32_t t = mul24(c[a], g - 1u);
+        i0[a] = mul24(t, 32897u) >> 23;
+        const uint32_t fr = t - mul24(i0[a], 255u);
+        key[a] = fr << 2 | static_cast<uint32_t>(3 - a);
     }
-    auto cs = [](uint32_t &x, uint32_t &y) {
-        const uint32_t hi = max(x, y);
-        y = min(x, y);
-        x = hi;
+    st[0] = g * g;  // channel 0 outermost of the first three axes; K outermost of all
+    st[1] = g;
+    st[2] = 1u;
+    if constexpr (D == 4) st[3] = g * g * g;
+    // base node by Horner's rule over (K,) channel 0, 1, 2: every partial
+    // product stays below 2^32 with operands below 2^24
+    uint32_t base = D == 4 ? mad24(i0[D - 1], g, i0[0]) : i0[0];
+    base = mad24(base, g, i0[1]);
+    base = mad24(base, g, i0[2]);
+    auto cs = [&](int x, int y) {
+        const bool sw = key[y] > key[x];
+        const uint32_t hi = max(key[x], key[y]), lo = min(key[x], key[y]);
+        const uint32_t sx = sw ? st[y] : st[x], sy = sw ? st[x] : st[y];
+        key[x] = hi;
+        key[y] = lo;
+        st[x] = sx;
+        st[y] = sy;
     };
     if constexpr (D == 4) {
-        cs(key[0], key[1]);
-        cs(key[2], key[3]);
-        cs(key[0], key[2]);
-        cs(key[1], key[3]);
-        cs(key[1], key[2]);
+        cs(0, 1);
+        cs(2, 3);
+        cs(0, 2);
+        cs(1, 3);
+        cs(1, 2);
     } else {
-        cs(key[0], key[1]);
-        cs(key[1], key[2]);
-        cs(key[0], key[1]);
+        cs(0, 1);
+        cs(1, 2);
+        cs(0, 1);
     }
     idx[0] = base;
     w[0] = 255u - (key[0] >> 2);
-    int32_t cur = base;
+    uint32_t cur = base;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-        const uint32_t sel = key[k] & 3u;  // 3 - axis
-        int32_t s = sel == 3u ? step[0] : (sel == 2u ? step[1] : step[2]);
-        if constexpr (D == 4) s = sel == 0u ? step[3] : s;
-        cur += s;
-        idx[k + 1] = cur;
+        cur += st[k];
+        idx[k + 1] = min(cur, last_node);
         w[k + 1] = (key[k] >> 2) - (k + 1 < D ? key[k + 1] >> 2 : 0u);
     }
 }
@@ -125,8 +152,10 @@ __device__ __forceinline__ void place_dword(const ImgArgs &A, const uint16_t *lg
         const double dx_rel = static_cast<double>(dx) - A.e;
         double u = fma(A.d * dx_rel, A.inv_det, u_row);
         double v = fma((-A.b) * dx_rel, A.inv_det, v_row);
-        u = u < 0.0 ? 0.0 : (u > 1.0 ? 1.0 : u);
-        v = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+        // clamps to [0, 1]; a NaN u or v (from overflowing products) ends as
+        // ix = iw - 1, iy = ih - 1, as it does when the clamp keeps the NaN
+        u = fmax(fmin(u, 1.0), 0.0);
+        v = fmin(fmax(v, 0.0), 1.0);
         const int32_t ix = static_cast<int32_t>(fmin(u * A.iw, A.iw - 1.0));
         const int32_t iy = static_cast<int32_t>(fmin((1.0 - v) * A.ih, A.ih - 1.0));
         // a pixel outside the bbox reads image pixel 0 and is discarded
@@ -137,24 +166,38 @@ __device__ __forceinline__ void place_dword(const ImgArgs &A, const uint16_t *lg
 #pragma unroll
             for (int a = 0; a < 4; ++a) c[k][a] = (q >> (8 * a)) & 0xffu;
         } else {
+            // the 3 bytes as one aligned 16-bit load and one byte load, both
+            // inside the pixel
+            const uintptr_t p = reinterpret_cast<uintptr_t>(A.pixels) + static_cast<uintptr_t>(idx) * 3;
+            const bool odd = p & 1u;
+            const uint32_t h = *reinterpret_cast<const uint16_t *>(odd ? p + 1 : p);
+            const uint32_t b = *reinterpret_cast<const uint8_t *>(odd ? p : p + 2);
+            const uint32_t q = odd ? (b | h << 8) : (h | b << 16);
 #pragma unroll
-            for (int a = 0; a < D; ++a) c[k][a] = A.pixels[idx * D + a];
+            for (int a = 0; a < 3; ++a) c[k][a] = (q >> (8 * a)) & 0xffu;
         }
     }
-    uint32_t tap[4][D + 1], wt[4][D + 1];
+    uint32_t tap[4][D + 1], wt[4][D + 1], gi[4][D + 1];
+    uint32_t gmax = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        int32_t gi[D + 1];
-        clut_cell<D>(c[k], A.g, gi, wt[k]);
+        clut_cell<D>(c[k], A.g, A.last_node, gi[k], wt[k]);
 #pragma unroll
-        for (int v = 0; v <= D; ++v) {
-            // an unconditional LDS read (node 0 stands in past the copy) and a
-            // separate global read, so the compiler cannot merge the two into
-            // one generic-pointer load
-            const bool far = gi[v] >= A.lds_nodes;
-            tap[k][v] = lgrid[far ? 0 : gi[v]];
-            if (far) tap[k][v] = A.grid[gi[v]];
-        }
+        for (int v = 0; v <= D; ++v) gmax = max(gmax, gi[k][v]);
+    }
+    // a lane whose taps all lie in the LDS copy reads them there; any other
+    // lane reads all of its taps from the grid in global memory (one branch,
+    // so each side's loads are in flight together)
+    if (gmax < A.lds_nodes) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int v = 0; v <= D; ++v) tap[k][v] = lgrid[gi[k][v]];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int v = 0; v <= D; ++v) tap[k][v] = A.grid[gi[k][v]];
     }
     uint32_t *dst = reinterpret_cast<uint32_t *>(A.canvas + static_cast<int64_t>(dy) * A.stride + x0);
     const uint32_t old = *dst;
@@ -166,12 +209,14 @@ __device__ __forceinline__ void place_dword(const ImgArgs &A, const uint16_t *lg
         any = true;
         uint32_t acc = 0;
 #pragma unroll
-        for (int v = 0; v <= D; ++v) acc += wt[k][v] * tap[k][v];
-        uint32_t s = (acc + 32767u) / 65535u;
+        for (int v = 0; v <= D; ++v) acc = mad24(wt[k][v], tap[k][v], acc);
+        // x / 65535 == (x + 1 + (x >> 16)) >> 16 for x < 2^24
+        const uint32_t x = acc + 32767u;
+        uint32_t s = (x + 1u + (x >> 16)) >> 16;
         const uint32_t dv = (old >> (8 * k)) & 0xffu;
         if (A.blend == 1) s = div255(s * dv);  // blend_multiply
         const uint32_t al = alpha[k];
-        const uint32_t r = al == 255u ? s : (s * al + dv * (255u - al)) / 255u;  // blend_u8
+        const uint32_t r = al == 255u ? s : mul24(s * al + dv * (255u - al), 32897u) >> 23;  // blend_u8, / 255
         out = (out & ~(0xffu << (8 * k))) | (r << (8 * k));
     }
     if (any) *dst = out;
@@ -193,7 +238,7 @@ __global__ void __launch_bounds__(kImgThreads) k_place_image(ImgArgs A) {
         for (int32_t i = tid; i < n16; i += kImgThreads) lds_raw[i] = src[i];
         done = n16 << 3;
     }
-    for (int32_t i = done + tid; i < A.lds_nodes; i += kImgThreads) lgrid[i] = A.grid[i];
+    for (int32_t i = done + tid; i < static_cast<int32_t>(A.lds_nodes); i += kImgThreads) lgrid[i] = A.grid[i];
     __syncthreads();
     // a wave covers a 32 px x 8-row block (8 x 8 lanes), so that under rotation
     // its image samples fall in a compact patch of few cache lines
@@ -376,7 +421,7 @@ int32_t rr_hip_record_place_images(rr_hip_ctx *c, const rr_image_batch *b) {
         A.smask = im.smask;
         A.canvas = pg.canvas;
         A.grid = grid;
-        A.g = g;
+        A.g = static_cast<uint32_t>(g);
         A.channels = im.channels;
         A.blend = im.blend;
         A.img_w = im.width;
@@ -403,7 +448,8 @@ int32_t rr_hip_record_place_images(rr_hip_ctx *c, const rr_image_batch *b) {
         for (int k = 1; k < im.channels; ++k) nodes *= g;
         const int64_t lds_nodes = std::min<int64_t>(nodes, (lds_max / 2) & ~int64_t{7});
         if (lds_nodes < 8) return rrint::fail(c, RR_E_DEVICE, "image: device has no LDS for the grid");
-        A.lds_nodes = static_cast<int32_t>(lds_nodes);
+        A.lds_nodes = static_cast<uint32_t>(lds_nodes);
+        A.last_node = static_cast<uint32_t>(nodes - 1);
         const size_t lds_bytes = static_cast<size_t>(lds_nodes) * 2;
         const void *kfn = im.channels == 4 ? reinterpret_cast<const void *>(&k_place_image<4>)
                                            : reinterpret_cast<const void *>(&k_place_image<3>);

@@ -125,6 +125,25 @@ def test_grid_beyond_lds_and_unaligned_grid(backend, channels, g, shift):
     _compare(got, want, f"g {g} shift {shift}")
 
 
+@pytest.mark.parametrize("channels", [3, 4])
+def test_grid_corner_colours(backend, channels):
+    # Channel bytes at the grid's first and last nodes (0, 255) and next to
+    # cell boundaries: vertices of weight zero, steps off the grid's last node.
+    rng = np.random.default_rng(80 + channels)
+    W, H, pw, ph = 48, 40, 150, 140
+    vals = np.array([0, 1, 15, 16, 128, 239, 240, 254, 255], np.uint8)
+    px = vals[rng.integers(0, len(vals), (H, W, channels))]
+    g = 17 if channels == 4 else 33
+    grid = rng.integers(0, 65536, g ** channels).astype(np.uint16)
+    th, s = np.deg2rad(12.0), 2.5
+    ctm = (s * W * np.cos(th), s * W * np.sin(th), -s * H * np.sin(th), s * H * np.cos(th), 20.0, 5.0)
+    bg = rng.integers(0, 256, (ph, pw)).astype(np.uint8)
+    got = _single(backend, px, None, ctm, 0, grid, g, pw, ph, bg)
+    want = bg.copy()
+    assert O.place_image(want, px, None, ctm, 0, grid, g) >= 0
+    _compare(got, want, "corner colours")
+
+
 def test_axis_aligned_ctm_is_rejected(backend):
     px = np.zeros((8, 8, 3), np.uint8)
     with pytest.raises(R.RrError) as ei:
