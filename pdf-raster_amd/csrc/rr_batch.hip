@@ -152,7 +152,7 @@ struct BatchArgs {
     long long sitems_cap;
     const long long *work_base;  // n_paths + 1: work items per path, exclusive prefix
     long long shdr_cap;
-    SpanHdr *shdr;
+    SpanHead *shead;  // shdr_cap heads, then shdr_cap count rows (span_cnt)
     int2 *spans;
     long long spans_cap;
     const long long *band_base;
@@ -176,7 +176,11 @@ struct BatchArgs {
 // without skipping (the sizing pass before a capture); or (wide pass only)
 // the skipped items of paths an unbalanced row widened after they were skipped.
 enum { SPAN_ALL = 0, SPAN_ALL_NODRAW = 1, SPAN_ALL_NOSKIP = 2, SPAN_FIXUP = 3 };
-constexpr long long kSkipped = -2;  // SpanHdr::off of an item skipped as occluded
+// Count row of span item it (after the shdr_cap heads).
+__device__ __forceinline__ SpanCnt *span_cnt(const BatchArgs &B, long long it) {
+    return reinterpret_cast<SpanCnt *>(B.shead + B.shdr_cap) + it;
+}
+constexpr long long kSkipped = -2;  // SpanHead::off of an item skipped as occluded
 
 // The dispatcher, in-kernel ahead of the fill: the class of every path (the
 // caller's route, or for RR_ROUTE_AUTO the MLP's smallest predicted span-pass
@@ -350,7 +354,7 @@ constexpr int kMaxTiles = 512;  // a 32767-pixel page at 64-pixel tiles
 template <bool PLACE>
 __global__ void __launch_bounds__(64) k_tile_bins(const DevPage *__restrict__ pages, int32_t n_pages,
                                                   const long long *__restrict__ band_base,
-                                                  const int2 *__restrict__ band_list, const SpanHdr *__restrict__ shdr,
+                                                  const int2 *__restrict__ band_list, const SpanHead *__restrict__ shead,
                                                   long long shdr_cap, int32_t chunk_px, const int32_t *__restrict__ occ,
                                                   const rr_path_attr *__restrict__ attrs,
                                                   int32_t *__restrict__ bin_cnt, const long long *__restrict__ bin_off,
@@ -385,7 +389,7 @@ __global__ void __launch_bounds__(64) k_tile_bins(const DevPage *__restrict__ pa
         if (q < l1) {
             ent = band_list[q];
             if (ent.y >= 0 && ent.y < shdr_cap) {
-                const int4 h = *reinterpret_cast<const int4 *>(shdr + ent.y);
+                const int4 h = *reinterpret_cast<const int4 *>(shead + ent.y);
                 fx = h.z;
                 const long long off = static_cast<long long>((static_cast<uint64_t>(static_cast<uint32_t>(h.y)) << 32) |
                                                              static_cast<uint32_t>(h.x));
@@ -449,7 +453,7 @@ struct ItemPath {
 
 // First 16 bytes of a span header (offset, full-cover and touched columns),
 // one store by lane 0.
-__device__ __forceinline__ void hdr_head(SpanHdr *H, long long off, int32_t fx0, int32_t fx1, int32_t cx0,
+__device__ __forceinline__ void hdr_head(SpanHead *H, long long off, int32_t fx0, int32_t fx1, int32_t cx0,
                                          int32_t cx1) {
     if (threadIdx.x == 0)
         *reinterpret_cast<int4 *>(H) =
@@ -496,7 +500,7 @@ __device__ __forceinline__ int walk_short(int n, Ptr X0, Ptr X1, Ptr C, int32_t 
 // nint, unb: the walk's interval count of this lane's row (in X0/X1 slots)
 // and whether the row is unbalanced.
 template <typename Ptr>
-__device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &P, int32_t Y0, SpanHdr *H,
+__device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &P, int32_t Y0, long long it,
                                             const int32_t *occ_row, Ptr X0, Ptr X1, int nint, bool unb,
                                             bool count_draw, bool occlusion, uint64_t vmask, SpanCursor &cur) {
     const FillArgs &A = B.F;
@@ -599,8 +603,8 @@ __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &
         int2 *dst = B.spans + off + excl;
         for (int k = 0; k < keep; ++k) dst[k] = make_int2(X0[k * 64 + lane], X1[k * 64 + lane]);
     }
-    H->cnt[lane] = fits ? static_cast<uint16_t>(keep) : static_cast<uint16_t>(0);
-    hdr_head(H, fits ? static_cast<long long>(off) : -1ll, fx0, fx1, cx0, cx1);
+    span_cnt(B, it)->cnt[lane] = fits ? static_cast<uint16_t>(keep) : static_cast<uint16_t>(0);
+    hdr_head(B.shead + it, fits ? static_cast<long long>(off) : -1ll, fx0, fx1, cx0, cx1);
     // publish: composite tiles this opaque item covers fully (the same test
     // as the composite pass's occlusion scan)
     if (occlusion && Y0 + kBandRows <= P.height && fx0 <= fx1) {
@@ -614,7 +618,7 @@ __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &
 }
 
 // Header of an item skipped as occluded (its interval counts are never read).
-__device__ __forceinline__ void write_skipped(SpanHdr *H) { hdr_head(H, kSkipped, 1, 0, 1, 0); }
+__device__ __forceinline__ void write_skipped(SpanHead *H) { hdr_head(H, kSkipped, 1, 0, 1, 0); }
 
 // One (path, band).  sm0: the path's first chunk summaries
 // (chunk_summary_first).  check: test the band for occlusion first (else the
@@ -628,7 +632,6 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
                                           uint64_t vmask, int2 sm0, SpanCursor &cur) {
     const FillArgs &A = B.F;
     const int lane = threadIdx.x;
-    SpanHdr *H = B.shdr + it;
     const int32_t band = Y0 / kBandRows;
     const int32_t *occ_row = B.occ + P.occ_base + static_cast<long long>(band) * P.ctiles;
     if (check && P.px0 <= P.px1) {
@@ -642,7 +645,7 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
         for (int32_t t = t0 + 64 + lane; vmask == 0 && t <= t1; t += 64) vmask = __ballot(occ_row[t] <= P.p);
         if (vmask == 0) {
             ++cur.skipped;
-            write_skipped(H);
+            write_skipped(B.shead + it);
             return;
         }
     }
@@ -664,7 +667,7 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
         bool unb = false;
         const int nint = __ballot(n > 2) == 0 ? walk_short(n, L.ex0, L.ex1, L.ec, mrule, P.width * 4, &unb)
                                                : sort_and_walk(n, L.ex0, L.ex1, L.ec, mrule, P.width * 4, &unb);
-        span_finish(B, P, Y0, H, occ_row, L.ex0, L.ex1, nint, unb, count_draw, occlusion, vmask, cur);
+        span_finish(B, P, Y0, it, occ_row, L.ex0, L.ex1, nint, unb, count_draw, occlusion, vmask, cur);
         return;
     }
     if constexpr (DEFER) {
@@ -689,8 +692,8 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
         off = first_lane_u64(off);
         if (static_cast<long long>(off) + static_cast<long long>(need) * 192 > A.spill_cap) {
             if (lane == 0) atomicOr(A.err, 1);
-            hdr_head(H, -1, 1, 0, 1, 0);
-            H->cnt[lane] = 0;
+            hdr_head(B.shead + it, -1, 1, 0, 1, 0);
+            span_cnt(B, it)->cnt[lane] = 0;
             return;
         }
         G = A.spill + off;
@@ -699,7 +702,7 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
     n = build_entries(A, L, P.e0, P.e1, band_lo, band_hi, literal, Y0, valid, r, mrule, G, G1, G2, need, sm0);
     bool unb = false;
     const int nint = sort_and_walk(n, G, G1, G2, mrule, P.width * 4, &unb);
-    span_finish(B, P, Y0, H, occ_row, G, G1, nint, unb, count_draw, occlusion, vmask, cur);
+    span_finish(B, P, Y0, it, occ_row, G, G1, nint, unb, count_draw, occlusion, vmask, cur);
 }
 
 constexpr int kSpanGroup = 16;  // items tested for occlusion together (4 lanes each)
@@ -847,7 +850,7 @@ __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
             const bool was_tested = (tmask >> (4 * k)) & 1;
             if (was_tested && ((smask >> (4 * k)) & 0xf) == 0) {
                 ++cur.skipped;
-                write_skipped(B.shdr + iw);
+                write_skipped(B.shead + iw);
                 continue;
             }
             const ItemPath P = item_path_lanes(gq, ip, 4 * k);
@@ -902,7 +905,7 @@ __global__ void __launch_bounds__(64) k_spans_wide(BatchArgs B, int32_t mode) {
             const ItemPath P = item_path(B, item.x);
             for (int32_t bnd = 0; bnd < item.z; ++bnd) {
                 const long long hi = static_cast<long long>(item.w) + bnd;
-                if (B.shdr[hi].off != kSkipped) continue;
+                if (B.shead[hi].off != kSkipped) continue;
                 span_band<K, false>(B, L, P, item.y + kBandRows * bnd, hi, blockIdx.x, false, false, false, ~0ull,
                                     chunk_summary_first(B.F, P.e0, P.e1), cur);
             }
@@ -977,9 +980,9 @@ struct SpanView {
 };
 
 __device__ __forceinline__ void hdr_fetch(const BatchArgs &B, int32_t hi, long long *off, int32_t *n) {
-    const SpanHdr *H = B.shdr + __builtin_amdgcn_readfirstlane(hi);
-    *off = H->off;
-    *n = H->cnt[threadIdx.x];
+    const int32_t h = __builtin_amdgcn_readfirstlane(hi);
+    *off = B.shead[h].off;
+    *n = span_cnt(B, h)->cnt[threadIdx.x];
 }
 
 __device__ __forceinline__ SpanView span_fetch(const BatchArgs &B, long long off, int32_t n) {
@@ -1538,7 +1541,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     B.sitems_cap = static_cast<long long>(c->sitems_cap);
     B.work_base = c->work_base;
     B.shdr_cap = static_cast<long long>(c->shdr_cap);
-    B.shdr = c->shdr;
+    B.shead = reinterpret_cast<SpanHead *>(c->shdr);
     B.spans = c->spans;
     B.spans_cap = static_cast<long long>(c->spans_cap);
     B.band_base = c->band_base;
@@ -1559,8 +1562,8 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     // the span pass: skipped items and items without intervals are left out)
     auto count_bins = [&]() -> int32_t {
         hipLaunchKernelGGL(k_tile_bins<false>, dim3(static_cast<unsigned>(n_bands)), dim3(64), 0, c->stream, c->pages,
-                           b.n_pages, c->band_base, c->band_list, c->shdr, static_cast<long long>(c->shdr_cap), chunk_px,
-                           c->occ, b.attrs, c->bin_cnt, c->bin_off, c->bins, static_cast<long long>(c->bins_cap),
+                           b.n_pages, c->band_base, c->band_list, B.shead, static_cast<long long>(c->shdr_cap),
+                           chunk_px, c->occ, b.attrs, c->bin_cnt, c->bin_off, c->bins, static_cast<long long>(c->bins_cap),
                            c->err_flag);
         RR_TRY(c, hipGetLastError());
         return scan_exclusive<int32_t, long long>(c, c->bin_cnt, c->bin_off, n_citems, c->bin_off + n_citems,
@@ -1569,8 +1572,8 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     auto place_bins = [&]() -> int32_t {
         B.bins = c->bins;
         hipLaunchKernelGGL(k_tile_bins<true>, dim3(static_cast<unsigned>(n_bands)), dim3(64), 0, c->stream, c->pages,
-                           b.n_pages, c->band_base, c->band_list, c->shdr, static_cast<long long>(c->shdr_cap), chunk_px,
-                           c->occ, b.attrs, c->bin_cnt, c->bin_off, c->bins, static_cast<long long>(c->bins_cap),
+                           b.n_pages, c->band_base, c->band_list, B.shead, static_cast<long long>(c->shdr_cap),
+                           chunk_px, c->occ, b.attrs, c->bin_cnt, c->bin_off, c->bins, static_cast<long long>(c->bins_cap),
                            c->err_flag);
         RR_TRY(c, hipGetLastError());
         return RR_OK;

@@ -47,21 +47,30 @@ struct Fence {
     uint64_t id = 0;
 };
 
-// Spans of one (path, 16-row band) item written by the batched span pass:
-// lane l's disjoint sorted intervals [s, e) (AA x units) are the cnt[l]
-// int2 records starting at off + sum(cnt[0..l)).  off = -1: not stored;
-// off = -2: skipped as occluded (the counts are not written).  fx0..fx1:
-// pixel columns, within the path's drawn columns, that every one of the
-// band's 64 AA rows covers fully; empty (fx0 > fx1) unless the path is opaque,
-// so that the path paints them with exactly its colour.  cx0..cx1: the pixel
-// columns the intervals touch within the drawn columns (empty when none), the
-// item's extent for the composite tile bins.
-struct SpanHdr {
+// Spans of one (path, 16-row band) item written by the batched span pass, in
+// two arrays over the item index: a 16-byte head and a 128-byte row of counts.
+// Lane l's disjoint sorted intervals [s, e) (AA x units) are the cnt[l] int2
+// records starting at off + sum(cnt[0..l)).  off = -1: not stored; off = -2:
+// skipped as occluded (the counts are not written).  fx0..fx1: pixel columns,
+// within the path's drawn columns, that every one of the band's 64 AA rows
+// covers fully; empty (fx0 > fx1) unless the path is opaque, so that the path
+// paints them with exactly its colour.  cx0..cx1: the pixel columns the
+// intervals touch within the drawn columns (empty when none), the item's
+// extent for the composite tile bins, which read the heads alone.
+struct SpanHead {
     long long off;
     int16_t fx0, fx1, cx0, cx1;
+};
+struct SpanCnt {
     uint16_t cnt[64];
 };
-static_assert(sizeof(SpanHdr) == 144, "SpanHdr layout");
+// Sizing unit of the span workspace: a workspace of cap units holds cap
+// heads followed by cap count rows.
+struct SpanSlot {
+    SpanHead h;
+    SpanCnt c;
+};
+static_assert(sizeof(SpanHead) == 16 && sizeof(SpanCnt) == 128 && sizeof(SpanSlot) == 144, "span head layout");
 
 // Everything a span item needs about its path and page, one 64-byte record
 // per path (written by k_dispatch): a group of items is then described by two
@@ -119,7 +128,7 @@ struct rr_hip_ctx {
     size_t scan_tmp_cap = 0;
     int4 *sitems = nullptr;  // span-pass work items (path, first row, bands, first span item)
     size_t sitems_cap = 0;
-    rrdev::SpanHdr *shdr = nullptr;
+    rrdev::SpanSlot *shdr = nullptr;  // heads, then count rows (see SpanSlot)
     size_t shdr_cap = 0;
     int2 *spans = nullptr;
     size_t spans_cap = 0;
