@@ -200,25 +200,24 @@ static __constant__ uint8_t kGamma[17] = {0, 4, 11, 21, 32, 45, 59, 74, 90, 108,
 // ---------------------------------------------------------------------------
 // exact f64 helpers (crates/color/src/convert.rs:221-264; scanner/mod.rs:51-66)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int32_t sat_i32_of_int_valued(double f) {
-    if (f >= 2147483647.0) return INT32_MAX;
-    if (f <= -2147483648.0) return INT32_MIN;
-    return static_cast<int32_t>(f);
+// v_cvt_i32_f64: truncation toward zero, values beyond the int32 range (the
+// infinities included) clamp to INT32_MIN / INT32_MAX, NaN gives 0 -- the
+// saturating f64 -> i32 conversion of the reference's `as i32`, in one
+// instruction.
+__device__ __forceinline__ int32_t cvt_i32_sat(double x) {
+    int32_t r;
+    asm("v_cvt_i32_f64 %0, %1" : "=v"(r) : "v"(x));
+    return r;
 }
 
+// floor, saturated to int32; NaN gives INT32_MIN
 __device__ __forceinline__ int32_t splash_floor(double x) {
-    if (isnan(x)) return INT32_MIN;
-    if (isinf(x)) return x > 0.0 ? INT32_MAX : INT32_MIN;
-    return sat_i32_of_int_valued(floor(x));
+    const int32_t f = cvt_i32_sat(floor(x));
+    return isnan(x) ? INT32_MIN : f;
 }
 
-__device__ __forceinline__ int32_t fp_from_f64(double x) {
-    const double fp = x * 65536.0;
-    if (fp >= 2147483647.0) return INT32_MAX;
-    if (fp <= -2147483648.0) return INT32_MIN;
-    if (isnan(fp)) return 0;
-    return static_cast<int32_t>(fp);  // truncation toward zero
-}
+// 16.16 fixed point, truncated toward zero and saturated; NaN gives 0
+__device__ __forceinline__ int32_t fp_from_f64(double x) { return cvt_i32_sat(x * 65536.0); }
 
 __device__ __forceinline__ uint64_t bits_of(double v) { return __double_as_longlong(v); }
 

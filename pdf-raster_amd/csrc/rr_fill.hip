@@ -139,15 +139,7 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
                     y1 = t;
                     flipped = true;
                 }
-                const double f = dxdy * 65536.0;
-                if (f >= 2147483647.0)
-                    dxdy_fp = INT32_MAX;
-                else if (f <= -2147483648.0)
-                    dxdy_fp = INT32_MIN;
-                else if (isnan(f))
-                    dxdy_fp = 0;
-                else
-                    dxdy_fp = static_cast<int32_t>(round(f));
+                dxdy_fp = cvt_i32_sat(round(dxdy * 65536.0));  // saturated, NaN -> 0
             }
             x0 *= 4.0;
             y0 *= 4.0;

@@ -217,3 +217,35 @@ def test_rows_beyond_the_lds_slots_go_to_the_wide_pass(backend):
             canvases_match(db, pages, oc, f"variant {variant} without coverage outputs")
     finally:
         db.free()
+
+
+def extreme_page():
+    """Edges far outside the int32 and 16.16 ranges, nearly horizontal slopes,
+    and NaN / infinite coordinates (the f64 -> int conversions of the edge
+    setup saturate, and NaN edges are dropped, as in the reference)."""
+    inf, nan = np.inf, np.nan
+    W, H = 96, 80
+    paths = [
+        [(10, 10, 1e12, 40), (1e12, 40, 12, 70), (12, 70, 10, 10)],
+        [(-3e9, 5, 50, 5.5), (50, 5.5, 60, 30), (60, 30, -3e9, 29.5), (-3e9, 29.5, -3e9, 5)],
+        [(5, 50, 90, 50 + 1e-9), (90, 50 + 1e-9, 90, 60), (90, 60, 5, 60 - 1e-9), (5, 60 - 1e-9, 5, 50)],
+        [(20, -1e15, 30, 1e15), (30, 1e15, 40, 1e15), (40, 1e15, 35, -1e15), (35, -1e15, 20, -1e15)],
+        [(60, 60, 80, 62), (80, 62, nan, 70), (nan, 70, 70, 75), (70, 75, inf, 78), (inf, 78, 60, 60),
+         (60, 60, 60, -inf), (65, 65, 66, 66)],
+        [(-1e10, 2, -1e10, 70), (-1e10, 70, 8, 72), (8, 72, 8, 2), (8, 2, -1e10, 2)],
+        [(536870911.75, 1, 2, 3), (2, 3, 3, 9), (3, 9, 536870912.25, 1)],
+    ]
+    edges = np.array([e for p in paths for e in p], np.float64)
+    off = np.cumsum([0] + [len(p) for p in paths]).astype(np.int64)
+    n = len(paths)
+    attrs = np.array([(i % 2, (40 * i) % 256, 255 if i % 3 else 170, 0) for i in range(n)],
+                     dtype=R.PATH_ATTR_DTYPE)
+    rows = np.tile(np.array([[0, H - 1]], np.int32), (n, 1))
+    return R.Page(W, H, edges, off, attrs, rows, np.zeros((0, 2)), np.zeros(0, np.uint8),
+                  np.zeros(n + 1, np.int64))
+
+
+@pytest.mark.parametrize("mode", [R.MODE_DOCUMENTED, R.MODE_LITERAL])
+@pytest.mark.parametrize("launch", [R.LAUNCH_BATCHED, R.LAUNCH_PLAIN])
+def test_extreme_and_non_finite_coordinates(backend, mode, launch):
+    check(backend, [extreme_page()], mode, launch=launch)
