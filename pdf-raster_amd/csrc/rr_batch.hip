@@ -341,16 +341,20 @@ __global__ void __launch_bounds__(64) k_band_lists(const int4 *__restrict__ sel,
 }
 
 // Composite tile bins: for every (band, tile) composite item, the span items
-// of the paths whose stored intervals touch the tile, in path order.  One
-// wave per band sweeps the band's path list (already in path order); each
-// visible item's header gives its column extent.  The counting pass writes
-// the bin sizes, indexed like the composite items (cbase + band * ctiles +
-// tile); the placing pass puts each item at bin_off[bin] + its rank among the
-// list's earlier items in that bin.  Items skipped as occluded, items
-// without intervals and (item, tile) pairs that a later opaque path covers
-// fully (occ[tile] above the path: the composite starts after that path) are
-// left out.
+// of the paths whose stored intervals touch the tile, in path order.  A
+// band's path list (already in path order) is cut into kBinSegs consecutive
+// segments, one wave each, so long lists are swept in parallel; each visible
+// item's head gives its column extent.  The counting pass writes the count of
+// every (bin, segment), indexed (cbase + band * ctiles + tile) * kBinSegs +
+// segment, so that one exclusive scan yields each segment's first position
+// inside its bin, and bin ci spans [bin_off[ci * kBinSegs],
+// bin_off[(ci + 1) * kBinSegs]); the placing pass puts each item at its
+// segment's offset + its rank among the segment's earlier items in that bin.
+// Items skipped as occluded, items without intervals and (item, tile) pairs
+// that a later opaque path covers fully (occ[tile] above the path: the
+// composite starts after that path) are left out.
 constexpr int kMaxTiles = 512;  // a 32767-pixel page at 64-pixel tiles
+constexpr int kBinSegs = 8;
 template <bool PLACE>
 __global__ void __launch_bounds__(64) k_tile_bins(const DevPage *__restrict__ pages, int32_t n_pages,
                                                   const long long *__restrict__ band_base,
@@ -362,7 +366,8 @@ __global__ void __launch_bounds__(64) k_tile_bins(const DevPage *__restrict__ pa
                                                   int32_t *__restrict__ err) {
     __shared__ long long run[kMaxTiles];
     __shared__ int32_t cover[kMaxTiles];
-    const long long gb = blockIdx.x;
+    const long long gb = blockIdx.x / kBinSegs;
+    const int32_t seg = static_cast<int32_t>(blockIdx.x % kBinSegs);
     const int lane = threadIdx.x;
     int32_t lo = 0, hi = n_pages - 1;
     while (lo < hi) {
@@ -377,11 +382,14 @@ __global__ void __launch_bounds__(64) k_tile_bins(const DevPage *__restrict__ pa
     const int32_t T = min(pg.ctiles, kMaxTiles);
     const long long ci0 = static_cast<long long>(pg.cbase) + static_cast<long long>(band) * pg.ctiles;
     for (int32_t t = lane; t < T; t += 64) {
-        run[t] = PLACE ? bin_off[ci0 + t] : 0ll;
+        run[t] = PLACE ? bin_off[(ci0 + t) * kBinSegs + seg] : 0ll;
         cover[t] = occ[ci0 + t];
     }
     __syncthreads();
-    const long long l0 = band_base[gb], l1 = band_base[gb + 1];
+    // this wave's segment of the band's list, in whole 64-entry chunks
+    const long long b0 = band_base[gb], b1 = band_base[gb + 1];
+    const long long seg_len = ((b1 - b0 + kBinSegs - 1) / kBinSegs + 63) & ~63ll;
+    const long long l0 = min(b1, b0 + seg * seg_len), l1 = min(b1, l0 + seg_len);
     for (long long lb = l0; lb < l1; lb += 64) {
         const long long q = lb + lane;
         int2 ent = make_int2(0, 0);
@@ -420,7 +428,7 @@ __global__ void __launch_bounds__(64) k_tile_bins(const DevPage *__restrict__ pa
         }
     }
     if (!PLACE)
-        for (int32_t t = lane; t < T; t += 64) bin_cnt[ci0 + t] = static_cast<int32_t>(run[t]);
+        for (int32_t t = lane; t < T; t += 64) bin_cnt[(ci0 + t) * kBinSegs + seg] = static_cast<int32_t>(run[t]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1192,7 +1200,7 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, long long ci
     const FillArgs &A = B.F;
     const int lane = threadIdx.x;
     constexpr int kRowWords = ComposeLds<CH>::kRowWords, kRowPitch = ComposeLds<CH>::kRowPitch;
-    const long long b0 = B.bin_off[ci], b1 = B.bin_off[ci + 1];
+    const long long b0 = B.bin_off[ci * kBinSegs], b1 = B.bin_off[(ci + 1) * kBinSegs];
     if (b1 <= b0) return;
     const DevPage &pg = A.pages[page];
     const int32_t Y0 = kBandRows * band, tx0 = CH * tile;
@@ -1453,8 +1461,9 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     if (n_paths >= 0x7fffffffll) return fail(c, RR_E_INVARIANT, "batched launch: too many paths");
     long long n_bands = 0;
     for (int32_t i = 0; i < b.n_pages; ++i) n_bands += (b.pages[i].height + kBandRows - 1) / kBandRows;
+    if (n_bands * kBinSegs >= 0x7fffffffll) return fail(c, RR_E_INVARIANT, "batched launch: too many bands");
     int32_t rc = ensure(c, &c->item_base, &c->item_base_cap, static_cast<size_t>(n_paths + 1));
-    const long long scan_n = std::max<long long>(std::max<long long>(n_paths, n_bands), c->batch_citems);
+    const long long scan_n = std::max<long long>(std::max<long long>(n_paths, n_bands), c->batch_citems * kBinSegs);
     if (!rc) rc = ensure(c, &c->scan_tmp, &c->scan_tmp_cap, static_cast<size_t>((scan_n + kScanTile - 1) / kScanTile + 1));
     if (!rc) rc = ensure(c, &c->band_cnt, &c->band_cnt_cap, static_cast<size_t>(n_bands + 2));
     if (!rc) rc = ensure(c, &c->band_run, &c->band_run_cap, static_cast<size_t>(n_bands + 2));
@@ -1464,8 +1473,9 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
 
     long long n_citems = c->batch_citems;  // set with the page table by the caller
     rc = ensure(c, &c->occ, &c->occ_cap, static_cast<size_t>(std::max<long long>(n_citems, 1)));
-    if (!rc) rc = ensure(c, &c->bin_cnt, &c->bin_cnt_cap, static_cast<size_t>(std::max<long long>(n_citems, 1)));
-    if (!rc) rc = ensure(c, &c->bin_off, &c->bin_off_cap, static_cast<size_t>(n_citems + 1));
+    const long long n_binsegs = n_citems * kBinSegs;  // (bin, segment) counts
+    if (!rc) rc = ensure(c, &c->bin_cnt, &c->bin_cnt_cap, static_cast<size_t>(std::max<long long>(n_binsegs, 1)));
+    if (!rc) rc = ensure(c, &c->bin_off, &c->bin_off_cap, static_cast<size_t>(n_binsegs + 1));
     if (rc) return rc;
     if (!c->capturing)
         RR_TRY(c, hipMemcpyAsync(c->cpages, c->host_cpages.data(), sizeof(CPage) * b.n_pages,
@@ -1561,18 +1571,18 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     // tile bins of the composite pass: sizes, then offsets (the sizes follow
     // the span pass: skipped items and items without intervals are left out)
     auto count_bins = [&]() -> int32_t {
-        hipLaunchKernelGGL(k_tile_bins<false>, dim3(static_cast<unsigned>(n_bands)), dim3(64), 0, c->stream, c->pages,
-                           b.n_pages, c->band_base, c->band_list, B.shead, static_cast<long long>(c->shdr_cap),
+        hipLaunchKernelGGL(k_tile_bins<false>, dim3(static_cast<unsigned>(n_bands * kBinSegs)), dim3(64), 0, c->stream,
+                           c->pages, b.n_pages, c->band_base, c->band_list, B.shead, static_cast<long long>(c->shdr_cap),
                            chunk_px, c->occ, b.attrs, c->bin_cnt, c->bin_off, c->bins, static_cast<long long>(c->bins_cap),
                            c->err_flag);
         RR_TRY(c, hipGetLastError());
-        return scan_exclusive<int32_t, long long>(c, c->bin_cnt, c->bin_off, n_citems, c->bin_off + n_citems,
+        return scan_exclusive<int32_t, long long>(c, c->bin_cnt, c->bin_off, n_binsegs, c->bin_off + n_binsegs,
                                                   c->scan_tmp);
     };
     auto place_bins = [&]() -> int32_t {
         B.bins = c->bins;
-        hipLaunchKernelGGL(k_tile_bins<true>, dim3(static_cast<unsigned>(n_bands)), dim3(64), 0, c->stream, c->pages,
-                           b.n_pages, c->band_base, c->band_list, B.shead, static_cast<long long>(c->shdr_cap),
+        hipLaunchKernelGGL(k_tile_bins<true>, dim3(static_cast<unsigned>(n_bands * kBinSegs)), dim3(64), 0, c->stream,
+                           c->pages, b.n_pages, c->band_base, c->band_list, B.shead, static_cast<long long>(c->shdr_cap),
                            chunk_px, c->occ, b.attrs, c->bin_cnt, c->bin_off, c->bins, static_cast<long long>(c->bins_cap),
                            c->err_flag);
         RR_TRY(c, hipGetLastError());
@@ -1615,7 +1625,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
         unsigned long long k[kCounters] = {};
         long long n_bin = 0;
         RR_TRY(c, hipMemcpyAsync(k, c->counters, sizeof(k), hipMemcpyDeviceToHost, c->stream));
-        RR_TRY(c, hipMemcpyAsync(&n_bin, c->bin_off + n_citems, sizeof(n_bin), hipMemcpyDeviceToHost, c->stream));
+        RR_TRY(c, hipMemcpyAsync(&n_bin, c->bin_off + n_binsegs, sizeof(n_bin), hipMemcpyDeviceToHost, c->stream));
         RR_TRY(c, hipStreamSynchronize(c->stream));
         const unsigned long long used = k[kCntPoolTop], deferred = k[kCntDeferred];
         c->batch_bins = n_bin;
