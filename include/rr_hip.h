@@ -200,6 +200,32 @@ int32_t rr_xpath_edges(const double *xy, const uint8_t *flags, int64_t n_pts,
                        const double matrix[6], double flatness, double *out_edges, int64_t cap,
                        int64_t *n_out);
 
+/* ---- device path ingestion (no counterpart in the reference, which runs
+ *      XPath::new on the CPU; the same walk and flatten_curve on the GPU) ----
+ * Every path p is points [pt_off[p], pt_off[p+1]) of xy (2 doubles each, user
+ * space) and flags (PathFlags bytes), transformed by matrix[6p..6p+6) (NULL:
+ * identity) as XPath::new does (crates/raster/src/xpath.rs:162-260,
+ * 421-426; close_subpaths = true, the fill case) and flattened with
+ * flatten_curve (crates/raster/src/path/flatten.rs:109-215).  Writes path p's
+ * edges, bit-identical to rr_xpath_edges, to edges[4*edge_off[p] ..
+ * 4*edge_off[p+1]) -- the layout rr_fill_pages takes.  All pointers are
+ * device memory.  Synchronous; not inside an open page.  *n_edges receives
+ * the total; when it exceeds edge_cap nothing is written to edges and the
+ * call returns RR_E_CAPACITY (edge_off is written either way).  pt_off must
+ * rise from 0 to n_pts (RR_E_INVARIANT otherwise). */
+typedef struct rr_xpath_batch {
+    const double *xy;
+    const uint8_t *flags;
+    const int64_t *pt_off;
+    const double *matrix;
+    int64_t n_paths, n_pts;
+    double flatness;
+    double *edges;
+    int64_t edge_cap;
+    int64_t *edge_off;
+} rr_xpath_batch;
+int32_t rr_hip_xpath_edges(rr_hip_ctx *ctx, const rr_xpath_batch *batch, int64_t *n_edges);
+
 /* Coverage-hash term of one composited pixel (index y*width+x, shape 1..255). */
 uint64_t rr_cov_hash_term(int64_t pixel_index, uint8_t shape);
 

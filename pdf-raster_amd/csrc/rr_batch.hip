@@ -1438,6 +1438,12 @@ int32_t read_back(rr_hip_ctx *c, const T *dev, T *host) {
 
 namespace rrint {
 
+int32_t scan_counts(rr_hip_ctx *c, const int32_t *in, long long *out, long long n) {
+    const int32_t rc = ensure(c, &c->scan_tmp, &c->scan_tmp_cap, static_cast<size_t>((n + kScanTile - 1) / kScanTile + 1));
+    if (rc) return rc;
+    return scan_exclusive<int32_t, long long>(c, in, out, n, out + n, c->scan_tmp);
+}
+
 int32_t init_batched(rr_hip_ctx *c) {
     for (int v = 0; v < kNumVariants; ++v) {
         int a = 0, b = 0;

@@ -808,7 +808,8 @@ int32_t rr_hip_destroy(rr_hip_ctx *c) {
                     c->item_base, c->scan_tmp, c->sitems,     c->shdr,    c->spans,     c->band_cnt,
                     c->band_base, c->band_list, c->cpages,    c->counters, c->occ,      c->pstate,
                     c->feat,      c->route_cls, c->n_work,    c->work_base, c->mlp,       c->spath,
-                    c->ovq,       c->bin_cnt,  c->bin_off,    c->bins,    c->queues,    c->band_run};
+                    c->ovq,       c->bin_cnt,  c->bin_off,    c->bins,    c->queues,    c->band_run,
+                    c->xp_t,      c->xp_role,  c->xp_close,   c->xp_cnt,  c->xp_eoff,   c->xp_err};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->t0) (void)hipEventDestroy(c->t0);

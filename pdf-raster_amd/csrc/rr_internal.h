@@ -126,6 +126,20 @@ struct rr_hip_ctx {
     size_t item_base_cap = 0;
     long long *scan_tmp = nullptr;
     size_t scan_tmp_cap = 0;
+    // device path ingestion (rr_xpath.hip): per point transformed position,
+    // role, closing-line target, edge count and edge offset; an error flag
+    double2 *xp_t = nullptr;
+    size_t xp_t_cap = 0;
+    uint8_t *xp_role = nullptr;
+    size_t xp_role_cap = 0;
+    long long *xp_close = nullptr;
+    size_t xp_close_cap = 0;
+    int32_t *xp_cnt = nullptr;
+    size_t xp_cnt_cap = 0;
+    long long *xp_eoff = nullptr;
+    size_t xp_eoff_cap = 0;
+    int32_t *xp_err = nullptr;
+    size_t xp_err_cap = 0;
     int4 *sitems = nullptr;  // span-pass work items (path, first row, bands, first span item)
     size_t sitems_cap = 0;
     rrdev::SpanSlot *shdr = nullptr;  // heads, then count rows (see SpanSlot)
@@ -245,6 +259,8 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const rrdev::FillA
                        int chunk_px, int variant);
 // Occupancy-derived persistent wave counts of the batched kernels.
 int32_t init_batched(rr_hip_ctx *c);
+// Exclusive scan of n int32 counts into out[0..n), out[n] = total (rr_batch.hip).
+int32_t scan_counts(rr_hip_ctx *c, const int32_t *in, long long *out, long long n);
 // Zero n 64-bit words with a kernel (safe inside graph capture).
 int32_t zero_u64(rr_hip_ctx *c, unsigned long long *p, long long n);
 int32_t fill_u64(rr_hip_ctx *c, unsigned long long *p, long long n, unsigned long long value);

@@ -1,0 +1,307 @@
+// rr_xpath.hip -- device path ingestion: XPath::new with flatten_curve on the
+// GPU, bit-exact to the reference's f64 arithmetic (and to the host twin in
+// rr_host.cpp), producing the add_segment edge list the fill kernels consume.
+//
+// XPath::new (crates/raster/src/xpath.rs:162-260, close_subpaths = true as
+// fill_impl passes it, fill/mod.rs:125) walks a PathBuilder point list
+// (path/mod.rs:70-79 flags) and emits, per subpath, a line per LINE point, the
+// flattened pieces of each curve (flatten_curve, path/flatten.rs:109-215) and
+// a closing line when the walk ends away from the subpath start.  Every
+// segment starts at the point before it, so segments are independent once the
+// walk has assigned each point its role:
+//
+//   k_xpath_walk   one thread per path: transform (xpath.rs:421-426) and role
+//                  of every point, and where a closing line follows;
+//   k_xpath_count  one thread per point: edges it emits (1 per line, the
+//                  leaves of its curve, 1 per closing line);
+//   scan           edge offset of every point;
+//   k_xpath_emit   one thread per point: writes its edges at its offset.
+//
+// flatten_curve subdivides on a linked slot stack of 1024 slots: node [a, b)
+// either is a leaf (b - a == 1, or both control points within flatness of the
+// chord midpoint) and emits its end point, or splits at c = (a + b) / 2 and
+// continues with [a, c) before [c, b).  The slots are dyadic intervals, so the
+// node after a leaf ending at slot a is [a, a + lowbit(a)), and a node's
+// control points are those obtained by splitting the root down to it: a thread
+// keeps only the current node and recomputes a right turn from the root (at
+// most ten splits), instead of the reference's 49 KB slot arrays.  Every value
+// is produced by the same operations as in the reference.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <string>
+
+#include "rr_hip.h"
+#include "rr_internal.h"
+
+namespace {
+
+constexpr uint8_t kFirst = 1, kLast = 2, kCurve = 8;  // path/mod.rs:70-79
+constexpr int32_t kSlots = 1024;                     // types.rs MAX_CURVE_SPLITS
+constexpr uint8_t kRoleNone = 0, kRoleLine = 1, kRoleCurve = 2;
+
+// f64::midpoint for finite operands: (a + b) / 2 unless that overflows or
+// loses the low bits of a subnormal half.
+__host__ __device__ __forceinline__ double midpoint(double a, double b) {
+    const double hi = 8.98846567431157953865e307;   // f64::MAX / 2
+    const double lo = 4.45014771701440227211e-308;  // 2 * f64::MIN_POSITIVE
+    const double aa = fabs(a), ab = fabs(b);
+    if (aa <= hi && ab <= hi) return (a + b) / 2.0;
+    if (aa < lo) return a + b / 2.0;
+    if (ab < lo) return a / 2.0 + b;
+    return a / 2.0 + b / 2.0;
+}
+
+struct Cubic {
+    double x0, y0, x1, y1, x2, y2, x3, y3;
+};
+
+// De Casteljau halves of c (flatten.rs:186-205).
+__host__ __device__ __forceinline__ void split(const Cubic &c, Cubic &l, Cubic &r) {
+    const double lx1 = midpoint(c.x0, c.x1), ly1 = midpoint(c.y0, c.y1);
+    const double hx = midpoint(c.x1, c.x2), hy = midpoint(c.y1, c.y2);
+    const double lx2 = midpoint(lx1, hx), ly2 = midpoint(ly1, hy);
+    const double rx2 = midpoint(c.x2, c.x3), ry2 = midpoint(c.y2, c.y3);
+    const double rx1 = midpoint(hx, rx2), ry1 = midpoint(hy, ry2);
+    const double rx0 = midpoint(lx2, rx1), ry0 = midpoint(ly2, ry1);
+    l = Cubic{c.x0, c.y0, lx1, ly1, lx2, ly2, rx0, ry0};
+    r = Cubic{rx0, ry0, rx1, ry1, rx2, ry2, c.x3, c.y3};
+}
+
+// The leaves of flatten_curve in emission order: leaf(x, y) gets each piece's
+// end point (the start of the first piece is the curve's p0).
+template <typename F>
+__host__ __device__ __forceinline__ void flatten_leaves(const Cubic &root, double flat_sq, F &&leaf) {
+    Cubic c = root;
+    int32_t a = 0, size = kSlots;
+    for (;;) {
+        const double mx = midpoint(c.x0, c.x3), my = midpoint(c.y0, c.y3);
+        const double ax = c.x1 - mx, ay = c.y1 - my, bx = c.x2 - mx, by = c.y2 - my;
+        const double d1 = fma(ax, ax, ay * ay);
+        const double d2 = fma(bx, bx, by * by);
+        if (size == 1 || (d1 <= flat_sq && d2 <= flat_sq)) {
+            leaf(c.x3, c.y3);
+            a += size;
+            if (a >= kSlots) return;
+            size = a & -a;
+            // node [a, a + size): split the root down to it
+            c = root;
+            int32_t lo = 0;
+            for (int32_t s = kSlots; s > size; s >>= 1) {
+                Cubic l, r;
+                split(c, l, r);
+                if (a >= lo + (s >> 1)) {
+                    c = r;
+                    lo += s >> 1;
+                } else {
+                    c = l;
+                }
+            }
+            continue;
+        }
+        Cubic l, r;
+        split(c, l, r);
+        c = l;
+        size >>= 1;
+    }
+}
+
+struct XpathArgs {
+    const double *xy;       // user space, 2 per point
+    const uint8_t *flags;   // 1 per point
+    const int64_t *pt_off;  // n_paths + 1
+    const double *matrix;   // 6 per path, or null: identity
+    int64_t n_paths, n_pts;
+    double flat_sq;
+    double2 *t;          // transformed points
+    uint8_t *role;       // kRole* per point
+    long long *close_to;    // subpath start the closing line after this point goes to, or -1
+    int32_t *cnt;           // edges per point
+    const long long *eoff;  // exclusive scan of cnt (n_pts + 1)
+    double4 *edges;
+    int64_t *edge_off;      // n_paths + 1
+    int32_t *err;           // set when a path's point range is out of order or out of bounds
+};
+
+// The XPath::new walk of one path (xpath.rs:174-258).  A curve never checks
+// the LAST flag, so a subpath ending in a curve runs on into the next point.
+__global__ void __launch_bounds__(256) k_xpath_walk(XpathArgs A) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= A.n_paths) return;
+    const int64_t q0 = A.pt_off[p], q1 = A.pt_off[p + 1];
+    if (q0 < 0 || q1 < q0 || q1 > A.n_pts) {
+        atomicOr(A.err, 1);
+        return;
+    }
+    double m[6] = {1.0, 0.0, 0.0, 1.0, 0.0, 0.0};
+    if (A.matrix)
+        for (int k = 0; k < 6; ++k) m[k] = A.matrix[6 * p + k];
+    for (int64_t i = q0; i < q1; ++i) {
+        const double xi = A.xy[2 * i], yi = A.xy[2 * i + 1];
+        A.t[i] = make_double2(fma(xi, m[0], fma(yi, m[2], m[4])), fma(xi, m[1], fma(yi, m[3], m[5])));
+        A.role[i] = kRoleNone;
+        A.close_to[i] = -1;
+    }
+    int64_t i = q0;
+    while (i < q1) {
+        if (!(A.flags[i] & kFirst)) {
+            ++i;
+            continue;
+        }
+        const int64_t s = i;
+        ++i;
+        while (i < q1) {
+            const uint8_t f = A.flags[i];
+            if (f & kCurve) {
+                if (i + 2 >= q1) break;
+                A.role[i] = kRoleCurve;
+                i += 3;
+            } else {
+                A.role[i] = kRoleLine;
+                ++i;
+                if (f & kLast) break;
+            }
+        }
+        A.close_to[i - 1] = s;  // the walk's current point is the last one consumed
+    }
+}
+
+__device__ __forceinline__ Cubic curve_at(const XpathArgs &A, int64_t i) {
+    const double2 p0 = A.t[i - 1], p1 = A.t[i], p2 = A.t[i + 1], p3 = A.t[i + 2];
+    return Cubic{p0.x, p0.y, p1.x, p1.y, p2.x, p2.y, p3.x, p3.y};
+}
+
+__device__ __forceinline__ bool closes(const XpathArgs &A, int64_t i, double2 *cur, double2 *start) {
+    const long long s = A.close_to[i];
+    if (s < 0) return false;
+    *cur = A.t[i];
+    *start = A.t[s];
+    return fabs(cur->x - start->x) > 1e-10 || fabs(cur->y - start->y) > 1e-10;
+}
+
+__global__ void __launch_bounds__(256) k_xpath_count(XpathArgs A) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= A.n_pts) return;
+    int32_t n = 0;
+    const uint8_t r = A.role[i];
+    if (r == kRoleLine) {
+        n = 1;
+    } else if (r == kRoleCurve) {
+        flatten_leaves(curve_at(A, i), A.flat_sq, [&](double, double) { ++n; });
+    }
+    double2 cur, start;
+    if (closes(A, i, &cur, &start)) ++n;
+    A.cnt[i] = n;
+}
+
+__global__ void __launch_bounds__(256) k_xpath_emit(XpathArgs A) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= A.n_pts) return;
+    int64_t o = A.eoff[i];
+    const uint8_t r = A.role[i];
+    if (r == kRoleLine) {
+        const double2 a = A.t[i - 1], b = A.t[i];
+        A.edges[o++] = make_double4(a.x, a.y, b.x, b.y);
+    } else if (r == kRoleCurve) {
+        const Cubic c = curve_at(A, i);
+        double px = c.x0, py = c.y0;
+        flatten_leaves(c, A.flat_sq, [&](double x, double y) {
+            A.edges[o++] = make_double4(px, py, x, y);
+            px = x;
+            py = y;
+        });
+    }
+    double2 cur, start;
+    if (closes(A, i, &cur, &start)) A.edges[o] = make_double4(cur.x, cur.y, start.x, start.y);
+}
+
+__global__ void __launch_bounds__(256) k_xpath_offsets(XpathArgs A) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p <= A.n_paths) A.edge_off[p] = A.eoff[A.pt_off[p]];
+}
+
+#define RR_TRY(ctx, expr)                                                                                   \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess) return rrint::fail(ctx, RR_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+}  // namespace
+
+extern "C" int32_t rr_hip_xpath_edges(rr_hip_ctx *c, const rr_xpath_batch *b, int64_t *n_edges) {
+    if (!c || !b || !n_edges) return RR_E_INVARIANT;
+    if (rrint::page_open(c)) return rrint::fail(c, RR_E_INVARIANT, "xpath_edges inside an open page");
+    if (b->n_paths < 0 || b->n_pts < 0 || !b->pt_off || !b->edge_off || (b->n_pts > 0 && (!b->xy || !b->flags)) ||
+        b->edge_cap < 0 || (b->edge_cap > 0 && !b->edges) || !(b->flatness >= 0.0))
+        return rrint::fail(c, RR_E_INVARIANT, "xpath_edges: bad batch");
+    if (b->n_pts >= 0x7fffffffll * 256ll || b->n_paths >= 0x7fffffffll * 256ll)
+        return rrint::fail(c, RR_E_INVARIANT, "xpath_edges: batch too large");
+    RR_TRY(c, hipSetDevice(rrint::device(c)));
+    *n_edges = 0;
+    const int64_t n_pts = b->n_pts;
+    int32_t rc = rrint::ensure(c, &c->xp_t, &c->xp_t_cap, static_cast<size_t>(std::max<int64_t>(n_pts, 1)));
+    if (!rc) rc = rrint::ensure(c, &c->xp_role, &c->xp_role_cap, static_cast<size_t>(std::max<int64_t>(n_pts, 1)));
+    if (!rc) rc = rrint::ensure(c, &c->xp_close, &c->xp_close_cap, static_cast<size_t>(std::max<int64_t>(n_pts, 1)));
+    if (!rc) rc = rrint::ensure(c, &c->xp_cnt, &c->xp_cnt_cap, static_cast<size_t>(std::max<int64_t>(n_pts, 1)));
+    if (!rc) rc = rrint::ensure(c, &c->xp_eoff, &c->xp_eoff_cap, static_cast<size_t>(n_pts + 1));
+    if (!rc) rc = rrint::ensure(c, &c->xp_err, &c->xp_err_cap, 1);
+    if (rc) return rc;
+    XpathArgs A;
+    A.xy = b->xy;
+    A.flags = b->flags;
+    A.pt_off = b->pt_off;
+    A.matrix = b->matrix;
+    A.n_paths = b->n_paths;
+    A.n_pts = n_pts;
+    A.flat_sq = b->flatness * b->flatness;
+    A.t = c->xp_t;
+    A.role = c->xp_role;
+    A.close_to = c->xp_close;
+    A.cnt = c->xp_cnt;
+    A.eoff = c->xp_eoff;
+    A.edges = reinterpret_cast<double4 *>(b->edges);
+    A.edge_off = b->edge_off;
+    A.err = c->xp_err;
+    hipStream_t st = rrint::stream(c);
+    // the point ranges must tile [0, n_pts): checked before any point is read
+    // by the per-point kernels
+    int64_t ends[2] = {0, 0};
+    int32_t err = 0;
+    RR_TRY(c, hipMemsetAsync(c->xp_err, 0, sizeof(int32_t), st));
+    RR_TRY(c, hipMemcpyAsync(&ends[0], b->pt_off, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    RR_TRY(c, hipMemcpyAsync(&ends[1], b->pt_off + b->n_paths, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    if (b->n_paths > 0) {
+        hipLaunchKernelGGL(k_xpath_walk, dim3(blocks_for(b->n_paths)), dim3(256), 0, st, A);
+        RR_TRY(c, hipGetLastError());
+    }
+    RR_TRY(c, hipMemcpyAsync(&err, c->xp_err, sizeof(err), hipMemcpyDeviceToHost, st));
+    RR_TRY(c, hipStreamSynchronize(st));
+    if (ends[0] != 0 || ends[1] != n_pts || err)
+        return rrint::fail(c, RR_E_INVARIANT, "xpath_edges: pt_off must rise from 0 to n_pts");
+    if (n_pts > 0) {
+        hipLaunchKernelGGL(k_xpath_count, dim3(blocks_for(n_pts)), dim3(256), 0, st, A);
+        RR_TRY(c, hipGetLastError());
+    }
+    rc = rrint::scan_counts(c, c->xp_cnt, c->xp_eoff, n_pts);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_xpath_offsets, dim3(blocks_for(b->n_paths + 1)), dim3(256), 0, st, A);
+    RR_TRY(c, hipGetLastError());
+    long long total = 0;
+    RR_TRY(c, hipMemcpyAsync(&total, c->xp_eoff + n_pts, sizeof(total), hipMemcpyDeviceToHost, st));
+    RR_TRY(c, hipStreamSynchronize(st));
+    *n_edges = total;
+    if (total > b->edge_cap)
+        return rrint::fail(c, RR_E_CAPACITY, "xpath_edges: edge buffer holds " + std::to_string(b->edge_cap) +
+                                                 " edges, the paths give " + std::to_string(total));
+    if (n_pts > 0) {
+        hipLaunchKernelGGL(k_xpath_emit, dim3(blocks_for(n_pts)), dim3(256), 0, st, A);
+        RR_TRY(c, hipGetLastError());
+    }
+    RR_TRY(c, hipStreamSynchronize(st));
+    return RR_OK;
+}

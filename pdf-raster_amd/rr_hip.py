@@ -17,6 +17,7 @@ RR_OK = 0
 MODE_DOCUMENTED, MODE_LITERAL = 0, 1
 FLAVOUR_GRAY8, FLAVOUR_RGB8_FAST = 0, 1
 LAUNCH_PLAIN, LAUNCH_BATCHED = 0, 1
+RR_OK, RR_E_ZERO_SIZE_ALLOC, RR_E_INVARIANT, RR_E_DEVICE, RR_E_NO_DEVICE, RR_E_CAPACITY = 0, -1, -2, -3, -4, -5
 ROUTE_SPAN, ROUTE_TILE16, ROUTE_TILE32, ROUTE_TILE64, ROUTE_CPU, ROUTE_AUTO = 0, 1, 2, 3, 4, 255
 MLP_FEATURES, MLP_CLASSES, MLP_PARAMS = 12, 5, 5341
 
@@ -72,6 +73,13 @@ class ImageBatch(ctypes.Structure):
                 ("grid3", ctypes.c_void_p), ("grid3_n", ctypes.c_int32)]
 
 
+class XpathBatch(ctypes.Structure):
+    _fields_ = [("xy", ctypes.c_void_p), ("flags", ctypes.c_void_p), ("pt_off", ctypes.c_void_p),
+                ("matrix", ctypes.c_void_p), ("n_paths", ctypes.c_int64), ("n_pts", ctypes.c_int64),
+                ("flatness", ctypes.c_double), ("edges", ctypes.c_void_p), ("edge_cap", ctypes.c_int64),
+                ("edge_off", ctypes.c_void_p)]
+
+
 _lib = None
 
 
@@ -110,6 +118,7 @@ def lib():
                                              ctypes.POINTER(i64)]),
             "rr_xpath_edges": (i32, [P, P, i64, P, ctypes.c_double, P, i64,
                                      ctypes.POINTER(i64)]),
+            "rr_hip_xpath_edges": (i32, [P, ctypes.POINTER(XpathBatch), ctypes.POINTER(i64)]),
             "rr_cov_hash_term": (u64, [i64, ctypes.c_uint8]),
             "rr_workload_create": (i32, [i32, u64, i32, ctypes.POINTER(ctypes.c_void_p)]),
             "rr_workload_destroy": (None, [P]),
@@ -430,6 +439,17 @@ class Backend:
         finally:
             self.free(d_f)
             self.free(d_o)
+
+    def xpath_edges(self, xy, flags, pt_off, n_paths, n_pts, edges, edge_cap, edge_off, matrix=None,
+                    flatness=0.1, check=True):
+        """Device path ingestion (XPath::new + flatten_curve on the GPU) over
+        device buffers; returns (return code, total edges)."""
+        b = XpathBatch(xy, flags, pt_off, matrix, n_paths, n_pts, flatness, edges, edge_cap, edge_off)
+        n = ctypes.c_int64(0)
+        rc = self.L.rr_hip_xpath_edges(self.h, ctypes.byref(b), ctypes.byref(n))
+        if check:
+            self._check(rc, "xpath_edges")
+        return rc, n.value
 
     def train_dispatcher(self, features, targets, steps=4000, batch=64, lr=2e-3, seed=1):
         """Train the MLP on the device; targets (n, 5) log2 microseconds, NaN = invalid.
