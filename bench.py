@@ -74,6 +74,10 @@ WORKLOADS["dispatch"] = (5, 0xC0F10005, 0, None,
                          "MLP trained on span-pass traces of 8 pages, evaluated on 32 held-out pages, "
                          "a quarter of them also carrying 4 CMYK/RGB images each")
 DISPATCH_TRAIN_PAGES, DISPATCH_EVAL_PAGES = 8, 32
+WORKLOADS["ingest-glyphs"] = (6, 0xC0F10005, 8, None,
+                              "device path ingestion (XPath::new + flatten_curve, flatness 0.1) of 8 glyph "
+                              "pages: 48,000 outlines of 1-3 contours of 12-40 quadratic/cubic segments; "
+                              "points resident in HBM, edges written to HBM")
 GPU_ROUTES = (R.ROUTE_SPAN, R.ROUTE_TILE16, R.ROUTE_TILE32, R.ROUTE_TILE64)
 
 
@@ -231,6 +235,110 @@ def bench_images(args, desc):
             "value": round(px / cdt / 1e6, 3), "unit": "Mpx/s", "cores": threads, "kind": "port",
             "sample": f"{n_img} image placements by the C oracle over {threads} threads "
                       f"(thread t cycles images t, t+{threads}, ...), {cdt:.1f} s"}
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_ingest(pages, budget_s, threads):
+    """Time the oracle's XPath::new + flatten_curve: thread t walks paths t,
+    t+threads, ... of the pages, cycling, until the budget is spent."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O  # the checker, timed here as the CPU reference
+
+    paths = [(pg, i) for pg in pages for i in range(pg.n_paths)]
+    done_edges = [0] * threads
+    done_paths = [0] * threads
+
+    def worker(t):
+        t_end = time.perf_counter() + budget_s
+        k = t
+        while time.perf_counter() < t_end:
+            pg, i = paths[k % len(paths)]
+            xy, fl = pg.path_points(i)
+            done_edges[t] += len(O.xpath_edges(xy, fl, flatness=0.1))
+            done_paths[t] += 1
+            k += threads
+
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=worker, args=(t,)) for t in range(threads)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    return sum(done_edges), sum(done_paths), time.perf_counter() - t0
+
+
+def bench_ingest(args, desc):
+    """Device path ingestion.  A step turns every path's PathBuilder points
+    (resident in HBM) into its add_segment edge list in HBM with one
+    rr_hip_xpath_edges call (walk, count, scan, emit, two host syncs); value is
+    edges written per second."""
+    _, seed, max_pages, _, _ = WORKLOADS[args.workload]
+    t_gen = time.perf_counter()
+    pages = R.workload(5, seed, max_pages)
+    n_paths = sum(pg.n_paths for pg in pages)
+    n_pts = sum(len(pg.pt_flags) for pg in pages)
+    n_edges = sum(pg.n_edges for pg in pages)
+    print(f"bench: {args.workload}: {len(pages)} pages, {n_paths} paths, {n_pts} points, {n_edges} edges "
+          f"generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+    pts = np.concatenate([pg.pts for pg in pages]).astype(np.float64)
+    flags = np.concatenate([pg.pt_flags for pg in pages]).astype(np.uint8)
+    off = [np.zeros(1, np.int64)]
+    base = 0
+    for pg in pages:
+        off.append(pg.pt_off[1:] + base)
+        base += len(pg.pt_flags)
+    pt_off = np.concatenate(off).astype(np.int64)
+    want = np.concatenate([pg.edges.reshape(-1, 4) for pg in pages])
+    be = R.Backend(0)
+    d_xy, d_fl, d_off = be.to_device(pts.reshape(-1)), be.to_device(flags), be.to_device(pt_off)
+    d_e, d_eoff = be.alloc(32 * n_edges), be.alloc(8 * (n_paths + 1))
+
+    def step():
+        be.xpath_edges(d_xy, d_fl, d_off, n_paths, n_pts, d_e, n_edges, d_eoff)
+
+    step()
+    got = be.download(d_e, np.zeros((n_edges, 4), np.float64))
+    parity = "match-host" if np.array_equal(got.view(np.uint64), want.view(np.uint64)) else "MISMATCH"
+    for _ in range(args.warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    sec = (time.perf_counter() - t0) / args.steps
+    print(f"bench: ingest {sec * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
+    be.set_profiling(True)
+    step()
+    walk_ms, count_ms, emit_ms = be.last_xpath_stats()
+    be.set_profiling(False)
+    # emit: reads 4 points (16 B) + a flag per point, its role and offset, and
+    # writes 32 B per edge; the dominant kernel
+    emit_bytes = n_pts * (16 + 1 + 1 + 8 + 8) + 32 * n_edges
+    achieved = emit_bytes / (emit_ms / 1e3) / 1e9 if emit_ms > 0 else 0.0
+    out = {
+        "metric": METRIC, "value": round(n_edges / sec / 1e6, 3), "unit": "Medges/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "description": desc, "pages": len(pages), "paths": n_paths,
+                   "points": n_pts, "edges": n_edges},
+        "paths_per_s": round(n_paths / sec, 1), "kernel_ms": {"k_xpath_walk": round(walk_ms, 4),
+                                                               "k_xpath_count": round(count_ms, 4),
+                                                               "k_xpath_emit": round(emit_ms, 4)},
+        "parity": parity,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "k_xpath_emit",
+                     "avg_launch_us": round(emit_ms * 1e3, 3), "alg_bytes_per_launch": emit_bytes},
+    }
+    for b in (d_xy, d_fl, d_off, d_e, d_eoff):
+        be.free(b)
+    be.close()
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        ne, npth, cdt = cpu_baseline_ingest(pages, args.cpu_budget, threads)
+        out["cpu_baseline"] = {
+            "value": round(ne / cdt / 1e6, 3), "unit": "Medges/s", "cores": threads, "kind": "port",
+            "sample": f"{npth} paths ({ne} edges) through the C oracle's XPath::new over {threads} threads, "
+                      f"{cdt:.1f} s"}
     print(json.dumps(out), flush=True)
 
 
@@ -465,6 +573,8 @@ def main():
         return bench_images(args, desc)
     if cfg == 5:
         return bench_dispatch(args, desc)
+    if cfg == 6:
+        return bench_ingest(args, desc)
     if args.pages:
         max_pages = args.pages
     mode = R.MODE_LITERAL if args.mode == "literal" else R.MODE_DOCUMENTED

@@ -225,6 +225,10 @@ typedef struct rr_xpath_batch {
     int64_t *edge_off;
 } rr_xpath_batch;
 int32_t rr_hip_xpath_edges(rr_hip_ctx *ctx, const rr_xpath_batch *batch, int64_t *n_edges);
+/* Kernel milliseconds of the last rr_hip_xpath_edges call (walk, count, emit;
+ * HIP events, filled while per-launch profiling is on, else 0); no
+ * counterpart in the reference. */
+int32_t rr_hip_last_xpath_stats(rr_hip_ctx *ctx, double ms[3]);
 
 /* Coverage-hash term of one composited pixel (index y*width+x, shape 1..255). */
 uint64_t rr_cov_hash_term(int64_t pixel_index, uint8_t shape);

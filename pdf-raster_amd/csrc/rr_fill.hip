@@ -812,6 +812,8 @@ int32_t rr_hip_destroy(rr_hip_ctx *c) {
                     c->xp_t,      c->xp_role,  c->xp_close,   c->xp_cnt,  c->xp_eoff,   c->xp_err};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
+    for (auto e : c->xp_ev)
+        if (e) (void)hipEventDestroy(e);
     if (c->t0) (void)hipEventDestroy(c->t0);
     if (c->t1) (void)hipEventDestroy(c->t1);
     for (auto &f : c->fences)

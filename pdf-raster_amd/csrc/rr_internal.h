@@ -140,6 +140,8 @@ struct rr_hip_ctx {
     size_t xp_eoff_cap = 0;
     int32_t *xp_err = nullptr;
     size_t xp_err_cap = 0;
+    hipEvent_t xp_ev[6] = {};  // walk, count, emit start/stop (profiling on)
+    double xp_ms[3] = {0.0, 0.0, 0.0};
     int4 *sitems = nullptr;  // span-pass work items (path, first row, bands, first span item)
     size_t sitems_cap = 0;
     rrdev::SpanSlot *shdr = nullptr;  // heads, then count rows (see SpanSlot)

@@ -268,6 +268,11 @@ extern "C" int32_t rr_hip_xpath_edges(rr_hip_ctx *c, const rr_xpath_batch *b, in
     A.edge_off = b->edge_off;
     A.err = c->xp_err;
     hipStream_t st = rrint::stream(c);
+    const bool prof = c->profiling;
+    for (int k = 0; prof && k < 6; ++k)
+        if (!c->xp_ev[k]) RR_TRY(c, hipEventCreate(&c->xp_ev[k]));
+    auto mark = [&](int k) -> hipError_t { return prof ? hipEventRecord(c->xp_ev[k], st) : hipSuccess; };
+    c->xp_ms[0] = c->xp_ms[1] = c->xp_ms[2] = 0.0;
     // the point ranges must tile [0, n_pts): checked before any point is read
     // by the per-point kernels
     int64_t ends[2] = {0, 0};
@@ -275,18 +280,22 @@ extern "C" int32_t rr_hip_xpath_edges(rr_hip_ctx *c, const rr_xpath_batch *b, in
     RR_TRY(c, hipMemsetAsync(c->xp_err, 0, sizeof(int32_t), st));
     RR_TRY(c, hipMemcpyAsync(&ends[0], b->pt_off, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     RR_TRY(c, hipMemcpyAsync(&ends[1], b->pt_off + b->n_paths, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    RR_TRY(c, mark(0));
     if (b->n_paths > 0) {
         hipLaunchKernelGGL(k_xpath_walk, dim3(blocks_for(b->n_paths)), dim3(256), 0, st, A);
         RR_TRY(c, hipGetLastError());
     }
+    RR_TRY(c, mark(1));
     RR_TRY(c, hipMemcpyAsync(&err, c->xp_err, sizeof(err), hipMemcpyDeviceToHost, st));
     RR_TRY(c, hipStreamSynchronize(st));
     if (ends[0] != 0 || ends[1] != n_pts || err)
         return rrint::fail(c, RR_E_INVARIANT, "xpath_edges: pt_off must rise from 0 to n_pts");
+    RR_TRY(c, mark(2));
     if (n_pts > 0) {
         hipLaunchKernelGGL(k_xpath_count, dim3(blocks_for(n_pts)), dim3(256), 0, st, A);
         RR_TRY(c, hipGetLastError());
     }
+    RR_TRY(c, mark(3));
     rc = rrint::scan_counts(c, c->xp_cnt, c->xp_eoff, n_pts);
     if (rc) return rc;
     hipLaunchKernelGGL(k_xpath_offsets, dim3(blocks_for(b->n_paths + 1)), dim3(256), 0, st, A);
@@ -298,10 +307,24 @@ extern "C" int32_t rr_hip_xpath_edges(rr_hip_ctx *c, const rr_xpath_batch *b, in
     if (total > b->edge_cap)
         return rrint::fail(c, RR_E_CAPACITY, "xpath_edges: edge buffer holds " + std::to_string(b->edge_cap) +
                                                  " edges, the paths give " + std::to_string(total));
+    RR_TRY(c, mark(4));
     if (n_pts > 0) {
         hipLaunchKernelGGL(k_xpath_emit, dim3(blocks_for(n_pts)), dim3(256), 0, st, A);
         RR_TRY(c, hipGetLastError());
     }
+    RR_TRY(c, mark(5));
     RR_TRY(c, hipStreamSynchronize(st));
+    if (prof)
+        for (int k = 0; k < 3; ++k) {
+            float ms = 0.f;
+            RR_TRY(c, hipEventElapsedTime(&ms, c->xp_ev[2 * k], c->xp_ev[2 * k + 1]));
+            c->xp_ms[k] = ms;
+        }
+    return RR_OK;
+}
+
+extern "C" int32_t rr_hip_last_xpath_stats(rr_hip_ctx *c, double ms[3]) {
+    if (!c || !ms) return RR_E_INVARIANT;
+    for (int k = 0; k < 3; ++k) ms[k] = c->xp_ms[k];
     return RR_OK;
 }

@@ -119,6 +119,7 @@ def lib():
             "rr_xpath_edges": (i32, [P, P, i64, P, ctypes.c_double, P, i64,
                                      ctypes.POINTER(i64)]),
             "rr_hip_xpath_edges": (i32, [P, ctypes.POINTER(XpathBatch), ctypes.POINTER(i64)]),
+            "rr_hip_last_xpath_stats": (i32, [P, P]),
             "rr_cov_hash_term": (u64, [i64, ctypes.c_uint8]),
             "rr_workload_create": (i32, [i32, u64, i32, ctypes.POINTER(ctypes.c_void_p)]),
             "rr_workload_destroy": (None, [P]),
@@ -450,6 +451,12 @@ class Backend:
         if check:
             self._check(rc, "xpath_edges")
         return rc, n.value
+
+    def last_xpath_stats(self):
+        """Kernel ms of the last xpath_edges call: (walk, count, emit)."""
+        ms = np.zeros(3, np.float64)
+        self._check(self.L.rr_hip_last_xpath_stats(self.h, _p(ms)), "last_xpath_stats")
+        return tuple(float(v) for v in ms)
 
     def train_dispatcher(self, features, targets, steps=4000, batch=64, lr=2e-3, seed=1):
         """Train the MLP on the device; targets (n, 5) log2 microseconds, NaN = invalid.
