@@ -138,6 +138,10 @@ struct rr_hip_ctx {
     size_t xp_cnt_cap = 0;
     long long *xp_eoff = nullptr;
     size_t xp_eoff_cap = 0;
+    long long *xp_sidx = nullptr;  // segment index per point
+    size_t xp_sidx_cap = 0;
+    long long *xp_seg = nullptr;   // point per segment
+    size_t xp_seg_cap = 0;
     int32_t *xp_err = nullptr;
     size_t xp_err_cap = 0;
     hipEvent_t xp_ev[6] = {};  // walk, count, emit start/stop (profiling on)

@@ -10,12 +10,16 @@
 // segment starts at the point before it, so segments are independent once the
 // walk has assigned each point its role:
 //
-//   k_xpath_walk   one thread per path: transform (xpath.rs:421-426) and role
-//                  of every point, and where a closing line follows;
-//   k_xpath_count  one thread per point: edges it emits (1 per line, the
-//                  leaves of its curve, 1 per closing line);
-//   scan           edge offset of every point;
-//   k_xpath_emit   one thread per point: writes its edges at its offset.
+//   k_xpath_walk     one thread per path: transform (xpath.rs:421-426) and
+//                    role of every point, and where a closing line follows;
+//   scan, compact    the points that emit edges (lines, curve starts, points a
+//                    closing line follows), in order: the segments;
+//   k_xpath_count    one thread per segment: edges it emits (1 per line, the
+//                    leaves of its curve, 1 per closing line);
+//   scan             edge offset of every segment;
+//   k_xpath_emit     one thread per segment: writes its edges at its offset.
+// Threads go to segments, not points, so the two control points that follow
+// each curve start leave no lane idle.
 //
 // flatten_curve subdivides on a linked slot stack of 1024 slots: node [a, b)
 // either is a leaf (b - a == 1, or both control points within flatness of the
@@ -119,8 +123,10 @@ struct XpathArgs {
     double2 *t;          // transformed points
     uint8_t *role;       // kRole* per point
     long long *close_to;    // subpath start the closing line after this point goes to, or -1
-    int32_t *cnt;           // edges per point
-    const long long *eoff;  // exclusive scan of cnt (n_pts + 1)
+    int32_t *cnt;           // per point: 1 if it emits edges; then per segment: edges it emits
+    const long long *sidx;  // exclusive scan of the point flags (n_pts + 1): segment index, segment count
+    long long *seg;         // segment -> point
+    const long long *eoff;  // exclusive scan of the segment edge counts (n_pts + 1)
     double4 *edges;
     int64_t *edge_off;      // n_paths + 1
     int32_t *err;           // set when a path's point range is out of order or out of bounds
@@ -167,6 +173,12 @@ __global__ void __launch_bounds__(256) k_xpath_walk(XpathArgs A) {
         }
         A.close_to[i - 1] = s;  // the walk's current point is the last one consumed
     }
+    for (int64_t k = q0; k < q1; ++k) A.cnt[k] = A.role[k] != kRoleNone || A.close_to[k] >= 0;
+}
+
+__global__ void __launch_bounds__(256) k_xpath_compact(XpathArgs A) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < A.n_pts && A.cnt[i]) A.seg[A.sidx[i]] = i;
 }
 
 __device__ __forceinline__ Cubic curve_at(const XpathArgs &A, int64_t i) {
@@ -183,8 +195,13 @@ __device__ __forceinline__ bool closes(const XpathArgs &A, int64_t i, double2 *c
 }
 
 __global__ void __launch_bounds__(256) k_xpath_count(XpathArgs A) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= A.n_pts) return;
+    const int64_t sg = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (sg >= A.n_pts) return;
+    if (sg >= A.sidx[A.n_pts]) {
+        A.cnt[sg] = 0;  // past the last segment: adds nothing to the offsets
+        return;
+    }
+    const int64_t i = A.seg[sg];
     int32_t n = 0;
     const uint8_t r = A.role[i];
     if (r == kRoleLine) {
@@ -194,13 +211,14 @@ __global__ void __launch_bounds__(256) k_xpath_count(XpathArgs A) {
     }
     double2 cur, start;
     if (closes(A, i, &cur, &start)) ++n;
-    A.cnt[i] = n;
+    A.cnt[sg] = n;
 }
 
 __global__ void __launch_bounds__(256) k_xpath_emit(XpathArgs A) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= A.n_pts) return;
-    int64_t o = A.eoff[i];
+    const int64_t sg = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (sg >= A.n_pts || sg >= A.sidx[A.n_pts]) return;
+    const int64_t i = A.seg[sg];
+    int64_t o = A.eoff[sg];
     const uint8_t r = A.role[i];
     if (r == kRoleLine) {
         const double2 a = A.t[i - 1], b = A.t[i];
@@ -220,7 +238,7 @@ __global__ void __launch_bounds__(256) k_xpath_emit(XpathArgs A) {
 
 __global__ void __launch_bounds__(256) k_xpath_offsets(XpathArgs A) {
     const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p <= A.n_paths) A.edge_off[p] = A.eoff[A.pt_off[p]];
+    if (p <= A.n_paths) A.edge_off[p] = A.eoff[A.sidx[A.pt_off[p]]];
 }
 
 #define RR_TRY(ctx, expr)                                                                                   \
@@ -249,6 +267,8 @@ extern "C" int32_t rr_hip_xpath_edges(rr_hip_ctx *c, const rr_xpath_batch *b, in
     if (!rc) rc = rrint::ensure(c, &c->xp_close, &c->xp_close_cap, static_cast<size_t>(std::max<int64_t>(n_pts, 1)));
     if (!rc) rc = rrint::ensure(c, &c->xp_cnt, &c->xp_cnt_cap, static_cast<size_t>(std::max<int64_t>(n_pts, 1)));
     if (!rc) rc = rrint::ensure(c, &c->xp_eoff, &c->xp_eoff_cap, static_cast<size_t>(n_pts + 1));
+    if (!rc) rc = rrint::ensure(c, &c->xp_sidx, &c->xp_sidx_cap, static_cast<size_t>(n_pts + 1));
+    if (!rc) rc = rrint::ensure(c, &c->xp_seg, &c->xp_seg_cap, static_cast<size_t>(std::max<int64_t>(n_pts, 1)));
     if (!rc) rc = rrint::ensure(c, &c->xp_err, &c->xp_err_cap, 1);
     if (rc) return rc;
     XpathArgs A;
@@ -264,6 +284,8 @@ extern "C" int32_t rr_hip_xpath_edges(rr_hip_ctx *c, const rr_xpath_batch *b, in
     A.close_to = c->xp_close;
     A.cnt = c->xp_cnt;
     A.eoff = c->xp_eoff;
+    A.sidx = c->xp_sidx;
+    A.seg = c->xp_seg;
     A.edges = reinterpret_cast<double4 *>(b->edges);
     A.edge_off = b->edge_off;
     A.err = c->xp_err;
@@ -290,6 +312,12 @@ extern "C" int32_t rr_hip_xpath_edges(rr_hip_ctx *c, const rr_xpath_batch *b, in
     RR_TRY(c, hipStreamSynchronize(st));
     if (ends[0] != 0 || ends[1] != n_pts || err)
         return rrint::fail(c, RR_E_INVARIANT, "xpath_edges: pt_off must rise from 0 to n_pts");
+    rc = rrint::scan_counts(c, c->xp_cnt, c->xp_sidx, n_pts);
+    if (rc) return rc;
+    if (n_pts > 0) {
+        hipLaunchKernelGGL(k_xpath_compact, dim3(blocks_for(n_pts)), dim3(256), 0, st, A);
+        RR_TRY(c, hipGetLastError());
+    }
     RR_TRY(c, mark(2));
     if (n_pts > 0) {
         hipLaunchKernelGGL(k_xpath_count, dim3(blocks_for(n_pts)), dim3(256), 0, st, A);
