@@ -24,12 +24,14 @@
 // flatten_curve subdivides on a linked slot stack of 1024 slots: node [a, b)
 // either is a leaf (b - a == 1, or both control points within flatness of the
 // chord midpoint) and emits its end point, or splits at c = (a + b) / 2 and
-// continues with [a, c) before [c, b).  The slots are dyadic intervals, so the
-// node after a leaf ending at slot a is [a, a + lowbit(a)), and a node's
-// control points are those obtained by splitting the root down to it: a thread
-// keeps only the current node and recomputes a right turn from the root (at
-// most ten splits), instead of the reference's 49 KB slot arrays.  Every value
-// is produced by the same operations as in the reference.
+// continues with [a, c) before [c, b): a depth-first walk of a binary tree at
+// most ten levels deep.  The slots are dyadic intervals, so the node after a
+// leaf ending at slot a is [a, a + lowbit(a)), and a node's control points are
+// those obtained by splitting the root down to it: a thread keeps the current
+// node and the two newest pending right halves in registers and rebuilds an
+// older one from the root (at most ten splits), instead of the reference's
+// 49 KB slot arrays.  Every value is produced by the same operations as in the
+// reference.
 
 #include <hip/hip_runtime.h>
 
@@ -76,11 +78,20 @@ __host__ __device__ __forceinline__ void split(const Cubic &c, Cubic &l, Cubic &
 }
 
 // The leaves of flatten_curve in emission order: leaf(x, y) gets each piece's
-// end point (the start of the first piece is the curve's p0).
+// end point (the start of the first piece is the curve's p0).  Depth first,
+// left half before right half.  The two newest pending right halves are kept
+// (without their first point, the end point of the last leaf emitted before
+// them); an older one is rebuilt by splitting the root down to it.
+struct Half {
+    double x1, y1, x2, y2, x3, y3;
+};
+
 template <typename F>
 __host__ __device__ __forceinline__ void flatten_leaves(const Cubic &root, double flat_sq, F &&leaf) {
     Cubic c = root;
-    int32_t a = 0, size = kSlots;
+    int32_t a = 0, size = kSlots;  // the current node is slots [a, a + size)
+    int32_t pending = 0, kept = 0;  // pending right halves; how many of the newest are kept
+    Half h0{}, h1{};                // kept: h0 newest
     for (;;) {
         const double mx = midpoint(c.x0, c.x3), my = midpoint(c.y0, c.y3);
         const double ax = c.x1 - mx, ay = c.y1 - my, bx = c.x2 - mx, by = c.y2 - my;
@@ -88,10 +99,17 @@ __host__ __device__ __forceinline__ void flatten_leaves(const Cubic &root, doubl
         const double d2 = fma(bx, bx, by * by);
         if (size == 1 || (d1 <= flat_sq && d2 <= flat_sq)) {
             leaf(c.x3, c.y3);
+            if (pending == 0) return;
+            // the next node is the newest pending right half: [a, a + lowbit(a))
+            --pending;
             a += size;
-            if (a >= kSlots) return;
             size = a & -a;
-            // node [a, a + size): split the root down to it
+            if (kept > 0) {
+                c = Cubic{c.x3, c.y3, h0.x1, h0.y1, h0.x2, h0.y2, h0.x3, h0.y3};
+                h0 = h1;
+                --kept;
+                continue;
+            }
             c = root;
             int32_t lo = 0;
             for (int32_t s = kSlots; s > size; s >>= 1) {
@@ -108,6 +126,10 @@ __host__ __device__ __forceinline__ void flatten_leaves(const Cubic &root, doubl
         }
         Cubic l, r;
         split(c, l, r);
+        h1 = h0;
+        h0 = Half{r.x1, r.y1, r.x2, r.y2, r.x3, r.y3};
+        kept = kept < 2 ? kept + 1 : 2;
+        ++pending;
         c = l;
         size >>= 1;
     }
