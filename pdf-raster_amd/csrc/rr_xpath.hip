@@ -10,7 +10,7 @@
 // segment starts at the point before it, so segments are independent once the
 // walk has assigned each point its role:
 //
-//   k_xpath_walk     one thread per path: transform (xpath.rs:421-426) and
+//   k_xpath_walk     one wave per path: transform (xpath.rs:421-426) and
 //                    role of every point, and where a closing line follows;
 //   scan, compact    the points that emit edges (lines, curve starts, points a
 //                    closing line follows), in order: the segments;
@@ -154,48 +154,112 @@ struct XpathArgs {
     int32_t *err;           // set when a path's point range is out of order or out of bounds
 };
 
-// The XPath::new walk of one path (xpath.rs:174-258).  A curve never checks
-// the LAST flag, so a subpath ending in a curve runs on into the next point.
+// Roles of one 64-point chunk from its masks: lines, curve starts, points a
+// closing line follows, subpath starts; carry = the subpath start in force
+// when the chunk begins.
+__device__ __forceinline__ void put_chunk(const XpathArgs &A, int64_t base, int64_t q1, uint64_t m_line,
+                                          uint64_t m_curve, uint64_t m_close, uint64_t m_start, int64_t carry) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i = base + lane;
+    if (i >= q1) return;
+    const uint8_t r = (m_line >> lane) & 1 ? kRoleLine : ((m_curve >> lane) & 1 ? kRoleCurve : kRoleNone);
+    const bool cl = (m_close >> lane) & 1;
+    const uint64_t upto = m_start & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+    const int64_t target = upto ? base + 63 - __clzll(static_cast<long long>(upto)) : carry;
+    A.role[i] = r;
+    A.close_to[i] = cl ? target : -1;
+    A.cnt[i] = r != kRoleNone || cl;
+}
+
+// The XPath::new walk of one path (xpath.rs:174-258), one wave per path.  The
+// lanes transform a chunk of 64 points (xpath.rs:421-426); the walk itself is
+// a state machine over the chunk's flags run on wave-uniform values (scalar
+// unit), which collects the roles as 64-bit masks.  A curve never checks the
+// LAST flag, so a subpath ending in a curve runs on into the next point; a
+// curve without its two further points ends the subpath at the point before
+// it, which may lie in the previous chunk, so a chunk's roles are stored one
+// chunk late.
 __global__ void __launch_bounds__(256) k_xpath_walk(XpathArgs A) {
-    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
     if (p >= A.n_paths) return;
     const int64_t q0 = A.pt_off[p], q1 = A.pt_off[p + 1];
     if (q0 < 0 || q1 < q0 || q1 > A.n_pts) {
-        atomicOr(A.err, 1);
+        if (lane == 0) atomicOr(A.err, 1);
         return;
     }
     double m[6] = {1.0, 0.0, 0.0, 1.0, 0.0, 0.0};
     if (A.matrix)
         for (int k = 0; k < 6; ++k) m[k] = A.matrix[6 * p + k];
-    for (int64_t i = q0; i < q1; ++i) {
-        const double xi = A.xy[2 * i], yi = A.xy[2 * i + 1];
-        A.t[i] = make_double2(fma(xi, m[0], fma(yi, m[2], m[4])), fma(xi, m[1], fma(yi, m[3], m[5])));
-        A.role[i] = kRoleNone;
-        A.close_to[i] = -1;
-    }
-    int64_t i = q0;
-    while (i < q1) {
-        if (!(A.flags[i] & kFirst)) {
-            ++i;
-            continue;
+    enum { kOut, kIn, kSkip2, kSkip1 };
+    int state = kOut;
+    int64_t s = -1;  // start of the subpath being walked
+    uint64_t p_line = 0, p_curve = 0, p_close = 0, p_start = 0;
+    int64_t p_base = -1, p_carry = -1;
+    for (int64_t base = q0; base < q1; base += 64) {
+        const int64_t i = base + lane;
+        uint32_t f = 0;
+        if (i < q1) {
+            f = A.flags[i];
+            const double xi = A.xy[2 * i], yi = A.xy[2 * i + 1];
+            A.t[i] = make_double2(fma(xi, m[0], fma(yi, m[2], m[4])), fma(xi, m[1], fma(yi, m[3], m[5])));
         }
-        const int64_t s = i;
-        ++i;
-        while (i < q1) {
-            const uint8_t f = A.flags[i];
-            if (f & kCurve) {
-                if (i + 2 >= q1) break;
-                A.role[i] = kRoleCurve;
-                i += 3;
-            } else {
-                A.role[i] = kRoleLine;
-                ++i;
-                if (f & kLast) break;
+        const int nk = static_cast<int>(min<int64_t>(64, q1 - base));
+        const int64_t carry = s;
+        uint64_t m_line = 0, m_curve = 0, m_close = 0, m_start = 0;
+        for (int k = 0; k < nk; ++k) {
+            const uint32_t fk = __builtin_amdgcn_readlane(f, k);
+            const int64_t ik = base + k;
+            for (;;) {
+                if (state == kOut) {
+                    if (fk & kFirst) {
+                        state = kIn;
+                        s = ik;
+                        m_start |= 1ull << k;
+                    }
+                    break;
+                }
+                if (state == kSkip2) {
+                    state = kSkip1;
+                    break;
+                }
+                if (state == kSkip1) {
+                    state = kIn;
+                    break;
+                }
+                if (fk & kCurve) {
+                    if (ik + 2 >= q1) {
+                        // too few points left: the subpath ends at the point
+                        // before, and this point is looked at again outside it
+                        if (k > 0)
+                            m_close |= 1ull << (k - 1);
+                        else
+                            p_close |= 1ull << 63;
+                        state = kOut;
+                        continue;
+                    }
+                    m_curve |= 1ull << k;
+                    state = kSkip2;
+                    break;
+                }
+                m_line |= 1ull << k;
+                if (fk & kLast) {
+                    m_close |= 1ull << k;
+                    state = kOut;
+                }
+                break;
             }
         }
-        A.close_to[i - 1] = s;  // the walk's current point is the last one consumed
+        if (base + 64 >= q1 && state != kOut) m_close |= 1ull << (nk - 1);  // the path ends inside a subpath
+        if (p_base >= 0) put_chunk(A, p_base, q1, p_line, p_curve, p_close, p_start, p_carry);
+        p_line = m_line;
+        p_curve = m_curve;
+        p_close = m_close;
+        p_start = m_start;
+        p_base = base;
+        p_carry = carry;
     }
-    for (int64_t k = q0; k < q1; ++k) A.cnt[k] = A.role[k] != kRoleNone || A.close_to[k] >= 0;
+    if (p_base >= 0) put_chunk(A, p_base, q1, p_line, p_curve, p_close, p_start, p_carry);
 }
 
 __global__ void __launch_bounds__(256) k_xpath_compact(XpathArgs A) {
@@ -326,7 +390,7 @@ extern "C" int32_t rr_hip_xpath_edges(rr_hip_ctx *c, const rr_xpath_batch *b, in
     RR_TRY(c, hipMemcpyAsync(&ends[1], b->pt_off + b->n_paths, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     RR_TRY(c, mark(0));
     if (b->n_paths > 0) {
-        hipLaunchKernelGGL(k_xpath_walk, dim3(blocks_for(b->n_paths)), dim3(256), 0, st, A);
+        hipLaunchKernelGGL(k_xpath_walk, dim3(static_cast<unsigned>((b->n_paths + 3) / 4)), dim3(256), 0, st, A);
         RR_TRY(c, hipGetLastError());
     }
     RR_TRY(c, mark(1));
