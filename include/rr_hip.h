@@ -298,6 +298,15 @@ int32_t rr_hip_record_icc_clut(rr_hip_ctx *ctx, const uint8_t *cmyk, uint8_t *rg
 int32_t rr_hip_record_composite(rr_hip_ctx *ctx, const uint8_t *src, uint8_t *dst, int64_t n_pixels);
 int32_t rr_hip_record_apply_soft_mask(rr_hip_ctx *ctx, uint8_t *pixels, const uint8_t *mask, int64_t n_pixels);
 
+/* Page epilogue rgb_to_gray (crates/pdf_raster/src/render.rs:1306-1324, the
+ * BT.709 conversion raster_pdf applies to its Rgb8 page): gray[y][x] =
+ * (2126 r + 7152 g + 722 b + 5000) / 10000 of the Rgb8 bitmap (rows of
+ * rgb_stride bytes, width * 3 used) into a Gray8 bitmap (rows of gray_stride
+ * bytes), height <= 65535.  Identity on grey pixels: a grey canvas rendered
+ * with RR_FLAVOUR_RGB8_FAST already is raster_pdf's output. */
+int32_t rr_hip_record_rgb_to_gray(rr_hip_ctx *ctx, const uint8_t *rgb, int64_t rgb_stride, uint8_t *gray,
+                                  int64_t gray_stride, int32_t width, int32_t height);
+
 /* HIP-graph capture of a batched recording (no reference counterpart: the
  * batched pattern as one replayable graph).  graph_create runs an eager
  * sizing pass (edge setup, span pass; the canvases are not touched), sizes the

@@ -338,6 +338,46 @@ __global__ void __launch_bounds__(256) k_apply_soft_mask(uint8_t *__restrict__ p
         px[4 * i + 3] = static_cast<uint8_t>((static_cast<uint32_t>(px[4 * i + 3]) * mask[i] + 127u) / 255u);
 }
 
+// rgb_to_gray (crates/pdf_raster/src/render.rs:1306-1324): BT.709 integer
+// weights, (2126 r + 7152 g + 722 b + 5000) / 10000.  One thread per 4 pixels
+// of a row: three dword loads when the row's 12-byte groups are 4-byte aligned
+// (aligned == 1), byte loads otherwise; one dword store when the grey row is.
+__global__ void __launch_bounds__(256) k_rgb_to_gray(const uint8_t *__restrict__ rgb, int64_t rgb_stride,
+                                                     uint8_t *__restrict__ gray, int64_t gray_stride, int32_t width,
+                                                     int32_t height, int32_t aligned) {
+    const int32_t x0 = 4 * (static_cast<int32_t>(blockIdx.x) * 256 + static_cast<int32_t>(threadIdx.x));
+    const int32_t y = static_cast<int32_t>(blockIdx.y);
+    if (x0 >= width || y >= height) return;
+    const uint8_t *src = rgb + y * rgb_stride + 3 * static_cast<int64_t>(x0);
+    uint8_t *dst = gray + y * gray_stride + x0;
+    const int32_t n = min(4, width - x0);
+    uint32_t c[12];
+    if (aligned && n == 4) {
+        const uint32_t *s4 = reinterpret_cast<const uint32_t *>(src);
+        const uint32_t w0 = s4[0], w1 = s4[1], w2 = s4[2];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            c[k] = (w0 >> (8 * k)) & 0xffu;
+            c[4 + k] = (w1 >> (8 * k)) & 0xffu;
+            c[8 + k] = (w2 >> (8 * k)) & 0xffu;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) c[k] = k < 3 * n ? src[k] : 0u;
+    }
+    uint32_t out = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t g = (2126u * c[3 * k] + 7152u * c[3 * k + 1] + 722u * c[3 * k + 2] + 5000u) / 10000u;
+        out |= g << (8 * k);
+    }
+    if (aligned && n == 4) {
+        *reinterpret_cast<uint32_t *>(dst) = out;
+    } else {
+        for (int k = 0; k < n; ++k) dst[k] = static_cast<uint8_t>(out >> (8 * k));
+    }
+}
+
 #define RR_TRY(ctx, expr)                                                                                   \
     do {                                                                                                    \
         hipError_t e_ = (expr);                                                                             \
@@ -499,6 +539,24 @@ int32_t rr_hip_record_composite(rr_hip_ctx *c, const uint8_t *src, uint8_t *dst,
     RR_TRY(c, hipSetDevice(rrint::device(c)));
     return launch_profiled(c, [&] {
         hipLaunchKernelGGL(k_composite_rgba8, dim3(grid_for(n_pixels)), dim3(256), 0, rrint::stream(c), src, dst, n_pixels);
+    });
+}
+
+int32_t rr_hip_record_rgb_to_gray(rr_hip_ctx *c, const uint8_t *rgb, int64_t rgb_stride, uint8_t *gray,
+                                  int64_t gray_stride, int32_t width, int32_t height) {
+    if (!c) return RR_E_INVARIANT;
+    if (!rrint::page_open(c)) return rrint::fail(c, RR_E_INVARIANT, "record outside begin_page/submit_page");
+    if (width < 0 || height < 0 || height > 65535 || rgb_stride < 3ll * width || gray_stride < width ||
+        (width > 0 && height > 0 && (!rgb || !gray)))
+        return rrint::fail(c, RR_E_INVARIANT, "rgb_to_gray: bad bitmap geometry");
+    if (width == 0 || height == 0) return RR_OK;
+    RR_TRY(c, hipSetDevice(rrint::device(c)));
+    const int32_t aligned = ((reinterpret_cast<uintptr_t>(rgb) | reinterpret_cast<uintptr_t>(gray) |
+                              static_cast<uintptr_t>(rgb_stride) | static_cast<uintptr_t>(gray_stride)) & 3u) == 0;
+    const dim3 grid(static_cast<unsigned>((width + 1023) / 1024), static_cast<unsigned>(height));
+    return launch_profiled(c, [&] {
+        hipLaunchKernelGGL(k_rgb_to_gray, grid, dim3(256), 0, rrint::stream(c), rgb, rgb_stride, gray, gray_stride,
+                           width, height, aligned);
     });
 }
 

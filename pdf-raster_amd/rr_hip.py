@@ -130,6 +130,7 @@ def lib():
             "rr_hip_record_icc_clut": (i32, [P, P, P, P, i32, i64]),
             "rr_hip_record_composite": (i32, [P, P, P, i64]),
             "rr_hip_record_apply_soft_mask": (i32, [P, P, P, i64]),
+            "rr_hip_record_rgb_to_gray": (i32, [P, P, i64, P, i64, i32, i32]),
             "rr_hip_record_scan": (i32, [P, P, P, ctypes.c_uint32]),
             "rr_hip_last_batch_stats": (i32, [P, P, P]),
             "rr_hip_set_dispatcher": (i32, [P, P, i32]),
@@ -398,6 +399,10 @@ class Backend:
 
     def record_composite(self, src, dst, n_pixels):
         self._check(self.L.rr_hip_record_composite(self.h, src, dst, n_pixels), "record_composite")
+
+    def record_rgb_to_gray(self, rgb, rgb_stride, gray, gray_stride, width, height):
+        self._check(self.L.rr_hip_record_rgb_to_gray(self.h, rgb, rgb_stride, gray, gray_stride, width, height),
+                    "record_rgb_to_gray")
 
     def record_apply_soft_mask(self, pixels, mask, n_pixels):
         self._check(self.L.rr_hip_record_apply_soft_mask(self.h, pixels, mask, n_pixels),

@@ -223,3 +223,31 @@ def test_elementwise_empty_is_noop(backend):
     _run(backend, lambda: (backend.record_composite(None, None, 0),
                            backend.record_apply_soft_mask(None, None, 0),
                            backend.record_icc_clut(None, None, None, 0, 0)))
+
+
+@pytest.mark.parametrize("width,rgb_pad,gray_pad", [(1024, 0, 0), (1001, 1, 3), (7, 5, 1)])
+def test_rgb_to_gray(backend, width, rgb_pad, gray_pad):
+    # Aligned rows (dword path) and ragged ones (byte path); padding bytes of
+    # the grey bitmap stay untouched; grey pixels map to themselves.
+    rng = np.random.default_rng(90 + width)
+    H = 37
+    rs, gs = 3 * width + rgb_pad, width + gray_pad
+    rgb = rng.integers(0, 256, (H, rs)).astype(np.uint8)
+    v = rng.integers(0, 256, width).astype(np.uint8)
+    rgb[5, :3 * width] = np.repeat(v, 3)  # a grey row
+    d_rgb = backend.to_device(rgb)
+    d_gray = backend.to_device(np.full((H, gs), 77, np.uint8))
+    try:
+        _run(backend, lambda: backend.record_rgb_to_gray(d_rgb, rs, d_gray, gs, width, H))
+        got = backend.download(d_gray, np.zeros((H, gs), np.uint8))
+    finally:
+        backend.free(d_rgb)
+        backend.free(d_gray)
+    L = O.lib()
+    want = np.full((H, gs), 77, np.uint8)
+    for y in range(H):
+        for x in range(width):
+            r, g, b = (int(c) for c in rgb[y, 3 * x:3 * x + 3])
+            want[y, x] = L.rro_rgb_to_gray(r, g, b)
+    assert np.array_equal(got, want), np.argwhere(got != want)[:5]
+    assert np.array_equal(got[5, :width], v)
