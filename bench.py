@@ -109,7 +109,7 @@ def cpu_baseline(pages, w_per_path, budget_s, threads, mode):
     thread t renders page t mod len(pages) path by path, in order, until the
     budget is spent; Mpx/s counts the reference composites of the paths done."""
     sys.path.insert(0, ROOT)
-    from oracle import oracle as O  # the checker, timed here as the CPU reference
+    from oracle import oracle as O  # the oracle, timed here as the CPU reference
 
     base = np.cumsum([0] + [pg.n_paths for pg in pages])
     done_w = [0] * threads
@@ -146,7 +146,7 @@ def cpu_baseline_images(iset, budget_s, threads):
     """Time the oracle's image placement: thread t places images t, t+threads, ...
     onto its own page canvas, cycling, until the budget is spent."""
     sys.path.insert(0, ROOT)
-    from oracle import oracle as O  # the checker, timed here as the CPU reference
+    from oracle import oracle as O  # the oracle, timed here as the CPU reference
 
     done_px = [0] * threads
     done_img = [0] * threads
@@ -242,7 +242,7 @@ def cpu_baseline_ingest(pages, budget_s, threads):
     """Time the oracle's XPath::new + flatten_curve: thread t walks paths t,
     t+threads, ... of the pages, cycling, until the budget is spent."""
     sys.path.insert(0, ROOT)
-    from oracle import oracle as O  # the checker, timed here as the CPU reference
+    from oracle import oracle as O  # the oracle, timed here as the CPU reference
 
     paths = [(pg, i) for pg in pages for i in range(pg.n_paths)]
     done_edges = [0] * threads
@@ -380,7 +380,7 @@ def cpu_path_costs(pages, threads):
     """Reference CPU raster (the C oracle) time of every path, one page per
     thread, divided by the threads used: each path's share of the CPU pool."""
     sys.path.insert(0, ROOT)
-    from oracle import oracle as O  # the checker, timed here as the CPU reference
+    from oracle import oracle as O  # the oracle, timed here as the CPU reference
 
     out = [None] * len(pages)
 

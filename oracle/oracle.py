@@ -1,8 +1,8 @@
 """ctypes binding of the CPU oracle (oracle/rr_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker or the timed CPU
-reference, never as part of the product path.
+cpu_baseline leg of bench.py, always as the reference the HIP results are
+compared with or the timed CPU baseline, never as part of the product path.
 """
 import ctypes
 import os

@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
  * CPU rasteriser (crates/raster, crates/color of pthomasfournet/pdf-raster),
- * used as the checker for the HIP product path and as the timed CPU baseline.
+ * against which the HIP product path is compared, and the timed CPU baseline.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it.  Nothing in the product library links or calls it.
  *

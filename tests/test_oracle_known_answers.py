@@ -1,7 +1,7 @@
 """The oracle re-runs the reference's own known-answer tests for the hot path.
 
 Each test names the reference test it restates (file:line).  These pin the
-oracle before it is trusted as the checker of the HIP kernels.
+oracle before the HIP kernels are compared with it.
 """
 import math
 
