@@ -564,7 +564,7 @@ def main():
                     help="end with a 1 GiB device-to-device copy (FETCH/WRITE counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; --gpus must be 1 "
+        print("bench.py: this library runs on one GPU only; --gpus must be 1 "
               "(multi-GPU sharding is a design note, not a measured configuration)", file=sys.stderr)
         sys.exit(2)
 
