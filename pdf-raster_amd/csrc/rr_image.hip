@@ -42,6 +42,9 @@ struct ImgArgs {
     uint32_t last_node;      // g^D - 1
     int32_t tiles_x, n_tiles;  // 256 px x 16-row canvas tiles over the bbox
     double a, b, c, d, e, f, inv_det, page_h, iw, ih;
+    int64_t n_img_px;          // width * height of the image
+    const uint16_t *grey;      // grey | alpha << 8 per image pixel, when converted ahead
+    uint16_t *grey_out;        // where k_image_grey writes them
 };
 
 __device__ __forceinline__ uint32_t div255(uint32_t x) {
@@ -133,50 +136,31 @@ constexpr int kImgThreads = 1024;  // 16 waves, each on a 32 px x 8-row block
 constexpr int kImgTileW = 256;     // tile: 8 x 2 wave blocks
 constexpr int kImgTileH = 16;
 
-// One thread per aligned canvas dword (4 pixels).  The work runs in three
-// phases over the 4 pixels, so that each phase's loads are in flight together:
-// colour bytes and soft-mask bytes of the sampled image pixels, then the
-// simplex grid taps, then the blends (blend_multiply, blend_u8).
+// Colour bytes and soft-mask byte of image pixel idx.
 template <int D>
-__device__ __forceinline__ void place_dword(const ImgArgs &A, const uint16_t *lgrid, int32_t x0, int32_t dy) {
-    const double dy_pdf = A.page_h - static_cast<double>(dy);
-    const double dy_rel = dy_pdf - A.f;
-    const double u_row = (-A.c * dy_rel) * A.inv_det;
-    const double v_row = (A.a * dy_rel) * A.inv_det;
-    uint32_t c[4][D], alpha[4];
-    bool in[4];
+__device__ __forceinline__ void load_pixel(const ImgArgs &A, int64_t idx, uint32_t c[D], uint32_t &alpha) {
+    alpha = A.smask ? A.smask[idx] : 255u;
+    if constexpr (D == 4) {
+        const uint32_t q = *reinterpret_cast<const uint32_t *>(A.pixels + idx * 4);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int32_t dx = x0 + k;
-        in[k] = dx >= A.bx0 && dx < A.bx1;
-        const double dx_rel = static_cast<double>(dx) - A.e;
-        double u = fma(A.d * dx_rel, A.inv_det, u_row);
-        double v = fma((-A.b) * dx_rel, A.inv_det, v_row);
-        // clamps to [0, 1]; a NaN u or v (from overflowing products) ends as
-        // ix = iw - 1, iy = ih - 1, as it does when the clamp keeps the NaN
-        u = fmax(fmin(u, 1.0), 0.0);
-        v = fmin(fmax(v, 0.0), 1.0);
-        const int32_t ix = static_cast<int32_t>(fmin(u * A.iw, A.iw - 1.0));
-        const int32_t iy = static_cast<int32_t>(fmin((1.0 - v) * A.ih, A.ih - 1.0));
-        // a pixel outside the bbox reads image pixel 0 and is discarded
-        const int64_t idx = in[k] ? static_cast<int64_t>(iy) * A.img_w + ix : 0;
-        alpha[k] = A.smask ? A.smask[idx] : 255u;
-        if constexpr (D == 4) {
-            const uint32_t q = *reinterpret_cast<const uint32_t *>(A.pixels + idx * 4);
+        for (int a = 0; a < 4; ++a) c[a] = (q >> (8 * a)) & 0xffu;
+    } else {
+        // the 3 bytes as one aligned 16-bit load and one byte load, both
+        // inside the pixel
+        const uintptr_t p = reinterpret_cast<uintptr_t>(A.pixels) + static_cast<uintptr_t>(idx) * 3;
+        const bool odd = p & 1u;
+        const uint32_t h = *reinterpret_cast<const uint16_t *>(odd ? p + 1 : p);
+        const uint32_t b = *reinterpret_cast<const uint8_t *>(odd ? p : p + 2);
+        const uint32_t q = odd ? (b | h << 8) : (h | b << 16);
 #pragma unroll
-            for (int a = 0; a < 4; ++a) c[k][a] = (q >> (8 * a)) & 0xffu;
-        } else {
-            // the 3 bytes as one aligned 16-bit load and one byte load, both
-            // inside the pixel
-            const uintptr_t p = reinterpret_cast<uintptr_t>(A.pixels) + static_cast<uintptr_t>(idx) * 3;
-            const bool odd = p & 1u;
-            const uint32_t h = *reinterpret_cast<const uint16_t *>(odd ? p + 1 : p);
-            const uint32_t b = *reinterpret_cast<const uint8_t *>(odd ? p : p + 2);
-            const uint32_t q = odd ? (b | h << 8) : (h | b << 16);
-#pragma unroll
-            for (int a = 0; a < 3; ++a) c[k][a] = (q >> (8 * a)) & 0xffu;
-        }
+        for (int a = 0; a < 3; ++a) c[a] = (q >> (8 * a)) & 0xffu;
     }
+}
+
+// Grey values of 4 colours through the grid (the simplex taps, then the
+// rounded division), taps from the LDS copy when all of a lane's lie in it.
+template <int D>
+__device__ __forceinline__ void grey4(const ImgArgs &A, const uint16_t *lgrid, const uint32_t c[4][D], uint32_t s[4]) {
     uint32_t tap[4][D + 1], wt[4][D + 1], gi[4][D + 1];
     uint32_t gmax = 0;
 #pragma unroll
@@ -199,6 +183,60 @@ __device__ __forceinline__ void place_dword(const ImgArgs &A, const uint16_t *lg
 #pragma unroll
             for (int v = 0; v <= D; ++v) tap[k][v] = A.grid[gi[k][v]];
     }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int v = 0; v <= D; ++v) acc = mad24(wt[k][v], tap[k][v], acc);
+        // x / 65535 == (x + 1 + (x >> 16)) >> 16 for x < 2^24
+        const uint32_t x = acc + 32767u;
+        s[k] = (x + 1u + (x >> 16)) >> 16;
+    }
+}
+
+// One thread per aligned canvas dword (4 pixels).  The work runs in phases
+// over the 4 pixels, so that each phase's loads are in flight together:
+// sample positions, then the sampled image pixels' colour and soft-mask bytes
+// (or their grey + alpha pairs when the image was converted ahead, GREY),
+// then the simplex grid taps, then the blends (blend_multiply, blend_u8).
+template <int D, bool GREY>
+__device__ __forceinline__ void place_dword(const ImgArgs &A, const uint16_t *lgrid, int32_t x0, int32_t dy) {
+    const double dy_pdf = A.page_h - static_cast<double>(dy);
+    const double dy_rel = dy_pdf - A.f;
+    const double u_row = (-A.c * dy_rel) * A.inv_det;
+    const double v_row = (A.a * dy_rel) * A.inv_det;
+    int64_t idx[4];
+    bool in[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int32_t dx = x0 + k;
+        in[k] = dx >= A.bx0 && dx < A.bx1;
+        const double dx_rel = static_cast<double>(dx) - A.e;
+        double u = fma(A.d * dx_rel, A.inv_det, u_row);
+        double v = fma((-A.b) * dx_rel, A.inv_det, v_row);
+        // clamps to [0, 1]; a NaN u or v (from overflowing products) ends as
+        // ix = iw - 1, iy = ih - 1, as it does when the clamp keeps the NaN
+        u = fmax(fmin(u, 1.0), 0.0);
+        v = fmin(fmax(v, 0.0), 1.0);
+        const int32_t ix = static_cast<int32_t>(fmin(u * A.iw, A.iw - 1.0));
+        const int32_t iy = static_cast<int32_t>(fmin((1.0 - v) * A.ih, A.ih - 1.0));
+        // a pixel outside the bbox reads image pixel 0 and is discarded
+        idx[k] = in[k] ? static_cast<int64_t>(iy) * A.img_w + ix : 0;
+    }
+    uint32_t s[4], alpha[4];
+    if constexpr (GREY) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t q = A.grey[idx[k]];
+            s[k] = q & 0xffu;
+            alpha[k] = q >> 8;
+        }
+    } else {
+        uint32_t c[4][D];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) load_pixel<D>(A, idx[k], c[k], alpha[k]);
+        grey4<D>(A, lgrid, c, s);
+    }
     uint32_t *dst = reinterpret_cast<uint32_t *>(A.canvas + static_cast<int64_t>(dy) * A.stride + x0);
     const uint32_t old = *dst;
     uint32_t out = old;
@@ -207,27 +245,23 @@ __device__ __forceinline__ void place_dword(const ImgArgs &A, const uint16_t *lg
     for (int k = 0; k < 4; ++k) {
         if (!in[k] || alpha[k] == 0) continue;
         any = true;
-        uint32_t acc = 0;
-#pragma unroll
-        for (int v = 0; v <= D; ++v) acc = mad24(wt[k][v], tap[k][v], acc);
-        // x / 65535 == (x + 1 + (x >> 16)) >> 16 for x < 2^24
-        const uint32_t x = acc + 32767u;
-        uint32_t s = (x + 1u + (x >> 16)) >> 16;
+        uint32_t sk = s[k];
         const uint32_t dv = (old >> (8 * k)) & 0xffu;
-        if (A.blend == 1) s = div255(s * dv);  // blend_multiply
+        // blend_multiply.  The product goes through mul24: with both factors
+        // known to be bytes, the compiler folds the plain div255(sk * dv)
+        // into a v_dot4_u32_u8 sequence that computes (2 x + 136) >> 8
+        // instead of (x + (x >> 8) + 128) >> 8.
+        if (A.blend == 1) sk = div255(mul24(sk, dv));
         const uint32_t al = alpha[k];
-        const uint32_t r = al == 255u ? s : mul24(s * al + dv * (255u - al), 32897u) >> 23;  // blend_u8, / 255
+        const uint32_t r = al == 255u ? sk : mul24(sk * al + dv * (255u - al), 32897u) >> 23;  // blend_u8, / 255
         out = (out & ~(0xffu << (8 * k))) | (r << (8 * k));
     }
     if (any) *dst = out;
 }
 
-// Persistent blocks: copy grid nodes [0, lds_nodes) into LDS (16-byte loads
-// when the grid is 16-byte aligned), then place tiles blockIdx.x,
-// blockIdx.x + gridDim.x, ...
-template <int D>
-__global__ void __launch_bounds__(kImgThreads) k_place_image(ImgArgs A) {
-    extern __shared__ uint4 lds_raw[];
+// Copy grid nodes [0, lds_nodes) into LDS (16-byte loads when the grid is
+// 16-byte aligned).
+__device__ __forceinline__ void grid_to_lds(const ImgArgs &A, uint4 *lds_raw) {
     uint16_t *lgrid = reinterpret_cast<uint16_t *>(lds_raw);
     const int32_t tid = static_cast<int32_t>(threadIdx.x);
     int32_t done = 0;
@@ -240,6 +274,16 @@ __global__ void __launch_bounds__(kImgThreads) k_place_image(ImgArgs A) {
     }
     for (int32_t i = done + tid; i < static_cast<int32_t>(A.lds_nodes); i += kImgThreads) lgrid[i] = A.grid[i];
     __syncthreads();
+}
+
+// Persistent blocks: the grid into LDS (unless the image was converted ahead),
+// then tiles blockIdx.x, blockIdx.x + gridDim.x, ...
+template <int D, bool GREY>
+__global__ void __launch_bounds__(kImgThreads) k_place_image(ImgArgs A) {
+    extern __shared__ uint4 lds_raw[];
+    if constexpr (!GREY) grid_to_lds(A, lds_raw);
+    const uint16_t *lgrid = reinterpret_cast<const uint16_t *>(lds_raw);
+    const int32_t tid = static_cast<int32_t>(threadIdx.x);
     // a wave covers a 32 px x 8-row block (8 x 8 lanes), so that under rotation
     // its image samples fall in a compact patch of few cache lines
     const int32_t xs = A.bx0 & ~3;
@@ -249,7 +293,32 @@ __global__ void __launch_bounds__(kImgThreads) k_place_image(ImgArgs A) {
         const int32_t ty = t / A.tiles_x, tx = t - ty * A.tiles_x;
         const int32_t x0 = xs + tx * kImgTileW + ox;
         const int32_t dy = A.by0 + ty * kImgTileH + oy;
-        if (dy < A.by1 && x0 < A.bx1) place_dword<D>(A, lgrid, x0, dy);
+        if (dy < A.by1 && x0 < A.bx1) place_dword<D, GREY>(A, lgrid, x0, dy);
+    }
+}
+
+// Grey + alpha of every image pixel ahead of an enlarging placement, 4 pixels
+// a thread: grey[i] = grey value | soft-mask byte << 8 (the values the
+// placement would compute for each canvas pixel sampling pixel i).
+template <int D>
+__global__ void __launch_bounds__(kImgThreads) k_image_grey(ImgArgs A) {
+    extern __shared__ uint4 lds_raw[];
+    grid_to_lds(A, lds_raw);
+    const uint16_t *lgrid = reinterpret_cast<const uint16_t *>(lds_raw);
+    const int64_t n = A.n_img_px;
+    for (int64_t g = static_cast<int64_t>(blockIdx.x) * kImgThreads + threadIdx.x; 4 * g < n;
+         g += static_cast<int64_t>(gridDim.x) * kImgThreads) {
+        uint32_t c[4][D], alpha[4], s[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) load_pixel<D>(A, min(4 * g + k, n - 1), c[k], alpha[k]);
+        grey4<D>(A, lgrid, c, s);
+        uint16_t *out = A.grey_out + 4 * g;
+        if (4 * g + 4 <= n) {
+            *reinterpret_cast<uint2 *>(out) = make_uint2(s[0] | alpha[0] << 8 | (s[1] | alpha[1] << 8) << 16,
+                                                         s[2] | alpha[2] << 8 | (s[3] | alpha[3] << 8) << 16);
+        } else {
+            for (int k = 0; 4 * g + k < n; ++k) out[k] = static_cast<uint16_t>(s[k] | alpha[k] << 8);
+        }
     }
 }
 
@@ -491,20 +560,61 @@ int32_t rr_hip_record_place_images(rr_hip_ctx *c, const rr_image_batch *b) {
         A.lds_nodes = static_cast<uint32_t>(lds_nodes);
         A.last_node = static_cast<uint32_t>(nodes - 1);
         const size_t lds_bytes = static_cast<size_t>(lds_nodes) * 2;
-        const void *kfn = im.channels == 4 ? reinterpret_cast<const void *>(&k_place_image<4>)
-                                           : reinterpret_cast<const void *>(&k_place_image<3>);
-        RR_TRY(c, hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_bytes)));
-        int per_cu = 0;
-        RR_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kImgThreads, lds_bytes));
-        const int64_t blocks = std::min<int64_t>(A.n_tiles, static_cast<int64_t>(c->n_cu) * std::max(1, per_cu));
-        const int32_t rc = launch_profiled(c, [&] {
-            if (im.channels == 4)
-                hipLaunchKernelGGL(k_place_image<4>, dim3(static_cast<unsigned>(blocks)), dim3(kImgThreads), lds_bytes,
+        // an image smaller than 0.8 of its canvas footprint is converted to
+        // grey + alpha once per image pixel, so that the placement samples
+        // those instead of running the grid interpolation per canvas pixel
+        A.n_img_px = static_cast<int64_t>(im.width) * im.height;
+        A.grey = nullptr;
+        A.grey_out = nullptr;
+        const int64_t n_box = static_cast<int64_t>(A.bx1 - A.bx0) * (A.by1 - A.by0);
+        const bool ahead = A.n_img_px * 5 < n_box * 4;
+        auto blocks_of = [&](const void *kfn, size_t lds, int64_t work, int64_t *blocks) -> int32_t {
+            RR_TRY(c, hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+            int per_cu = 0;
+            RR_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kImgThreads, lds));
+            *blocks = std::max<int64_t>(1, std::min<int64_t>(work, static_cast<int64_t>(c->n_cu) * std::max(1, per_cu)));
+            return RR_OK;
+        };
+        int64_t blocks = 0;
+        int32_t rc = RR_OK;
+        if (ahead) {
+            rc = rrint::ensure(c, &c->img_grey, &c->img_grey_cap, static_cast<size_t>(A.n_img_px));
+            if (rc) return rc;
+            A.grey_out = c->img_grey;
+            const void *kg = im.channels == 4 ? reinterpret_cast<const void *>(&k_image_grey<4>)
+                                              : reinterpret_cast<const void *>(&k_image_grey<3>);
+            rc = blocks_of(kg, lds_bytes, (A.n_img_px + 4 * kImgThreads - 1) / (4 * kImgThreads), &blocks);
+            if (rc) return rc;
+            rc = launch_profiled(c, [&] {
+                if (im.channels == 4)
+                    hipLaunchKernelGGL(k_image_grey<4>, dim3(static_cast<unsigned>(blocks)), dim3(kImgThreads),
+                                       lds_bytes, rrint::stream(c), A);
+                else
+                    hipLaunchKernelGGL(k_image_grey<3>, dim3(static_cast<unsigned>(blocks)), dim3(kImgThreads),
+                                       lds_bytes, rrint::stream(c), A);
+            });
+            if (rc) return rc;
+            A.grey = c->img_grey;
+            rc = blocks_of(reinterpret_cast<const void *>(&k_place_image<4, true>), 0, A.n_tiles, &blocks);
+            if (rc) return rc;
+            rc = launch_profiled(c, [&] {
+                hipLaunchKernelGGL((k_place_image<4, true>), dim3(static_cast<unsigned>(blocks)), dim3(kImgThreads), 0,
                                    rrint::stream(c), A);
-            else
-                hipLaunchKernelGGL(k_place_image<3>, dim3(static_cast<unsigned>(blocks)), dim3(kImgThreads), lds_bytes,
-                                   rrint::stream(c), A);
-        });
+            });
+        } else {
+            const void *kfn = im.channels == 4 ? reinterpret_cast<const void *>(&k_place_image<4, false>)
+                                               : reinterpret_cast<const void *>(&k_place_image<3, false>);
+            rc = blocks_of(kfn, lds_bytes, A.n_tiles, &blocks);
+            if (rc) return rc;
+            rc = launch_profiled(c, [&] {
+                if (im.channels == 4)
+                    hipLaunchKernelGGL((k_place_image<4, false>), dim3(static_cast<unsigned>(blocks)), dim3(kImgThreads),
+                                       lds_bytes, rrint::stream(c), A);
+                else
+                    hipLaunchKernelGGL((k_place_image<3, false>), dim3(static_cast<unsigned>(blocks)), dim3(kImgThreads),
+                                       lds_bytes, rrint::stream(c), A);
+            });
+        }
         if (rc) return rc;
     }
     return RR_OK;

@@ -145,6 +145,9 @@ struct rr_hip_ctx {
     int32_t *xp_err = nullptr;
     size_t xp_err_cap = 0;
     hipEvent_t xp_ev[6] = {};  // walk, count, emit start/stop (profiling on)
+    // image placement: grey + alpha per image pixel of an image converted ahead
+    uint16_t *img_grey = nullptr;
+    size_t img_grey_cap = 0;
     double xp_ms[3] = {0.0, 0.0, 0.0};
     int4 *sitems = nullptr;  // span-pass work items (path, first row, bands, first span item)
     size_t sitems_cap = 0;

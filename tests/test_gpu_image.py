@@ -107,16 +107,19 @@ def test_random_small_affine(backend, seed):
         _compare(got, want, f"seed {seed} image {k}")
 
 
+@pytest.mark.parametrize("scale", [1.7, 0.6])
 @pytest.mark.parametrize("channels,g,shift", [(4, 40, 0), (3, 64, 0), (4, 17, 1), (3, 33, 3)])
-def test_grid_beyond_lds_and_unaligned_grid(backend, channels, g, shift):
+def test_grid_beyond_lds_and_unaligned_grid(backend, channels, g, shift, scale):
     # 40^4 and 64^3 u16 grids are larger than a block's LDS, so taps past the
     # LDS copy read global memory; shifted grids take the 2-byte copy loop.
+    # Scale 1.7 enlarges the image (grey + alpha converted ahead per image
+    # pixel), scale 0.6 shrinks it (grid interpolation per canvas pixel).
     rng = np.random.default_rng(70 + g + shift)
     W, H, pw, ph = 120, 90, 260, 230
     px = rng.integers(0, 256, (H, W, channels)).astype(np.uint8)
     sm = rng.integers(0, 256, (H, W)).astype(np.uint8)
     grid = rng.integers(0, 65536, g ** channels).astype(np.uint16)
-    th, s = np.deg2rad(20.0), 1.7
+    th, s = np.deg2rad(20.0), scale
     ctm = (s * W * np.cos(th), s * W * np.sin(th), -s * H * np.sin(th), s * H * np.cos(th), 70.0, 15.0)
     bg = rng.integers(0, 256, (ph, pw)).astype(np.uint8)
     got = _single(backend, px, sm, ctm, int(channels == 3), grid, g, pw, ph, bg, grid_shift=shift)
