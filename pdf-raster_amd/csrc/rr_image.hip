@@ -249,8 +249,8 @@ __device__ __forceinline__ void place_dword(const ImgArgs &A, const uint16_t *lg
         const uint32_t dv = (old >> (8 * k)) & 0xffu;
         // blend_multiply.  The product goes through mul24: with both factors
         // known to be bytes, the compiler folds the plain div255(sk * dv)
-        // into a v_dot4_u32_u8 sequence that computes (2 x + 136) >> 8
-        // instead of (x + (x >> 8) + 128) >> 8.
+        // into a v_dot4_u32_u8 sequence whose results differ from the
+        // formula's (seen on enlarged images, where sk is a loaded byte).
         if (A.blend == 1) sk = div255(mul24(sk, dv));
         const uint32_t al = alpha[k];
         const uint32_t r = al == 255u ? sk : mul24(sk * al + dv * (255u - al), 32897u) >> 23;  // blend_u8, / 255
