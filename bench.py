@@ -509,7 +509,8 @@ def bench_dispatch(args, desc):
     step(True, True)
     ms, st = be.last_batch_stats()
     be.set_profiling(False)
-    kernels = batched_kernel_bytes(ms, st, R.variants()[variant][1], sum(p.n_paths for p in eval_pages))
+    ch = R.variants()[variant][1]
+    kernels = batched_kernel_bytes(ms, st, ch, sum(p.n_paths for p in eval_pages))
     kern = max(("k_spans", "k_compose"), key=lambda k: kernels[k]["ms"])
     out = {
         "metric": METRIC, "value": round(w_ref / learned_sec / 1e6, 3), "unit": "Mpx/s", "n_gpus": 1,
