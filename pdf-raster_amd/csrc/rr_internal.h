@@ -142,6 +142,8 @@ struct rr_hip_ctx {
     size_t xp_sidx_cap = 0;
     long long *xp_seg = nullptr;   // point per segment
     size_t xp_seg_cap = 0;
+    double2 *xp_stage = nullptr;   // staged leaf points
+    size_t xp_stage_cap = 0;
     int32_t *xp_err = nullptr;
     size_t xp_err_cap = 0;
     hipEvent_t xp_ev[6] = {};  // walk, count, emit start/stop (profiling on)

@@ -15,9 +15,12 @@
 //   scan, compact    the points that emit edges (lines, curve starts, points a
 //                    closing line follows), in order: the segments;
 //   k_xpath_count    one thread per segment: edges it emits (1 per line, the
-//                    leaves of its curve, 1 per closing line);
+//                    leaves of its curve, 1 per closing line), keeping the
+//                    first 64 leaf points of a curve in a staging area;
 //   scan             edge offset of every segment;
-//   k_xpath_emit     one thread per segment: writes its edges at its offset.
+//   k_xpath_emit     one wave per 64 segments: writes their contiguous run of
+//                    edges with coalesced stores, from the staged leaf points
+//                    (a curve with more leaves is flattened again).
 // Threads go to segments, not points, so the two control points that follow
 // each curve start leave no lane idle.
 //
@@ -151,6 +154,8 @@ struct XpathArgs {
     const long long *eoff;  // exclusive scan of the segment edge counts (n_pts + 1)
     double4 *edges;
     int64_t *edge_off;      // n_paths + 1
+    double2 *stage;         // per segment, the first stage_n leaf points of its curve (see leaf_slot)
+    int32_t stage_n;
     int32_t *err;           // set when a path's point range is out of order or out of bounds
 };
 
@@ -280,6 +285,12 @@ __device__ __forceinline__ bool closes(const XpathArgs &A, int64_t i, double2 *c
     return fabs(cur->x - start->x) > 1e-10 || fabs(cur->y - start->y) > 1e-10;
 }
 
+// Staging slot of leaf k of segment sg (a segment's leaves are contiguous, so
+// the emit pass reads an edge's two points from one place).
+__device__ __forceinline__ int64_t leaf_slot(const XpathArgs &A, int64_t sg, int32_t k) {
+    return sg * A.stage_n + k;
+}
+
 __global__ void __launch_bounds__(256) k_xpath_count(XpathArgs A) {
     const int64_t sg = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (sg >= A.n_pts) return;
@@ -293,23 +304,80 @@ __global__ void __launch_bounds__(256) k_xpath_count(XpathArgs A) {
     if (r == kRoleLine) {
         n = 1;
     } else if (r == kRoleCurve) {
-        flatten_leaves(curve_at(A, i), A.flat_sq, [&](double, double) { ++n; });
+        // count the leaves, keeping the first stage_n for the emit pass
+        flatten_leaves(curve_at(A, i), A.flat_sq, [&](double x, double y) {
+            if (n < A.stage_n) A.stage[leaf_slot(A, sg, n)] = make_double2(x, y);
+            ++n;
+        });
     }
     double2 cur, start;
     if (closes(A, i, &cur, &start)) ++n;
     A.cnt[sg] = n;
 }
 
-__global__ void __launch_bounds__(256) k_xpath_emit(XpathArgs A) {
-    const int64_t sg = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (sg >= A.n_pts || sg >= A.sidx[A.n_pts]) return;
-    const int64_t i = A.seg[sg];
-    int64_t o = A.eoff[sg];
-    const uint8_t r = A.role[i];
-    if (r == kRoleLine) {
+// Emit: a wave takes 64 consecutive segments, whose edges form one contiguous
+// run of the output, and writes that run with coalesced stores: output edge q
+// finds its segment by a binary search over the 65 segment offsets held in
+// LDS, then takes its two points from the transformed points or the staged
+// leaves.  A curve with more leaves than were staged is flattened again by
+// its own lane afterwards.
+constexpr uint32_t kSegCurve = 1, kSegClose = 2, kSegRedo = 4;
+
+__device__ __forceinline__ double4 seg_edge(const XpathArgs &A, int64_t sg, int64_t i, uint32_t info, int64_t k,
+                                            int64_t leaves) {
+    if (info & kSegCurve) {
+        if (k < leaves) {
+            const double2 b = A.stage[leaf_slot(A, sg, static_cast<int32_t>(k))];
+            const double2 a = k == 0 ? A.t[i - 1] : A.stage[leaf_slot(A, sg, static_cast<int32_t>(k - 1))];
+            return make_double4(a.x, a.y, b.x, b.y);
+        }
+    } else if (A.role[i] == kRoleLine && k == 0) {
         const double2 a = A.t[i - 1], b = A.t[i];
-        A.edges[o++] = make_double4(a.x, a.y, b.x, b.y);
-    } else if (r == kRoleCurve) {
+        return make_double4(a.x, a.y, b.x, b.y);
+    }
+    // the closing line after point i
+    const double2 cur = A.t[i], start = A.t[A.close_to[i]];
+    return make_double4(cur.x, cur.y, start.x, start.y);
+}
+
+__global__ void __launch_bounds__(256) k_xpath_emit(XpathArgs A) {
+    __shared__ long long off[4][65];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t n_seg = A.sidx[A.n_pts];
+    const int64_t sg0 = (static_cast<int64_t>(blockIdx.x) * 4 + w) * 64, sg = sg0 + lane;
+    const bool valid = sg < n_seg;
+    // this lane's segment: point, kind, edge count (eoff is flat past n_seg)
+    int64_t i = 0, leaves = 0;
+    uint32_t info = 0;
+    const long long o0 = A.eoff[min(sg, A.n_pts)];
+    off[w][lane] = o0;
+    if (lane == 63) off[w][64] = A.eoff[min(sg0 + 64, A.n_pts)];
+    if (valid) {
+        i = A.seg[sg];
+        double2 cur, start;
+        const bool cl = closes(A, i, &cur, &start);
+        leaves = A.eoff[sg + 1] - o0 - (cl ? 1 : 0);
+        info = (cl ? kSegClose : 0u);
+        if (A.role[i] == kRoleCurve) info |= kSegCurve | (leaves > A.stage_n ? kSegRedo : 0u);
+    }
+    __shared__ uint32_t s_info[4][64];
+    __shared__ long long s_pt[4][64], s_leaves[4][64];
+    s_info[w][lane] = info;
+    s_pt[w][lane] = i;
+    s_leaves[w][lane] = leaves;
+    __syncthreads();
+    const long long base = off[w][0], end = off[w][64];
+    for (long long q = base + lane; q < end; q += 64) {
+        int j = 0;  // last segment j with off[j] <= q
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+            if (off[w][j + step] <= q) j += step;
+        const uint32_t inf = s_info[w][j];
+        if (inf & kSegRedo) continue;
+        A.edges[q] = seg_edge(A, sg0 + j, s_pt[w][j], inf, q - off[w][j], s_leaves[w][j]);
+    }
+    if (info & kSegRedo) {
+        long long o = o0;
         const Cubic c = curve_at(A, i);
         double px = c.x0, py = c.y0;
         flatten_leaves(c, A.flat_sq, [&](double x, double y) {
@@ -317,9 +385,9 @@ __global__ void __launch_bounds__(256) k_xpath_emit(XpathArgs A) {
             px = x;
             py = y;
         });
+        double2 cur, start;
+        if (closes(A, i, &cur, &start)) A.edges[o] = make_double4(cur.x, cur.y, start.x, start.y);
     }
-    double2 cur, start;
-    if (closes(A, i, &cur, &start)) A.edges[o] = make_double4(cur.x, cur.y, start.x, start.y);
 }
 
 __global__ void __launch_bounds__(256) k_xpath_offsets(XpathArgs A) {
@@ -334,6 +402,9 @@ __global__ void __launch_bounds__(256) k_xpath_offsets(XpathArgs A) {
     } while (0)
 
 unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+constexpr long long kStageLeaves = 64;         // leaves kept per curve by the count pass
+constexpr long long kStageBytes = 4ll << 30;   // at most this much staging
 
 }  // namespace
 
@@ -400,10 +471,23 @@ extern "C" int32_t rr_hip_xpath_edges(rr_hip_ctx *c, const rr_xpath_batch *b, in
         return rrint::fail(c, RR_E_INVARIANT, "xpath_edges: pt_off must rise from 0 to n_pts");
     rc = rrint::scan_counts(c, c->xp_cnt, c->xp_sidx, n_pts);
     if (rc) return rc;
+    long long n_seg = 0;
+    RR_TRY(c, hipMemcpyAsync(&n_seg, c->xp_sidx + n_pts, sizeof(n_seg), hipMemcpyDeviceToHost, st));
     if (n_pts > 0) {
         hipLaunchKernelGGL(k_xpath_compact, dim3(blocks_for(n_pts)), dim3(256), 0, st, A);
         RR_TRY(c, hipGetLastError());
     }
+    RR_TRY(c, hipStreamSynchronize(st));
+    // leaf staging: up to kStageLeaves leaves per segment within kStageBytes
+    const long long seg_pad = (n_seg + 63) / 64 * 64;
+    A.stage_n = seg_pad > 0 ? static_cast<int32_t>(std::min<long long>(
+                                  kStageLeaves, kStageBytes / (seg_pad * static_cast<long long>(sizeof(double2)))))
+                            : 0;
+    if (A.stage_n > 0) {
+        rc = rrint::ensure(c, &c->xp_stage, &c->xp_stage_cap, static_cast<size_t>(seg_pad * A.stage_n));
+        if (rc) return rc;
+    }
+    A.stage = c->xp_stage;
     RR_TRY(c, mark(2));
     if (n_pts > 0) {
         hipLaunchKernelGGL(k_xpath_count, dim3(blocks_for(n_pts)), dim3(256), 0, st, A);
@@ -422,8 +506,8 @@ extern "C" int32_t rr_hip_xpath_edges(rr_hip_ctx *c, const rr_xpath_batch *b, in
         return rrint::fail(c, RR_E_CAPACITY, "xpath_edges: edge buffer holds " + std::to_string(b->edge_cap) +
                                                  " edges, the paths give " + std::to_string(total));
     RR_TRY(c, mark(4));
-    if (n_pts > 0) {
-        hipLaunchKernelGGL(k_xpath_emit, dim3(blocks_for(n_pts)), dim3(256), 0, st, A);
+    if (n_seg > 0) {
+        hipLaunchKernelGGL(k_xpath_emit, dim3(static_cast<unsigned>((n_seg + 255) / 256)), dim3(256), 0, st, A);
         RR_TRY(c, hipGetLastError());
     }
     RR_TRY(c, mark(5));
