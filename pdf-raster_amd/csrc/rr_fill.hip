@@ -810,7 +810,8 @@ int32_t rr_hip_destroy(rr_hip_ctx *c) {
                     c->feat,      c->route_cls, c->n_work,    c->work_base, c->mlp,       c->spath,
                     c->ovq,       c->bin_cnt,  c->bin_off,    c->bins,    c->queues,    c->band_run,
                     c->xp_t,      c->xp_role,  c->xp_close,   c->xp_cnt,  c->xp_eoff,   c->xp_err,
-                    c->xp_sidx,   c->xp_seg,   c->img_grey,   c->xp_stage};
+                    c->xp_sidx,   c->xp_seg,   c->img_grey,   c->xp_stage,  c->xp_bkt,
+                    c->xp_perm,   c->xp_ccnt,  c->xp_coff};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (auto e : c->xp_ev)

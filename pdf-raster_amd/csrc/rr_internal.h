@@ -144,6 +144,14 @@ struct rr_hip_ctx {
     size_t xp_seg_cap = 0;
     double2 *xp_stage = nullptr;   // staged leaf points
     size_t xp_stage_cap = 0;
+    uint8_t *xp_bkt = nullptr;     // work class per segment
+    size_t xp_bkt_cap = 0;
+    long long *xp_perm = nullptr;  // segments ordered by work class
+    size_t xp_perm_cap = 0;
+    int32_t *xp_ccnt = nullptr;    // segments per (class, block)
+    size_t xp_ccnt_cap = 0;
+    long long *xp_coff = nullptr;  // their exclusive scan
+    size_t xp_coff_cap = 0;
     int32_t *xp_err = nullptr;
     size_t xp_err_cap = 0;
     hipEvent_t xp_ev[6] = {};  // walk, count, emit start/stop (profiling on)

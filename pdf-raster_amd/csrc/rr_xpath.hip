@@ -14,6 +14,7 @@
 //                    role of every point, and where a closing line follows;
 //   scan, compact    the points that emit edges (lines, curve starts, points a
 //                    closing line follows), in order: the segments;
+//   classify, order  segments grouped by the flatten work their curve needs;
 //   k_xpath_count    one thread per segment: edges it emits (1 per line, the
 //                    leaves of its curve, 1 per closing line), keeping the
 //                    first 64 leaf points of a curve in a staging area;
@@ -156,6 +157,10 @@ struct XpathArgs {
     int64_t *edge_off;      // n_paths + 1
     double2 *stage;         // per segment, the first stage_n leaf points of its curve (see leaf_slot)
     int32_t stage_n;
+    uint8_t *bkt;           // per segment: work class (kClasses)
+    int32_t *cls_cnt;       // per (class, block of 256 segments): segments
+    const long long *cls_off;  // exclusive scan of cls_cnt, class-major
+    long long *perm;        // segments ordered by work class: the count pass's thread -> segment
     int32_t *err;           // set when a path's point range is out of order or out of bounds
 };
 
@@ -291,13 +296,61 @@ __device__ __forceinline__ int64_t leaf_slot(const XpathArgs &A, int64_t sg, int
     return sg * A.stage_n + k;
 }
 
-__global__ void __launch_bounds__(256) k_xpath_count(XpathArgs A) {
+// Work classes for the count pass: a curve's flatten work grows with how far
+// its root's control points lie from the chord midpoint, relative to the
+// flatness (about a quarter power of the ratio in leaves); class 2 + log2 of
+// that ratio, 1 for a curve that is flat at once, 0 for a line or closing
+// line.  Waves then take segments of one class, so that their lanes finish
+// together.  The order only affects which thread counts which segment.
+constexpr int kClasses = 64;
+
+__device__ __forceinline__ int32_t work_class(const XpathArgs &A, int64_t i) {
+    if (A.role[i] != kRoleCurve) return 0;
+    const Cubic c = curve_at(A, i);
+    const double mx = midpoint(c.x0, c.x3), my = midpoint(c.y0, c.y3);
+    const double ax = c.x1 - mx, ay = c.y1 - my, bx = c.x2 - mx, by = c.y2 - my;
+    const double d = fmax(fma(ax, ax, ay * ay), fma(bx, bx, by * by));
+    if (isnan(d)) return kClasses - 1;
+    if (!(d > A.flat_sq)) return 1;
+    const double q = d / A.flat_sq;
+    if (!(q < 1e300)) return kClasses - 1;
+    return min(2 + ilogb(q), kClasses - 1);
+}
+
+__global__ void __launch_bounds__(256) k_xpath_classify(XpathArgs A, int64_t n_seg, int64_t n_blk) {
+    __shared__ int32_t hist[kClasses];
+    if (threadIdx.x < kClasses) hist[threadIdx.x] = 0;
+    __syncthreads();
     const int64_t sg = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (sg >= A.n_pts) return;
-    if (sg >= A.sidx[A.n_pts]) {
-        A.cnt[sg] = 0;  // past the last segment: adds nothing to the offsets
+    if (sg < n_seg) {
+        const int32_t k = work_class(A, A.seg[sg]);
+        A.bkt[sg] = static_cast<uint8_t>(k);
+        atomicAdd(&hist[k], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < kClasses) A.cls_cnt[threadIdx.x * n_blk + blockIdx.x] = hist[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(256) k_xpath_order(XpathArgs A, int64_t n_seg, int64_t n_blk) {
+    __shared__ int32_t rank[kClasses];
+    if (threadIdx.x < kClasses) rank[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t sg = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (sg < n_seg) {
+        const int32_t k = A.bkt[sg];
+        const int32_t r = atomicAdd(&rank[k], 1);
+        A.perm[A.cls_off[k * n_blk + blockIdx.x] + r] = sg;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_xpath_count(XpathArgs A) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= A.n_pts) return;
+    if (t >= A.sidx[A.n_pts]) {
+        A.cnt[t] = 0;  // past the last segment: adds nothing to the offsets
         return;
     }
+    const int64_t sg = A.perm[t];
     const int64_t i = A.seg[sg];
     int32_t n = 0;
     const uint8_t r = A.role[i];
@@ -488,6 +541,28 @@ extern "C" int32_t rr_hip_xpath_edges(rr_hip_ctx *c, const rr_xpath_batch *b, in
         if (rc) return rc;
     }
     A.stage = c->xp_stage;
+    // segments ordered by work class for the count pass
+    const long long n_blk = (n_seg + 255) / 256;
+    rc = rrint::ensure(c, &c->xp_bkt, &c->xp_bkt_cap, static_cast<size_t>(std::max<long long>(n_seg, 1)));
+    if (!rc) rc = rrint::ensure(c, &c->xp_perm, &c->xp_perm_cap, static_cast<size_t>(std::max<long long>(n_seg, 1)));
+    if (!rc)
+        rc = rrint::ensure(c, &c->xp_ccnt, &c->xp_ccnt_cap, static_cast<size_t>(std::max<long long>(kClasses * n_blk, 1)));
+    if (!rc) rc = rrint::ensure(c, &c->xp_coff, &c->xp_coff_cap, static_cast<size_t>(kClasses * n_blk + 1));
+    if (rc) return rc;
+    A.bkt = c->xp_bkt;
+    A.perm = c->xp_perm;
+    A.cls_cnt = c->xp_ccnt;
+    A.cls_off = c->xp_coff;
+    if (n_seg > 0) {
+        hipLaunchKernelGGL(k_xpath_classify, dim3(static_cast<unsigned>(n_blk)), dim3(256), 0, st, A,
+                           static_cast<int64_t>(n_seg), static_cast<int64_t>(n_blk));
+        RR_TRY(c, hipGetLastError());
+        rc = rrint::scan_counts(c, c->xp_ccnt, c->xp_coff, kClasses * n_blk);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_xpath_order, dim3(static_cast<unsigned>(n_blk)), dim3(256), 0, st, A,
+                           static_cast<int64_t>(n_seg), static_cast<int64_t>(n_blk));
+        RR_TRY(c, hipGetLastError());
+    }
     RR_TRY(c, mark(2));
     if (n_pts > 0) {
         hipLaunchKernelGGL(k_xpath_count, dim3(blocks_for(n_pts)), dim3(256), 0, st, A);
