@@ -354,7 +354,7 @@ __global__ void __launch_bounds__(64) k_band_lists(const int4 *__restrict__ sel,
 // that a later opaque path covers fully (occ[tile] above the path: the
 // composite starts after that path) are left out.
 constexpr int kMaxTiles = 512;  // a 32767-pixel page at 64-pixel tiles
-constexpr int kBinSegs = 8;
+constexpr int kBinSegs = 4;
 template <bool PLACE>
 __global__ void __launch_bounds__(64) k_tile_bins(const DevPage *__restrict__ pages, int32_t n_pages,
                                                   const long long *__restrict__ band_base,
