@@ -249,3 +249,19 @@ def extreme_page():
 @pytest.mark.parametrize("launch", [R.LAUNCH_BATCHED, R.LAUNCH_PLAIN])
 def test_extreme_and_non_finite_coordinates(backend, mode, launch):
     check(backend, [extreme_page()], mode, launch=launch)
+
+
+def test_empty_pages_and_pathless_paths_in_a_batch(backend):
+    # A page without paths between two others, a path without edges, and a
+    # page whose width is not a multiple of the composite tile.
+    poly = R.workload(1, 0xC0F10001)[0].subset(range(0, 500, 25))
+    glyph = R.workload(2, 0xC0F10002, max_pages=1)[0].subset(range(0, 6000, 300))
+    empty = R.Page(333, 77, np.zeros((0, 4)), np.zeros(1, np.int64), np.zeros(0, dtype=R.PATH_ATTR_DTYPE),
+                   np.zeros((0, 2), np.int32), np.zeros((0, 2)), np.zeros(0, np.uint8), np.zeros(1, np.int64))
+    e = glyph.edges.reshape(-1, 4)
+    off = np.concatenate([glyph.edge_off[:3], glyph.edge_off[2:]])  # path 2 has no edges
+    attrs = np.concatenate([glyph.attrs[:2], glyph.attrs[1:2], glyph.attrs[2:]])
+    rows = np.concatenate([glyph.host_rows[:2], np.array([[0, -1]], np.int32), glyph.host_rows[2:]])
+    holed = R.Page(glyph.width - 37, glyph.height, e, off, attrs, rows, np.zeros((0, 2)), np.zeros(0, np.uint8),
+                   np.zeros(len(off), np.int64))
+    check(backend, [poly, empty, holed])

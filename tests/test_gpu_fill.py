@@ -188,3 +188,24 @@ def test_empty_batch_and_repeat_determinism(backend):
     a = gpu_page(backend, page)
     b = gpu_page(backend, page)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("clip", [(13.5, 7.25, 50.7, 40.2), (0.0, 0.0, 30.0, 47.999), (20.0, 10.0, 20.5, 10.5),
+                                  (-5.0, -3.0, 80.0, 90.0)])
+@pytest.mark.parametrize("launch", [R.LAUNCH_PLAIN, R.LAUNCH_BATCHED])
+def test_clip_rectangles(backend, clip, launch):
+    # Clip::new + test_rect + the clip_aa_line clamp on rectangles inside,
+    # on the edge of and beyond the page, including one under a pixel wide.
+    page = hand_page(HAND)
+    want = np.full((page.height, page.width), 255, np.uint8)
+    hashes = []
+    for i in range(page.n_paths):
+        a = page.attrs[i]
+        _, h, _ = O.fill(want, page.path_edges(i), a["eo"], a["color"], a["a_input"], clip=np.array(clip))
+        hashes.append(h)
+    dp = R.DevicePage(backend, page)
+    try:
+        got = dp.render(R.MODE_DOCUMENTED, R.FLAVOUR_GRAY8, launch, hashes=True, clip=clip)
+    finally:
+        dp.free()
+    assert_same(got, (want, np.array(hashes, np.uint64)), f"clip {clip} launch {launch}")
