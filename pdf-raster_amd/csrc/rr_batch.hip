@@ -947,6 +947,7 @@ struct PathRef {
     int64_t p;
     int32_t hi;   // span item
     uint32_t at;  // rr_path_attr bytes {eo, color, a_input, route}
+    int32_t cx0, cx1;  // columns its intervals touch (from the span head)
 };
 
 // Walks a tile bin in order, 64 entries per chunk.  A chunk already loaded
@@ -987,9 +988,13 @@ struct SpanView {
     int2 v0;
 };
 
-__device__ __forceinline__ void hdr_fetch(const BatchArgs &B, int32_t hi, long long *off, int32_t *n) {
-    const int32_t h = __builtin_amdgcn_readfirstlane(hi);
-    *off = B.shead[h].off;
+__device__ __forceinline__ void hdr_fetch(const BatchArgs &B, PathRef &P, long long *off, int32_t *n) {
+    const int32_t h = __builtin_amdgcn_readfirstlane(P.hi);
+    const int4 head = *reinterpret_cast<const int4 *>(B.shead + h);
+    *off = static_cast<long long>((static_cast<uint64_t>(static_cast<uint32_t>(head.y)) << 32) |
+                                  static_cast<uint32_t>(head.x));
+    P.cx0 = static_cast<int16_t>(head.w & 0xffff);
+    P.cx1 = static_cast<int16_t>(head.w >> 16);
     *n = span_cnt(B, h)->cnt[threadIdx.x];
 }
 
@@ -1259,9 +1264,9 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, long long ci
     int32_t n0 = 0, n1 = 0;
     bool has1 = false;
     if (has0) {
-        hdr_fetch(B, P0.hi, &off0, &n0);
+        hdr_fetch(B, P0, &off0, &n0);
         has1 = bin_next(B, pg, cur, P1);
-        if (has1) hdr_fetch(B, P1.hi, &off1, &n1);
+        if (has1) hdr_fetch(B, P1, &off1, &n1);
     }
     const uint32_t fill4 = static_cast<uint32_t>(fill_color) * 0x01010101u;
     for (int idx = lane; idx < kBandRows * kRowWords; idx += 64) {
@@ -1289,9 +1294,11 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, long long ci
             V1 = span_fetch(B, off1, n1);
             if (off1 < 0 && lane == 0) atomicOr(A.err, 8);
             has2 = bin_next(B, pg, cur, P2);
-            if (has2) hdr_fetch(B, P2.hi, &off2, &n2);
+            if (has2) hdr_fetch(B, P2, &off2, &n2);
         }
-        compose_rows<CH, COV>(B, L, pg, P0.p, P0.at, V0, Y0, tx0, tx0, txe, Y0, Y0 + nrows - 1);
+        // only the columns the path's intervals touch (its span head's cx range)
+        const int32_t xa = max(tx0, P0.cx0), xb = min(txe, P0.cx1);
+        if (xa <= xb) compose_rows<CH, COV>(B, L, pg, P0.p, P0.at, V0, Y0, tx0, xa, xb, Y0, Y0 + nrows - 1);
         ++*n_paths;
         *n_ivals += static_cast<uint32_t>(V0.n);
         __syncthreads();
