@@ -393,8 +393,9 @@ __device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t 
                                              int32_t r, int32_t mrule, Ptr X0, Ptr X1, Ptr C, int cap, int2 sm0) {
     const int lane = threadIdx.x;
     int n = 0;
-    bool has_last = false;
-    int32_t lx0 = 0, lx1 = 0, lc = 0;
+    // the row's open entry; the initial one (count 0, empty range) merges
+    // into nothing it could change and retires as a no-op
+    int32_t lx0 = INT32_MAX, lx1 = INT32_MIN, lc = 0;
     auto retire = [&](int32_t x0, int32_t x1, int32_t c) {
         if ((c & mrule) == 0) return;  // no effect on the walk
         if (n < cap) {
@@ -430,18 +431,13 @@ __device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t 
             const int4 ga = L.recA[s][j], gb = L.recB[s][j];
             const EdgeRec g{ga.x, ga.y, ga.z, ga.w, gb.x, gb.y, gb.z, gb.w};
             const Entry en = row_entry(g, r);
-            if (has_last && static_cast<long long>(lx1) + 1 >= en.x0 &&
-                static_cast<long long>(lx0) <= static_cast<long long>(en.x1) + 1) {
-                lc = static_cast<int32_t>(static_cast<uint32_t>(lc) + static_cast<uint32_t>(en.c));
-                lx0 = min(lx0, en.x0);
-                lx1 = max(lx1, en.x1);
-            } else {
-                if (has_last) retire(lx0, lx1, lc);
-                lx0 = en.x0;
-                lx1 = en.x1;
-                lc = en.c;
-                has_last = true;
-            }
+            // write_intersect's test (saturating +1 on both sides)
+            const bool touch = __builtin_elementwise_add_sat(lx1, 1) >= en.x0 &&
+                               lx0 <= __builtin_elementwise_add_sat(en.x1, 1);
+            if (!touch) retire(lx0, lx1, lc);
+            lc = touch ? static_cast<int32_t>(static_cast<uint32_t>(lc) + static_cast<uint32_t>(en.c)) : en.c;
+            lx0 = touch ? min(lx0, en.x0) : en.x0;
+            lx1 = touch ? max(lx1, en.x1) : en.x1;
         }
         __syncthreads();
     };
@@ -478,7 +474,7 @@ __device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t 
             process(0, q.r0, q.r1);
         }
     }
-    if (has_last) retire(lx0, lx1, lc);
+    retire(lx0, lx1, lc);
     return n;
 }
 
