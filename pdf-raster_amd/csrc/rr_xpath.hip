@@ -55,28 +55,47 @@ constexpr uint8_t kRoleNone = 0, kRoleLine = 1, kRoleCurve = 2;
 
 // f64::midpoint for finite operands: (a + b) / 2 unless that overflows or
 // loses the low bits of a subnormal half.
+constexpr double kHalfMax = 8.98846567431157953865e307;  // f64::MAX / 2
+
 __host__ __device__ __forceinline__ double midpoint(double a, double b) {
-    const double hi = 8.98846567431157953865e307;   // f64::MAX / 2
     const double lo = 4.45014771701440227211e-308;  // 2 * f64::MIN_POSITIVE
     const double aa = fabs(a), ab = fabs(b);
-    if (aa <= hi && ab <= hi) return (a + b) / 2.0;
+    if (aa <= kHalfMax && ab <= kHalfMax) return (a + b) / 2.0;
     if (aa < lo) return a + b / 2.0;
     if (ab < lo) return a / 2.0 + b;
     return a / 2.0 + b / 2.0;
+}
+
+// midpoint() where both operands are known to lie in [-f64::MAX/2, f64::MAX/2]
+// (HALF), where it is (a + b) / 2.
+template <bool HALF>
+__host__ __device__ __forceinline__ double mid(double a, double b) {
+    return HALF ? (a + b) / 2.0 : midpoint(a, b);
 }
 
 struct Cubic {
     double x0, y0, x1, y1, x2, y2, x3, y3;
 };
 
+// Whether every coordinate of c lies in [-f64::MAX/2, f64::MAX/2] (false for
+// NaN).  The midpoint of two values of magnitude at most M is one again
+// (|a + b| <= 2M, and rounding to nearest cannot pass the representable
+// bounds 2M and M), so the whole subdivision tree of such a root stays in
+// range and every midpoint() in it takes its (a + b) / 2 branch.
+__host__ __device__ __forceinline__ bool in_half_range(const Cubic &c) {
+    return fabs(c.x0) <= kHalfMax && fabs(c.y0) <= kHalfMax && fabs(c.x1) <= kHalfMax && fabs(c.y1) <= kHalfMax &&
+           fabs(c.x2) <= kHalfMax && fabs(c.y2) <= kHalfMax && fabs(c.x3) <= kHalfMax && fabs(c.y3) <= kHalfMax;
+}
+
 // De Casteljau halves of c (flatten.rs:186-205).
+template <bool HALF>
 __host__ __device__ __forceinline__ void split(const Cubic &c, Cubic &l, Cubic &r) {
-    const double lx1 = midpoint(c.x0, c.x1), ly1 = midpoint(c.y0, c.y1);
-    const double hx = midpoint(c.x1, c.x2), hy = midpoint(c.y1, c.y2);
-    const double lx2 = midpoint(lx1, hx), ly2 = midpoint(ly1, hy);
-    const double rx2 = midpoint(c.x2, c.x3), ry2 = midpoint(c.y2, c.y3);
-    const double rx1 = midpoint(hx, rx2), ry1 = midpoint(hy, ry2);
-    const double rx0 = midpoint(lx2, rx1), ry0 = midpoint(ly2, ry1);
+    const double lx1 = mid<HALF>(c.x0, c.x1), ly1 = mid<HALF>(c.y0, c.y1);
+    const double hx = mid<HALF>(c.x1, c.x2), hy = mid<HALF>(c.y1, c.y2);
+    const double lx2 = mid<HALF>(lx1, hx), ly2 = mid<HALF>(ly1, hy);
+    const double rx2 = mid<HALF>(c.x2, c.x3), ry2 = mid<HALF>(c.y2, c.y3);
+    const double rx1 = mid<HALF>(hx, rx2), ry1 = mid<HALF>(hy, ry2);
+    const double rx0 = mid<HALF>(lx2, rx1), ry0 = mid<HALF>(ly2, ry1);
     l = Cubic{c.x0, c.y0, lx1, ly1, lx2, ly2, rx0, ry0};
     r = Cubic{rx0, ry0, rx1, ry1, rx2, ry2, c.x3, c.y3};
 }
@@ -90,14 +109,14 @@ struct Half {
     double x1, y1, x2, y2, x3, y3;
 };
 
-template <typename F>
-__host__ __device__ __forceinline__ void flatten_leaves(const Cubic &root, double flat_sq, F &&leaf) {
+template <bool HALF, typename F>
+__host__ __device__ __forceinline__ void flatten_leaves_t(const Cubic &root, double flat_sq, F &&leaf) {
     Cubic c = root;
     int32_t a = 0, size = kSlots;  // the current node is slots [a, a + size)
     int32_t pending = 0, kept = 0;  // pending right halves; how many of the newest are kept
     Half h0{}, h1{};                // kept: h0 newest
     for (;;) {
-        const double mx = midpoint(c.x0, c.x3), my = midpoint(c.y0, c.y3);
+        const double mx = mid<HALF>(c.x0, c.x3), my = mid<HALF>(c.y0, c.y3);
         const double ax = c.x1 - mx, ay = c.y1 - my, bx = c.x2 - mx, by = c.y2 - my;
         const double d1 = fma(ax, ax, ay * ay);
         const double d2 = fma(bx, bx, by * by);
@@ -118,7 +137,7 @@ __host__ __device__ __forceinline__ void flatten_leaves(const Cubic &root, doubl
             int32_t lo = 0;
             for (int32_t s = kSlots; s > size; s >>= 1) {
                 Cubic l, r;
-                split(c, l, r);
+                split<HALF>(c, l, r);
                 if (a >= lo + (s >> 1)) {
                     c = r;
                     lo += s >> 1;
@@ -129,7 +148,7 @@ __host__ __device__ __forceinline__ void flatten_leaves(const Cubic &root, doubl
             continue;
         }
         Cubic l, r;
-        split(c, l, r);
+        split<HALF>(c, l, r);
         h1 = h0;
         h0 = Half{r.x1, r.y1, r.x2, r.y2, r.x3, r.y3};
         kept = kept < 2 ? kept + 1 : 2;
@@ -137,6 +156,17 @@ __host__ __device__ __forceinline__ void flatten_leaves(const Cubic &root, doubl
         c = l;
         size >>= 1;
     }
+}
+
+// flatten_leaves_t with the (a + b) / 2 midpoint wherever the root allows it
+// (the same values as f64::midpoint there; a root with a coordinate beyond
+// f64::MAX/2, or NaN, takes the general form).
+template <typename F>
+__host__ __device__ __forceinline__ void flatten_leaves(const Cubic &root, double flat_sq, F &&leaf) {
+    if (in_half_range(root))
+        flatten_leaves_t<true>(root, flat_sq, leaf);
+    else
+        flatten_leaves_t<false>(root, flat_sq, leaf);
 }
 
 struct XpathArgs {

@@ -101,6 +101,42 @@ def test_random_paths_with_matrices(backend, seed, flatness):
     assert total == eoff[-1]
 
 
+def test_curves_beyond_half_range(backend):
+    """Curves with a control point beyond f64::MAX/2, infinite or NaN take
+    f64::midpoint's general form (and, never flat, split down to single
+    slots); f64::MAX/2 itself and in-range curves of the same batch take
+    (a + b) / 2.  Bit-identical to the oracle either way, except that a NaN
+    the arithmetic creates (inf/2 - inf/2) carries the processor's default
+    NaN bits (x86 sets the sign, the GPU does not), so NaNs compare by
+    position."""
+    special = [1.5e308, -1.7e308, 9e307, np.inf, np.nan, 8.98846567431157953865e307, -8.98846567431157953865e307]
+    rng = np.random.default_rng(11)
+    pts, flags, off = [], [], [0]
+    for k in range(42):
+        p = O.Path()
+        p.move_to(*rng.uniform(0, 100, 2))
+        c = rng.uniform(-50, 150, 6)
+        if k % 3 != 2:
+            c[k % 6] = special[k % len(special)]
+        p.curve_to(*c)
+        p.line_to(*rng.uniform(0, 100, 2))
+        p.close(False)
+        xy, fl = p.arrays()
+        pts.append(xy.reshape(-1, 2))
+        flags.append(fl)
+        off.append(off[-1] + len(fl))
+    pts, flags, off = np.concatenate(pts), np.concatenate(flags), np.array(off, np.int64)
+    rc, total, edges, eoff = device_xpath(backend, pts, flags, off)
+    assert rc == 0 and total == eoff[-1]
+    for i in range(len(off) - 1):
+        want = O.xpath_edges(pts[off[i]:off[i + 1]], flags[off[i]:off[i + 1]], flatness=0.1).reshape(-1, 4)
+        got = edges[eoff[i]:eoff[i + 1]]
+        assert got.shape == want.shape, f"path {i}"
+        assert np.array_equal(np.isnan(got), np.isnan(want)), f"path {i}"
+        ok = ~np.isnan(want)
+        assert np.array_equal(got[ok].view(np.uint64), want[ok].view(np.uint64)), f"path {i}"
+
+
 def test_capacity_empty_and_bad_offsets(backend):
     rng = np.random.default_rng(9)
     pts, flags, off, _ = random_paths(rng, 20)
