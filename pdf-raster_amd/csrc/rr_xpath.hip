@@ -211,14 +211,39 @@ __device__ __forceinline__ void put_chunk(const XpathArgs &A, int64_t base, int6
     A.cnt[i] = r != kRoleNone || cl;
 }
 
+// The walk's states, and the transition of one point as a byte table: byte x
+// is the state after the point when the walk reaches it in state x.  Applying
+// g and then f is one byte permute (byte x of the result is byte g[x] of f).
+using rrdev::dpp_keep;
+using rrdev::kDppRowBcast15;
+using rrdev::kDppRowBcast31;
+using rrdev::kDppRowShr1;
+using rrdev::kDppRowShr2;
+using rrdev::kDppRowShr4;
+using rrdev::kDppRowShr8;
+using rrdev::kDppWaveShr1;
+
+enum : uint32_t { kOut = 0, kIn = 1, kSkip2 = 2, kSkip1 = 3 };
+constexpr uint32_t kWalkIdentity = 0x03020100u;
+
+__device__ __forceinline__ uint32_t walk_then(uint32_t f, uint32_t g) { return __builtin_amdgcn_perm(f, f, g); }
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ uint32_t walk_from(uint32_t t) {
+    return static_cast<uint32_t>(
+        dpp_keep<CTRL, ROW_MASK>(static_cast<int32_t>(t), static_cast<int32_t>(kWalkIdentity)));
+}
+
 // The XPath::new walk of one path (xpath.rs:174-258), one wave per path.  The
-// lanes transform a chunk of 64 points (xpath.rs:421-426); the walk itself is
-// a state machine over the chunk's flags run on wave-uniform values (scalar
-// unit), which collects the roles as 64-bit masks.  A curve never checks the
-// LAST flag, so a subpath ending in a curve runs on into the next point; a
-// curve without its two further points ends the subpath at the point before
-// it, which may lie in the previous chunk, so a chunk's roles are stored one
-// chunk late.
+// lanes transform a chunk of 64 points (xpath.rs:421-426) and run the walk's
+// state machine over the chunk's flags in parallel: each lane forms its
+// point's transition table, a prefix scan of the tables (later after earlier)
+// gives every lane the state the walk reaches its point in, and the roles
+// follow from that state and the point's flags as 64-bit masks.  A curve
+// never checks the LAST flag, so a subpath ending in a curve runs on into the
+// next point; a curve without its two further points ends the subpath at the
+// point before it (which may lie in the previous chunk, so a chunk's roles
+// are stored one chunk late) and is looked at again outside a subpath.
 __global__ void __launch_bounds__(256) k_xpath_walk(XpathArgs A) {
     const int lane = threadIdx.x & 63;
     const int64_t p = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
@@ -231,66 +256,44 @@ __global__ void __launch_bounds__(256) k_xpath_walk(XpathArgs A) {
     double m[6] = {1.0, 0.0, 0.0, 1.0, 0.0, 0.0};
     if (A.matrix)
         for (int k = 0; k < 6; ++k) m[k] = A.matrix[6 * p + k];
-    enum { kOut, kIn, kSkip2, kSkip1 };
-    int state = kOut;
-    int64_t s = -1;  // start of the subpath being walked
+    uint32_t state = kOut;  // entering the chunk
+    int64_t s = -1;         // start of the subpath being walked
     uint64_t p_line = 0, p_curve = 0, p_close = 0, p_start = 0;
     int64_t p_base = -1, p_carry = -1;
     for (int64_t base = q0; base < q1; base += 64) {
         const int64_t i = base + lane;
+        const bool live = i < q1;
         uint32_t f = 0;
-        if (i < q1) {
+        if (live) {
             f = A.flags[i];
             const double xi = A.xy[2 * i], yi = A.xy[2 * i + 1];
             A.t[i] = make_double2(fma(xi, m[0], fma(yi, m[2], m[4])), fma(xi, m[1], fma(yi, m[3], m[5])));
         }
         const int nk = static_cast<int>(min<int64_t>(64, q1 - base));
         const int64_t carry = s;
-        uint64_t m_line = 0, m_curve = 0, m_close = 0, m_start = 0;
-        for (int k = 0; k < nk; ++k) {
-            const uint32_t fk = __builtin_amdgcn_readlane(f, k);
-            const int64_t ik = base + k;
-            for (;;) {
-                if (state == kOut) {
-                    if (fk & kFirst) {
-                        state = kIn;
-                        s = ik;
-                        m_start |= 1ull << k;
-                    }
-                    break;
-                }
-                if (state == kSkip2) {
-                    state = kSkip1;
-                    break;
-                }
-                if (state == kSkip1) {
-                    state = kIn;
-                    break;
-                }
-                if (fk & kCurve) {
-                    if (ik + 2 >= q1) {
-                        // too few points left: the subpath ends at the point
-                        // before, and this point is looked at again outside it
-                        if (k > 0)
-                            m_close |= 1ull << (k - 1);
-                        else
-                            p_close |= 1ull << 63;
-                        state = kOut;
-                        continue;
-                    }
-                    m_curve |= 1ull << k;
-                    state = kSkip2;
-                    break;
-                }
-                m_line |= 1ull << k;
-                if (fk & kLast) {
-                    m_close |= 1ull << k;
-                    state = kOut;
-                }
-                break;
-            }
-        }
+        const bool first = f & kFirst, curve = f & kCurve, last = f & kLast;
+        const bool near_end = i + 2 >= q1;  // a curve here lacks its two further points
+        const uint32_t n_out = first ? kIn : kOut;
+        const uint32_t n_in = curve ? (near_end ? n_out : kSkip2) : (last ? kOut : kIn);
+        uint32_t t = live ? (n_out | (n_in << 8) | (kSkip1 << 16) | (kIn << 24)) : kWalkIdentity;
+        t = walk_then(t, walk_from<kDppRowShr1>(t));
+        t = walk_then(t, walk_from<kDppRowShr2>(t));
+        t = walk_then(t, walk_from<kDppRowShr4>(t));
+        t = walk_then(t, walk_from<kDppRowShr8>(t));
+        t = walk_then(t, walk_from<kDppRowBcast15, 0xa>(t));
+        t = walk_then(t, walk_from<kDppRowBcast31, 0xc>(t));
+        const uint32_t x = (walk_from<kDppWaveShr1>(t) >> (8 * state)) & 0xffu;  // state at this point
+        const bool in = live && x == kIn;
+        const bool cut = in && curve && near_end;  // the subpath ends at the point before
+        const uint64_t m_start = __ballot((live && x == kOut && first) || (cut && first));
+        const uint64_t m_curve = __ballot(in && curve && !near_end);
+        const uint64_t m_line = __ballot(in && !curve);
+        const uint64_t m_cut = __ballot(cut);
+        uint64_t m_close = __ballot(in && !curve && last) | (m_cut >> 1);
+        if (m_cut & 1) p_close |= 1ull << 63;
+        state = (static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int32_t>(t), nk - 1)) >> (8 * state)) & 0xffu;
         if (base + 64 >= q1 && state != kOut) m_close |= 1ull << (nk - 1);  // the path ends inside a subpath
+        if (m_start) s = base + 63 - __clzll(static_cast<long long>(m_start));
         if (p_base >= 0) put_chunk(A, p_base, q1, p_line, p_curve, p_close, p_start, p_carry);
         p_line = m_line;
         p_curve = m_curve;
