@@ -32,7 +32,7 @@
 // most ten levels deep.  The slots are dyadic intervals, so the node after a
 // leaf ending at slot a is [a, a + lowbit(a)), and a node's control points are
 // those obtained by splitting the root down to it: a thread keeps the current
-// node and the two newest pending right halves in registers and rebuilds an
+// node and the three newest pending right halves in registers and rebuilds an
 // older one from the root (at most ten splits), instead of the reference's
 // 49 KB slot arrays.  Every value is produced by the same operations as in the
 // reference.
@@ -102,19 +102,22 @@ __host__ __device__ __forceinline__ void split(const Cubic &c, Cubic &l, Cubic &
 
 // The leaves of flatten_curve in emission order: leaf(x, y) gets each piece's
 // end point (the start of the first piece is the curve's p0).  Depth first,
-// left half before right half.  The two newest pending right halves are kept
-// (without their first point, the end point of the last leaf emitted before
-// them); an older one is rebuilt by splitting the root down to it.
+// left half before right half.  The kKeep newest pending right halves are
+// kept (without their first point, the end point of the last leaf emitted
+// before them); an older one is rebuilt by splitting the root down to it.  A
+// rebuild stalls every lane of the wave for its splits, so the kept depth is
+// traded against registers and the moves of each push and pop.
 struct Half {
     double x1, y1, x2, y2, x3, y3;
 };
+constexpr int kKeep = 3;
 
 template <bool HALF, typename F>
 __host__ __device__ __forceinline__ void flatten_leaves_t(const Cubic &root, double flat_sq, F &&leaf) {
     Cubic c = root;
     int32_t a = 0, size = kSlots;  // the current node is slots [a, a + size)
     int32_t pending = 0, kept = 0;  // pending right halves; how many of the newest are kept
-    Half h0{}, h1{};                // kept: h0 newest
+    Half h[kKeep] = {};             // kept: h[0] newest
     for (;;) {
         const double mx = mid<HALF>(c.x0, c.x3), my = mid<HALF>(c.y0, c.y3);
         const double ax = c.x1 - mx, ay = c.y1 - my, bx = c.x2 - mx, by = c.y2 - my;
@@ -128,8 +131,9 @@ __host__ __device__ __forceinline__ void flatten_leaves_t(const Cubic &root, dou
             a += size;
             size = a & -a;
             if (kept > 0) {
-                c = Cubic{c.x3, c.y3, h0.x1, h0.y1, h0.x2, h0.y2, h0.x3, h0.y3};
-                h0 = h1;
+                c = Cubic{c.x3, c.y3, h[0].x1, h[0].y1, h[0].x2, h[0].y2, h[0].x3, h[0].y3};
+#pragma unroll
+                for (int k = 0; k + 1 < kKeep; ++k) h[k] = h[k + 1];
                 --kept;
                 continue;
             }
@@ -149,9 +153,10 @@ __host__ __device__ __forceinline__ void flatten_leaves_t(const Cubic &root, dou
         }
         Cubic l, r;
         split<HALF>(c, l, r);
-        h1 = h0;
-        h0 = Half{r.x1, r.y1, r.x2, r.y2, r.x3, r.y3};
-        kept = kept < 2 ? kept + 1 : 2;
+#pragma unroll
+        for (int k = kKeep - 1; k > 0; --k) h[k] = h[k - 1];
+        h[0] = Half{r.x1, r.y1, r.x2, r.y2, r.x3, r.y3};
+        kept = kept < kKeep ? kept + 1 : kKeep;
         ++pending;
         c = l;
         size >>= 1;
