@@ -101,6 +101,33 @@ def test_random_paths_with_matrices(backend, seed, flatness):
     assert total == eoff[-1]
 
 
+def test_long_malformed_flag_runs(backend):
+    """Flag sequences of 65 to 400 points: the walk's state (inside a
+    subpath, or skipping a curve's control points) and the subpath start are
+    carried across 64-point chunks, and a curve cut short at a chunk's first
+    point closes the subpath at the previous chunk's last point."""
+    rng = np.random.default_rng(23)
+    choices = np.array([0, 0, 0, 8, 8, 8, 1, 2, 3, 9, 10, 4, 5, 6, 12], np.uint8)
+    pts, flags, off = [], [], [0]
+    for k in range(60):
+        n = int(rng.integers(65, 400))
+        fl = rng.choice(choices, n)
+        if k % 3 == 0:
+            fl[:] = 8  # all curve flags: skips running over chunk boundaries
+            fl[0] = 1
+        if k % 3 == 1:
+            fl[-(k % 5 + 1)] = 8  # a curve cut short near the end
+        pts.append(rng.uniform(-50, 150, (n, 2)))
+        flags.append(fl)
+        off.append(off[-1] + n)
+    pts, flags, off = np.concatenate(pts), np.concatenate(flags), np.array(off, np.int64)
+    rc, total, edges, eoff = device_xpath(backend, pts, flags, off)
+    assert rc == 0 and total == eoff[-1]
+    for i in range(len(off) - 1):
+        want = O.xpath_edges(pts[off[i]:off[i + 1]], flags[off[i]:off[i + 1]], flatness=0.1)
+        assert same_bits(edges[eoff[i]:eoff[i + 1]], want.reshape(-1, 4)), f"path {i}"
+
+
 def test_curves_beyond_half_range(backend):
     """Curves with a control point beyond f64::MAX/2, infinite or NaN take
     f64::midpoint's general form (and, never flat, split down to single
