@@ -146,10 +146,32 @@ const char *rr_hip_last_error(rr_hip_ctx *ctx);
  *      device_buffer_len, backend/mod.rs:241-282) ---- */
 int32_t rr_hip_alloc(rr_hip_ctx *ctx, size_t size, int32_t zeroed, void **buf);
 int32_t rr_hip_free(rr_hip_ctx *ctx, void *buf);
+/* GpuBackend::device_buffer_len (backend/mod.rs:267): the size a buffer was
+ * allocated with by rr_hip_alloc (RR_E_INVARIANT for any other pointer). */
+int32_t rr_hip_buffer_len(rr_hip_ctx *ctx, const void *buf, size_t *len);
+/* GpuBackend::alloc_host_pinned / free_host_pinned (backend/mod.rs:292-299):
+ * page-locked host memory for DMA transfers; size 0 is RR_E_ZERO_SIZE_ALLOC. */
+int32_t rr_hip_alloc_host_pinned(rr_hip_ctx *ctx, size_t size, void **buf);
+int32_t rr_hip_free_host_pinned(rr_hip_ctx *ctx, void *buf);
+/* GpuBackend::detect_vram_budget (backend/mod.rs:502; VramBudget :157-166):
+ * total device memory and the usable share, 3/4 of what the driver reports
+ * free (as the CUDA backend computes it, backend/cuda/mod.rs:211-220). */
+int32_t rr_hip_vram_budget(rr_hip_ctx *ctx, uint64_t *total_bytes, uint64_t *usable_bytes);
 
-/* ---- transfer (upload_async / download_async + wait, backend/mod.rs:422-457) ---- */
+/* ---- transfer (backend/mod.rs:422-496) ----
+ * upload / download: synchronous copies.  The _async forms enqueue the copy on
+ * the context stream, behind everything recorded before it, and return a
+ * fence (upload_async / download_async; the destination of a download holds
+ * the bytes once the fence is waited).  submit_transfer returns a fence after
+ * every transfer issued so far; wait_transfer blocks on any fence of the
+ * context (GpuBackend::wait_transfer / wait_download).  A fence is 1 + the
+ * number of fences handed out before it; the last 16 are tracked. */
 int32_t rr_hip_upload(rr_hip_ctx *ctx, void *dst, const void *src, size_t n);
 int32_t rr_hip_download(rr_hip_ctx *ctx, void *dst, const void *src, size_t n);
+int32_t rr_hip_upload_async(rr_hip_ctx *ctx, void *dst, const void *src, size_t n, uint64_t *fence);
+int32_t rr_hip_download_async(rr_hip_ctx *ctx, void *dst, const void *src, size_t n, uint64_t *fence);
+int32_t rr_hip_submit_transfer(rr_hip_ctx *ctx, uint64_t *fence);
+int32_t rr_hip_wait_transfer(rr_hip_ctx *ctx, uint64_t fence);
 int32_t rr_hip_memset(rr_hip_ctx *ctx, void *dst, int32_t value, size_t n);
 int32_t rr_hip_copy(rr_hip_ctx *ctx, void *dst, const void *src, size_t n); /* device to device */
 
@@ -158,8 +180,18 @@ int32_t rr_hip_copy(rr_hip_ctx *ctx, void *dst, const void *src, size_t n); /* d
 int32_t rr_hip_begin_page(rr_hip_ctx *ctx);
 int32_t rr_hip_record_fill_paths(rr_hip_ctx *ctx, const rr_fill_batch *batch);
 int32_t rr_hip_record_fill_pages(rr_hip_ctx *ctx, const rr_fill_pages *batch);
+/* submit_page returns the page's fence; wait_page blocks on it and returns
+ * that page's status (RR_E_CAPACITY when a device workspace overflowed and the
+ * page output is incomplete), which later pages cannot overwrite while the
+ * fence is among the last 16 handed out. */
 int32_t rr_hip_submit_page(rr_hip_ctx *ctx, uint64_t *fence);
 int32_t rr_hip_wait_page(rr_hip_ctx *ctx, uint64_t fence);
+/* Fault injection (no counterpart in the reference; for testing the callers'
+ * CPU fallback, crates/pdf_interp/src/renderer/page/vk_ops.rs:62-70): ORs
+ * `bits` (nonzero; 1 = overflow-entry pool exhausted) into the open page's
+ * error word, ordered after the work recorded so far, so that wait_page
+ * reports the page as incomplete exactly as a real overflow would. */
+int32_t rr_hip_inject_error(rr_hip_ctx *ctx, int32_t bits);
 /* Timing of the last waited page, from HIP events on the context stream:
  * span_ms from the first to the last recorded kernel; with profiling enabled
  * (rr_hip_set_profiling) fill_ms is the summed duration of the fill kernels
@@ -273,8 +305,11 @@ typedef struct {
 /* Images applied in order onto the pages' canvases (device pointers in the
  * descriptors, host descriptor arrays).  grid4: CMYK->grey u16 grid with
  * grid4_n nodes per axis, index ((k*n + c)*n + m)*n + y; grid3: RGB->grey u16
- * grid, index (r*n + g)*n + b.  Sampling follows the reference's general affine
- * path; CTMs that would take its axis-aligned path are rejected (RR_E_INVARIANT). */
+ * grid, index (r*n + g)*n + b.  Sampling follows blit_image's CPU path: its
+ * axis-aligned Q32 stepping when |b|*w < 0.5 and |c|*h < 0.5
+ * (crates/pdf_interp/src/renderer/page/mod.rs:1767-1898), its general affine
+ * path otherwise (:1899-1974).  Every descriptor is checked before anything is
+ * launched: a rejected batch leaves the canvases untouched. */
 typedef struct {
     const rr_image_desc *images;
     int32_t n_images;
@@ -292,9 +327,12 @@ int32_t rr_hip_record_place_images(rr_hip_ctx *ctx, const rr_image_batch *batch)
  * 321-336, params.rs:109-205), bit-exact to their CPU twins:
  * icc_cmyk_to_rgb_cpu (crates/gpu/src/cmyk.rs:523-634: clut == NULL selects the
  * reflectance matrix, color/src/convert.rs:131-161), composite_rgba8_cpu and
- * apply_soft_mask_cpu (crates/gpu/src/composite.rs:4-52). */
+ * apply_soft_mask_cpu (crates/gpu/src/composite.rs:4-52).  The CLUT's grid_n
+ * is recovered from clut_len (bytes) as grid_n_from_clut_len does
+ * (params.rs:127-147): clut_len must be grid_n^4 * 3 with 2 <= grid_n <= 255,
+ * else RR_E_INVARIANT. */
 int32_t rr_hip_record_icc_clut(rr_hip_ctx *ctx, const uint8_t *cmyk, uint8_t *rgb, const uint8_t *clut,
-                               int32_t grid_n, int64_t n_pixels);
+                               size_t clut_len, int64_t n_pixels);
 int32_t rr_hip_record_composite(rr_hip_ctx *ctx, const uint8_t *src, uint8_t *dst, int64_t n_pixels);
 int32_t rr_hip_record_apply_soft_mask(rr_hip_ctx *ctx, uint8_t *pixels, const uint8_t *mask, int64_t n_pixels);
 

@@ -149,14 +149,39 @@ uint8_t rro_clut_gray(const uint8_t *px, int32_t dims, const uint16_t *grid, int
     return (uint8_t)((acc + 32767u) / 65535u);
 }
 
-/* Places one image on a grey canvas with the reference's general affine path
+/* f64 -> i64 as Rust's `as i64` converts: truncation, saturating, NaN -> 0. */
+static int64_t sat_f64_i64(double x) {
+    if (isnan(x)) return 0;
+    if (x >= 9223372036854775807.0) return INT64_MAX;
+    if (x <= -9223372036854775808.0) return INT64_MIN;
+    return (int64_t)x;
+}
+
+/* f64::clamp(0.0, 1.0): NaN stays NaN. */
+static double clamp01(double x) { return x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x); }
+
+/* One canvas pixel from image pixel idx: soft mask 0 skips, 255 copies,
+ * else blend_u8 (page/mod.rs:1839-1852,1941-1950); the colour is the grey
+ * CLUT value, through blend_multiply first under Multiply.  Returns 1 when
+ * the pixel was written. */
+static int place_px(uint8_t *row, int64_t dx, const uint8_t *pixels, int64_t idx, int32_t channels,
+                    const uint8_t *smask, int32_t blend, const uint16_t *grid, int32_t g) {
+    const uint32_t alpha = smask ? smask[idx] : 255u;
+    if (alpha == 0) return 0;
+    uint8_t s = rro_clut_gray(pixels + idx * channels, channels, grid, g);
+    const uint8_t dst = row[dx];
+    if (blend == 1) s = rro_blend_multiply(s, dst);
+    row[dx] = alpha == 255u ? s : rro_blend_u8(s, dst, alpha);
+    return 1;
+}
+
+/* Places one image on a grey canvas as blit_image's CPU path does
  * (crates/pdf_interp/src/renderer/page/mod.rs:1607-1647 bbox, :1656-1676 inverse
- * CTM, :1899-1974 per-pixel sampling), converting the sampled pixel to grey
- * through rro_clut_gray and compositing with blend_u8 under the image-space
+ * CTM): the axis-aligned path when |b|*w < 0.5 and |c|*h < 0.5 (:1767-1898),
+ * else the general affine path (:1899-1974).  The sampled pixel goes to grey
+ * through rro_clut_gray and is composited with blend_u8 under the image-space
  * soft mask; blend = 1 replaces the source by blend_multiply(src, dst) first.
- * Returns the number of canvas pixels written, or -1 when the CTM takes the
- * reference's axis-aligned path (|b|*w < 0.5 and |c|*h < 0.5), which this
- * restatement does not cover. */
+ * Returns the number of canvas pixels written. */
 int64_t rro_place_image(uint8_t *canvas, int32_t width, int32_t height, int32_t stride, const uint8_t *pixels,
                         int32_t img_w, int32_t img_h, int32_t channels, const uint8_t *smask, const double ctm[6],
                         int32_t blend, const uint16_t *grid, int32_t g) {
@@ -173,7 +198,8 @@ int64_t rro_place_image(uint8_t *canvas, int32_t width, int32_t height, int32_t 
     double xmx = fmax(fmax(xs[0], xs[1]), fmax(xs[2], xs[3]));
     double ymn = fmin(fmin(page_h - ys[0], page_h - ys[1]), fmin(page_h - ys[2], page_h - ys[3]));
     double ymx = fmax(fmax(page_h - ys[0], page_h - ys[1]), fmax(page_h - ys[2], page_h - ys[3]));
-    int64_t dx0 = (int64_t)floor(xmn), dx1 = (int64_t)ceil(xmx), dy0 = (int64_t)floor(ymn), dy1 = (int64_t)ceil(ymx);
+    int64_t dx0 = sat_f64_i64(floor(xmn)), dx1 = sat_f64_i64(ceil(xmx)), dy0 = sat_f64_i64(floor(ymn)),
+            dy1 = sat_f64_i64(ceil(ymx));
 #define CL(v, hi) ((v) < 0 ? 0 : ((v) > (hi) ? (hi) : (v)))
     int64_t bx0 = CL(dx0, width), bx1 = CL(dx1, width), by0 = CL(dy0, height), by1 = CL(dy1, height);
 #undef CL
@@ -182,8 +208,31 @@ int64_t rro_place_image(uint8_t *canvas, int32_t width, int32_t height, int32_t 
     if (fabs(det) < 1e-12) return 0;
     const double inv_det = 1.0 / det;
     const double iw = (double)img_w, ih = (double)img_h;
-    if (fabs(b) * iw < 0.5 && fabs(c) * ih < 0.5) return -1;
     int64_t written = 0;
+    if (fabs(b) * iw < 0.5 && fabs(c) * ih < 0.5) {
+        /* axis-aligned path (page/mod.rs:1767-1898): per row v and iy in f64,
+         * per column a Q32 fixed-point image x stepped by img_w / a with
+         * wrapping adds, clamped to the image. */
+        const double fp_scale = 4294967296.0;
+        const int64_t ix_step = sat_f64_i64((iw / a) * fp_scale);
+        const int64_t img_max_x = (int64_t)img_w - 1;
+        for (int64_t dy = by0; dy < by1; ++dy) {
+            const double dy_pdf = page_h - (double)dy;
+            const double v = clamp01((dy_pdf - f) / d);
+            const int64_t iy = (int64_t)fmin((1.0 - v) * ih, ih - 1.0);
+            const int64_t row_base = iy * img_w;
+            const double u0 = clamp01(((double)bx0 - e) / a);
+            int64_t ix_fp = sat_f64_i64(u0 * iw * fp_scale);
+            uint8_t *row = canvas + dy * stride;
+            for (int64_t dx = bx0; dx < bx1; ++dx) {
+                int64_t ix = ix_fp >> 32;
+                ix = ix < 0 ? 0 : (ix > img_max_x ? img_max_x : ix);
+                ix_fp = (int64_t)((uint64_t)ix_fp + (uint64_t)ix_step);
+                written += place_px(row, dx, pixels, row_base + ix, channels, smask, blend, grid, g);
+            }
+        }
+        return written;
+    }
     for (int64_t dy = by0; dy < by1; ++dy) {
         const double dy_pdf = page_h - (double)dy;
         const double dy_rel = dy_pdf - f;
@@ -198,14 +247,7 @@ int64_t rro_place_image(uint8_t *canvas, int32_t width, int32_t height, int32_t 
             v = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
             const int64_t ix = (int64_t)fmin(u * iw, iw - 1.0);
             const int64_t iy = (int64_t)fmin((1.0 - v) * ih, ih - 1.0);
-            const int64_t idx = iy * img_w + ix;
-            const uint32_t alpha = smask ? smask[idx] : 255u;
-            if (alpha == 0) continue;
-            uint8_t s = rro_clut_gray(pixels + idx * channels, channels, grid, g);
-            const uint8_t dst = row[dx];
-            if (blend == 1) s = rro_blend_multiply(s, dst);
-            row[dx] = alpha == 255u ? s : rro_blend_u8(s, dst, alpha);
-            ++written;
+            written += place_px(row, dx, pixels, iy * img_w + ix, channels, smask, blend, grid, g);
         }
     }
     return written;

@@ -133,6 +133,9 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const Tin *in, Tout 
     }
 }
 
+__global__ void k_and_i32(int32_t *p, int32_t keep) { atomicAnd(p, keep); }
+__global__ void k_or_i32(int32_t *p, int32_t bits) { atomicOr(p, bits); }
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_fill(T *__restrict__ p, long long n, T value) {
     for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += static_cast<long long>(gridDim.x) * 256)
@@ -1642,12 +1645,14 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
         RR_TRY(c, hipStreamSynchronize(c->stream));
         const unsigned long long used = k[kCntPoolTop], deferred = k[kCntDeferred];
         c->batch_bins = n_bin;
-        if (c->size_only) {
+        const bool pool_ok = static_cast<long long>(used) <= B.spans_cap;
+        const bool queue_ok = static_cast<long long>(deferred) <= B.ovq_cap;
+        if (c->size_only && pool_ok && queue_ok) {
+            // the sizing pass stored every interval and deferral: its counts
+            // are the whole batch's
             c->prof_b = c->prof_used;
             return RR_OK;
         }
-        const bool pool_ok = static_cast<long long>(used) <= B.spans_cap;
-        const bool queue_ok = static_cast<long long>(deferred) <= B.ovq_cap;
         if (pool_ok && queue_ok) break;
         if (attempt >= 4) return fail(c, RR_E_CAPACITY, "span workspace still too small after growing it");
         // the interval pool or the deferred queue was too small: grow it and
@@ -1659,7 +1664,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
         if (!rc && !queue_ok)
             rc = ensure(c, &c->ovq, &c->ovq_cap,
                         std::max(static_cast<size_t>(deferred + deferred / 4), 2 * static_cast<size_t>(B.ovq_cap)));
-        if (!rc) rc = fill<int32_t>(c, c->err_flag, 1, 0);  // the lost deferrals set bit 32
+        if (!rc) rc = rrint::clear_err_bits(c, 32);  // the lost deferrals (other bits stay)
         if (!rc) rc = fill<unsigned long long>(c, c->counters, kCounters, 0ull);
         if (!rc) rc = fill<unsigned long long>(c, c->queues, kQueueWords, 0ull);
         if (!rc) rc = fill<unsigned long long>(c, c->spill_top, 1, 0ull);
@@ -1670,7 +1675,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
         B.spans_cap = static_cast<long long>(c->spans_cap);
         B.ovq = c->ovq;
         B.ovq_cap = static_cast<long long>(c->ovq_cap);
-        mode = SPAN_ALL_NODRAW;
+        mode = c->size_only ? SPAN_ALL_NOSKIP : SPAN_ALL_NODRAW;
         c->batch_reruns = attempt + 1;
     }
     rc = ensure(c, &c->bins, &c->bins_cap, static_cast<size_t>(std::max<long long>(c->batch_bins, 1)));
@@ -1687,6 +1692,17 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
 int32_t zero_u64(rr_hip_ctx *c, unsigned long long *p, long long n) { return fill<unsigned long long>(c, p, n, 0ull); }
 int32_t fill_u64(rr_hip_ctx *c, unsigned long long *p, long long n, unsigned long long value) {
     return fill<unsigned long long>(c, p, n, value);
+}
+int32_t fill_i32(rr_hip_ctx *c, int32_t *p, long long n, int32_t value) { return fill<int32_t>(c, p, n, value); }
+int32_t set_err_bits(rr_hip_ctx *c, int32_t bits) {
+    hipLaunchKernelGGL(k_or_i32, dim3(1), dim3(1), 0, c->stream, c->err_flag, bits);
+    RR_TRY(c, hipGetLastError());
+    return RR_OK;
+}
+int32_t clear_err_bits(rr_hip_ctx *c, int32_t mask) {
+    hipLaunchKernelGGL(k_and_i32, dim3(1), dim3(1), 0, c->stream, c->err_flag, ~mask);
+    RR_TRY(c, hipGetLastError());
+    return RR_OK;
 }
 
 }  // namespace rrint
@@ -1737,8 +1753,11 @@ extern "C" int32_t rr_hip_graph_create(rr_hip_ctx *c, const rr_fill_pages *b, rr
         // every interval plus each wave's unused chunk tail (main and wide
         // passes); every band the main pass can defer (nothing is skipped in
         // the sizing pass, so replays defer no more)
+        // (the sizing pass grew the pool and ran again until everything fit,
+        // so the stored count is complete; the pool top bounds it too)
         const size_t waves = static_cast<size_t>(c->span_waves[t.variant]) + static_cast<size_t>(c->wide_waves);
-        const size_t need = static_cast<size_t>(k[kCntStored]) + waves * static_cast<size_t>(kSpanChunk) + 1024;
+        const size_t need = std::max(static_cast<size_t>(k[kCntStored]) + waves * static_cast<size_t>(kSpanChunk),
+                                     static_cast<size_t>(k[kCntPoolTop])) + 1024;
         rc = rrint::ensure(c, &c->spans, &c->spans_cap, need);
         if (!rc) rc = rrint::ensure(c, &c->ovq, &c->ovq_cap, static_cast<size_t>(k[kCntDeferred]) + 1024);
         if (!rc) rc = rrint::ensure(c, &c->bins, &c->bins_cap, static_cast<size_t>(c->batch_bins) + 1024);

@@ -783,14 +783,14 @@ int32_t rr_hip_create(int32_t device, rr_hip_ctx **out) {
     if (rc == RR_OK && hipMalloc(&c->spill_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->counters, kCounters * sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->queues, kQueueWords * sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
-    if (rc == RR_OK && hipMalloc(&c->err_flag, sizeof(int32_t)) != hipSuccess) rc = RR_E_DEVICE;
+    if (rc == RR_OK && hipMalloc(&c->err_flag, (1 + kFenceSlots) * sizeof(int32_t)) != hipSuccess) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->trace_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
     for (auto &f : c->fences)
         if (rc == RR_OK && hipEventCreateWithFlags(&f.ev, hipEventDisableTiming) != hipSuccess) rc = RR_E_DEVICE;
     const long long pool_ints = 64ll << 20;  // 256 MiB shared pool for rows beyond a wave's slot
     if (rc == RR_OK && hipMalloc(&c->spill, pool_ints * sizeof(int32_t)) != hipSuccess) rc = RR_E_DEVICE;
     c->spill_cap = pool_ints;
-    if (rc == RR_OK && hipMemset(c->err_flag, 0, sizeof(int32_t)) != hipSuccess) rc = RR_E_DEVICE;
+    if (rc == RR_OK && hipMemset(c->err_flag, 0, (1 + kFenceSlots) * sizeof(int32_t)) != hipSuccess) rc = RR_E_DEVICE;
     if (rc != RR_OK) {
         rr_hip_destroy(c);
         return rc;
@@ -834,6 +834,7 @@ int32_t rr_hip_alloc(rr_hip_ctx *c, size_t size, int32_t zeroed, void **buf) {
     RR_HIP_TRY(c, hipSetDevice(c->device));
     RR_HIP_TRY(c, hipMalloc(buf, size));
     if (zeroed) RR_HIP_TRY(c, hipMemsetAsync(*buf, 0, size, c->stream));
+    c->buf_len[*buf] = size;
     return RR_OK;
 }
 
@@ -841,6 +842,42 @@ int32_t rr_hip_free(rr_hip_ctx *c, void *buf) {
     if (!c) return RR_E_INVARIANT;
     RR_HIP_TRY(c, hipStreamSynchronize(c->stream));
     RR_HIP_TRY(c, hipFree(buf));
+    c->buf_len.erase(buf);
+    return RR_OK;
+}
+
+int32_t rr_hip_buffer_len(rr_hip_ctx *c, const void *buf, size_t *len) {
+    if (!c || !len) return RR_E_INVARIANT;
+    const auto it = c->buf_len.find(buf);
+    if (it == c->buf_len.end()) return fail(c, RR_E_INVARIANT, "buffer_len: not a buffer from rr_hip_alloc");
+    *len = it->second;
+    return RR_OK;
+}
+
+int32_t rr_hip_vram_budget(rr_hip_ctx *c, uint64_t *total_bytes, uint64_t *usable_bytes) {
+    if (!c || !total_bytes || !usable_bytes) return RR_E_INVARIANT;
+    RR_HIP_TRY(c, hipSetDevice(c->device));
+    size_t free_b = 0, total_b = 0;
+    RR_HIP_TRY(c, hipMemGetInfo(&free_b, &total_b));
+    *total_bytes = total_b;
+    *usable_bytes = static_cast<uint64_t>(free_b) / 4 * 3 + static_cast<uint64_t>(free_b) % 4 * 3 / 4;
+    return RR_OK;
+}
+
+int32_t rr_hip_alloc_host_pinned(rr_hip_ctx *c, size_t size, void **buf) {
+    if (!c || !buf) return RR_E_INVARIANT;
+    *buf = nullptr;
+    if (size == 0) return fail(c, RR_E_ZERO_SIZE_ALLOC, "alloc_host_pinned called with size = 0");
+    RR_HIP_TRY(c, hipSetDevice(c->device));
+    RR_HIP_TRY(c, hipHostMalloc(buf, size, hipHostMallocDefault));
+    return RR_OK;
+}
+
+int32_t rr_hip_free_host_pinned(rr_hip_ctx *c, void *buf) {
+    if (!c) return RR_E_INVARIANT;
+    if (!buf) return RR_OK;
+    RR_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    RR_HIP_TRY(c, hipHostFree(buf));
     return RR_OK;
 }
 
@@ -873,12 +910,21 @@ int32_t rr_hip_memset(rr_hip_ctx *c, void *dst, int32_t value, size_t n) {
 
 int32_t rr_hip_begin_page(rr_hip_ctx *c) {
     if (!c) return RR_E_INVARIANT;
+    RR_HIP_TRY(c, hipSetDevice(c->device));
+    const int32_t rc = rrint::fill_i32(c, c->err_flag, 1, 0);  // the live word; fence slots keep theirs
+    if (rc) return rc;
     c->page_open = true;
     c->page_timed = false;
     c->fill_launches = 0;
     c->prof_used = 0;
-    RR_HIP_TRY(c, hipMemsetAsync(c->err_flag, 0, sizeof(int32_t), c->stream));
     return RR_OK;
+}
+
+int32_t rr_hip_inject_error(rr_hip_ctx *c, int32_t bits) {
+    if (!c || bits == 0) return RR_E_INVARIANT;
+    if (!c->page_open) return fail(c, RR_E_INVARIANT, "inject_error outside begin_page/submit_page");
+    RR_HIP_TRY(c, hipSetDevice(c->device));
+    return rrint::set_err_bits(c, bits);
 }
 
 int32_t rr_hip_record_fill_pages(rr_hip_ctx *c, const rr_fill_pages *b) {
@@ -918,25 +964,47 @@ int32_t rr_hip_submit_page(rr_hip_ctx *c, uint64_t *fence) {
     if (!c || !fence) return RR_E_INVARIANT;
     if (!c->page_open) return fail(c, RR_E_INVARIANT, "submit_page without begin_page");
     if (c->page_timed) RR_HIP_TRY(c, hipEventRecord(c->t1, c->stream));
-    Fence &f = c->fences[c->next_fence % 16];
+    const int slot = static_cast<int>(c->next_fence % kFenceSlots);
+    Fence &f = c->fences[slot];
+    // the page's error word goes with its fence: the next begin_page clears
+    // only the live word
+    RR_HIP_TRY(c, hipMemcpyAsync(c->err_flag + 1 + slot, c->err_flag, sizeof(int32_t), hipMemcpyDeviceToDevice,
+                                 c->stream));
     RR_HIP_TRY(c, hipEventRecord(f.ev, c->stream));
     f.id = c->next_fence;
+    f.page = true;
     *fence = c->next_fence++;
     c->page_open = false;
     c->last_launches = c->fill_launches;
     return RR_OK;
 }
 
-int32_t rr_hip_wait_page(rr_hip_ctx *c, uint64_t fence) {
-    if (!c) return RR_E_INVARIANT;
-    Fence &f = c->fences[fence % 16];
+// Blocks until fence f (a page or transfer fence of this context) has
+// signalled.  *slot receives its ring slot, or -1 for a fence older than the
+// ring (complete, but its page's error word is no longer held).
+static int32_t wait_fence(rr_hip_ctx *c, uint64_t fence, int *slot) {
+    if (fence == 0 || fence >= c->next_fence) return fail(c, RR_E_INVARIANT, "wait: unknown fence");
+    RR_HIP_TRY(c, hipSetDevice(c->device));
+    Fence &f = c->fences[fence % kFenceSlots];
     if (f.id != fence) {
         RR_HIP_TRY(c, hipStreamSynchronize(c->stream));
+        *slot = -1;
     } else {
         RR_HIP_TRY(c, hipEventSynchronize(f.ev));
+        *slot = static_cast<int>(fence % kFenceSlots);
     }
+    return RR_OK;
+}
+
+int32_t rr_hip_wait_page(rr_hip_ctx *c, uint64_t fence) {
+    if (!c) return RR_E_INVARIANT;
+    int slot = -1;
+    const int32_t wrc = wait_fence(c, fence, &slot);
+    if (wrc) return wrc;
+    if (slot < 0) return fail(c, RR_E_INVARIANT, "wait_page: fence older than the ring; its page status is gone");
+    if (!c->fences[slot].page) return fail(c, RR_E_INVARIANT, "wait_page: a transfer fence");
     int32_t flag = 0;
-    RR_HIP_TRY(c, hipMemcpy(&flag, c->err_flag, sizeof(int32_t), hipMemcpyDeviceToHost));
+    RR_HIP_TRY(c, hipMemcpy(&flag, c->err_flag + 1 + slot, sizeof(int32_t), hipMemcpyDeviceToHost));
     if (c->page_timed) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, c->t0, c->t1) == hipSuccess) c->last_ms = ms;
@@ -950,6 +1018,47 @@ int32_t rr_hip_wait_page(rr_hip_ctx *c, uint64_t fence) {
     if (flag & 1) return fail(c, RR_E_CAPACITY, "overflow-entry pool exhausted: page output is incomplete");
     if (flag) return fail(c, RR_E_CAPACITY, "batched workspace overflow (code " + std::to_string(flag) + "): page output is incomplete");
     return RR_OK;
+}
+
+// Transfer fence: an event on the context stream after everything recorded
+// so far (uploads, downloads and pages are ordered on that one stream).
+static int32_t transfer_fence(rr_hip_ctx *c, uint64_t *fence) {
+    Fence &f = c->fences[c->next_fence % kFenceSlots];
+    RR_HIP_TRY(c, hipEventRecord(f.ev, c->stream));
+    f.id = c->next_fence;
+    f.page = false;
+    *fence = c->next_fence++;
+    return RR_OK;
+}
+
+int32_t rr_hip_upload_async(rr_hip_ctx *c, void *dst, const void *src, size_t n, uint64_t *fence) {
+    if (!c || !fence || (n && (!dst || !src))) return RR_E_INVARIANT;
+    const auto it = c->buf_len.find(dst);
+    if (it != c->buf_len.end() && n > it->second) return fail(c, RR_E_INVARIANT, "upload_async: source longer than buffer");
+    RR_HIP_TRY(c, hipSetDevice(c->device));
+    if (n) RR_HIP_TRY(c, hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream));
+    return transfer_fence(c, fence);
+}
+
+int32_t rr_hip_download_async(rr_hip_ctx *c, void *dst, const void *src, size_t n, uint64_t *fence) {
+    if (!c || !fence || (n && (!dst || !src))) return RR_E_INVARIANT;
+    const auto it = c->buf_len.find(src);
+    if (it != c->buf_len.end() && n > it->second) return fail(c, RR_E_INVARIANT, "download_async: longer than buffer");
+    RR_HIP_TRY(c, hipSetDevice(c->device));
+    if (n) RR_HIP_TRY(c, hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, c->stream));
+    return transfer_fence(c, fence);
+}
+
+int32_t rr_hip_submit_transfer(rr_hip_ctx *c, uint64_t *fence) {
+    if (!c || !fence) return RR_E_INVARIANT;
+    RR_HIP_TRY(c, hipSetDevice(c->device));
+    return transfer_fence(c, fence);
+}
+
+int32_t rr_hip_wait_transfer(rr_hip_ctx *c, uint64_t fence) {
+    if (!c) return RR_E_INVARIANT;
+    int slot = -1;
+    return wait_fence(c, fence, &slot);
 }
 
 int32_t rr_hip_variant_info(int32_t variant, int32_t *lds_entries, int32_t *chunk_px) {

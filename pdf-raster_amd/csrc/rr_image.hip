@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <vector>
 
 #include "rr_hip.h"
 #include "rr_internal.h"
@@ -45,6 +46,10 @@ struct ImgArgs {
     int64_t n_img_px;          // width * height of the image
     const uint16_t *grey;      // grey | alpha << 8 per image pixel, when converted ahead
     uint16_t *grey_out;        // where k_image_grey writes them
+    // axis-aligned path (|b| w < 0.5 and |c| h < 0.5): image x of canvas column
+    // bx0 + k is (ix_fp0 + k * ix_step) >> 32 in Q32, wrapping, clamped
+    int32_t axis;
+    int64_t ix_fp0, ix_step;
 };
 
 __device__ __forceinline__ uint32_t div255(uint32_t x) {
@@ -203,25 +208,44 @@ template <int D, bool GREY>
 __device__ __forceinline__ void place_dword(const ImgArgs &A, const uint16_t *lgrid, int32_t x0, int32_t dy) {
     const double dy_pdf = A.page_h - static_cast<double>(dy);
     const double dy_rel = dy_pdf - A.f;
-    const double u_row = (-A.c * dy_rel) * A.inv_det;
-    const double v_row = (A.a * dy_rel) * A.inv_det;
     int64_t idx[4];
     bool in[4];
+    if (A.axis) {
+        // page/mod.rs:1811-1829: v and iy per row in f64; the Q32 column
+        // stepping of :1819-1833 in closed form (wrapping, as the reference's
+        // wrapping_add per column)
+        double v = dy_rel / A.d;
+        v = fmin(fmax(v, 0.0), 1.0);  // a NaN v ends as iy = ih - 1 (f64::min drops the NaN)
+        const int64_t row = static_cast<int64_t>(static_cast<int32_t>(fmin((1.0 - v) * A.ih, A.ih - 1.0))) * A.img_w;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int32_t dx = x0 + k;
-        in[k] = dx >= A.bx0 && dx < A.bx1;
-        const double dx_rel = static_cast<double>(dx) - A.e;
-        double u = fma(A.d * dx_rel, A.inv_det, u_row);
-        double v = fma((-A.b) * dx_rel, A.inv_det, v_row);
-        // clamps to [0, 1]; a NaN u or v (from overflowing products) ends as
-        // ix = iw - 1, iy = ih - 1, as it does when the clamp keeps the NaN
-        u = fmax(fmin(u, 1.0), 0.0);
-        v = fmin(fmax(v, 0.0), 1.0);
-        const int32_t ix = static_cast<int32_t>(fmin(u * A.iw, A.iw - 1.0));
-        const int32_t iy = static_cast<int32_t>(fmin((1.0 - v) * A.ih, A.ih - 1.0));
-        // a pixel outside the bbox reads image pixel 0 and is discarded
-        idx[k] = in[k] ? static_cast<int64_t>(iy) * A.img_w + ix : 0;
+        for (int k = 0; k < 4; ++k) {
+            const int32_t dx = x0 + k;
+            in[k] = dx >= A.bx0 && dx < A.bx1;
+            const uint64_t fp = static_cast<uint64_t>(A.ix_fp0) +
+                                static_cast<uint64_t>(static_cast<int64_t>(dx - A.bx0)) * static_cast<uint64_t>(A.ix_step);
+            int64_t ix = static_cast<int64_t>(fp) >> 32;
+            ix = ix < 0 ? 0 : (ix > A.img_w - 1 ? A.img_w - 1 : ix);
+            idx[k] = in[k] ? row + ix : 0;
+        }
+    } else {
+        const double u_row = (-A.c * dy_rel) * A.inv_det;
+        const double v_row = (A.a * dy_rel) * A.inv_det;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int32_t dx = x0 + k;
+            in[k] = dx >= A.bx0 && dx < A.bx1;
+            const double dx_rel = static_cast<double>(dx) - A.e;
+            double u = fma(A.d * dx_rel, A.inv_det, u_row);
+            double v = fma((-A.b) * dx_rel, A.inv_det, v_row);
+            // clamps to [0, 1]; a NaN u or v (from overflowing products) ends as
+            // ix = iw - 1, iy = ih - 1, as it does when the clamp keeps the NaN
+            u = fmax(fmin(u, 1.0), 0.0);
+            v = fmin(fmax(v, 0.0), 1.0);
+            const int32_t ix = static_cast<int32_t>(fmin(u * A.iw, A.iw - 1.0));
+            const int32_t iy = static_cast<int32_t>(fmin((1.0 - v) * A.ih, A.ih - 1.0));
+            // a pixel outside the bbox reads image pixel 0 and is discarded
+            idx[k] = in[k] ? static_cast<int64_t>(iy) * A.img_w + ix : 0;
+        }
     }
     uint32_t s[4], alpha[4];
     if constexpr (GREY) {
@@ -453,6 +477,14 @@ __global__ void __launch_bounds__(256) k_rgb_to_gray(const uint8_t *__restrict__
         if (e_ != hipSuccess) return rrint::fail(ctx, RR_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// f64 -> i64 as Rust's `as i64` converts: truncation, saturating, NaN -> 0.
+int64_t sat_i64(double x) {
+    if (std::isnan(x)) return 0;
+    if (x >= 9223372036854775807.0) return INT64_MAX;
+    if (x <= -9223372036854775808.0) return INT64_MIN;
+    return static_cast<int64_t>(x);
+}
+
 unsigned grid_for(int64_t n) {
     const int64_t blocks = (n + 255) / 256;
     return static_cast<unsigned>(blocks < 65536 ? (blocks > 0 ? blocks : 1) : 65536);
@@ -483,6 +515,11 @@ int32_t rr_hip_record_place_images(rr_hip_ctx *c, const rr_image_batch *b) {
     RR_TRY(c, hipSetDevice(rrint::device(c)));
     int lds_max = 0;
     RR_TRY(c, hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, rrint::device(c)));
+    // Pass 1: every descriptor is checked and its placement set up before
+    // anything is launched, so a rejected batch leaves every canvas as it was
+    // (a caller falling back to the CPU then applies each image exactly once).
+    std::vector<ImgArgs> plan;
+    plan.reserve(static_cast<size_t>(b->n_images));
     for (int32_t i = 0; i < b->n_images; ++i) {
         const rr_image_desc &im = b->images[i];
         if (im.page < 0 || im.page >= b->n_pages) return rrint::fail(c, RR_E_INVARIANT, "image: bad page index");
@@ -515,17 +552,15 @@ int32_t rr_hip_record_place_images(rr_hip_ctx *c, const rr_image_batch *b) {
         const double ymn = std::fmin(std::fmin(page_h - ys[0], page_h - ys[1]), std::fmin(page_h - ys[2], page_h - ys[3]));
         const double ymx = std::fmax(std::fmax(page_h - ys[0], page_h - ys[1]), std::fmax(page_h - ys[2], page_h - ys[3]));
         auto clampi = [](int64_t v, int64_t hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
-        const int64_t bx0 = clampi(static_cast<int64_t>(std::floor(xmn)), pg.width);
-        const int64_t bx1 = clampi(static_cast<int64_t>(std::ceil(xmx)), pg.width);
-        const int64_t by0 = clampi(static_cast<int64_t>(std::floor(ymn)), pg.height);
-        const int64_t by1 = clampi(static_cast<int64_t>(std::ceil(ymx)), pg.height);
+        const int64_t bx0 = clampi(sat_i64(std::floor(xmn)), pg.width);
+        const int64_t bx1 = clampi(sat_i64(std::ceil(xmx)), pg.width);
+        const int64_t by0 = clampi(sat_i64(std::floor(ymn)), pg.height);
+        const int64_t by1 = clampi(sat_i64(std::ceil(ymx)), pg.height);
         if (bx0 >= bx1 || by0 >= by1) continue;
         const double det = std::fma(m[0], m[3], -(m[1] * m[2]));
         if (std::fabs(det) < 1e-12) continue;
         const double iw = static_cast<double>(im.width), ih = static_cast<double>(im.height);
-        if (std::fabs(m[1]) * iw < 0.5 && std::fabs(m[2]) * ih < 0.5)
-            return rrint::fail(c, RR_E_INVARIANT, "image: CTM takes the axis-aligned path (not supported)");
-        ImgArgs A;
+        ImgArgs A{};
         A.pixels = im.pixels;
         A.smask = im.smask;
         A.canvas = pg.canvas;
@@ -549,6 +584,16 @@ int32_t rr_hip_record_place_images(rr_hip_ctx *c, const rr_image_batch *b) {
         A.page_h = page_h;
         A.iw = iw;
         A.ih = ih;
+        // axis-aligned path (page/mod.rs:1767-1809): the Q32 column step
+        // img_w / a and the start column's image x, as Rust's `as i64`
+        A.axis = std::fabs(m[1]) * iw < 0.5 && std::fabs(m[2]) * ih < 0.5;
+        if (A.axis) {
+            const double fp_scale = 4294967296.0;
+            A.ix_step = sat_i64((iw / m[0]) * fp_scale);
+            double u0 = (static_cast<double>(bx0) - m[4]) / m[0];
+            u0 = u0 < 0.0 ? 0.0 : (u0 > 1.0 ? 1.0 : u0);  // f64::clamp keeps a NaN
+            A.ix_fp0 = sat_i64(u0 * iw * fp_scale);
+        }
         const int32_t xs0 = A.bx0 & ~3;
         A.tiles_x = (A.bx1 - xs0 + kImgTileW - 1) / kImgTileW;
         A.n_tiles = A.tiles_x * ((A.by1 - A.by0 + kImgTileH - 1) / kImgTileH);
@@ -559,11 +604,15 @@ int32_t rr_hip_record_place_images(rr_hip_ctx *c, const rr_image_batch *b) {
         if (lds_nodes < 8) return rrint::fail(c, RR_E_DEVICE, "image: device has no LDS for the grid");
         A.lds_nodes = static_cast<uint32_t>(lds_nodes);
         A.last_node = static_cast<uint32_t>(nodes - 1);
-        const size_t lds_bytes = static_cast<size_t>(lds_nodes) * 2;
+        A.n_img_px = static_cast<int64_t>(im.width) * im.height;
+        plan.push_back(A);
+    }
+    // Pass 2: the launches, in image order.
+    for (ImgArgs &A : plan) {
+        const size_t lds_bytes = static_cast<size_t>(A.lds_nodes) * 2;
         // an image smaller than 0.8 of its canvas footprint is converted to
         // grey + alpha once per image pixel, so that the placement samples
         // those instead of running the grid interpolation per canvas pixel
-        A.n_img_px = static_cast<int64_t>(im.width) * im.height;
         A.grey = nullptr;
         A.grey_out = nullptr;
         const int64_t n_box = static_cast<int64_t>(A.bx1 - A.bx0) * (A.by1 - A.by0);
@@ -581,12 +630,12 @@ int32_t rr_hip_record_place_images(rr_hip_ctx *c, const rr_image_batch *b) {
             rc = rrint::ensure(c, &c->img_grey, &c->img_grey_cap, static_cast<size_t>(A.n_img_px));
             if (rc) return rc;
             A.grey_out = c->img_grey;
-            const void *kg = im.channels == 4 ? reinterpret_cast<const void *>(&k_image_grey<4>)
-                                              : reinterpret_cast<const void *>(&k_image_grey<3>);
+            const void *kg = A.channels == 4 ? reinterpret_cast<const void *>(&k_image_grey<4>)
+                                             : reinterpret_cast<const void *>(&k_image_grey<3>);
             rc = blocks_of(kg, lds_bytes, (A.n_img_px + 4 * kImgThreads - 1) / (4 * kImgThreads), &blocks);
             if (rc) return rc;
             rc = launch_profiled(c, [&] {
-                if (im.channels == 4)
+                if (A.channels == 4)
                     hipLaunchKernelGGL(k_image_grey<4>, dim3(static_cast<unsigned>(blocks)), dim3(kImgThreads),
                                        lds_bytes, rrint::stream(c), A);
                 else
@@ -602,12 +651,12 @@ int32_t rr_hip_record_place_images(rr_hip_ctx *c, const rr_image_batch *b) {
                                    rrint::stream(c), A);
             });
         } else {
-            const void *kfn = im.channels == 4 ? reinterpret_cast<const void *>(&k_place_image<4, false>)
-                                               : reinterpret_cast<const void *>(&k_place_image<3, false>);
+            const void *kfn = A.channels == 4 ? reinterpret_cast<const void *>(&k_place_image<4, false>)
+                                              : reinterpret_cast<const void *>(&k_place_image<3, false>);
             rc = blocks_of(kfn, lds_bytes, A.n_tiles, &blocks);
             if (rc) return rc;
             rc = launch_profiled(c, [&] {
-                if (im.channels == 4)
+                if (A.channels == 4)
                     hipLaunchKernelGGL((k_place_image<4, false>), dim3(static_cast<unsigned>(blocks)), dim3(kImgThreads),
                                        lds_bytes, rrint::stream(c), A);
                 else
@@ -620,13 +669,26 @@ int32_t rr_hip_record_place_images(rr_hip_ctx *c, const rr_image_batch *b) {
     return RR_OK;
 }
 
-int32_t rr_hip_record_icc_clut(rr_hip_ctx *c, const uint8_t *cmyk, uint8_t *rgb, const uint8_t *clut, int32_t grid_n,
+int32_t rr_hip_record_icc_clut(rr_hip_ctx *c, const uint8_t *cmyk, uint8_t *rgb, const uint8_t *clut, size_t clut_len,
                                int64_t n_pixels) {
     if (!c) return RR_E_INVARIANT;
     if (!rrint::page_open(c)) return rrint::fail(c, RR_E_INVARIANT, "record outside begin_page/submit_page");
     if (n_pixels < 0 || (n_pixels > 0 && (!cmyk || !rgb)) || (reinterpret_cast<uintptr_t>(cmyk) & 3u))
         return rrint::fail(c, RR_E_INVARIANT, "icc_clut: bad buffers (cmyk must be 4-byte aligned)");
-    if (clut && grid_n < 2) return rrint::fail(c, RR_E_INVARIANT, "icc_clut: grid_n must be >= 2");
+    // grid_n_from_clut_len (crates/gpu/src/backend/params.rs:127-147): the
+    // table must be exactly grid_n^4 RGB nodes with 2 <= grid_n <= 255
+    int32_t grid_n = 0;
+    if (clut) {
+        if (clut_len % 3 == 0) {
+            const size_t nodes = clut_len / 3;
+            for (size_t g = 2; g <= 255; ++g) {
+                const size_t p4 = g * g * g * g;
+                if (p4 == nodes) grid_n = static_cast<int32_t>(g);
+                if (p4 >= nodes) break;
+            }
+        }
+        if (grid_n == 0) return rrint::fail(c, RR_E_INVARIANT, "icc_clut: table length is not grid_n^4 * 3 (2 <= grid_n <= 255)");
+    }
     if (n_pixels == 0) return RR_OK;
     RR_TRY(c, hipSetDevice(rrint::device(c)));
     if (!clut)

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "rr_device.h"
@@ -42,9 +43,14 @@ constexpr int kVariantK[kNumVariants] = {4, 4, 4, 4, 8, 8, 8, 8, 16, 16, 16, 16,
 constexpr int kVariantCH[kNumVariants] = {64, 128, 256, 512, 64, 128, 256, 512,
                                           64, 128, 256, 512, 64, 128, 256, 512};
 
+// One slot of the fence ring (kFenceSlots deep).  A page fence carries the
+// error word of its page: submit_page copies the live word into err_flag[1 +
+// slot] ahead of the event, so a later page's begin_page cannot erase it.
+constexpr int kFenceSlots = 16;
 struct Fence {
     hipEvent_t ev = nullptr;
     uint64_t id = 0;
+    bool page = false;  // a page fence (else a transfer fence: no error word)
 };
 
 // Spans of one (path, 16-row band) item written by the batched span pass, in
@@ -120,7 +126,7 @@ struct rr_hip_ctx {
     int32_t *spill = nullptr;
     long long spill_cap = 0;
     unsigned long long *spill_top = nullptr;
-    int32_t *err_flag = nullptr;
+    int32_t *err_flag = nullptr;  // [0] live word of the open page, [1 + k] page fence slot k's copy
     // batched two-phase workspace
     long long *item_base = nullptr;  // per path: first span item (n_paths + 1)
     size_t item_base_cap = 0;
@@ -223,7 +229,9 @@ struct rr_hip_ctx {
     bool page_open = false, page_timed = false;
     double last_ms = 0.0;
     int64_t last_launches = 0;
-    rrdev::Fence fences[16];
+    rrdev::Fence fences[rrdev::kFenceSlots];
+    // sizes of the buffers rr_hip_alloc handed out (device_buffer_len)
+    std::unordered_map<const void *, size_t> buf_len;
     uint64_t next_fence = 1;
     // per-launch profiling
     bool profiling = false;
@@ -283,6 +291,11 @@ int32_t scan_counts(rr_hip_ctx *c, const int32_t *in, long long *out, long long 
 // Zero n 64-bit words with a kernel (safe inside graph capture).
 int32_t zero_u64(rr_hip_ctx *c, unsigned long long *p, long long n);
 int32_t fill_u64(rr_hip_ctx *c, unsigned long long *p, long long n, unsigned long long value);
+int32_t fill_i32(rr_hip_ctx *c, int32_t *p, long long n, int32_t value);
+// Clears the bits of `mask` in the context's live error word, keeping the
+// others (a kernel, so that it is ordered on the stream and capturable).
+int32_t clear_err_bits(rr_hip_ctx *c, int32_t mask);
+int32_t set_err_bits(rr_hip_ctx *c, int32_t bits);
 // Records the edge setup + fill of b (rr_fill.hip); used by graph capture.
 int32_t record_pages_internal(rr_hip_ctx *c, const rr_fill_pages &b);
 }  // namespace rrint

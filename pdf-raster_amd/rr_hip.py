@@ -101,11 +101,20 @@ def lib():
             "rr_hip_download": (i32, [P, P, P, ctypes.c_size_t]),
             "rr_hip_memset": (i32, [P, P, i32, ctypes.c_size_t]),
             "rr_hip_copy": (i32, [P, P, P, ctypes.c_size_t]),
+            "rr_hip_buffer_len": (i32, [P, P, ctypes.POINTER(ctypes.c_size_t)]),
+            "rr_hip_alloc_host_pinned": (i32, [P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+            "rr_hip_free_host_pinned": (i32, [P, P]),
+            "rr_hip_vram_budget": (i32, [P, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+            "rr_hip_upload_async": (i32, [P, P, P, ctypes.c_size_t, ctypes.POINTER(u64)]),
+            "rr_hip_download_async": (i32, [P, P, P, ctypes.c_size_t, ctypes.POINTER(u64)]),
+            "rr_hip_submit_transfer": (i32, [P, ctypes.POINTER(u64)]),
+            "rr_hip_wait_transfer": (i32, [P, u64]),
             "rr_hip_begin_page": (i32, [P]),
             "rr_hip_record_fill_paths": (i32, [P, ctypes.POINTER(FillBatch)]),
             "rr_hip_record_fill_pages": (i32, [P, ctypes.POINTER(FillPages)]),
             "rr_hip_submit_page": (i32, [P, ctypes.POINTER(u64)]),
             "rr_hip_wait_page": (i32, [P, u64]),
+            "rr_hip_inject_error": (i32, [P, i32]),
             "rr_hip_set_profiling": (i32, [P, i32]),
             "rr_hip_set_trace": (i32, [P, P, i64]),
             "rr_hip_variant_count": (i32, []),
@@ -127,7 +136,7 @@ def lib():
             "rr_workload_page_info": (i32, [P, i32, P]),
             "rr_workload_page_data": (i32, [P, i32] + [ctypes.POINTER(ctypes.c_void_p)] * 7),
             "rr_hip_record_place_images": (i32, [P, ctypes.POINTER(ImageBatch)]),
-            "rr_hip_record_icc_clut": (i32, [P, P, P, P, i32, i64]),
+            "rr_hip_record_icc_clut": (i32, [P, P, P, P, ctypes.c_size_t, i64]),
             "rr_hip_record_composite": (i32, [P, P, P, i64]),
             "rr_hip_record_apply_soft_mask": (i32, [P, P, P, i64]),
             "rr_hip_record_rgb_to_gray": (i32, [P, P, i64, P, i64, i32, i32]),
@@ -371,6 +380,44 @@ class Backend:
             self.upload(buf, arr)
         return buf
 
+    def buffer_len(self, buf):
+        n = ctypes.c_size_t()
+        self._check(self.L.rr_hip_buffer_len(self.h, buf, ctypes.byref(n)), "buffer_len")
+        return n.value
+
+    def vram_budget(self):
+        t, u = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self.L.rr_hip_vram_budget(self.h, ctypes.byref(t), ctypes.byref(u)), "vram_budget")
+        return t.value, u.value
+
+    def alloc_host_pinned(self, nbytes):
+        """A page-locked host buffer of nbytes, as a numpy uint8 view (free with free_host_pinned)."""
+        b = ctypes.c_void_p()
+        self._check(self.L.rr_hip_alloc_host_pinned(self.h, nbytes, ctypes.byref(b)), "alloc_host_pinned")
+        arr = np.ctypeslib.as_array((ctypes.c_uint8 * nbytes).from_address(b.value))
+        return b.value, arr
+
+    def free_host_pinned(self, ptr):
+        self._check(self.L.rr_hip_free_host_pinned(self.h, ptr), "free_host_pinned")
+
+    def upload_async(self, dst, src_ptr, nbytes):
+        f = ctypes.c_uint64()
+        self._check(self.L.rr_hip_upload_async(self.h, dst, src_ptr, nbytes, ctypes.byref(f)), "upload_async")
+        return f.value
+
+    def download_async(self, dst_ptr, src, nbytes):
+        f = ctypes.c_uint64()
+        self._check(self.L.rr_hip_download_async(self.h, dst_ptr, src, nbytes, ctypes.byref(f)), "download_async")
+        return f.value
+
+    def submit_transfer(self):
+        f = ctypes.c_uint64()
+        self._check(self.L.rr_hip_submit_transfer(self.h, ctypes.byref(f)), "submit_transfer")
+        return f.value
+
+    def wait_transfer(self, fence):
+        self._check(self.L.rr_hip_wait_transfer(self.h, fence), "wait_transfer")
+
     def copy(self, dst, src, nbytes):
         self._check(self.L.rr_hip_copy(self.h, dst, src, nbytes), "copy")
 
@@ -393,8 +440,8 @@ class Backend:
         self._check(self.L.rr_hip_record_place_images(self.h, ctypes.byref(batch)),
                     "record_place_images")
 
-    def record_icc_clut(self, cmyk, rgb, clut, grid_n, n_pixels):
-        self._check(self.L.rr_hip_record_icc_clut(self.h, cmyk, rgb, clut, grid_n, n_pixels),
+    def record_icc_clut(self, cmyk, rgb, clut, clut_len, n_pixels):
+        self._check(self.L.rr_hip_record_icc_clut(self.h, cmyk, rgb, clut, clut_len, n_pixels),
                     "record_icc_clut")
 
     def record_composite(self, src, dst, n_pixels):
@@ -481,6 +528,9 @@ class Backend:
 
     def record_scan(self, data, block_sums, n):
         self._check(self.L.rr_hip_record_scan(self.h, data, block_sums, n), "record_scan")
+
+    def inject_error(self, bits=1):
+        self._check(self.L.rr_hip_inject_error(self.h, bits), "inject_error")
 
     def submit_page(self):
         f = ctypes.c_uint64(0)

@@ -55,3 +55,27 @@ def test_graph_refuses_after_workspace_growth():
         db.free()
     finally:
         be.close()
+
+
+def test_capture_first_on_a_fresh_context_with_occlusion():
+    # The capture is the context's first recording: its sizing pass must grow
+    # the interval pool itself (whole mixed pages need more than the initial
+    # pool) so that replays, occlusion skipping on (no coverage outputs), fit
+    # and match the oracle on every replay.
+    from test_gpu_batched import oracle_pages
+    pages = R.workload(3, 0xC0F10003, 2)
+    oc, _ = oracle_pages(pages)
+    be = R.Backend(0)
+    db = R.DeviceBatch(be, pages)
+    g = None
+    try:
+        g = db.capture(hashes=False, variant=0)
+        for rep in range(3):
+            db.run_graph(g)
+            for i, o in enumerate(oc):
+                assert np.array_equal(db.read_canvas(i), o), f"replay {rep} page {i}"
+    finally:
+        if g:
+            db.destroy_graph(g)
+        db.free()
+        be.close()

@@ -147,13 +147,56 @@ def test_grid_corner_colours(backend, channels):
     _compare(got, want, "corner colours")
 
 
-def test_axis_aligned_ctm_is_rejected(backend):
-    px = np.zeros((8, 8, 3), np.uint8)
-    with pytest.raises(R.RrError) as ei:
-        _single(backend, px, None, (32.0, 0.0, 0.0, 32.0, 4.0, 4.0), 0, np.zeros(33 ** 3, np.uint16),
-                33, 64, 64, np.zeros((64, 64), np.uint8))
-    assert "axis-aligned" in str(ei.value)
-    backend.wait_page(backend.submit_page())  # close the page left open by the failed record
+@pytest.mark.parametrize("case", range(6))
+def test_axis_aligned_ctm(backend, case):
+    # CTMs without rotation take blit_image's axis-aligned path (Q32 column
+    # stepping, page/mod.rs:1767-1898): enlarged, shrunk, x-flipped (a < 0),
+    # y-flipped (d < 0), partly off the canvas, and a sub-threshold shear.
+    rng = np.random.default_rng(140 + case)
+    pw, ph = 203, 171
+    ch = 4 if case % 2 == 0 else 3
+    W, H = (int(v) for v in rng.integers(6, 70, 2))
+    a, d = [(3.1, 2.2), (0.45, 0.7), (-2.5, 1.3), (1.7, -2.9), (5.0, 4.0), (1.3, 1.1)][case]
+    a, d = a * W, d * H
+    b = c = 0.0
+    if case == 5:
+        b, c = 0.4 / W, -0.3 / H  # |b| w < 0.5 and |c| h < 0.5: still axis-aligned
+    e, f = [(20.0, 15.0), (31.5, 40.25), (180.0, 9.0), (12.0, 160.0), (-60.0, -30.0), (44.0, 13.0)][case]
+    ctm = (a, b, c, d, e, f)
+    px = rng.integers(0, 256, (H, W, ch)).astype(np.uint8)
+    sm = rng.integers(0, 256, (H, W)).astype(np.uint8) if case % 3 else None
+    g = 17 if ch == 4 else 33
+    grid = rng.integers(0, 65536, g ** ch).astype(np.uint16)
+    bg = rng.integers(0, 256, (ph, pw)).astype(np.uint8)
+    want = bg.copy()
+    assert O.place_image(want, px, sm, ctm, case % 2, grid, g) > 0
+    got = _single(backend, px, sm, ctm, case % 2, grid, g, pw, ph, bg)
+    _compare(got, want, f"axis-aligned case {case}")
+
+
+def test_rejected_batch_leaves_canvas_unchanged(backend):
+    # Image 1 names a page that does not exist: the whole batch is refused
+    # before image 0 is drawn, so a CPU fallback applies every image once.
+    rng = np.random.default_rng(150)
+    pw, ph = 96, 80
+    px = rng.integers(0, 256, (30, 40, 3)).astype(np.uint8)
+    ctm = (70.0, 8.0, -6.0, 50.0, 10.0, 10.0)
+    iset = R.ImageSet([R.Image(px, None, 3, 1, ctm, 0), R.Image(px, None, 3, 0, ctm, 0)], 1, pw, ph,
+                      np.zeros(16, np.uint16), 2, rng.integers(0, 65536, 33 ** 3).astype(np.uint16), 33)
+    di = R.DeviceImages(backend, iset)
+    try:
+        di.reset(77)
+        b = di.batch()
+        descs = ctypes.cast(b.images, ctypes.POINTER(R.ImageDesc))
+        descs[1].page = 5
+        backend.begin_page()
+        with pytest.raises(R.RrError) as ei:
+            backend.record_place_images(b)
+        assert "page index" in str(ei.value)
+        backend.wait_page(backend.submit_page())
+        assert np.all(di.read_canvas(0) == 77)
+    finally:
+        di.free()
 
 
 # ---- elementwise reference ops ----
@@ -173,13 +216,28 @@ def test_icc_clut(backend, g):
     d_in, d_t = backend.to_device(cmyk), backend.to_device(table)
     d_out = backend.alloc(n * 3)
     try:
-        _run(backend, lambda: backend.record_icc_clut(d_in, d_out, d_t, g, n))
+        _run(backend, lambda: backend.record_icc_clut(d_in, d_out, d_t, table.nbytes, n))
         got = backend.download(d_out, np.zeros((n, 3), np.uint8))
     finally:
         for b in (d_in, d_t, d_out):
             backend.free(b)
     want = O.icc_clut(cmyk, table, g)
     assert np.array_equal(got, want), np.argwhere(got != want)[:5]
+
+
+@pytest.mark.parametrize("nbytes", [0, 3, 17 ** 4 * 3 + 3, 17 ** 4 * 3 - 3, 256 ** 4 * 3])
+def test_icc_clut_rejects_bad_table_length(backend, nbytes):
+    # grid_n_from_clut_len (params.rs:127-147): only g^4 * 3 bytes with
+    # 2 <= g <= 255 name a table; anything else is refused before a launch.
+    d_t = backend.alloc(64)
+    try:
+        backend.begin_page()
+        with pytest.raises(R.RrError) as ei:
+            backend.record_icc_clut(d_t, d_t, d_t, nbytes, 16)
+        assert ei.value.code == R.RR_E_INVARIANT
+        backend.wait_page(backend.submit_page())
+    finally:
+        backend.free(d_t)
 
 
 def test_cmyk_matrix(backend):

@@ -265,10 +265,32 @@ def _ramp_grid3():
     return _grid(18, 3, lambda r, g, b: 257 * 15 * r)   # grey = R channel
 
 
-def test_place_image_axis_aligned_is_rejected():
-    canvas = np.full((64, 64), 255, np.uint8)
-    px = np.zeros((8, 8, 3), np.uint8)
-    assert O.place_image(canvas, px, None, (32.0, 0.0, 0.0, 32.0, 4.0, 4.0), 0, _ramp_grid3(), 18) == -1
+@pytest.mark.parametrize("k,n,flip", [(2, 4, False), (1, 2, False), (4, 1, True), (2, 2, True)])
+def test_place_image_axis_aligned_closed_form(k, n, flip):
+    # blit_image's axis-aligned path (page/mod.rs:1767-1898) for a CTM
+    # (+-k W, 0, 0, n H, e, f) with W, H, k, n powers of two and integer e, f: the
+    # Q32 column stepping and the f64 row mapping are exact, so the path is a
+    # nearest-neighbour scale: iy = (dy - by0) // n and, unflipped,
+    # ix = (dx - e) // k; x-flipped (a < 0, u = 1 at the bbox's left edge),
+    # ix = min(W - 1, W - ceil((dx - bx0) / k)).
+    W, H, ph, pw = 16, 16, 120, 100
+    e, f = (70.0 if flip else 9.0), 20.0
+    rng = np.random.default_rng(39 + k)
+    px = rng.integers(0, 256, (H, W, 3)).astype(np.uint8)
+    canvas = np.full((ph, pw), 7, np.uint8)
+    a = -k * W if flip else k * W
+    got_n = O.place_image(canvas, px, None, (float(a), 0.0, 0.0, float(n * H), e, f), 0, _ramp_grid3(), 18)
+    want = np.full((ph, pw), 7, np.uint8)
+    by0, by1 = int(ph - f - n * H), int(ph - f)
+    bx0 = int(e - k * W) if flip else int(e)
+    for dy in range(by0, by1):
+        for dx in range(bx0, bx0 + k * W):
+            iy = (dy - by0) // n
+            j = dx - bx0
+            ix = min(W - 1, W - (j + k - 1) // k) if flip else j // k
+            want[dy, dx] = px[iy, ix, 0]
+    assert got_n == k * W * n * H
+    assert np.array_equal(canvas, want)
 
 
 def test_place_image_quarter_turn_closed_form():
