@@ -44,61 +44,90 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        P = ctypes.c_void_p
-        i64, i32, u8, u32, dbl = (ctypes.c_int64, ctypes.c_int32, ctypes.c_uint8,
-                                  ctypes.c_uint32, ctypes.c_double)
-        sig = {
-            "rro_div255": (u8, [u32]),
-            "rro_splash_floor": (i32, [dbl]),
-            "rro_splash_ceil": (i32, [dbl]),
-            "rro_splash_round": (i32, [dbl]),
-            "rro_fp_from_f64": (i32, [dbl]),
-            "rro_path_new": (P, []),
-            "rro_path_free": (None, [P]),
-            "rro_path_move_to": (ctypes.c_int, [P, dbl, dbl]),
-            "rro_path_line_to": (ctypes.c_int, [P, dbl, dbl]),
-            "rro_path_curve_to": (ctypes.c_int, [P, dbl, dbl, dbl, dbl, dbl, dbl]),
-            "rro_path_close": (ctypes.c_int, [P, ctypes.c_int]),
-            "rro_path_len": (i64, [P]),
-            "rro_path_get": (None, [P, P, P]),
-            "rro_conic_to_cubic": (None, [P, P, P, P, P]),
-            "rro_flatten_curve": (i64, [P, dbl, P, i64]),
-            "rro_xpath_edges": (i64, [P, P, i64, P, dbl, ctypes.c_int, P, i64]),
-            "rro_add_segments": (None, [P, i64, P]),
-            "rro_aa_scale": (None, [P, i64]),
-            "rro_scanner_new": (P, [P, i64, ctypes.c_int, i32, i32]),
-            "rro_scanner_free": (None, [P]),
-            "rro_scanner_bbox": (None, [P, P]),
-            "rro_scanner_row": (i64, [P, i32, P, i64]),
-            "rro_scanner_test": (ctypes.c_int, [P, i32, i32]),
-            "rro_scanner_render_aa_line": (ctypes.c_int, [P, P, i32, P, P, i32]),
-            "rro_scanner_stats": (None, [P, P]),
-            "rro_aabuf_set_span": (None, [P, i32, i32, i32]),
-            "rro_aa_coverage_span": (None, [P, i32, i32, i32, P]),
-            "rro_composite_gray8": (u8, [u8, u8, u8, u8]),
-            "rro_composite_rgb8_fast": (u8, [u8, u8, u8, u8]),
-            "rro_rgb_to_gray": (i32, [i32, i32, i32]),
-            "rro_fill": (i64, [P, i32, i32, i32, P, i64, ctypes.c_int, u8, u8, ctypes.c_int,
-                               ctypes.c_int, P, P, P]),
-            "rro_fill_page": (i64, [P, i32, i32, i32, P, P, i64, P, P, P, ctypes.c_int,
-                                    ctypes.c_int, P]),
-            "rro_cov_hash_term": (ctypes.c_uint64, [i64, u8]),
-            "rro_blend_u8": (u8, [u8, u8, u32]),
-            "rro_blend_multiply": (u8, [u8, u8]),
-            "rro_composite_rgba8": (None, [P, P, i64]),
-            "rro_apply_soft_mask": (None, [P, P, i64]),
-            "rro_cmyk_matrix": (None, [P, P, i64]),
-            "rro_icc_clut_ref": (None, [P, P, i64, P, i32]),
-            "rro_clut_gray": (u8, [P, i32, P, i32]),
-            "rro_place_image": (i64, [P, i32, i32, i32, P, i32, i32, i32, P, P, i32, P, i32]),
-        }
-        for name, (res, args) in sig.items():
-            f = getattr(L, name)
-            f.restype = res
-            f.argtypes = args
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
+
+
+def mutant_path(m):
+    """Path of the oracle build with planted mistake m (see RRO_MUTANT in rr_oracle.c)."""
+    return os.path.join(_HERE, "build", f"librr_oracle_m{m}.so")
+
+
+class using:
+    """Context manager: route every function of this module through the oracle
+    library at `path` (a planted-mistake build) while the block runs."""
+
+    def __init__(self, path):
+        self.L = load(path)
+
+    def __enter__(self):
+        global _lib
+        self.saved = _lib
+        _lib = self.L
+        return self.L
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+        return False
+
+
+def load(path):
+    """A CDLL of an oracle build with the argument and result types bound."""
+    L = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    i64, i32, u8, u32, dbl = (ctypes.c_int64, ctypes.c_int32, ctypes.c_uint8,
+                              ctypes.c_uint32, ctypes.c_double)
+    sig = {
+        "rro_div255": (u8, [u32]),
+        "rro_splash_floor": (i32, [dbl]),
+        "rro_splash_ceil": (i32, [dbl]),
+        "rro_splash_round": (i32, [dbl]),
+        "rro_fp_from_f64": (i32, [dbl]),
+        "rro_path_new": (P, []),
+        "rro_path_free": (None, [P]),
+        "rro_path_move_to": (ctypes.c_int, [P, dbl, dbl]),
+        "rro_path_line_to": (ctypes.c_int, [P, dbl, dbl]),
+        "rro_path_curve_to": (ctypes.c_int, [P, dbl, dbl, dbl, dbl, dbl, dbl]),
+        "rro_path_close": (ctypes.c_int, [P, ctypes.c_int]),
+        "rro_path_len": (i64, [P]),
+        "rro_path_get": (None, [P, P, P]),
+        "rro_conic_to_cubic": (None, [P, P, P, P, P]),
+        "rro_flatten_curve": (i64, [P, dbl, P, i64]),
+        "rro_xpath_edges": (i64, [P, P, i64, P, dbl, ctypes.c_int, P, i64]),
+        "rro_add_segments": (None, [P, i64, P]),
+        "rro_aa_scale": (None, [P, i64]),
+        "rro_scanner_new": (P, [P, i64, ctypes.c_int, i32, i32]),
+        "rro_scanner_free": (None, [P]),
+        "rro_scanner_bbox": (None, [P, P]),
+        "rro_scanner_row": (i64, [P, i32, P, i64]),
+        "rro_scanner_test": (ctypes.c_int, [P, i32, i32]),
+        "rro_scanner_render_aa_line": (ctypes.c_int, [P, P, i32, P, P, i32]),
+        "rro_scanner_stats": (None, [P, P]),
+        "rro_aabuf_set_span": (None, [P, i32, i32, i32]),
+        "rro_aa_coverage_span": (None, [P, i32, i32, i32, P]),
+        "rro_composite_gray8": (u8, [u8, u8, u8, u8]),
+        "rro_composite_rgb8_fast": (u8, [u8, u8, u8, u8]),
+        "rro_rgb_to_gray": (i32, [i32, i32, i32]),
+        "rro_fill": (i64, [P, i32, i32, i32, P, i64, ctypes.c_int, u8, u8, ctypes.c_int,
+                           ctypes.c_int, P, P, P]),
+        "rro_fill_page": (i64, [P, i32, i32, i32, P, P, i64, P, P, P, ctypes.c_int,
+                                ctypes.c_int, P]),
+        "rro_cov_hash_term": (ctypes.c_uint64, [i64, u8]),
+        "rro_blend_u8": (u8, [u8, u8, u32]),
+        "rro_blend_multiply": (u8, [u8, u8]),
+        "rro_composite_rgba8": (None, [P, P, i64]),
+        "rro_apply_soft_mask": (None, [P, P, i64]),
+        "rro_cmyk_matrix": (None, [P, P, i64]),
+        "rro_icc_clut_ref": (None, [P, P, i64, P, i32]),
+        "rro_clut_gray": (u8, [P, i32, P, i32]),
+        "rro_place_image": (i64, [P, i32, i32, i32, P, i32, i32, i32, P, P, i32, P, i32]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
 
 
 def _p(a):

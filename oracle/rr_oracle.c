@@ -11,9 +11,13 @@
  *   add_segment (dxdy, dxdy_fp, flags), scanner merge/count_crossings,
  *   composite formulas: pinned by the reference's own known-answer tests,
  *   re-run in tests/test_oracle_known_answers.py.
- *   flatten_curve, XPath::new, scanner construction, render_aa_line, fill:
- *   pinned by the reference tests plus closed forms, invariants and a
- *   brute-force sampler (tests/test_oracle_invariants.py).
+ *   flatten_curve, XPath::new: pinned by the reference tests plus closed
+ *   forms (tests/test_oracle_known_answers.py).
+ *   Scanner construction, render_aa_line, the fill_impl AA loop (both row
+ *   modes), draw_aa_line: pinned by quarter-pixel rectangle closed forms,
+ *   row-count and sortedness invariants, non-zero = even-odd on convex
+ *   polygons and a brute-force winding sampler, each shown to reject a
+ *   planted mistake (RRO_MUTANT below; tests/test_oracle_invariants.py).
  *   Row sort: the reference sorts each row with Rust's sort_unstable_by_key
  *   (scanner/mod.rs:230).  For rows of at most 32 entries that sort is
  *   stable (insertion sort up to 20; the small-sort for <=32 is made of
@@ -22,6 +26,19 @@
  *   exposed to that ("parity unpinned" for rows > 32 entries with x0 ties).
  */
 #include "rr_oracle.h"
+
+/* RRO_MUTANT (default 0) plants one known mistake for the test-suite's
+ * mutation check (tests/test_oracle_invariants.py), which builds each mutant
+ * with `make mutants` and shows that the invariant checks reject it:
+ *   1 render_aa_line ends a span at x1 instead of x1 + 1
+ *   2 aa_coverage_span takes the nibbles in the wrong order
+ *   3 literal row mode draws at aa_y % 4 == 2 instead of 3
+ *   4 fp_from_f64 floors instead of truncating toward zero
+ *   5 fill_intersections gives a segment's first row its count
+ *   6 write_intersect merges overlapping entries only, not touching ones */
+#ifndef RRO_MUTANT
+#define RRO_MUTANT 0
+#endif
 
 #include <math.h>
 #include <stdlib.h>
@@ -59,7 +76,11 @@ int32_t rro_fp_from_f64(double x) {
     if (fp >= 2147483647.0) return INT32_MAX;
     if (fp <= -2147483648.0) return INT32_MIN;
     if (isnan(fp)) return 0; /* Rust `NaN as i32` == 0 */
+#if RRO_MUTANT == 4
+    return (int32_t)floor(fp);
+#else
     return (int32_t)fp;      /* truncation toward zero */
+#endif
 }
 
 /* crates/raster/src/fill/mod.rs:46-48 */
@@ -462,7 +483,11 @@ static void write_intersect(int32_t row, int32_t y_min, const uint32_t *alloc_st
     uint32_t cur = cursors[idx];
     if (cur > 0) {
         rro_intersect *last = &ints[base + cur - 1];
+#if RRO_MUTANT == 6
+        if (last->x1 >= lo && last->x0 <= hi) {
+#else
         if (sat_add_i32(last->x1, 1) >= lo && last->x0 <= sat_add_i32(hi, 1)) {
+#endif
             last->count = (int32_t)((uint32_t)last->count + (uint32_t)count);
             if (lo < last->x0) last->x0 = lo;
             if (hi > last->x1) last->x1 = hi;
@@ -579,7 +604,11 @@ rro_scanner *rro_scanner_new(const rro_seg *segs, int64_t n, int eo, int32_t cli
             int32_t cnt = (g->flags & RRO_SEG_FLIPPED) ? 1 : -1;
             int32_t x = rro_splash_floor(g->x0);
             for (int64_t r = r0; r <= r1; ++r) {
+#if RRO_MUTANT == 5
+                int32_t c = cnt;
+#else
                 int32_t c = (seg_y_min < (double)r) ? cnt : 0;
+#endif
                 write_intersect((int32_t)r, y_min, alloc_start, counts, ints, x, x, c);
             }
         } else {
@@ -596,7 +625,11 @@ rro_scanner *rro_scanner_new(const rro_seg *segs, int64_t n, int eo, int32_t cli
             for (int64_t r = r0; r <= r1; ++r) {
                 int32_t c1 = xx1_fp < fpmin ? fpmin : (xx1_fp > fpmax ? fpmax : xx1_fp);
                 int32_t px1 = c1 >> 16;
+#if RRO_MUTANT == 5
+                int32_t c = cnt;
+#else
                 int32_t c = (seg_y_min < (double)r) ? cnt : 0;
+#endif
                 write_intersect((int32_t)r, y_min, alloc_start, counts, ints, px0, px1, c);
                 px0 = px1;
                 xx1_fp = sat_add_i32(xx1_fp, g->dxdy_fp);
@@ -745,7 +778,11 @@ int rro_scanner_render_aa_line(const rro_scanner *s, uint8_t *aabuf, int32_t aa_
                     break;
                 }
                 count = (int32_t)((uint32_t)count + (uint32_t)row[i].count);
+#if RRO_MUTANT == 1
+                int32_t end = row[i].x1;
+#else
                 int32_t end = sat_add_i32(row[i].x1, 1);
+#endif
                 i++;
                 if ((count & m) == 0) {
                     span_x1 = end;
@@ -787,7 +824,11 @@ void rro_aa_coverage_span(const uint8_t *aabuf, int32_t row_bytes, int32_t x0, i
         uint8_t c = 0;
         for (int r = 0; r < 4; ++r) {
             uint8_t b = aabuf[(size_t)r * row_bytes + bi];
+#if RRO_MUTANT == 2
+            c += NIBBLE_POP[(x & 1) ? (b >> 4) : (b & 0x0f)];
+#else
             c += NIBBLE_POP[(x & 1) ? (b & 0x0f) : (b >> 4)];
+#endif
         }
         shape[i] = c;
     }
@@ -920,7 +961,11 @@ static int64_t fill_segs(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, 
                 if (x0 < clip->x_min_i) x0 = clip->x_min_i;
                 if (x1 > clip->x_max_i) x1 = clip->x_max_i;
             }
+#if RRO_MUTANT == 3
+            if (aa_row == 2) {
+#else
             if (aa_row == 3) {
+#endif
                 if (x0 <= x1 && y >= 0 && y < h)
                     drawn += draw_aa_line(canvas, w, stride, aabuf, rb, x0, x1, y, color, a_input,
                                           flavour, shape_buf, shape_out, cov_hash);
