@@ -17,6 +17,8 @@ MODE_DOCUMENTED = 0
 MODE_LITERAL = 1
 FLAVOUR_GRAY8 = 0
 FLAVOUR_RGB8_FAST = 1
+FLAVOUR_RGB8 = 2
+FILL_EVEN_ODD, FILL_MULTIPLY, FILL_SOFT_MASK = 1, 2, 4
 
 PATH_FIRST, PATH_LAST, PATH_CLOSED, PATH_CURVE = 1, 2, 4, 8
 SEG_HORIZ, SEG_VERT, SEG_FLIPPED = 1, 2, 4
@@ -114,6 +116,13 @@ def load(path):
         "rro_fill_page": (i64, [P, i32, i32, i32, P, P, i64, P, P, P, ctypes.c_int,
                                 ctypes.c_int, P]),
         "rro_cov_hash_term": (ctypes.c_uint64, [i64, u8]),
+        "rro_compute_a_src": (u32, [u32, u32, u32, ctypes.c_int, ctypes.c_int]),
+        "rro_composite_general": (u8, [u8, u8, u8, u8, u8, ctypes.c_int, ctypes.c_int]),
+        "rro_fill_ex": (i64, [P, i32, i32, i32, P, i64, u32, P, u8, ctypes.c_int, ctypes.c_int, P, P, i32,
+                              P, P]),
+        "rro_fill_page_ex": (i64, [P, i32, i32, i32, P, P, i64, P, P, P, ctypes.c_int, ctypes.c_int, P, i32,
+                                   P]),
+        "rro_rgb_to_gray_page": (None, [P, i32, P, i32, i32, i32]),
         "rro_blend_u8": (u8, [u8, u8, u32]),
         "rro_blend_multiply": (u8, [u8, u8]),
         "rro_composite_rgba8": (None, [P, P, i64]),
@@ -286,6 +295,53 @@ def fill_page(canvas, edges, edge_off, eo, color, a_input, mode=MODE_DOCUMENTED,
     d = lib().rro_fill_page(_p(canvas), w, h, stride, _p(edges), _p(edge_off), n, _p(eo),
                             _p(color), _p(a_input), mode, flavour, _p(hs))
     return int(d), hs
+
+
+def fill_ex(canvas, edges, flags, color, a_input=255, mode=MODE_DOCUMENTED, flavour=FLAVOUR_GRAY8,
+            clip=None, soft_mask=None, want_shape=False):
+    """rro_fill_ex on a (H, W) grey or (H, W, 3) Rgb8 uint8 canvas (in place).
+    color: a grey value or an (r, g, b) triple.  Returns (drawn, cov_hash, shape)."""
+    h, w = canvas.shape[:2]
+    assert canvas.dtype == np.uint8 and canvas.flags.c_contiguous
+    assert (canvas.ndim == 3) == (flavour == FLAVOUR_RGB8)
+    edges = np.ascontiguousarray(edges, np.float64).reshape(-1, 4)
+    clip = page_clip(w, h) if clip is None else np.ascontiguousarray(clip, np.float64)
+    c3 = np.ascontiguousarray(np.broadcast_to(np.asarray(color, np.uint8), (3,)))
+    sm = None if soft_mask is None else np.ascontiguousarray(soft_mask, np.uint8)
+    shape = np.zeros((h, w), np.uint8) if want_shape else None
+    hsh = ctypes.c_uint64(0)
+    d = lib().rro_fill_ex(_p(canvas), w, h, canvas.strides[0], _p(edges), len(edges), int(flags), _p(c3),
+                          a_input, mode, flavour, _p(clip), _p(sm), 0 if sm is None else sm.strides[0],
+                          _p(shape), ctypes.byref(hsh))
+    return int(d), hsh.value, shape
+
+
+def fill_page_ex(canvas, edges, edge_off, flags, colors, a_input, mode=MODE_DOCUMENTED,
+                 flavour=FLAVOUR_GRAY8, soft_mask=None, hashes=True):
+    """A page of paths in order on a grey or Rgb8 canvas.  colors: (n, 3)
+    uint8 (a grey page reads column 0).  Returns (drawn, hashes)."""
+    h, w = canvas.shape[:2]
+    edges = np.ascontiguousarray(edges, np.float64)
+    edge_off = np.ascontiguousarray(edge_off, np.int64)
+    n = len(edge_off) - 1
+    flags = np.ascontiguousarray(flags, np.uint8)
+    colors = np.ascontiguousarray(colors, np.uint8).reshape(n, 3)
+    a_input = np.ascontiguousarray(a_input, np.uint8)
+    sm = None if soft_mask is None else np.ascontiguousarray(soft_mask, np.uint8)
+    hs = np.zeros(n, np.uint64) if hashes else None
+    d = lib().rro_fill_page_ex(_p(canvas), w, h, canvas.strides[0], _p(edges), _p(edge_off), n, _p(flags),
+                               _p(colors), _p(a_input), mode, flavour, _p(sm),
+                               0 if sm is None else sm.strides[0], _p(hs))
+    return int(d), hs
+
+
+def rgb_to_gray_page(rgb):
+    """Gray8 of an (H, W, 3) Rgb8 bitmap (render.rs:1306-1324)."""
+    h, w = rgb.shape[:2]
+    rgb = np.ascontiguousarray(rgb, np.uint8)
+    out = np.zeros((h, w), np.uint8)
+    lib().rro_rgb_to_gray_page(_p(rgb), rgb.strides[0], _p(out), w, w, h)
+    return out
 
 
 def cov_hash_term(pix, shape):

@@ -35,7 +35,9 @@
  *   3 literal row mode draws at aa_y % 4 == 2 instead of 3
  *   4 fp_from_f64 floors instead of truncating toward zero
  *   5 fill_intersections gives a segment's first row its count
- *   6 write_intersect merges overlapping entries only, not touching ones */
+ *   6 write_intersect merges overlapping entries only, not touching ones
+ *   7 compute_a_src drops the soft mask when a shape is present
+ *   8 Multiply blends the source with itself instead of the destination */
 #ifndef RRO_MUTANT
 #define RRO_MUTANT 0
 #endif
@@ -898,10 +900,57 @@ static int clip_test_rect(const clip_rect *c, int32_t l, int32_t t, int32_t r, i
     return CLIP_PARTIAL;
 }
 
+/* compute_a_src (pipe/general.rs:326-344) for a fill (shape present). */
+uint32_t rro_compute_a_src(uint32_t a_input, uint32_t soft, uint32_t shape, int has_soft_mask, int has_shape) {
+#if RRO_MUTANT == 7
+    if (has_soft_mask && !has_shape) return rro_div255(a_input * soft);
+#else
+    if (has_soft_mask)
+        return has_shape ? rro_div255((uint32_t)rro_div255(a_input * soft) * shape) : rro_div255(a_input * soft);
+#endif
+    return has_shape ? rro_div255(a_input * shape) : a_input;
+}
+
+/* render_span_general_inner, no-alpha arm (pipe/general.rs:270-315), one
+ * channel: Normal c = div255((255 - a_src) d + a_src s); any other mode
+ * div255((255 - a_src) d + a_src c_b) with Multiply's c_b = div255(s d)
+ * (pipe/blend.rs:24-26). */
+uint8_t rro_composite_general(uint8_t dst, uint8_t src, uint8_t a_input, uint8_t shape, uint8_t soft,
+                              int has_soft_mask, int multiply) {
+    uint32_t a_src = rro_compute_a_src(a_input, soft, shape, has_soft_mask, 1);
+#if RRO_MUTANT == 8
+    uint32_t cb = multiply ? rro_div255((uint32_t)src * src) : src;
+#else
+    uint32_t cb = multiply ? rro_blend_multiply(src, dst) : src;
+#endif
+    return rro_div255((255u - a_src) * dst + a_src * cb);
+}
+
+/* The paint of one fill: render_span's choice (pipe/mod.rs:157-199) between
+ * the AA path (no soft mask, Normal) and the general pipe, for a 1-byte
+ * (grey, or Rgb8 with r = g = b) or a 3-byte Rgb8 canvas. */
+typedef struct {
+    uint32_t flags;      /* RRO_FILL_* */
+    uint8_t color[3];    /* r, g, b (a grey fill repeats its value) */
+    uint8_t a_input;
+    int flavour;
+    const uint8_t *soft; /* page-space soft mask (RRO_FILL_SOFT_MASK) */
+    int32_t soft_stride;
+} rro_paint;
+
+static uint8_t paint_channel(const rro_paint *pt, uint8_t d, uint8_t src, uint8_t sh, uint8_t sm) {
+    if (pt->flags & (RRO_FILL_MULTIPLY | RRO_FILL_SOFT_MASK))
+        return rro_composite_general(d, src, pt->a_input, sh, sm, (pt->flags & RRO_FILL_SOFT_MASK) != 0,
+                                     (pt->flags & RRO_FILL_MULTIPLY) != 0);
+    if (pt->flavour == RRO_FLAVOUR_GRAY8) return rro_composite_gray8(d, src, pt->a_input, sh);
+    return rro_composite_rgb8_fast(d, src, pt->a_input, sh);
+}
+
+/* draw_aa_line (fill/mod.rs:344-401): shapes of [x0, x1], skip when all are
+ * zero, else render_span over the whole range. */
 static int64_t draw_aa_line(uint8_t *canvas, int32_t w, int32_t stride, const uint8_t *aabuf,
-                            int32_t rb, int32_t x0, int32_t x1, int32_t y, uint8_t color,
-                            uint8_t a_input, int flavour, uint8_t *shape_buf, uint8_t *shape_out,
-                            uint64_t *cov_hash) {
+                            int32_t rb, int32_t x0, int32_t x1, int32_t y, const rro_paint *pt,
+                            uint8_t *shape_buf, uint8_t *shape_out, uint64_t *cov_hash) {
     int32_t count = x1 - x0 + 1;
     rro_aa_coverage_span(aabuf, rb, x0, count, shape_buf);
     int any = 0;
@@ -913,12 +962,14 @@ static int64_t draw_aa_line(uint8_t *canvas, int32_t w, int32_t stride, const ui
     if (!any) return 0;
     int64_t drawn = 0;
     uint8_t *row = canvas + (size_t)y * stride;
+    const int nch = pt->flavour == RRO_FLAVOUR_RGB8 ? 3 : 1;
     uint64_t hsum = 0;
     for (int32_t i = 0; i < count; ++i) {
         uint8_t sh = shape_buf[i];
         int32_t x = x0 + i;
-        row[x] = flavour == RRO_FLAVOUR_RGB8_FAST ? rro_composite_rgb8_fast(row[x], color, a_input, sh)
-                                                  : rro_composite_gray8(row[x], color, a_input, sh);
+        uint8_t sm = (pt->flags & RRO_FILL_SOFT_MASK) ? pt->soft[(size_t)y * pt->soft_stride + x] : 255;
+        for (int ch = 0; ch < nch; ++ch)
+            row[nch * x + ch] = paint_channel(pt, row[nch * x + ch], pt->color[ch], sh, sm);
         if (sh) {
             drawn++;
             if (shape_out) shape_out[(size_t)y * w + x] = sh;
@@ -930,7 +981,7 @@ static int64_t draw_aa_line(uint8_t *canvas, int32_t w, int32_t stride, const ui
 }
 
 static int64_t fill_segs(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, rro_seg *segs,
-                         int64_t n, int eo, uint8_t color, uint8_t a_input, int mode, int flavour,
+                         int64_t n, int eo, const rro_paint *pt, int mode,
                          const clip_rect *clip, uint8_t *shape_out, uint64_t *cov_hash) {
     if (n == 0) return 0;
     rro_aa_scale(segs, n);
@@ -967,8 +1018,8 @@ static int64_t fill_segs(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, 
             if (aa_row == 3) {
 #endif
                 if (x0 <= x1 && y >= 0 && y < h)
-                    drawn += draw_aa_line(canvas, w, stride, aabuf, rb, x0, x1, y, color, a_input,
-                                          flavour, shape_buf, shape_out, cov_hash);
+                    drawn += draw_aa_line(canvas, w, stride, aabuf, rb, x0, x1, y, pt, shape_buf, shape_out,
+                                          cov_hash);
                 memset(aabuf, 0, (size_t)rb * 4);
             }
         }
@@ -981,8 +1032,7 @@ static int64_t fill_segs(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, 
                 if (x1 > clip->x_max_i) x1 = clip->x_max_i;
             }
             if (x0 <= x1 && y >= 0 && y < h)
-                drawn += draw_aa_line(canvas, w, stride, aabuf, rb, x0, x1, y, color, a_input,
-                                      flavour, shape_buf, shape_out, cov_hash);
+                drawn += draw_aa_line(canvas, w, stride, aabuf, rb, x0, x1, y, pt, shape_buf, shape_out, cov_hash);
         }
     }
     free(shape_buf);
@@ -991,17 +1041,41 @@ static int64_t fill_segs(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, 
     return drawn;
 }
 
-int64_t rro_fill(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, const double *edges,
-                 int64_t n_edges, int eo, uint8_t color, uint8_t a_input, int mode, int flavour,
-                 const double clip[4], uint8_t *shape_out, uint64_t *cov_hash) {
+int64_t rro_fill_ex(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, const double *edges, int64_t n_edges,
+                    uint32_t flags, const uint8_t color[3], uint8_t a_input, int mode, int flavour,
+                    const double clip[4], const uint8_t *soft_mask, int32_t soft_stride, uint8_t *shape_out,
+                    uint64_t *cov_hash) {
     if (n_edges <= 0) return 0;
+    rro_paint pt = {flags, {color[0], color[1], color[2]}, a_input, flavour, soft_mask, soft_stride};
     rro_seg *segs = (rro_seg *)malloc((size_t)n_edges * sizeof(rro_seg));
     rro_add_segments(edges, n_edges, segs);
     clip_rect c = make_clip(clip);
-    int64_t d = fill_segs(canvas, w, h, stride, segs, n_edges, eo, color, a_input, mode, flavour,
-                          &c, shape_out, cov_hash);
+    int64_t d = fill_segs(canvas, w, h, stride, segs, n_edges, (flags & RRO_FILL_EVEN_ODD) != 0, &pt, mode, &c,
+                          shape_out, cov_hash);
     free(segs);
     return d;
+}
+
+int64_t rro_fill(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, const double *edges,
+                 int64_t n_edges, int eo, uint8_t color, uint8_t a_input, int mode, int flavour,
+                 const double clip[4], uint8_t *shape_out, uint64_t *cov_hash) {
+    const uint8_t c3[3] = {color, color, color};
+    return rro_fill_ex(canvas, w, h, stride, edges, n_edges, eo ? RRO_FILL_EVEN_ODD : 0u, c3, a_input, mode, flavour,
+                       clip, NULL, 0, shape_out, cov_hash);
+}
+
+int64_t rro_fill_page_ex(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, const double *edges,
+                         const int64_t *edge_off, int64_t n_paths, const uint8_t *flags, const uint8_t *colors,
+                         const uint8_t *a_input, int mode, int flavour, const uint8_t *soft_mask,
+                         int32_t soft_stride, uint64_t *cov_hashes) {
+    double clip[4] = {0.0, 0.0, (double)w - 0.001, (double)h - 0.001};
+    int64_t total = 0;
+    for (int64_t p = 0; p < n_paths; ++p) {
+        int64_t off = edge_off[p], n = edge_off[p + 1] - edge_off[p];
+        total += rro_fill_ex(canvas, w, h, stride, edges + 4 * off, n, flags[p], colors + 3 * p, a_input[p], mode,
+                             flavour, clip, soft_mask, soft_stride, NULL, cov_hashes ? &cov_hashes[p] : NULL);
+    }
+    return total;
 }
 
 int64_t rro_fill_page(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, const double *edges,
@@ -1016,4 +1090,14 @@ int64_t rro_fill_page(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, con
                           mode, flavour, clip, NULL, cov_hashes ? &cov_hashes[p] : NULL);
     }
     return total;
+}
+
+/* rgb_to_gray over a whole Rgb8 bitmap (render.rs:1306-1324). */
+void rro_rgb_to_gray_page(const uint8_t *rgb, int32_t rgb_stride, uint8_t *gray, int32_t gray_stride, int32_t w,
+                          int32_t h) {
+    for (int32_t y = 0; y < h; ++y)
+        for (int32_t x = 0; x < w; ++x) {
+            const uint8_t *p = rgb + (size_t)y * rgb_stride + 3 * (size_t)x;
+            gray[(size_t)y * gray_stride + x] = (uint8_t)rro_rgb_to_gray(p[0], p[1], p[2]);
+        }
 }

@@ -32,7 +32,11 @@ enum { RRO_SEG_HORIZ = 1, RRO_SEG_VERT = 2, RRO_SEG_FLIPPED = 4 };
 enum { RRO_MODE_DOCUMENTED = 0, RRO_MODE_LITERAL = 1 };
 /* Canvas flavour: gray8 exact div255 (pipe/aa.rs:192-208) or the Rgb8
  * opaque fast path (simd/composite.rs:56-114) on a grey page. */
-enum { RRO_FLAVOUR_GRAY8 = 0, RRO_FLAVOUR_RGB8_FAST = 1 };
+enum { RRO_FLAVOUR_GRAY8 = 0, RRO_FLAVOUR_RGB8_FAST = 1, RRO_FLAVOUR_RGB8 = 2 };
+/* Fill flags: even-odd rule, blend mode Multiply (PipeState::blend_mode), a
+ * page-space soft mask (PipeState::soft_mask, sampled at each pixel).  Either
+ * of the last two sends the paint to the general pipe (pipe/mod.rs:157-199). */
+enum { RRO_FILL_EVEN_ODD = 1, RRO_FILL_MULTIPLY = 2, RRO_FILL_SOFT_MASK = 4 };
 
 typedef struct {
     double x0, y0, x1, y1, dxdy;
@@ -109,6 +113,14 @@ void rro_aa_coverage_span(const uint8_t *aabuf, int32_t row_bytes, int32_t x0, i
 uint8_t rro_composite_gray8(uint8_t dst, uint8_t src, uint8_t a_input, uint8_t shape);
 uint8_t rro_composite_rgb8_fast(uint8_t dst, uint8_t src, uint8_t a_input, uint8_t shape);
 int32_t rro_rgb_to_gray(int32_t r, int32_t g, int32_t b);  /* render.rs:1306-1324 */
+void rro_rgb_to_gray_page(const uint8_t *rgb, int32_t rgb_stride, uint8_t *gray, int32_t gray_stride, int32_t w,
+                          int32_t h);
+/* general pipe, no-alpha arm (pipe/general.rs:270-344; blend.rs:24-26);
+ * pinned by general.rs:518-570 (compute_a_src cases), the no-alpha readings of
+ * :572-611 and closed forms (tests/test_oracle_general.py) */
+uint32_t rro_compute_a_src(uint32_t a_input, uint32_t soft, uint32_t shape, int has_soft_mask, int has_shape);
+uint8_t rro_composite_general(uint8_t dst, uint8_t src, uint8_t a_input, uint8_t shape, uint8_t soft,
+                              int has_soft_mask, int multiply);
 
 /* ---- fill (crates/raster/src/fill/mod.rs:110-244,344-401) ----
  * Fills one path given its add_segment edge list (pre aa_scale) onto a 1-byte
@@ -121,6 +133,20 @@ int32_t rro_rgb_to_gray(int32_t r, int32_t g, int32_t b);  /* render.rs:1306-132
 int64_t rro_fill(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, const double *edges,
                  int64_t n_edges, int eo, uint8_t color, uint8_t a_input, int mode, int flavour,
                  const double clip[4], uint8_t *shape_out, uint64_t *cov_hash);
+
+/* rro_fill with every paint option: flags RRO_FILL_*, colour r, g, b (a 1-byte
+ * canvas uses color[0]), flavour RRO_FLAVOUR_RGB8 for a 3-byte Rgb8 canvas
+ * (stride in bytes), soft_mask a W x H page-space mask (soft_stride bytes a
+ * row) read when flags has RRO_FILL_SOFT_MASK. */
+int64_t rro_fill_ex(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, const double *edges, int64_t n_edges,
+                    uint32_t flags, const uint8_t color[3], uint8_t a_input, int mode, int flavour,
+                    const double clip[4], const uint8_t *soft_mask, int32_t soft_stride, uint8_t *shape_out,
+                    uint64_t *cov_hash);
+/* A page of rro_fill_ex paths (flags[p], colors[3p..3p+2], a_input[p]). */
+int64_t rro_fill_page_ex(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, const double *edges,
+                         const int64_t *edge_off, int64_t n_paths, const uint8_t *flags, const uint8_t *colors,
+                         const uint8_t *a_input, int mode, int flavour, const uint8_t *soft_mask,
+                         int32_t soft_stride, uint64_t *cov_hashes);
 
 /* One page of paths, in order.  edge_off has n_paths+1 entries.  Returns the
  * total number of shape>0 pixels composited. */
