@@ -35,19 +35,35 @@ extern "C" {
 #define RR_MODE_LITERAL 1
 /* Canvas flavour. GRAY8: Bitmap<Gray8>, exact div255 (pipe/aa.rs:192-208).
  * RGB8_FAST: Bitmap<Rgb8> on a grey page, (v+255)>>8 (simd/composite.rs:56-114),
- * stored one byte per pixel because r=g=b stays invariant. */
+ * stored one byte per pixel because r=g=b stays invariant.
+ * RGB8: Bitmap<Rgb8> as raster_pdf renders it (crates/pdf_interp/src/renderer/
+ * page/mod.rs:479-480): 3 bytes a pixel (r, g, b), every path with its own
+ * colour (rr_fill_pages.colors), composited per channel with the same
+ * formulas; rows of `stride` bytes, stride a multiple of 4 and at least
+ * 3 * (width rounded up to 4).  A page's optional Gray8 output receives
+ * rgb_to_gray of the finished page (crates/pdf_raster/src/render.rs:1306-1324). */
 #define RR_FLAVOUR_GRAY8 0
 #define RR_FLAVOUR_RGB8_FAST 1
+#define RR_FLAVOUR_RGB8 2
 /* Execution shape of record_fill_paths. */
 #define RR_LAUNCH_PLAIN 0   /* one path per kernel launch, in order */
 #define RR_LAUNCH_BATCHED 1 /* one persistent launch for all paths of the batch */
 
 /* Per-path fill attributes (device array).  Mirrors the arguments of
  * raster::fill / eo_fill (crates/raster/src/fill/mod.rs:55,83) for a solid
- * source: fill rule, grey value, PipeState::a_input. */
+ * source: fill rule, grey value, PipeState::a_input, and the two PipeState
+ * fields that send a fill to the general pipe instead of the AA path
+ * (render_span, crates/raster/src/pipe/mod.rs:157-199): blend mode Multiply
+ * and a soft mask.  The general pipe composites with compute_a_src and the
+ * no-alpha formulas of crates/raster/src/pipe/general.rs:270-344 (Multiply:
+ * blend_multiply, pipe/blend.rs:24-26); the soft mask is the page's
+ * (rr_page_desc.soft_mask), read at each pixel. */
+#define RR_FILL_EVEN_ODD 1  /* even-odd rule (else non-zero winding) */
+#define RR_FILL_MULTIPLY 2  /* BlendMode::Multiply (else Normal) */
+#define RR_FILL_SOFT_MASK 4 /* PipeState::soft_mask = the page's soft mask */
 typedef struct {
-    uint8_t eo;      /* 1 = even-odd, 0 = non-zero winding */
-    uint8_t color;   /* solid grey source value */
+    uint8_t eo;      /* RR_FILL_* bits; 1 alone = even-odd, 0 = non-zero winding */
+    uint8_t color;   /* solid grey source value (RR_FLAVOUR_RGB8: see colors) */
     uint8_t a_input; /* fill alpha, 255 = opaque */
     uint8_t route;   /* fill route chosen by the dispatcher (RR_ROUTE_*) */
 } rr_path_attr;
@@ -103,15 +119,30 @@ typedef struct {
     uint64_t *cov_hash;
     uint64_t *cov_count;
     int32_t variant; /* band-fill shape, 0 <= variant < rr_hip_variant_count() */
+    int32_t reserved;
+    const uint8_t *colors;     /* RR_FLAVOUR_RGB8: device, 4 bytes a path {r, g, b, 0}, 4-byte aligned */
+    const uint8_t *soft_mask;  /* device page-space soft mask, 1 byte a pixel, or NULL */
+    int64_t soft_mask_stride;  /* bytes a row, a multiple of 4 (the mask 4-byte aligned) */
+    uint8_t *gray;             /* RR_FLAVOUR_RGB8: optional device Gray8 output, or NULL */
+    int64_t gray_stride;       /* bytes a row, a multiple of 4 >= width (the output 4-byte aligned) */
 } rr_fill_batch;
 
 /* One page of a multi-page batch: its canvas and the range of the batch's
- * paths that belong to it (pages tile [0, n_paths) in order). */
+ * paths that belong to it (pages tile [0, n_paths) in order).  With a gray
+ * output, the batched launch writes it in the composite pass's epilogue
+ * (every tile of the page, touched or not); the plain launch converts the
+ * page after its last path.  wait_page returns RR_E_INVARIANT when a path
+ * asked for the soft mask of a page that has none (that path is filled as if
+ * the mask were 255). */
 typedef struct {
-    uint8_t *canvas; /* device, 1 byte per pixel, row pitch `stride` */
+    uint8_t *canvas; /* device, 1 byte per pixel (3 for RR_FLAVOUR_RGB8), row pitch `stride` */
     int32_t width, height, stride, reserved;
     int64_t path_begin, path_end;
     double clip[4]; /* page clip is {0,0,W-0.001,H-0.001} (crates/raster/src/state.rs:108-121) */
+    const uint8_t *soft_mask; /* as in rr_fill_batch; required by paths with RR_FILL_SOFT_MASK */
+    int64_t soft_mask_stride;
+    uint8_t *gray;            /* as in rr_fill_batch (RR_FLAVOUR_RGB8 only) */
+    int64_t gray_stride;
 } rr_page_desc;
 
 /* Many pages of paths in one recording.  edges/edge_off/attrs/cov_* are as in
@@ -132,6 +163,8 @@ typedef struct {
     uint64_t *cov_hash;
     uint64_t *cov_count;
     int32_t variant;
+    int32_t reserved;
+    const uint8_t *colors; /* as in rr_fill_batch, indexed by the batch's path */
 } rr_fill_pages;
 
 

@@ -222,7 +222,10 @@ __global__ void __launch_bounds__(256) k_dispatch(const rr_path_attr *__restrict
         sp.px_hi = pg.px_hi;
         sp.occ_base = pg.cbase;
         sp.ctiles = pg.ctiles;
-        sp.flags = (at.eo ? 1u : 0u) | (static_cast<uint32_t>(at.a_input) << 8) |
+        // bit 1: an occluder candidate -- an opaque paint on the AA path, which
+        // leaves exactly its colour under full coverage
+        const bool opaque = at.a_input == 255 && !(at.eo & kFillGeneral);
+        sp.flags = (at.eo & RR_FILL_EVEN_ODD) | (opaque ? 2u : 0u) | (static_cast<uint32_t>(at.a_input) << 8) |
                    (static_cast<uint32_t>(min(max(pg.px_lo, 0), 0xffff)) << 16);
         spath[p] = sp;
     }
@@ -459,7 +462,7 @@ struct SpanCursor {
 struct ItemPath {
     int64_t p, e0, e1;
     int32_t y_min, y_max, row0, row1, px0, px1, width, height, px_lo, px_hi, occ_base, ctiles;
-    int32_t eo, a_input;
+    int32_t eo, opaque;
 };
 
 // First 16 bytes of a span header (offset, full-cover and touched columns),
@@ -559,7 +562,7 @@ __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &
     }
     fx0 = max(wave_max(fx0), P.px0);
     fx1 = min(wave_min(fx1), px1);
-    if (P.a_input != 255 || fx0 > fx1) {
+    if (!P.opaque || fx0 > fx1) {
         fx0 = 1;
         fx1 = 0;
     }
@@ -769,7 +772,7 @@ __device__ __forceinline__ ItemPath item_path(const BatchArgs &B, int32_t p) {
     P.occ_base = S.occ_base;
     P.ctiles = S.ctiles;
     P.eo = S.flags & 1;
-    P.a_input = (S.flags >> 8) & 0xff;
+    P.opaque = (S.flags >> 1) & 1;
     P.px_lo = (S.flags >> 16) & 0xffff;
     return P;
 }
@@ -796,7 +799,7 @@ __device__ __forceinline__ ItemPath item_path_lanes(const int4 &q, int32_t p, in
     P.ctiles = rl(q.z, 3);
     const uint32_t fl = static_cast<uint32_t>(rl(q.w, 3));
     P.eo = fl & 1;
-    P.a_input = (fl >> 8) & 0xff;
+    P.opaque = (fl >> 1) & 1;
     P.px_lo = (fl >> 16) & 0xffff;
     return P;
 }
@@ -934,13 +937,17 @@ __global__ void __launch_bounds__(64) k_spans_wide(BatchArgs B, int32_t mode) {
 // ---------------------------------------------------------------------------
 // phase B: composite of one (page, band, tile)
 // ---------------------------------------------------------------------------
-template <int CH>
+// The tile's canvas bytes live in LDS as NPL channel planes (1: a grey or
+// grey-Rgb8 canvas; 3: an Rgb8 canvas, de-interleaved on load and
+// re-interleaved on store), 16 rows of CH pixels each.
+template <int CH, int NPL>
 struct ComposeLds {
     static constexpr int kWords = CH / 8;  // bitmap dwords per AA row (4 bits per pixel)
     static constexpr int kPitch = kWords + 1;
-    static constexpr int kRowWords = CH / 4;  // canvas dwords per output row
+    static constexpr int kRowWords = CH / 4;  // canvas dwords per output row and plane
     static constexpr int kRowPitch = kRowWords + 1;
-    uint32_t cv[kBandRows * kRowPitch];
+    static constexpr int kPlane = kBandRows * kRowPitch;
+    uint32_t cv[NPL * kPlane];
     uint32_t bm[64 * kPitch];
     uint8_t gamma[32];
 };
@@ -949,7 +956,8 @@ struct ComposeLds {
 struct PathRef {
     int64_t p;
     int32_t hi;   // span item
-    uint32_t at;  // rr_path_attr bytes {eo, color, a_input, route}
+    uint32_t at;  // rr_path_attr bytes {flags, color, a_input, route}
+    uint32_t rgb;  // Rgb8 canvas: the path's colour r | g << 8 | b << 16
     int32_t cx0, cx1;  // columns its intervals touch (from the span head)
 };
 
@@ -961,7 +969,8 @@ struct BinCursor {
     int4 ent;
 };
 
-__device__ __forceinline__ bool bin_next(const BatchArgs &B, const DevPage &pg, BinCursor &c, PathRef &out) {
+__device__ __forceinline__ bool bin_next(const BatchArgs &B, const DevPage &pg, BinCursor &c, PathRef &out,
+                                         const uint32_t *colors) {
     const int lane = threadIdx.x;
     if (c.k >= c.n) {
         c.lb += 64;
@@ -979,7 +988,9 @@ __device__ __forceinline__ bool bin_next(const BatchArgs &B, const DevPage &pg, 
         if (lane == 0) atomicOr(B.F.err, 16);  // a bin entry that was never written
         out.hi = 0;
         out.at = 0;
+        out.p = pg.path_begin;
     }
+    out.rgb = colors && out.p < pg.path_end ? colors[out.p] : 0u;
     return true;
 }
 
@@ -1011,17 +1022,6 @@ __device__ __forceinline__ SpanView span_fetch(const BatchArgs &B, long long off
     return v;
 }
 
-// Source-over of one pixel (pipe/aa.rs:192-208 Gray8; simd/composite.rs:56-114
-// Rgb8 fast path, identical per channel on a grey canvas).
-__device__ __forceinline__ uint32_t comp1(uint32_t dv, uint32_t shape, uint32_t a_in, uint32_t src, int flavour) {
-    if (flavour == RR_FLAVOUR_RGB8_FAST) {
-        const uint32_t a = (a_in * shape + 255u) >> 8;
-        return ((255u - a) * dv + a * src + 255u) >> 8;
-    }
-    const uint32_t a = div255(a_in * shape);
-    return div255((255u - a) * dv + a * src);
-}
-
 // Path p over the tile held in LDS.  Lane l holds AA row l of the band; the
 // four lanes of output row d = l/4 work that row together.  A row whose four
 // AA rows each have at most one interval in the tile is done from the
@@ -1029,23 +1029,35 @@ __device__ __forceinline__ uint32_t comp1(uint32_t dv, uint32_t shape, uint32_t 
 // dwords at a time, the rest get their coverage count in closed form
 // (sum over AA rows of the interval's overlap with [4x, 4x+4)).  Rows with
 // more intervals go through the 4x4-bit coverage bitmap.  Both give the
-// nibble-popcount coverage of fill/mod.rs:344-401 exactly.
+// nibble-popcount coverage of fill/mod.rs:344-401 exactly.  Each covered
+// pixel is painted in every channel plane with paint_px: the AA path, or the
+// general pipe for a Multiply or soft-masked path (its soft-mask byte read
+// from the page's mask).
 // COV: per-path coverage outputs (hashes / counts) were asked for; the kernel
 // instance without them carries none of that code.
-template <int CH, bool COV>
-__device__ __forceinline__ void compose_rows(const BatchArgs &B, ComposeLds<CH> &L, const DevPage &pg, int64_t p,
-                                             uint32_t at, const SpanView &V, int32_t Y0, int32_t tx0, int32_t xa,
-                                             int32_t xb, int32_t ya, int32_t yb) {
+template <int CH, bool COV, int NPL>
+__device__ __forceinline__ void compose_rows(const BatchArgs &B, ComposeLds<CH, NPL> &L, const DevPage &pg, int64_t p,
+                                             uint32_t at, uint32_t rgb, const SpanView &V, int32_t Y0, int32_t tx0,
+                                             int32_t xa, int32_t xb, int32_t ya, int32_t yb) {
     const FillArgs &A = B.F;
     const int lane = threadIdx.x, d = lane >> 2, j = lane & 3, g0 = lane & ~3;
-    constexpr int kPitch = ComposeLds<CH>::kPitch, kRowPitch = ComposeLds<CH>::kRowPitch;
-    const uint32_t a_in = (at >> 16) & 0xffu, src = (at >> 8) & 0xffu, src4 = src * 0x01010101u;
-    const int flavour = A.flavour;
+    using LdsT = ComposeLds<CH, NPL>;
+    constexpr int kPitch = LdsT::kPitch, kRowPitch = LdsT::kRowPitch, kPlane = LdsT::kPlane;
+    const uint32_t a_in = (at >> 16) & 0xffu, fl = at & 0xffu;
+    const bool opaque = a_in == 255u && !(fl & kFillGeneral);
+    const bool fast = A.flavour != RR_FLAVOUR_GRAY8;
+    uint32_t srcc[NPL];
+#pragma unroll
+    for (int c = 0; c < NPL; ++c) srcc[c] = NPL == 1 ? (at >> 8) & 0xffu : (rgb >> (8 * c)) & 0xffu;
     constexpr bool want_cov = COV;
     uint64_t hsum = 0;
     unsigned long long w_cov = 0;
     const int32_t yy = Y0 + d;
     const bool rowv = yy >= ya && yy <= yb;
+    // soft-mask byte of pixel x of this row (255 when the path has none)
+    const uint8_t *mrow = ((fl & RR_FILL_SOFT_MASK) && pg.smask) ? pg.smask + static_cast<long long>(yy) * pg.smask_stride
+                                                                   : nullptr;
+    auto sm_at = [&](int32_t x) -> uint32_t { return mrow ? mrow[x] : 255u; };
     const int32_t Alo = 4 * xa, Ahi = 4 * (xb + 1);
     const int32_t n = rowv ? V.n : 0;
     int32_t s0 = 0, e0 = 0;
@@ -1072,8 +1084,7 @@ __device__ __forceinline__ void compose_rows(const BatchArgs &B, ComposeLds<CH> 
     ub = max(ub, dpp_i32<kDppQuadXor2>(ub));
     const int32_t sk[4] = {quad_bcast<0>(s0), quad_bcast<1>(s0), quad_bcast<2>(s0), quad_bcast<3>(s0)};
     const int32_t ek[4] = {quad_bcast<0>(e0), quad_bcast<1>(e0), quad_bcast<2>(e0), quad_bcast<3>(e0)};
-    uint32_t *rowc = L.cv + d * kRowPitch;
-    uint8_t *rowb = reinterpret_cast<uint8_t *>(rowc);
+    uint32_t *rowc = L.cv + d * kRowPitch;  // plane 0; plane c at + c * kPlane
     const int32_t dwbase = tx0 >> 2;
     const long long pixrow = static_cast<long long>(yy) * pg.width;
     if (rowv && !cx && ua <= ub) {
@@ -1083,16 +1094,20 @@ __device__ __forceinline__ void compose_rows(const BatchArgs &B, ComposeLds<CH> 
             dB = ((fb + 1) >> 2) - 1;
         }
         for (int32_t w = dA + j; w <= dB; w += 4) {
-            uint32_t *cell = rowc + (w - dwbase);
-            uint32_t out = src4;
-            if (a_in != 255u) {
-                const uint32_t old = *cell;
-                out = 0;
 #pragma unroll
-                for (int px = 0; px < 4; ++px)
-                    out |= comp1((old >> (8 * px)) & 0xffu, 255u, a_in, src, flavour) << (8 * px);
+            for (int c = 0; c < NPL; ++c) {
+                uint32_t *cell = rowc + c * kPlane + (w - dwbase);
+                uint32_t out = srcc[c] * 0x01010101u;
+                if (!opaque) {
+                    const uint32_t old = *cell;
+                    out = 0;
+#pragma unroll
+                    for (int px = 0; px < 4; ++px)
+                        out |= paint_px((old >> (8 * px)) & 0xffu, 255u, a_in, srcc[c], fl, sm_at(4 * w + px), fast)
+                               << (8 * px);
+                }
+                *cell = out;
             }
-            *cell = out;
             if (want_cov) {
                 w_cov += 4;
                 if (A.cov_hash)
@@ -1113,10 +1128,14 @@ __device__ __forceinline__ void compose_rows(const BatchArgs &B, ComposeLds<CH> 
 #pragma unroll
             for (int q = 0; q < 4; ++q) cnt += max(0, min(ek[q], X1) - max(sk[q], X0));
             if (cnt == 0) continue;
-            uint8_t *cb = rowb + (x - tx0);
             const uint32_t shape = L.gamma[cnt];
-            const uint32_t res = (cnt == 16 && a_in == 255u) ? src : comp1(*cb, shape, a_in, src, flavour);
-            *cb = static_cast<uint8_t>(res);
+            const uint32_t sm = sm_at(x);
+#pragma unroll
+            for (int c = 0; c < NPL; ++c) {
+                uint8_t *cb = reinterpret_cast<uint8_t *>(rowc + c * kPlane) + (x - tx0);
+                const uint32_t res = (cnt == 16 && opaque) ? srcc[c] : paint_px(*cb, shape, a_in, srcc[c], fl, sm, fast);
+                *cb = static_cast<uint8_t>(res);
+            }
             if (want_cov) {
                 ++w_cov;
                 if (A.cov_hash) hsum += cov_term(pixrow + x, shape);
@@ -1157,24 +1176,29 @@ __device__ __forceinline__ void compose_rows(const BatchArgs &B, ComposeLds<CH> 
                 cvv += (t & 0xfu) | ((t & 0xf0u) << 4) | ((t & 0xf00u) << 8) | ((t & 0xf000u) << 12);
             }
             if (cvv == 0) continue;
-            uint32_t *cell = rowc + w;
-            const uint32_t old = *cell;
-            uint32_t out;
-            if (cvv == 0x10101010u && a_in == 255u) {
-                out = src4;
-            } else {
-                out = old;
+            const int32_t x0 = tx0 + 4 * w;
 #pragma unroll
-                for (int px = 0; px < 4; ++px) {
-                    const uint32_t cnt = (cvv >> (8 * px)) & 0xffu;
-                    if (cnt == 0) continue;
-                    const uint32_t res = comp1((out >> (8 * px)) & 0xffu, L.gamma[cnt], a_in, src, flavour);
-                    out = (out & ~(0xffu << (8 * px))) | (res << (8 * px));
+            for (int c = 0; c < NPL; ++c) {
+                uint32_t *cell = rowc + c * kPlane + w;
+                const uint32_t old = *cell;
+                uint32_t out;
+                if (cvv == 0x10101010u && opaque) {
+                    out = srcc[c] * 0x01010101u;
+                } else {
+                    out = old;
+#pragma unroll
+                    for (int px = 0; px < 4; ++px) {
+                        const uint32_t cnt = (cvv >> (8 * px)) & 0xffu;
+                        if (cnt == 0) continue;
+                        const uint32_t res =
+                            paint_px((out >> (8 * px)) & 0xffu, L.gamma[cnt], a_in, srcc[c], fl, sm_at(x0 + px), fast);
+                        out = (out & ~(0xffu << (8 * px))) | (res << (8 * px));
+                    }
                 }
+                if (out != old) *cell = out;
             }
-            if (out != old) *cell = out;
             if (want_cov) {
-                const long long pix0 = pixrow + tx0 + 4 * w;
+                const long long pix0 = pixrow + x0;
 #pragma unroll
                 for (int px = 0; px < 4; ++px) {
                     const uint32_t cnt = (cvv >> (8 * px)) & 0xffu;
@@ -1197,33 +1221,55 @@ __device__ __forceinline__ void compose_rows(const BatchArgs &B, ComposeLds<CH> 
     }
 }
 
+// The Gray8 output of an Rgb8 tile read straight from the canvas (a tile no
+// path touches keeps the canvas as it was; the epilogue still converts it).
+__device__ __forceinline__ void tile_gray_direct(const DevPage &pg, int32_t Y0, int32_t tx0, int32_t nrows,
+                                                 int32_t ndw) {
+    for (int idx = threadIdx.x; idx < nrows * ndw; idx += 64) {
+        const int r = idx / ndw, w = idx - (idx / ndw) * ndw;
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(pg.canvas + static_cast<long long>(Y0 + r) * pg.stride +
+                                                                 3 * (tx0 + 4 * w));
+        uint32_t cr, cg, cb;
+        deinterleave4(src[0], src[1], src[2], cr, cg, cb);
+        *reinterpret_cast<uint32_t *>(pg.gray + static_cast<long long>(Y0 + r) * pg.gray_stride + tx0 + 4 * w) =
+            gray4(cr, cg, cb);
+    }
+}
+
 // The paths of tile bin `ci` (band `band`, tile `tile` of page `page`), in
 // path order, composited into the tile held in LDS.  Two-deep software
 // pipeline: while path j composites, the first span of path j+1 and the
-// header of path j+2 are in flight.
-template <int CH, bool COV>
-__device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, long long ci, int32_t page, int32_t band,
+// header of path j+2 are in flight.  On an Rgb8 page with a Gray8 output the
+// tile's rgb_to_gray is written as it is stored (every tile of the page).
+template <int CH, bool COV, int NPL>
+__device__ void compose_tile(const BatchArgs &B, ComposeLds<CH, NPL> &L, long long ci, int32_t page, int32_t band,
                              int32_t tile, unsigned long long *n_paths, unsigned long long *n_tiles,
                              unsigned long long *n_filled, uint32_t *n_ivals) {
     const FillArgs &A = B.F;
     const int lane = threadIdx.x;
-    constexpr int kRowWords = ComposeLds<CH>::kRowWords, kRowPitch = ComposeLds<CH>::kRowPitch;
-    const long long b0 = B.bin_off[ci * kBinSegs], b1 = B.bin_off[(ci + 1) * kBinSegs];
-    if (b1 <= b0) return;
+    using LdsT = ComposeLds<CH, NPL>;
+    constexpr int kRowWords = LdsT::kRowWords, kRowPitch = LdsT::kRowPitch, kPlane = LdsT::kPlane;
     const DevPage &pg = A.pages[page];
     const int32_t Y0 = kBandRows * band, tx0 = CH * tile;
     const int32_t width = pg.width;
     const int32_t txe = min(tx0 + CH, width) - 1;
-    if (txe < pg.px_lo || tx0 > pg.px_hi) return;
     const int32_t nrows = min(kBandRows, pg.height - Y0);
     const int32_t ndw = ((txe - tx0) >> 2) + 1;
+    const bool gray = NPL == 3 && pg.gray != nullptr;
+    const uint32_t *colors = NPL == 3 ? A.colors : nullptr;
+    const long long b0 = B.bin_off[ci * kBinSegs], b1 = B.bin_off[(ci + 1) * kBinSegs];
+    if (b1 <= b0 || txe < pg.px_lo || tx0 > pg.px_hi) {
+        if (gray) tile_gray_direct(pg, Y0, tx0, nrows, ndw);
+        return;
+    }
     // Occlusion: the last opaque path that covers the whole tile on all 64 AA
     // rows leaves exactly its colour there (both composites give src for
     // coverage 16 at a_input 255), so compositing starts after it on a tile
     // filled with that colour.  The bin is searched from its end.  Skipped
     // when per-path coverage is counted.
     long long start = b0;
-    int32_t fill_color = -1;
+    bool filled = false;
+    uint32_t fill_c = 0;  // the cover's colour, one byte per plane
     BinCursor cur;
     cur.lb = b0 - 64;
     cur.l1 = b1;
@@ -1248,7 +1294,14 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, long long ci
         }
         if (last >= 0) {
             start = last + 1;
-            fill_color = (__builtin_amdgcn_readlane(e.w, static_cast<int>(last - lb)) >> 8) & 0xff;
+            filled = true;
+            const int k = static_cast<int>(last - lb);
+            if (NPL == 3) {
+                const int32_t pc = __builtin_amdgcn_readlane(e.y, k);
+                fill_c = (pc >= pg.path_begin && pc < pg.path_end) ? colors[pc] : 0u;
+            } else {
+                fill_c = (__builtin_amdgcn_readlane(e.w, k) >> 8) & 0xff;
+            }
         }
         // the walk starts inside the chunk just searched
         cur.lb = lb;
@@ -1257,27 +1310,43 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, long long ci
         cur.ent = e;
     }
     PathRef P0, P1;
-    const bool has0 = bin_next(B, pg, cur, P0);
-    if (!has0 && fill_color < 0) return;
+    const bool has0 = bin_next(B, pg, cur, P0, colors);
+    if (!has0 && !filled) {
+        if (gray) tile_gray_direct(pg, Y0, tx0, nrows, ndw);
+        return;
+    }
     ++*n_tiles;
-    if (fill_color >= 0) ++*n_filled;
-    uint8_t *const base = pg.canvas + static_cast<long long>(Y0) * pg.stride + tx0;
+    if (filled) ++*n_filled;
+    uint8_t *const base = pg.canvas + static_cast<long long>(Y0) * pg.stride + NPL * tx0;
     const int32_t stride = pg.stride;
     long long off0 = -1, off1 = -1;
     int32_t n0 = 0, n1 = 0;
     bool has1 = false;
     if (has0) {
         hdr_fetch(B, P0, &off0, &n0);
-        has1 = bin_next(B, pg, cur, P1);
+        has1 = bin_next(B, pg, cur, P1, colors);
         if (has1) hdr_fetch(B, P1, &off1, &n1);
     }
-    const uint32_t fill4 = static_cast<uint32_t>(fill_color) * 0x01010101u;
     for (int idx = lane; idx < kBandRows * kRowWords; idx += 64) {
         const int r = idx / kRowWords, w = idx % kRowWords;
-        if (r < nrows && w < ndw)
-            L.cv[r * kRowPitch + w] =
-                fill_color >= 0 ? fill4
-                                : *reinterpret_cast<const uint32_t *>(base + static_cast<long long>(r) * stride + 4 * w);
+        if (r < nrows && w < ndw) {
+            if (NPL == 1) {
+                L.cv[r * kRowPitch + w] =
+                    filled ? fill_c * 0x01010101u
+                           : *reinterpret_cast<const uint32_t *>(base + static_cast<long long>(r) * stride + 4 * w);
+            } else {
+                uint32_t pl[3];
+                if (filled) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) pl[c] = ((fill_c >> (8 * c)) & 0xffu) * 0x01010101u;
+                } else {
+                    const uint32_t *src = reinterpret_cast<const uint32_t *>(base + static_cast<long long>(r) * stride) + 3 * w;
+                    deinterleave4(src[0], src[1], src[2], pl[0], pl[1], pl[2]);
+                }
+#pragma unroll
+                for (int c = 0; c < NPL; ++c) L.cv[c * kPlane + r * kRowPitch + w] = pl[c];
+            }
+        }
     }
     SpanView V0;
     V0.n = 0;
@@ -1296,12 +1365,13 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, long long ci
         if (has1) {
             V1 = span_fetch(B, off1, n1);
             if (off1 < 0 && lane == 0) atomicOr(A.err, 8);
-            has2 = bin_next(B, pg, cur, P2);
+            has2 = bin_next(B, pg, cur, P2, colors);
             if (has2) hdr_fetch(B, P2, &off2, &n2);
         }
         // only the columns the path's intervals touch (its span head's cx range)
         const int32_t xa = max(tx0, P0.cx0), xb = min(txe, P0.cx1);
-        if (xa <= xb) compose_rows<CH, COV>(B, L, pg, P0.p, P0.at, V0, Y0, tx0, xa, xb, Y0, Y0 + nrows - 1);
+        if (xa <= xb)
+            compose_rows<CH, COV, NPL>(B, L, pg, P0.p, P0.at, P0.rgb, V0, Y0, tx0, xa, xb, Y0, Y0 + nrows - 1);
         ++*n_paths;
         *n_ivals += static_cast<uint32_t>(V0.n);
         __syncthreads();
@@ -1315,15 +1385,30 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH> &L, long long ci
     }
     for (int idx = lane; idx < kBandRows * kRowWords; idx += 64) {
         const int r = idx / kRowWords, w = idx % kRowWords;
-        if (r < nrows && w < ndw)
-            *reinterpret_cast<uint32_t *>(base + static_cast<long long>(r) * stride + 4 * w) = L.cv[r * kRowPitch + w];
+        if (r < nrows && w < ndw) {
+            if (NPL == 1) {
+                *reinterpret_cast<uint32_t *>(base + static_cast<long long>(r) * stride + 4 * w) = L.cv[r * kRowPitch + w];
+            } else {
+                const uint32_t cr = L.cv[r * kRowPitch + w], cg = L.cv[kPlane + r * kRowPitch + w],
+                               cb = L.cv[2 * kPlane + r * kRowPitch + w];
+                uint32_t o0, o1, o2;
+                interleave4(cr, cg, cb, o0, o1, o2);
+                uint32_t *dst = reinterpret_cast<uint32_t *>(base + static_cast<long long>(r) * stride) + 3 * w;
+                dst[0] = o0;
+                dst[1] = o1;
+                dst[2] = o2;
+                if (gray)
+                    *reinterpret_cast<uint32_t *>(pg.gray + static_cast<long long>(Y0 + r) * pg.gray_stride + tx0 +
+                                                  4 * w) = gray4(cr, cg, cb);
+            }
+        }
     }
     __syncthreads();
 }
 
-template <int CH, bool COV>
+template <int CH, bool COV, int NPL>
 __global__ void __launch_bounds__(64) k_compose(BatchArgs B, long long n_items) {
-    __shared__ ComposeLds<CH> L;
+    __shared__ ComposeLds<CH, NPL> L;
     if (threadIdx.x < 17) L.gamma[threadIdx.x] = kGamma[threadIdx.x];
     __syncthreads();
     const int lane = threadIdx.x;
@@ -1347,7 +1432,7 @@ __global__ void __launch_bounds__(64) k_compose(BatchArgs B, long long n_items) 
         const CPage cp = B.cpages[lo];
         const int32_t local = static_cast<int32_t>(it - cp.base);
         const int32_t band = local / cp.tiles, tile = local - band * cp.tiles;
-        compose_tile<CH, COV>(B, L, it, cp.page, band, tile, &n_paths, &n_tiles, &n_filled, &n_ivals);
+        compose_tile<CH, COV, NPL>(B, L, it, cp.page, band, tile, &n_paths, &n_tiles, &n_filled, &n_ivals);
         ++it;
     }
     const int32_t ivals = __builtin_amdgcn_readlane(wave_incl_add(static_cast<int32_t>(n_ivals)), 63);
@@ -1379,22 +1464,17 @@ const void *span_kernel(int k) {
     }
 }
 
-const void *compose_kernel(int ch, bool cov) {
-    if (cov) {
-        switch (ch) {
-            case 64: return reinterpret_cast<const void *>(&k_compose<64, true>);
-            case 128: return reinterpret_cast<const void *>(&k_compose<128, true>);
-            case 256: return reinterpret_cast<const void *>(&k_compose<256, true>);
-            default: return reinterpret_cast<const void *>(&k_compose<512, true>);
-        }
-    }
-    switch (ch) {
-        case 64: return reinterpret_cast<const void *>(&k_compose<64, false>);
-        case 128: return reinterpret_cast<const void *>(&k_compose<128, false>);
-        case 256: return reinterpret_cast<const void *>(&k_compose<256, false>);
-        default: return reinterpret_cast<const void *>(&k_compose<512, false>);
-    }
+#define RR_COMPOSE(ch, cov, npl) reinterpret_cast<const void *>(&k_compose<ch, cov, npl>)
+const void *compose_kernel(int ch, bool cov, int npl) {
+    static const void *const table[2][2][4] = {
+        {{RR_COMPOSE(64, false, 1), RR_COMPOSE(128, false, 1), RR_COMPOSE(256, false, 1), RR_COMPOSE(512, false, 1)},
+         {RR_COMPOSE(64, true, 1), RR_COMPOSE(128, true, 1), RR_COMPOSE(256, true, 1), RR_COMPOSE(512, true, 1)}},
+        {{RR_COMPOSE(64, false, 3), RR_COMPOSE(128, false, 3), RR_COMPOSE(256, false, 3), RR_COMPOSE(512, false, 3)},
+         {RR_COMPOSE(64, true, 3), RR_COMPOSE(128, true, 3), RR_COMPOSE(256, true, 3), RR_COMPOSE(512, true, 3)}}};
+    const int ci = ch == 64 ? 0 : (ch == 128 ? 1 : (ch == 256 ? 2 : 3));
+    return table[npl == 3 ? 1 : 0][cov ? 1 : 0][ci];
 }
+#undef RR_COMPOSE
 
 template <typename T>
 int32_t fill(rr_hip_ctx *c, T *p, long long n, T value) {
@@ -1457,11 +1537,16 @@ int32_t scan_counts(rr_hip_ctx *c, const int32_t *in, long long *out, long long 
 int32_t init_batched(rr_hip_ctx *c) {
     for (int v = 0; v < kNumVariants; ++v) {
         int a = 0, b = 0;
+        int b3 = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, span_kernel(kVariantK[v]), 64, 0) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, compose_kernel(kVariantCH[v], false), 64, 0) != hipSuccess)
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, compose_kernel(kVariantCH[v], false, 1), 64, 0) !=
+                hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b3, compose_kernel(kVariantCH[v], false, 3), 64, 0) !=
+                hipSuccess)
             return fail(c, RR_E_DEVICE, "occupancy query failed");
         c->span_waves[v] = std::max(1, c->n_cu * std::max(1, a));
         c->compose_waves[v] = std::max(1, c->n_cu * std::max(1, b));
+        c->compose_waves3[v] = std::max(1, c->n_cu * std::max(1, b3));
     }
     int w = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&w, reinterpret_cast<const void *>(&k_spans_wide<kWideK>), 64, 0) !=
@@ -1607,6 +1692,10 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     // the sizing pass before a capture counts every interval (no skipping)
     int32_t mode = c->size_only ? SPAN_ALL_NOSKIP : SPAN_ALL;
     void *a_args[] = {&B, &mode};
+    // composite pass: channel planes of the canvas and its persistent grid
+    const int npl = b.flavour == RR_FLAVOUR_RGB8 ? 3 : 1;
+    const unsigned b_waves = static_cast<unsigned>(
+        std::min<long long>(npl == 3 ? c->compose_waves3[variant] : c->compose_waves[variant], n_citems));
     c->batch_valid = true;
     c->batch_paths = n_paths;
     c->batch_items = n_items;
@@ -1634,9 +1723,8 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
             rc = place_bins();
             if (rc) return rc;
             c->prof_b = c->prof_used;
-            const unsigned b_waves = static_cast<unsigned>(std::min<long long>(c->compose_waves[variant], n_citems));
             void *b_args[] = {&B, &n_citems};
-            return launch_profiled(c, compose_kernel(chunk_px, A.cov_hash || A.cov_count), b_waves, b_args);
+            return launch_profiled(c, compose_kernel(chunk_px, A.cov_hash || A.cov_count, npl), b_waves, b_args);
         }
         unsigned long long k[kCounters] = {};
         long long n_bin = 0;
@@ -1684,9 +1772,8 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     c->prof_b = c->prof_used;
 
     // phase B
-    const unsigned b_waves = static_cast<unsigned>(std::min<long long>(c->compose_waves[variant], n_citems));
     void *b_args[] = {&B, &n_citems};
-    return launch_profiled(c, compose_kernel(chunk_px, A.cov_hash || A.cov_count), b_waves, b_args);
+    return launch_profiled(c, compose_kernel(chunk_px, A.cov_hash || A.cov_count, npl), b_waves, b_args);
 }
 
 int32_t zero_u64(rr_hip_ctx *c, unsigned long long *p, long long n) { return fill<unsigned long long>(c, p, n, 0ull); }

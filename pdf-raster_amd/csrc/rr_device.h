@@ -53,6 +53,10 @@ struct DevPage {
     int32_t band_base;     // batch-global index of the page's first 16-row band
     int32_t cbase, ctiles;  // batched: first composite tile item of the page, tiles per band
     int32_t pad2;
+    const uint8_t *smask;  // page-space soft mask (RR_FILL_SOFT_MASK paths) or null
+    long long smask_stride;
+    uint8_t *gray;  // RR_FLAVOUR_RGB8: Gray8 output of the finished page, or null
+    long long gray_stride;
 };
 
 struct FillArgs {
@@ -72,6 +76,7 @@ struct FillArgs {
     int32_t *err;                   // set to 1 when the pool runs out
     long long spill_cap;
     int32_t mode, flavour;
+    const uint32_t *colors;  // RR_FLAVOUR_RGB8: per path r | g << 8 | b << 16
     // optional timing trace: one record per (path, band) fill
     longlong4 *trace;  // {path, (page << 32) | first row, t_start, t_end} wall-clock ticks
     unsigned long long *trace_top;
@@ -195,6 +200,11 @@ __device__ __forceinline__ unsigned long long wave_shr1_u64(unsigned long long v
     return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
 
+// Error-word bits a kernel sets when a page's output is incomplete (1, 2, 4,
+// 8, 16, 32, 64: workspace capacity) or its input is inconsistent (128: a
+// path asks for the soft mask of a page without one).
+constexpr int32_t kErrNoSoftMask = 128;
+
 static __constant__ uint8_t kGamma[17] = {0, 4, 11, 21, 32, 45, 59, 74, 90, 108, 126, 145, 166, 187, 209, 231, 255};
 
 // ---------------------------------------------------------------------------
@@ -228,6 +238,68 @@ __device__ __forceinline__ int32_t sat64_to_i32(long long v) {
 __device__ __forceinline__ uint32_t div255(uint32_t x) {
     const uint32_t s = (x + (x >> 8) + 0x80u) >> 8;
     return s < 255u ? s : 255u;
+}
+
+// The general pipe's share of a path's fill flags (rr_path_attr.eo bits):
+// with either, the paint is compute_a_src + the no-alpha general composite.
+constexpr uint32_t kFillGeneral = RR_FILL_MULTIPLY | RR_FILL_SOFT_MASK;
+
+// One channel of the fill's composite (pipe/mod.rs:157-199): the general
+// pipe's no-alpha arm (pipe/general.rs:270-344: a_src = compute_a_src with a
+// shape; Normal c = div255((255 - a) d + a s), Multiply with blend_multiply
+// = div255(s d), blend.rs:24-26) when the path is Multiply or soft-masked,
+// else the AA path: Gray8 exact div255 (pipe/aa.rs:192-208) or the Rgb8 fast
+// (v + 255) >> 8 (simd/composite.rs:56-114).
+// Products of bytes go through v_mul_u32_u24: with both factors known to be
+// bytes the compiler may otherwise fold div255(s * d) into a dot-product
+// sequence whose results differ from the formula's.
+__device__ __forceinline__ uint32_t mul24_u(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ uint32_t paint_px(uint32_t dv, uint32_t shape, uint32_t a_in, uint32_t src, uint32_t fl,
+                                             uint32_t sm, bool fast) {
+    if (fl & kFillGeneral) {
+        const uint32_t a = (fl & RR_FILL_SOFT_MASK) ? div255(mul24_u(div255(mul24_u(a_in, sm)), shape))
+                                                    : div255(mul24_u(a_in, shape));
+        const uint32_t cb = (fl & RR_FILL_MULTIPLY) ? div255(mul24_u(src, dv)) : src;
+        return div255(mul24_u(255u - a, dv) + mul24_u(a, cb));
+    }
+    if (fast) {
+        const uint32_t a = (a_in * shape + 255u) >> 8;
+        return ((255u - a) * dv + a * src + 255u) >> 8;
+    }
+    const uint32_t a = div255(a_in * shape);
+    return div255((255u - a) * dv + a * src);
+}
+
+// rgb_to_gray (crates/pdf_raster/src/render.rs:1306-1324) of 4 pixels held
+// as three channel dwords.
+__device__ __forceinline__ uint32_t gray4(uint32_t r, uint32_t g, uint32_t b) {
+    uint32_t out = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t v = (2126u * ((r >> (8 * k)) & 0xffu) + 7152u * ((g >> (8 * k)) & 0xffu) +
+                            722u * ((b >> (8 * k)) & 0xffu) + 5000u) / 10000u;
+        out |= v << (8 * k);
+    }
+    return out;
+}
+
+// 4 interleaved Rgb8 pixels (three dwords w0..w2, bytes r0 g0 b0 r1 ...) as
+// three channel dwords, and back.
+__device__ __forceinline__ void deinterleave4(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t &r, uint32_t &g,
+                                              uint32_t &b) {
+    r = (w0 & 0xffu) | ((w0 >> 24) & 0xffu) << 8 | ((w1 >> 16) & 0xffu) << 16 | ((w2 >> 8) & 0xffu) << 24;
+    g = ((w0 >> 8) & 0xffu) | (w1 & 0xffu) << 8 | ((w1 >> 24) & 0xffu) << 16 | ((w2 >> 16) & 0xffu) << 24;
+    b = ((w0 >> 16) & 0xffu) | ((w1 >> 8) & 0xffu) << 8 | (w2 & 0xffu) << 16 | ((w2 >> 24) & 0xffu) << 24;
+}
+__device__ __forceinline__ void interleave4(uint32_t r, uint32_t g, uint32_t b, uint32_t &w0, uint32_t &w1,
+                                            uint32_t &w2) {
+    w0 = (r & 0xffu) | (g & 0xffu) << 8 | (b & 0xffu) << 16 | ((r >> 8) & 0xffu) << 24;
+    w1 = ((g >> 8) & 0xffu) | ((b >> 8) & 0xffu) << 8 | ((r >> 16) & 0xffu) << 16 | ((g >> 16) & 0xffu) << 24;
+    w2 = ((b >> 16) & 0xffu) | ((r >> 24) & 0xffu) << 8 | ((g >> 24) & 0xffu) << 16 | ((b >> 24) & 0xffu) << 24;
 }
 
 __device__ __forceinline__ uint64_t cov_term(long long pix, uint32_t shape) {

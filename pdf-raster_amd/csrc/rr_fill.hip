@@ -94,7 +94,8 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
                                                     EdgeRec *__restrict__ recs, PathHdr *__restrict__ hdrs,
                                                     int2 *__restrict__ rows, int4 *__restrict__ sel,
                                                     int32_t *__restrict__ nb, const rr_path_attr *__restrict__ attrs,
-                                                    float *__restrict__ feat, int2 *__restrict__ csum, int32_t mode) {
+                                                    float *__restrict__ feat, int2 *__restrict__ csum, int32_t mode,
+                                                    int32_t *__restrict__ err) {
     const int64_t p = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (p >= n_paths) return;
@@ -217,6 +218,7 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
         dy_sum = wave_sum_f64(dy_sum);
     }
     if (lane == 0) {
+        if ((attrs[p].eo & RR_FILL_SOFT_MASK) && !pg.smask) atomicOr(err, kErrNoSoftMask);
         PathHdr h{0, -1, 1, 0, HDR_EMPTY, lo, 0, 0};
         int2 drawn = make_int2(1, 0);
         if (any) {
@@ -262,7 +264,7 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
             f[3] = log2f(1.f + area);
             f[4] = log2f(1.f + E / fmaxf(1.f, 4.f * Hh));
             f[5] = static_cast<float>(bands);
-            f[6] = at.eo ? 1.f : 0.f;
+            f[6] = (at.eo & RR_FILL_EVEN_ODD) ? 1.f : 0.f;
             f[7] = log2f((W + 1.f) / (Hh + 1.f));
             f[8] = fminf(1.f, static_cast<float>(fabs(0.5 * shoelace)) / fmaxf(1.f, area));
             f[9] = log2f(1.f + static_cast<float>(4.0 * dy_sum) / fmaxf(1.f, E));
@@ -298,7 +300,7 @@ __device__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &p
     const int32_t band_lo = literal ? 16 * Y0 + 12 : 4 * Y0;
     const int32_t band_hi = literal ? 16 * (Y0 + kBandRows - 1) + 15 : 4 * Y0 + 4 * kBandRows - 1;
     const bool valid = y >= ylo && y <= yhi && r >= h.y_min && r <= h.y_max;
-    const int32_t mrule = at.eo ? 1 : -1;
+    const int32_t mrule = (at.eo & RR_FILL_EVEN_ODD) ? 1 : -1;
     const int32_t aa_width = pg.width * 4;
 
     int32_t *X0 = L.ex0, *X1 = L.ex1, *C = L.ec;
@@ -352,7 +354,13 @@ __device__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &p
     const int32_t px_hi = min(pg.px_hi, any_unb ? pg.width - 1 : floor_div4(h.x_max));
     const bool any_px = px_lo <= px_hi;
     const int32_t xs = px_lo & ~3;
-    const uint32_t a_in = at.a_input, src = at.color;
+    const uint32_t a_in = at.a_input, src = at.color, fl = at.eo;
+    // RR_FLAVOUR_RGB8: three channel planes of the path's colour; every other
+    // flavour one grey byte a pixel
+    const bool rgb = A.flavour == RR_FLAVOUR_RGB8, fast = A.flavour != RR_FLAVOUR_GRAY8;
+    const uint32_t col = rgb ? A.colors[p] : src * 0x010101u;
+    const bool opaque = a_in == 255u && !(fl & kFillGeneral);
+    const bool use_mask = (fl & RR_FILL_SOFT_MASK) && pg.smask;
     uint64_t hsum = 0;
     unsigned long long w_cov = 0;
     int ip = 0;  // first interval of this lane not yet fully left of the chunk
@@ -388,7 +396,7 @@ __device__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &p
         const int32_t xq = cx + 4 * q;  // this lane's 4 pixels
         const int sh = (q & 1) * 16;
         const bool lane_in = xq <= px_hi;
-        const bool full_ok = a_in == 255u && xq >= px_lo && xq + 3 <= px_hi;
+        const bool full_ok = opaque && xq >= px_lo && xq + 3 <= px_hi;
         // Rows in groups: coverage from the bitmap, then the canvas loads of
         // the group issued back to back, then composite + store.
         for (int i0 = 0; i0 < kIters; i0 += kGroup) {
@@ -409,7 +417,7 @@ __device__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &p
                 }
                 cov[k] = cv;
                 old[k] = 0;
-                if (cv != 0 && !(cv == 0x10101010u && full_ok))
+                if (!rgb && cv != 0 && !(cv == 0x10101010u && full_ok))
                     old[k] = *reinterpret_cast<const uint32_t *>(pg.canvas + static_cast<long long>(yy) * pg.stride + xq);
             }
 #pragma unroll
@@ -417,30 +425,57 @@ __device__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &p
                 const uint32_t cv = cov[k];
                 if (cv == 0) continue;
                 const int32_t yy = Y0 + (i0 + k) * kRowsPerIter + sub;
-                uint32_t *dst = reinterpret_cast<uint32_t *>(pg.canvas + static_cast<long long>(yy) * pg.stride + xq);
-                uint32_t out;
-                if (cv == 0x10101010u && full_ok) {
-                    out = src4;  // full coverage of an opaque source: both composites give src
-                } else {
-                    out = old[k];
-                    for (int px = 0; px < 4; ++px) {
-                        const uint32_t cnt = (cv >> (8 * px)) & 0xffu;
-                        const int32_t x = xq + px;
-                        if (cnt == 0 || x < px_lo || x > px_hi) continue;
-                        const uint32_t shape = L.gamma[cnt];
-                        const uint32_t dv = (out >> (8 * px)) & 0xffu;
-                        uint32_t res;
-                        if (A.flavour == RR_FLAVOUR_RGB8_FAST) {
-                            const uint32_t a = (a_in * shape + 255u) >> 8;
-                            res = ((255u - a) * dv + a * src + 255u) >> 8;
-                        } else {
-                            const uint32_t a = div255(a_in * shape);
-                            res = div255((255u - a) * dv + a * src);
+                const bool full = cv == 0x10101010u && full_ok;
+                // soft-mask bytes of the 4 pixels (page space, 4-byte aligned rows)
+                const uint32_t smw = use_mask && !full
+                                         ? *reinterpret_cast<const uint32_t *>(pg.smask + static_cast<long long>(yy) *
+                                                                                              pg.smask_stride + xq)
+                                         : ~0u;
+                if (rgb) {
+                    uint32_t *dst = reinterpret_cast<uint32_t *>(pg.canvas + static_cast<long long>(yy) * pg.stride + 3 * xq);
+                    uint32_t pl[3];
+                    const uint32_t w0 = full ? 0u : dst[0], w1 = full ? 0u : dst[1], w2 = full ? 0u : dst[2];
+                    if (full) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) pl[c] = ((col >> (8 * c)) & 0xffu) * 0x01010101u;
+                    } else {
+                        deinterleave4(w0, w1, w2, pl[0], pl[1], pl[2]);
+                        for (int px = 0; px < 4; ++px) {
+                            const uint32_t cnt = (cv >> (8 * px)) & 0xffu;
+                            const int32_t x = xq + px;
+                            if (cnt == 0 || x < px_lo || x > px_hi) continue;
+                            const uint32_t shape = L.gamma[cnt], sm = (smw >> (8 * px)) & 0xffu;
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) {
+                                const uint32_t res = paint_px((pl[c] >> (8 * px)) & 0xffu, shape, a_in,
+                                                              (col >> (8 * c)) & 0xffu, fl, sm, fast);
+                                pl[c] = (pl[c] & ~(0xffu << (8 * px))) | (res << (8 * px));
+                            }
                         }
-                        out = (out & ~(0xffu << (8 * px))) | (res << (8 * px));
                     }
+                    uint32_t o0, o1, o2;
+                    interleave4(pl[0], pl[1], pl[2], o0, o1, o2);
+                    if (full || o0 != w0) dst[0] = o0;
+                    if (full || o1 != w1) dst[1] = o1;
+                    if (full || o2 != w2) dst[2] = o2;
+                } else {
+                    uint32_t *dst = reinterpret_cast<uint32_t *>(pg.canvas + static_cast<long long>(yy) * pg.stride + xq);
+                    uint32_t out;
+                    if (full) {
+                        out = src4;  // full coverage of an opaque source: both composites give src
+                    } else {
+                        out = old[k];
+                        for (int px = 0; px < 4; ++px) {
+                            const uint32_t cnt = (cv >> (8 * px)) & 0xffu;
+                            const int32_t x = xq + px;
+                            if (cnt == 0 || x < px_lo || x > px_hi) continue;
+                            const uint32_t res = paint_px((out >> (8 * px)) & 0xffu, L.gamma[cnt], a_in, src, fl,
+                                                          (smw >> (8 * px)) & 0xffu, fast);
+                            out = (out & ~(0xffu << (8 * px))) | (res << (8 * px));
+                        }
+                    }
+                    if (out != old[k] || full) *dst = out;
                 }
-                if (out != old[k] || cv == 0x10101010u) *dst = out;
                 if (A.cov_count || A.cov_hash) {
                     for (int px = 0; px < 4; ++px) {
                         const uint32_t cnt = (cv >> (8 * px)) & 0xffu;
@@ -575,12 +610,21 @@ ClipRect make_clip(const double cl[4]) {
     return r;
 }
 
-int32_t validate_page(rr_hip_ctx *c, const rr_page_desc &p) {
-    if (p.width <= 0 || p.height <= 0 || p.stride < ((p.width + 3) & ~3) || (p.stride & 3) != 0)
-        return fail(c, RR_E_INVARIANT, "page: bad canvas geometry (stride must be a multiple of 4 >= width)");
+int32_t validate_page(rr_hip_ctx *c, const rr_page_desc &p, int32_t flavour) {
+    const int32_t bpp = flavour == RR_FLAVOUR_RGB8 ? 3 : 1;
+    const int32_t w4 = (p.width + 3) & ~3;
+    if (p.width <= 0 || p.height <= 0 || p.stride < bpp * w4 || (p.stride & 3) != 0)
+        return fail(c, RR_E_INVARIANT,
+                    "page: bad canvas geometry (stride must be a multiple of 4 >= bytes of width rounded up to 4)");
     if (!p.canvas || (reinterpret_cast<uintptr_t>(p.canvas) & 3u))
         return fail(c, RR_E_INVARIANT, "page: canvas must be a 4-byte aligned device pointer");
     if (p.path_begin < 0 || p.path_end < p.path_begin) return fail(c, RR_E_INVARIANT, "page: bad path range");
+    if (p.soft_mask && ((reinterpret_cast<uintptr_t>(p.soft_mask) & 3u) || (p.soft_mask_stride & 3) ||
+                        p.soft_mask_stride < w4))
+        return fail(c, RR_E_INVARIANT, "page: soft mask must be 4-byte aligned with a stride a multiple of 4 >= width");
+    if (p.gray && flavour != RR_FLAVOUR_RGB8) return fail(c, RR_E_INVARIANT, "page: a gray output needs RR_FLAVOUR_RGB8");
+    if (p.gray && ((reinterpret_cast<uintptr_t>(p.gray) & 3u) || (p.gray_stride & 3) || p.gray_stride < w4))
+        return fail(c, RR_E_INVARIANT, "page: gray output must be 4-byte aligned with a stride a multiple of 4 >= width");
     return RR_OK;
 }
 
@@ -589,8 +633,10 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     if (!c->page_open) return fail(c, RR_E_INVARIANT, "record outside begin_page/submit_page");
     if (b.n_paths < 0 || b.n_edges < 0 || b.n_pages < 0) return fail(c, RR_E_INVARIANT, "negative counts");
     if (b.mode != RR_MODE_DOCUMENTED && b.mode != RR_MODE_LITERAL) return fail(c, RR_E_INVARIANT, "unknown AA row mode");
-    if (b.flavour != RR_FLAVOUR_GRAY8 && b.flavour != RR_FLAVOUR_RGB8_FAST)
+    if (b.flavour != RR_FLAVOUR_GRAY8 && b.flavour != RR_FLAVOUR_RGB8_FAST && b.flavour != RR_FLAVOUR_RGB8)
         return fail(c, RR_E_INVARIANT, "unknown canvas flavour");
+    if (b.flavour == RR_FLAVOUR_RGB8 && b.n_paths > 0 && (!b.colors || (reinterpret_cast<uintptr_t>(b.colors) & 3u)))
+        return fail(c, RR_E_INVARIANT, "RR_FLAVOUR_RGB8 needs a 4-byte aligned device array of path colours");
     if (b.launch != RR_LAUNCH_PLAIN && b.launch != RR_LAUNCH_BATCHED)
         return fail(c, RR_E_INVARIANT, "unknown launch shape");
     if (b.variant < 0 || b.variant >= kNumVariants) return fail(c, RR_E_INVARIANT, "unknown fill variant");
@@ -603,7 +649,7 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     c->host_pages.resize(static_cast<size_t>(b.n_pages));
     for (int32_t i = 0; i < b.n_pages; ++i) {
         const rr_page_desc &p = b.pages[i];
-        int32_t rc = validate_page(c, p);
+        int32_t rc = validate_page(c, p, b.flavour);
         if (rc) return rc;
         if (p.path_begin != prev_end) return fail(c, RR_E_INVARIANT, "pages must tile [0, n_paths) in order");
         prev_end = p.path_end;
@@ -621,6 +667,10 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
         d.cbase = 0;
         d.ctiles = 0;
         d.pad2 = 0;
+        d.smask = p.soft_mask;
+        d.smask_stride = p.soft_mask_stride;
+        d.gray = p.gray;
+        d.gray_stride = p.gray_stride;
     }
     if (prev_end != b.n_paths) return fail(c, RR_E_INVARIANT, "pages must tile [0, n_paths) in order");
     RR_HIP_TRY(c, hipSetDevice(c->device));
@@ -694,6 +744,7 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     A.spill_cap = c->spill_cap;
     A.mode = b.mode;
     A.flavour = b.flavour;
+    A.colors = reinterpret_cast<const uint32_t *>(b.colors);
     A.slot_ints = kSlotInts;
     A.trace = c->trace;
     A.trace_cap = c->trace_cap;
@@ -711,7 +762,7 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     if (rc) return rc;
     hipLaunchKernelGGL(k_edge_setup, dim3(static_cast<unsigned>((b.n_paths + 3) / 4)), dim3(256), 0, c->stream,
                        b.edges, b.edge_off, b.n_paths, c->pages, b.n_pages, c->recs, c->hdrs, c->rows, c->sel, c->nb,
-                       b.attrs, b.launch == RR_LAUNCH_BATCHED ? c->feat : nullptr, c->csum, b.mode);
+                       b.attrs, b.launch == RR_LAUNCH_BATCHED ? c->feat : nullptr, c->csum, b.mode, c->err_flag);
     RR_HIP_TRY(c, hipGetLastError());
 
     if (b.launch == RR_LAUNCH_PLAIN) {
@@ -736,6 +787,11 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
                 RR_HIP_TRY(c, hipLaunchKernel(kPlainKernels[b.variant], dim3(bands), dim3(64), args, 0, c->stream));
                 if (ev) RR_HIP_TRY(c, hipEventRecord(ev[1], c->stream));
                 ++c->fill_launches;
+            }
+            if (pg.gray) {
+                // the page's Gray8 output once its last path is composited
+                rc = rrint::rgb_to_gray(c, pg.canvas, pg.stride, pg.gray, pg.gray_stride, pg.width, pg.height);
+                if (rc) return rc;
             }
         }
         RR_HIP_TRY(c, hipGetLastError());
@@ -957,6 +1013,11 @@ int32_t rr_hip_record_fill_paths(rr_hip_ctx *c, const rr_fill_batch *b) {
     fp.cov_hash = b->cov_hash;
     fp.cov_count = b->cov_count;
     fp.variant = b->variant;
+    fp.colors = b->colors;
+    pg.soft_mask = b->soft_mask;
+    pg.soft_mask_stride = b->soft_mask_stride;
+    pg.gray = b->gray;
+    pg.gray_stride = b->gray_stride;
     return record_pages(c, fp);
 }
 
@@ -1015,6 +1076,8 @@ int32_t rr_hip_wait_page(rr_hip_ctx *c, uint64_t fence) {
         if (hipEventElapsedTime(&ms, c->prof_ev[k], c->prof_ev[k + 1]) == hipSuccess) fill += ms;
     }
     c->last_fill_ms = fill;
+    if (flag & kErrNoSoftMask)
+        return fail(c, RR_E_INVARIANT, "a path asked for the soft mask of a page that has none (filled unmasked)");
     if (flag & 1) return fail(c, RR_E_CAPACITY, "overflow-entry pool exhausted: page output is incomplete");
     if (flag) return fail(c, RR_E_CAPACITY, "batched workspace overflow (code " + std::to_string(flag) + "): page output is incomplete");
     return RR_OK;

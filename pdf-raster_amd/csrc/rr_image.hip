@@ -505,6 +505,21 @@ int32_t launch_profiled(rr_hip_ctx *c, F &&launch) {
 
 }  // namespace
 
+namespace rrint {
+int32_t rgb_to_gray(rr_hip_ctx *c, const uint8_t *rgb, int64_t rgb_stride, uint8_t *gray, int64_t gray_stride,
+                    int32_t width, int32_t height) {
+    if (width <= 0 || height <= 0) return RR_OK;
+    if (height > 65535) return fail(c, RR_E_INVARIANT, "rgb_to_gray: height > 65535");
+    const int32_t aligned = ((reinterpret_cast<uintptr_t>(rgb) | reinterpret_cast<uintptr_t>(gray) |
+                              static_cast<uintptr_t>(rgb_stride) | static_cast<uintptr_t>(gray_stride)) & 3u) == 0;
+    const dim3 grid(static_cast<unsigned>((width + 1023) / 1024), static_cast<unsigned>(height));
+    return launch_profiled(c, [&] {
+        hipLaunchKernelGGL(k_rgb_to_gray, grid, dim3(256), 0, stream(c), rgb, rgb_stride, gray, gray_stride, width,
+                           height, aligned);
+    });
+}
+}  // namespace rrint
+
 extern "C" {
 
 int32_t rr_hip_record_place_images(rr_hip_ctx *c, const rr_image_batch *b) {

@@ -89,7 +89,7 @@ struct SpanPath {
     int32_t width, height;     // canvas
     int32_t px_hi;             // last column of the page clip
     int32_t occ_base, ctiles;  // the page's first composite tile, tiles per band
-    uint32_t flags;            // bit 0 even-odd, bits 8-15 a_input, bits 16-31 first clip column (<= 65535)
+    uint32_t flags;            // bit 0 even-odd, bit 1 opaque AA paint, bits 8-15 a_input, bits 16-31 first clip column
 };
 static_assert(sizeof(SpanPath) == 64, "SpanPath layout");
 
@@ -222,6 +222,7 @@ struct rr_hip_ctx {
     uint64_t ws_gen = 0;  // bumped whenever a workspace array is reallocated
     int span_waves[rrdev::kNumVariants] = {};
     int compose_waves[rrdev::kNumVariants] = {};
+    int compose_waves3[rrdev::kNumVariants] = {};  // the Rgb8 (three-plane) composite pass
     int wide_waves = 1;  // persistent grid of the wide span pass
     std::vector<rrdev::CPage> host_cpages;
     // page state
@@ -296,6 +297,9 @@ int32_t fill_i32(rr_hip_ctx *c, int32_t *p, long long n, int32_t value);
 // others (a kernel, so that it is ordered on the stream and capturable).
 int32_t clear_err_bits(rr_hip_ctx *c, int32_t mask);
 int32_t set_err_bits(rr_hip_ctx *c, int32_t bits);
+// Records rgb_to_gray of an Rgb8 bitmap into a Gray8 one (rr_image.hip).
+int32_t rgb_to_gray(rr_hip_ctx *c, const uint8_t *rgb, int64_t rgb_stride, uint8_t *gray, int64_t gray_stride,
+                    int32_t width, int32_t height);
 // Records the edge setup + fill of b (rr_fill.hip); used by graph capture.
 int32_t record_pages_internal(rr_hip_ctx *c, const rr_fill_pages &b);
 }  // namespace rrint

@@ -15,7 +15,8 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "rr_hip.h")
 
 RR_OK = 0
 MODE_DOCUMENTED, MODE_LITERAL = 0, 1
-FLAVOUR_GRAY8, FLAVOUR_RGB8_FAST = 0, 1
+FLAVOUR_GRAY8, FLAVOUR_RGB8_FAST, FLAVOUR_RGB8 = 0, 1, 2
+FILL_EVEN_ODD, FILL_MULTIPLY, FILL_SOFT_MASK = 1, 2, 4
 LAUNCH_PLAIN, LAUNCH_BATCHED = 0, 1
 RR_OK, RR_E_ZERO_SIZE_ALLOC, RR_E_INVARIANT, RR_E_DEVICE, RR_E_NO_DEVICE, RR_E_CAPACITY = 0, -1, -2, -3, -4, -5
 ROUTE_SPAN, ROUTE_TILE16, ROUTE_TILE32, ROUTE_TILE64, ROUTE_CPU, ROUTE_AUTO = 0, 1, 2, 3, 4, 255
@@ -39,14 +40,19 @@ class FillBatch(ctypes.Structure):
                 ("height", ctypes.c_int32), ("stride", ctypes.c_int32), ("mode", ctypes.c_int32),
                 ("flavour", ctypes.c_int32), ("launch", ctypes.c_int32),
                 ("clip", ctypes.c_double * 4), ("cov_hash", ctypes.c_void_p),
-                ("cov_count", ctypes.c_void_p), ("variant", ctypes.c_int32)]
+                ("cov_count", ctypes.c_void_p), ("variant", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("colors", ctypes.c_void_p), ("soft_mask", ctypes.c_void_p),
+                ("soft_mask_stride", ctypes.c_int64), ("gray", ctypes.c_void_p),
+                ("gray_stride", ctypes.c_int64)]
 
 
 class PageDesc(ctypes.Structure):
     _fields_ = [("canvas", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
                 ("stride", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("path_begin", ctypes.c_int64), ("path_end", ctypes.c_int64),
-                ("clip", ctypes.c_double * 4)]
+                ("clip", ctypes.c_double * 4), ("soft_mask", ctypes.c_void_p),
+                ("soft_mask_stride", ctypes.c_int64), ("gray", ctypes.c_void_p),
+                ("gray_stride", ctypes.c_int64)]
 
 
 class FillPages(ctypes.Structure):
@@ -56,7 +62,8 @@ class FillPages(ctypes.Structure):
                 ("pages", ctypes.c_void_p), ("n_pages", ctypes.c_int32),
                 ("mode", ctypes.c_int32), ("flavour", ctypes.c_int32), ("launch", ctypes.c_int32),
                 ("cov_hash", ctypes.c_void_p), ("cov_count", ctypes.c_void_p),
-                ("variant", ctypes.c_int32)]
+                ("variant", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("colors", ctypes.c_void_p)]
 
 
 class ImageDesc(ctypes.Structure):
@@ -638,10 +645,15 @@ class DevicePage:
 
 class DeviceBatch:
     """Several pages resident on the device: edges, offsets and attributes
-    concatenated with global path indices, one canvas per page."""
+    concatenated with global path indices, one canvas per page.
 
-    def __init__(self, be, pages, attrs=None):
+    rgb: 3-byte Rgb8 canvases with per-path colours (colors: n_paths x 3
+    uint8); gray: a Gray8 output per page (Rgb8 only); soft_masks: one
+    (H, W) uint8 page-space mask per page, or None."""
+
+    def __init__(self, be, pages, attrs=None, rgb=False, colors=None, gray=False, soft_masks=None):
         self.be, self.pages = be, pages
+        self.rgb, self.bpp = rgb, (3 if rgb else 1)
         offs = [0]
         e_base = 0
         edge_off = []
@@ -659,10 +671,19 @@ class DeviceBatch:
         self.attrs_host = a.copy()
         self.attrs = be.to_device(self.attrs_host)
         self.host_rows = np.ascontiguousarray(np.concatenate([pg.host_rows for pg in pages]), np.int32)
-        self.strides = [stride_for(pg.width) for pg in pages]
+        self.strides = [self.bpp * stride_for(pg.width) for pg in pages]
         self.canvases = [be.alloc(st * pg.height) for st, pg in zip(self.strides, pages)]
         self.hashes = be.alloc(8 * max(self.n_paths, 1))
         self.counts = be.alloc(16 * max(self.n_paths, 1))
+        self.extra = []
+        self.colors = None
+        if rgb:
+            c4 = np.zeros((max(self.n_paths, 1), 4), np.uint8)
+            if colors is not None:
+                c4[:self.n_paths, :3] = colors
+            self.colors = be.to_device(c4)
+            self.extra.append(self.colors)
+        self.grays = [be.alloc(stride_for(pg.width) * pg.height) for pg in pages] if gray else None
         self.descs = (PageDesc * len(pages))()
         for i, pg in enumerate(pages):
             d = self.descs[i]
@@ -670,9 +691,19 @@ class DeviceBatch:
             d.path_begin, d.path_end = int(self.path_begin[i]), int(self.path_begin[i + 1])
             for k, v in enumerate((0.0, 0.0, pg.width - 0.001, pg.height - 0.001)):
                 d.clip[k] = v
+            if soft_masks is not None and soft_masks[i] is not None:
+                sm = np.zeros((pg.height, stride_for(pg.width)), np.uint8)
+                sm[:, :pg.width] = soft_masks[i]
+                d.soft_mask = be.to_device(sm)
+                d.soft_mask_stride = sm.shape[1]
+                self.extra.append(d.soft_mask)
+            if gray:
+                d.gray, d.gray_stride = self.grays[i], stride_for(pg.width)
 
     def free(self):
-        for b in [self.edges, self.edge_off, self.attrs, self.hashes, self.counts] + self.canvases:
+        for b in [self.edges, self.edge_off, self.attrs, self.hashes, self.counts] + self.canvases + self.extra:
+            self.be.free(b)
+        for b in self.grays or []:
             self.be.free(b)
 
     def fill_pages(self, mode=MODE_DOCUMENTED, flavour=FLAVOUR_GRAY8, launch=LAUNCH_BATCHED,
@@ -687,6 +718,7 @@ class DeviceBatch:
         b.cov_hash = self.hashes if hashes else None
         b.cov_count = self.counts if counts else None
         b.variant = variant
+        b.colors = self.colors
         return b
 
     def reset(self, background=255):
@@ -739,6 +771,21 @@ class DeviceBatch:
         pg = self.pages[i]
         out = np.zeros((pg.height, self.strides[i]), np.uint8)
         self.be.download(self.canvases[i], out)
+        if self.rgb:
+            return out[:, :3 * pg.width].reshape(pg.height, pg.width, 3).copy()
+        return out[:, :pg.width].copy()
+
+    def upload_canvas(self, i, img):
+        """Set page i's canvas to img ((H, W) or, Rgb8, (H, W, 3) uint8)."""
+        pg = self.pages[i]
+        buf = np.zeros((pg.height, self.strides[i]), np.uint8)
+        buf[:, :self.bpp * pg.width] = np.asarray(img, np.uint8).reshape(pg.height, -1)
+        self.be.upload(self.canvases[i], buf)
+
+    def read_gray(self, i):
+        pg = self.pages[i]
+        out = np.zeros((pg.height, stride_for(pg.width)), np.uint8)
+        self.be.download(self.grays[i], out)
         return out[:, :pg.width].copy()
 
     def read_hashes(self):
