@@ -96,6 +96,19 @@ def image_bbox_px(im, page_w, page_h):
     return max(x1 - x0, 0) * max(y1 - y0, 0)
 
 
+def measured_traffic(workload, variant, kernel):
+    """HBM bytes per launch of `kernel` on this workload and variant from the
+    committed rocprofv3 counter runs (profiles/hbm_traffic.json), or None."""
+    p = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        for e in json.load(f)["entries"]:
+            if e["workload"] == workload and e["variant"] == variant and e["kernel"] == kernel:
+                return e["bytes"]
+    return None
+
+
 def load_golden(name):
     p = os.path.join(ROOT, "tests", "golden", "page_hashes.json")
     if not os.path.exists(p):
@@ -104,10 +117,36 @@ def load_golden(name):
         return json.load(f).get(name)
 
 
+def host_cpu():
+    """CPUs this process may run on, the machine's count, and its model name."""
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return allowed, os.cpu_count() or 1, model
+
+
+# Worker threads of the CPU baseline: every CPU the process may use, at most
+# 16 -- the host CPU share of one GPU on the machines this runs on.
+CPU_THREADS_CAP = 16
+
+
+def cpu_threads():
+    allowed, total, model = host_cpu()
+    return min(CPU_THREADS_CAP, allowed), {"host_cpus": total, "allowed_cpus": allowed, "cpu_model": model}
+
+
 def cpu_baseline(pages, w_per_path, budget_s, threads, mode):
     """Time the oracle (the reference CPU raster restated) on the same paths:
-    thread t renders page t mod len(pages) path by path, in order, until the
-    budget is spent; Mpx/s counts the reference composites of the paths done."""
+    thread t renders page t mod len(pages) in order, 32 paths per call into the
+    C oracle, until the budget is spent; Mpx/s counts the reference composites
+    of the paths done."""
     sys.path.insert(0, ROOT)
     from oracle import oracle as O  # the oracle, timed here as the CPU reference
 
@@ -120,17 +159,23 @@ def cpu_baseline(pages, w_per_path, budget_s, threads, mode):
         pg = pages[pi]
         canvas = np.full((pg.height, pg.width), 255, np.uint8)
         a = pg.attrs
+        eo = np.ascontiguousarray(a["eo"])
+        col = np.ascontiguousarray(a["color"])
+        ain = np.ascontiguousarray(a["a_input"])
+        wp = np.asarray(w_per_path[base[pi]:base[pi + 1]], np.int64)
         t_end = time.perf_counter() + budget_s
         p = 0
         while time.perf_counter() < t_end:
             if p == pg.n_paths:
                 canvas[:] = 255
                 p = 0
-            O.fill(canvas, pg.path_edges(p), int(a["eo"][p]), int(a["color"][p]),
-                   int(a["a_input"][p]), mode)
-            done_w[t] += int(w_per_path[base[pi] + p])
-            done_paths[t] += 1
-            p += 1
+            q = min(p + 32, pg.n_paths)
+            O.lib().rro_fill_page(O._p(canvas), pg.width, pg.height, pg.width, O._p(pg.edges),
+                                  O._p(pg.edge_off[p:q + 1]), q - p, O._p(eo[p:q]), O._p(col[p:q]),
+                                  O._p(ain[p:q]), mode, 0, None)
+            done_w[t] += int(wp[p:q].sum())
+            done_paths[t] += q - p
+            p = q
 
     t0 = time.perf_counter()
     ths = [threading.Thread(target=worker, args=(t,)) for t in range(threads)]
@@ -229,12 +274,12 @@ def bench_images(args, desc):
     di.free()
     be.close()
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        threads, host = cpu_threads()
         px, n_img, cdt = cpu_baseline_images(iset, args.cpu_budget, threads)
         out["cpu_baseline"] = {
             "value": round(px / cdt / 1e6, 3), "unit": "Mpx/s", "cores": threads, "kind": "port",
             "sample": f"{n_img} image placements by the C oracle over {threads} threads "
-                      f"(thread t cycles images t, t+{threads}, ...), {cdt:.1f} s"}
+                      f"(thread t cycles images t, t+{threads}, ...), {cdt:.1f} s", **host}
     print(json.dumps(out), flush=True)
 
 
@@ -333,12 +378,12 @@ def bench_ingest(args, desc):
         be.free(b)
     be.close()
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        threads, host = cpu_threads()
         ne, npth, cdt = cpu_baseline_ingest(pages, args.cpu_budget, threads)
         out["cpu_baseline"] = {
             "value": round(ne / cdt / 1e6, 3), "unit": "Medges/s", "cores": threads, "kind": "port",
             "sample": f"{npth} paths ({ne} edges) through the C oracle's XPath::new over {threads} threads, "
-                      f"{cdt:.1f} s"}
+                      f"{cdt:.1f} s", **host}
     print(json.dumps(out), flush=True)
 
 
@@ -448,7 +493,7 @@ def bench_dispatch(args, desc):
     c_tr, c_ev = cost_table(dtr), cost_table(dev)
     cpu_note = "cpu class not measured (--no-cpu-baseline)"
     cpu_wall = None
-    threads = min(16, os.cpu_count() or 1)
+    threads, _ = cpu_threads()
     if not args.no_cpu_baseline:
         c_tr[:, 4], _ = cpu_path_costs(train_pages, threads)
         c_ev[:, 4], cpu_wall = cpu_path_costs(eval_pages, threads)
@@ -610,7 +655,15 @@ def main():
     digest.update(db.read_hashes().astype("<u8").tobytes())
     fingerprint = digest.hexdigest()
     golden = load_golden(f"{args.workload}:{args.mode}")
-    parity = "no-fixture" if golden is None else ("match" if golden == fingerprint else "MISMATCH")
+    parity_cov = "no-fixture" if golden is None else ("match" if golden == fingerprint else "MISMATCH")
+
+    def canvas_parity():
+        """The canvases as the last run left them against the oracle's."""
+        d = hashlib.sha256()
+        for i in range(len(pages)):
+            d.update(db.read_canvas(i).tobytes())
+        g = load_golden(f"{args.workload}:{args.mode}:canvas")
+        return d.hexdigest(), ("no-fixture" if g is None else ("match" if g == d.hexdigest() else "MISMATCH"))
 
     def timed(shape, steps, warmup):
         for _ in range(warmup):
@@ -622,6 +675,7 @@ def main():
 
     sec = timed(launch, args.steps, args.warmup)
     print(f"bench: primary {sec * 1e3:.2f} ms/step", file=sys.stderr, flush=True)
+    timed_fp, parity_timed = canvas_parity()  # the eager timed configuration
     graph_sec = None
     eager_sec = sec
     if launch == R.LAUNCH_BATCHED:
@@ -634,6 +688,9 @@ def main():
             db.run_graph(g)
         graph_sec = (time.perf_counter() - t0) / args.steps
         db.destroy_graph(g)
+        fp_g, par_g = canvas_parity()  # the replayed graph's output
+        if par_g != "match" or parity_timed != "match":
+            parity_timed = par_g if par_g != "match" else parity_timed
         print(f"bench: graph replay {graph_sec * 1e3:.2f} ms/step", file=sys.stderr, flush=True)
         if graph_sec < sec:
             sec = graph_sec
@@ -679,14 +736,30 @@ def main():
         "eager_ms_per_step": round(eager_sec * 1e3, 3),
         "graph_ms_per_step": None if graph_sec is None else round(graph_sec * 1e3, 3),
         "covered_px_per_step": w_cov, "ref_composites_per_step": w_ref,
-        "fill_launches_per_step": launches, "parity": parity, "fingerprint": fingerprint[:16],
+        "fill_launches_per_step": launches,
+        "value_basis": "reference-equivalent pixel composites: the summed draw_aa_line widths "
+                       "(crates/raster/src/fill/mod.rs:344-401) of every path, as the CPU raster "
+                       "performs them; pixels hidden under a later opaque cover are not recomputed "
+                       "on the GPU (see gpu_work)",
+        "parity": "match" if parity_cov == parity_timed == "match" else
+                  ("no-fixture" if "no-fixture" in (parity_cov, parity_timed) else "MISMATCH"),
+        "parity_detail": {"coverage_run": parity_cov, "timed_run_canvases": parity_timed},
+        "fingerprint": fingerprint[:16], "timed_fingerprint": timed_fp[:16],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
+                     "traffic": measured_traffic(args.workload, variant, kernel),
                      "kernel": kernel, "avg_launch_us": round(avg_launch_s * 1e6, 3),
                      "alg_bytes_per_launch": round(per_launch_bytes, 1)},
     }
     if kernels:
         out["kernels"] = kernels
+        st = kernels["counts"]
+        ch = R.variants()[variant][1]
+        out["gpu_work"] = {
+            "path_tile_composites": st["path_tiles"],
+            "composited_px_upper_bound": st["path_tiles"] * 16 * ch,
+            "span_items": st["items"], "span_items_occluded": st["skipped"],
+            "tiles_started_from_cover": st["filled_tiles"]}
     if not args.no_plain:
         other = R.LAUNCH_PLAIN if launch == R.LAUNCH_BATCHED else R.LAUNCH_BATCHED
         sec_o = timed(other, 1, 0)
@@ -704,7 +777,7 @@ def main():
         be.free(b)
     be.close()
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        threads, host = cpu_threads()
         w_per_path = cnt[:, 1]
         w_done, p_done, cdt = cpu_baseline(pages, w_per_path, args.cpu_budget, threads, mode)
         out["cpu_baseline"] = {
@@ -712,7 +785,8 @@ def main():
             "kind": "port",
             "sample": f"{p_done} paths of the first {min(threads, len(pages))} page(s) rendered "
                       f"in order by the C oracle, one page per thread over {threads} threads, "
-                      f"{cdt:.1f} s"}
+                      f"{cdt:.1f} s",
+            "pages_per_s": round(w_done / cdt / (w_ref / len(pages)), 3), **host}
     print(json.dumps(out), flush=True)
 
 

@@ -123,6 +123,7 @@ def load(path):
         "rro_fill_page_ex": (i64, [P, i32, i32, i32, P, P, i64, P, P, P, ctypes.c_int, ctypes.c_int, P, i32,
                                    P]),
         "rro_rgb_to_gray_page": (None, [P, i32, P, i32, i32, i32]),
+        "rro_ref_bbox_area": (i64, [P, i64, i32, i32]),
         "rro_blend_u8": (u8, [u8, u8, u32]),
         "rro_blend_multiply": (u8, [u8, u8]),
         "rro_composite_rgba8": (None, [P, P, i64]),
@@ -333,6 +334,12 @@ def fill_page_ex(canvas, edges, edge_off, flags, colors, a_input, mode=MODE_DOCU
                                _p(colors), _p(a_input), mode, flavour, _p(sm),
                                0 if sm is None else sm.strides[0], _p(hs))
     return int(d), hs
+
+
+def ref_bbox_area(edges, w, h):
+    """The reference dispatcher's bbox area of one path (-1: stays on the CPU)."""
+    edges = np.ascontiguousarray(edges, np.float64).reshape(-1, 4)
+    return int(lib().rro_ref_bbox_area(_p(edges), len(edges), w, h))
 
 
 def rgb_to_gray_page(rgb):

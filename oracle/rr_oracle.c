@@ -1101,3 +1101,27 @@ void rro_rgb_to_gray_page(const uint8_t *rgb, int32_t rgb_stride, uint8_t *gray,
             gray[(size_t)y * gray_stride + x] = (uint8_t)rro_rgb_to_gray(p[0], p[1], p[2]);
         }
 }
+
+/* The reference dispatcher's pixel bbox area (gpu_fill_segs,
+ * crates/pdf_interp/src/renderer/page/gpu_ops.rs:52-84) over an edge list:
+ * f64 min / max of the endpoints (NaN skipped, as Rust's f64::min / max),
+ * clamped to [0, w] x [0, h], floored / ceiled; -1 when not finite or empty.
+ * A fill goes to the GPU when this is at least 256 (crates/gpu/src/lib.rs:
+ * 112,120; gpu_ops.rs:242,289). */
+int64_t rro_ref_bbox_area(const double *edges, int64_t n, int32_t w, int32_t h) {
+    double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+    for (int64_t i = 0; i < n; ++i) {
+        const double *e = edges + 4 * i;
+        x0 = fmin(fmin(x0, e[0]), e[2]);
+        y0 = fmin(fmin(y0, e[1]), e[3]);
+        x1 = fmax(fmax(x1, e[0]), e[2]);
+        y1 = fmax(fmax(y1, e[1]), e[3]);
+    }
+    if (!isfinite(x0) || !isfinite(y0) || !isfinite(x1) || !isfinite(y1)) return -1;
+    x0 = floor(fmax(x0, 0.0));
+    y0 = floor(fmax(y0, 0.0));
+    x1 = ceil(fmin(x1, (double)w));
+    y1 = ceil(fmin(y1, (double)h));
+    if (x1 <= x0 || y1 <= y0) return -1;
+    return (int64_t)(x1 - x0) * (int64_t)(y1 - y0);
+}

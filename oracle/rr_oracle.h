@@ -155,6 +155,10 @@ int64_t rro_fill_page(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, con
                       const uint8_t *color, const uint8_t *a_input, int mode, int flavour,
                       uint64_t *cov_hashes);
 
+/* The reference dispatcher's bbox area (gpu_ops.rs:52-84), -1 = stays on the
+ * CPU; pinned by hand-computed cases (tests/test_oracle_general.py). */
+int64_t rro_ref_bbox_area(const double *edges, int64_t n, int32_t w, int32_t h);
+
 /* Order-independent coverage hash term for one pixel. */
 uint64_t rro_cov_hash_term(int64_t pixel_index, uint8_t shape);
 

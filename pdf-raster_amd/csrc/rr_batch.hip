@@ -242,10 +242,12 @@ __global__ void __launch_bounds__(256) k_dispatch(const rr_path_attr *__restrict
             for (int c = 1; c < kClasses; ++c)
                 if (valid(c) && out[c] < out[cls]) cls = c;
         } else {
-            // static rule: bbox area < 256 px -> CPU (crates/gpu/src/lib.rs:112,120),
-            // else the smallest tile that holds the path's bands, else span
-            const long long w = s.z <= s.w ? s.w - s.z + 1 : 0, h = s.x <= s.y ? s.y - s.x + 1 : 0;
-            if (w * h < 256)
+            // static rule: the reference's bbox area (gpu_ops.rs:52-84) below
+            // 256 px, or no finite bbox -> CPU (GPU_TILE/AA_FILL_THRESHOLD,
+            // crates/gpu/src/lib.rs:112,120; gpu_ops.rs:242,289), else the
+            // smallest tile that holds the path's bands, else span
+            const int32_t area = hdrs[p].ref_area;
+            if (area < 256)
                 cls = RR_ROUTE_CPU;
             else if (bands <= 1)
                 cls = RR_ROUTE_TILE16;

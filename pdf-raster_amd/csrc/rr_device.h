@@ -35,7 +35,9 @@ enum { HDR_EMPTY = 1, HDR_OUTSIDE = 2 };
 // Scanner bounding box of one path in AA space (scanner/mod.rs:138-169).
 struct PathHdr {
     int32_t y_min, y_max, x_min, x_max;
-    int32_t flags, page, pad1, pad2;
+    int32_t flags, page;
+    int32_t ref_area;  // the reference dispatcher's bbox area in pixels, -1 when it keeps the fill on the CPU
+    int32_t pad2;
 };
 
 struct ClipRect {

@@ -83,6 +83,21 @@ __device__ __forceinline__ void to_fast_form(EdgeRec &r) {
     }
 }
 
+// Area of the reference dispatcher's pixel bbox (gpu_ops.rs:52-84): clamped to
+// the bitmap, floored / ceiled to whole pixels; -1 when the bbox is not finite
+// or empty (the reference then keeps the fill on the CPU), saturated at
+// INT32_MAX.
+__device__ __forceinline__ int32_t ref_bbox_area(double x0, double y0, double x1, double y1, int32_t w, int32_t h) {
+    if (!isfinite(x0) || !isfinite(y0) || !isfinite(x1) || !isfinite(y1)) return -1;
+    x0 = floor(fmax(x0, 0.0));
+    y0 = floor(fmax(y0, 0.0));
+    x1 = ceil(fmin(x1, static_cast<double>(w)));
+    y1 = ceil(fmin(y1, static_cast<double>(h)));
+    if (x1 <= x0 || y1 <= y0) return -1;
+    const double a = (x1 - x0) * (y1 - y0);
+    return a >= 2147483647.0 ? INT32_MAX : static_cast<int32_t>(a);
+}
+
 // One wave per path (four per block): add_segment + aa_scale of every edge
 // into its EdgeRec, the scanner bbox, the chunk summaries (min first row, max
 // last row of every global 64-edge chunk: a plain store for a chunk inside the
@@ -114,12 +129,20 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
     int32_t xmn = INT32_MAX, xmx = INT32_MIN, ymn = INT32_MAX, ymx = INT32_MIN;
     int any = 0;
     double shoelace = 0.0, dy_sum = 0.0;  // dispatcher features
+    // the reference dispatcher's pixel bbox over the segment endpoints
+    // (gpu_fill_segs, crates/pdf_interp/src/renderer/page/gpu_ops.rs:52-84):
+    // f64 min / max, which skip NaN like Rust's f64::min / max
+    double rx0 = INFINITY, ry0 = INFINITY, rx1 = -INFINITY, ry1 = -INFINITY;
     for (int64_t base = e0 & ~63ll; base < e1; base += 64) {
         const int64_t i = base + lane;
         int32_t cr0 = INT32_MAX, cr1 = INT32_MIN;
         if (i >= e0 && i < e1) {
             const double4 q = reinterpret_cast<const double4 *>(edges)[i];
             double x0 = q.x, y0 = q.y, x1 = q.z, y1 = q.w;
+            rx0 = fmin(fmin(rx0, x0), x1);
+            ry0 = fmin(fmin(ry0, y0), y1);
+            rx1 = fmax(fmax(rx1, x0), x1);
+            ry1 = fmax(fmax(ry1, y0), y1);
             if (feat && isfinite(x0) && isfinite(y0) && isfinite(x1) && isfinite(y1)) {
                 shoelace += x0 * y1 - x1 * y0;
                 dy_sum += fabs(y1 - y0);
@@ -213,6 +236,12 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
     ymn = wave_min(ymn);
     ymx = wave_max(ymx);
     any = __ballot(any != 0) != 0;
+    for (int o = 32; o > 0; o >>= 1) {
+        rx0 = fmin(rx0, __shfl_xor(rx0, o));
+        ry0 = fmin(ry0, __shfl_xor(ry0, o));
+        rx1 = fmax(rx1, __shfl_xor(rx1, o));
+        ry1 = fmax(ry1, __shfl_xor(ry1, o));
+    }
     if (feat) {
         shoelace = wave_sum_f64(shoelace);
         dy_sum = wave_sum_f64(dy_sum);
@@ -220,6 +249,7 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
     if (lane == 0) {
         if ((attrs[p].eo & RR_FILL_SOFT_MASK) && !pg.smask) atomicOr(err, kErrNoSoftMask);
         PathHdr h{0, -1, 1, 0, HDR_EMPTY, lo, 0, 0};
+        h.ref_area = ref_bbox_area(rx0, ry0, rx1, ry1, pg.width, pg.height);
         int2 drawn = make_int2(1, 0);
         if (any) {
             const int32_t y_min = max(ymn, cy0), y_max = min(ymx, cy1);

@@ -59,10 +59,55 @@ def test_every_route_is_bit_exact(backend, pages, route):
             assert np.array_equal(db.read_canvas(i), o), f"page {i}"
         assert np.array_equal(db.read_hashes(), oh)
         _, cls = backend.last_dispatch(db.n_paths)
-        if route == R.ROUTE_AUTO:  # static rule: area < 256 -> CPU, else the smallest tile that fits
+        if route == R.ROUTE_AUTO:  # static rule: the reference's bbox area < 256 -> CPU
+            want_cpu = np.concatenate([[O.ref_bbox_area(pg.path_edges(i), pg.width, pg.height) < 256
+                                        for i in range(pg.n_paths)] for pg in pages])
+            assert np.array_equal(cls == R.ROUTE_CPU, want_cpu)
             assert set(np.unique(cls)) <= {0, 1, 2, 3, 4}
         elif route in (R.ROUTE_SPAN, R.ROUTE_CPU):
             assert np.all(cls == route)
+    finally:
+        db.free()
+
+
+def _rect(x0, y0, x1, y1):
+    return [[x0, y0, x1, y0], [x1, y0, x1, y1], [x1, y1, x0, y1], [x0, y1, x0, y0]]
+
+
+def test_static_rule_around_the_threshold(backend):
+    # Rectangles whose reference bbox area (gpu_ops.rs:52-84) falls just
+    # below, at and above 256 px: fractional corners (floor / ceil), parts
+    # off the page (clamped), a NaN endpoint (skipped) and an infinite one
+    # (no bbox: CPU).  Without an MLP, AUTO takes the CPU class exactly when
+    # the area is below 256 (crates/gpu/src/lib.rs:112,120), and every class
+    # still gives the oracle's pixels.
+    from test_gpu_batched import edge_page
+    rng = np.random.default_rng(31)
+    W, H = 300, 200
+    paths = []
+    for k in range(160):
+        x0, y0 = rng.uniform(-6, W - 20), rng.uniform(-6, H - 20)
+        w = rng.choice([14.0, 15.0, 15.5, 16.0, 16.2, 17.0]) + rng.choice([0.0, 0.3, 0.75])
+        h = rng.choice([14.0, 15.0, 15.9, 16.0, 17.0])
+        e = _rect(x0, y0, x0 + w, y0 + h)
+        if k % 40 == 7:
+            e[1][2] = float("nan")
+        if k % 40 == 9:
+            e[0][2] = float("inf")
+        paths.append((e, k % 2, int(rng.integers(0, 256)), 255))
+    pg = edge_page(W, H, paths)
+    areas = np.array([O.ref_bbox_area(pg.path_edges(i), W, H) for i in range(pg.n_paths)])
+    assert (areas < 256).any() and (areas >= 256).any() and (areas == 256).any()
+    db = R.DeviceBatch(backend, [pg])
+    try:
+        backend.set_dispatcher(None)
+        db.set_routes(R.ROUTE_AUTO)
+        db.run(launch=R.LAUNCH_BATCHED, hashes=True, variant=0)
+        _, cls = backend.last_dispatch(db.n_paths)
+        assert np.array_equal(cls == R.ROUTE_CPU, areas < 256), np.nonzero((cls == R.ROUTE_CPU) != (areas < 256))
+        oc, oh = _oracle([pg])
+        assert np.array_equal(db.read_canvas(0), oc[0])
+        assert np.array_equal(db.read_hashes(), oh)
     finally:
         db.free()
 

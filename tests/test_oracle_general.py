@@ -169,3 +169,19 @@ def test_planted_mistake_is_rejected(mutant_builds, samples, m, check):
     with O.using(O.mutant_path(m)):
         with pytest.raises(AssertionError):
             check(samples)
+
+
+# ---- the static dispatch rule's bbox (gpu_ops.rs:52-84) ----
+
+@pytest.mark.parametrize("edges,want", [
+    ([[0, 0, 16, 0], [16, 0, 16, 16], [16, 16, 0, 0]], 256),          # 16 x 16 exactly: the threshold
+    ([[0, 0, 15, 0], [15, 0, 15, 17], [15, 17, 0, 0]], 255),          # 15 x 17: below it
+    ([[0.5, 0.5, 15.2, 0.5], [15.2, 0.5, 0.5, 15.9]], 256),           # floor / ceil to whole pixels
+    ([[-40, -3, 10, -3], [10, -3, 10, 12], [10, 12, -40, -3]], 120),  # clamped at 0: 10 x 12
+    ([[95, 70, 130, 70], [130, 70, 95, 99]], 50),                     # clamped at the 100 x 80 page: 5 x 10
+    ([[3, 3, float("nan"), 9], [3, 3, 20, 19]], 17 * 16),            # NaN skipped by f64::min / max
+    ([[3, 3, float("inf"), 9], [3, 3, 20, 19]], -1),                 # not finite: CPU
+    ([[200, 3, 230, 9]], -1),                                         # right of the page: empty
+])
+def test_ref_bbox_area_cases(edges, want):
+    assert O.ref_bbox_area(np.array(edges, np.float64), 100, 80) == want
