@@ -437,6 +437,17 @@ int32_t rr_hip_train_dispatcher(rr_hip_ctx *ctx, const float *features, const fl
  * composite tiles started from an opaque cover's colour (the canvas tile is
  * not read), intervals the composite pass read}. */
 int32_t rr_hip_last_batch_stats(rr_hip_ctx *ctx, double ms[2], int64_t counts[12]);
+/* Diagnostics of the last batched recording (no reference counterpart):
+ * copies up to cap elements of one internal table to host memory and sets
+ * *count to its length.  table 0: per composite tile item, the highest opaque
+ * path covering it fully (int32, -1 none); 1: tile-bin offsets (int64, 4 per
+ * item + 1); 2: bin entries (int32 x 4 {span item, path, full-cover columns,
+ * attribute bytes}); 3: span heads (16 bytes {int64 offset, int16 fx0, fx1,
+ * cx0, cx1}); 4: first span item of each path (int64, n_paths + 1); 5: the
+ * 64-byte path records of the span pass; 6: per span item, the interval
+ * count of each of its 64 AA rows (uint16 x 64); 7: the interval pool (int32
+ * x 2 [s, e) in AA columns, a head's offset indexes it). */
+int32_t rr_hip_debug_batch_table(rr_hip_ctx *ctx, int32_t table, void *dst, int64_t cap, int64_t *count);
 
 /* In-place exclusive prefix sum of len_elems uint32 (wrapping), replacing the
  * three record_scan phases of the reference (GpuBackend::record_scan,

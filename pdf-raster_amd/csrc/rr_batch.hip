@@ -655,11 +655,17 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
         // path covering it fully: the composite pass starts after that path,
         // so these intervals would never be read.  Should the path be widened
         // later by an unbalanced row, the fixup pass redoes the item.
+        // vmask: the first 64 tiles; beyond them only whether any tile is
+        // visible (the loop is wave-uniform: every lane takes the same branch
+        // below, so the wave stays whole for the build and the pool
+        // allocation of span_finish)
         const int32_t t0 = P.px0 >> B.chunk_shift, t1 = P.px1 >> B.chunk_shift;
-        bool seen = t0 + lane <= t1 && occ_row[t0 + lane] <= P.p;
+        const bool seen = t0 + lane <= t1 && occ_row[t0 + lane] <= P.p;
         vmask = __ballot(seen);
-        for (int32_t t = t0 + 64 + lane; vmask == 0 && t <= t1; t += 64) vmask = __ballot(occ_row[t] <= P.p);
-        if (vmask == 0) {
+        bool visible = vmask != 0;
+        for (int32_t tb = t0 + 64; !visible && tb <= t1; tb += 64)
+            visible = __ballot(tb + lane <= t1 && occ_row[tb + lane] <= P.p) != 0;
+        if (!visible) {
             ++cur.skipped;
             write_skipped(B.shead + it);
             return;
@@ -1955,6 +1961,31 @@ __global__ void __launch_bounds__(256) k_built_paths(const SpanPath *__restrict_
     }
 }
 }  // namespace
+
+extern "C" int32_t rr_hip_debug_batch_table(rr_hip_ctx *c, int32_t table, void *dst, int64_t cap, int64_t *count) {
+    if (!c || !count || cap < 0 || (cap > 0 && !dst)) return RR_E_INVARIANT;
+    if (!c->batch_valid && !c->batch_paths) return rrint::fail(c, RR_E_INVARIANT, "no batched recording");
+    RR_TRY(c, hipSetDevice(c->device));
+    RR_TRY(c, hipStreamSynchronize(c->stream));
+    const void *src = nullptr;
+    size_t elem = 0;
+    long long n = 0;
+    switch (table) {
+        case 0: src = c->occ; elem = sizeof(int32_t); n = c->batch_citems; break;
+        case 1: src = c->bin_off; elem = sizeof(long long); n = c->batch_citems * kBinSegs + 1; break;
+        case 2: src = c->bins; elem = sizeof(int4); n = c->batch_bins; break;
+        case 3: src = c->shdr; elem = sizeof(SpanHead); n = c->batch_items; break;
+        case 4: src = c->item_base; elem = sizeof(long long); n = c->batch_paths + 1; break;
+        case 5: src = c->spath; elem = sizeof(SpanPath); n = c->batch_paths; break;
+        case 6: src = reinterpret_cast<const SpanHead *>(c->shdr) + c->shdr_cap; elem = sizeof(SpanCnt); n = c->batch_items; break;
+        case 7: src = c->spans; elem = sizeof(int2); n = static_cast<long long>(c->spans_cap); break;
+        default: return rrint::fail(c, RR_E_INVARIANT, "debug_batch_table: unknown table");
+    }
+    *count = n;
+    const long long m = std::min<long long>(n, cap);
+    if (m > 0 && src) RR_TRY(c, hipMemcpy(dst, src, static_cast<size_t>(m) * elem, hipMemcpyDeviceToHost));
+    return RR_OK;
+}
 
 extern "C" int32_t rr_hip_last_batch_stats(rr_hip_ctx *c, double ms[2], int64_t counts[12]) {
     if (!c || !ms || !counts) return RR_E_INVARIANT;

@@ -149,6 +149,7 @@ def lib():
             "rr_hip_record_rgb_to_gray": (i32, [P, P, i64, P, i64, i32, i32]),
             "rr_hip_record_scan": (i32, [P, P, P, ctypes.c_uint32]),
             "rr_hip_last_batch_stats": (i32, [P, P, P]),
+            "rr_hip_debug_batch_table": (i32, [P, i32, P, i64, ctypes.POINTER(i64)]),
             "rr_hip_set_dispatcher": (i32, [P, P, i32]),
             "rr_hip_last_dispatch": (i32, [P, P, P, i64]),
             "rr_hip_mlp_forward": (i32, [P, P, P, i64]),
@@ -472,6 +473,18 @@ class Backend:
         keys = ("items", "intervals", "composite_items", "path_tiles", "tiles", "reruns", "skipped", "deferred",
                 "built_paths", "built_edges", "filled_tiles", "intervals_read")
         return {"span_ms": float(ms[0]), "compose_ms": float(ms[1])}, dict(zip(keys, (int(v) for v in cnt)))
+
+    def debug_batch_table(self, table):
+        """One internal table of the last batched recording (see rr_hip.h)."""
+        dtypes = {0: np.int32, 1: np.int64, 2: np.dtype((np.int32, 4)), 3: np.dtype(
+            [("off", "<i8"), ("fx0", "<i2"), ("fx1", "<i2"), ("cx0", "<i2"), ("cx1", "<i2")]),
+            4: np.int64, 5: np.dtype((np.int32, 16)), 6: np.dtype((np.uint16, 64)), 7: np.dtype((np.int32, 2))}
+        n = ctypes.c_int64(0)
+        self._check(self.L.rr_hip_debug_batch_table(self.h, table, None, 0, ctypes.byref(n)), "debug_batch_table")
+        out = np.zeros(max(n.value, 1), dtypes[table])
+        self._check(self.L.rr_hip_debug_batch_table(self.h, table, _p(out), n.value, ctypes.byref(n)),
+                    "debug_batch_table")
+        return out[:n.value]
 
     def set_dispatcher(self, params):
         """Load the dispatcher MLP (MLP_PARAMS float32) or clear it (None)."""
