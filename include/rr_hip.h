@@ -456,6 +456,16 @@ int32_t rr_hip_debug_batch_table(rr_hip_ctx *ctx, int32_t table, void *dst, int6
  * block_sums: device scratch of ceil(len_elems / 1024) uint32.  len_elems == 0
  * is an invariant violation, as in ScanParams::validate. */
 int32_t rr_hip_record_scan(rr_hip_ctx *ctx, uint32_t *data, uint32_t *block_sums, uint32_t len_elems);
+/* rr_hip_record_scan without a block_sums argument, allocating nothing (no
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid it).  The block sums live in a 64 KiB device array inside the
+ * library, one per GPU, held until the process ends, so len_elems is at most
+ * 16777216 (RR_E_INVARIANT above that).  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (several contexts or streams)
+ * race; calls recorded on one context are ordered by its stream.  No part of
+ * the array needs clearing between calls. */
+int32_t rr_hip_record_scan_static(rr_hip_ctx *ctx, uint32_t *data, uint32_t len_elems);
 
 /* Seeded image workload: n_images images, images_per_page per 2550x3300 page,
  * image i is CMYK + Normal when i is even, RGB + Multiply when odd.  info =

@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2028,6 +2029,48 @@ extern "C" int32_t rr_hip_last_batch_stats(rr_hip_ctx *c, double ms[2], int64_t 
     counts[10] = static_cast<int64_t>(k[kCntFilledTiles]);
     counts[11] = static_cast<int64_t>(k[kCntIvalsRead]);
     return RR_OK;
+}
+
+// Block sums of rr_hip_record_scan_static: a file-scope device array, so that
+// the entry point allocates nothing.  Host code never uses the array's name as
+// an address: each context keeps the per-device address hipGetSymbolAddress
+// returned for it (rrint::init_static_scratch, at rr_hip_create).
+constexpr uint32_t kStaticScanMax = 1u << 24;                   // elements per call at most
+constexpr uint32_t kStaticScanSums = kStaticScanMax / kScanTile;  // 16384 uint32 (64 KiB)
+static __device__ __align__(256) uint32_t scratch[kStaticScanSums];
+
+namespace {
+std::mutex g_scratch_mu;
+uint32_t *g_scratch_ptr[256] = {};  // per device ordinal
+}  // namespace
+
+namespace rrint {
+int32_t init_static_scratch(rr_hip_ctx *c) {
+    if (c->device < 0 || c->device >= 256) return fail(c, RR_E_INVARIANT, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[c->device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));  // the device is current
+        if (e != hipSuccess || !ptr)
+            return fail(c, RR_E_DEVICE, std::string("static scratch address: ") + hipGetErrorString(e));
+        g_scratch_ptr[c->device] = static_cast<uint32_t *>(ptr);
+    }
+    c->static_scratch = g_scratch_ptr[c->device];
+    return RR_OK;
+}
+}  // namespace rrint
+
+extern "C" int32_t rr_hip_record_scan_static(rr_hip_ctx *c, uint32_t *data, uint32_t len_elems) {
+    if (!c) return RR_E_INVARIANT;
+    if (!rrint::page_open(c)) return rrint::fail(c, RR_E_INVARIANT, "record outside begin_page/submit_page");
+    if (len_elems == 0) return rrint::fail(c, RR_E_INVARIANT, "scan: len_elems must be > 0");
+    if (len_elems > kStaticScanMax)
+        return rrint::fail(c, RR_E_INVARIANT, "scan_static: len_elems exceeds 16777216 (the static scratch holds "
+                                              "16384 block sums); use rr_hip_record_scan");
+    if (!data) return rrint::fail(c, RR_E_INVARIANT, "scan: null buffer");
+    if (!c->static_scratch) return rrint::fail(c, RR_E_DEVICE, "static scratch not initialised");
+    RR_TRY(c, hipSetDevice(c->device));
+    return scan_exclusive<uint32_t, uint32_t>(c, data, data, len_elems, nullptr, c->static_scratch);
 }
 
 extern "C" int32_t rr_hip_record_scan(rr_hip_ctx *c, uint32_t *data, uint32_t *block_sums, uint32_t len_elems) {

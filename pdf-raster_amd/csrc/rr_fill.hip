@@ -865,6 +865,7 @@ int32_t rr_hip_create(int32_t device, rr_hip_ctx **out) {
     int32_t rc = RR_OK;
     if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) rc = RR_E_DEVICE;
     if (rc == RR_OK) rc = rrint::init_batched(c);
+    if (rc == RR_OK) rc = rrint::init_static_scratch(c);
     if (rc == RR_OK && (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess)) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->spill_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->counters, kCounters * sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;

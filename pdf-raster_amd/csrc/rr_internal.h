@@ -234,6 +234,8 @@ struct rr_hip_ctx {
     // sizes of the buffers rr_hip_alloc handed out (device_buffer_len)
     std::unordered_map<const void *, size_t> buf_len;
     uint64_t next_fence = 1;
+    // this device's address of the static scan scratch (rr_hip_record_scan_static)
+    uint32_t *static_scratch = nullptr;
     // per-launch profiling
     bool profiling = false;
     std::vector<hipEvent_t> prof_ev;  // pairs
@@ -297,6 +299,9 @@ int32_t fill_i32(rr_hip_ctx *c, int32_t *p, long long n, int32_t value);
 // others (a kernel, so that it is ordered on the stream and capturable).
 int32_t clear_err_bits(rr_hip_ctx *c, int32_t mask);
 int32_t set_err_bits(rr_hip_ctx *c, int32_t bits);
+// Looks up (once per device, that device current) the address of the static
+// scan scratch and keeps it in the context (rr_batch.hip).
+int32_t init_static_scratch(rr_hip_ctx *c);
 // Records rgb_to_gray of an Rgb8 bitmap into a Gray8 one (rr_image.hip).
 int32_t rgb_to_gray(rr_hip_ctx *c, const uint8_t *rgb, int64_t rgb_stride, uint8_t *gray, int64_t gray_stride,
                     int32_t width, int32_t height);

@@ -18,6 +18,7 @@ MODE_DOCUMENTED, MODE_LITERAL = 0, 1
 FLAVOUR_GRAY8, FLAVOUR_RGB8_FAST, FLAVOUR_RGB8 = 0, 1, 2
 FILL_EVEN_ODD, FILL_MULTIPLY, FILL_SOFT_MASK = 1, 2, 4
 LAUNCH_PLAIN, LAUNCH_BATCHED = 0, 1
+SCAN_STATIC_MAX = 1 << 24  # elements per rr_hip_record_scan_static call
 RR_OK, RR_E_ZERO_SIZE_ALLOC, RR_E_INVARIANT, RR_E_DEVICE, RR_E_NO_DEVICE, RR_E_CAPACITY = 0, -1, -2, -3, -4, -5
 ROUTE_SPAN, ROUTE_TILE16, ROUTE_TILE32, ROUTE_TILE64, ROUTE_CPU, ROUTE_AUTO = 0, 1, 2, 3, 4, 255
 MLP_FEATURES, MLP_CLASSES, MLP_PARAMS = 12, 5, 5341
@@ -148,6 +149,7 @@ def lib():
             "rr_hip_record_apply_soft_mask": (i32, [P, P, P, i64]),
             "rr_hip_record_rgb_to_gray": (i32, [P, P, i64, P, i64, i32, i32]),
             "rr_hip_record_scan": (i32, [P, P, P, ctypes.c_uint32]),
+            "rr_hip_record_scan_static": (i32, [P, P, ctypes.c_uint32]),
             "rr_hip_last_batch_stats": (i32, [P, P, P]),
             "rr_hip_debug_batch_table": (i32, [P, i32, P, i64, ctypes.POINTER(i64)]),
             "rr_hip_set_dispatcher": (i32, [P, P, i32]),
@@ -548,6 +550,11 @@ class Backend:
 
     def record_scan(self, data, block_sums, n):
         self._check(self.L.rr_hip_record_scan(self.h, data, block_sums, n), "record_scan")
+
+    def record_scan_static(self, data, n):
+        """record_scan with the library's own static block-sum array: no
+        allocation; at most SCAN_STATIC_MAX elements."""
+        self._check(self.L.rr_hip_record_scan_static(self.h, data, n), "record_scan_static")
 
     def inject_error(self, bits=1):
         self._check(self.L.rr_hip_inject_error(self.h, bits), "inject_error")
