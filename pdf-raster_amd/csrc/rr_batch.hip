@@ -174,6 +174,19 @@ struct BatchArgs {
     unsigned long long *queues;  // sliced queue counters: span pass, then composite pass
 };
 
+// A field of the kernel's BatchArgs argument, read from the kernel-argument
+// segment where it is used (one scalar load) instead of being held in SGPRs
+// for the whole kernel: for what the span pass needs rarely (error word,
+// counters, caps, the deferred queue).  Only for kernels whose first argument
+// is the BatchArgs (k_spans, k_spans_wide).
+#define RR_KARG_AS __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ T karg_at(size_t off) {
+    const RR_KARG_AS char *base = (const RR_KARG_AS char *)(__builtin_amdgcn_kernarg_segment_ptr());
+    return *reinterpret_cast<const volatile RR_KARG_AS T *>(base + off);
+}
+#define RR_COLD(field) karg_at<decltype(BatchArgs{}.field)>(offsetof(BatchArgs, field))
+
 // Span pass modes: every item, counting drawn widths and skipping occluded
 // items (skipping only when no per-path coverage output is asked for); the
 // same without counting (a rerun after growing the interval pool); every item
@@ -456,8 +469,9 @@ struct SpanLds {
 // kSpanChunk records instead of one per item.
 constexpr long long kSpanChunk = 8192;
 struct SpanCursor {
-    unsigned long long next = 0, end = 0, stored = 0;
-    unsigned long long skipped = 0;  // bands skipped as occluded
+    unsigned long long next = 0, end = 0;
+    uint32_t stored = 0;   // intervals stored by this wave
+    uint32_t skipped = 0;  // bands skipped as occluded
     int32_t marked = -1;             // last path marked as having a visible band
 };
 
@@ -520,18 +534,17 @@ template <typename Ptr>
 __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &P, int32_t Y0, long long it,
                                             const int32_t *occ_row, Ptr X0, Ptr X1, int nint, bool unb,
                                             bool count_draw, bool occlusion, uint64_t vmask, SpanCursor &cur) {
-    const FillArgs &A = B.F;
     const int lane = threadIdx.x;
     const int32_t y = Y0 + (lane >> 2), j = lane & 3;
     // an unbalanced row spans to the canvas edge: widen the path's columns;
     // its items skipped as occluded are redone by the fixup pass
     const bool any_unb = __ballot(unb) != 0;
     if (any_unb && lane == 0) {
-        atomicMax(reinterpret_cast<int32_t *>(B.sel + P.p) + 3, P.px_hi);
-        if ((atomicOr(B.pstate + P.p, 1) & 1) == 0) atomicAdd(B.counters + kCntWidened, 1ull);
+        atomicMax(reinterpret_cast<int32_t *>(RR_COLD(sel) + P.p) + 3, P.px_hi);
+        if ((atomicOr(RR_COLD(pstate) + P.p, 1) & 1) == 0) atomicAdd(RR_COLD(counters) + kCntWidened, 1ull);
     }
     const int32_t px1 = any_unb ? P.px_hi : P.px1;  // last drawn column of this item
-    if (A.cov_count && count_draw) {
+    if (count_draw && RR_COLD(F.cov_count)) {
         // draw_aa_line range of each output row (scanner/mod.rs:459-466)
         int32_t smin = nint ? X0[lane] : INT32_MAX;
         int32_t emax = nint ? X1[(nint - 1) * 64 + lane] : -1;
@@ -546,7 +559,7 @@ __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &
         }
         const int32_t tot = __builtin_amdgcn_readlane(wave_incl_add(w_draw), 63);
         if (lane == 0 && tot)
-            atomicAdd(reinterpret_cast<unsigned long long *>(A.cov_count + 2 * P.p + 1),
+            atomicAdd(reinterpret_cast<unsigned long long *>(RR_COLD(F.cov_count) + 2 * P.p + 1),
                       static_cast<unsigned long long>(tot));
     }
     // columns fully covered on all 64 AA rows: per row the longest interval's
@@ -607,20 +620,21 @@ __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &
         const unsigned long long want =
             static_cast<unsigned long long>(total > kSpanChunk ? total : kSpanChunk);
         unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(B.counters + kCntPoolTop, want);
+        if (lane == 0) base = atomicAdd(RR_COLD(counters) + kCntPoolTop, want);
         base = first_lane_u64(base);
         cur.next = base;
         cur.end = base + want;
     }
     const unsigned long long off = cur.next;
     cur.next += static_cast<unsigned long long>(total);
-    const bool fits = static_cast<long long>(off) + total <= B.spans_cap;
-    cur.stored += fits ? total : 0;
+    const bool fits = static_cast<long long>(off) + total <= RR_COLD(spans_cap);
+    cur.stored += fits ? static_cast<uint32_t>(total) : 0u;
     if (fits) {
         int2 *dst = B.spans + off + excl;
         for (int k = 0; k < keep; ++k) dst[k] = make_int2(X0[k * 64 + lane], X1[k * 64 + lane]);
     }
-    span_cnt(B, it)->cnt[lane] = fits ? static_cast<uint16_t>(keep) : static_cast<uint16_t>(0);
+    reinterpret_cast<SpanCnt *>(B.shead + RR_COLD(shdr_cap))[it].cnt[lane] =
+        fits ? static_cast<uint16_t>(keep) : static_cast<uint16_t>(0);
     hdr_head(B.shead + it, fits ? static_cast<long long>(off) : -1ll, fx0, fx1, cx0, cx1);
     // publish: composite tiles this opaque item covers fully (the same test
     // as the composite pass's occlusion scan)
@@ -674,10 +688,10 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
     }
     // the path has a band that is built (pstate bit 1; for the statistics)
     if (P.p != cur.marked) {
-        if (lane == 0) atomicOr(B.pstate + P.p, 2);
+        if (lane == 0) atomicOr(RR_COLD(pstate) + P.p, 2);
         cur.marked = static_cast<int32_t>(P.p);
     }
-    const bool literal = A.mode == RR_MODE_LITERAL;
+    const bool literal = RR_COLD(F.mode) == RR_MODE_LITERAL;
     const int32_t y = Y0 + (lane >> 2), j = lane & 3;
     const int32_t r = literal ? 16 * y + 12 + j : 4 * y + j;
     const int32_t band_lo = literal ? 16 * Y0 + 12 : 4 * Y0;
@@ -695,11 +709,11 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
     }
     if constexpr (DEFER) {
         if (lane == 0) {
-            const unsigned long long q = atomicAdd(B.counters + kCntDeferred, 1ull);
-            if (static_cast<long long>(q) < B.ovq_cap)
-                B.ovq[q] = make_int4(static_cast<int32_t>(P.p), Y0, static_cast<int32_t>(it), 0);
+            const unsigned long long q = atomicAdd(RR_COLD(counters) + kCntDeferred, 1ull);
+            if (static_cast<long long>(q) < RR_COLD(ovq_cap))
+                RR_COLD(ovq)[q] = make_int4(static_cast<int32_t>(P.p), Y0, static_cast<int32_t>(it), 0);
             else
-                atomicOr(A.err, 32);
+                atomicOr(RR_COLD(F.err), 32);
         }
         return;
     }
@@ -714,7 +728,7 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
         if (lane == 0) off = atomicAdd(A.spill_top, static_cast<unsigned long long>(need) * 192ull);
         off = first_lane_u64(off);
         if (static_cast<long long>(off) + static_cast<long long>(need) * 192 > A.spill_cap) {
-            if (lane == 0) atomicOr(A.err, 1);
+            if (lane == 0) atomicOr(RR_COLD(F.err), 1);
             hdr_head(B.shead + it, -1, 1, 0, 1, 0);
             span_cnt(B, it)->cnt[lane] = 0;
             return;
@@ -820,8 +834,9 @@ __device__ __forceinline__ ItemPath item_path_lanes(const int4 &q, int32_t p, in
 // items reaching at most 4 composite tiles are tested for
 // occlusion together (4 lanes per item), so a run of hidden items costs one
 // round trip, and a path's chunk summaries are loaded once for its run of
-// consecutive items.
-template <int K>
+// consecutive items.  TRACE: record one timing record per item (a separate
+// instance, so that the untraced pass carries none of its state).
+template <int K, bool TRACE>
 __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
     __shared__ SpanLds<K> L;
     const FillArgs &A = B.F;
@@ -838,7 +853,7 @@ __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
     const long long grab = max(1ll, min(static_cast<long long>(kSpanGrab), n_items / (8ll * gridDim.x)));
     for (;;) {
         long long g_end = 0;
-        const long long g0 = slice_grab(B.queues, n_items, grab, sc, &g_end);
+        const long long g0 = slice_grab(RR_COLD(queues), n_items, grab, sc, &g_end);
         if (g0 >= n_items) break;
       for (long long g = g0; g < g_end; g += kSpanGroup) {
         const int cnt = static_cast<int>(min(static_cast<long long>(kSpanGroup), g_end - g));
@@ -847,8 +862,8 @@ __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
         const int qi = lane >> 2, sub = lane & 3;
         int4 gi = make_int4(0, 0, 0, -1), gq = make_int4(0, 0, 0, 0);
         if (qi < cnt) {
-            gi = B.sitems[n_items - 1 - g - qi];
-            gq = reinterpret_cast<const int4 *>(B.spath + gi.x)[sub];
+            gi = RR_COLD(sitems)[n_items - 1 - g - qi];
+            gq = reinterpret_cast<const int4 *>(RR_COLD(spath) + gi.x)[sub];
         }
         // {px0, px1, width, height} sit in quarter 2, {px_hi, occ_base, ctiles, flags} in quarter 3
         const int32_t px0 = quad_bcast<2>(gq.x), px1 = quad_bcast<2>(gq.y);
@@ -866,8 +881,8 @@ __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
         for (int k = 0; k < cnt; ++k) {
             const int32_t ip = __builtin_amdgcn_readlane(gi.x, 4 * k), iy = __builtin_amdgcn_readlane(gi.y, 4 * k);
             const int32_t iz = __builtin_amdgcn_readlane(gi.z, 4 * k), iw = __builtin_amdgcn_readlane(gi.w, 4 * k);
-            if (iw < 0 || static_cast<long long>(iw) + iz > B.shdr_cap) {
-                if (lane == 0) atomicOr(A.err, 2);
+            if (iw < 0 || static_cast<long long>(iw) + iz > RR_COLD(shdr_cap)) {
+                if (lane == 0) atomicOr(RR_COLD(F.err), 2);
                 continue;
             }
             const bool was_tested = (tmask >> (4 * k)) & 1;
@@ -882,12 +897,12 @@ __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
                 sm_path = ip;
             }
             long long t0 = 0;
-            if (A.trace) t0 = static_cast<long long>(wall_clock64());
+            if constexpr (TRACE) t0 = static_cast<long long>(wall_clock64());
             for (int32_t bnd = 0; bnd < iz; ++bnd)
                 span_band<K, true>(B, L, P, iy + kBandRows * bnd, static_cast<long long>(iw) + bnd, 0, count_draw,
                                    occlusion, occlusion && !was_tested,
                                    was_tested ? (smask >> (4 * k)) & 0xfull : ~0ull, sm0, cur);
-            if (A.trace && lane == 0) {
+            if (TRACE && lane == 0) {
                 const long long t1 = static_cast<long long>(wall_clock64());
                 const unsigned long long kk = atomicAdd(A.trace_top, 1ull);
                 if (static_cast<long long>(kk) < A.trace_cap)
@@ -897,8 +912,8 @@ __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
         }
       }
     }
-    if (lane == 0 && cur.stored) atomicAdd(B.counters + kCntStored, cur.stored);
-    if (lane == 0 && cur.skipped) atomicAdd(B.counters + kCntSkipped, cur.skipped);
+    if (lane == 0 && cur.stored) atomicAdd(RR_COLD(counters) + kCntStored, static_cast<unsigned long long>(cur.stored));
+    if (lane == 0 && cur.skipped) atomicAdd(RR_COLD(counters) + kCntSkipped, static_cast<unsigned long long>(cur.skipped));
 }
 
 // Wide span pass: the bands the main pass deferred (mode = the main pass's
@@ -939,8 +954,8 @@ __global__ void __launch_bounds__(64) k_spans_wide(BatchArgs B, int32_t mode) {
                                 chunk_summary_first(B.F, P.e0, P.e1), cur);
         }
     }
-    if (lane == 0 && cur.stored) atomicAdd(B.counters + kCntStored, cur.stored);
-    if (lane == 0 && cur.skipped) atomicAdd(B.counters + kCntSkipped, cur.skipped);
+    if (lane == 0 && cur.stored) atomicAdd(RR_COLD(counters) + kCntStored, static_cast<unsigned long long>(cur.stored));
+    if (lane == 0 && cur.skipped) atomicAdd(RR_COLD(counters) + kCntSkipped, static_cast<unsigned long long>(cur.skipped));
 }
 
 // ---------------------------------------------------------------------------
@@ -1464,12 +1479,20 @@ __global__ void __launch_bounds__(64) k_compose(BatchArgs B, long long n_items) 
 
 constexpr long long kSlotInts = static_cast<long long>(kSlotK) * 192;
 
-const void *span_kernel(int k) {
+const void *span_kernel(int k, bool trace) {
+    if (trace) {
+        switch (k) {
+            case 4: return reinterpret_cast<const void *>(&k_spans<4, true>);
+            case 8: return reinterpret_cast<const void *>(&k_spans<8, true>);
+            case 16: return reinterpret_cast<const void *>(&k_spans<16, true>);
+            default: return reinterpret_cast<const void *>(&k_spans<32, true>);
+        }
+    }
     switch (k) {
-        case 4: return reinterpret_cast<const void *>(&k_spans<4>);
-        case 8: return reinterpret_cast<const void *>(&k_spans<8>);
-        case 16: return reinterpret_cast<const void *>(&k_spans<16>);
-        default: return reinterpret_cast<const void *>(&k_spans<32>);
+        case 4: return reinterpret_cast<const void *>(&k_spans<4, false>);
+        case 8: return reinterpret_cast<const void *>(&k_spans<8, false>);
+        case 16: return reinterpret_cast<const void *>(&k_spans<16, false>);
+        default: return reinterpret_cast<const void *>(&k_spans<32, false>);
     }
 }
 
@@ -1547,7 +1570,7 @@ int32_t init_batched(rr_hip_ctx *c) {
     for (int v = 0; v < kNumVariants; ++v) {
         int a = 0, b = 0;
         int b3 = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, span_kernel(kVariantK[v]), 64, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, span_kernel(kVariantK[v], false), 64, 0) != hipSuccess ||
             hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, compose_kernel(kVariantCH[v], false, 1), 64, 0) !=
                 hipSuccess ||
             hipOccupancyMaxActiveBlocksPerMultiprocessor(&b3, compose_kernel(kVariantCH[v], false, 3), 64, 0) !=
@@ -1713,7 +1736,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     c->prof_a0 = c->prof_used;
     for (int attempt = 0;; ++attempt) {
         const int32_t pass_mode = mode;
-        rc = launch_profiled(c, span_kernel(k_entries), a_waves, a_args);
+        rc = launch_profiled(c, span_kernel(k_entries, A.trace != nullptr), a_waves, a_args);
         // the bands the main pass deferred (exits at once when none)
         if (!rc) rc = launch_profiled(c, reinterpret_cast<const void *>(&k_spans_wide<kWideK>), w_waves, a_args);
         if (rc) return rc;
