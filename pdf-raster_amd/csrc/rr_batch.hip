@@ -648,8 +648,13 @@ __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &
     }
 }
 
-// Header of an item skipped as occluded (its interval counts are never read).
+// Header of an item skipped as occluded (its interval counts are never read);
+// the _lane form is stored by the calling lane.
 __device__ __forceinline__ void write_skipped(SpanHead *H) { hdr_head(H, kSkipped, 1, 0, 1, 0); }
+__device__ __forceinline__ void hdr_skipped_lane(SpanHead *H) {
+    *reinterpret_cast<int4 *>(H) = make_int4(static_cast<int32_t>(kSkipped & 0xffffffffll),
+                                             static_cast<int32_t>(kSkipped >> 32), 1, 1);
+}
 
 // One (path, band).  sm0: the path's first chunk summaries
 // (chunk_summary_first).  check: test the band for occlusion first (else the
@@ -742,7 +747,8 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
     span_finish(B, P, Y0, it, occ_row, G, G1, nint, unb, count_draw, occlusion, vmask, cur);
 }
 
-constexpr int kSpanGroup = 16;  // items tested for occlusion together (4 lanes each)
+constexpr int kSpanGroup = 16;   // items tested for occlusion together (4 lanes each)
+constexpr int kGroupTiles = 16;  // composite tiles of an item tested there (4 per lane)
 
 // Work queue split into kSlices contiguous slices with one counter each (in
 // its own 256-byte line).  A wave starts on slice blockIdx.x % kSlices --
@@ -778,49 +784,25 @@ constexpr int kWideK = 32;     // LDS entry slots of the wide pass
 
 // The path record of item path p as wave-uniform values (scalar loads).
 __device__ __forceinline__ ItemPath item_path(const BatchArgs &B, int32_t p) {
-    const SpanPath &S = B.spath[p];
+    // through the constant address space: scalar loads (the records are
+    // written by an earlier kernel and only read here)
+    const RR_KARG_AS SpanPath *S = ((const RR_KARG_AS SpanPath *)(B.spath)) + p;
     ItemPath P;
     P.p = p;
-    P.e0 = S.e0;
-    P.e1 = S.e1;
-    P.y_min = S.y_min;
-    P.y_max = S.y_max;
-    P.row0 = S.row0;
-    P.row1 = S.row1;
-    P.px0 = S.px0;
-    P.px1 = S.px1;
-    P.width = S.width;
-    P.height = S.height;
-    P.px_hi = S.px_hi;
-    P.occ_base = S.occ_base;
-    P.ctiles = S.ctiles;
-    P.eo = S.flags & 1;
-    P.opaque = (S.flags >> 1) & 1;
-    P.px_lo = (S.flags >> 16) & 0xffff;
-    return P;
-}
-
-// The same from the record's quarters held by lanes base..base+3.
-__device__ __forceinline__ ItemPath item_path_lanes(const int4 &q, int32_t p, int base) {
-    auto rl = [&](int32_t v, int k) { return __builtin_amdgcn_readlane(v, base + k); };
-    ItemPath P;
-    P.p = p;
-    P.e0 = static_cast<int64_t>((static_cast<uint64_t>(static_cast<uint32_t>(rl(q.y, 0))) << 32) |
-                                static_cast<uint32_t>(rl(q.x, 0)));
-    P.e1 = static_cast<int64_t>((static_cast<uint64_t>(static_cast<uint32_t>(rl(q.w, 0))) << 32) |
-                                static_cast<uint32_t>(rl(q.z, 0)));
-    P.y_min = rl(q.x, 1);
-    P.y_max = rl(q.y, 1);
-    P.row0 = rl(q.z, 1);
-    P.row1 = rl(q.w, 1);
-    P.px0 = rl(q.x, 2);
-    P.px1 = rl(q.y, 2);
-    P.width = rl(q.z, 2);
-    P.height = rl(q.w, 2);
-    P.px_hi = rl(q.x, 3);
-    P.occ_base = rl(q.y, 3);
-    P.ctiles = rl(q.z, 3);
-    const uint32_t fl = static_cast<uint32_t>(rl(q.w, 3));
+    P.e0 = S->e0;
+    P.e1 = S->e1;
+    P.y_min = S->y_min;
+    P.y_max = S->y_max;
+    P.row0 = S->row0;
+    P.row1 = S->row1;
+    P.px0 = S->px0;
+    P.px1 = S->px1;
+    P.width = S->width;
+    P.height = S->height;
+    P.px_hi = S->px_hi;
+    P.occ_base = S->occ_base;
+    P.ctiles = S->ctiles;
+    const uint32_t fl = S->flags;
     P.eo = fl & 1;
     P.opaque = (fl >> 1) & 1;
     P.px_lo = (fl >> 16) & 0xffff;
@@ -830,11 +812,12 @@ __device__ __forceinline__ ItemPath item_path_lanes(const int4 &q, int32_t p, in
 // Main span pass: every work item, descending path order (occluders are met
 // first), entry lists in LDS only; bands whose rows need more than K entries
 // are deferred to the wide pass.  Items are taken kSpanGrab at a time from
-// the sliced queue and handled kSpanGroup at a time; the group's single-band
-// items reaching at most 4 composite tiles are tested for
-// occlusion together (4 lanes per item), so a run of hidden items costs one
-// round trip, and a path's chunk summaries are loaded once for its run of
-// consecutive items.  TRACE: record one timing record per item (a separate
+// the sliced queue and handled kSpanGroup at a time; the first kGroupTiles
+// composite tiles of the group's single-band items are tested for occlusion
+// together (4 lanes per item, 4 tiles per lane), so a run of hidden items
+// costs one round trip (a wider item none of whose first tiles is seen gets
+// the full test of span_band), and a path's chunk summaries are loaded once
+// for its run of consecutive items.  TRACE: record one timing record per item (a separate
 // instance, so that the untraced pass carries none of its state).
 template <int K, bool TRACE>
 __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
@@ -857,41 +840,57 @@ __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
         if (g0 >= n_items) break;
       for (long long g = g0; g < g_end; g += kSpanGroup) {
         const int cnt = static_cast<int>(min(static_cast<long long>(kSpanGroup), g_end - g));
-        // lanes 4k..4k+3 hold item k of the group and quarter (lane & 3) of
-        // its path record, read back per item with readlane
+        // lanes 4k..4k+3 hold item k of the group; lanes 4k+2, 4k+3 quarters
+        // 2 and 3 of its path record: {px0, px1, width, height},
+        // {px_hi, occ_base, ctiles, flags}
         const int qi = lane >> 2, sub = lane & 3;
         int4 gi = make_int4(0, 0, 0, -1), gq = make_int4(0, 0, 0, 0);
         if (qi < cnt) {
             gi = RR_COLD(sitems)[n_items - 1 - g - qi];
-            gq = reinterpret_cast<const int4 *>(RR_COLD(spath) + gi.x)[sub];
+            if (sub >= 2) gq = reinterpret_cast<const int4 *>(RR_COLD(spath) + gi.x)[sub];
         }
-        // {px0, px1, width, height} sit in quarter 2, {px_hi, occ_base, ctiles, flags} in quarter 3
         const int32_t px0 = quad_bcast<2>(gq.x), px1 = quad_bcast<2>(gq.y);
         const int32_t obase = quad_bcast<3>(gq.y), ctl = quad_bcast<3>(gq.z);
-        bool tested = false, seen = false;
-        if (occlusion && qi < cnt && gi.w >= 0 && gi.z == 1) {
+        const bool live = qi < cnt && gi.w >= 0 && static_cast<long long>(gi.w) + gi.z <= RR_COLD(shdr_cap);
+        if (qi < cnt && !live && sub == 0) atomicOr(RR_COLD(F.err), 2);
+        // single-band items: lane sub tests tiles t0 + sub + 4i (i < 4), so the
+        // first kGroupTiles tiles of each item are tested at once
+        bool tested = false, wide = false, seen[4] = {false, false, false, false};
+        if (occlusion && live && gi.z == 1 && px0 <= px1) {
             const int32_t t0 = px0 >> B.chunk_shift, t1 = px1 >> B.chunk_shift;
-            if (px0 <= px1 && t1 - t0 < 4) {
-                tested = true;
-                const int32_t t = t0 + sub;
-                if (t <= t1) seen = B.occ[obase + static_cast<long long>(gi.y / kBandRows) * ctl + t] <= gi.x;
+            const int32_t *row = B.occ + obase + static_cast<long long>(gi.y / kBandRows) * ctl;
+            tested = true;
+            wide = t1 - t0 >= kGroupTiles;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int32_t t = t0 + sub + 4 * i;
+                if (t <= t1) seen[i] = row[t] <= gi.x;
             }
         }
-        const uint64_t tmask = __ballot(tested), smask = __ballot(seen);
-        for (int k = 0; k < cnt; ++k) {
+        // an item none of whose tiles is seen is skipped here: its lane 4k
+        // writes the header
+        int any = (seen[0] || seen[1] || seen[2] || seen[3]) ? 1 : 0;
+        any |= dpp_i32<kDppQuadXor1>(any);
+        any |= dpp_i32<kDppQuadXor2>(any);
+        const bool hidden = tested && !wide && any == 0;
+        if (hidden && sub == 0) hdr_skipped_lane(B.shead + gi.w);
+        cur.skipped += static_cast<uint32_t>(__popcll(__ballot(hidden && sub == 0)));
+        const uint64_t wmask = __ballot(wide), tmask = __ballot(tested);
+        const uint64_t smask0 = __ballot(seen[0]), smask1 = __ballot(seen[1]);
+        const uint64_t smask2 = __ballot(seen[2]), smask3 = __ballot(seen[3]);
+        // the group's other items, in order
+        for (uint64_t todo = __ballot(live && !hidden && sub == 0); todo; todo &= todo - 1) {
+            const int k = (__ffsll(static_cast<unsigned long long>(todo)) - 1) >> 2;
             const int32_t ip = __builtin_amdgcn_readlane(gi.x, 4 * k), iy = __builtin_amdgcn_readlane(gi.y, 4 * k);
             const int32_t iz = __builtin_amdgcn_readlane(gi.z, 4 * k), iw = __builtin_amdgcn_readlane(gi.w, 4 * k);
-            if (iw < 0 || static_cast<long long>(iw) + iz > RR_COLD(shdr_cap)) {
-                if (lane == 0) atomicOr(RR_COLD(F.err), 2);
-                continue;
-            }
-            const bool was_tested = (tmask >> (4 * k)) & 1;
-            if (was_tested && ((smask >> (4 * k)) & 0xf) == 0) {
-                ++cur.skipped;
-                write_skipped(B.shead + iw);
-                continue;
-            }
-            const ItemPath P = item_path_lanes(gq, ip, 4 * k);
+            // bit i of vm: tile t0 + i seen (not covered by a later opaque path)
+            const bool was_tested = (tmask >> (4 * k)) & 1, was_wide = (wmask >> (4 * k)) & 1;
+            const uint64_t vm = ((smask0 >> (4 * k)) & 0xf) | (((smask1 >> (4 * k)) & 0xf) << 4) |
+                                (((smask2 >> (4 * k)) & 0xf) << 8) | (((smask3 >> (4 * k)) & 0xf) << 12);
+            // a wide item none of whose first tiles is seen gets the full test
+            const bool full_check = occlusion && (!was_tested || (was_wide && vm == 0));
+            const uint64_t vmask = !was_tested ? ~0ull : (was_wide ? vm | (~0ull << kGroupTiles) : vm);
+            const ItemPath P = item_path(B, ip);
             if (ip != sm_path) {
                 sm0 = chunk_summary_first(A, P.e0, P.e1);
                 sm_path = ip;
@@ -900,8 +899,7 @@ __global__ void __launch_bounds__(64) k_spans(BatchArgs B, int32_t mode) {
             if constexpr (TRACE) t0 = static_cast<long long>(wall_clock64());
             for (int32_t bnd = 0; bnd < iz; ++bnd)
                 span_band<K, true>(B, L, P, iy + kBandRows * bnd, static_cast<long long>(iw) + bnd, 0, count_draw,
-                                   occlusion, occlusion && !was_tested,
-                                   was_tested ? (smask >> (4 * k)) & 0xfull : ~0ull, sm0, cur);
+                                   occlusion, full_check, vmask, sm0, cur);
             if (TRACE && lane == 0) {
                 const long long t1 = static_cast<long long>(wall_clock64());
                 const unsigned long long kk = atomicAdd(A.trace_top, 1ull);
