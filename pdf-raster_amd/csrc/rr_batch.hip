@@ -497,10 +497,10 @@ __device__ __forceinline__ void hdr_head(SpanHead *H, long long off, int32_t fx0
 // wave's global slot): sort and walk, drawn widths, fully covered columns,
 // store, header, occlusion publish.
 // Sort and walk of a row holding at most two entries, in registers (what
-// sort_and_walk does for n <= 2): its one interval goes back to slot 0.
+// sort_and_walk does for n <= 2): its one interval [*s, *e), if any.
 template <typename Ptr>
 __device__ __forceinline__ int walk_short(int n, Ptr X0, Ptr X1, Ptr C, int32_t mrule, int32_t aa_width,
-                                          bool *unbalanced) {
+                                          bool *unbalanced, int32_t *s, int32_t *e) {
     const int lane = threadIdx.x;
     if (n == 0) return 0;
     const int32_t a0 = X0[lane], b0 = X1[lane], c0 = C[lane];
@@ -523,17 +523,28 @@ __device__ __forceinline__ int walk_short(int n, Ptr X0, Ptr X1, Ptr C, int32_t 
     int32_t bx1 = end < aa_width ? end : aa_width;
     if (bx1 < 0) bx1 = 0;
     if (bx0 >= bx1) return 0;
-    X0[lane] = bx0;
-    X1[lane] = bx1;
+    *s = bx0;
+    *e = bx1;
     return 1;
 }
 
+// One interval per lane held in a register, indexed like an interval list
+// (any index is slot 0): span_finish over the short walk's result.
+struct RegIval {
+    int32_t v;
+    __device__ __forceinline__ int32_t &operator[](long long) { return v; }
+    __device__ __forceinline__ int32_t operator[](long long) const { return v; }
+};
+
 // nint, unb: the walk's interval count of this lane's row (in X0/X1 slots)
 // and whether the row is unbalanced.
-template <typename Ptr>
+// MAXN: 1 when every lane holds at most one interval (RegIval lists), 0 for
+// any count.
+template <int MAXN, typename Ptr>
 __device__ __forceinline__ void span_finish(const BatchArgs &B, const ItemPath &P, int32_t Y0, long long it,
-                                            const int32_t *occ_row, Ptr X0, Ptr X1, int nint, bool unb,
+                                            const int32_t *occ_row, Ptr &X0, Ptr &X1, int nint_in, bool unb,
                                             bool count_draw, bool occlusion, uint64_t vmask, SpanCursor &cur) {
+    const int nint = MAXN == 1 ? min(nint_in, 1) : nint_in;
     const int lane = threadIdx.x;
     const int32_t y = Y0 + (lane >> 2), j = lane & 3;
     // an unbalanced row spans to the canvas edge: widen the path's columns;
@@ -707,9 +718,15 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
                           sm0);
     if (__ballot(n > K) == 0) {
         bool unb = false;
-        const int nint = __ballot(n > 2) == 0 ? walk_short(n, L.ex0, L.ex1, L.ec, mrule, P.width * 4, &unb)
-                                               : sort_and_walk(n, L.ex0, L.ex1, L.ec, mrule, P.width * 4, &unb);
-        span_finish(B, P, Y0, it, occ_row, L.ex0, L.ex1, nint, unb, count_draw, occlusion, vmask, cur);
+        if (__ballot(n > 2) == 0) {
+            RegIval s0{0}, e0{0};
+            const int nint = walk_short(n, L.ex0, L.ex1, L.ec, mrule, P.width * 4, &unb, &s0.v, &e0.v);
+            span_finish<1>(B, P, Y0, it, occ_row, s0, e0, nint, unb, count_draw, occlusion, vmask, cur);
+        } else {
+            const int nint = sort_and_walk(n, L.ex0, L.ex1, L.ec, mrule, P.width * 4, &unb);
+            int32_t *X0 = L.ex0, *X1 = L.ex1;
+            span_finish<0>(B, P, Y0, it, occ_row, X0, X1, nint, unb, count_draw, occlusion, vmask, cur);
+        }
         return;
     }
     if constexpr (DEFER) {
@@ -744,7 +761,7 @@ __device__ __forceinline__ void span_band(const BatchArgs &B, SpanLds<K> &L, con
     n = build_entries(A, L, P.e0, P.e1, band_lo, band_hi, literal, Y0, valid, r, mrule, G, G1, G2, need, sm0);
     bool unb = false;
     const int nint = sort_and_walk(n, G, G1, G2, mrule, P.width * 4, &unb);
-    span_finish(B, P, Y0, it, occ_row, G, G1, nint, unb, count_draw, occlusion, vmask, cur);
+    span_finish<0>(B, P, Y0, it, occ_row, G, G1, nint, unb, count_draw, occlusion, vmask, cur);
 }
 
 constexpr int kSpanGroup = 16;   // items tested for occlusion together (4 lanes each)
