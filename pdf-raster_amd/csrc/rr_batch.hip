@@ -1446,7 +1446,7 @@ __device__ void compose_tile(const BatchArgs &B, ComposeLds<CH, NPL> &L, long lo
 }
 
 template <int CH, bool COV, int NPL>
-__global__ void __launch_bounds__(64) k_compose(BatchArgs B, long long n_items) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) k_compose(BatchArgs B, long long n_items) {
     __shared__ ComposeLds<CH, NPL> L;
     if (threadIdx.x < 17) L.gamma[threadIdx.x] = kGamma[threadIdx.x];
     __syncthreads();
