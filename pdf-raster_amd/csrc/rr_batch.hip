@@ -1497,14 +1497,14 @@ constexpr long long kSlotInts = static_cast<long long>(kSlotK) * 192;
 const void *span_kernel(int k, bool trace) {
     if (trace) {
         switch (k) {
-            case 4: return reinterpret_cast<const void *>(&k_spans<4, true>);
+            case 6: return reinterpret_cast<const void *>(&k_spans<6, true>);
             case 8: return reinterpret_cast<const void *>(&k_spans<8, true>);
             case 16: return reinterpret_cast<const void *>(&k_spans<16, true>);
             default: return reinterpret_cast<const void *>(&k_spans<32, true>);
         }
     }
     switch (k) {
-        case 4: return reinterpret_cast<const void *>(&k_spans<4, false>);
+        case 6: return reinterpret_cast<const void *>(&k_spans<6, false>);
         case 8: return reinterpret_cast<const void *>(&k_spans<8, false>);
         case 16: return reinterpret_cast<const void *>(&k_spans<16, false>);
         default: return reinterpret_cast<const void *>(&k_spans<32, false>);

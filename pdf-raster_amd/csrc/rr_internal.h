@@ -39,7 +39,7 @@ constexpr int kQueueWords = 2 * 8 * 32;
 constexpr int kNumVariants = 16;  // instantiated fill shapes (the autotuner's search space)
 // K = LDS entry slots per AA row; CH = pixels per composite sweep (plain) /
 // pixels per composite tile (batched).
-constexpr int kVariantK[kNumVariants] = {4, 4, 4, 4, 8, 8, 8, 8, 16, 16, 16, 16, 32, 32, 32, 32};
+constexpr int kVariantK[kNumVariants] = {6, 6, 6, 6, 8, 8, 8, 8, 16, 16, 16, 16, 32, 32, 32, 32};
 constexpr int kVariantCH[kNumVariants] = {64, 128, 256, 512, 64, 128, 256, 512,
                                           64, 128, 256, 512, 64, 128, 256, 512};
 
