@@ -461,7 +461,24 @@ __device__ __forceinline__ int2 chunk_summary_first(const FillArgs &A, int64_t e
     return sm;
 }
 
-template <typename Lds, typename Ptr>
+// Barrier of a band's wave: the whole (one-wave) workgroup, or with WS only
+// this wave (a wave of a larger workgroup working alone: its LDS accesses
+// execute in order, so a compiler fence suffices).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <bool WS>
+__device__ __forceinline__ void band_sync() {
+    if constexpr (WS) {
+        wave_sync();
+    } else {
+        __syncthreads();
+    }
+}
+
+template <bool WS = false, typename Lds, typename Ptr>
 __device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t e0, int64_t e1,
                                              int32_t band_lo, int32_t band_hi, bool literal, int32_t Y0, bool valid,
                                              int32_t r, int32_t mrule, Ptr X0, Ptr X1, Ptr C, int cap, int2 sm0) {
@@ -486,7 +503,7 @@ __device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t 
         int la = 64, lb = -1;
         if (r0 <= r1) lane_range(r0, r1, literal, Y0, band_lo, band_hi, &la, &lb);
         L.smask[lane] = 0ull;
-        __syncthreads();
+        band_sync<WS>();
         // edge j toggles its bit on at lane la and off at lane lb + 1: the
         // prefix XOR leaves it set exactly on lanes la..lb.  Edges that start
         // at lane 0 (most: they enter the band from above) are a ballot
@@ -496,7 +513,7 @@ __device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t 
             if (la > 0) atomicXor(&L.smask[la], 1ull << lane);
             if (lb < 63) atomicXor(&L.smask[lb + 1], 1ull << lane);
         }
-        __syncthreads();
+        band_sync<WS>();
         unsigned long long m = wave_incl_xor64(L.smask[lane]) ^ from0;
         if (!valid) m = 0ull;
         while (m) {
@@ -513,7 +530,7 @@ __device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t 
             lx0 = touch ? min(lx0, en.x0) : en.x0;
             lx1 = touch ? max(lx1, en.x1) : en.x1;
         }
-        __syncthreads();
+        band_sync<WS>();
     };
     if (e1 <= e0) return 0;
     const int64_t c_first = e0 >> 6, c_last = (e1 - 1) >> 6;
@@ -559,6 +576,10 @@ __device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t 
 // when the winding never returned to zero (the reference would loop forever
 // there; the span runs to aa_width and the row stops).
 template <typename Ptr>
+__device__ __forceinline__ int walk_sorted(int n, Ptr X0, Ptr X1, Ptr C, int32_t mrule, int32_t aa_width,
+                                           bool *unbalanced);
+
+template <typename Ptr>
 __device__ __forceinline__ int sort_and_walk(int n, Ptr X0, Ptr X1, Ptr C, int32_t mrule, int32_t aa_width,
                                              bool *unbalanced) {
     const int lane = threadIdx.x;
@@ -579,6 +600,16 @@ __device__ __forceinline__ int sort_and_walk(int n, Ptr X0, Ptr X1, Ptr C, int32
             C[b * 64 + lane] = vc;
         }
     }
+    return walk_sorted(n, X0, X1, C, mrule, aa_width, unbalanced);
+}
+
+// The winding walk of render_aa_line over entries already sorted by x0 (lane
+// (threadIdx.x & 63)'s list, [slot * 64 + lane]): disjoint sorted intervals
+// written back in place over X0 (s) / X1 (e); returns their count.
+template <typename Ptr>
+__device__ __forceinline__ int walk_sorted(int n, Ptr X0, Ptr X1, Ptr C, int32_t mrule, int32_t aa_width,
+                                           bool *unbalanced) {
+    const int lane = threadIdx.x & 63;
     int i = 0, nout = 0;
     int32_t cs = 0, ce = -1;
     int32_t count = 0;

@@ -307,78 +307,42 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
 // Row summary of every 64-edge chunk (global edge index / 64): the smallest r0
 // and largest r1 of its non-empty records, so that a band skips whole chunks.
 
-// Rasterises path p over output rows [Y0, Y0+15] of page `pg` and composites
-// into its canvas.  Called by one full wave64 (blockDim.x == 64); `slot` is
-// this wave's private overflow slot.
-template <int K, int CH>
-__device__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &pg, int64_t p, int32_t Y0,
-                          int64_t slot) {
-    const int lane = threadIdx.x;
-    // independent loads first, so their latencies overlap
-    const PathHdr h = A.hdrs[p];
-    const int2 drawn = A.rows[p];
-    const rr_path_attr at = A.attrs[p];
-    const int64_t e0 = A.edge_off[p], e1 = A.edge_off[p + 1];
-    if (h.flags & (HDR_EMPTY | HDR_OUTSIDE)) return;
-    const int32_t ylo = drawn.x, yhi = drawn.y;
-    if (Y0 > yhi || Y0 + kBandRows - 1 < ylo) return;
-    const bool literal = A.mode == RR_MODE_LITERAL;
-
-    const int32_t d = lane >> 2, j = lane & 3;
-    const int32_t y = Y0 + d;
-    const int32_t r = literal ? 16 * y + 12 + j : 4 * y + j;
-    const int32_t band_lo = literal ? 16 * Y0 + 12 : 4 * Y0;
-    const int32_t band_hi = literal ? 16 * (Y0 + kBandRows - 1) + 15 : 4 * Y0 + 4 * kBandRows - 1;
-    const bool valid = y >= ylo && y <= yhi && r >= h.y_min && r <= h.y_max;
-    const int32_t mrule = (at.eo & RR_FILL_EVEN_ODD) ? 1 : -1;
-    const int32_t aa_width = pg.width * 4;
-
-    int32_t *X0 = L.ex0, *X1 = L.ex1, *C = L.ec;
-    const int2 sm0 = chunk_summary_first(A, e0, e1);
-    int n = build_entries(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, K, sm0);
-    int need = n;
-    for (int o = 32; o > 0; o >>= 1) need = max(need, __shfl_xor(need, o));
-    if (need > K) {
-        // A row with more state-changing entries than the LDS holds: redo the
-        // band with the entry lists in this wave's global slot, or for very
-        // long rows in a slice of the shared pool.
-        int32_t *G;
-        if (static_cast<long long>(need) * 192 <= A.slot_ints) {
-            G = A.slots + slot * A.slot_ints;
-        } else {
-            unsigned long long off = 0;
-            if (lane == 0) off = atomicAdd(A.spill_top, static_cast<unsigned long long>(need) * 192ull);
-            off = __shfl(off, 0);
-            if (static_cast<long long>(off) + static_cast<long long>(need) * 192 > A.spill_cap) {
-                if (lane == 0) atomicExch(A.err, 1);
-                return;
-            }
-            G = A.spill + off;
-        }
-        X0 = G;
-        X1 = G + static_cast<long long>(need) * 64;
-        C = G + static_cast<long long>(need) * 128;
-        n = build_entries(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, need, sm0);
+// draw_aa_line's pixel count of the band (the reference composites x0..x1 of
+// every drawn row, x0 = xx_min/4, x1 = (xx_max-1)/4 over its 4 AA rows,
+// scanner/mod.rs:459-466, clamped by the clip): summed on the rows' first
+// lanes.  X0/X1: each lane's disjoint sorted intervals ([k * 64 + lane]).
+template <typename Ptr>
+__device__ __forceinline__ unsigned long long band_draw_width(const DevPage &pg, int32_t Y0, int32_t ylo,
+                                                              int32_t yhi, int nint, Ptr X0, Ptr X1) {
+    const int lane = threadIdx.x & 63;
+    const int32_t y = Y0 + (lane >> 2);
+    int32_t smin = nint ? X0[lane] : INT32_MAX;
+    int32_t emax = nint ? X1[(nint - 1) * 64 + lane] : -1;
+    smin = min(smin, __shfl_xor(smin, 1));
+    smin = min(smin, __shfl_xor(smin, 2));
+    emax = max(emax, __shfl_xor(emax, 1));
+    emax = max(emax, __shfl_xor(emax, 2));
+    unsigned long long w = 0;
+    if ((lane & 3) == 0 && y >= ylo && y <= yhi && smin < emax) {
+        const int32_t x0 = max(smin / 4, pg.px_lo), x1 = min((emax - 1) / 4, pg.px_hi);
+        if (x0 <= x1) w = static_cast<unsigned long long>(x1 - x0 + 1);
     }
-    bool unb = false;
-    const int nint = sort_and_walk(n, X0, X1, C, mrule, aa_width, &unb);
-    const bool any_unb = __ballot(unb) != 0;
-    unsigned long long w_draw = 0;
-    if (A.cov_count) {
-        // draw_aa_line range of each output row: x0 = xx_min/4, x1 = (xx_max-1)/4
-        // over its 4 AA rows (scanner/mod.rs:459-466), clamped by the clip.
-        int32_t smin = nint ? X0[lane] : INT32_MAX;
-        int32_t emax = nint ? X1[(nint - 1) * 64 + lane] : -1;
-        smin = min(smin, __shfl_xor(smin, 1));
-        smin = min(smin, __shfl_xor(smin, 2));
-        emax = max(emax, __shfl_xor(emax, 1));
-        emax = max(emax, __shfl_xor(emax, 2));
-        if (j == 0 && y >= ylo && y <= yhi && smin < emax) {
-            const int32_t x0 = max(smin / 4, pg.px_lo), x1 = min((emax - 1) / 4, pg.px_hi);
-            if (x0 <= x1) w_draw = static_cast<unsigned long long>(x1 - x0 + 1);
-        }
-    }
+    return w;
+}
 
+// Composite sweep of one band: this wave takes the CH-pixel steps wv, wv +
+// nwv, ... of the band's covered columns, builds each step's 64-row coverage
+// bitmap in its own `bm` (64 rows of CH/8 dwords, padded by one), and
+// composites 4 pixels per lane per dword.  X0/X1: each lane's disjoint sorted
+// intervals; any_unb: a row of the band never closed its winding (its span
+// runs to the page edge).  Adds the path's coverage hash / pixel counts (and
+// w_draw, the caller's draw_aa_line count) when asked.
+template <int CH, bool WS, typename Ptr>
+__device__ __forceinline__ void composite_band(const FillArgs &A, uint32_t *bm, const uint8_t *gamma,
+                                               const DevPage &pg, int64_t p, int32_t Y0, const PathHdr &h,
+                                               int32_t ylo, int32_t yhi, const rr_path_attr &at, int nint, Ptr X0,
+                                               Ptr X1, bool any_unb, int wv, int nwv, unsigned long long w_draw) {
+    const int lane = threadIdx.x & 63;
     // pixel range that can hold coverage, intersected with the clip
     const int32_t px_lo = max(pg.px_lo, floor_div4(h.x_min));
     const int32_t px_hi = min(pg.px_hi, any_unb ? pg.width - 1 : floor_div4(h.x_max));
@@ -394,15 +358,15 @@ __device__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &p
     uint64_t hsum = 0;
     unsigned long long w_cov = 0;
     int ip = 0;  // first interval of this lane not yet fully left of the chunk
-    constexpr int kWords = BandLds<K, CH>::kWords, kPitch = BandLds<K, CH>::kPitch;
+    constexpr int kWords = CH / 8, kPitch = kWords + 1;  // bitmap dwords per AA row, padded
     constexpr int kLanesPerRow = CH / 4;            // one dword (4 pixels) per lane
     constexpr int kRowsPerIter = 64 / kLanesPerRow;  // output rows composited per pass
     constexpr int kIters = kBandRows / kRowsPerIter;
     constexpr int kGroup = kIters < 4 ? kIters : 4;
-    uint32_t *bm_row = L.bm + lane * kPitch;
+    uint32_t *bm_row = bm + lane * kPitch;
     const uint32_t src4 = src * 0x01010101u;
     const int q = lane % kLanesPerRow, sub = lane / kLanesPerRow;
-    for (int32_t cx = xs; any_px && cx <= px_hi; cx += CH) {
+    for (int32_t cx = xs + wv * CH; any_px && cx <= px_hi; cx += nwv * CH) {
         const int32_t A0 = 4 * cx, A1 = A0 + 4 * CH;
         while (ip < nint && X1[ip * 64 + lane] <= A0) ++ip;
         if (__ballot(ip < nint && X0[ip * 64 + lane] < A1) == 0) continue;  // nothing covered here
@@ -422,7 +386,7 @@ __device__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &p
                 bm_row[wh] |= mh;
             }
         }
-        __syncthreads();
+        band_sync<WS>();
         const int32_t xq = cx + 4 * q;  // this lane's 4 pixels
         const int sh = (q & 1) * 16;
         const bool lane_in = xq <= px_hi;
@@ -439,7 +403,7 @@ __device__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &p
                 if (lane_in && yy >= ylo && yy <= yhi) {
 #pragma unroll
                     for (int rr = 0; rr < 4; ++rr) {
-                        const uint32_t f = (L.bm[(4 * dd + rr) * kPitch + (q >> 1)] >> sh) & 0xffffu;
+                        const uint32_t f = (bm[(4 * dd + rr) * kPitch + (q >> 1)] >> sh) & 0xffffu;
                         uint32_t t = f - ((f >> 1) & 0x5555u);
                         t = (t & 0x3333u) + ((t >> 2) & 0x3333u);
                         cv += (t & 0xfu) | ((t & 0xf0u) << 4) | ((t & 0xf00u) << 8) | ((t & 0xf000u) << 12);
@@ -474,7 +438,7 @@ __device__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &p
                             const uint32_t cnt = (cv >> (8 * px)) & 0xffu;
                             const int32_t x = xq + px;
                             if (cnt == 0 || x < px_lo || x > px_hi) continue;
-                            const uint32_t shape = L.gamma[cnt], sm = (smw >> (8 * px)) & 0xffu;
+                            const uint32_t shape = gamma[cnt], sm = (smw >> (8 * px)) & 0xffu;
 #pragma unroll
                             for (int c = 0; c < 3; ++c) {
                                 const uint32_t res = paint_px((pl[c] >> (8 * px)) & 0xffu, shape, a_in,
@@ -499,7 +463,7 @@ __device__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &p
                             const uint32_t cnt = (cv >> (8 * px)) & 0xffu;
                             const int32_t x = xq + px;
                             if (cnt == 0 || x < px_lo || x > px_hi) continue;
-                            const uint32_t res = paint_px((out >> (8 * px)) & 0xffu, L.gamma[cnt], a_in, src, fl,
+                            const uint32_t res = paint_px((out >> (8 * px)) & 0xffu, gamma[cnt], a_in, src, fl,
                                                           (smw >> (8 * px)) & 0xffu, fast);
                             out = (out & ~(0xffu << (8 * px))) | (res << (8 * px));
                         }
@@ -512,12 +476,12 @@ __device__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &p
                         const int32_t x = xq + px;
                         if (cnt == 0 || x < px_lo || x > px_hi) continue;
                         ++w_cov;
-                        if (A.cov_hash) hsum += cov_term(static_cast<long long>(yy) * pg.width + x, L.gamma[cnt]);
+                        if (A.cov_hash) hsum += cov_term(static_cast<long long>(yy) * pg.width + x, gamma[cnt]);
                     }
                 }
             }
         }
-        __syncthreads();
+        band_sync<WS>();
     }
     if (A.cov_hash) {
         for (int o = 32; o > 0; o >>= 1) hsum += __shfl_xor(hsum, o);
@@ -535,47 +499,422 @@ __device__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &p
     }
 }
 
-// fill_band plus, when tracing, one wall-clock record of its duration.
-template <int K, int CH>
-__device__ __forceinline__ void fill_band_traced(const FillArgs &A, BandLds<K, CH> &L, const DevPage &pg, int32_t page,
-                                                 int64_t p, int32_t Y0, int64_t slot) {
-    if (!A.trace) {
-        fill_band<K, CH>(A, L, pg, p, Y0, slot);
-        return;
+// Rasterises path p over output rows [Y0, Y0+15] of page `pg` and composites
+// into its canvas.  Called by one full wave64, lanes threadIdx.x 0..63: the
+// whole workgroup, or with WS the first wave of a larger one working alone;
+// `slot` is this wave's private overflow slot.
+template <int K, int CH, bool WS>
+__device__ __forceinline__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &pg, int64_t p, int32_t Y0,
+                          int64_t slot) {
+    const int lane = threadIdx.x;
+    // independent loads first, so their latencies overlap
+    const PathHdr h = A.hdrs[p];
+    const int2 drawn = A.rows[p];
+    const rr_path_attr at = A.attrs[p];
+    const int64_t e0 = A.edge_off[p], e1 = A.edge_off[p + 1];
+    if (h.flags & (HDR_EMPTY | HDR_OUTSIDE)) return;
+    const int32_t ylo = drawn.x, yhi = drawn.y;
+    if (Y0 > yhi || Y0 + kBandRows - 1 < ylo) return;
+    const bool literal = A.mode == RR_MODE_LITERAL;
+
+    const int32_t d = lane >> 2, j = lane & 3;
+    const int32_t y = Y0 + d;
+    const int32_t r = literal ? 16 * y + 12 + j : 4 * y + j;
+    const int32_t band_lo = literal ? 16 * Y0 + 12 : 4 * Y0;
+    const int32_t band_hi = literal ? 16 * (Y0 + kBandRows - 1) + 15 : 4 * Y0 + 4 * kBandRows - 1;
+    const bool valid = y >= ylo && y <= yhi && r >= h.y_min && r <= h.y_max;
+    const int32_t mrule = (at.eo & RR_FILL_EVEN_ODD) ? 1 : -1;
+    const int32_t aa_width = pg.width * 4;
+
+    int32_t *X0 = L.ex0, *X1 = L.ex1, *C = L.ec;
+    const int2 sm0 = chunk_summary_first(A, e0, e1);
+    int n = build_entries<WS>(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, K, sm0);
+    int need = n;
+    for (int o = 32; o > 0; o >>= 1) need = max(need, __shfl_xor(need, o));
+    if (need > K) {
+        // A row with more state-changing entries than the LDS holds: redo the
+        // band with the entry lists in this wave's global slot, or for very
+        // long rows in a slice of the shared pool.
+        int32_t *G;
+        if (static_cast<long long>(need) * 192 <= A.slot_ints) {
+            G = A.slots + slot * A.slot_ints;
+        } else {
+            unsigned long long off = 0;
+            if (lane == 0) off = atomicAdd(A.spill_top, static_cast<unsigned long long>(need) * 192ull);
+            off = __shfl(off, 0);
+            if (static_cast<long long>(off) + static_cast<long long>(need) * 192 > A.spill_cap) {
+                if (lane == 0) atomicExch(A.err, 1);
+                return;
+            }
+            G = A.spill + off;
+        }
+        X0 = G;
+        X1 = G + static_cast<long long>(need) * 64;
+        C = G + static_cast<long long>(need) * 128;
+        n = build_entries<WS>(A, L, e0, e1, band_lo, band_hi, literal, Y0, valid, r, mrule, X0, X1, C, need, sm0);
     }
-    const long long t0 = static_cast<long long>(wall_clock64());
-    fill_band<K, CH>(A, L, pg, p, Y0, slot);
-    const long long t1 = static_cast<long long>(wall_clock64());
-    if (threadIdx.x == 0) {
+    bool unb = false;
+    const int nint = sort_and_walk(n, X0, X1, C, mrule, aa_width, &unb);
+    const bool any_unb = __ballot(unb) != 0;
+    const unsigned long long w_draw = A.cov_count ? band_draw_width(pg, Y0, ylo, yhi, nint, X0, X1) : 0ull;
+    composite_band<CH, WS>(A, L.bm, L.gamma, pg, p, Y0, h, ylo, yhi, at, nint, X0, X1, any_unb, 0, 1, w_draw);
+}
+
+// ---------------------------------------------------------------------------
+// PLAIN: several waves per band
+// ---------------------------------------------------------------------------
+// A one-path launch holds only that path's bands (two or three workgroups for
+// a glyph), so a band's time is the latency of its serial chain.  Here NW
+// waves share the band.  Pass by pass, wave w takes the w-th of the next NW
+// edge chunks (64 edges each) that overlap the band, marks per AA row (one
+// lane per row) the chunk edges crossing it, and writes each crossing's
+// intersection (fill_intersections, scanner/mod.rs:528-585) into the row's
+// LDS list at its place in edge order: the waves' crossing counts are summed
+// in wave order ahead of the writes, so the list comes out in the order the
+// reference walks the edges.  The first wave then folds each row's list with
+// write_intersect's merge (scanner/mod.rs:600-642) -- now a short loop over
+// LDS instead of a loop over the path's edges.  The stable sort by x0 is a
+// rank count split over the waves, the walk is render_aa_line's, and the
+// composite sweep's CH-pixel steps go to the waves in turn.  A row with more
+// intersections or entries than the LDS lists hold sends the band, untouched,
+// to the one-wave fill_band (first wave, global entry slots).
+constexpr int kMwWaves = 8;   // waves per band
+constexpr int kMwPool = 128;  // intersections per AA row of the band in LDS
+constexpr int kMwK = 40;      // entries per AA row after the merge
+constexpr int kMwBlock = 8;   // entries read into registers at a time
+
+template <int NW, int P, int K2, int CH>
+struct MwLds {
+    static constexpr int kPitch = CH / 8 + 1;
+    struct Build {
+        int4 recA[NW][64], recB[NW][64];   // each wave's chunk as two 16-byte halves per edge
+        unsigned long long smask[NW][64];  // per wave, per lane row: chunk edges crossing it
+        int32_t cnt[2][NW][64];            // crossings of (wave's chunk, row), by pass parity
+        // the merged entries [k * 64 + lane]; row K2 takes the stores the
+        // branch-free loops discard
+        int32_t b0[(K2 + 1) * 64], b1[(K2 + 1) * 64], bc[(K2 + 1) * 64];
+    };
+    union {
+        Build b;
+        uint32_t bm[NW][64 * kPitch];  // each wave's coverage bitmap (composite sweep)
+    } u;
+    // intersections of each row in edge order [k * 64 + lane]; then the
+    // sorted entries; then the intervals
+    // (row P takes the stores of intersections beyond the lists)
+    int32_t p0[(P + 1) * 64], p1[(P + 1) * 64], pc[(P + 1) * 64];
+    int32_t nb[64];
+    uint8_t gamma[32];
+    int32_t over, unb;
+};
+
+// Returns false, with nothing drawn, when a row outgrew the LDS lists.
+template <int NW, int P, int K2, int CH>
+__device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2, CH> &L, const DevPage &pg,
+                                             int64_t p, int32_t Y0) {
+    static_assert(P >= K2, "the sorted entries reuse the intersection lists");
+    const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const PathHdr h = A.hdrs[p];
+    const int2 drawn = A.rows[p];
+    const rr_path_attr at = A.attrs[p];
+    const int64_t e0 = A.edge_off[p], e1 = A.edge_off[p + 1];
+    if (h.flags & (HDR_EMPTY | HDR_OUTSIDE)) return true;
+    const int32_t ylo = drawn.x, yhi = drawn.y;
+    if (Y0 > yhi || Y0 + kBandRows - 1 < ylo) return true;
+    const bool literal = A.mode == RR_MODE_LITERAL;
+    const int32_t y = Y0 + (lane >> 2);
+    const int32_t r = literal ? 16 * y + 12 + (lane & 3) : 4 * y + (lane & 3);
+    const int32_t band_lo = literal ? 16 * Y0 + 12 : 4 * Y0;
+    const int32_t band_hi = literal ? 16 * (Y0 + kBandRows - 1) + 15 : 4 * Y0 + 4 * kBandRows - 1;
+    const bool valid = y >= ylo && y <= yhi && r >= h.y_min && r <= h.y_max;
+    const int32_t mrule = (at.eo & RR_FILL_EVEN_ODD) ? 1 : -1;
+    if (threadIdx.x < 17) L.gamma[threadIdx.x] = kGamma[threadIdx.x];
+    if (threadIdx.x == 0) L.over = 0;
+    __syncthreads();
+
+    int top = 0;  // intersections of this lane's row so far (every wave keeps the same count)
+    int par = 0;
+    if (e1 > e0) {
+        const int64_t c_first = e0 >> 6, c_last = (e1 - 1) >> 6;
+        for (int64_t cb = c_first; cb <= c_last; cb += 64) {
+            bool hit = false;
+            if (cb + lane <= c_last) {
+                const int2 sm = A.csum[cb + lane];
+                hit = sm.x <= band_hi && sm.y >= band_lo;
+            }
+            uint64_t cm = __ballot(hit);  // the same in every wave
+            // this wave's chunk of a pass: the wv-th set bit of the mask left
+            auto pick = [&](uint64_t m) {
+                for (int i = 0; i < wv; ++i) m &= m - 1;
+                return m;
+            };
+            auto load_rec = [&](uint64_t m) {
+                EdgeRec q{1, 0, 0, 0, 0, 0, 0, 0};
+                const int64_t i = ((cb + (__ffsll(static_cast<unsigned long long>(m)) - 1)) << 6) + lane;
+                if (i >= e0 && i < e1) q = A.recs[i];
+                return q;
+            };
+            uint64_t mine = pick(cm);
+            EdgeRec qn{1, 0, 0, 0, 0, 0, 0, 0};
+            if (mine) qn = load_rec(mine);
+            while (cm) {
+                for (int i = 0; i < NW; ++i) cm &= cm - 1;
+                const EdgeRec q = qn;
+                const bool have = mine != 0;
+                mine = pick(cm);
+                if (mine) qn = load_rec(mine);  // the next pass's chunk, in flight during this one
+                unsigned long long m = 0ull;
+                if (have) {
+                    L.u.b.recA[wv][lane] = make_int4(q.r0, q.r1, q.meta, q.a);
+                    L.u.b.recB[wv][lane] = make_int4(q.b, q.c, q.d, q.e);
+                    int la = 64, lb = -1;
+                    if (q.r0 <= q.r1) lane_range(q.r0, q.r1, literal, Y0, band_lo, band_hi, &la, &lb);
+                    L.u.b.smask[wv][lane] = 0ull;
+                    wave_sync();
+                    // edge j toggles its bit on at lane la and off at lane lb + 1
+                    const unsigned long long from0 = __ballot(la == 0 && lb >= 0);
+                    if (la <= lb) {
+                        if (la > 0) atomicXor(&L.u.b.smask[wv][la], 1ull << lane);
+                        if (lb < 63) atomicXor(&L.u.b.smask[wv][lb + 1], 1ull << lane);
+                    }
+                    wave_sync();
+                    m = wave_incl_xor64(L.u.b.smask[wv][lane]) ^ from0;
+                    if (!valid) m = 0ull;
+                }
+                L.u.b.cnt[par][wv][lane] = __popcll(m);
+                __syncthreads();
+                // this wave's place in the row's list: after the chunks before it
+                int off = top;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) {
+                    const int c = L.u.b.cnt[par][w][lane];
+                    off += w < wv ? c : 0;
+                    top += c;
+                }
+                par ^= 1;
+                // the intersections, in edge order; the next record's LDS
+                // load is issued ahead
+                int4 ga = make_int4(0, 0, 0, 0), gb = ga;
+                if (m) {
+                    const int jj = __ffsll(m) - 1;
+                    ga = L.u.b.recA[wv][jj];
+                    gb = L.u.b.recB[wv][jj];
+                }
+                while (m) {
+                    m &= m - 1;
+                    const EdgeRec g{ga.x, ga.y, ga.z, ga.w, gb.x, gb.y, gb.z, gb.w};
+                    if (m) {
+                        const int jn = __ffsll(m) - 1;
+                        ga = L.u.b.recA[wv][jn];
+                        gb = L.u.b.recB[wv][jn];
+                    }
+                    const Entry en = row_entry(g, r);
+                    const int slot = min(off, P);
+                    L.p0[slot * 64 + lane] = en.x0;
+                    L.p1[slot * 64 + lane] = en.x1;
+                    L.pc[slot * 64 + lane] = en.c;
+                    ++off;
+                }
+            }
+        }
+    }
+    if (__ballot(top > P) != 0 && lane == 0) L.over = 1;
+    __syncthreads();
+    // Entries are read from LDS kMwBlock at a time into registers and the
+    // loops over them are unrolled and branch-free (a store that is not
+    // wanted goes to the spare row).
+    constexpr int kB = kMwBlock;
+    if (wv == 0 && L.over == 0) {
+        // write_intersect over the row's intersections: merge into the open
+        // entry while they touch it; a retired entry is kept only when it can
+        // change the walk's inside/outside state
+        int32_t lx0 = INT32_MAX, lx1 = INT32_MIN, lc = 0;
+        int nb = 0;
+        for (int k0 = 0; k0 < top; k0 += kB) {
+            int32_t x0[kB], x1[kB], cc[kB];
+#pragma unroll
+            for (int j = 0; j < kB; ++j) {
+                const int k = min(k0 + j, P);
+                x0[j] = L.p0[k * 64 + lane];
+                x1[j] = L.p1[k * 64 + lane];
+                cc[j] = L.pc[k * 64 + lane];
+            }
+#pragma unroll
+            for (int j = 0; j < kB; ++j) {
+                const bool v = k0 + j < top;
+                const bool touch = __builtin_elementwise_add_sat(lx1, 1) >= x0[j] &&
+                                   lx0 <= __builtin_elementwise_add_sat(x1[j], 1);
+                const bool st = v && !touch && (lc & mrule) != 0;
+                const int slot = st ? min(nb, K2) : K2;
+                L.u.b.b0[slot * 64 + lane] = lx0;
+                L.u.b.b1[slot * 64 + lane] = lx1;
+                L.u.b.bc[slot * 64 + lane] = lc;
+                nb += st ? 1 : 0;
+                const int32_t sum = static_cast<int32_t>(static_cast<uint32_t>(lc) + static_cast<uint32_t>(cc[j]));
+                lc = v ? (touch ? sum : cc[j]) : lc;
+                lx0 = v ? (touch ? min(lx0, x0[j]) : x0[j]) : lx0;
+                lx1 = v ? (touch ? max(lx1, x1[j]) : x1[j]) : lx1;
+            }
+        }
+        if ((lc & mrule) != 0) {
+            const int slot = min(nb, K2);
+            L.u.b.b0[slot * 64 + lane] = lx0;
+            L.u.b.b1[slot * 64 + lane] = lx1;
+            L.u.b.bc[slot * 64 + lane] = lc;
+            ++nb;
+        }
+        L.nb[lane] = min(nb, K2);
+        if (__ballot(nb > K2) != 0 && lane == 0) L.over = 1;
+    }
+    __syncthreads();
+    const bool fallback = L.over != 0;
+    __syncthreads();  // every wave has read the flag before the fallback reuses the LDS
+    if (fallback) return false;
+
+    // stable sort by x0: entry i of the row goes to its rank; each thread
+    // ranks the entries wv, wv + NW, ... against every entry of the row
+    {
+        // key = x0 * 256 + index: one 64-bit compare per pair; entries past
+        // the row's end get the largest key
+        constexpr int kPer = (K2 + NW - 1) / NW;
+        const int n = L.nb[lane];
+        long long key[kPer];
+        int rank[kPer];
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) {
+            const int i = wv + e * NW;
+            const int32_t x = L.u.b.b0[min(i, K2) * 64 + lane];
+            key[e] = i < n ? static_cast<long long>(x) * 256 + i : LLONG_MAX;
+            rank[e] = 0;
+        }
+        for (int k0 = 0; k0 < n; k0 += kB) {
+            long long kk[kB];
+#pragma unroll
+            for (int j = 0; j < kB; ++j) {
+                const int k = k0 + j;
+                const int32_t x = L.u.b.b0[min(k, K2) * 64 + lane];
+                kk[j] = k < n ? static_cast<long long>(x) * 256 + k : LLONG_MAX;
+            }
+#pragma unroll
+            for (int j = 0; j < kB; ++j) {
+#pragma unroll
+                for (int e = 0; e < kPer; ++e) rank[e] += kk[j] < key[e] ? 1 : 0;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) {
+            const int i = wv + e * NW;
+            if (i < n) {
+                L.p0[rank[e] * 64 + lane] = static_cast<int32_t>(key[e] >> 8);
+                L.p1[rank[e] * 64 + lane] = L.u.b.b1[i * 64 + lane];
+                L.pc[rank[e] * 64 + lane] = L.u.b.bc[i * 64 + lane];
+            }
+        }
+    }
+    __syncthreads();
+    unsigned long long w_draw = 0;
+    if (wv == 0) {
+        // render_aa_line's walk (scanner/mod.rs:393-467) over the sorted
+        // entries: an entry met outside opens a span at its x0, the entry that
+        // brings the winding back to outside closes it at its x1 + 1; spans
+        // clamped to [0, aa_width) and merged into disjoint intervals, written
+        // in place (an interval never lands past the entries already read)
+        const int n = L.nb[lane];
+        const int32_t aa_width = pg.width * 4;
+        int32_t count = 0, xx = 0, cs = 0, ce = -1;
+        bool open = false;
+        int nint = 0;
+        auto put = [&](int32_t bx0, int32_t bx1, bool go) {
+            // merge [bx0, bx1) into the current interval or start a new one
+            const bool any = go && bx0 < bx1;
+            const bool join = ce >= 0 && bx0 <= ce;
+            const bool emit = any && !join && ce >= 0;
+            const int slot = emit ? nint : P;
+            L.p0[slot * 64 + lane] = cs;
+            L.p1[slot * 64 + lane] = ce;
+            nint += emit ? 1 : 0;
+            cs = any && !join ? bx0 : cs;
+            ce = any ? (join ? max(ce, bx1) : bx1) : ce;
+        };
+        for (int k0 = 0; k0 < n; k0 += kB) {
+            int32_t x0[kB], x1[kB], cc[kB];
+#pragma unroll
+            for (int j = 0; j < kB; ++j) {
+                const int k = min(k0 + j, P);
+                x0[j] = L.p0[k * 64 + lane];
+                x1[j] = L.p1[k * 64 + lane];
+                cc[j] = L.pc[k * 64 + lane];
+            }
+#pragma unroll
+            for (int j = 0; j < kB; ++j) {
+                const bool v = k0 + j < n;
+                count = v ? static_cast<int32_t>(static_cast<uint32_t>(count) + static_cast<uint32_t>(cc[j])) : count;
+                const bool opener = v && !open;
+                const bool closer = v && open && (count & mrule) == 0;
+                xx = opener ? x0[j] : xx;
+                const int32_t end = x1[j] == INT32_MAX ? INT32_MAX : x1[j] + 1;
+                put(max(xx, 0), max(min(end, aa_width), 0), closer);
+                open = opener ? true : (closer ? false : open);
+            }
+        }
+        // a span the winding never closed runs to the row's end
+        const bool unb = open;
+        put(max(xx, 0), aa_width, unb);
+        if (ce >= 0) {
+            L.p0[nint * 64 + lane] = cs;
+            L.p1[nint * 64 + lane] = ce;
+            ++nint;
+        }
+        const bool any_unb = __ballot(unb) != 0;
+        L.nb[lane] = nint;
+        if (lane == 0) L.unb = any_unb ? 1 : 0;
+        if (A.cov_count) w_draw = band_draw_width(pg, Y0, ylo, yhi, nint, L.p0, L.p1);
+    }
+    __syncthreads();
+    composite_band<CH, true>(A, L.u.bm[wv], L.gamma, pg, p, Y0, h, ylo, yhi, at, L.nb[lane], L.p0, L.p1,
+                             L.unb != 0, wv, NW, w_draw);
+    return true;
+}
+
+template <int NW, int P, int K2, int CH, int FK>
+union MwUnion {
+    MwLds<NW, P, K2, CH> m;
+    BandLds<FK, CH> f;  // the one-wave fallback
+};
+
+// PLAIN: one launch per path, NW waves per band of 16 output rows.
+template <int NW, int P, int K2, int CH, int FK>
+__global__ void __launch_bounds__(64 * NW) k_fill_plain(FillArgs A, int32_t page, int64_t p, int32_t row0) {
+    __shared__ MwUnion<NW, P, K2, CH, FK> U;
+    const DevPage &pg = A.pages[page];
+    const int32_t Y0 = row0 + kBandRows * static_cast<int32_t>(blockIdx.x);
+    const long long t0 = A.trace ? static_cast<long long>(wall_clock64()) : 0;
+    if (!fill_band_mw<NW, P, K2, CH>(A, U.m, pg, p, Y0)) {
+        if (threadIdx.x >= 64) return;
+        if (threadIdx.x < 17) U.f.gamma[threadIdx.x] = kGamma[threadIdx.x];
+        wave_sync();
+        fill_band<FK, CH, true>(A, U.f, pg, p, Y0, blockIdx.x);
+    }
+    if (A.trace && threadIdx.x == 0) {
+        // wall-clock span of the first wave's share of the band
+        const long long t1 = static_cast<long long>(wall_clock64());
         const unsigned long long k = atomicAdd(A.trace_top, 1ull);
         if (static_cast<long long>(k) < A.trace_cap)
             A.trace[k] = make_longlong4(p, (static_cast<long long>(page) << 32) | static_cast<uint32_t>(Y0), t0, t1);
     }
 }
 
-template <int K, int CH>
-__device__ __forceinline__ void init_lds(BandLds<K, CH> &L) {
-    if (threadIdx.x < 17) L.gamma[threadIdx.x] = kGamma[threadIdx.x];
-    __syncthreads();
-}
-
-// PLAIN: one launch per path, one wave per band of 16 output rows.
-template <int K, int CH>
-__global__ void __launch_bounds__(64) k_fill_plain(FillArgs A, int32_t page, int64_t p, int32_t row0) {
-    __shared__ BandLds<K, CH> L;
-    init_lds(L);
-    fill_band_traced<K, CH>(A, L, A.pages[page], page, p, row0 + kBandRows * static_cast<int32_t>(blockIdx.x),
-                            blockIdx.x);
-}
-
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 
-// Plain-launch kernel of every instantiated fill shape (kVariantK/kVariantCH);
-// the plain composite sweep is at most 256 pixels (one dword per lane per row).
-constexpr int plain_ch(int v) { return kVariantCH[v] > 256 ? 256 : kVariantCH[v]; }
-#define RR_PLAIN(v) reinterpret_cast<const void *>(&k_fill_plain<kVariantK[v], plain_ch(v)>)
+// Plain-launch kernel of every instantiated fill shape: CH from kVariantCH
+// (the plain composite sweep is at most 128 pixels, which keeps the bitmaps
+// inside the LDS the intersection lists leave), K = kVariantK the LDS entry
+// slots of the one-wave fallback.
+constexpr int plain_ch(int v) { return kVariantCH[v] > 128 ? 128 : kVariantCH[v]; }
+#define RR_PLAIN(v) \
+    reinterpret_cast<const void *>(&k_fill_plain<kMwWaves, kMwPool, kMwK, plain_ch(v), kVariantK[v]>)
 const void *const kPlainKernels[kNumVariants] = {RR_PLAIN(0),  RR_PLAIN(1),  RR_PLAIN(2),  RR_PLAIN(3),
                                                  RR_PLAIN(4),  RR_PLAIN(5),  RR_PLAIN(6),  RR_PLAIN(7),
                                                  RR_PLAIN(8),  RR_PLAIN(9),  RR_PLAIN(10), RR_PLAIN(11),
@@ -814,7 +1153,8 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
                 if (c->profiling && !ev) return fail(c, RR_E_DEVICE, "profiling event allocation failed");
                 if (ev) RR_HIP_TRY(c, hipEventRecord(ev[0], c->stream));
                 void *args[] = {&A, const_cast<int32_t *>(&pi), &p, &r0};
-                RR_HIP_TRY(c, hipLaunchKernel(kPlainKernels[b.variant], dim3(bands), dim3(64), args, 0, c->stream));
+                RR_HIP_TRY(c, hipLaunchKernel(kPlainKernels[b.variant], dim3(bands), dim3(64 * kMwWaves), args, 0,
+                                              c->stream));
                 if (ev) RR_HIP_TRY(c, hipEventRecord(ev[1], c->stream));
                 ++c->fill_launches;
             }
