@@ -601,6 +601,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="mixed-64", choices=sorted(WORKLOADS))
     ap.add_argument("--pages", type=int, default=0, help="limit the page count (0 = all)")
+    ap.add_argument("--path-stride", type=int, default=1,
+                    help="keep every n-th path of each page (a smaller run for counter passes)")
     ap.add_argument("--mode", default="documented", choices=["documented", "literal"])
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU oracle work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -626,6 +628,8 @@ def main():
     mode = R.MODE_LITERAL if args.mode == "literal" else R.MODE_DOCUMENTED
     t_gen = time.perf_counter()
     pages = R.workload(cfg, seed, max_pages)
+    if args.path_stride > 1:
+        pages = [pg.subset(range(0, pg.n_paths, args.path_stride)) for pg in pages]
     t_gen = time.perf_counter() - t_gen
     print(f"bench: {args.workload}: {len(pages)} pages generated in {t_gen:.1f} s", file=sys.stderr,
           flush=True)
@@ -726,6 +730,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
         "data": "synthetic",
         "config": {"workload": args.workload, "description": desc, "pages": len(pages),
+                   **({"path_stride": args.path_stride} if args.path_stride > 1 else {}),
                    "paths": n_paths, "edges": n_edges, "aa_row_mode": args.mode,
                    "canvas": "gray8-exact",
                    "launch": "batched" if launch == R.LAUNCH_BATCHED else "plain",
