@@ -72,8 +72,13 @@ IMAGES_PER_PAGE = 4
 WORKLOADS["dispatch"] = (5, 0xC0F10005, 0, None,
                          "learned dispatch over a glyph-page stream (6000 glyphs per Letter@300 page); "
                          "MLP trained on span-pass traces of 8 pages, evaluated on 32 held-out pages, "
-                         "a quarter of them also carrying 4 CMYK/RGB images each")
+                         "a quarter of them also carrying 4 CMYK/RGB images each; the stream re-renders "
+                         "the 32 held-out pages (images included) in recordings of 32 pages")
+WORKLOADS["dispatch-mixed"] = (3, 0xC0F10003, 0, None,
+                               "learned dispatch on the mixed 64-page batch: MLP trained on span-pass traces "
+                               "and CPU times of 8 other mixed pages (seed + 7), evaluated on the 64 pages")
 DISPATCH_TRAIN_PAGES, DISPATCH_EVAL_PAGES = 8, 32
+DISPATCH_STREAM_PAGES = 10000
 WORKLOADS["ingest-glyphs"] = (6, 0xC0F10005, 8, None,
                               "device path ingestion (XPath::new + flatten_curve, flatness 0.1) of 8 glyph "
                               "pages: 48,000 outlines of 1-3 contours of 12-40 quadratic/cubic segments; "
@@ -398,7 +403,10 @@ def traced_costs(be, db, route, variant, reps=2):
     its work items scaled by (kernel time / total wave time), i.e. its share of
     the span kernel's wall time, in microseconds (min over reps)."""
     n = db.n_paths
-    cap = 8 * n + 4096
+    db.set_routes(R.ROUTE_SPAN)
+    db.run(launch=R.LAUNCH_BATCHED, variant=variant)
+    _, st = be.last_batch_stats()
+    cap = max(8 * n, int(st["items"])) + 4096  # at most one record per span item
     buf = be.alloc(cap * 32)
     best = np.full(n, np.inf)
     db.set_routes(route)
@@ -421,9 +429,10 @@ def traced_costs(be, db, route, variant, reps=2):
     return best
 
 
-def cpu_path_costs(pages, threads):
+def cpu_path_costs(pages, threads, mode=R.MODE_DOCUMENTED):
     """Reference CPU raster (the C oracle) time of every path, one page per
-    thread, divided by the threads used: each path's share of the CPU pool."""
+    thread, each path timed inside the C loop (rro_fill_page_timed), divided
+    by the threads used: each path's share of the CPU pool, in microseconds."""
     sys.path.insert(0, ROOT)
     from oracle import oracle as O  # the oracle, timed here as the CPU reference
 
@@ -434,12 +443,14 @@ def cpu_path_costs(pages, threads):
             pg = pages[pi]
             canvas = np.full((pg.height, pg.width), 255, np.uint8)
             a = pg.attrs
-            ts = np.zeros(pg.n_paths)
-            for p in range(pg.n_paths):
-                t0 = time.perf_counter()
-                O.fill(canvas, pg.path_edges(p), int(a["eo"][p]), int(a["color"][p]), int(a["a_input"][p]))
-                ts[p] = time.perf_counter() - t0
-            out[pi] = ts
+            eo = np.ascontiguousarray(a["eo"])
+            col = np.ascontiguousarray(a["color"])
+            ain = np.ascontiguousarray(a["a_input"])
+            ns = np.zeros(max(pg.n_paths, 1), np.int64)
+            O.lib().rro_fill_page_timed(O._p(canvas), pg.width, pg.height, pg.width, O._p(pg.edges),
+                                        O._p(pg.edge_off), pg.n_paths, O._p(eo), O._p(col), O._p(ain), mode,
+                                        O._p(ns))
+            out[pi] = ns[:pg.n_paths] / 1e3
 
     t0 = time.perf_counter()
     ths = [threading.Thread(target=worker, args=(t,)) for t in range(threads)]
@@ -448,34 +459,45 @@ def cpu_path_costs(pages, threads):
     for th in ths:
         th.join()
     wall = time.perf_counter() - t0
-    return np.concatenate(out) * 1e6 / threads, wall
+    return np.concatenate(out) / threads, wall
 
 
 def bench_dispatch(args, desc):
     """Learned dispatcher: per-path classes from an FP32 MLP trained on the
-    bench's own span-pass timing traces, against the static area rule."""
-    _, seed, _, _, _ = WORKLOADS[args.workload]
+    bench's own span-pass timing traces and per-path CPU times, against the
+    static area rule.  dispatch: glyph pages (+ images) and a 10,000-page
+    stream; dispatch-mixed: the mixed 64-page batch."""
+    cfg, seed, _, _, _ = WORKLOADS[args.workload]
+    mixed = cfg == 3
     t_gen = time.perf_counter()
-    train_pages = R.workload(5, seed, DISPATCH_TRAIN_PAGES)
-    eval_pages = R.workload(5, seed + 1, DISPATCH_EVAL_PAGES)
-    img_pages = list(range(0, DISPATCH_EVAL_PAGES, 4))  # a quarter of the pages
-    iset = R.images(0xC0F10004, IMAGES_PER_PAGE * len(img_pages), IMAGES_PER_PAGE)
-    print(f"bench: dispatch: {len(train_pages)} train + {len(eval_pages)} eval pages, {len(iset.images)} images, "
-          f"generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+    if mixed:
+        train_pages = R.workload(3, seed + 7, DISPATCH_TRAIN_PAGES)
+        eval_pages = R.workload(3, seed, args.pages if args.pages else 0)
+        img_pages = []
+    else:
+        train_pages = R.workload(5, seed, DISPATCH_TRAIN_PAGES)
+        eval_pages = R.workload(5, seed + 1, DISPATCH_EVAL_PAGES)
+        img_pages = list(range(0, DISPATCH_EVAL_PAGES, 4))  # a quarter of the pages
+    iset = R.images(0xC0F10004, IMAGES_PER_PAGE * len(img_pages), IMAGES_PER_PAGE) if img_pages else None
+    print(f"bench: {args.workload}: {len(train_pages)} train + {len(eval_pages)} eval pages, "
+          f"{len(iset.images) if iset else 0} images, generated in {time.perf_counter() - t_gen:.1f} s",
+          file=sys.stderr, flush=True)
     be = R.Backend(0)
     dtr, dev = R.DeviceBatch(be, train_pages), R.DeviceBatch(be, eval_pages)
-    di = R.DeviceImages(be, iset, n_pages=0)
-    # images of eval page k*4 go onto that page's canvas
-    img_descs = (R.ImageDesc * len(iset.images))()
-    for i, im in enumerate(iset.images):
-        img_descs[i] = di.descs[i]
-        img_descs[i].page = img_pages[im.page]
-    ib = R.ImageBatch()
-    ib.images = ctypes.addressof(img_descs)
-    ib.n_images = len(iset.images)
-    ib.pages = ctypes.cast(dev.descs, ctypes.c_void_p)
-    ib.n_pages = len(eval_pages)
-    ib.grid4, ib.grid4_n, ib.grid3, ib.grid3_n = di.grid4, iset.grid4_n, di.grid3, iset.grid3_n
+    di, ib = None, None
+    if iset:
+        di = R.DeviceImages(be, iset, n_pages=0)
+        # images of eval page k*4 go onto that page's canvas
+        img_descs = (R.ImageDesc * len(iset.images))()
+        for i, im in enumerate(iset.images):
+            img_descs[i] = di.descs[i]
+            img_descs[i].page = img_pages[im.page]
+        ib = R.ImageBatch()
+        ib.images = ctypes.addressof(img_descs)
+        ib.n_images = len(iset.images)
+        ib.pages = ctypes.cast(dev.descs, ctypes.c_void_p)
+        ib.n_pages = len(eval_pages)
+        ib.grid4, ib.grid4_n, ib.grid3, ib.grid3_n = di.grid4, iset.grid4_n, di.grid3, iset.grid3_n
 
     dev.set_routes(R.ROUTE_SPAN)
     variant = args.variant if args.variant >= 0 else dev.autotune(reps=1)[0]
@@ -486,18 +508,21 @@ def bench_dispatch(args, desc):
         return be.last_dispatch(db.n_paths)[0]
 
     def cost_table(db):
-        cols = [traced_costs(be, db, r, variant) for r in GPU_ROUTES]
+        cols = [traced_costs(be, db, r, variant, reps=3) for r in GPU_ROUTES]
         return np.stack(cols + [np.full(db.n_paths, np.nan)], 1)
 
     f_tr, f_ev = features(dtr), features(dev)
     c_tr, c_ev = cost_table(dtr), cost_table(dev)
+    print(f"bench: span-pass traces done", file=sys.stderr, flush=True)
     cpu_note = "cpu class not measured (--no-cpu-baseline)"
     cpu_wall = None
-    threads, _ = cpu_threads()
+    threads, host = cpu_threads()
     if not args.no_cpu_baseline:
         c_tr[:, 4], _ = cpu_path_costs(train_pages, threads)
         c_ev[:, 4], cpu_wall = cpu_path_costs(eval_pages, threads)
-        cpu_note = f"cpu class = reference CPU raster time per path / {threads} threads"
+        cpu_note = (f"cpu class = reference CPU raster time per path (C oracle, timed per path inside the C loop) "
+                    f"/ {threads} threads")
+        print(f"bench: CPU path times done ({cpu_wall:.1f} s for the eval pages)", file=sys.stderr, flush=True)
     v_tr, v_ev = route_valid(f_tr[:, 5]), route_valid(f_ev[:, 5])
     c_tr[~v_tr] = np.nan
     c_ev[~v_ev] = np.nan
@@ -515,7 +540,8 @@ def bench_dispatch(args, desc):
         dev.reset(255)
         be.begin_page()
         be.record_fill_pages(dev.fill_pages(launch=R.LAUNCH_BATCHED, variant=variant))
-        be.record_place_images(ib)
+        if ib is not None:
+            be.record_place_images(ib)
         be.wait_page(be.submit_page())
 
     results = {}
@@ -542,6 +568,9 @@ def bench_dispatch(args, desc):
         results[name]["regret_us"] = round(results[name]["cost_us"] - best_total, 3)
         results[name]["cost_us"] = round(results[name]["cost_us"], 3)
     parity = "match" if len(set(fingerprints.values())) == 1 else "MISMATCH"
+    golden = load_golden("mixed-64:documented:canvas") if mixed and not args.pages else None
+    if golden is not None and fingerprints["learned"] != golden:
+        parity = "MISMATCH"
     be.set_dispatcher(params)
     dev.set_routes(R.ROUTE_AUTO)
     dev.reset(255)
@@ -550,6 +579,22 @@ def bench_dispatch(args, desc):
     be.wait_page(be.submit_page())
     w_ref = int(dev.read_counts()[:, 1].sum())
     learned_sec = results["learned"]["ms_per_step"] / 1e3
+    # the stream: the held-out pages re-rendered, recording after recording,
+    # with the learned dispatch (images included), timed as a whole
+    stream = {}
+    n_stream = args.stream_pages if args.stream_pages >= 0 else (0 if mixed else DISPATCH_STREAM_PAGES)
+    if n_stream:
+        batches = -(-n_stream // len(eval_pages))
+        step(True, True)
+        t0 = time.perf_counter()
+        for _ in range(batches):
+            step(True, True)
+        st_sec = time.perf_counter() - t0
+        stream = {"pages": batches * len(eval_pages), "recordings": batches, "seconds": round(st_sec, 3),
+                  "pages_per_s": round(batches * len(eval_pages) / st_sec, 3),
+                  "note": f"{batches} recordings of the {len(eval_pages)} held-out pages (the stream is drawn "
+                          f"from them), learned dispatch in-kernel, images on every fourth page"}
+        print(f"bench: stream of {stream['pages']} pages in {st_sec:.2f} s", file=sys.stderr, flush=True)
     be.set_profiling(True)
     step(True, True)
     ms, st = be.last_batch_stats()
@@ -557,22 +602,35 @@ def bench_dispatch(args, desc):
     ch = R.variants()[variant][1]
     kernels = batched_kernel_bytes(ms, st, ch, sum(p.n_paths for p in eval_pages))
     kern = max(("k_spans", "k_compose"), key=lambda k: kernels[k]["ms"])
+    plain = {}
+    if mixed and not args.no_plain:
+        dev.set_routes(R.ROUTE_SPAN)
+        dev.reset(255)
+        t0 = time.perf_counter()
+        be.begin_page()
+        be.record_fill_pages(dev.fill_pages(launch=R.LAUNCH_PLAIN, variant=variant))
+        be.wait_page(be.submit_page())
+        p_sec = time.perf_counter() - t0
+        plain = {"plain_ms_per_step": round(p_sec * 1e3, 3), "plain_pages_per_s": round(len(eval_pages) / p_sec, 3),
+                 "learned_over_plain": round(p_sec / learned_sec, 3)}
     out = {
         "metric": METRIC, "value": round(w_ref / learned_sec / 1e6, 3), "unit": "Mpx/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": results["learned"]["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8+f32",
         "data": "synthetic",
         "config": {"workload": args.workload, "description": desc, "train_pages": len(train_pages),
-                   "eval_pages": len(eval_pages), "image_pages": len(img_pages), "images": len(iset.images),
+                   "eval_pages": len(eval_pages), "image_pages": len(img_pages),
+                   "images": len(iset.images) if iset else 0,
                    "eval_paths": dev.n_paths, "variant": {"id": variant, "lds_entries": R.variants()[variant][0],
                                                           "chunk_px": ch}},
         "pages_per_s": results["learned"]["pages_per_s"],
         "dispatch": {"learned": results["learned"], "static": results["static"], "span_only": results["span"],
                      "best_cost_us": round(best_total, 3),
                      "cost_definition": "span-pass share of kernel wall time per path (traced wave time x "
-                                        "kernel time / total wave time); " + cpu_note,
+                                        "kernel time / total wave time, min of 3 runs); " + cpu_note,
                      "classes_order": ["span", "tile16", "tile32", "tile64", "cpu"],
                      "train_loss": round(loss, 5), "train_s": round(t_train, 3)},
+        **({"stream": stream} if stream else {}), **plain,
         "parity": parity, "fingerprint": fingerprints["learned"][:16],
         "roofline": {"bound": "hbm", "achieved": kernels[kern]["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": kernels[kern]["frac"], "traffic": None, "kernel": kern,
@@ -585,11 +643,12 @@ def bench_dispatch(args, desc):
         out["cpu_baseline"] = {"value": round(len(eval_pages) / cpu_wall, 3), "unit": "pages/s", "cores": threads,
                                "kind": "port",
                                "sample": f"all {n_ev} paths of the {len(eval_pages)} eval pages filled by the C "
-                                         f"oracle, one page per thread, {cpu_wall:.1f} s (no images)"}
+                                         f"oracle, one page per thread, {cpu_wall:.1f} s (no images)", **host}
     be.set_dispatcher(None)
     dtr.free()
     dev.free()
-    di.free()
+    if di is not None:
+        di.free()
     be.close()
     print(json.dumps(out), flush=True)
 
@@ -608,6 +667,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-path-per-launch comparison")
     ap.add_argument("--variant", type=int, default=-1, help="band-fill variant (-1 = autotune)")
+    ap.add_argument("--stream-pages", type=int, default=-1,
+                    help="dispatch workload: pages streamed through the learned dispatch (-1 = 10000 for the "
+                         "glyph stream, 0 for the mixed batch)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with a 1 GiB device-to-device copy (FETCH/WRITE counter calibration)")
     args = ap.parse_args()
@@ -619,7 +681,7 @@ def main():
     cfg, seed, max_pages, launch, desc = WORKLOADS[args.workload]
     if cfg == 4:
         return bench_images(args, desc)
-    if cfg == 5:
+    if cfg == 5 or args.workload == "dispatch-mixed":
         return bench_dispatch(args, desc)
     if cfg == 6:
         return bench_ingest(args, desc)

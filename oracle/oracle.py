@@ -115,6 +115,7 @@ def load(path):
                            ctypes.c_int, P, P, P]),
         "rro_fill_page": (i64, [P, i32, i32, i32, P, P, i64, P, P, P, ctypes.c_int,
                                 ctypes.c_int, P]),
+        "rro_fill_page_timed": (i64, [P, i32, i32, i32, P, P, i64, P, P, P, ctypes.c_int, P]),
         "rro_cov_hash_term": (ctypes.c_uint64, [i64, u8]),
         "rro_compute_a_src": (u32, [u32, u32, u32, ctypes.c_int, ctypes.c_int]),
         "rro_composite_general": (u8, [u8, u8, u8, u8, u8, ctypes.c_int, ctypes.c_int]),

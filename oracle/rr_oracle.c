@@ -25,6 +25,9 @@
  *   here (stable order is used).  rro_scanner_stats reports how many rows are
  *   exposed to that ("parity unpinned" for rows > 32 entries with x0 ties).
  */
+#define _POSIX_C_SOURCE 199309L  /* clock_gettime (rro_fill_page_timed) */
+#include <time.h>
+
 #include "rr_oracle.h"
 
 /* RRO_MUTANT (default 0) plants one known mistake for the test-suite's
@@ -1088,6 +1091,25 @@ int64_t rro_fill_page(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, con
         int64_t off = edge_off[p], n = edge_off[p + 1] - edge_off[p];
         total += rro_fill(canvas, w, h, stride, edges + 4 * off, n, eo[p], color[p], a_input[p],
                           mode, flavour, clip, NULL, cov_hashes ? &cov_hashes[p] : NULL);
+    }
+    return total;
+}
+
+/* rro_fill_page with each path's wall time (CLOCK_MONOTONIC, ns) in ns[p]:
+ * the per-path CPU cost the dispatcher bench prices the CPU class with. */
+int64_t rro_fill_page_timed(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, const double *edges,
+                            const int64_t *edge_off, int64_t n_paths, const uint8_t *eo, const uint8_t *color,
+                            const uint8_t *a_input, int mode, int64_t *ns) {
+    double clip[4] = {0.0, 0.0, (double)w - 0.001, (double)h - 0.001};
+    int64_t total = 0;
+    for (int64_t p = 0; p < n_paths; ++p) {
+        struct timespec t0, t1;
+        clock_gettime(CLOCK_MONOTONIC, &t0);
+        int64_t off = edge_off[p], n = edge_off[p + 1] - edge_off[p];
+        total += rro_fill(canvas, w, h, stride, edges + 4 * off, n, eo[p], color[p], a_input[p], mode,
+                          RRO_FLAVOUR_GRAY8, clip, NULL, NULL);
+        clock_gettime(CLOCK_MONOTONIC, &t1);
+        ns[p] = (int64_t)(t1.tv_sec - t0.tv_sec) * 1000000000 + (t1.tv_nsec - t0.tv_nsec);
     }
     return total;
 }

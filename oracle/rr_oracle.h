@@ -155,6 +155,11 @@ int64_t rro_fill_page(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, con
                       const uint8_t *color, const uint8_t *a_input, int mode, int flavour,
                       uint64_t *cov_hashes);
 
+/* rro_fill_page (Gray8, no hashes) with each path's wall time in ns[p]. */
+int64_t rro_fill_page_timed(uint8_t *canvas, int32_t w, int32_t h, int32_t stride, const double *edges,
+                            const int64_t *edge_off, int64_t n_paths, const uint8_t *eo, const uint8_t *color,
+                            const uint8_t *a_input, int mode, int64_t *ns);
+
 /* The reference dispatcher's bbox area (gpu_ops.rs:52-84), -1 = stays on the
  * CPU; pinned by hand-computed cases (tests/test_oracle_general.py). */
 int64_t rro_ref_bbox_area(const double *edges, int64_t n, int32_t w, int32_t h);
