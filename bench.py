@@ -20,6 +20,10 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "pdf-raster_amd"))
+# The plain route spreads independent paths over 16 HIP streams; HIP maps
+# streams onto GPU_MAX_HW_QUEUES hardware queues (its default is 4), so the
+# bench asks for 16 before the runtime starts (read once, at initialisation).
+os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 import numpy as np  # noqa: E402
 

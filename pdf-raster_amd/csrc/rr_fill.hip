@@ -884,7 +884,8 @@ union MwUnion {
 
 // PLAIN: one launch per path, NW waves per band of 16 output rows.
 template <int NW, int P, int K2, int CH, int FK>
-__global__ void __launch_bounds__(64 * NW) k_fill_plain(FillArgs A, int32_t page, int64_t p, int32_t row0) {
+__global__ void __launch_bounds__(64 * NW) k_fill_plain(FillArgs A, int32_t page, int64_t p, int32_t row0,
+                                                        int32_t slot0) {
     __shared__ MwUnion<NW, P, K2, CH, FK> U;
     const DevPage &pg = A.pages[page];
     const int32_t Y0 = row0 + kBandRows * static_cast<int32_t>(blockIdx.x);
@@ -893,7 +894,7 @@ __global__ void __launch_bounds__(64 * NW) k_fill_plain(FillArgs A, int32_t page
         if (threadIdx.x >= 64) return;
         if (threadIdx.x < 17) U.f.gamma[threadIdx.x] = kGamma[threadIdx.x];
         wave_sync();
-        fill_band<FK, CH, true>(A, U.f, pg, p, Y0, blockIdx.x);
+        fill_band<FK, CH, true>(A, U.f, pg, p, Y0, slot0 + static_cast<int64_t>(blockIdx.x));
     }
     if (A.trace && threadIdx.x == 0) {
         // wall-clock span of the first wave's share of the band
@@ -1135,35 +1136,92 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     RR_HIP_TRY(c, hipGetLastError());
 
     if (b.launch == RR_LAUNCH_PLAIN) {
+        constexpr int NS = rr_hip_ctx::kPlainStreams, CELL = rr_hip_ctx::kPlainCellRows;
         int32_t max_bands = 1;
         for (int32_t i = 0; i < b.n_pages; ++i)
             max_bands = std::max(max_bands, (b.pages[i].height + kBandRows) / kBandRows + 1);
-        rc = rrint::ensure(c, &c->slots, &c->slots_cap, static_cast<size_t>(max_bands) * kSlotInts);
+        // one block of overflow slots per side stream (concurrent launches
+        // never share a slot)
+        rc = rrint::ensure(c, &c->slots, &c->slots_cap, static_cast<size_t>(NS) * max_bands * kSlotInts);
         if (rc) return rc;
+        if (!c->side_start) {
+            for (int s = 0; s < NS; ++s) {
+                RR_HIP_TRY(c, hipStreamCreateWithFlags(&c->side[s], hipStreamNonBlocking));
+                RR_HIP_TRY(c, hipEventCreateWithFlags(&c->side_ev[s], hipEventDisableTiming));
+            }
+            RR_HIP_TRY(c, hipEventCreateWithFlags(&c->side_start, hipEventDisableTiming));
+        }
         A.slots = c->slots;
+        // the side streams start after the work recorded on the context stream
+        RR_HIP_TRY(c, hipEventRecord(c->side_start, c->stream));
+        uint32_t started = 0;
+        auto start = [&](int s) -> int32_t {
+            if (!((started >> s) & 1u)) {
+                RR_HIP_TRY(c, hipStreamWaitEvent(c->side[s], c->side_start, 0));
+                started |= 1u << s;
+            }
+            return RR_OK;
+        };
+        auto join_all = [&](hipStream_t into) -> int32_t {
+            for (int s = 0; s < NS; ++s) {
+                if (!((started >> s) & 1u)) continue;
+                RR_HIP_TRY(c, hipEventRecord(c->side_ev[s], c->side[s]));
+                RR_HIP_TRY(c, hipStreamWaitEvent(into, c->side_ev[s], 0));
+            }
+            return RR_OK;
+        };
+        int next = 0;
         for (int32_t pi = 0; pi < b.n_pages; ++pi) {
             const rr_page_desc &pg = b.pages[pi];
+            c->cell_stream.assign(static_cast<size_t>((pg.height + CELL - 1) / CELL), static_cast<int8_t>(-1));
             for (int64_t p = pg.path_begin; p < pg.path_end; ++p) {
                 int32_t r0 = b.host_rows[2 * p], r1 = b.host_rows[2 * p + 1];
                 r0 = std::max(r0, 0);
                 r1 = std::min(r1, pg.height - 1);
                 if (r0 > r1) continue;
+                // side streams that drew a row cell of this path last
+                uint32_t dep = 0;
+                for (int32_t q = r0 / CELL; q <= r1 / CELL; ++q)
+                    if (c->cell_stream[q] >= 0) dep |= 1u << c->cell_stream[q];
+                int s;
+                if (dep == 0) {
+                    s = next;
+                    next = (next + 1) % NS;
+                } else {
+                    s = __builtin_ctz(dep);
+                }
+                rc = start(s);
+                if (rc) return rc;
+                for (uint32_t o = dep & ~(1u << s); o; o &= o - 1) {
+                    const int t = __builtin_ctz(o);
+                    RR_HIP_TRY(c, hipEventRecord(c->side_ev[t], c->side[t]));
+                    RR_HIP_TRY(c, hipStreamWaitEvent(c->side[s], c->side_ev[t], 0));
+                }
                 const unsigned bands = static_cast<unsigned>((r1 - r0) / kBandRows + 1);
+                const int32_t slot0 = s * max_bands;
                 hipEvent_t *ev = c->profiling ? prof_pair(c) : nullptr;
                 if (c->profiling && !ev) return fail(c, RR_E_DEVICE, "profiling event allocation failed");
-                if (ev) RR_HIP_TRY(c, hipEventRecord(ev[0], c->stream));
-                void *args[] = {&A, const_cast<int32_t *>(&pi), &p, &r0};
+                if (ev) RR_HIP_TRY(c, hipEventRecord(ev[0], c->side[s]));
+                void *args[] = {&A, const_cast<int32_t *>(&pi), &p, &r0, const_cast<int32_t *>(&slot0)};
                 RR_HIP_TRY(c, hipLaunchKernel(kPlainKernels[b.variant], dim3(bands), dim3(64 * kMwWaves), args, 0,
-                                              c->stream));
-                if (ev) RR_HIP_TRY(c, hipEventRecord(ev[1], c->stream));
+                                              c->side[s]));
+                if (ev) RR_HIP_TRY(c, hipEventRecord(ev[1], c->side[s]));
                 ++c->fill_launches;
+                for (int32_t q = r0 / CELL; q <= r1 / CELL; ++q) c->cell_stream[q] = static_cast<int8_t>(s);
             }
             if (pg.gray) {
                 // the page's Gray8 output once its last path is composited
-                rc = rrint::rgb_to_gray(c, pg.canvas, pg.stride, pg.gray, pg.gray_stride, pg.width, pg.height);
+                rc = join_all(c->stream);
+                if (!rc) rc = rrint::rgb_to_gray(c, pg.canvas, pg.stride, pg.gray, pg.gray_stride, pg.width, pg.height);
                 if (rc) return rc;
+                RR_HIP_TRY(c, hipEventRecord(c->side_start, c->stream));
+                started = 0;
             }
         }
+        // the context stream (the page fence, later recordings) waits for
+        // every side stream
+        rc = join_all(c->stream);
+        if (rc) return rc;
         RR_HIP_TRY(c, hipGetLastError());
         return RR_OK;
     }
@@ -1230,6 +1288,8 @@ int32_t rr_hip_destroy(rr_hip_ctx *c) {
     if (!c) return RR_E_INVARIANT;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int k = 0; k < rr_hip_ctx::kPlainStreams; ++k)
+        if (c->side[k]) (void)hipStreamSynchronize(c->side[k]);
     void *bufs[] = {c->recs,      c->hdrs,     c->rows,       c->sel,     c->nb,        c->csum,
                     c->pages,     c->slots,    c->spill,      c->spill_top, c->err_flag, c->trace_top,
                     c->item_base, c->scan_tmp, c->sitems,     c->shdr,    c->spans,     c->band_cnt,
@@ -1248,6 +1308,11 @@ int32_t rr_hip_destroy(rr_hip_ctx *c) {
     for (auto &f : c->fences)
         if (f.ev) (void)hipEventDestroy(f.ev);
     for (auto e : c->prof_ev) (void)hipEventDestroy(e);
+    for (int k = 0; k < rr_hip_ctx::kPlainStreams; ++k) {
+        if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
+        if (c->side_ev[k]) (void)hipEventDestroy(c->side_ev[k]);
+    }
+    if (c->side_start) (void)hipEventDestroy(c->side_start);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return RR_OK;

@@ -228,6 +228,15 @@ struct rr_hip_ctx {
     // page state
     hipEvent_t t0 = nullptr, t1 = nullptr;
     bool page_open = false, page_timed = false;
+    // Plain launches go to kPlainStreams side streams (created on first use):
+    // a path whose drawn rows meet those of an earlier path of its page runs
+    // after it (same stream, or an event wait); other paths run concurrently.
+    static constexpr int kPlainStreams = 16;
+    static constexpr int kPlainCellRows = 4;  // row granularity of the overlap test
+    hipStream_t side[kPlainStreams] = {};
+    hipEvent_t side_ev[kPlainStreams] = {};  // dependency and join points of each side stream
+    hipEvent_t side_start = nullptr;         // the context stream's work before the launches
+    std::vector<int8_t> cell_stream;         // per row cell of the page: last side stream drawing it
     double last_ms = 0.0;
     int64_t last_launches = 0;
     rrdev::Fence fences[rrdev::kFenceSlots];
