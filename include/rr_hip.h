@@ -46,7 +46,8 @@ extern "C" {
 #define RR_FLAVOUR_RGB8_FAST 1
 #define RR_FLAVOUR_RGB8 2
 /* Execution shape of record_fill_paths. */
-#define RR_LAUNCH_PLAIN 0   /* one path per kernel launch, in order */
+#define RR_LAUNCH_PLAIN 0   /* one path per kernel launch; launches of paths whose drawn rows \
+                               meet run in path order, the others may run concurrently */
 #define RR_LAUNCH_BATCHED 1 /* one persistent launch for all paths of the batch */
 
 /* Per-path fill attributes (device array).  Mirrors the arguments of
