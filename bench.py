@@ -22,8 +22,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "pdf-raster_amd"))
 # The plain route spreads independent paths over 16 HIP streams; HIP maps
 # streams onto GPU_MAX_HW_QUEUES hardware queues (its default is 4), so the
-# bench asks for 16 before the runtime starts (read once, at initialisation).
-os.environ["GPU_MAX_HW_QUEUES"] = "16"
+# bench asks for 16 (--hw-queues) before the runtime starts (it reads the
+# variable once, at initialisation).
+_hwq = [a.split("=", 1)[1] for a in sys.argv if a.startswith("--hw-queues=")]
+os.environ["GPU_MAX_HW_QUEUES"] = _hwq[0] if _hwq else "16"
 
 import numpy as np  # noqa: E402
 
@@ -48,8 +50,14 @@ def batched_kernel_bytes(ms, st, ch, n_paths):
     # entry and a header per composited path-tile
     b_bytes = (16 * ch * (2 * st["tiles"] - st["filled_tiles"]) + 8 * st["intervals_read"]
                + (16 + 144) * st["path_tiles"])
+    # the bands the main span pass deferred (rows with more entries than its
+    # LDS slots) are built by the wide pass: their share of the span bytes,
+    # in proportion to the bands, goes with the wide pass's time
+    wide_share = st["deferred"] / built if built > 0 else 0.0
     kernels = {}
-    for name, t_ms, nbytes in (("k_spans", ms["span_ms"], a_bytes), ("k_compose", ms["compose_ms"], b_bytes)):
+    for name, t_ms, nbytes in (("k_spans", ms["span_ms"], a_bytes * (1.0 - wide_share)),
+                               ("k_spans_wide", ms["wide_ms"], a_bytes * wide_share),
+                               ("k_compose", ms["compose_ms"], b_bytes)):
         gbs = nbytes / (t_ms / 1e3) / 1e9 if t_ms > 0 else 0.0
         kernels[name] = {"ms": round(t_ms, 3), "alg_bytes": int(nbytes), "achieved_gbs": round(gbs, 3),
                          "frac": round(gbs / HBM_PEAK_GBS, 6)}
@@ -671,6 +679,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-path-per-launch comparison")
     ap.add_argument("--variant", type=int, default=-1, help="band-fill variant (-1 = autotune)")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES for this process (pass as --hw-queues=N)")
     ap.add_argument("--stream-pages", type=int, default=-1,
                     help="dispatch workload: pages streamed through the learned dispatch (-1 = 10000 for the "
                          "glyph stream, 0 for the mixed batch)")
@@ -766,6 +776,16 @@ def main():
             sec = graph_sec
     value = w_ref / sec / 1e6
     pages_per_s = len(pages) / sec
+    # the whole step against the compulsory bytes: every canvas pixel read and
+    # written once, every edge's four f64 endpoints read once (one recording
+    # of the batch; "effective" counts the pixels with coverage instead)
+    canvas_px = sum(pg.width * pg.height for pg in pages)
+    comp = 2 * canvas_px + 32 * n_edges
+    eff = 2 * w_cov + 32 * n_edges
+    pipeline = {"compulsory_bytes": comp, "achieved_gbs": round(comp / sec / 1e9, 3),
+                "frac": round(comp / sec / 1e9 / HBM_PEAK_GBS, 6), "effective_bytes": eff,
+                "effective_gbs": round(eff / sec / 1e9, 3),
+                "basis": "2 B per canvas pixel + 32 B per edge, over ms_per_step"}
 
     # roofline: every fill launch bracketed by its own HIP event pair
     be.set_profiling(True)
@@ -816,6 +836,7 @@ def main():
                   ("no-fixture" if "no-fixture" in (parity_cov, parity_timed) else "MISMATCH"),
         "parity_detail": {"coverage_run": parity_cov, "timed_run_canvases": parity_timed},
         "fingerprint": fingerprint[:16], "timed_fingerprint": timed_fp[:16],
+        "pipeline": pipeline,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
                      "traffic": measured_traffic(args.workload, variant, kernel),
@@ -858,6 +879,10 @@ def main():
                       f"in order by the C oracle, one page per thread over {threads} threads, "
                       f"{cdt:.1f} s",
             "pages_per_s": round(w_done / cdt / (w_ref / len(pages)), 3), **host}
+        # the same on one thread (a quarter of the budget)
+        w1, p1, dt1 = cpu_baseline(pages, w_per_path, args.cpu_budget / 4, 1, mode)
+        out["cpu_baseline"]["single_thread"] = {"value": round(w1 / dt1 / 1e6, 3), "unit": "Mpx/s",
+                                                "sample": f"{p1} paths of page 0, {dt1:.1f} s"}
     print(json.dumps(out), flush=True)
 
 

@@ -428,8 +428,9 @@ int32_t rr_hip_train_dispatcher(rr_hip_ctx *ctx, const float *features, const fl
                                 float *loss_out);
 
 /* Statistics of the last batched recording (no reference counterpart; read
- * after wait_page).  ms: {span pass, composite pass} kernel milliseconds when
- * per-launch profiling was on for that page, else 0.  counts: {(path, band)
+ * after wait_page).  ms: {main span pass (k_spans), the wide span passes
+ * (deferred bands, then the widened paths' skipped bands), composite pass}
+ * kernel milliseconds when per-launch profiling was on for that page, else 0.  counts: {(path, band)
  * span items, intervals stored, (page, band, tile) composite items,
  * (path, tile) composites, tiles holding at least one path, span-pass reruns
  * after growing the interval pool or the deferred queue, (path, band) items
@@ -437,7 +438,7 @@ int32_t rr_hip_train_dispatcher(rr_hip_ctx *ctx, const float *features, const fl
  * paths with at least one band built, the edge records of those paths,
  * composite tiles started from an opaque cover's colour (the canvas tile is
  * not read), intervals the composite pass read}. */
-int32_t rr_hip_last_batch_stats(rr_hip_ctx *ctx, double ms[2], int64_t counts[12]);
+int32_t rr_hip_last_batch_stats(rr_hip_ctx *ctx, double ms[3], int64_t counts[12]);
 /* Diagnostics of the last batched recording (no reference counterpart):
  * copies up to cap elements of one internal table to host memory and sets
  * *count to its length.  table 0: per composite tile item, the highest opaque

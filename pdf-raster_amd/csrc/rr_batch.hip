@@ -2026,21 +2026,26 @@ extern "C" int32_t rr_hip_debug_batch_table(rr_hip_ctx *c, int32_t table, void *
     return RR_OK;
 }
 
-extern "C" int32_t rr_hip_last_batch_stats(rr_hip_ctx *c, double ms[2], int64_t counts[12]) {
+extern "C" int32_t rr_hip_last_batch_stats(rr_hip_ctx *c, double ms[3], int64_t counts[12]) {
     if (!c || !ms || !counts) return RR_E_INVARIANT;
     if (!c->batch_valid) return rrint::fail(c, RR_E_INVARIANT, "no batched recording since the context was created");
     RR_TRY(c, hipSetDevice(c->device));
     RR_TRY(c, hipStreamSynchronize(c->stream));
     unsigned long long k[kCounters] = {};
     RR_TRY(c, hipMemcpy(k, c->counters, sizeof(k), hipMemcpyDeviceToHost));
-    ms[0] = ms[1] = 0.0;
+    ms[0] = ms[1] = ms[2] = 0.0;
     if (c->profiling && c->prof_b + 2 <= c->prof_used) {
+        // the recording's span launches: the last attempt's main pass, then
+        // its wide passes
+        const size_t main = c->prof_a1 >= c->prof_a0 + 6 ? c->prof_a1 - 6 : c->prof_a0;
         for (size_t i = c->prof_a0; i + 1 < c->prof_a1; i += 2) {
             float m = 0.f;
-            if (hipEventElapsedTime(&m, c->prof_ev[i], c->prof_ev[i + 1]) == hipSuccess) ms[0] += m;
+            if (hipEventElapsedTime(&m, c->prof_ev[i], c->prof_ev[i + 1]) != hipSuccess) continue;
+            if (i == main) ms[0] = m;
+            else if (i > main) ms[1] += m;
         }
         float m = 0.f;
-        if (hipEventElapsedTime(&m, c->prof_ev[c->prof_b], c->prof_ev[c->prof_b + 1]) == hipSuccess) ms[1] = m;
+        if (hipEventElapsedTime(&m, c->prof_ev[c->prof_b], c->prof_ev[c->prof_b + 1]) == hipSuccess) ms[2] = m;
     }
     counts[0] = c->batch_items;
     counts[1] = static_cast<int64_t>(k[kCntStored]);

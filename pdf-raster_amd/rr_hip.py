@@ -466,15 +466,16 @@ class Backend:
                     "record_apply_soft_mask")
 
     def last_batch_stats(self):
-        """({span_ms, compose_ms}, {items, intervals, composite_items, path_tiles,
+        """({span_ms, wide_ms, compose_ms}, {items, intervals, composite_items, path_tiles,
         tiles, reruns, skipped, deferred, built_paths, built_edges, filled_tiles,
         intervals_read}) of the last batched recording."""
-        ms = np.zeros(2, np.float64)
+        ms = np.zeros(3, np.float64)
         cnt = np.zeros(12, np.int64)
         self._check(self.L.rr_hip_last_batch_stats(self.h, _p(ms), _p(cnt)), "last_batch_stats")
         keys = ("items", "intervals", "composite_items", "path_tiles", "tiles", "reruns", "skipped", "deferred",
                 "built_paths", "built_edges", "filled_tiles", "intervals_read")
-        return {"span_ms": float(ms[0]), "compose_ms": float(ms[1])}, dict(zip(keys, (int(v) for v in cnt)))
+        return ({"span_ms": float(ms[0]), "wide_ms": float(ms[1]), "compose_ms": float(ms[2])},
+                dict(zip(keys, (int(v) for v in cnt))))
 
     def debug_batch_table(self, table):
         """One internal table of the last batched recording (see rr_hip.h)."""
