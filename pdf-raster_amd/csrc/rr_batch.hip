@@ -1453,22 +1453,31 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8)))
     const int lane = threadIdx.x;
     unsigned long long n_paths = 0, n_tiles = 0, n_filled = 0;
     uint32_t n_ivals = 0;  // intervals this lane read
+    // One item per grab: larger grabs unbalance the tail (heavy tiles come
+    // in runs).
     SliceCursor sc{static_cast<int32_t>(blockIdx.x % kSlices), 0};
     long long it = 0, it_end = 0;
+    // the page of the last item: consecutive items are mostly on one page
+    CPage cp{0, 0, 0, 0};
+    long long cp_lo = 1, cp_hi = 0;
     for (;;) {
         if (it >= it_end) {
             it = slice_grab(B.queues + kSlices * kSliceStride, n_items, 1, sc, &it_end);
             if (it >= n_items) break;
         }
-        int32_t lo = 0, hi = B.n_cpages - 1;
-        while (lo < hi) {
-            const int32_t mid = (lo + hi + 1) / 2;
-            if (static_cast<long long>(B.cpages[mid].base) <= it)
-                lo = mid;
-            else
-                hi = mid - 1;
+        if (it < cp_lo || it >= cp_hi) {
+            int32_t lo = 0, hi = B.n_cpages - 1;
+            while (lo < hi) {
+                const int32_t mid = (lo + hi + 1) / 2;
+                if (static_cast<long long>(B.cpages[mid].base) <= it)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            cp = B.cpages[lo];
+            cp_lo = cp.base;
+            cp_hi = lo + 1 < B.n_cpages ? static_cast<long long>(B.cpages[lo + 1].base) : n_items;
         }
-        const CPage cp = B.cpages[lo];
         const int32_t local = static_cast<int32_t>(it - cp.base);
         const int32_t band = local / cp.tiles, tile = local - band * cp.tiles;
         compose_tile<CH, COV, NPL>(B, L, it, cp.page, band, tile, &n_paths, &n_tiles, &n_filled, &n_ivals);

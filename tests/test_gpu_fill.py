@@ -209,3 +209,22 @@ def test_clip_rectangles(backend, clip, launch):
     finally:
         dp.free()
     assert_same(got, (want, np.array(hashes, np.uint64)), f"clip {clip} launch {launch}")
+
+
+@pytest.mark.parametrize("mode", [R.MODE_DOCUMENTED, R.MODE_LITERAL])
+def test_plain_launch_order_across_streams(backend, mode):
+    # Twenty strips in disjoint rows (each starts on its own stream), then
+    # half-transparent paths that meet several strips' rows (waits across
+    # streams), then strips again over them, then a path meeting everything:
+    # the per-pixel composite order is the path order, as on the oracle.
+    paths = []
+    for k in range(20):
+        paths.append((rect(2 + k, 4 * k + 0.3, 40 + k, 4 * k + 3.1), k & 1, (37 * k) & 255, 255))
+    for k in range(5):
+        paths.append((rect(5 + 3 * k, 10 * k + 0.5, 60 - k, 10 * k + 27.5), 0, (91 * k + 13) & 255, 128))
+    for k in range(20):
+        paths.append((rect(20 - k * 0.5, 4 * k + 1.2, 50 + k * 0.25, 4 * k + 2.9), 1, (53 * k + 7) & 255, 200))
+    paths.append((donut, 0, 30, 100))
+    paths.append((rect(0.25, 0.25, 63.5, 79.5), 0, 250, 64))
+    page = hand_page(paths, width=64, height=80)
+    assert_same(gpu_page(backend, page, mode), oracle_page(page, mode), "strips across streams")
