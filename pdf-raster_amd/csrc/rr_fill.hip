@@ -98,30 +98,24 @@ __device__ __forceinline__ int32_t ref_bbox_area(double x0, double y0, double x1
     return a >= 2147483647.0 ? INT32_MAX : static_cast<int32_t>(a);
 }
 
-// One wave per path (four per block): add_segment + aa_scale of every edge
-// into its EdgeRec, the scanner bbox, the chunk summaries (min first row, max
-// last row of every global 64-edge chunk: a plain store for a chunk inside the
-// path, atomics for one it shares with a neighbour; csum starts at
-// {INT32_MAX, INT32_MIN}), and the dispatcher's features.
-__global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ edges,
-                                                    const int64_t *__restrict__ edge_off, int64_t n_paths,
-                                                    const DevPage *__restrict__ pages, int32_t n_pages,
-                                                    EdgeRec *__restrict__ recs, PathHdr *__restrict__ hdrs,
-                                                    int2 *__restrict__ rows, int4 *__restrict__ sel,
-                                                    int32_t *__restrict__ nb, const rr_path_attr *__restrict__ attrs,
-                                                    float *__restrict__ feat, int2 *__restrict__ csum, int32_t mode,
-                                                    int32_t *__restrict__ err) {
-    const int64_t p = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+// Persistent waves (four per block) over the paths: for each path,
+// add_segment + aa_scale of every edge into its EdgeRec, the scanner bbox, the
+// chunk summaries (min first row, max last row of every global 64-edge chunk:
+// a plain store for a chunk inside the path, atomics for one it shares with a
+// neighbour; csum starts at {INT32_MAX, INT32_MIN}), and the dispatcher's
+// features.  Long-lived waves keep the chip full where one short wave per path
+// would leave it waiting on wave launches, and the page of a path is found by
+// a forward scan of the pages' first paths kept in LDS.
+constexpr int kSetupPagesLds = 1024;  // pages whose first path index the block keeps in LDS
+__device__ __forceinline__ void edge_setup_path(const double *__restrict__ edges,
+                                                const int64_t *__restrict__ edge_off, const int64_t p,
+                                                const DevPage *__restrict__ pages, const int32_t lo,
+                                                EdgeRec *__restrict__ recs, PathHdr *__restrict__ hdrs,
+                                                int2 *__restrict__ rows, int4 *__restrict__ sel,
+                                                int32_t *__restrict__ nb, const rr_path_attr *__restrict__ attrs,
+                                                float *__restrict__ feat, int2 *__restrict__ csum, int32_t mode,
+                                                int32_t *__restrict__ err) {
     const int lane = threadIdx.x & 63;
-    if (p >= n_paths) return;
-    int32_t lo = 0, hi = n_pages - 1;  // page of path p (pages sorted by path_begin)
-    while (lo < hi) {
-        const int32_t mid = (lo + hi + 1) / 2;
-        if (pages[mid].path_begin <= p)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
     const DevPage &pg = pages[lo];
     const ClipRect clip = pg.clip;
     const int64_t e0 = edge_off[p], e1 = edge_off[p + 1];
@@ -133,11 +127,19 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
     // (gpu_fill_segs, crates/pdf_interp/src/renderer/page/gpu_ops.rs:52-84):
     // f64 min / max, which skip NaN like Rust's f64::min / max
     double rx0 = INFINITY, ry0 = INFINITY, rx1 = -INFINITY, ry1 = -INFINITY;
+    // the next chunk's edges are loaded while this one is converted
+    auto load_edge = [&](int64_t i) {
+        double4 q = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (i >= e0 && i < e1) q = reinterpret_cast<const double4 *>(edges)[i];
+        return q;
+    };
+    double4 qn = load_edge((e0 & ~63ll) + lane);
     for (int64_t base = e0 & ~63ll; base < e1; base += 64) {
         const int64_t i = base + lane;
         int32_t cr0 = INT32_MAX, cr1 = INT32_MIN;
+        const double4 q = qn;
+        if (base + 64 < e1) qn = load_edge(base + 64 + lane);
         if (i >= e0 && i < e1) {
-            const double4 q = reinterpret_cast<const double4 *>(edges)[i];
             double x0 = q.x, y0 = q.y, x1 = q.z, y1 = q.w;
             rx0 = fmin(fmin(rx0, x0), x1);
             ry0 = fmin(fmin(ry0, y0), y1);
@@ -301,6 +303,40 @@ __global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ e
             f[10] = static_cast<float>(at.a_input) / 255.f;
             f[11] = log2f(1.f + area / fmaxf(1.f, static_cast<float>(pg.width) * static_cast<float>(pg.height)));
         }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_edge_setup(const double *__restrict__ edges,
+                                                    const int64_t *__restrict__ edge_off, int64_t n_paths,
+                                                    const DevPage *__restrict__ pages, int32_t n_pages,
+                                                    EdgeRec *__restrict__ recs, PathHdr *__restrict__ hdrs,
+                                                    int2 *__restrict__ rows, int4 *__restrict__ sel,
+                                                    int32_t *__restrict__ nb, const rr_path_attr *__restrict__ attrs,
+                                                    float *__restrict__ feat, int2 *__restrict__ csum, int32_t mode,
+                                                    int32_t *__restrict__ err) {
+    __shared__ int64_t pb[kSetupPagesLds];
+    const bool in_lds = n_pages <= kSetupPagesLds;
+    if (in_lds)
+        for (int32_t i = threadIdx.x; i < n_pages; i += blockDim.x) pb[i] = pages[i].path_begin;
+    __syncthreads();
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * 4;
+    const int32_t wave = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(threadIdx.x >> 6));
+    int32_t lo = 0;  // page of the current path (pages tile the paths in order)
+    for (int64_t p = static_cast<int64_t>(blockIdx.x) * 4 + wave; p < n_paths; p += stride) {
+        if (in_lds) {
+            while (lo + 1 < n_pages && pb[lo + 1] <= p) ++lo;
+        } else {
+            int32_t hi = n_pages - 1;
+            while (lo < hi) {
+                const int32_t mid = (lo + hi + 1) / 2;
+                if (pages[mid].path_begin <= p)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+        }
+        lo = __builtin_amdgcn_readfirstlane(lo);
+        edge_setup_path(edges, edge_off, p, pages, lo, recs, hdrs, rows, sel, nb, attrs, feat, csum, mode, err);
     }
 }
 
@@ -1130,7 +1166,9 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     if (n_chunks > 0) rc = rrint::fill_u64(c, reinterpret_cast<unsigned long long *>(c->csum), n_chunks,
                                             0x800000007fffffffull);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_edge_setup, dim3(static_cast<unsigned>((b.n_paths + 3) / 4)), dim3(256), 0, c->stream,
+    const unsigned setup_blocks =
+        static_cast<unsigned>(std::min<long long>((b.n_paths + 3) / 4, static_cast<long long>(c->n_cu) * c->setup_bpc));
+    hipLaunchKernelGGL(k_edge_setup, dim3(setup_blocks), dim3(256), 0, c->stream,
                        b.edges, b.edge_off, b.n_paths, c->pages, b.n_pages, c->recs, c->hdrs, c->rows, c->sel, c->nb,
                        b.attrs, b.launch == RR_LAUNCH_BATCHED ? c->feat : nullptr, c->csum, b.mode, c->err_flag);
     RR_HIP_TRY(c, hipGetLastError());
@@ -1263,6 +1301,11 @@ int32_t rr_hip_create(int32_t device, rr_hip_ctx **out) {
     int32_t rc = RR_OK;
     if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) rc = RR_E_DEVICE;
     if (rc == RR_OK) rc = rrint::init_batched(c);
+    if (rc == RR_OK &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&c->setup_bpc, reinterpret_cast<const void *>(&k_edge_setup), 256,
+                                                     0) != hipSuccess)
+        rc = RR_E_DEVICE;
+    c->setup_bpc = std::max(1, c->setup_bpc);
     if (rc == RR_OK) rc = rrint::init_static_scratch(c);
     if (rc == RR_OK && (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess)) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->spill_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;

@@ -104,6 +104,7 @@ struct CPage {
 struct rr_hip_ctx {
     int device = 0;
     int n_cu = 0;
+    int setup_bpc = 1;  // resident k_edge_setup blocks per CU
     hipStream_t stream = nullptr;
     std::string err;
     // workspace shared by both launch shapes
