@@ -614,7 +614,7 @@ __device__ __forceinline__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, 
 // composite sweep's CH-pixel steps go to the waves in turn.  A row with more
 // intersections or entries than the LDS lists hold sends the band, untouched,
 // to the one-wave fill_band (first wave, global entry slots).
-constexpr int kMwWaves = 8;   // waves per band
+constexpr int kMwWaves = 16;  // waves per band
 constexpr int kMwPool = 128;  // intersections per AA row of the band in LDS
 constexpr int kMwK = 40;      // entries per AA row after the merge
 constexpr int kMwBlock = 8;   // entries read into registers at a time
@@ -626,17 +626,18 @@ struct MwLds {
         int4 recA[NW][64], recB[NW][64];   // each wave's chunk as two 16-byte halves per edge
         unsigned long long smask[NW][64];  // per wave, per lane row: chunk edges crossing it
         int32_t cnt[2][NW][64];            // crossings of (wave's chunk, row), by pass parity
-        // the merged entries [k * 64 + lane]; row K2 takes the stores the
-        // branch-free loops discard
-        int32_t b0[(K2 + 1) * 64], b1[(K2 + 1) * 64], bc[(K2 + 1) * 64];
+    };
+    struct Sorted {
+        int32_t s0[K2 * 64], s1[K2 * 64], sc[K2 * 64];  // the merged entries sorted by x0 [k * 64 + lane]
     };
     union {
         Build b;
+        Sorted s;
         uint32_t bm[NW][64 * kPitch];  // each wave's coverage bitmap (composite sweep)
     } u;
-    // intersections of each row in edge order [k * 64 + lane]; then the
-    // sorted entries; then the intervals
-    // (row P takes the stores of intersections beyond the lists)
+    // intersections of each row in edge order [k * 64 + lane]; then, in place,
+    // the merged entries; then the intervals.  Row P takes the stores the
+    // branch-free loops discard.
     int32_t p0[(P + 1) * 64], p1[(P + 1) * 64], pc[(P + 1) * 64];
     int32_t nb[64];
     uint8_t gamma[32];
@@ -647,7 +648,7 @@ struct MwLds {
 template <int NW, int P, int K2, int CH>
 __device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2, CH> &L, const DevPage &pg,
                                              int64_t p, int32_t Y0) {
-    static_assert(P >= K2, "the sorted entries reuse the intersection lists");
+    static_assert(P >= K2, "the merged entries and intervals live in the intersection lists");
     const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
     const PathHdr h = A.hdrs[p];
@@ -781,10 +782,12 @@ __device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2,
                 const bool touch = __builtin_elementwise_add_sat(lx1, 1) >= x0[j] &&
                                    lx0 <= __builtin_elementwise_add_sat(x1[j], 1);
                 const bool st = v && !touch && (lc & mrule) != 0;
-                const int slot = st ? min(nb, K2) : K2;
-                L.u.b.b0[slot * 64 + lane] = lx0;
-                L.u.b.b1[slot * 64 + lane] = lx1;
-                L.u.b.bc[slot * 64 + lane] = lc;
+                // in place: at most k entries have retired by intersection k,
+                // whose block is already in registers
+                const int slot = st ? nb : P;
+                L.p0[slot * 64 + lane] = lx0;
+                L.p1[slot * 64 + lane] = lx1;
+                L.pc[slot * 64 + lane] = lc;
                 nb += st ? 1 : 0;
                 const int32_t sum = static_cast<int32_t>(static_cast<uint32_t>(lc) + static_cast<uint32_t>(cc[j]));
                 lc = v ? (touch ? sum : cc[j]) : lc;
@@ -793,10 +796,9 @@ __device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2,
             }
         }
         if ((lc & mrule) != 0) {
-            const int slot = min(nb, K2);
-            L.u.b.b0[slot * 64 + lane] = lx0;
-            L.u.b.b1[slot * 64 + lane] = lx1;
-            L.u.b.bc[slot * 64 + lane] = lc;
+            L.p0[nb * 64 + lane] = lx0;
+            L.p1[nb * 64 + lane] = lx1;
+            L.pc[nb * 64 + lane] = lc;
             ++nb;
         }
         L.nb[lane] = min(nb, K2);
@@ -819,7 +821,7 @@ __device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2,
 #pragma unroll
         for (int e = 0; e < kPer; ++e) {
             const int i = wv + e * NW;
-            const int32_t x = L.u.b.b0[min(i, K2) * 64 + lane];
+            const int32_t x = L.p0[min(i, P) * 64 + lane];
             key[e] = i < n ? static_cast<long long>(x) * 256 + i : LLONG_MAX;
             rank[e] = 0;
         }
@@ -828,7 +830,7 @@ __device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2,
 #pragma unroll
             for (int j = 0; j < kB; ++j) {
                 const int k = k0 + j;
-                const int32_t x = L.u.b.b0[min(k, K2) * 64 + lane];
+                const int32_t x = L.p0[min(k, P) * 64 + lane];
                 kk[j] = k < n ? static_cast<long long>(x) * 256 + k : LLONG_MAX;
             }
 #pragma unroll
@@ -841,9 +843,9 @@ __device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2,
         for (int e = 0; e < kPer; ++e) {
             const int i = wv + e * NW;
             if (i < n) {
-                L.p0[rank[e] * 64 + lane] = static_cast<int32_t>(key[e] >> 8);
-                L.p1[rank[e] * 64 + lane] = L.u.b.b1[i * 64 + lane];
-                L.pc[rank[e] * 64 + lane] = L.u.b.bc[i * 64 + lane];
+                L.u.s.s0[rank[e] * 64 + lane] = static_cast<int32_t>(key[e] >> 8);
+                L.u.s.s1[rank[e] * 64 + lane] = L.p1[i * 64 + lane];
+                L.u.s.sc[rank[e] * 64 + lane] = L.pc[i * 64 + lane];
             }
         }
     }
@@ -854,7 +856,7 @@ __device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2,
         // entries: an entry met outside opens a span at its x0, the entry that
         // brings the winding back to outside closes it at its x1 + 1; spans
         // clamped to [0, aa_width) and merged into disjoint intervals, written
-        // in place (an interval never lands past the entries already read)
+        // over the (read) merged entries in the row lists
         const int n = L.nb[lane];
         const int32_t aa_width = pg.width * 4;
         int32_t count = 0, xx = 0, cs = 0, ce = -1;
@@ -876,10 +878,10 @@ __device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2,
             int32_t x0[kB], x1[kB], cc[kB];
 #pragma unroll
             for (int j = 0; j < kB; ++j) {
-                const int k = min(k0 + j, P);
-                x0[j] = L.p0[k * 64 + lane];
-                x1[j] = L.p1[k * 64 + lane];
-                cc[j] = L.pc[k * 64 + lane];
+                const int k = min(k0 + j, K2 - 1);
+                x0[j] = L.u.s.s0[k * 64 + lane];
+                x1[j] = L.u.s.s1[k * 64 + lane];
+                cc[j] = L.u.s.sc[k * 64 + lane];
             }
 #pragma unroll
             for (int j = 0; j < kB; ++j) {
@@ -946,10 +948,10 @@ __global__ void __launch_bounds__(64 * NW) k_fill_plain(FillArgs A, int32_t page
 // ---------------------------------------------------------------------------
 
 // Plain-launch kernel of every instantiated fill shape: CH from kVariantCH
-// (the plain composite sweep is at most 128 pixels, which keeps the bitmaps
-// inside the LDS the intersection lists leave), K = kVariantK the LDS entry
-// slots of the one-wave fallback.
-constexpr int plain_ch(int v) { return kVariantCH[v] > 128 ? 128 : kVariantCH[v]; }
+// (the plain composite sweep is at most 64 pixels a step, which keeps the
+// sixteen waves' bitmaps inside the LDS the intersection lists leave), K =
+// kVariantK the LDS entry slots of the one-wave fallback.
+constexpr int plain_ch(int v) { return kVariantCH[v] > 64 ? 64 : kVariantCH[v]; }
 #define RR_PLAIN(v) \
     reinterpret_cast<const void *>(&k_fill_plain<kMwWaves, kMwPool, kMwK, plain_ch(v), kVariantK[v]>)
 const void *const kPlainKernels[kNumVariants] = {RR_PLAIN(0),  RR_PLAIN(1),  RR_PLAIN(2),  RR_PLAIN(3),
