@@ -20,12 +20,13 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "pdf-raster_amd"))
-# The plain route spreads independent paths over 16 HIP streams; HIP maps
+# The plain route spreads independent paths over 20 HIP streams; HIP maps
 # streams onto GPU_MAX_HW_QUEUES hardware queues (its default is 4), so the
-# bench asks for 16 (--hw-queues) before the runtime starts (it reads the
-# variable once, at initialisation).
+# bench asks for 24 (--hw-queues) before the runtime starts (it reads the
+# variable once, at initialisation): a queue for each of those streams, the
+# context stream and HIP's own.
 _hwq = [a.split("=", 1)[1] for a in sys.argv if a.startswith("--hw-queues=")]
-os.environ["GPU_MAX_HW_QUEUES"] = _hwq[0] if _hwq else "16"
+os.environ["GPU_MAX_HW_QUEUES"] = _hwq[0] if _hwq else "24"
 
 import numpy as np  # noqa: E402
 
@@ -679,7 +680,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-path-per-launch comparison")
     ap.add_argument("--variant", type=int, default=-1, help="band-fill variant (-1 = autotune)")
-    ap.add_argument("--hw-queues", type=int, default=16,
+    ap.add_argument("--hw-queues", type=int, default=24,
                     help="GPU_MAX_HW_QUEUES for this process (pass as --hw-queues=N)")
     ap.add_argument("--stream-pages", type=int, default=-1,
                     help="dispatch workload: pages streamed through the learned dispatch (-1 = 10000 for the "

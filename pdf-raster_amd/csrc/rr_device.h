@@ -457,7 +457,8 @@ __device__ __forceinline__ void lane_range(int32_t a, int32_t b, bool literal, i
 // X0/X1/C are LDS or global pointers (the two are separate instantiations).
 __device__ __forceinline__ int2 chunk_summary_first(const FillArgs &A, int64_t e0, int64_t e1) {
     int2 sm = make_int2(INT32_MAX, INT32_MIN);
-    if (e1 > e0 && (e0 >> 6) + threadIdx.x <= ((e1 - 1) >> 6)) sm = A.csum[(e0 >> 6) + threadIdx.x];
+    const int lane = threadIdx.x & 63;
+    if (e1 > e0 && (e0 >> 6) + lane <= ((e1 - 1) >> 6)) sm = A.csum[(e0 >> 6) + lane];
     return sm;
 }
 
@@ -482,7 +483,7 @@ template <bool WS = false, typename Lds, typename Ptr>
 __device__ __forceinline__ int build_entries(const FillArgs &A, Lds &L, int64_t e0, int64_t e1,
                                              int32_t band_lo, int32_t band_hi, bool literal, int32_t Y0, bool valid,
                                              int32_t r, int32_t mrule, Ptr X0, Ptr X1, Ptr C, int cap, int2 sm0) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     int n = 0;
     // the row's open entry; the initial one (count 0, empty range) merges
     // into nothing it could change and retires as a no-op
@@ -582,7 +583,7 @@ __device__ __forceinline__ int walk_sorted(int n, Ptr X0, Ptr X1, Ptr C, int32_t
 template <typename Ptr>
 __device__ __forceinline__ int sort_and_walk(int n, Ptr X0, Ptr X1, Ptr C, int32_t mrule, int32_t aa_width,
                                              bool *unbalanced) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     for (int a = 1; a < n; ++a) {
         const int32_t vx0 = X0[a * 64 + lane], vx1 = X1[a * 64 + lane], vc = C[a * 64 + lane];
         int b = a;

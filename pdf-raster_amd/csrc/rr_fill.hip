@@ -536,13 +536,13 @@ __device__ __forceinline__ void composite_band(const FillArgs &A, uint32_t *bm, 
 }
 
 // Rasterises path p over output rows [Y0, Y0+15] of page `pg` and composites
-// into its canvas.  Called by one full wave64, lanes threadIdx.x 0..63: the
-// whole workgroup, or with WS the first wave of a larger one working alone;
-// `slot` is this wave's private overflow slot.
+// into its canvas.  Called by one full wave64 (lane = threadIdx.x & 63): the
+// whole workgroup, or with WS one wave of a larger one working alone; `slot`
+// is this wave's private overflow slot.
 template <int K, int CH, bool WS>
 __device__ __forceinline__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, const DevPage &pg, int64_t p, int32_t Y0,
                           int64_t slot) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     // independent loads first, so their latencies overlap
     const PathHdr h = A.hdrs[p];
     const int2 drawn = A.rows[p];
@@ -599,25 +599,36 @@ __device__ __forceinline__ void fill_band(const FillArgs &A, BandLds<K, CH> &L, 
 // ---------------------------------------------------------------------------
 // PLAIN: several waves per band
 // ---------------------------------------------------------------------------
-// A one-path launch holds only that path's bands (two or three workgroups for
-// a glyph), so a band's time is the latency of its serial chain.  Here NW
-// waves share the band.  Pass by pass, wave w takes the w-th of the next NW
-// edge chunks (64 edges each) that overlap the band, marks per AA row (one
-// lane per row) the chunk edges crossing it, and writes each crossing's
-// intersection (fill_intersections, scanner/mod.rs:528-585) into the row's
-// LDS list at its place in edge order: the waves' crossing counts are summed
-// in wave order ahead of the writes, so the list comes out in the order the
-// reference walks the edges.  The first wave then folds each row's list with
-// write_intersect's merge (scanner/mod.rs:600-642) -- now a short loop over
-// LDS instead of a loop over the path's edges.  The stable sort by x0 is a
-// rank count split over the waves, the walk is render_aa_line's, and the
-// composite sweep's CH-pixel steps go to the waves in turn.  A row with more
-// intersections or entries than the LDS lists hold sends the band, untouched,
-// to the one-wave fill_band (first wave, global entry slots).
-constexpr int kMwWaves = 16;  // waves per band
-constexpr int kMwPool = 128;  // intersections per AA row of the band in LDS
+// A one-path launch holds only that path's bands, so a band's time is the
+// latency of its serial chain.  Here a group of NW waves shares the band.
+// Pass by pass, wave w of the group takes the w-th of the next NW edge chunks
+// (64 edges each) that overlap the band, marks per AA row (one lane per row)
+// the chunk edges crossing it, and writes each crossing's intersection
+// (fill_intersections, scanner/mod.rs:528-585) into the row's LDS list at its
+// place in edge order: the waves' crossing counts are summed in wave order
+// ahead of the writes, so the list comes out in the order the reference walks
+// the edges.  The group's first wave then folds each row's list with
+// write_intersect's merge (scanner/mod.rs:600-642) -- a short loop over LDS
+// instead of a loop over the path's edges.  The stable sort by x0 is a rank
+// count split over the group's waves, the walk is render_aa_line's, and the
+// composite sweep's CH-pixel steps go to the waves in turn.
+//
+// A workgroup holds one group (one band; the launch has a workgroup per band)
+// or G groups (the path's G bands in one workgroup: the launch is a single
+// workgroup, which the device dispatches at a higher rate than several).  The
+// groups of a workgroup pass the same barriers: every group runs as many chunk
+// passes as the busiest one.  Intersections are kept packed in 4 bytes, x0 and
+// x1 as 15-bit offsets from the path's leftmost AA column plus the 2-bit count,
+// so that three bands' lists fit one workgroup's LDS.  A row with more
+// intersections or entries than the LDS lists hold, or with an x beyond the
+// 15-bit range, sends the band, untouched, to the one-wave fill_band (the
+// group's first wave, global entry slots) once the groups' shared barriers are
+// behind.
+constexpr int kMwWaves = 16;  // waves per band, one band per workgroup
+constexpr int kMwPool = 128;  // intersections per AA row of a band in LDS
 constexpr int kMwK = 40;      // entries per AA row after the merge
 constexpr int kMwBlock = 8;   // entries read into registers at a time
+constexpr uint32_t kMwSpan = 32767u;  // largest packed x offset (AA columns)
 
 template <int NW, int P, int K2, int CH>
 struct MwLds {
@@ -627,59 +638,90 @@ struct MwLds {
         unsigned long long smask[NW][64];  // per wave, per lane row: chunk edges crossing it
         int32_t cnt[2][NW][64];            // crossings of (wave's chunk, row), by pass parity
     };
-    struct Sorted {
-        int32_t s0[K2 * 64], s1[K2 * 64], sc[K2 * 64];  // the merged entries sorted by x0 [k * 64 + lane]
+    struct Merged {
+        // write_intersect's entries [k * 64 + lane]: x0 | x1 << 16 as offsets
+        // from the path's leftmost column, and the count; row K2 takes the
+        // stores the branch-free loop discards
+        uint32_t mx[(K2 + 1) * 64];
+        int32_t mc[(K2 + 1) * 64];
     };
     union {
         Build b;
-        Sorted s;
+        Merged m;
         uint32_t bm[NW][64 * kPitch];  // each wave's coverage bitmap (composite sweep)
     } u;
-    // intersections of each row in edge order [k * 64 + lane]; then, in place,
-    // the merged entries; then the intervals.  Row P takes the stores the
-    // branch-free loops discard.
-    int32_t p0[(P + 1) * 64], p1[(P + 1) * 64], pc[(P + 1) * 64];
+    union {
+        // the intersections of each row in edge order [k * 64 + lane],
+        // (x0 - xb) | (x1 - xb) << 15 | (count + 1) << 30; row P takes the
+        // stores the branch-free loop discards
+        uint32_t pk[(P + 1) * 64];
+        // then the merged entries sorted by x0 (sx packed like mx, sc the
+        // count), and in place over them the row's disjoint intervals
+        // (absolute AA columns, starts in sx, ends in sc); row K2 = discards
+        struct {
+            int32_t sx[(K2 + 1) * 64], sc[(K2 + 1) * 64];
+        } s;
+    } q;
     int32_t nb[64];
     uint8_t gamma[32];
     int32_t over, unb;
 };
 
-// Returns false, with nothing drawn, when a row outgrew the LDS lists.
+// The band at rows Y0.. of path p, filled and composited by the NW waves of
+// group `grp` (of `ng` in the workgroup; npass: 2 x ng ints of LDS shared by
+// the groups).  Returns false, with nothing drawn, when a row outgrew the LDS
+// lists; every wave of the workgroup passes the same barriers either way.
 template <int NW, int P, int K2, int CH>
-__device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2, CH> &L, const DevPage &pg,
-                                             int64_t p, int32_t Y0) {
-    static_assert(P >= K2, "the merged entries and intervals live in the intersection lists");
-    const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+__device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2, CH> &L, int32_t *npass, int ng,
+                                             int grp, const DevPage &pg, int64_t p, int32_t Y0) {
+    static_assert(2 * (K2 + 1) <= P + 1, "the sorted entries live in the intersection lists");
+    static_assert(K2 < 256, "sort keys hold the entry index in 8 bits");
+    const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6)) - grp * NW;
     const int lane = threadIdx.x & 63;
+    const int lt = wv * 64 + lane;  // thread index within the group
     const PathHdr h = A.hdrs[p];
     const int2 drawn = A.rows[p];
     const rr_path_attr at = A.attrs[p];
     const int64_t e0 = A.edge_off[p], e1 = A.edge_off[p + 1];
-    if (h.flags & (HDR_EMPTY | HDR_OUTSIDE)) return true;
+    if (h.flags & (HDR_EMPTY | HDR_OUTSIDE)) return true;  // the same for every group
     const int32_t ylo = drawn.x, yhi = drawn.y;
-    if (Y0 > yhi || Y0 + kBandRows - 1 < ylo) return true;
+    // a band outside the drawn rows has nothing to do but the barriers
+    const bool active = !(Y0 > yhi || Y0 + kBandRows - 1 < ylo);
     const bool literal = A.mode == RR_MODE_LITERAL;
     const int32_t y = Y0 + (lane >> 2);
     const int32_t r = literal ? 16 * y + 12 + (lane & 3) : 4 * y + (lane & 3);
     const int32_t band_lo = literal ? 16 * Y0 + 12 : 4 * Y0;
     const int32_t band_hi = literal ? 16 * (Y0 + kBandRows - 1) + 15 : 4 * Y0 + 4 * kBandRows - 1;
-    const bool valid = y >= ylo && y <= yhi && r >= h.y_min && r <= h.y_max;
+    const bool valid = active && y >= ylo && y <= yhi && r >= h.y_min && r <= h.y_max;
     const int32_t mrule = (at.eo & RR_FILL_EVEN_ODD) ? 1 : -1;
-    if (threadIdx.x < 17) L.gamma[threadIdx.x] = kGamma[threadIdx.x];
-    if (threadIdx.x == 0) L.over = 0;
+    // every intersection lies in [h.x_min, h.x_max]; x is kept as x - xb
+    const int32_t xb = h.x_min;
+    if (lt < 17) L.gamma[lt] = kGamma[lt];
+    if (lt == 0) L.over = 0;
     __syncthreads();
 
     int top = 0;  // intersections of this lane's row so far (every wave keeps the same count)
     int par = 0;
+    bool wide = false;  // an x beyond the packed range
     if (e1 > e0) {
         const int64_t c_first = e0 >> 6, c_last = (e1 - 1) >> 6;
+        int np_par = 0;
         for (int64_t cb = c_first; cb <= c_last; cb += 64) {
             bool hit = false;
-            if (cb + lane <= c_last) {
+            if (active && cb + lane <= c_last) {
                 const int2 sm = A.csum[cb + lane];
                 hit = sm.x <= band_hi && sm.y >= band_lo;
             }
-            uint64_t cm = __ballot(hit);  // the same in every wave
+            uint64_t cm = __ballot(hit);  // the same in every wave of the group
+            int passes = (__popcll(cm) + NW - 1) / NW;
+            if (ng > 1) {
+                // as many passes as the busiest group (double-buffered: a
+                // group writes a slot again only after the next barrier)
+                if (wv == 0 && lane == 0) npass[np_par * ng + grp] = passes;
+                __syncthreads();
+                for (int g = 0; g < ng; ++g) passes = max(passes, npass[np_par * ng + g]);
+                np_par ^= 1;
+            }
             // this wave's chunk of a pass: the wv-th set bit of the mask left
             auto pick = [&](uint64_t m) {
                 for (int i = 0; i < wv; ++i) m &= m - 1;
@@ -694,7 +736,7 @@ __device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2,
             uint64_t mine = pick(cm);
             EdgeRec qn{1, 0, 0, 0, 0, 0, 0, 0};
             if (mine) qn = load_rec(mine);
-            while (cm) {
+            for (int it = 0; it < passes; ++it) {
                 for (int i = 0; i < NW; ++i) cm &= cm - 1;
                 const EdgeRec q = qn;
                 const bool have = mine != 0;
@@ -746,16 +788,18 @@ __device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2,
                         gb = L.u.b.recB[wv][jn];
                     }
                     const Entry en = row_entry(g, r);
+                    const uint32_t a0 = static_cast<uint32_t>(en.x0) - static_cast<uint32_t>(xb);
+                    const uint32_t a1 = static_cast<uint32_t>(en.x1) - static_cast<uint32_t>(xb);
+                    wide = wide || a0 > kMwSpan || a1 > kMwSpan;
                     const int slot = min(off, P);
-                    L.p0[slot * 64 + lane] = en.x0;
-                    L.p1[slot * 64 + lane] = en.x1;
-                    L.pc[slot * 64 + lane] = en.c;
+                    L.q.pk[slot * 64 + lane] = (a0 & kMwSpan) | (a1 & kMwSpan) << 15 |
+                                               static_cast<uint32_t>(en.c + 1) << 30;
                     ++off;
                 }
             }
         }
     }
-    if (__ballot(top > P) != 0 && lane == 0) L.over = 1;
+    if (__ballot(top > P || wide) != 0 && lane == 0) L.over = 1;
     __syncthreads();
     // Entries are read from LDS kMwBlock at a time into registers and the
     // loops over them are unrolled and branch-free (a store that is not
@@ -771,23 +815,19 @@ __device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2,
             int32_t x0[kB], x1[kB], cc[kB];
 #pragma unroll
             for (int j = 0; j < kB; ++j) {
-                const int k = min(k0 + j, P);
-                x0[j] = L.p0[k * 64 + lane];
-                x1[j] = L.p1[k * 64 + lane];
-                cc[j] = L.pc[k * 64 + lane];
+                const uint32_t v = L.q.pk[min(k0 + j, P) * 64 + lane];
+                x0[j] = static_cast<int32_t>(v & kMwSpan);
+                x1[j] = static_cast<int32_t>((v >> 15) & kMwSpan);
+                cc[j] = static_cast<int32_t>(v >> 30) - 1;
             }
 #pragma unroll
             for (int j = 0; j < kB; ++j) {
                 const bool v = k0 + j < top;
-                const bool touch = __builtin_elementwise_add_sat(lx1, 1) >= x0[j] &&
-                                   lx0 <= __builtin_elementwise_add_sat(x1[j], 1);
+                const bool touch = lx1 + 1 >= x0[j] && lx0 <= x1[j] + 1;
                 const bool st = v && !touch && (lc & mrule) != 0;
-                // in place: at most k entries have retired by intersection k,
-                // whose block is already in registers
-                const int slot = st ? nb : P;
-                L.p0[slot * 64 + lane] = lx0;
-                L.p1[slot * 64 + lane] = lx1;
-                L.pc[slot * 64 + lane] = lc;
+                const int slot = st && nb < K2 ? nb : K2;
+                L.u.m.mx[slot * 64 + lane] = static_cast<uint32_t>(lx0) | static_cast<uint32_t>(lx1) << 16;
+                L.u.m.mc[slot * 64 + lane] = lc;
                 nb += st ? 1 : 0;
                 const int32_t sum = static_cast<int32_t>(static_cast<uint32_t>(lc) + static_cast<uint32_t>(cc[j]));
                 lc = v ? (touch ? sum : cc[j]) : lc;
@@ -796,22 +836,20 @@ __device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2,
             }
         }
         if ((lc & mrule) != 0) {
-            L.p0[nb * 64 + lane] = lx0;
-            L.p1[nb * 64 + lane] = lx1;
-            L.pc[nb * 64 + lane] = lc;
+            const int slot = nb < K2 ? nb : K2;
+            L.u.m.mx[slot * 64 + lane] = static_cast<uint32_t>(lx0) | static_cast<uint32_t>(lx1) << 16;
+            L.u.m.mc[slot * 64 + lane] = lc;
             ++nb;
         }
         L.nb[lane] = min(nb, K2);
         if (__ballot(nb > K2) != 0 && lane == 0) L.over = 1;
     }
     __syncthreads();
-    const bool fallback = L.over != 0;
-    __syncthreads();  // every wave has read the flag before the fallback reuses the LDS
-    if (fallback) return false;
+    const bool fallback = L.over != 0;  // written only ahead of the barrier above
 
     // stable sort by x0: entry i of the row goes to its rank; each thread
     // ranks the entries wv, wv + NW, ... against every entry of the row
-    {
+    if (!fallback) {
         // key = x0 * 256 + index: one 64-bit compare per pair; entries past
         // the row's end get the largest key
         constexpr int kPer = (K2 + NW - 1) / NW;
@@ -821,7 +859,7 @@ __device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2,
 #pragma unroll
         for (int e = 0; e < kPer; ++e) {
             const int i = wv + e * NW;
-            const int32_t x = L.p0[min(i, P) * 64 + lane];
+            const uint32_t x = L.u.m.mx[min(i, K2) * 64 + lane] & 0xffffu;
             key[e] = i < n ? static_cast<long long>(x) * 256 + i : LLONG_MAX;
             rank[e] = 0;
         }
@@ -830,7 +868,7 @@ __device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2,
 #pragma unroll
             for (int j = 0; j < kB; ++j) {
                 const int k = k0 + j;
-                const int32_t x = L.p0[min(k, P) * 64 + lane];
+                const uint32_t x = L.u.m.mx[min(k, K2) * 64 + lane] & 0xffffu;
                 kk[j] = k < n ? static_cast<long long>(x) * 256 + k : LLONG_MAX;
             }
 #pragma unroll
@@ -843,20 +881,19 @@ __device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2,
         for (int e = 0; e < kPer; ++e) {
             const int i = wv + e * NW;
             if (i < n) {
-                L.u.s.s0[rank[e] * 64 + lane] = static_cast<int32_t>(key[e] >> 8);
-                L.u.s.s1[rank[e] * 64 + lane] = L.p1[i * 64 + lane];
-                L.u.s.sc[rank[e] * 64 + lane] = L.pc[i * 64 + lane];
+                L.q.s.sx[rank[e] * 64 + lane] = static_cast<int32_t>(L.u.m.mx[i * 64 + lane]);
+                L.q.s.sc[rank[e] * 64 + lane] = L.u.m.mc[i * 64 + lane];
             }
         }
     }
     __syncthreads();
     unsigned long long w_draw = 0;
-    if (wv == 0) {
+    if (wv == 0 && !fallback) {
         // render_aa_line's walk (scanner/mod.rs:393-467) over the sorted
         // entries: an entry met outside opens a span at its x0, the entry that
         // brings the winding back to outside closes it at its x1 + 1; spans
         // clamped to [0, aa_width) and merged into disjoint intervals, written
-        // over the (read) merged entries in the row lists
+        // in place over the entries already read
         const int n = L.nb[lane];
         const int32_t aa_width = pg.width * 4;
         int32_t count = 0, xx = 0, cs = 0, ce = -1;
@@ -867,9 +904,9 @@ __device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2,
             const bool any = go && bx0 < bx1;
             const bool join = ce >= 0 && bx0 <= ce;
             const bool emit = any && !join && ce >= 0;
-            const int slot = emit ? nint : P;
-            L.p0[slot * 64 + lane] = cs;
-            L.p1[slot * 64 + lane] = ce;
+            const int slot = emit ? nint : K2;
+            L.q.s.sx[slot * 64 + lane] = cs;
+            L.q.s.sc[slot * 64 + lane] = ce;
             nint += emit ? 1 : 0;
             cs = any && !join ? bx0 : cs;
             ce = any ? (join ? max(ce, bx1) : bx1) : ce;
@@ -878,10 +915,13 @@ __device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2,
             int32_t x0[kB], x1[kB], cc[kB];
 #pragma unroll
             for (int j = 0; j < kB; ++j) {
-                const int k = min(k0 + j, K2 - 1);
-                x0[j] = L.u.s.s0[k * 64 + lane];
-                x1[j] = L.u.s.s1[k * 64 + lane];
-                cc[j] = L.u.s.sc[k * 64 + lane];
+                // an interval is written at most at the index of the entry
+                // that closes it, so this block is read before any write to it
+                const int k = min(k0 + j, K2);
+                const uint32_t v = static_cast<uint32_t>(L.q.s.sx[k * 64 + lane]);
+                x0[j] = static_cast<int32_t>(static_cast<uint32_t>(xb) + (v & 0xffffu));
+                x1[j] = static_cast<int32_t>(static_cast<uint32_t>(xb) + (v >> 16));
+                cc[j] = L.q.s.sc[k * 64 + lane];
             }
 #pragma unroll
             for (int j = 0; j < kB; ++j) {
@@ -899,17 +939,18 @@ __device__ __forceinline__ bool fill_band_mw(const FillArgs &A, MwLds<NW, P, K2,
         const bool unb = open;
         put(max(xx, 0), aa_width, unb);
         if (ce >= 0) {
-            L.p0[nint * 64 + lane] = cs;
-            L.p1[nint * 64 + lane] = ce;
+            L.q.s.sx[nint * 64 + lane] = cs;
+            L.q.s.sc[nint * 64 + lane] = ce;
             ++nint;
         }
         const bool any_unb = __ballot(unb) != 0;
         L.nb[lane] = nint;
         if (lane == 0) L.unb = any_unb ? 1 : 0;
-        if (A.cov_count) w_draw = band_draw_width(pg, Y0, ylo, yhi, nint, L.p0, L.p1);
+        if (A.cov_count) w_draw = band_draw_width(pg, Y0, ylo, yhi, nint, L.q.s.sx, L.q.s.sc);
     }
     __syncthreads();
-    composite_band<CH, true>(A, L.u.bm[wv], L.gamma, pg, p, Y0, h, ylo, yhi, at, L.nb[lane], L.p0, L.p1,
+    if (fallback) return false;
+    composite_band<CH, true>(A, L.u.bm[wv], L.gamma, pg, p, Y0, h, ylo, yhi, at, L.nb[lane], L.q.s.sx, L.q.s.sc,
                              L.unb != 0, wv, NW, w_draw);
     return true;
 }
@@ -920,22 +961,31 @@ union MwUnion {
     BandLds<FK, CH> f;  // the one-wave fallback
 };
 
-// PLAIN: one launch per path, NW waves per band of 16 output rows.
-template <int NW, int P, int K2, int CH, int FK>
-__global__ void __launch_bounds__(64 * NW) k_fill_plain(FillArgs A, int32_t page, int64_t p, int32_t row0,
-                                                        int32_t slot0) {
-    __shared__ MwUnion<NW, P, K2, CH, FK> U;
+// PLAIN: one launch per path.  G == 1: a workgroup of NW waves per band of 16
+// output rows (grid = the path's bands); G > 1: one workgroup whose groups of
+// NW waves take the path's bands (blockDim = 64 * NW * bands, bands <= G).
+template <int G, int NW, int P, int K2, int CH, int FK>
+__global__ void __launch_bounds__(64 * G * NW) k_fill_plain(FillArgs A, int32_t page, int64_t p, int32_t row0,
+                                                            int32_t slot0) {
+    static_assert(sizeof(MwUnion<NW, P, K2, CH, FK>) * G + 8 * G <= 163840, "LDS of one workgroup");
+    __shared__ MwUnion<NW, P, K2, CH, FK> U[G];
+    __shared__ int32_t npass[2 * G];
+    const int grp = G == 1 ? 0 : __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6)) / NW;
+    const int ng = G == 1 ? 1 : static_cast<int>(blockDim.x) / (64 * NW);
+    const int band = G == 1 ? static_cast<int>(blockIdx.x) : grp;
     const DevPage &pg = A.pages[page];
-    const int32_t Y0 = row0 + kBandRows * static_cast<int32_t>(blockIdx.x);
+    const int32_t Y0 = row0 + kBandRows * band;
+    const int lt = static_cast<int>(threadIdx.x) - grp * NW * 64;
     const long long t0 = A.trace ? static_cast<long long>(wall_clock64()) : 0;
-    if (!fill_band_mw<NW, P, K2, CH>(A, U.m, pg, p, Y0)) {
-        if (threadIdx.x >= 64) return;
-        if (threadIdx.x < 17) U.f.gamma[threadIdx.x] = kGamma[threadIdx.x];
+    if (!fill_band_mw<NW, P, K2, CH>(A, U[grp].m, npass, ng, grp, pg, p, Y0)) {
+        // past the groups' last shared barrier: the group's first wave alone
+        if (lt >= 64) return;
+        if (lt < 17) U[grp].f.gamma[lt] = kGamma[lt];
         wave_sync();
-        fill_band<FK, CH, true>(A, U.f, pg, p, Y0, slot0 + static_cast<int64_t>(blockIdx.x));
+        fill_band<FK, CH, true>(A, U[grp].f, pg, p, Y0, slot0 + static_cast<int64_t>(band));
     }
-    if (A.trace && threadIdx.x == 0) {
-        // wall-clock span of the first wave's share of the band
+    if (A.trace && lt == 0) {
+        // wall-clock span of the group's first wave's share of the band
         const long long t1 = static_cast<long long>(wall_clock64());
         const unsigned long long k = atomicAdd(A.trace_top, 1ull);
         if (static_cast<long long>(k) < A.trace_cap)
@@ -947,17 +997,25 @@ __global__ void __launch_bounds__(64 * NW) k_fill_plain(FillArgs A, int32_t page
 // host side
 // ---------------------------------------------------------------------------
 
-// Plain-launch kernel of every instantiated fill shape: CH from kVariantCH
-// (the plain composite sweep is at most 64 pixels a step, which keeps the
-// sixteen waves' bitmaps inside the LDS the intersection lists leave), K =
-// kVariantK the LDS entry slots of the one-wave fallback.
+// Plain-launch kernels of every instantiated fill shape, by launch shape:
+// [0] a workgroup of kMwWaves waves per band, [1] one workgroup for a path of
+// two bands (8 waves each), [2] one workgroup for a path of three bands (5
+// waves each).  CH from kVariantCH, at most 64 pixels a step on this route
+// (which keeps the waves' bitmaps inside the LDS the intersection lists
+// leave); FK = kVariantK, the LDS entry slots of the one-wave fallback.
+constexpr int kPlainShapes = 3;
+constexpr int kPlainGroups[kPlainShapes] = {1, 2, 3};
+constexpr int kPlainGroupWaves[kPlainShapes] = {kMwWaves, 8, 5};
 constexpr int plain_ch(int v) { return kVariantCH[v] > 64 ? 64 : kVariantCH[v]; }
-#define RR_PLAIN(v) \
-    reinterpret_cast<const void *>(&k_fill_plain<kMwWaves, kMwPool, kMwK, plain_ch(v), kVariantK[v]>)
-const void *const kPlainKernels[kNumVariants] = {RR_PLAIN(0),  RR_PLAIN(1),  RR_PLAIN(2),  RR_PLAIN(3),
-                                                 RR_PLAIN(4),  RR_PLAIN(5),  RR_PLAIN(6),  RR_PLAIN(7),
-                                                 RR_PLAIN(8),  RR_PLAIN(9),  RR_PLAIN(10), RR_PLAIN(11),
-                                                 RR_PLAIN(12), RR_PLAIN(13), RR_PLAIN(14), RR_PLAIN(15)};
+#define RR_PLAIN(s, v)                                                                                    \
+    reinterpret_cast<const void *>(&k_fill_plain<kPlainGroups[s], kPlainGroupWaves[s], kMwPool, kMwK, \
+                                                 plain_ch(v), kVariantK[v]>)
+#define RR_PLAIN_ROW(s)                                                                                   \
+    {RR_PLAIN(s, 0),  RR_PLAIN(s, 1),  RR_PLAIN(s, 2),  RR_PLAIN(s, 3),  RR_PLAIN(s, 4),  RR_PLAIN(s, 5), \
+     RR_PLAIN(s, 6),  RR_PLAIN(s, 7),  RR_PLAIN(s, 8),  RR_PLAIN(s, 9),  RR_PLAIN(s, 10), RR_PLAIN(s, 11), \
+     RR_PLAIN(s, 12), RR_PLAIN(s, 13), RR_PLAIN(s, 14), RR_PLAIN(s, 15)}
+const void *const kPlainKernels[kPlainShapes][kNumVariants] = {RR_PLAIN_ROW(0), RR_PLAIN_ROW(1), RR_PLAIN_ROW(2)};
+#undef RR_PLAIN_ROW
 #undef RR_PLAIN
 
 }  // namespace
@@ -1239,12 +1297,15 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
                 }
                 const unsigned bands = static_cast<unsigned>((r1 - r0) / kBandRows + 1);
                 const int32_t slot0 = s * max_bands;
+                // two or three bands: one workgroup (the device dispatches
+                // single-workgroup launches at a higher rate)
+                const int shape = bands == 2 ? 1 : (bands == 3 ? 2 : 0);
+                const dim3 grid(shape ? 1u : bands), block(shape ? 64u * kPlainGroupWaves[shape] * bands : 64u * kMwWaves);
                 hipEvent_t *ev = c->profiling ? prof_pair(c) : nullptr;
                 if (c->profiling && !ev) return fail(c, RR_E_DEVICE, "profiling event allocation failed");
                 if (ev) RR_HIP_TRY(c, hipEventRecord(ev[0], c->side[s]));
                 void *args[] = {&A, const_cast<int32_t *>(&pi), &p, &r0, const_cast<int32_t *>(&slot0)};
-                RR_HIP_TRY(c, hipLaunchKernel(kPlainKernels[b.variant], dim3(bands), dim3(64 * kMwWaves), args, 0,
-                                              c->side[s]));
+                RR_HIP_TRY(c, hipLaunchKernel(kPlainKernels[shape][b.variant], grid, block, args, 0, c->side[s]));
                 if (ev) RR_HIP_TRY(c, hipEventRecord(ev[1], c->side[s]));
                 ++c->fill_launches;
                 for (int32_t q = r0 / CELL; q <= r1 / CELL; ++q) c->cell_stream[q] = static_cast<int8_t>(s);

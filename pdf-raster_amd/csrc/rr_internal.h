@@ -232,7 +232,9 @@ struct rr_hip_ctx {
     // Plain launches go to kPlainStreams side streams (created on first use):
     // a path whose drawn rows meet those of an earlier path of its page runs
     // after it (same stream, or an event wait); other paths run concurrently.
-    static constexpr int kPlainStreams = 16;
+    // Each stream wants a hardware queue of its own (GPU_MAX_HW_QUEUES of at
+    // least kPlainStreams + 2: these, the context stream and HIP's own).
+    static constexpr int kPlainStreams = 20;
     static constexpr int kPlainCellRows = 4;  // row granularity of the overlap test
     hipStream_t side[kPlainStreams] = {};
     hipEvent_t side_ev[kPlainStreams] = {};  // dependency and join points of each side stream

@@ -228,3 +228,47 @@ def test_plain_launch_order_across_streams(backend, mode):
     paths.append((rect(0.25, 0.25, 63.5, 79.5), 0, 250, 64))
     page = hand_page(paths, width=64, height=80)
     assert_same(gpu_page(backend, page, mode), oracle_page(page, mode), "strips across streams")
+
+
+def comb(x0, x1, teeth, y_tip, y_base, y_spine):
+    def b(p):
+        w = (x1 - x0) / teeth
+        p.move_to(x0, y_spine)
+        p.line_to(x0, y_base)
+        for k in range(teeth):
+            p.line_to(x0 + (k + 0.5) * w, y_tip)
+            p.line_to(x0 + (k + 1) * w, y_base)
+        p.line_to(x1, y_spine)
+        p.close(False)
+    return b
+
+
+@pytest.mark.parametrize("mode", [R.MODE_DOCUMENTED, R.MODE_LITERAL])
+@pytest.mark.parametrize("variant", [0, 12])
+def test_plain_workgroup_shapes(backend, mode, variant):
+    # Paths of one band, two and three bands (one workgroup holding every
+    # band) and five bands (a workgroup per band): a comb whose tooth rows
+    # hold some 300 intersections (those bands go to the one-wave fallback
+    # while the spine's band does not), a path wider than the packed 15-bit
+    # x range (fallback), a path whose last band holds no drawn row in literal
+    # mode, half-transparent overlaps.
+    paths = [
+        (rect(3.3, 3.3, 120.7, 25.7), 0, 40, 255),                  # 2 bands
+        (donut, 1, 90, 200),                                        # 3 bands
+        (comb(150, 750, 150, 5.2, 30.4, 45.6), 0, 10, 255),        # 3 bands, fallback in two
+        (comb(800, 1400, 150, 20.1, 40.3, 33.2), 1, 170, 128),     # 2 bands, both fall back
+        (rect(10.5, 40.2, 8900.5, 57.9), 0, 120, 160),              # 2 bands, wider than 8191 px
+        (rect(1500.2, 15.9, 1600.7, 47.05), 0, 200, 255),           # 3 bands, last one empty in literal
+        (rect(1700.1, 2.2, 1780.9, 9.8), 1, 60, 255),               # 1 band
+        (rect(1800.4, 2.6, 1900.2, 77.4), 0, 220, 100),             # 5 bands
+        (chained_glyph, 0, 0, 255),
+    ]
+    page = hand_page(paths, width=9000, height=80)
+    want = oracle_page(page, mode)
+    db = R.DeviceBatch(backend, [page])
+    try:
+        db.run(mode, R.FLAVOUR_GRAY8, R.LAUNCH_PLAIN, hashes=True, variant=variant)
+        got = (db.read_canvas(0), db.read_hashes())
+    finally:
+        db.free()
+    assert_same(got, want, f"workgroup shapes, variant {variant}")
