@@ -79,7 +79,11 @@ typedef struct {
  * AUTO: chosen in-kernel ahead of the fill: by the dispatcher MLP when one is
  *   loaded (rr_hip_set_dispatcher), else by the static rule (area < 256 px ->
  *   CPU, else the smallest TILE that holds the path's bands, else SPAN).
- * Routes are honoured by RR_LAUNCH_BATCHED; RR_LAUNCH_PLAIN is always SPAN. */
+ * Routes are honoured by RR_LAUNCH_BATCHED.  RR_LAUNCH_PLAIN takes its shape
+ * from the path's host_rows instead (the route lives in device memory): a path
+ * of two or three 16-row bands runs as one workgroup holding all its bands
+ * (the tile shape of a small path, a launch the device dispatches fastest),
+ * any other one workgroup per band. */
 #define RR_ROUTE_SPAN 0
 #define RR_ROUTE_TILE16 1
 #define RR_ROUTE_TILE32 2
@@ -229,8 +233,9 @@ int32_t rr_hip_inject_error(rr_hip_ctx *ctx, int32_t bits);
 /* Timing of the last waited page, from HIP events on the context stream:
  * span_ms from the first to the last recorded kernel; with profiling enabled
  * (rr_hip_set_profiling) fill_ms is the summed duration of the fill kernels
- * alone, each bracketed by its own event pair; launches counts the fill-kernel
- * launches (the edge-setup pass excluded). */
+ * alone, each bracketed by its own event pair (RR_LAUNCH_PLAIN then runs its
+ * launches one after another instead of concurrently); launches counts the
+ * fill-kernel launches (the edge-setup pass excluded). */
 int32_t rr_hip_set_profiling(rr_hip_ctx *ctx, int32_t enable);
 /* Per-fill timing trace (no counterpart in the reference; it feeds the
  * autotuner and the learned dispatcher).  While set, every fill of one path

@@ -1284,7 +1284,10 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
                 int s;
                 if (dep == 0) {
                     s = next;
-                    next = (next + 1) % NS;
+                    // per-launch profiling runs the launches one after
+                    // another on the first stream, so that each event pair
+                    // brackets its kernel alone
+                    next = c->profiling ? 0 : (next + 1) % NS;
                 } else {
                     s = __builtin_ctz(dep);
                 }
