@@ -38,6 +38,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1234,7 +1235,8 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
     RR_HIP_TRY(c, hipGetLastError());
 
     if (b.launch == RR_LAUNCH_PLAIN) {
-        constexpr int NS = rr_hip_ctx::kPlainStreams, CELL = rr_hip_ctx::kPlainCellRows;
+        const int NS = c->plain_streams;
+        constexpr int CELL = rr_hip_ctx::kPlainCellRows;
         int32_t max_bands = 1;
         for (int32_t i = 0; i < b.n_pages; ++i)
             max_bands = std::max(max_bands, (b.pages[i].height + kBandRows) / kBandRows + 1);
@@ -1372,6 +1374,16 @@ int32_t rr_hip_create(int32_t device, rr_hip_ctx **out) {
                                                      0) != hipSuccess)
         rc = RR_E_DEVICE;
     c->setup_bpc = std::max(1, c->setup_bpc);
+    // side streams of the plain launch: one hardware queue each, two left for
+    // the context stream and HIP's own (HIP puts streams beyond its queue
+    // count on shared queues, which serialises them); with few queues (HIP's
+    // default is 4) every queue is shared anyway and the full set keeps the
+    // most launches in flight
+    {
+        const char *q = std::getenv("GPU_MAX_HW_QUEUES");
+        const int hwq = q ? std::atoi(q) : 4;
+        c->plain_streams = hwq >= 8 ? std::min(rr_hip_ctx::kPlainStreams, hwq - 2) : rr_hip_ctx::kPlainStreams;
+    }
     if (rc == RR_OK) rc = rrint::init_static_scratch(c);
     if (rc == RR_OK && (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess)) rc = RR_E_DEVICE;
     if (rc == RR_OK && hipMalloc(&c->spill_top, sizeof(unsigned long long)) != hipSuccess) rc = RR_E_DEVICE;

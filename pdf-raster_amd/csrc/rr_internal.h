@@ -235,6 +235,7 @@ struct rr_hip_ctx {
     // Each stream wants a hardware queue of its own (GPU_MAX_HW_QUEUES of at
     // least kPlainStreams + 2: these, the context stream and HIP's own).
     static constexpr int kPlainStreams = 20;
+    int plain_streams = kPlainStreams;  // side streams in use (from GPU_MAX_HW_QUEUES)
     static constexpr int kPlainCellRows = 4;  // row granularity of the overlap test
     hipStream_t side[kPlainStreams] = {};
     hipEvent_t side_ev[kPlainStreams] = {};  // dependency and join points of each side stream
