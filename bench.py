@@ -778,6 +778,27 @@ def main():
         print(f"bench: graph replay {graph_sec * 1e3:.2f} ms/step", file=sys.stderr, flush=True)
         if graph_sec < sec:
             sec = graph_sec
+    nocull = None
+    if launch == R.LAUNCH_BATCHED:
+        # the same batch with every path at a_input 254: no path is an opaque
+        # cover, so nothing is culled -- every (path, band) item is built and
+        # every path composited on every tile it touches, with the alpha
+        # composite (the rate on pages without opaque covers)
+        a_saved = db.attrs_host["a_input"].copy()
+        db.attrs_host["a_input"] = 254
+        be.upload(db.attrs, db.attrs_host)
+        step(launch)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            step(launch)
+        nc_sec = (time.perf_counter() - t0) / 3
+        db.attrs_host["a_input"] = a_saved
+        be.upload(db.attrs, db.attrs_host)
+        nocull = {"ms_per_step": round(nc_sec * 1e3, 3), "pages_per_s": round(len(pages) / nc_sec, 3),
+                  "ref_mpx_per_s": round(w_ref / nc_sec / 1e6, 3),
+                  "note": "every path at a_input 254 (no opaque covers, so no occlusion culling), eager, 3 steps "
+                          "after 1 warmup"}
+        print(f"bench: without opaque covers {nc_sec * 1e3:.2f} ms/step", file=sys.stderr, flush=True)
     value = w_ref / sec / 1e6
     pages_per_s = len(pages) / sec
     # the whole step against the compulsory bytes: every canvas pixel read and
@@ -856,6 +877,8 @@ def main():
             "composited_px_upper_bound": st["path_tiles"] * 16 * ch,
             "span_items": st["items"], "span_items_occluded": st["skipped"],
             "tiles_started_from_cover": st["filled_tiles"]}
+    if nocull:
+        out["without_occlusion_culling"] = nocull
     if not args.no_plain:
         other = R.LAUNCH_PLAIN if launch == R.LAUNCH_BATCHED else R.LAUNCH_BATCHED
         key = "plain" if other == R.LAUNCH_PLAIN else "batched"
