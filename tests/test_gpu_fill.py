@@ -5,6 +5,10 @@ and through the oracle, and compares the final u8 canvas and the per-path
 coverage hash (sum of a 64-bit mix of (pixel, shape) over every pixel the path
 composited with coverage > 0).
 """
+import os
+import subprocess
+import sys
+
 import numpy as np
 import pytest
 
@@ -272,3 +276,36 @@ def test_plain_workgroup_shapes(backend, mode, variant):
     finally:
         db.free()
     assert_same(got, want, f"workgroup shapes, variant {variant}")
+
+
+_FEW_QUEUES = r"""
+import sys
+import numpy as np
+sys.path[:0] = [sys.argv[1], sys.argv[1] + "/pdf-raster_amd"]
+import rr_hip as R
+from oracle import oracle as O
+pg = R.workload(2, 0xC0F10002)[0].subset(range(0, 6000, 7))
+be = R.Backend(0)
+dp = R.DevicePage(be, pg)
+got, gh = dp.render(R.MODE_DOCUMENTED, R.FLAVOUR_GRAY8, R.LAUNCH_PLAIN, hashes=True)
+dp.free()
+be.close()
+want = np.full((pg.height, pg.width), 255, np.uint8)
+_, wh = O.fill_page(want, pg.edges, pg.edge_off, pg.attrs["eo"], pg.attrs["color"], pg.attrs["a_input"],
+                    O.MODE_DOCUMENTED, O.FLAVOUR_GRAY8)
+assert np.array_equal(got, want), f"{int((got != want).sum())} pixels differ"
+assert np.array_equal(gh, np.asarray(wh, np.uint64)), "coverage hashes differ"
+print("ok", pg.n_paths)
+"""
+
+
+def test_plain_route_with_ten_hardware_queues():
+    # A process started with GPU_MAX_HW_QUEUES=10 spreads the plain launches
+    # over 8 side streams (two fewer than the queues): 858 glyphs of the glyph
+    # page against the oracle in a fresh process.
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, GPU_MAX_HW_QUEUES="10")
+    r = subprocess.run([sys.executable, "-c", _FEW_QUEUES, root], env=env, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, (r.stdout + r.stderr)[-2000:]
+    assert r.stdout.startswith("ok")
