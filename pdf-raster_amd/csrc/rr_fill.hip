@@ -962,13 +962,26 @@ union MwUnion {
     BandLds<FK, CH> f;  // the one-wave fallback
 };
 
+// Writes the plain launches' shared arguments to device memory (launch
+// arguments are copied when the launch is issued, so the host may reuse A at
+// once).
+__global__ void __launch_bounds__(64) k_store_fill_args(FillArgs A, FillArgs *dst) {
+    if (threadIdx.x == 0) *dst = A;
+}
+
 // PLAIN: one launch per path.  G == 1: a workgroup of NW waves per band of 16
 // output rows (grid = the path's bands); G > 1: one workgroup whose groups of
 // NW waves take the path's bands (blockDim = 64 * NW * bands, bands <= G).
+// The recording's shared arguments reach the launches through device memory
+// (Ap, written by k_store_fill_args ahead of them): a launch then carries 32
+// bytes of arguments instead of the whole FillArgs, which the host issues at a
+// markedly higher rate (one launch per path is bound by that rate on glyph
+// pages).
 template <int G, int NW, int P, int K2, int CH, int FK>
-__global__ void __launch_bounds__(64 * G * NW) k_fill_plain(FillArgs A, int32_t page, int64_t p, int32_t row0,
-                                                            int32_t slot0) {
+__global__ void __launch_bounds__(64 * G * NW) k_fill_plain(const FillArgs *__restrict__ Ap, int32_t page, int64_t p,
+                                                            int32_t row0, int32_t slot0) {
     static_assert(sizeof(MwUnion<NW, P, K2, CH, FK>) * G + 8 * G <= 163840, "LDS of one workgroup");
+    const FillArgs &A = *Ap;
     __shared__ MwUnion<NW, P, K2, CH, FK> U[G];
     __shared__ int32_t npass[2 * G];
     const int grp = G == 1 ? 0 : __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6)) / NW;
@@ -1252,6 +1265,11 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
             RR_HIP_TRY(c, hipEventCreateWithFlags(&c->side_start, hipEventDisableTiming));
         }
         A.slots = c->slots;
+        if (!c->plain_args) RR_HIP_TRY(c, hipMalloc(&c->plain_args, sizeof(FillArgs)));
+        // ordered after the launches of earlier recordings (the context stream
+        // joined their side streams) and ahead of this one's
+        hipLaunchKernelGGL(k_store_fill_args, dim3(1), dim3(64), 0, c->stream, A, c->plain_args);
+        RR_HIP_TRY(c, hipGetLastError());
         // the side streams start after the work recorded on the context stream
         RR_HIP_TRY(c, hipEventRecord(c->side_start, c->stream));
         uint32_t started = 0;
@@ -1309,7 +1327,8 @@ int32_t record_pages(rr_hip_ctx *c, const rr_fill_pages &b) {
                 hipEvent_t *ev = c->profiling ? prof_pair(c) : nullptr;
                 if (c->profiling && !ev) return fail(c, RR_E_DEVICE, "profiling event allocation failed");
                 if (ev) RR_HIP_TRY(c, hipEventRecord(ev[0], c->side[s]));
-                void *args[] = {&A, const_cast<int32_t *>(&pi), &p, &r0, const_cast<int32_t *>(&slot0)};
+                const FillArgs *pa = c->plain_args;
+                void *args[] = {&pa, const_cast<int32_t *>(&pi), &p, &r0, const_cast<int32_t *>(&slot0)};
                 RR_HIP_TRY(c, hipLaunchKernel(kPlainKernels[shape][b.variant], grid, block, args, 0, c->side[s]));
                 if (ev) RR_HIP_TRY(c, hipEventRecord(ev[1], c->side[s]));
                 ++c->fill_launches;
@@ -1419,7 +1438,7 @@ int32_t rr_hip_destroy(rr_hip_ctx *c) {
                     c->ovq,       c->bin_cnt,  c->bin_off,    c->bins,    c->queues,    c->band_run,
                     c->xp_t,      c->xp_role,  c->xp_close,   c->xp_cnt,  c->xp_eoff,   c->xp_err,
                     c->xp_sidx,   c->xp_seg,   c->img_grey,   c->xp_stage,  c->xp_bkt,
-                    c->xp_perm,   c->xp_ccnt,  c->xp_coff};
+                    c->xp_perm,   c->xp_ccnt,  c->xp_coff,   c->plain_args};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (auto e : c->xp_ev)

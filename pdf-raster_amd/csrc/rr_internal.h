@@ -236,6 +236,7 @@ struct rr_hip_ctx {
     // least kPlainStreams + 2: these, the context stream and HIP's own).
     static constexpr int kPlainStreams = 20;
     int plain_streams = kPlainStreams;  // side streams in use (from GPU_MAX_HW_QUEUES)
+    rrdev::FillArgs *plain_args = nullptr;  // device copy of the plain launches' shared arguments
     static constexpr int kPlainCellRows = 4;  // row granularity of the overlap test
     hipStream_t side[kPlainStreams] = {};
     hipEvent_t side_ev[kPlainStreams] = {};  // dependency and join points of each side stream
