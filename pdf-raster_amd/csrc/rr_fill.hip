@@ -973,10 +973,9 @@ __global__ void __launch_bounds__(64) k_store_fill_args(FillArgs A, FillArgs *ds
 // output rows (grid = the path's bands); G > 1: one workgroup whose groups of
 // NW waves take the path's bands (blockDim = 64 * NW * bands, bands <= G).
 // The recording's shared arguments reach the launches through device memory
-// (Ap, written by k_store_fill_args ahead of them): a launch then carries 32
-// bytes of arguments instead of the whole FillArgs, which the host issues at a
-// markedly higher rate (one launch per path is bound by that rate on glyph
-// pages).
+// (Ap, written by k_store_fill_args ahead of them): a launch carries 32 bytes
+// of arguments instead of the 160-byte FillArgs, less for the host to copy on
+// each of the thousands of launches of a page.
 template <int G, int NW, int P, int K2, int CH, int FK>
 __global__ void __launch_bounds__(64 * G * NW) k_fill_plain(const FillArgs *__restrict__ Ap, int32_t page, int64_t p,
                                                             int32_t row0, int32_t slot0) {
