@@ -154,7 +154,9 @@ typedef struct {
  * rr_fill_batch with path indices global to the batch; pages is a host array.
  * With launch = RR_LAUNCH_BATCHED every page goes through one edge-setup pass
  * and one persistent fill+composite launch; with RR_LAUNCH_PLAIN each path is
- * its own launch (host_rows required). */
+ * its own launch (host_rows required), spread over up to 20 HIP streams (two
+ * fewer than GPU_MAX_HW_QUEUES when that is 8 or more; the variable must be set
+ * before the HIP runtime starts) and ordered by the rows the paths draw. */
 typedef struct {
     const double *edges;
     const int64_t *edge_off;
