@@ -236,8 +236,9 @@ int32_t rr_hip_inject_error(rr_hip_ctx *ctx, int32_t bits);
  * span_ms from the first to the last recorded kernel; with profiling enabled
  * (rr_hip_set_profiling) fill_ms is the summed duration of the fill kernels
  * alone, each bracketed by its own event pair (RR_LAUNCH_PLAIN then runs its
- * launches one after another instead of concurrently); launches counts the
- * fill-kernel launches (the edge-setup pass excluded). */
+ * launches one after another instead of concurrently, and RR_LAUNCH_BATCHED
+ * builds its band lists in line instead of beside the span pass); launches
+ * counts the fill-kernel launches (the edge-setup pass excluded). */
 int32_t rr_hip_set_profiling(rr_hip_ctx *ctx, int32_t enable);
 /* Per-fill timing trace (no counterpart in the reference; it feeds the
  * autotuner and the learned dispatcher).  While set, every fill of one path

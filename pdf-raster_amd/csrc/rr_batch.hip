@@ -1686,10 +1686,26 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     if (!rc) rc = scan_exclusive<long long, long long>(c, c->band_run + 1, c->band_base, n_bands,
                                                        c->band_base + n_bands, c->scan_tmp);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_band_lists, dim3(static_cast<unsigned>(n_bands)), dim3(64), 0, c->stream, c->sel, c->pages,
-                       b.n_pages, c->item_base, c->band_base, c->band_list, static_cast<long long>(c->band_list_cap),
-                       c->err_flag);
+    // the band lists are read only by the tile bins after the span pass, so
+    // they are built on the auxiliary stream while the span pass runs (the
+    // span pass's widening touches only the .w of the sel records they read);
+    // with per-launch profiling on, in line, so that the span pass's events
+    // time it alone
+    const bool overlap = !c->profiling;
+    if (overlap && !c->aux) {
+        RR_TRY(c, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+        RR_TRY(c, hipEventCreateWithFlags(&c->aux_fork, hipEventDisableTiming));
+        RR_TRY(c, hipEventCreateWithFlags(&c->aux_join, hipEventDisableTiming));
+    }
+    if (overlap) {
+        RR_TRY(c, hipEventRecord(c->aux_fork, c->stream));
+        RR_TRY(c, hipStreamWaitEvent(c->aux, c->aux_fork, 0));
+    }
+    hipLaunchKernelGGL(k_band_lists, dim3(static_cast<unsigned>(n_bands)), dim3(64), 0, overlap ? c->aux : c->stream,
+                       c->sel, c->pages, b.n_pages, c->item_base, c->band_base, c->band_list,
+                       static_cast<long long>(c->band_list_cap), c->err_flag);
     RR_TRY(c, hipGetLastError());
+    if (overlap) RR_TRY(c, hipEventRecord(c->aux_join, c->aux));
 
     // phase A
     const unsigned a_waves = static_cast<unsigned>(std::min<long long>(c->span_waves[variant], std::max(n_work, 1ll)));
@@ -1728,6 +1744,7 @@ int32_t record_batched(rr_hip_ctx *c, const rr_fill_pages &b, const FillArgs &A,
     // tile bins of the composite pass: sizes, then offsets (the sizes follow
     // the span pass: skipped items and items without intervals are left out)
     auto count_bins = [&]() -> int32_t {
+        if (overlap) RR_TRY(c, hipStreamWaitEvent(c->stream, c->aux_join, 0));
         hipLaunchKernelGGL(k_tile_bins<false>, dim3(static_cast<unsigned>(n_bands * kBinSegs)), dim3(64), 0, c->stream,
                            c->pages, b.n_pages, c->band_base, c->band_list, B.shead, static_cast<long long>(c->shdr_cap),
                            chunk_px, c->occ, b.attrs, c->bin_cnt, c->bin_off, c->bins, static_cast<long long>(c->bins_cap),

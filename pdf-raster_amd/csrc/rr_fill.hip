@@ -1429,6 +1429,7 @@ int32_t rr_hip_destroy(rr_hip_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int k = 0; k < rr_hip_ctx::kPlainStreams; ++k)
         if (c->side[k]) (void)hipStreamSynchronize(c->side[k]);
+    if (c->aux) (void)hipStreamSynchronize(c->aux);
     void *bufs[] = {c->recs,      c->hdrs,     c->rows,       c->sel,     c->nb,        c->csum,
                     c->pages,     c->slots,    c->spill,      c->spill_top, c->err_flag, c->trace_top,
                     c->item_base, c->scan_tmp, c->sitems,     c->shdr,    c->spans,     c->band_cnt,
@@ -1452,6 +1453,9 @@ int32_t rr_hip_destroy(rr_hip_ctx *c) {
         if (c->side_ev[k]) (void)hipEventDestroy(c->side_ev[k]);
     }
     if (c->side_start) (void)hipEventDestroy(c->side_start);
+    if (c->aux) (void)hipStreamDestroy(c->aux);
+    if (c->aux_fork) (void)hipEventDestroy(c->aux_fork);
+    if (c->aux_join) (void)hipEventDestroy(c->aux_join);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return RR_OK;

@@ -237,6 +237,10 @@ struct rr_hip_ctx {
     static constexpr int kPlainStreams = 20;
     int plain_streams = kPlainStreams;  // side streams in use (from GPU_MAX_HW_QUEUES)
     rrdev::FillArgs *plain_args = nullptr;  // device copy of the plain launches' shared arguments
+    // batched recordings: the band lists are built on `aux` beside the span
+    // pass (fork after the band offsets, join before the tile bins)
+    hipStream_t aux = nullptr;
+    hipEvent_t aux_fork = nullptr, aux_join = nullptr;
     static constexpr int kPlainCellRows = 4;  // row granularity of the overlap test
     hipStream_t side[kPlainStreams] = {};
     hipEvent_t side_ev[kPlainStreams] = {};  // dependency and join points of each side stream
